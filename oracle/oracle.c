@@ -1,0 +1,728 @@
+/*
+ * oracle.c -- CPU restatement of the sjavis/minim hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: tests/, the smoke()
+ * entry and bench.py's cpu_baseline leg load it (liboracle.so); the product
+ * library (minim_amd/) never links, loads or calls it, and has no CPU fallback.
+ *
+ * Serial FP64, written to follow the reference's operation order so that with
+ * -ffp-contract=off it agrees bit for bit with a single-thread build of the
+ * reference (oracle/_ref, built from /root/reference by oracle/Makefile).
+ *
+ * Parity pins (see tests/test_oracle.py):
+ *   orc_pf_*            reference unit-test known answers (PhaseField_test.cpp:31-68,
+ *                       130-145), bit-exact vs oracle/_ref on random grids, finite
+ *                       differences, symmetry E(-phi)=E(phi), translation invariance.
+ *   orc_lbfgs_minimise  bit-exact trajectory vs oracle/_ref (iterations, energy,
+ *                       coordinates); n-step convergence on quadratics.
+ *   orc_fire_minimise   Fire_test.cpp:25-40 exact first step; convergence KAT
+ *                       Fire_test.cpp:43-52; bit-exact vs oracle/_ref.
+ *   orc_graddescent_*   bit-exact vs oracle/_ref.
+ *   orc_lj_*            State_test.cpp:33-51 dimer KAT; vs oracle/_ref; Newton's 3rd law.
+ *   orc_ljpbc_*         NO reference implementation exists (the reference LjNd has
+ *                       no cutoff or box).  Pinned by: equality with orc_lj_* when
+ *                       rc exceeds every pair distance and the box is large; E
+ *                       continuous and zero at r = rc; finite differences; sum g = 0.
+ *   orc_bah_*           BarAndHinge_test.cpp:13-56 minima; vs oracle/_ref; the
+ *                       reference's 2x stretching-gradient factor (BarAndHinge.cpp:282)
+ *                       checked against a finite difference of the energy.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------- */
+/* PhaseField                                                                */
+/* ------------------------------------------------------------------------- */
+
+/* PhaseField::assignFluidCoefficients (src/potentials/PhaseField.cpp:78-134),
+ * PhaseField::init (:137-158), setDefaults (:961-967), checkArraySizes (:970-995).
+ * Only MODEL_BASIC with spatially uniform coefficients is restated. */
+int orc_pf_init(orc_pf* pf, const int grid[3], int nfluid, double resolution,
+                const double* surface_tension, int n_st,
+                const double* interface_size, int n_iw,
+                int density, double density_const, const unsigned char* fix_fluid) {
+  memset(pf, 0, sizeof(*pf));
+  if (nfluid < 1 || nfluid > ORC_MAXF) return 1;
+  for (int d = 0; d < 3; d++) {
+    if (grid[d] < 1) return 1;
+    pf->grid[d] = grid[d];
+  }
+  pf->nfluid = nfluid;
+  pf->resolution = resolution;
+  pf->density = density;
+  pf->density_const = density_const;
+  int nparams = nfluid; /* PhaseField.cpp:141, MODEL_BASIC */
+
+  double st[ORC_MAXF], iw[ORC_MAXF];
+  if (n_st == 1) {
+    for (int i = 0; i < nparams; i++) st[i] = surface_tension[0];
+  } else if (n_st == nparams) {
+    for (int i = 0; i < nparams; i++) st[i] = surface_tension[i];
+  } else {
+    return 2; /* "Invalid size of surfaceTension array." :86 */
+  }
+  /* vec::sum is std::accumulate from 0 in order, then / size (:88) */
+  double sum = 0;
+  for (int i = 0; i < nparams; i++) sum += st[i];
+  pf->st_mean = sum / nparams;
+
+  /* setDefaults: interfaceSize := resolution for every fluid if unset (:962) */
+  if (n_iw == 0) {
+    for (int i = 0; i < nfluid; i++) iw[i] = resolution;
+    n_iw = nfluid;
+  } else if (n_iw == 1) {
+    for (int i = 0; i < nparams; i++) iw[i] = interface_size[0];
+  } else if (n_iw == nparams) {
+    for (int i = 0; i < nparams; i++) iw[i] = interface_size[i];
+  } else {
+    return 3; /* "Invalid size of interfaceSize array." :95 */
+  }
+
+  if (nfluid <= 2) {
+    /* kappa = 3*st/iw, kappaP = 3*st*iw  (:110-111) */
+    for (int i = 0; i < nparams; i++) {
+      pf->kappa[i] = (3 * st[i]) / iw[i];
+      pf->kappaP[i] = (3 * st[i]) * iw[i];
+    }
+  } else {
+    /* ternary combination (:116-131) */
+    double ks[ORC_MAXF], kps[ORC_MAXF];
+    for (int i = 0; i < nparams; i++) {
+      ks[i] = (6 * st[i]) / iw[i];
+      kps[i] = (6 * st[i]) * iw[i];
+    }
+    int l = nfluid - 1;
+    pf->kappa[0] = 0.5 * (ks[0] + ks[1] - ks[l]);
+    pf->kappa[1] = 0.5 * (ks[0] - ks[1] + ks[l]);
+    pf->kappa[2] = 0.5 * (-ks[0] + ks[1] + ks[l]);
+    pf->kappaP[0] = 0.5 * (kps[0] + kps[1] - kps[l]);
+    pf->kappaP[1] = 0.5 * (kps[0] - kps[1] + kps[l]);
+    pf->kappaP[2] = 0.5 * (-kps[0] + kps[1] + kps[l]);
+    for (int i = 3; i < nfluid; i++) {
+      pf->kappa[i] = ks[i - 1] - pf->kappa[0];
+      pf->kappaP[i] = kps[i - 1] - pf->kappaP[0];
+    }
+  }
+  pf->convergence = 1e-8 * pf->st_mean * pow(resolution, 2); /* :146 */
+  for (int i = 0; i < nfluid; i++) pf->fix[i] = fix_fluid ? fix_fluid[i] : 0;
+  if (density == ORC_DENSITY_FIXED) pf->fix[nfluid - 1] = 1; /* :147 */
+  return 0;
+}
+
+static int pf_idx(const orc_pf* pf, int x, int y, int z) {
+  /* getIdx with periodic wrap (PhaseField.cpp:44-49) */
+  const int* n = pf->grid;
+  x = (x + n[0]) % n[0];
+  y = (y + n[1]) % n[1];
+  z = (z + n[2]) % n[2];
+  return (x * n[1] + y) * n[2] + z;
+}
+
+/* PhaseField::energyGradient (PhaseField.cpp:662-703) with one thread, no solid
+ * nodes, nodeVol = resolution^3 everywhere (:194-208 with type 0), calling
+ * fluidEnergy (:365-393) -> phaseGradient (:238-288, "no solid" branch :252-263)
+ * and densityConstraintEnergy (:457-473).  Pressure, surface, force and
+ * confinement terms are inactive (zero) in this restatement.
+ * Caller semantics of State::basicEG (src/State.cpp:48-56): e is set (not
+ * accumulated) and g is zeroed before the scatter. */
+void orc_pf_energy_gradient(const orc_pf* pf, const double* x, double* e, double* g) {
+  const int nF = pf->nfluid;
+  const int* n = pf->grid;
+  const size_t ngrid = (size_t)n[0] * n[1] * n[2];
+  const double nodeVol = 1 * pow(pf->resolution, 3);
+  double eLocal = 0;
+  if (g) memset(g, 0, sizeof(double) * ngrid * nF);
+
+  for (int ix = 0; ix < n[0]; ix++) {
+    for (int iy = 0; iy < n[1]; iy++) {
+      for (int iz = 0; iz < n[2]; iz++) {
+        int iGrid = pf_idx(pf, ix, iy, iz);
+        int nb[6];
+        nb[0] = pf_idx(pf, ix - 1, iy, iz);
+        nb[1] = pf_idx(pf, ix + 1, iy, iz);
+        nb[2] = pf_idx(pf, ix, iy - 1, iz);
+        nb[3] = pf_idx(pf, ix, iy + 1, iz);
+        nb[4] = pf_idx(pf, ix, iy, iz - 1);
+        nb[5] = pf_idx(pf, ix, iy, iz + 1);
+        /* fluidEnergy */
+        for (int iF = 0; iF < nF; iF++) {
+          int iDof = iGrid * nF + iF;
+          double c = x[iDof];
+          if (nF == 1) {
+            double factor = pf->kappa[iF] / 16 * nodeVol;
+            if (e) eLocal += factor * pow(c + 1, 2) * pow(c - 1, 2);
+            if (g) g[iDof] += factor * 4 * c * (c * c - 1);
+          } else {
+            double factor = 0.5 * pf->kappa[iF] * nodeVol;
+            if (e) eLocal += factor * pow(c, 2) * pow(c - 1, 2);
+            if (g) g[iDof] += factor * 2 * c * (c - 1) * (2 * c - 1);
+          }
+          double factor = (nF == 1) ? 0.25 * pf->kappaP[iF] * nodeVol : 0.5 * pf->kappaP[iF] * nodeVol;
+          factor = factor / pow(pf->resolution, 2);
+          /* phaseGradient */
+          double grad2 = 0;
+          for (int iDir = 0; iDir < 3; iDir++) {
+            if (n[iDir] == 1) continue;
+            int im = nb[2 * iDir] * nF + iF;
+            int ip = nb[2 * iDir + 1] * nF + iF;
+            double gradm = c - x[im];
+            double gradp = c - x[ip];
+            if (e) grad2 += 0.5 * (pow(gradm, 2) + pow(gradp, 2));
+            if (g) {
+              g[iDof] += factor * (gradm + gradp);
+              g[im] -= factor * gradm;
+              g[ip] -= factor * gradp;
+            }
+          }
+          if (e) eLocal += factor * grad2;
+        }
+        /* densityConstraintEnergy (soft constraint) */
+        if (nF > 1 && pf->density == ORC_DENSITY_SOFT) {
+          double coef = pf->density_const * pf->st_mean * pow(pf->resolution, 2);
+          double rhoDiff = -1;
+          for (int iF = 0; iF < nF; iF++) rhoDiff += x[iGrid * nF + iF];
+          if (e) eLocal += coef * pow(rhoDiff, 2);
+          if (g) {
+            for (int iF = 0; iF < nF; iF++) g[iGrid * nF + iF] += 2 * coef * rhoDiff;
+          }
+        }
+      }
+    }
+  }
+  if (e) {
+    double tot = 0;
+    tot += eLocal; /* basicEG zeroes *e, the OpenMP region adds eLocal (:697-701) */
+    *e = tot;
+  }
+}
+
+/* PhaseField::specialisedConstraints (PhaseField.cpp:580-612): zero fixed
+ * fluids, then remove the per-site mean over the variable fluids (hard density
+ * constraint).  The volume projection (:616-658) is inactive here. */
+void orc_pf_constraints(const orc_pf* pf, double* data) {
+  const int nF = pf->nfluid;
+  const size_t ngrid = (size_t)pf->grid[0] * pf->grid[1] * pf->grid[2];
+  int var[ORC_MAXF], nvar = 0;
+  for (int iF = 0; iF < nF; iF++) {
+    if (pf->fix[iF]) {
+      for (size_t i = 0; i < ngrid; i++) data[i * nF + iF] = 0;
+    } else {
+      var[nvar++] = iF;
+    }
+  }
+  if (nF > 1 && pf->density == ORC_DENSITY_HARD) {
+    double normFactor = 1.0 / nvar;
+    for (size_t i = 0; i < ngrid; i++) {
+      double component = 0;
+      for (int k = 0; k < nvar; k++) component += data[i * nF + var[k]];
+      component *= normFactor;
+      for (int k = 0; k < nvar; k++) data[i * nF + var[k]] -= component;
+    }
+  }
+}
+
+static void pf_eg_cb(const void* ctx, const double* x, double* e, double* g) {
+  orc_pf_energy_gradient((const orc_pf*)ctx, x, e, g);
+}
+static void pf_con_cb(const void* ctx, const double* x, double* data) {
+  (void)x;
+  orc_pf_constraints((const orc_pf*)ctx, data);
+}
+void orc_pf_potential(const orc_pf* pf, orc_pot* out) {
+  out->ctx = pf;
+  out->ndof = (size_t)pf->grid[0] * pf->grid[1] * pf->grid[2] * pf->nfluid;
+  out->eg = pf_eg_cb;
+  out->constrain = pf_con_cb;
+  out->convergence = pf->convergence;
+}
+
+/* ------------------------------------------------------------------------- */
+/* State-level helpers                                                       */
+/* ------------------------------------------------------------------------- */
+
+/* CommGrid::dotProduct on one rank (src/communicators/CommGrid.cpp:187-194):
+ * serial loop over RangeI = every index in order. */
+double orc_dot(const double* a, const double* b, size_t n) {
+  double v = 0;
+  for (size_t i = 0; i < n; i++) v += a[i] * b[i];
+  return v;
+}
+
+typedef struct {
+  const orc_pot* pot;
+  long ne, ng;
+} orc_state;
+
+/* State::energy(coords) (src/State.cpp:134-152) -> basicEG with e only. */
+static double st_energy(orc_state* s, const double* x) {
+  double e = 0;
+  s->ne++;
+  s->pot->eg(s->pot->ctx, x, &e, NULL);
+  return e;
+}
+
+/* State::procGradient (src/State.cpp:243-248) -> basicEG with g, then
+ * Potential::applyConstraints (src/Potential.cpp:102-115). */
+static void st_gradient(orc_state* s, const double* x, double* g) {
+  s->ng++;
+  s->pot->eg(s->pot->ctx, x, NULL, g);
+  if (s->pot->constrain) s->pot->constrain(s->pot->ctx, x, g);
+}
+
+static void st_constrain(orc_state* s, const double* x, double* data) {
+  if (s->pot->constrain) s->pot->constrain(s->pot->ctx, x, data);
+}
+
+/* backtrackingLinesearch (src/minimisers/linesearch.cpp:8-29).  step is
+ * updated in place; x becomes x+step (the last, possibly unevaluated, trial). */
+static double linesearch(orc_state* s, double* x, double* step, double de0, double* tmp, size_t n) {
+  const double c = 0.5, tau = 0.5;
+  double t = -c * de0;
+  double e0 = st_energy(s, x);
+  double mult = 1;
+  for (size_t j = 0; j < n; j++) tmp[j] = x[j] + step[j];
+  for (int i = 0; i < 10; i++) {
+    double e = st_energy(s, tmp);
+    if (e0 - e >= t) break;
+    for (size_t j = 0; j < n; j++) step[j] = step[j] * tau;
+    for (size_t j = 0; j < n; j++) tmp[j] = x[j] + step[j];
+    mult = mult * tau;
+    t = t * tau;
+  }
+  memcpy(x, tmp, sizeof(double) * n);
+  return mult;
+}
+
+/* ------------------------------------------------------------------------- */
+/* L-BFGS                                                                    */
+/* ------------------------------------------------------------------------- */
+
+/* Minimiser::minimise (src/Minimiser.cpp:27-38) driving Lbfgs::init
+ * (src/minimisers/Lbfgs.cpp:34-38), Lbfgs::iteration (:41-81),
+ * Lbfgs::getDirection (:84-127) and Lbfgs::checkConvergence (:130-133). */
+int orc_lbfgs_minimise(const orc_pot* pot, double* x, int m, int max_iter, double max_step,
+                       double init_step, int ls, double convergence, orc_stats* st) {
+  size_t n = pot->ndof;
+  orc_state s = {pot, 0, 0};
+  double* g = calloc(n, sizeof(double));
+  double* gNew = calloc(n, sizeof(double));
+  double* step = calloc(n, sizeof(double));
+  double* tmp = calloc(n, sizeof(double));
+  double* S = calloc((size_t)m * n, sizeof(double));
+  double* Y = calloc((size_t)m * n, sizeof(double));
+  double* rho = calloc(m, sizeof(double));
+  double* alpha = calloc(m, sizeof(double));
+  if (!g || !gNew || !step || !tmp || !S || !Y || !rho || !alpha) return 1;
+  int I = 0, iter, conv = 0;
+  for (iter = 0; iter <= max_iter; iter++) {
+    /* ---- Lbfgs::iteration ---- */
+    if (iter == 0) {
+      st_gradient(&s, x, g);
+      I = 0;
+    } else {
+      I++;
+    }
+    /* getDirection */
+    int m_tmp = I < m ? I : m;
+    int ic = I % m;
+    for (size_t j = 0; j < n; j++) step[j] = -g[j];
+    if (I == 0) {
+      double gnorm = sqrt(orc_dot(g, g, n));
+      if (gnorm > 0) {
+        double f = init_step / gnorm;
+        for (size_t j = 0; j < n; j++) step[j] *= f;
+      }
+    } else {
+      for (int i1 = 0; i1 < m_tmp; i1++) {
+        int k = ((ic - 1 - i1) + m) % m;
+        alpha[k] = rho[k] * orc_dot(step, S + (size_t)k * n, n);
+        const double* yk = Y + (size_t)k * n;
+        for (size_t j = 0; j < n; j++) step[j] -= alpha[k] * yk[j];
+      }
+      int k = (ic - 1 + m) % m;
+      double gamma = 1 / (rho[k] * orc_dot(Y + (size_t)k * n, Y + (size_t)k * n, n));
+      for (size_t j = 0; j < n; j++) step[j] *= gamma;
+      for (int i1 = 0; i1 < m_tmp; i1++) {
+        int kk = ((ic - m_tmp + i1) + m) % m;
+        double beta = rho[kk] * orc_dot(step, Y + (size_t)kk * n, n);
+        double f = alpha[kk] - beta;
+        const double* sk = S + (size_t)kk * n;
+        for (size_t j = 0; j < n; j++) {
+          double t = f * sk[j];
+          step[j] += t;
+        }
+      }
+      if (max_step != 0) {
+        double ss = sqrt(orc_dot(step, step, n));
+        if (ss > max_step) {
+          double f = max_step / ss;
+          for (size_t j = 0; j < n; j++) step[j] *= f;
+        }
+      }
+    }
+    st_constrain(&s, x, step);
+    double gs = orc_dot(g, step, n);
+    if (gs > 0) {
+      gs = -gs;
+      for (size_t j = 0; j < n; j++) step[j] = -step[j];
+    }
+    if (ls == ORC_LS_BACKTRACKING) {
+      linesearch(&s, x, step, gs, tmp, n);
+    } else {
+      for (size_t j = 0; j < n; j++) x[j] = x[j] + step[j];
+    }
+    st_gradient(&s, x, gNew);
+    for (size_t j = 0; j < n; j++) tmp[j] = gNew[j] - g[j];
+    double sy = orc_dot(step, tmp, n);
+    if (sy != 0) {
+      int k = I % m;
+      memcpy(S + (size_t)k * n, step, sizeof(double) * n);
+      memcpy(Y + (size_t)k * n, tmp, sizeof(double) * n);
+      rho[k] = 1 / sy;
+    } else {
+      I--;
+    }
+    memcpy(g, gNew, sizeof(double) * n);
+    /* checkConvergence */
+    double rms = sqrt(orc_dot(g, g, n) / n);
+    if (rms < convergence) {
+      conv = 1;
+      break;
+    }
+  }
+  if (st) {
+    st->iterations = iter;
+    st->converged = conv;
+    st->grad_rms = sqrt(orc_dot(g, g, n) / n);
+    double e = 0;
+    pot->eg(pot->ctx, x, &e, NULL);
+    st->energy = e;
+    st->n_energy = s.ne;
+    st->n_gradient = s.ng;
+  }
+  free(g); free(gNew); free(step); free(tmp); free(S); free(Y); free(rho); free(alpha);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* FIRE                                                                      */
+/* ------------------------------------------------------------------------- */
+
+/* Fire::init (src/minimisers/Fire.cpp:27-38), Fire::iteration (:41-84),
+ * Fire::checkConvergence (:87-90); constants include/minimisers/Fire.h:24-28.
+ * dt_max == 0 requests the reference's estimate 0.1/sqrt(|g|). */
+int orc_fire_minimise(const orc_pot* pot, double* x, double dt_max, int max_iter,
+                      int ls, double convergence, orc_stats* st, double* dt_max_out) {
+  const int nMin = 5;
+  const double fInc = 1.1, fDec = 0.5, fA = 0.99, aStart = 0.1;
+  size_t n = pot->ndof;
+  orc_state s = {pot, 0, 0};
+  double* g = calloc(n, sizeof(double));
+  double* v = calloc(n, sizeof(double));
+  double* step = calloc(n, sizeof(double));
+  double* tmp = calloc(n, sizeof(double));
+  if (!g || !v || !step || !tmp) return 1;
+  double gNorm = 0, a = aStart, dt = 0;
+  int nSteps = 0;
+  if (dt_max == 0) {
+    st_gradient(&s, x, g);
+    gNorm = sqrt(orc_dot(g, g, n));
+    dt_max = (gNorm != 0) ? 0.1 / sqrt(gNorm) : 1;
+  }
+  int iter, conv = 0;
+  for (iter = 0; iter <= max_iter; iter++) {
+    if (iter == 0) {
+      dt = dt_max;
+      st_gradient(&s, x, g);
+      gNorm = sqrt(orc_dot(g, g, n));
+    }
+    double p = -orc_dot(v, g, n);
+    if (p > 0) {
+      double vNorm = sqrt(orc_dot(v, v, n));
+      double f1 = 1 - a;
+      double f2 = a * vNorm / gNorm + dt;
+      for (size_t j = 0; j < n; j++) {
+        double t1 = f1 * v[j];
+        double t2 = f2 * g[j];
+        v[j] = t1 - t2;
+      }
+      nSteps++;
+    } else {
+      double f = -dt;
+      for (size_t j = 0; j < n; j++) v[j] = f * g[j];
+      nSteps = 0;
+    }
+    if ((p > 0) && (nSteps > nMin)) {
+      a = a * fA;
+      dt = dt * fInc;
+      if (dt > dt_max) dt = dt_max;
+    } else if (p <= 0) {
+      a = aStart;
+      dt = dt * fDec;
+    }
+    for (size_t j = 0; j < n; j++) step[j] = dt * v[j];
+    st_constrain(&s, x, step);
+    if (ls == ORC_LS_BACKTRACKING) {
+      double gs = orc_dot(g, step, n);
+      linesearch(&s, x, step, gs, tmp, n);
+    } else {
+      for (size_t j = 0; j < n; j++) x[j] = x[j] + step[j];
+    }
+    st_gradient(&s, x, g);
+    gNorm = sqrt(orc_dot(g, g, n));
+    if (gNorm / sqrt((double)n) < convergence) {
+      conv = 1;
+      break;
+    }
+  }
+  if (st) {
+    st->iterations = iter;
+    st->converged = conv;
+    st->grad_rms = gNorm / sqrt((double)n);
+    double e = 0;
+    pot->eg(pot->ctx, x, &e, NULL);
+    st->energy = e;
+    st->n_energy = s.ne;
+    st->n_gradient = s.ng;
+  }
+  if (dt_max_out) *dt_max_out = dt_max;
+  free(g); free(v); free(step); free(tmp);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Gradient descent                                                          */
+/* ------------------------------------------------------------------------- */
+
+/* GradDescent::iteration / checkConvergence (src/minimisers/GradDescent.cpp:27-46). */
+int orc_graddescent_minimise(const orc_pot* pot, double* x, double alpha, int max_iter,
+                             int ls, double convergence, orc_stats* st) {
+  size_t n = pot->ndof;
+  orc_state s = {pot, 0, 0};
+  double* g = calloc(n, sizeof(double));
+  double* step = calloc(n, sizeof(double));
+  double* tmp = calloc(n, sizeof(double));
+  if (!g || !step || !tmp) return 1;
+  int iter, conv = 0;
+  for (iter = 0; iter <= max_iter; iter++) {
+    st_gradient(&s, x, g);
+    double f = -alpha;
+    for (size_t j = 0; j < n; j++) step[j] = f * g[j];
+    if (ls == ORC_LS_BACKTRACKING) {
+      double gs = orc_dot(g, step, n);
+      linesearch(&s, x, step, gs, tmp, n);
+    } else {
+      for (size_t j = 0; j < n; j++) x[j] = x[j] + step[j];
+    }
+    if (sqrt(orc_dot(g, g, n) / n) < convergence) {
+      conv = 1;
+      break;
+    }
+  }
+  if (st) {
+    st->iterations = iter;
+    st->converged = conv;
+    st->grad_rms = sqrt(orc_dot(g, g, n) / n);
+    double e = 0;
+    pot->eg(pot->ctx, x, &e, NULL);
+    st->energy = e;
+    st->n_energy = s.ne;
+    st->n_gradient = s.ng;
+  }
+  free(g); free(step); free(tmp);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Lennard-Jones                                                             */
+/* ------------------------------------------------------------------------- */
+
+/* LjNd::init element order (src/potentials/LjNd.cpp:12-26: all i<j pairs) and
+ * LjNd::elementEnergyGradient (:29-53), accumulated as State::elementEG
+ * (src/State.cpp:58-79). */
+void orc_lj_energy_gradient(const orc_lj* lj, const double* x, double* e, double* g) {
+  const int D = lj->ndim, N = lj->nparticle;
+  double etot = 0;
+  if (g) memset(g, 0, sizeof(double) * (size_t)N * D);
+  double dx[8];
+  for (int i = 0; i < N; i++) {
+    for (int j = i + 1; j < N; j++) {
+      double r2 = 0;
+      for (int d = 0; d < D; d++) {
+        dx[d] = x[D * i + d] - x[D * j + d];
+        r2 += dx[d] * dx[d];
+      }
+      double lj6 = pow(lj->sigma, 6) / pow(r2, 3);
+      double lj12 = lj6 * lj6;
+      if (e) etot += 4 * lj->epsilon * (lj12 - lj6);
+      if (g) {
+        double factor = -24 * lj->epsilon * (2 * lj12 - lj6) / r2;
+        for (int d = 0; d < D; d++) {
+          g[D * i + d] += dx[d] * factor;
+          g[D * j + d] -= dx[d] * factor;
+        }
+      }
+    }
+  }
+  if (e) *e = etot;
+}
+
+static void lj_eg_cb(const void* ctx, const double* x, double* e, double* g) {
+  orc_lj_energy_gradient((const orc_lj*)ctx, x, e, g);
+}
+void orc_lj_potential(const orc_lj* lj, size_t ndof, orc_pot* out) {
+  out->ctx = lj;
+  out->ndof = ndof;
+  out->eg = lj_eg_cb;
+  out->constrain = NULL;
+  out->convergence = 1e-6; /* Potential::convergence default (include/Potential.h:25) */
+}
+
+/* Truncated-and-shifted LJ in a periodic cube, minimum image, all pairs.
+ * E_pair = 4 eps [(s/r)^12 - (s/r)^6] - 4 eps [(s/rc)^12 - (s/rc)^6] for r < rc.
+ * New semantics (the reference has no cutoff or periodic box); pinned in
+ * tests against orc_lj_energy_gradient for rc > all distances, continuity at
+ * rc, finite differences and momentum conservation. */
+void orc_ljpbc_energy_gradient(const orc_ljpbc* lj, const double* x, double* e, double* g) {
+  const int N = lj->nparticle;
+  const double L = lj->box, rc2 = lj->rc * lj->rc;
+  double src6 = pow(lj->sigma / lj->rc, 6);
+  double shift = 4 * lj->epsilon * (src6 * src6 - src6);
+  double etot = 0;
+  if (g) memset(g, 0, sizeof(double) * (size_t)N * 3);
+  for (int i = 0; i < N; i++) {
+    for (int j = i + 1; j < N; j++) {
+      double dx[3], r2 = 0;
+      for (int d = 0; d < 3; d++) {
+        double t = x[3 * i + d] - x[3 * j + d];
+        t -= L * nearbyint(t / L);
+        dx[d] = t;
+        r2 += t * t;
+      }
+      if (r2 >= rc2) continue;
+      double s2 = lj->sigma * lj->sigma / r2;
+      double lj6 = s2 * s2 * s2;
+      double lj12 = lj6 * lj6;
+      if (e) etot += 4 * lj->epsilon * (lj12 - lj6) - shift;
+      if (g) {
+        double factor = -24 * lj->epsilon * (2 * lj12 - lj6) / r2;
+        for (int d = 0; d < 3; d++) {
+          g[3 * i + d] += dx[d] * factor;
+          g[3 * j + d] -= dx[d] * factor;
+        }
+      }
+    }
+  }
+  if (e) *e = etot;
+}
+
+/* ------------------------------------------------------------------------- */
+/* BarAndHinge                                                               */
+/* ------------------------------------------------------------------------- */
+
+static double orc_pi(void) { return 4 * atan(1); }
+
+static int all_fixed(const orc_bah* b, const int* nodes, int nn) {
+  if (!b->fixed) return 0;
+  for (int k = 0; k < nn; k++)
+    for (int d = 0; d < 3; d++)
+      if (!b->fixed[3 * nodes[k] + d]) return 0;
+  return 1;
+}
+
+/* Element order of BarAndHinge::init (src/potentials/BarAndHinge.cpp:16-67):
+ * bars, then hinges, then one external-force element per node, skipping
+ * elements whose DOFs are all fixed.  Kernels: stretching (:269-290, including
+ * the reference's factor 2 in the gradient, :282), bending (:293-356),
+ * forceEnergy (:359-369).  The substrate wall (:372-387) is not restated. */
+void orc_bah_energy_gradient(const orc_bah* b, const double* x, double* e, double* g) {
+  double etot = 0;
+  if (g) memset(g, 0, sizeof(double) * 3 * (size_t)b->nnode);
+  for (int ib = 0; ib < b->nbar; ib++) {
+    const int* nd = b->bar + 2 * ib;
+    if (all_fixed(b, nd, 2)) continue;
+    double dx[3];
+    for (int d = 0; d < 3; d++) dx[d] = x[3 * nd[0] + d] - x[3 * nd[1] + d];
+    double l = sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+    double dl = l - b->bar_l0[ib];
+    if (e) etot += 0.5 * b->bar_k[ib] * pow(dl, 2);
+    if (g) {
+      double gf = 2 * b->bar_k[ib] * dl / l;
+      for (int d = 0; d < 3; d++) g[3 * nd[0] + d] += gf * dx[d];
+      for (int d = 0; d < 3; d++) g[3 * nd[1] + d] -= gf * dx[d];
+    }
+  }
+  const double pi = orc_pi();
+  for (int ih = 0; ih < b->nhinge; ih++) {
+    const int* nd = b->hinge + 4 * ih;
+    if (all_fixed(b, nd, 4)) continue;
+    double x1[3], x2[3], x3[3], x4[3], b1[3], b2[3], b3[3];
+    for (int d = 0; d < 3; d++) {
+      x1[d] = x[3 * nd[0] + d];
+      x2[d] = x[3 * nd[1] + d];
+      x3[d] = x[3 * nd[2] + d];
+      x4[d] = x[3 * nd[3] + d];
+    }
+    for (int d = 0; d < 3; d++) {
+      b1[d] = x2[d] - x1[d];
+      b2[d] = x3[d] - x2[d];
+      b3[d] = x4[d] - x3[d];
+    }
+    double b2m = sqrt(b2[0] * b2[0] + b2[1] * b2[1] + b2[2] * b2[2]);
+    double n1[3] = {b1[1] * b2[2] - b1[2] * b2[1], b1[2] * b2[0] - b1[0] * b2[2], b1[0] * b2[1] - b1[1] * b2[0]};
+    double n2[3] = {b2[1] * b3[2] - b2[2] * b3[1], b2[2] * b3[0] - b2[0] * b3[2], b2[0] * b3[1] - b2[1] * b3[0]};
+    double n1sq = n1[0] * n1[0] + n1[1] * n1[1] + n1[2] * n1[2];
+    double n2sq = n2[0] * n2[0] + n2[1] * n2[1] + n2[2] * n2[2];
+    double n12m = sqrt(n1sq * n2sq);
+    double c = (n1[0] * n2[0] + n1[1] * n2[1] + n1[2] * n2[2]) / n12m;
+    c = c < -1.0 ? -1.0 : c;
+    c = c > 1.0 ? 1.0 : c;
+    double n1b3 = n1[0] * b3[0] + n1[1] * b3[1] + n1[2] * b3[2];
+    double theta = (n1b3 >= 0) ? acos(c) : 2 * pi - acos(c);
+    double dtheta = theta - b->hinge_t0[ih];
+    dtheta = fmod(dtheta + pi, 2 * pi) - pi;
+    if (e) etot += 0.5 * b->hinge_k[ih] * pow(dtheta, 2);
+    if (g) {
+      double gf = b->hinge_k[ih] * (dtheta);
+      double f1 = b2m / n1sq, f2 = b2m / n2sq;
+      double n1h[3] = {f1 * n1[0], f1 * n1[1], f1 * n1[2]};
+      double n2h[3] = {f2 * n2[0], f2 * n2[1], f2 * n2[2]};
+      double skew1 = -(b1[0] * b2[0] + b1[1] * b2[1] + b1[2] * b2[2]) / (b2m * b2m);
+      double skew2 = -(b3[0] * b2[0] + b3[1] * b2[1] + b3[2] * b2[2]) / (b2m * b2m);
+      for (int d = 0; d < 3; d++) g[3 * nd[0] + d] += -gf * n1h[d];
+      for (int d = 0; d < 3; d++) g[3 * nd[1] + d] += gf * ((1 - skew1) * n1h[d] - skew2 * n2h[d]);
+      for (int d = 0; d < 3; d++) g[3 * nd[2] + d] += -gf * ((1 - skew2) * n2h[d] - skew1 * n1h[d]);
+      for (int d = 0; d < 3; d++) g[3 * nd[3] + d] += gf * n2h[d];
+    }
+  }
+  for (int in = 0; in < b->nnode; in++) {
+    if (all_fixed(b, &in, 1)) continue;
+    double fz = 0;
+    if (b->force) fz = b->force_per_node ? b->force[3 * in + 2] : b->force[2];
+    if (e) etot -= fz * x[3 * in + 2];
+    if (g) g[3 * in + 2] -= fz;
+  }
+  if (e) *e = etot;
+}
+
+/* Potential::applyConstraints for single-DOF constraints (src/Potential.cpp:102-115). */
+static void bah_con_cb(const void* ctx, const double* x, double* data) {
+  (void)x;
+  const orc_bah* b = (const orc_bah*)ctx;
+  if (!b->fixed) return;
+  for (int i = 0; i < 3 * b->nnode; i++)
+    if (b->fixed[i]) data[i] = 0;
+}
+static void bah_eg_cb(const void* ctx, const double* x, double* e, double* g) {
+  orc_bah_energy_gradient((const orc_bah*)ctx, x, e, g);
+}
+void orc_bah_potential(const orc_bah* b, orc_pot* out) {
+  out->ctx = b;
+  out->ndof = 3 * (size_t)b->nnode;
+  out->eg = bah_eg_cb;
+  out->constrain = bah_con_cb;
+  out->convergence = 1e-6;
+}

@@ -1,0 +1,114 @@
+/*
+ * oracle.h -- CPU restatement of sjavis/minim's energy/gradient and minimiser
+ * hot path, used ONLY as the checker in tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py.  Nothing in the product (minim_amd/) links,
+ * loads or calls this code.
+ *
+ * Every routine is plain serial FP64 C that follows the reference's operation
+ * order (file:line cited at each function) so that, compiled with
+ * -ffp-contract=off, it reproduces the reference's single-thread results bit for
+ * bit.  Parity status of each function is listed in oracle.c's header.
+ */
+#ifndef MINIM_ORACLE_H
+#define MINIM_ORACLE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAXF 16
+
+enum { ORC_DENSITY_HARD = 0, ORC_DENSITY_SOFT = 1, ORC_DENSITY_NONE = 2, ORC_DENSITY_FIXED = 3 };
+
+/* ---- generic potential interface (State::basicEG / elementEG semantics) ---- */
+typedef struct orc_pot {
+  const void* ctx;
+  size_t ndof;
+  /* e, g may be NULL.  g is overwritten (zeroed then accumulated). */
+  void (*eg)(const void* ctx, const double* x, double* e, double* g);
+  /* in-place projection of a gradient/step (Potential::applyConstraints) */
+  void (*constrain)(const void* ctx, const double* x, double* data);
+  double convergence;
+} orc_pot;
+
+/* ---- PhaseField (grid potential, no solid nodes) ---- */
+typedef struct orc_pf {
+  int grid[3];
+  int nfluid;
+  double resolution;
+  double kappa[ORC_MAXF];
+  double kappaP[ORC_MAXF];
+  double st_mean;
+  int density;
+  double density_const;
+  unsigned char fix[ORC_MAXF];
+  double convergence;
+} orc_pf;
+
+/* Returns 0 on success, nonzero on the reference's std::invalid_argument cases. */
+int orc_pf_init(orc_pf* pf, const int grid[3], int nfluid, double resolution,
+                const double* surface_tension, int n_st,
+                const double* interface_size, int n_iw,
+                int density, double density_const, const unsigned char* fix_fluid);
+void orc_pf_energy_gradient(const orc_pf* pf, const double* x, double* e, double* g);
+void orc_pf_constraints(const orc_pf* pf, double* data);
+void orc_pf_potential(const orc_pf* pf, orc_pot* out);
+
+/* ---- reductions (CommGrid::dotProduct, single rank) ---- */
+double orc_dot(const double* a, const double* b, size_t n);
+
+/* ---- minimisers ---- */
+enum { ORC_LS_BACKTRACKING = 0, ORC_LS_NONE = 1 };
+
+typedef struct orc_stats {
+  int iterations;      /* value of Minimiser::iter when the loop ended */
+  int converged;
+  long n_energy;       /* energy-only evaluations */
+  long n_gradient;     /* gradient-only evaluations */
+  double energy;       /* energy at the returned coordinates */
+  double grad_rms;     /* sqrt(g.g/ndof) of the last gradient */
+} orc_stats;
+
+int orc_lbfgs_minimise(const orc_pot* pot, double* x, int m, int max_iter, double max_step,
+                       double init_step, int linesearch, double convergence, orc_stats* st);
+int orc_fire_minimise(const orc_pot* pot, double* x, double dt_max, int max_iter,
+                      int linesearch, double convergence, orc_stats* st, double* dt_max_out);
+int orc_graddescent_minimise(const orc_pot* pot, double* x, double alpha, int max_iter,
+                             int linesearch, double convergence, orc_stats* st);
+
+/* ---- Lennard-Jones, all pairs (reference LjNd semantics) ---- */
+typedef struct orc_lj {
+  int ndim;
+  int nparticle;
+  double sigma;
+  double epsilon;
+} orc_lj;
+void orc_lj_energy_gradient(const orc_lj* lj, const double* x, double* e, double* g);
+void orc_lj_potential(const orc_lj* lj, size_t ndof, orc_pot* out);
+
+/* ---- Lennard-Jones, truncated and shifted at rc, periodic cube of side L,
+ *      minimum image, brute force over all pairs (new semantics, no reference) ---- */
+typedef struct orc_ljpbc {
+  int nparticle;
+  double sigma, epsilon, rc, box;
+} orc_ljpbc;
+void orc_ljpbc_energy_gradient(const orc_ljpbc* lj, const double* x, double* e, double* g);
+
+/* ---- BarAndHinge element kernels (bars, hinges, external z force) ---- */
+typedef struct orc_bah {
+  int nnode;
+  int nbar;   const int* bar;   const double* bar_k;   const double* bar_l0;   /* bar[2*i]  */
+  int nhinge; const int* hinge; const double* hinge_k; const double* hinge_t0; /* hinge[4*i] */
+  const double* force;        /* NULL, a 3-vector (force_per_node=0) or 3*nnode values */
+  int force_per_node;
+  const unsigned char* fixed; /* NULL or per-DOF fixed flags (Potential::setConstraints(iFix)) */
+} orc_bah;
+void orc_bah_energy_gradient(const orc_bah* b, const double* x, double* e, double* g);
+void orc_bah_potential(const orc_bah* b, orc_pot* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
