@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""Benchmark: energy+gradient evals/s and time-to-converge of the 256^3
+three-fluid PhaseField, mixed precision vs FP64, on one MI355X.
+
+One step = one energy+gradient evaluation of the whole 256^3 x 3 field (the
+stencil pass of the minimiser's hot path), inputs resident in HBM.  The
+headline value is the mixed path's evaluation (FP32 storage and arithmetic,
+FP64 energy accumulation); the FP64 path is reported beside it, as is the
+time-to-converge of full device L-BFGS (m=5) runs in both precisions and the
+reference library timed on the host cores.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "energy+grad evals/s and time-to-converge, 256³ 3-fluid PF: mixed vs FP64"
+HBM_PEAK_GBS = 8000.0
+GRID = (256, 256, 256)
+NF = 3
+SIGMA = [1.0, 0.8, 1.2]
+WIDTH = 2.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_input(grid, seed=2024):
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(0.0, 1.0, (int(np.prod(grid)), NF))
+    return (u / u.sum(axis=1, keepdims=True)).reshape(-1)
+
+
+def time_evals(m, h, prec, x, g, e_dev, n, steps, warmup, stream, with_energy_copy=True):
+    import torch
+
+    with torch.cuda.stream(stream):
+        for _ in range(warmup):
+            m.potential_eval_async(h, prec, x.data_ptr(), n, g_ptr=g.data_ptr(),
+                                   e_dev_ptr=e_dev.data_ptr() if with_energy_copy else None,
+                                   want_energy=True, stream=stream.cuda_stream)
+        torch.cuda.synchronize()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record(stream)
+        for _ in range(steps):
+            m.potential_eval_async(h, prec, x.data_ptr(), n, g_ptr=g.data_ptr(),
+                                   e_dev_ptr=e_dev.data_ptr() if with_energy_copy else None,
+                                   want_energy=True, stream=stream.cuda_stream)
+        t1.record(stream)
+        torch.cuda.synchronize()
+    return t0.elapsed_time(t1) / steps  # ms
+
+
+def converge(m, x0, mode, max_iter=20000):
+    import torch
+
+    pf = m.PhaseField().setNFluid(NF).setGridSize(list(GRID)).setSurfaceTension(SIGMA).setInterfaceSize(WIDTH)
+    s = m.State(pf, x0, "fp64")
+    e0 = s.energy()
+    mn = m.Lbfgs().setM(5).setMaxIter(max_iter).setPrecision(mode)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    mn.minimise(s)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    st = dict(mn.stats)
+    st.update(seconds=dt, initial_energy=e0, final_energy=s.energy())
+    return st
+
+
+def cpu_baseline(x, budget_s=12.0):
+    """The reference library (oracle/_ref, built from its own sources) timed on
+    the host cores: State construction excluded, then as many
+    State::energyGradient calls on the full 256^3 x 3 field as fit the budget."""
+    import numpy as np
+
+    threads = int(os.environ.get("OMP_NUM_THREADS") or min(os.cpu_count() or 1, 16))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    from oracle_bind import RefState, ref_available
+
+    if ref_available():
+        t = time.perf_counter()
+        r = RefState.phasefield(GRID, x, NF, 1.0, SIGMA, [WIDTH], "hard")
+        setup = time.perf_counter() - t
+        n, t = 0, time.perf_counter()
+        while True:
+            r.energy_gradient(x)
+            n += 1
+            if time.perf_counter() - t > budget_s or n >= 10:
+                break
+        el = time.perf_counter() - t
+        return dict(value=n / el, unit="evals/s", cores=threads, kind="reference",
+                    sample="%d State::energyGradient calls on the 256^3x3 field (reference compiled from source, "
+                           "OpenMP %d threads); State construction %.1f s excluded" % (n, threads, setup))
+    from oracle_bind import OraclePF
+
+    o = OraclePF(GRID, NF, 1.0, SIGMA, [WIDTH])
+    n, t = 0, time.perf_counter()
+    while True:
+        o.energy(x)
+        o.gradient(x)
+        n += 1
+        if time.perf_counter() - t > budget_s:
+            break
+    el = time.perf_counter() - t
+    return dict(value=n / el, unit="evals/s", cores=1, kind="port",
+                sample="%d energy+gradient evaluations of the 256^3x3 field by the serial C restatement" % n)
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the FP32 stencil from the committed rocprofv3
+    --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+    p = os.path.join(ROOT, "profiles", "pf_stencil_256cubed_traffic.json")
+    try:
+        with open(p) as f:
+            return json.load(f).get("fp32_eg_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-converge", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the whole 256^3 field fits one MI355X); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import minim_amd as m
+
+    torch.cuda.set_device(0)
+    n = int(np.prod(GRID)) * NF
+    log("input %s x %d (%d DOF)" % (GRID, NF, n))
+    x0 = make_input(GRID)
+    pf = m.PhaseField().setNFluid(NF).setGridSize(list(GRID)).setSurfaceTension(SIGMA).setInterfaceSize(WIDTH)
+    h = pf.handle(x0)
+    stream = torch.cuda.Stream()
+    x64 = torch.from_numpy(x0).cuda()
+    x32 = x64.float()
+    g64 = torch.empty_like(x64)
+    g32 = torch.empty_like(x32)
+    e_dev = torch.zeros(1, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+
+    ms32 = time_evals(m, h, "fp32", x32, g32, e_dev, n, args.steps, args.warmup, stream)
+    ms32k = time_evals(m, h, "fp32", x32, g32, e_dev, n, args.steps, 1, stream, with_energy_copy=False)
+    ms64 = time_evals(m, h, "fp64", x64, g64, e_dev, n, args.steps, args.warmup, stream)
+    ms64k = time_evals(m, h, "fp64", x64, g64, e_dev, n, args.steps, 1, stream, with_energy_copy=False)
+    log("E+G fp32 %.3f ms (kernel %.3f), fp64 %.3f ms (kernel %.3f)" % (ms32, ms32k, ms64, ms64k))
+
+    sites = int(np.prod(GRID))
+    bytes32 = sites * NF * (4 + 4)  # coordinates read once + gradient written once
+    bytes64 = sites * NF * (8 + 8)
+    ach32 = bytes32 / (ms32k * 1e-3) / 1e9
+    ach64 = bytes64 / (ms64k * 1e-3) / 1e9
+
+    ttc = None
+    if not args.no_converge:
+        del x64, x32, g64, g32
+        torch.cuda.empty_cache()
+        res = {}
+        for mode in ("fp64", "mixed"):
+            log("L-BFGS time-to-converge, %s ..." % mode)
+            res[mode] = converge(m, x0, mode)
+            log("  %s: %.3f s, %d it, E %.6g -> %.6g, gRMS %.3g" % (
+                mode, res[mode]["seconds"], res[mode]["iterations"], res[mode]["initial_energy"],
+                res[mode]["final_energy"], res[mode]["grad_rms"]))
+        e_init = res["fp64"]["initial_energy"]
+        ttc = dict(
+            mixed_s=res["mixed"]["seconds"], fp64_s=res["fp64"]["seconds"],
+            speedup=res["fp64"]["seconds"] / res["mixed"]["seconds"],
+            mixed=res["mixed"], fp64=res["fp64"],
+            energy_agreement=abs(res["mixed"]["final_energy"] - res["fp64"]["final_energy"]) / max(
+                abs(res["fp64"]["final_energy"]), abs(e_init)),
+            tolerance="gradient RMS < 1e-8 (reference convergence for sigma_mean=1, res=1); energies compared "
+                      "with the absolute floor 1e-8*E_initial since the minimum is E~0")
+
+    cpu = None if args.no_cpu else cpu_baseline(x0, args.cpu_budget)
+
+    out = {
+        "metric": METRIC,
+        "value": 1e3 / ms32,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: concentrations u~U[0,1] per fluid (numpy default_rng seed 2024), normalised per site",
+        "config": {"workload": "PhaseField 256^3 periodic grid x 3 fluids, surface tensions (1.0,0.8,1.2), "
+                               "interface width 2, hard density constraint; one step = one energy+gradient "
+                               "evaluation (FP32 storage/arithmetic, FP64 energy accumulation)",
+                   "grid": list(GRID), "n_fluid": NF, "dof": n, "parallelism": "single GPU"},
+        "fp64_evals_per_s": 1e3 / ms64,
+        "fp64_ms_per_step": ms64,
+        "roofline": {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach32 / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
+                     "kernel": "pf_kernel<float,3,E+G>", "kernel_ms": ms32k, "algorithmic_bytes": bytes32,
+                     "fp64": {"achieved": ach64, "frac": ach64 / HBM_PEAK_GBS, "kernel_ms": ms64k,
+                              "algorithmic_bytes": bytes64}},
+        "time_to_converge": ttc,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,143 @@
+// Minimiser.h -- device-resident minimisers, mirroring the reference's
+// Minimiser (include/Minimiser.h:13-33), Lbfgs (include/minimisers/Lbfgs.h:14-38),
+// Fire (include/minimisers/Fire.h:9-36) and GradDescent
+// (include/minimisers/GradDescent.h:9-23).
+#ifndef MINIM_AMD_MINIMISER_H
+#define MINIM_AMD_MINIMISER_H
+
+#include <functional>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace minim {
+class State;
+
+// Counters and timings filled by every minimise() call.
+struct MinimiseStats {
+  long nEnergy = 0;      // energy-only passes (line-search trials)
+  long nGradient = 0;    // gradient passes
+  long nEnergyReused = 0;  // e0 values reused instead of re-evaluated
+  int iterFP32 = 0;      // iterations run with FP32 storage (mixed schedule)
+  int iterFP64 = 0;
+  bool converged = false;
+  double gradRms = 0;    // sqrt(g.g/ndof) of the last gradient
+  double energy = 0;     // last accepted energy (FP64 accumulation)
+};
+
+class Minimiser {
+ public:
+  int maxIter = 100000;
+  std::string linesearch = "backtracking";
+  int iter = 0;
+
+  virtual ~Minimiser() = default;
+  virtual std::unique_ptr<Minimiser> clone() const = 0;
+
+  virtual Minimiser& setMaxIter(int maxIter);
+  Minimiser& setLinesearch(std::string method);
+
+  std::vector<double> minimise(State& state, std::function<void(int, State&)> adjustState = nullptr);
+  std::vector<double> minimise(State& state, std::string logType);
+
+  virtual void init(State& state) {}
+  virtual void iteration(State& state) = 0;
+  virtual bool checkConvergence(const State& state) { return false; }
+  virtual void finish(State& state) {}
+
+  MinimiseStats stats;
+};
+
+template <typename Derived>
+class NewMinimiser : public Minimiser {
+ public:
+  std::unique_ptr<Minimiser> clone() const override {
+    return std::make_unique<Derived>(static_cast<const Derived&>(*this));
+  }
+};
+
+// L-BFGS with the reference's semantics (src/minimisers/Lbfgs.cpp:41-133 and
+// src/minimisers/linesearch.cpp:8-29), evaluated as a Gram-matrix recursion
+// over device vectors.
+class Lbfgs : public NewMinimiser<Lbfgs> {
+ public:
+  Lbfgs();
+  Lbfgs(const Lbfgs& o);
+  Lbfgs& operator=(const Lbfgs& o);
+  ~Lbfgs() override;
+
+  Lbfgs& setM(int m);
+  Lbfgs& setMaxIter(int maxIter) override;
+  Lbfgs& setMaxStep(double maxStep);
+  Lbfgs& setInitStep(double initStep);
+  // "state" (use the State's precision), "fp64", "fp32" or "mixed": FP32
+  // storage until sqrt(g.g/ndof) < switchRms (or the FP32 line search stalls),
+  // then FP64 storage to the requested convergence.
+  Lbfgs& setPrecision(std::string mode, double switchRms = 0);
+
+  void init(State& state) override;
+  void iteration(State& state) override;
+  bool checkConvergence(const State& state) override;
+  void finish(State& state) override;
+
+  int m() const { return _m; }
+
+ private:
+  int _m = 5;
+  double _maxStep = 0;
+  double _initStep = 1;
+  std::string _precMode = "state";
+  double _switchRms = 0;
+  struct Impl;
+  std::unique_ptr<Impl> impl_;
+};
+
+class Fire : public NewMinimiser<Fire> {
+ public:
+  Fire();
+  explicit Fire(double dtMax);
+  Fire(const Fire& o);
+  Fire& operator=(const Fire& o);
+  ~Fire() override;
+
+  double dtMax = 0;
+  Fire& setMaxIter(int maxIter) override;
+  Fire& setDtMax(double dtMax);
+
+  void init(State& state) override;
+  void iteration(State& state) override;
+  bool checkConvergence(const State& state) override;
+
+ private:
+  int _nMin = 5;
+  double _fInc = 1.1;
+  double _fDec = 0.5;
+  double _fA = 0.99;
+  double _aStart = 0.1;
+  struct Impl;
+  std::unique_ptr<Impl> impl_;
+};
+
+class GradDescent : public NewMinimiser<GradDescent> {
+ public:
+  GradDescent();
+  GradDescent(const GradDescent& o);
+  GradDescent& operator=(const GradDescent& o);
+  ~GradDescent() override;
+
+  GradDescent& setAlpha(double alpha);
+  GradDescent& setMaxIter(int maxIter) override;
+
+  void init(State& state) override;
+  void iteration(State& state) override;
+  bool checkConvergence(const State& state) override;
+
+ private:
+  double _alpha = 1e-1;
+  struct Impl;
+  std::unique_ptr<Impl> impl_;
+};
+
+}  // namespace minim
+
+#endif

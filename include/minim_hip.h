@@ -1,0 +1,197 @@
+/*
+ * minim_hip.h -- C ABI of the MI355X-native minim hot path (libminim_amd.so).
+ *
+ * This is the drop-in boundary: plain C types, device pointers and sizes, no
+ * HIP or torch types.  Each entry point names the reference interface
+ * (sjavis/minim, file:line) whose role it takes over.  Every function returns
+ * MINIM_OK or an error code; minim_last_error() gives the message.  The codes
+ * map the reference's exceptions: std::invalid_argument -> MINIM_EINVAL,
+ * std::runtime_error -> MINIM_ERUNTIME.
+ *
+ * Device vectors are flat arrays of double (MINIM_FP64) or float (MINIM_FP32)
+ * in GPU memory.  Streams are hipStream_t passed as void* (NULL = default).
+ * Calls that reduce (energies, dots) share per-device scratch memory and must
+ * not overlap on one GPU.
+ */
+#ifndef MINIM_HIP_H
+#define MINIM_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { MINIM_OK = 0, MINIM_EINVAL = 1, MINIM_ERUNTIME = 2, MINIM_EHIP = 3, MINIM_ENOMEM = 4 };
+enum { MINIM_FP64 = 0, MINIM_FP32 = 1 };
+
+/* Message of the last failed call on this thread. */
+const char* minim_last_error(void);
+int minim_abi_version(void);
+
+typedef struct minim_potential minim_potential;
+typedef struct minim_state minim_state;
+typedef struct minim_minimiser minim_minimiser;
+
+/* ------------------------------------------------------------------ potentials */
+
+/* PhaseField setters (include/potentials/PhaseField.h:20-65, src/potentials/PhaseField.cpp:747-900).
+ * n_interface_size: 0 = default (resolution), 1 = setInterfaceSize(double),
+ * n_fluid = setInterfaceSize(vector).  density_constraint: 0 hard, 1 soft,
+ * 2 none, 3 fixed (PhaseField.cpp:802-815). */
+typedef struct minim_pf_params {
+  int grid[3];
+  int n_fluid;
+  double resolution;
+  const double* surface_tension;
+  int n_surface_tension;
+  const double* interface_size;
+  int n_interface_size;
+  int density_constraint;
+  double density_const;
+  const unsigned char* fix_fluid; /* NULL or n_fluid flags (setFixFluid) */
+  int stencil_xchunk;             /* 0 = automatic x-march length */
+  int stencil_target_blocks;      /* 0 = automatic */
+} minim_pf_params;
+int minim_pf_create(const minim_pf_params* p, minim_potential** out);
+
+/* kappa / kappaP per fluid after init (PhaseField::assignFluidCoefficients, PhaseField.cpp:78-134). */
+int minim_pf_coefficients(const minim_potential* pot, double* kappa, double* kappa_p);
+
+/* LjNd(nDim, sigma, epsilon) (include/potentials/LjNd.h:12-44): all pairs, no cutoff. */
+int minim_lj_create(int ndim, double sigma, double epsilon, minim_potential** out);
+/* Truncated-and-shifted cutoff rc in a periodic box with a cell list (no
+ * reference counterpart: the reference's LjNd has neither, LjNd.cpp:29-53).
+ * ndim must be 3; skin is the neighbour-list margin (rebuild when a particle
+ * has moved more than skin/2). */
+int minim_lj_set_cutoff(minim_potential* pot, double rc, const double box[3], double skin);
+
+/* BarAndHinge with explicit element lists (BarAndHinge::setBondList /
+ * setHingeList / setRigidity / setLength0 / setTheta0 / setForce,
+ * include/potentials/BarAndHinge.h:17-43).  bars: 2 node ids per bar;
+ * hinges: 4 node ids per hinge in the reference's order (BarAndHinge.cpp:119-131).
+ * k/l0/theta0 arrays of length n_bar / n_hinge; NaN entries of bar_l0 or
+ * hinge_theta0 are computed from the initial coordinates (computeLength0 /
+ * computeTheta0, BarAndHinge.cpp:213-248). force: NULL, a 3-vector, or 3 per node. */
+typedef struct minim_bah_params {
+  int n_node;
+  int n_bar;
+  const int* bars;
+  const double* bar_k;
+  const double* bar_l0;
+  int n_hinge;
+  const int* hinges;
+  const double* hinge_k;
+  const double* hinge_theta0;
+  const double* force;
+  int force_per_node;
+} minim_bah_params;
+int minim_bah_create(const minim_bah_params* p, minim_potential** out);
+
+/* Potential::setConstraints(vector<int> iFix) (src/Potential.cpp:80-85). */
+int minim_potential_set_fixed(minim_potential* pot, const int* idof, size_t n);
+/* Potential::init / initLocal on the initial coordinates (include/Potential.h:62-63). */
+int minim_potential_init(minim_potential* pot, const double* coords, size_t ndof);
+/* Potential::convergence after init (include/Potential.h:25, PhaseField.cpp:146). */
+int minim_potential_convergence(const minim_potential* pot, double* out);
+int minim_potential_destroy(minim_potential* pot);
+
+/* Energy and/or gradient at x (or x + t*d when d != NULL) on device vectors:
+ * Potential::energyGradient followed, for the gradient, by
+ * Potential::applyConstraints, as State::basicEG / elementEG do
+ * (src/State.cpp:48-79; PhaseField.cpp:662-703; LjNd.cpp:29-53;
+ * BarAndHinge.cpp:251-369).  e: host double or NULL; g: device or NULL.
+ * A gradient together with d != NULL is not supported. */
+int minim_potential_eval(const minim_potential* pot, int precision, const void* x, size_t ndof,
+                         const void* d, double t, double* e, void* g, void* stream);
+/* As minim_potential_eval, without a host synchronisation: the energy (when
+ * want_energy) is written to the device double e_dev, in stream order. */
+int minim_potential_eval_async(const minim_potential* pot, int precision, const void* x, size_t ndof,
+                               const void* d, double t, int want_energy, double* e_dev, void* g, void* stream);
+/* Potential::applyConstraints on a device vector (src/Potential.cpp:102-115,
+ * PhaseField::specialisedConstraints PhaseField.cpp:580-612). */
+int minim_potential_constrain(const minim_potential* pot, int precision, void* v, size_t n, void* stream);
+
+/* CommGrid::dotProduct / Communicator::dotProduct on one GPU
+ * (src/communicators/CommGrid.cpp:187-194, src/Communicator.cpp:170-173): FP64 sum. */
+int minim_dot(int precision, const void* a, const void* b, size_t n, double* out, void* stream);
+
+/* ------------------------------------------------------------------ State */
+
+/* State(pot, coords) (src/State.cpp:12-26): clones and initialises pot, uploads coords. */
+int minim_state_create(const minim_potential* pot, const double* coords, size_t ndof, int precision,
+                       minim_state** out);
+/* State copy constructor (src/State.cpp:28-35). */
+int minim_state_copy(const minim_state* s, minim_state** out);
+int minim_state_destroy(minim_state* s);
+/* State::energy() / energy(coords) (src/State.cpp:84-87, 134-152); coords NULL = own. */
+int minim_state_energy(minim_state* s, const double* coords, double* e);
+/* State::gradient() / gradient(coords) (src/State.cpp:89-92, 154-172) into host g. */
+int minim_state_gradient(minim_state* s, const double* coords, double* g);
+/* State::energyGradient (src/State.cpp:94-97, 174-192); e or g may be NULL. */
+int minim_state_energy_gradient(minim_state* s, const double* coords, double* e, double* g);
+/* State::coords() get / set (src/State.cpp:282-288). */
+int minim_state_get_coords(minim_state* s, double* out);
+int minim_state_set_coords(minim_state* s, const double* in);
+/* State::applyConstraints on a host vector (src/State.cpp:321-323). */
+int minim_state_apply_constraints(minim_state* s, double* data);
+/* State::failed (src/State.cpp:326-328) and the public fields of include/State.h:15-19,72. */
+int minim_state_failed(minim_state* s);
+int minim_state_info(const minim_state* s, size_t* ndof, int* precision, double* convergence, int* is_failed);
+int minim_state_set_convergence(minim_state* s, double convergence);
+int minim_state_set_precision(minim_state* s, int precision);
+int minim_state_set_stream(minim_state* s, void* stream);
+/* Device address of the coordinates (ndof entries of the State's precision). */
+void* minim_state_device_coords(minim_state* s);
+
+/* ------------------------------------------------------------------ minimisers */
+
+/* Lbfgs (include/minimisers/Lbfgs.h:14-38), Fire (include/minimisers/Fire.h:9-36),
+ * GradDescent (include/minimisers/GradDescent.h:9-23) with
+ * Minimiser::setMaxIter / setLinesearch (src/Minimiser.cpp:12-24). */
+enum { MINIM_LBFGS = 0, MINIM_FIRE = 1, MINIM_GRADDESCENT = 2 };
+enum { MINIM_LS_BACKTRACKING = 0, MINIM_LS_NONE = 1 };
+enum { MINIM_PREC_STATE = 0, MINIM_PREC_FP64 = 1, MINIM_PREC_FP32 = 2, MINIM_PREC_MIXED = 3 };
+typedef struct minim_minimiser_params {
+  int kind;
+  int max_iter;
+  int linesearch;
+  int m;            /* Lbfgs::setM */
+  double max_step;  /* Lbfgs::setMaxStep */
+  double init_step; /* Lbfgs::setInitStep */
+  int precision_mode;
+  double switch_rms; /* mixed: gradient RMS at which FP32 storage is left (0 = 1e3*convergence) */
+  double dt_max;     /* Fire::setDtMax (0 = estimate, Fire.cpp:29-37) */
+  double alpha;      /* GradDescent::setAlpha */
+} minim_minimiser_params;
+
+typedef struct minim_result {
+  int iterations; /* Minimiser::iter at exit */
+  int converged;
+  long n_energy;
+  long n_gradient;
+  long n_energy_reused;
+  int iter_fp32;
+  int iter_fp64;
+  double energy;
+  double grad_rms;
+  double dt_max; /* Fire: dtMax after init */
+} minim_result;
+
+typedef void (*minim_adjust_fn)(int iter, minim_state* state, void* user);
+
+int minim_minimiser_create(const minim_minimiser_params* p, minim_minimiser** out);
+int minim_minimiser_destroy(minim_minimiser* m);
+/* Minimiser::minimise(state, adjustState) (src/Minimiser.cpp:27-38); coords_out
+ * (host, ndof) receives state.coords() and may be NULL. */
+int minim_minimise(minim_minimiser* m, minim_state* s, minim_adjust_fn fn, void* user, double* coords_out,
+                   minim_result* r);
+/* Minimiser::minimise(state, "<fields>-<N>") (src/Minimiser.cpp:41-69). */
+int minim_minimise_log(minim_minimiser* m, minim_state* s, const char* log_type, double* coords_out,
+                       minim_result* r);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,636 @@
+"""Python mirror of minim's State / Potential / Minimiser API over the C ABI.
+
+Names, setters, defaults and error behaviour follow the reference's C++ API
+(include/State.h, include/Potential.h, include/potentials/*.h,
+include/minimisers/*.h); every computation runs in libminim_amd.so on the GPU.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _capi as A
+
+_DENSITY = {"gradient": 0, "hard": 0, "energy penalty": 1, "soft": 1, "none": 2, "fixed": 3}
+_PREC = {"fp64": A.MINIM_FP64, "fp32": A.MINIM_FP32, A.MINIM_FP64: A.MINIM_FP64, A.MINIM_FP32: A.MINIM_FP32}
+
+
+def _dptr(a):
+    return a.ctypes.data_as(A.dp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1))
+
+
+class _Handle:
+    """Owns one C handle and frees it with the given destructor."""
+
+    def __init__(self, ptr, destroy):
+        self.ptr = ptr
+        self._destroy = destroy
+
+    def __del__(self):
+        if self.ptr:
+            try:
+                self._destroy(self.ptr)
+            except Exception:
+                pass
+            self.ptr = None
+
+
+# ----------------------------------------------------------------- potentials
+class Potential:
+    """Base of the device potentials (include/Potential.h:16-108)."""
+
+    def __init__(self):
+        self.fixedDofs = []
+
+    def setConstraints(self, iFix):
+        """Potential::setConstraints(vector<int> iFix) (src/Potential.cpp:80-85)."""
+        self.fixedDofs = list(self.fixedDofs) + [int(i) for i in iFix]
+        return self
+
+    def newState(self, coords, precision="fp64"):
+        return State(self, coords, precision)
+
+    def _create(self):
+        raise NotImplementedError
+
+    def handle(self, coords):
+        """A fresh, initialised C potential for the given coordinates."""
+        L = A.load()
+        h = self._create()
+        if self.fixedDofs:
+            idx = np.ascontiguousarray(self.fixedDofs, dtype=np.int32)
+            A.check(L.minim_potential_set_fixed(h.ptr, idx.ctypes.data_as(A.ip), idx.size))
+        x = _f64(coords)
+        A.check(L.minim_potential_init(h.ptr, _dptr(x), x.size))
+        return h
+
+
+class PhaseField(Potential):
+    """include/potentials/PhaseField.h:13-123 (bulk, gradient, density constraint, fixed fluids)."""
+
+    potentialType = 2
+
+    def __init__(self):
+        super().__init__()
+        self.nFluid = 1
+        self.resolution = 1.0
+        self.gridSize = None
+        self.interfaceSize = []
+        self.surfaceTension = [1.0]
+        self.densityConstraint = 0
+        self.densityConst = 1.0
+        self.fixFluid = []
+        self.stencilXChunk = 0
+        self.stencilTargetBlocks = 0
+
+    def setNFluid(self, n):
+        self.nFluid = int(n)
+        return self
+
+    def setGridSize(self, g):
+        self.gridSize = [int(v) for v in g]
+        return self
+
+    def setResolution(self, r):
+        self.resolution = float(r)
+        return self
+
+    def setInterfaceSize(self, w):
+        # the scalar form sizes the vector by the current nFluid (PhaseField.cpp:764-767)
+        self.interfaceSize = [float(w)] * self.nFluid if np.isscalar(w) else [float(v) for v in w]
+        return self
+
+    def setSurfaceTension(self, s):
+        self.surfaceTension = [float(s)] * self.nFluid if np.isscalar(s) else [float(v) for v in s]
+        return self
+
+    def setDensityConstraint(self, method):
+        if method not in _DENSITY:
+            raise ValueError("PhaseField: Invalid density constraint. Allowed methods are: fixed, none, "
+                             "gradient / hard, or energy penalty / soft")
+        self.densityConstraint = _DENSITY[method]
+        return self
+
+    def setFixFluid(self, iFluid, fix=True):
+        if len(self.fixFluid) != self.nFluid:
+            self.fixFluid = [False] * self.nFluid
+        self.fixFluid[iFluid] = bool(fix)
+        return self
+
+    def _create(self):
+        if self.gridSize is None or len(self.gridSize) != 3:
+            raise ValueError("PhaseField: gridSize must have three entries.")
+        p = A.PFParams()
+        for i in range(3):
+            p.grid[i] = self.gridSize[i]
+        p.n_fluid = self.nFluid
+        p.resolution = self.resolution
+        self._st = _f64(self.surfaceTension)
+        p.surface_tension = _dptr(self._st)
+        p.n_surface_tension = self._st.size
+        self._iw = _f64(self.interfaceSize if self.interfaceSize else [0.0])
+        p.interface_size = _dptr(self._iw)
+        p.n_interface_size = len(self.interfaceSize)
+        if p.n_interface_size == 1 and self.nFluid != 1:
+            p.n_interface_size = 1
+        p.density_constraint = self.densityConstraint
+        p.density_const = self.densityConst
+        if self.fixFluid:
+            self._fx = (C.c_ubyte * self.nFluid)(*[1 if f else 0 for f in self.fixFluid])
+            p.fix_fluid = self._fx
+        p.stencil_xchunk = self.stencilXChunk
+        p.stencil_target_blocks = self.stencilTargetBlocks
+        h = C.c_void_p()
+        A.check(A.load().minim_pf_create(C.byref(p), C.byref(h)))
+        return _Handle(h.value, A.load().minim_potential_destroy)
+
+    def coefficients(self, coords=None):
+        """(kappa, kappaP, convergence) as PhaseField::init computes them."""
+        n = int(np.prod(self.gridSize)) * self.nFluid
+        h = self.handle(np.zeros(n) if coords is None else coords)
+        k = np.zeros(self.nFluid)
+        kp = np.zeros(self.nFluid)
+        A.check(A.load().minim_pf_coefficients(h.ptr, _dptr(k), _dptr(kp)))
+        conv = C.c_double()
+        A.check(A.load().minim_potential_convergence(h.ptr, C.byref(conv)))
+        return k, kp, conv.value
+
+
+class LjNd(Potential):
+    """include/potentials/LjNd.h:12-44; setCutoff adds a truncated-shifted periodic mode."""
+
+    potentialType = 1
+
+    def __init__(self, nDim, sigma=1.0, epsilon=1.0):
+        super().__init__()
+        self.nDim = int(nDim)
+        self.sigma = float(sigma)
+        self.epsilon = float(epsilon)
+        self.cutoff = 0.0
+        self.box = None
+        self.skin = 0.3
+
+    def setSigma(self, s):
+        self.sigma = float(s)
+        return self
+
+    def setEpsilon(self, e):
+        self.epsilon = float(e)
+        return self
+
+    def setCutoff(self, rc, box, skin=0.3):
+        self.cutoff = float(rc)
+        self.box = [float(b) for b in (box if np.ndim(box) else [box] * 3)]
+        self.skin = float(skin)
+        return self
+
+    def _create(self):
+        L = A.load()
+        h = C.c_void_p()
+        A.check(L.minim_lj_create(self.nDim, self.sigma, self.epsilon, C.byref(h)))
+        hd = _Handle(h.value, L.minim_potential_destroy)
+        if self.cutoff > 0:
+            b = _f64(self.box)
+            A.check(L.minim_lj_set_cutoff(hd.ptr, self.cutoff, _dptr(b), self.skin))
+        return hd
+
+
+class Lj2d(LjNd):
+    def __init__(self, sigma=1.0, epsilon=1.0):
+        super().__init__(2, sigma, epsilon)
+
+
+class Lj3d(LjNd):
+    def __init__(self, sigma=1.0, epsilon=1.0):
+        super().__init__(3, sigma, epsilon)
+
+
+class BarAndHinge(Potential):
+    """include/potentials/BarAndHinge.h:11-70 with explicit rigidities (types 0, 1, 2)."""
+
+    potentialType = 1
+
+    def __init__(self):
+        super().__init__()
+        self.triangles = None
+        self.bonds = None
+        self.hingesList = None
+        self.kBond = None
+        self.kHinge = None
+        self.length0 = None
+        self.theta0 = None
+        self.force = None
+
+    def setTriangulation(self, tri):
+        self.triangles = np.asarray(tri, dtype=np.int32).reshape(-1, 3)
+        return self
+
+    def setBondList(self, b):
+        self.bonds = np.asarray(b, dtype=np.int32).reshape(-1, 2)
+        return self
+
+    def setHingeList(self, h):
+        self.hingesList = np.asarray(h, dtype=np.int32).reshape(-1, 4)
+        return self
+
+    def setRigidity(self, kBond, kHinge):
+        self.kBond = kBond
+        self.kHinge = kHinge
+        return self
+
+    def setLength0(self, l0):
+        self.length0 = l0
+        return self
+
+    def setTheta0(self, t0):
+        self.theta0 = t0
+        return self
+
+    def setForce(self, f):
+        self.force = _f64(f)
+        return self
+
+    def _lists(self):
+        if self.bonds is not None and self.hingesList is not None:
+            return self.bonds, self.hingesList
+        if self.triangles is None:
+            raise ValueError("BarAndHinge: The triangulation has not been set.")
+        return triangulation_lists(self.triangles)
+
+    def _create(self):
+        raise RuntimeError("BarAndHinge handles are created by handle()")
+
+    def handle(self, coords):
+        L = A.load()
+        x = _f64(coords)
+        nnode = x.size // 3
+        bonds, hinges = self._lists()
+        nb, nh = len(bonds), len(hinges)
+        if self.kBond is None or self.kHinge is None:
+            raise ValueError("BarAndHinge requires the rigidities to be set")
+
+        def per(v, n):
+            a = np.full(n, float(v)) if np.isscalar(v) else _f64(v)
+            return a
+
+        self._keep = []
+        bb = np.ascontiguousarray(bonds, dtype=np.int32).reshape(-1)
+        hh = np.ascontiguousarray(hinges, dtype=np.int32).reshape(-1)
+        kb, kh = per(self.kBond, nb), per(self.kHinge, nh)
+        p = A.BaHParams()
+        p.n_node = nnode
+        p.n_bar = nb
+        p.bars = bb.ctypes.data_as(A.ip)
+        p.bar_k = _dptr(kb)
+        p.n_hinge = nh
+        p.hinges = hh.ctypes.data_as(A.ip)
+        p.hinge_k = _dptr(kh)
+        self._keep += [bb, hh, kb, kh]
+        if self.length0 is not None:
+            l0 = per(self.length0, nb)
+            self._keep.append(l0)
+            p.bar_l0 = _dptr(l0)
+        if self.theta0 is not None:
+            t0 = per(self.theta0, nh)
+            self._keep.append(t0)
+            p.hinge_theta0 = _dptr(t0)
+        if self.force is not None:
+            self._keep.append(self.force)
+            p.force = _dptr(self.force)
+            p.force_per_node = 1 if self.force.size == x.size else 0
+        h = C.c_void_p()
+        A.check(L.minim_bah_create(C.byref(p), C.byref(h)))
+        hd = _Handle(h.value, L.minim_potential_destroy)
+        if self.fixedDofs:
+            idx = np.ascontiguousarray(self.fixedDofs, dtype=np.int32)
+            A.check(L.minim_potential_set_fixed(hd.ptr, idx.ctypes.data_as(A.ip), idx.size))
+        A.check(L.minim_potential_init(hd.ptr, _dptr(x), x.size))
+        return hd
+
+
+def triangulation_lists(tri):
+    """Bond and hinge lists of a triangulation in the reference's order
+    (BarAndHinge::computeBondList / computeHingeList, BarAndHinge.cpp:70-147),
+    built in O(T) from an edge map."""
+    tri = np.asarray(tri, dtype=np.int64).reshape(-1, 3)
+    st = np.sort(tri, axis=1)
+    pairs = np.concatenate([st[:, [0, 1]], st[:, [0, 2]], st[:, [1, 2]]])
+    bonds = np.unique(pairs, axis=0).astype(np.int32)
+    edge = {}
+    for it, t in enumerate(tri.tolist()):
+        for k in range(3):
+            a, b = t[k], t[(k + 1) % 3]
+            edge.setdefault((min(a, b), max(a, b)), []).append(it)
+    hinges = []
+    tl = tri.tolist()
+    for it1, t1 in enumerate(tl):
+        cand = set()
+        for k in range(3):
+            a, b = t1[k], t1[(k + 1) % 3]
+            for it2 in edge[(min(a, b), max(a, b))]:
+                if it2 > it1:
+                    cand.add(it2)
+        for it2 in sorted(cand):
+            t2 = tl[it2]
+            s1 = [a in t2 for a in t1]
+            if sum(s1) != 2:
+                continue
+            a = s1.index(False)
+            nodes = [t1[a], t1[(a + 1) % 3], t1[(a + 2) % 3], [b for b in t2 if b not in t1][0]]
+            hinges.append(nodes)
+    return bonds, np.asarray(hinges, dtype=np.int32).reshape(-1, 4)
+
+
+# ----------------------------------------------------------------- State
+class State:
+    """include/State.h:12-76 with coordinates resident on the GPU."""
+
+    def __init__(self, pot, coords, precision="fp64"):
+        L = A.load()
+        x = _f64(coords)
+        self.pot = pot
+        self._ph = pot.handle(x)
+        h = C.c_void_p()
+        A.check(L.minim_state_create(self._ph.ptr, _dptr(x), x.size, _PREC[precision], C.byref(h)))
+        self._h = _Handle(h.value, L.minim_state_destroy)
+        self.ndof = x.size
+
+    @property
+    def ptr(self):
+        return self._h.ptr
+
+    def copy(self):
+        L = A.load()
+        h = C.c_void_p()
+        A.check(L.minim_state_copy(self.ptr, C.byref(h)))
+        s = State.__new__(State)
+        s.pot, s._ph, s.ndof = self.pot, self._ph, self.ndof
+        s._h = _Handle(h.value, L.minim_state_destroy)
+        return s
+
+    def _info(self):
+        n, p, c, f = C.c_size_t(), C.c_int(), C.c_double(), C.c_int()
+        A.check(A.load().minim_state_info(self.ptr, C.byref(n), C.byref(p), C.byref(c), C.byref(f)))
+        return n.value, p.value, c.value, bool(f.value)
+
+    @property
+    def convergence(self):
+        return self._info()[2]
+
+    @convergence.setter
+    def convergence(self, v):
+        A.check(A.load().minim_state_set_convergence(self.ptr, float(v)))
+
+    @property
+    def isFailed(self):
+        return self._info()[3]
+
+    @property
+    def precision(self):
+        return "fp64" if self._info()[1] == A.MINIM_FP64 else "fp32"
+
+    def setPrecision(self, p):
+        A.check(A.load().minim_state_set_precision(self.ptr, _PREC[p]))
+
+    def failed(self):
+        A.check(A.load().minim_state_failed(self.ptr))
+
+    def energy(self, coords=None):
+        e = C.c_double()
+        x = None if coords is None else _f64(coords)
+        A.check(A.load().minim_state_energy(self.ptr, None if x is None else _dptr(x), C.byref(e)))
+        return e.value
+
+    def gradient(self, coords=None):
+        g = np.zeros(self.ndof)
+        x = None if coords is None else _f64(coords)
+        A.check(A.load().minim_state_gradient(self.ptr, None if x is None else _dptr(x), _dptr(g)))
+        return g
+
+    def energyGradient(self, coords=None):
+        g = np.zeros(self.ndof)
+        e = C.c_double()
+        x = None if coords is None else _f64(coords)
+        A.check(A.load().minim_state_energy_gradient(self.ptr, None if x is None else _dptr(x), C.byref(e),
+                                                     _dptr(g)))
+        return e.value, g
+
+    # single-GPU equivalents of the parallel variants (src/State.cpp:99-274)
+    blockEnergy = procEnergy = allEnergy = energy
+    blockGradient = procGradient = allGradient = gradient
+
+    def coords(self, new=None):
+        if new is not None:
+            x = _f64(new)
+            if x.size != self.ndof:
+                raise ValueError("State: coordinates have the wrong size.")
+            A.check(A.load().minim_state_set_coords(self.ptr, _dptr(x)))
+            return None
+        out = np.zeros(self.ndof)
+        A.check(A.load().minim_state_get_coords(self.ptr, _dptr(out)))
+        return out
+
+    blockCoords = coords
+
+    def applyConstraints(self, data):
+        d = np.array(data, dtype=np.float64)
+        A.check(A.load().minim_state_apply_constraints(self.ptr, _dptr(d)))
+        return d
+
+    def device_coords(self):
+        return A.load().minim_state_device_coords(self.ptr)
+
+    def setStream(self, stream_ptr):
+        A.check(A.load().minim_state_set_stream(self.ptr, stream_ptr))
+
+
+# ----------------------------------------------------------------- minimisers
+class Minimiser:
+    """include/Minimiser.h:13-33."""
+
+    kind = None
+
+    def __init__(self):
+        self.maxIter = 100000
+        self.linesearch = "backtracking"
+        self.iter = 0
+        self.stats = {}
+        self._h = None
+
+    def setMaxIter(self, n):
+        self.maxIter = int(n)
+        self._h = None
+        return self
+
+    def setLinesearch(self, method):
+        if method not in ("backtracking", "none"):
+            raise ValueError("Invalid line search method.")
+        self.linesearch = method
+        self._h = None
+        return self
+
+    def _params(self):
+        p = A.MinimiserParams()
+        p.kind = self.kind
+        p.max_iter = self.maxIter
+        p.linesearch = A.MINIM_LS_BACKTRACKING if self.linesearch == "backtracking" else A.MINIM_LS_NONE
+        return p
+
+    def _handle(self):
+        if self._h is None:
+            h = C.c_void_p()
+            A.check(A.load().minim_minimiser_create(C.byref(self._params()), C.byref(h)))
+            self._h = _Handle(h.value, A.load().minim_minimiser_destroy)
+        return self._h
+
+    def _finish(self, r):
+        self.iter = r.iterations
+        self.stats = dict(iterations=r.iterations, converged=bool(r.converged), n_energy=r.n_energy,
+                          n_gradient=r.n_gradient, n_energy_reused=r.n_energy_reused, iter_fp32=r.iter_fp32,
+                          iter_fp64=r.iter_fp64, energy=r.energy, grad_rms=r.grad_rms, dt_max=r.dt_max)
+
+    def minimise(self, state, adjust=None):
+        """Minimiser::minimise(state, adjustState) / minimise(state, "<fields>-<N>")
+        (src/Minimiser.cpp:27-69)."""
+        L = A.load()
+        h = self._handle()
+        out = np.zeros(state.ndof)
+        r = A.Result()
+        if isinstance(adjust, str):
+            A.check(L.minim_minimise_log(h.ptr, state.ptr, adjust.encode(), _dptr(out), C.byref(r)))
+        else:
+            cb = A.ADJUST_FN(0)
+            if adjust is not None:
+                def _cb(it, sp, user):
+                    adjust(it, state)
+                cb = A.ADJUST_FN(_cb)
+            A.check(L.minim_minimise(h.ptr, state.ptr, cb, None, _dptr(out), C.byref(r)))
+        self._finish(r)
+        return out
+
+
+class Lbfgs(Minimiser):
+    """include/minimisers/Lbfgs.h:14-38 (+ precision schedule)."""
+
+    kind = A.MINIM_LBFGS
+
+    def __init__(self):
+        super().__init__()
+        self.m = 5
+        self.maxStep = 0.0
+        self.initStep = 1.0
+        self.precisionMode = "state"
+        self.switchRms = 0.0
+
+    def setM(self, m):
+        self.m = int(m)
+        self._h = None
+        return self
+
+    def setMaxStep(self, v):
+        self.maxStep = float(v)
+        self._h = None
+        return self
+
+    def setInitStep(self, v):
+        self.initStep = float(v)
+        self._h = None
+        return self
+
+    def setPrecision(self, mode, switchRms=0.0):
+        if mode not in ("state", "fp64", "fp32", "mixed"):
+            raise ValueError("Lbfgs: precision must be state, fp64, fp32 or mixed.")
+        self.precisionMode = mode
+        self.switchRms = float(switchRms)
+        self._h = None
+        return self
+
+    def _params(self):
+        p = super()._params()
+        p.m = self.m
+        p.max_step = self.maxStep
+        p.init_step = self.initStep
+        p.precision_mode = {"state": 0, "fp64": 1, "fp32": 2, "mixed": 3}[self.precisionMode]
+        p.switch_rms = self.switchRms
+        return p
+
+
+class Fire(Minimiser):
+    """include/minimisers/Fire.h:9-36.  Fire(dtMax) disables the line search (Fire.cpp:11-13)."""
+
+    kind = A.MINIM_FIRE
+
+    def __init__(self, dtMax=None):
+        super().__init__()
+        self.dtMax = 0.0
+        if dtMax is not None:
+            self.dtMax = float(dtMax)
+            self.linesearch = "none"
+
+    def setDtMax(self, v):
+        self.dtMax = float(v)
+        self._h = None
+        return self
+
+    def _params(self):
+        p = super()._params()
+        p.dt_max = self.dtMax
+        return p
+
+    def minimise(self, state, adjust=None):
+        out = super().minimise(state, adjust)
+        self.dtMax = self.stats["dt_max"]  # Fire::init stores its estimate (Fire.cpp:29-37)
+        return out
+
+
+class GradDescent(Minimiser):
+    """include/minimisers/GradDescent.h:9-23."""
+
+    kind = A.MINIM_GRADDESCENT
+
+    def __init__(self):
+        super().__init__()
+        self.alpha = 0.1
+
+    def setAlpha(self, a):
+        self.alpha = float(a)
+        self._h = None
+        return self
+
+    def _params(self):
+        p = super()._params()
+        p.alpha = self.alpha
+        return p
+
+
+# ----------------------------------------------------------------- kernel level
+def dot(precision, a_ptr, b_ptr, n, stream=None):
+    """CommGrid::dotProduct on device vectors (FP64 result)."""
+    out = C.c_double()
+    A.check(A.load().minim_dot(_PREC[precision], a_ptr, b_ptr, n, C.byref(out), stream))
+    return out.value
+
+
+def potential_eval(handle, precision, x_ptr, ndof, g_ptr=None, want_energy=True, d_ptr=None, t=0.0, stream=None):
+    """Energy and/or gradient of an initialised C potential on device vectors."""
+    e = C.c_double()
+    A.check(A.load().minim_potential_eval(handle.ptr, _PREC[precision], x_ptr, ndof, d_ptr, float(t),
+                                          C.byref(e) if want_energy else None, g_ptr, stream))
+    return e.value if want_energy else None
+
+
+def potential_eval_async(handle, precision, x_ptr, ndof, g_ptr=None, e_dev_ptr=None, d_ptr=None, t=0.0,
+                         stream=None, want_energy=None):
+    """Evaluation without a host synchronisation; the energy lands in e_dev_ptr
+    (computed but not copied out when want_energy and e_dev_ptr is None)."""
+    we = (e_dev_ptr is not None) if want_energy is None else bool(want_energy)
+    A.check(A.load().minim_potential_eval_async(handle.ptr, _PREC[precision], x_ptr, ndof, d_ptr, float(t),
+                                                1 if we else 0, e_dev_ptr, g_ptr, stream))
+
+
+def potential_constrain(handle, precision, v_ptr, n, stream=None):
+    A.check(A.load().minim_potential_constrain(handle.ptr, _PREC[precision], v_ptr, n, stream))

@@ -1,0 +1,549 @@
+// bah.hip -- bar-and-hinge elastic sheet on MI355X.
+//
+// Element kernels follow BarAndHinge::stretching (src/potentials/BarAndHinge.cpp:269-290,
+// including its gradient factor 2*k*dl/l, twice the derivative of its
+// 0.5*k*dl^2 energy), bending (:293-356) and forceEnergy (:359-369); element
+// order and the all-DOFs-fixed skip follow BarAndHinge::init (:16-67).
+//
+// The reference loops over heap elements and scatters into g.  Here the
+// energy is one thread per element with an FP64 reduction, and the gradient is
+// gathered: one thread per node walks its incident bars and hinges (CSR built
+// once on the host, in element order) and recomputes each element's
+// contribution to that node, so every gradient entry is written once, without
+// atomics, in a fixed order.
+//
+// Bending angle: the FP64 path uses the reference's acos form (bit-level
+// parity); the FP32 path uses atan2(|b2| n1.b3, n1.n2), the same angle without
+// acos's loss of precision near 0 and pi.
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <stdexcept>
+#include <unordered_map>
+
+#include "common.hpp"
+#include "minim/BarAndHinge.h"
+#include "runtime.hpp"
+
+namespace minim_amd {
+namespace {
+
+struct BaHArgs {
+  int nnode, nbar, nhinge;
+  const int* bar;     // 2 per bar
+  const double* bk;   // per bar
+  const double* bl0;
+  const int* hinge;   // 4 per hinge
+  const double* hk;
+  const double* ht0;
+  const int* bptr;    // node -> incident bars CSR (nnode+1)
+  const int* bidx;    // (bar << 1) | local
+  const int* hptr;    // node -> incident hinges CSR
+  const int* hidx;    // (hinge << 2) | local
+  const double* fz;   // per-node z force or nullptr
+  const unsigned char* nodeActive;  // force element exists for the node
+};
+
+template <typename T>
+__device__ inline void ld3(const T* __restrict__ x, const T* __restrict__ d, T t, int n, T (&p)[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    T c = x[(size_t)n * 3 + k];
+    if (d) {
+      T s = t * d[(size_t)n * 3 + k];
+      c = c + s;
+    }
+    p[k] = c;
+  }
+}
+
+template <typename T>
+__device__ inline void cross(const T (&a)[3], const T (&b)[3], T (&c)[3]) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+template <typename T>
+__device__ inline T dot3(const T (&a)[3], const T (&b)[3]) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+// Hinge geometry shared by energy and gradient.
+template <typename T>
+struct HingeGeo {
+  T b1[3], b2[3], b3[3], n1[3], n2[3];
+  T b2m, n1sq, n2sq, dtheta;
+};
+
+template <typename T>
+__device__ inline void hinge_geo(const T* __restrict__ x, const T* __restrict__ d, T t, const BaHArgs& a, int h,
+                                 HingeGeo<T>& G) {
+  const int* nd = a.hinge + 4 * (size_t)h;
+  T x1[3], x2[3], x3[3], x4[3];
+  ld3(x, d, t, nd[0], x1);
+  ld3(x, d, t, nd[1], x2);
+  ld3(x, d, t, nd[2], x3);
+  ld3(x, d, t, nd[3], x4);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    G.b1[k] = x2[k] - x1[k];
+    G.b2[k] = x3[k] - x2[k];
+    G.b3[k] = x4[k] - x3[k];
+  }
+  G.b2m = sqrt(dot3(G.b2, G.b2));
+  cross(G.b1, G.b2, G.n1);
+  cross(G.b2, G.b3, G.n2);
+  G.n1sq = dot3(G.n1, G.n1);
+  G.n2sq = dot3(G.n2, G.n2);
+  const T n1b3 = dot3(G.n1, G.b3);
+  const T pi = (T)3.14159265358979323846;
+  T theta;
+  if (sizeof(T) == 8) {
+    T c = dot3(G.n1, G.n2) / sqrt(G.n1sq * G.n2sq);
+    c = c < (T)-1 ? (T)-1 : c;
+    c = c > (T)1 ? (T)1 : c;
+    theta = (n1b3 >= 0) ? acos(c) : (T)2 * pi - acos(c);
+  } else {
+    const T yv = G.b2m * n1b3;
+    const T at = atan2(yv, dot3(G.n1, G.n2));
+    theta = (yv >= 0) ? at : (T)2 * pi + at;
+  }
+  T dth = theta - (T)a.ht0[h];
+  dth = fmod(dth + pi, (T)2 * pi) - pi;
+  G.dtheta = dth;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bah_energy_kernel(BaHArgs a, const T* __restrict__ x,
+                                                            const T* __restrict__ d, T t, RedScratch rs) {
+  double acc[1] = {0.0};
+  const size_t tot = (size_t)a.nbar + a.nhinge + a.nnode;
+  for (size_t w = (size_t)blockIdx.x * kBlock + threadIdx.x; w < tot; w += (size_t)gridDim.x * kBlock) {
+    if (w < (size_t)a.nbar) {
+      const int b = (int)w;
+      T p1[3], p2[3];
+      ld3(x, d, t, a.bar[2 * b], p1);
+      ld3(x, d, t, a.bar[2 * b + 1], p2);
+      T dx[3] = {p1[0] - p2[0], p1[1] - p2[1], p1[2] - p2[2]};
+      const T l = sqrt(dot3(dx, dx));
+      const T dl = l - (T)a.bl0[b];
+      acc[0] += (double)((T)0.5 * (T)a.bk[b] * (dl * dl));
+    } else if (w < (size_t)a.nbar + a.nhinge) {
+      const int h = (int)(w - a.nbar);
+      HingeGeo<T> G;
+      hinge_geo(x, d, t, a, h, G);
+      acc[0] += (double)((T)0.5 * (T)a.hk[h] * (G.dtheta * G.dtheta));
+    } else if (a.fz) {
+      const int n = (int)(w - a.nbar - a.nhinge);
+      if (a.nodeActive[n]) {
+        T p[3];
+        ld3(x, d, t, n, p);
+        acc[0] -= a.fz[n] * (double)p[2];
+      }
+    }
+  }
+  grid_sum_store<1>(acc, rs);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bah_gradient_kernel(BaHArgs a, const T* __restrict__ x,
+                                                              T* __restrict__ g) {
+  for (int n = blockIdx.x * kBlock + threadIdx.x; n < a.nnode; n += gridDim.x * kBlock) {
+    T gn[3] = {(T)0, (T)0, (T)0};
+    for (int q = a.bptr[n]; q < a.bptr[n + 1]; q++) {
+      const int code = a.bidx[q];
+      const int b = code >> 1, loc = code & 1;
+      T p1[3], p2[3];
+      ld3<T>(x, nullptr, (T)0, a.bar[2 * b], p1);
+      ld3<T>(x, nullptr, (T)0, a.bar[2 * b + 1], p2);
+      T dx[3] = {p1[0] - p2[0], p1[1] - p2[1], p1[2] - p2[2]};
+      const T l = sqrt(dot3(dx, dx));
+      const T dl = l - (T)a.bl0[b];
+      const T gf = (T)2 * (T)a.bk[b] * dl / l;
+      const T sg = loc == 0 ? (T)1 : (T)-1;
+#pragma unroll
+      for (int k = 0; k < 3; k++) gn[k] += sg * (gf * dx[k]);
+    }
+    for (int q = a.hptr[n]; q < a.hptr[n + 1]; q++) {
+      const int code = a.hidx[q];
+      const int h = code >> 2, loc = code & 3;
+      HingeGeo<T> G;
+      hinge_geo<T>(x, nullptr, (T)0, a, h, G);
+      const T gf = (T)a.hk[h] * (G.dtheta);
+      const T f1 = G.b2m / G.n1sq, f2 = G.b2m / G.n2sq;
+      T n1h[3], n2h[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        n1h[k] = f1 * G.n1[k];
+        n2h[k] = f2 * G.n2[k];
+      }
+      const T bb = G.b2m * G.b2m;
+      const T skew1 = -dot3(G.b1, G.b2) / bb;
+      const T skew2 = -dot3(G.b3, G.b2) / bb;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        T c;
+        if (loc == 0) c = -gf * n1h[k];
+        else if (loc == 1) c = gf * (((T)1 - skew1) * n1h[k] - skew2 * n2h[k]);
+        else if (loc == 2) c = -gf * (((T)1 - skew2) * n2h[k] - skew1 * n1h[k]);
+        else c = gf * n2h[k];
+        gn[k] += c;
+      }
+    }
+    if (a.fz && a.nodeActive[n]) gn[2] -= (T)a.fz[n];
+#pragma unroll
+    for (int k = 0; k < 3; k++) g[(size_t)n * 3 + k] = gn[k];
+  }
+}
+
+}  // namespace
+}  // namespace minim_amd
+
+namespace minim {
+
+using minim_amd::DeviceBuffer;
+using minim_amd::kBlock;
+
+struct BarAndHinge::Dev {
+  minim_amd::BaHArgs args{};
+  DeviceBuffer mem;
+  std::vector<int> activeBars, activeHinges;  // element index -> original bond/hinge index
+};
+
+BarAndHinge& BarAndHinge::setTriangulation(const vector2d<int>& triList) {
+  tri_ = triList;
+  return *this;
+}
+BarAndHinge& BarAndHinge::setBondList(const vector2d<int>& b) {
+  bonds_ = b;
+  return *this;
+}
+BarAndHinge& BarAndHinge::setHingeList(const vector2d<int>& h) {
+  hinges_ = h;
+  return *this;
+}
+BarAndHinge& BarAndHinge::setRigidity(double kb, double kh) {
+  kBond_ = {kb};
+  kHinge_ = {kh};
+  return *this;
+}
+BarAndHinge& BarAndHinge::setRigidity(const vector<double>& kb, const vector<double>& kh) {
+  if (!kb.empty()) kBond_ = kb;
+  if (!kh.empty()) kHinge_ = kh;
+  return *this;
+}
+BarAndHinge& BarAndHinge::setLength0(double l) {
+  length0_ = {l};
+  return *this;
+}
+BarAndHinge& BarAndHinge::setLength0(const vector<double>& l) {
+  length0_ = l;
+  return *this;
+}
+BarAndHinge& BarAndHinge::setTheta0(double t) {
+  theta0_ = {t};
+  return *this;
+}
+BarAndHinge& BarAndHinge::setTheta0(const vector<double>& t) {
+  theta0_ = t;
+  return *this;
+}
+BarAndHinge& BarAndHinge::setForce(const vector<double>& f) {
+  force_ = f;
+  return *this;
+}
+
+namespace {
+
+// Bond list of BarAndHinge::computeBondList (BarAndHinge.cpp:70-93): sorted
+// node pairs of every triangle, lexicographically sorted and unique.
+vector2d<int> bonds_from_tris(const vector2d<int>& tris) {
+  std::vector<std::pair<int, int>> e;
+  e.reserve(tris.size() * 3);
+  for (const auto& t0 : tris) {
+    if (t0.size() != 3) throw std::invalid_argument("BarAndHinge: triangles need three nodes.");
+    int t[3] = {t0[0], t0[1], t0[2]};
+    std::sort(t, t + 3);
+    e.push_back({t[0], t[1]});
+    e.push_back({t[0], t[2]});
+    e.push_back({t[1], t[2]});
+  }
+  std::sort(e.begin(), e.end());
+  e.erase(std::unique(e.begin(), e.end()), e.end());
+  vector2d<int> out;
+  out.reserve(e.size());
+  for (auto& p : e) out.push_back({p.first, p.second});
+  return out;
+}
+
+// Hinge list of BarAndHinge::computeHingeList (BarAndHinge.cpp:96-147) in
+// the same order, built in O(T) from an edge -> triangles map instead of the
+// reference's all-pairs triangle loop: for each triangle it1 and each later
+// triangle it2 sharing exactly two nodes, nodes = [unshared of it1, the next
+// two of it1 in cyclic order, unshared of it2].
+vector2d<int> hinges_from_tris(const vector2d<int>& tris) {
+  std::unordered_map<long long, std::vector<int>> edge;
+  edge.reserve(tris.size() * 3);
+  auto key = [](int a, int b) {
+    if (a > b) std::swap(a, b);
+    return ((long long)a << 32) | (unsigned)b;
+  };
+  for (int it = 0; it < (int)tris.size(); it++)
+    for (int k = 0; k < 3; k++) edge[key(tris[it][k], tris[it][(k + 1) % 3])].push_back(it);
+  vector2d<int> out;
+  for (int it1 = 0; it1 < (int)tris.size(); it1++) {
+    std::vector<int> cand;
+    for (int k = 0; k < 3; k++)
+      for (int it2 : edge[key(tris[it1][k], tris[it1][(k + 1) % 3])])
+        if (it2 > it1) cand.push_back(it2);
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    for (int it2 : cand) {
+      bool s1[3] = {false, false, false}, s2[3] = {false, false, false};
+      int nShared = 0;
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+          if (tris[it1][a] == tris[it2][b]) {
+            nShared++;
+            s1[a] = true;
+            s2[b] = true;
+          }
+      if (nShared != 2) continue;
+      std::vector<int> nodes(4);
+      for (int a = 0; a < 3; a++)
+        if (!s1[a]) {
+          nodes[0] = tris[it1][a];
+          nodes[1] = tris[it1][(a + 1) % 3];
+          nodes[2] = tris[it1][(a + 2) % 3];
+        }
+      for (int b = 0; b < 3; b++)
+        if (!s2[b]) nodes[3] = tris[it2][b];
+      out.push_back(nodes);
+    }
+  }
+  return out;
+}
+
+double pi() { return 4 * std::atan(1.0); }
+
+}  // namespace
+
+// BarAndHinge::init (BarAndHinge.cpp:16-67)
+void BarAndHinge::init(const vector<double>& coords) {
+  if (coords.size() % 3 != 0) throw std::invalid_argument("BarAndHinge: coordinates must be 3 per node.");
+  const int nnode = (int)(coords.size() / 3);
+  if (bonds_.empty()) {
+    if (tri_.empty()) throw std::invalid_argument("BarAndHinge: The triangulation has not been set.");
+    bonds_ = bonds_from_tris(tri_);
+  }
+  if (hinges_.empty()) {
+    if (tri_.empty()) throw std::invalid_argument("BarAndHinge: The triangulation has not been set.");
+    hinges_ = hinges_from_tris(tri_);
+  }
+  for (auto& b : bonds_)
+    if (b.size() < 2 || b[0] < 0 || b[1] < 0 || b[0] >= nnode || b[1] >= nnode)
+      throw std::invalid_argument("BarAndHinge: bond references a node outside the coordinates.");
+  for (auto& h : hinges_) {
+    if (h.size() < 4) throw std::invalid_argument("BarAndHinge: hinges need four nodes.");
+    for (int k = 0; k < 4; k++)
+      if (h[k] < 0 || h[k] >= nnode) throw std::invalid_argument("BarAndHinge: hinge references a node outside the coordinates.");
+  }
+  // computeRigidities (:150-160), computeLength0 (:213-226), computeTheta0 (:229-248)
+  if (kBond_.empty() || kHinge_.empty())
+    throw std::invalid_argument("BarAndHinge requires the rigidities to be set (thickness-derived rigidities are not supported)");
+  if (kBond_.size() == 1) kBond_ = vector<double>(bonds_.size(), kBond_[0]);
+  if (kHinge_.size() == 1) kHinge_ = vector<double>(hinges_.size(), kHinge_[0]);
+  if (kBond_.size() != bonds_.size() || kHinge_.size() != hinges_.size())
+    throw std::invalid_argument("BarAndHinge: rigidity arrays do not match the element lists.");
+  auto X = [&](int n, int k) { return coords[3 * (size_t)n + k]; };
+  if (length0_.empty()) {
+    length0_.resize(bonds_.size());
+    for (size_t i = 0; i < bonds_.size(); i++) {
+      double s = 0;
+      for (int k = 0; k < 3; k++) {
+        double v = X(bonds_[i][0], k) - X(bonds_[i][1], k);
+        s += v * v;
+      }
+      length0_[i] = std::sqrt(s);
+    }
+  } else if (length0_.size() == 1) {
+    length0_ = vector<double>(bonds_.size(), length0_[0]);
+  }
+  if (theta0_.empty()) {
+    theta0_.resize(hinges_.size());
+    for (size_t i = 0; i < hinges_.size(); i++) {
+      const auto& h = hinges_[i];
+      double b1[3], b2[3], b3[3];
+      for (int k = 0; k < 3; k++) {
+        b1[k] = X(h[1], k) - X(h[0], k);
+        b2[k] = X(h[2], k) - X(h[1], k);
+        b3[k] = X(h[3], k) - X(h[2], k);
+      }
+      double n1[3] = {b1[1] * b2[2] - b1[2] * b2[1], b1[2] * b2[0] - b1[0] * b2[2], b1[0] * b2[1] - b1[1] * b2[0]};
+      double n2[3] = {b2[1] * b3[2] - b2[2] * b3[1], b2[2] * b3[0] - b2[0] * b3[2], b2[0] * b3[1] - b2[1] * b3[0]};
+      double nn1 = std::sqrt(n1[0] * n1[0] + n1[1] * n1[1] + n1[2] * n1[2]);
+      double nn2 = std::sqrt(n2[0] * n2[0] + n2[1] * n2[1] + n2[2] * n2[2]);
+      double c = (n1[0] * n2[0] + n1[1] * n2[1] + n1[2] * n2[2]) / (nn1 * nn2);
+      double n1b3 = n1[0] * b3[0] + n1[1] * b3[1] + n1[2] * b3[2];
+      theta0_[i] = (n1b3 >= 0) ? std::acos(c) : 2 * pi() - std::acos(c);
+    }
+  } else if (theta0_.size() == 1) {
+    theta0_ = vector<double>(hinges_.size(), theta0_[0]);
+  }
+  if (length0_.size() != bonds_.size() || theta0_.size() != hinges_.size())
+    throw std::invalid_argument("BarAndHinge: rest length/angle arrays do not match the element lists.");
+  if (!force_.empty() && force_.size() != 3 && force_.size() != coords.size())
+    throw std::invalid_argument("BarAndHinge: force must have 3 or 3*nNode entries.");
+  uploadFixed(coords.size());
+  nNodeHint = nnode;
+  upload();
+}
+
+void BarAndHinge::setHingeTheta0(int iHinge, double t0) {
+  theta0_.at(iHinge) = t0;
+  if (dev_) upload();
+}
+
+void BarAndHinge::upload() {
+  const int nnode = nNodeHint;
+  std::vector<char> fixed(3 * (size_t)nnode, 0);
+  for (int i : fixedDofs)
+    if (i >= 0 && (size_t)i < fixed.size()) fixed[i] = 1;
+  auto allFixed = [&](const int* nd, int k) {
+    for (int q = 0; q < k; q++)
+      for (int c = 0; c < 3; c++)
+        if (!fixed[3 * (size_t)nd[q] + c]) return false;
+    return true;
+  };
+  auto dev = std::make_shared<Dev>();
+  std::vector<int> bar, hin;
+  std::vector<double> bk, bl, hk, ht;
+  for (size_t i = 0; i < bonds_.size(); i++) {
+    int nd[2] = {bonds_[i][0], bonds_[i][1]};
+    if (allFixed(nd, 2)) continue;
+    bar.push_back(nd[0]);
+    bar.push_back(nd[1]);
+    bk.push_back(kBond_[i]);
+    bl.push_back(length0_[i]);
+    dev->activeBars.push_back((int)i);
+  }
+  for (size_t i = 0; i < hinges_.size(); i++) {
+    int nd[4] = {hinges_[i][0], hinges_[i][1], hinges_[i][2], hinges_[i][3]};
+    if (allFixed(nd, 4)) continue;
+    for (int k = 0; k < 4; k++) hin.push_back(nd[k]);
+    hk.push_back(kHinge_[i]);
+    ht.push_back(theta0_[i]);
+    dev->activeHinges.push_back((int)i);
+  }
+  const int nb = (int)bk.size(), nh = (int)hk.size();
+  // node -> incident element CSR, in element order
+  std::vector<int> bptr(nnode + 1, 0), hptr(nnode + 1, 0);
+  for (int b = 0; b < nb; b++)
+    for (int l = 0; l < 2; l++) bptr[bar[2 * b + l] + 1]++;
+  for (int h = 0; h < nh; h++)
+    for (int l = 0; l < 4; l++) hptr[hin[4 * h + l] + 1]++;
+  for (int n = 0; n < nnode; n++) {
+    bptr[n + 1] += bptr[n];
+    hptr[n + 1] += hptr[n];
+  }
+  std::vector<int> bidx(bptr[nnode]), hidx(hptr[nnode]);
+  {
+    std::vector<int> cb(bptr.begin(), bptr.end() - 1), ch(hptr.begin(), hptr.end() - 1);
+    for (int b = 0; b < nb; b++)
+      for (int l = 0; l < 2; l++) bidx[cb[bar[2 * b + l]]++] = (b << 1) | l;
+    for (int h = 0; h < nh; h++)
+      for (int l = 0; l < 4; l++) hidx[ch[hin[4 * h + l]]++] = (h << 2) | l;
+  }
+  std::vector<double> fz;
+  std::vector<unsigned char> active(nnode, 0);
+  if (!force_.empty()) {
+    fz.resize(nnode);
+    for (int n = 0; n < nnode; n++) {
+      fz[n] = force_.size() == 3 ? force_[2] : force_[3 * (size_t)n + 2];
+      int nd[1] = {n};
+      active[n] = allFixed(nd, 1) ? 0 : 1;
+    }
+  }
+  // one allocation, 256-byte aligned sub-buffers
+  size_t off = 0;
+  auto reserve = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) / 256 * 256;
+    return o;
+  };
+  const size_t o_bar = reserve(sizeof(int) * bar.size()), o_bk = reserve(sizeof(double) * bk.size()),
+               o_bl = reserve(sizeof(double) * bl.size()), o_hin = reserve(sizeof(int) * hin.size()),
+               o_hk = reserve(sizeof(double) * hk.size()), o_ht = reserve(sizeof(double) * ht.size()),
+               o_bptr = reserve(sizeof(int) * bptr.size()), o_bidx = reserve(sizeof(int) * bidx.size()),
+               o_hptr = reserve(sizeof(int) * hptr.size()), o_hidx = reserve(sizeof(int) * hidx.size()),
+               o_fz = reserve(sizeof(double) * fz.size()), o_act = reserve(active.size());
+  dev->mem.reset(off + 256);
+  char* base = static_cast<char*>(dev->mem.get());
+  auto put = [&](size_t o, const void* src, size_t bytes) {
+    if (bytes) MINIM_HIP_CHECK(hipMemcpy(base + o, src, bytes, hipMemcpyHostToDevice));
+  };
+  put(o_bar, bar.data(), sizeof(int) * bar.size());
+  put(o_bk, bk.data(), sizeof(double) * bk.size());
+  put(o_bl, bl.data(), sizeof(double) * bl.size());
+  put(o_hin, hin.data(), sizeof(int) * hin.size());
+  put(o_hk, hk.data(), sizeof(double) * hk.size());
+  put(o_ht, ht.data(), sizeof(double) * ht.size());
+  put(o_bptr, bptr.data(), sizeof(int) * bptr.size());
+  put(o_bidx, bidx.data(), sizeof(int) * bidx.size());
+  put(o_hptr, hptr.data(), sizeof(int) * hptr.size());
+  put(o_hidx, hidx.data(), sizeof(int) * hidx.size());
+  put(o_fz, fz.data(), sizeof(double) * fz.size());
+  put(o_act, active.data(), active.size());
+  auto& a = dev->args;
+  a.nnode = nnode;
+  a.nbar = nb;
+  a.nhinge = nh;
+  a.bar = (const int*)(base + o_bar);
+  a.bk = (const double*)(base + o_bk);
+  a.bl0 = (const double*)(base + o_bl);
+  a.hinge = (const int*)(base + o_hin);
+  a.hk = (const double*)(base + o_hk);
+  a.ht0 = (const double*)(base + o_ht);
+  a.bptr = (const int*)(base + o_bptr);
+  a.bidx = (const int*)(base + o_bidx);
+  a.hptr = (const int*)(base + o_hptr);
+  a.hidx = (const int*)(base + o_hidx);
+  a.fz = fz.empty() ? nullptr : (const double*)(base + o_fz);
+  a.nodeActive = (const unsigned char*)(base + o_act);
+  dev_ = dev;
+}
+
+void BarAndHinge::evaluate(int prec, const void* x, const void* d, double t, double* e, void* g,
+                           void* stream) const {
+  if (!dev_) throw std::runtime_error("BarAndHinge: not initialised (construct a State first).");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  auto& ws = minim_amd::Workspace::current();
+  const auto& a = dev_->args;
+  if (e) {
+    const size_t tot = (size_t)a.nbar + a.nhinge + a.nnode;
+    const int nb = minim_amd::grid_blocks(tot, 4, 2048);
+    if (prec == FP64)
+      minim_amd::bah_energy_kernel<double><<<nb, kBlock, 0, s>>>(a, (const double*)x, (const double*)d, t, ws.rs());
+    else
+      minim_amd::bah_energy_kernel<float><<<nb, kBlock, 0, s>>>(a, (const float*)x, (const float*)d, (float)t, ws.rs());
+    MINIM_HIP_CHECK(hipGetLastError());
+  }
+  if (g) {
+    if (d) throw std::invalid_argument("BarAndHinge: gradients at a trial point are not supported.");
+    const int nb = minim_amd::grid_blocks(a.nnode, 1, 1 << 20);
+    if (prec == FP64)
+      minim_amd::bah_gradient_kernel<double><<<nb, kBlock, 0, s>>>(a, (const double*)x, (double*)g);
+    else
+      minim_amd::bah_gradient_kernel<float><<<nb, kBlock, 0, s>>>(a, (const float*)x, (float*)g);
+    MINIM_HIP_CHECK(hipGetLastError());
+    applyConstraints(prec, g, 3 * (size_t)a.nnode, stream);
+  }
+  if (e) {
+    double r[1];
+    ws.fetch(s, 1, r);
+    *e = r[0];
+  }
+}
+
+}  // namespace minim

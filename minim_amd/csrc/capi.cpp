@@ -1,0 +1,462 @@
+// capi.cpp -- the extern "C" boundary declared in include/minim_hip.h.
+// Maps the C++ classes' exceptions to status codes (std::invalid_argument ->
+// MINIM_EINVAL, HIP failures -> MINIM_EHIP, other std::exception -> MINIM_ERUNTIME).
+#include "minim_hip.h"
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+#include "minim/LjNd.h"
+#include "minim/BarAndHinge.h"
+#include "minim/Minimiser.h"
+#include "minim/PhaseField.h"
+#include "minim/State.h"
+#include "runtime.hpp"
+#include "vec.hpp"
+
+struct minim_potential {
+  std::unique_ptr<minim::Potential> p;
+  bool initialised = false;
+  size_t ndof = 0;
+};
+struct minim_state {
+  std::unique_ptr<minim::State> s;
+};
+struct minim_minimiser {
+  std::unique_ptr<minim::Minimiser> m;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <typename F>
+int guard(F&& f) {
+  try {
+    f();
+    return MINIM_OK;
+  } catch (const std::invalid_argument& e) {
+    g_err = e.what();
+    return MINIM_EINVAL;
+  } catch (const minim_amd::HipError& e) {
+    g_err = e.what();
+    return MINIM_EHIP;
+  } catch (const std::bad_alloc& e) {
+    g_err = "out of memory";
+    return MINIM_ENOMEM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return MINIM_ERUNTIME;
+  } catch (...) {
+    g_err = "unknown error";
+    return MINIM_ERUNTIME;
+  }
+}
+
+void need(const void* p, const char* what) {
+  if (!p) throw std::invalid_argument(std::string(what) + " must not be NULL");
+}
+
+void check_prec(int prec) {
+  if (prec != MINIM_FP64 && prec != MINIM_FP32) throw std::invalid_argument("precision must be MINIM_FP64 or MINIM_FP32");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* minim_last_error(void) { return g_err.c_str(); }
+int minim_abi_version(void) { return 1; }
+
+// ---------------------------------------------------------------- potentials
+int minim_pf_create(const minim_pf_params* p, minim_potential** out) {
+  return guard([&] {
+    need(p, "params");
+    need(out, "out");
+    auto pf = std::make_unique<minim::PhaseField>();
+    pf->setNFluid(p->n_fluid);
+    pf->setGridSize({p->grid[0], p->grid[1], p->grid[2]});
+    pf->setResolution(p->resolution);
+    if (p->n_surface_tension > 0) {
+      need(p->surface_tension, "surface_tension");
+      pf->setSurfaceTension(std::vector<double>(p->surface_tension, p->surface_tension + p->n_surface_tension));
+    }
+    if (p->n_interface_size == 1) {
+      need(p->interface_size, "interface_size");
+      pf->setInterfaceSize(p->interface_size[0]);
+    } else if (p->n_interface_size > 1) {
+      need(p->interface_size, "interface_size");
+      pf->setInterfaceSize(std::vector<double>(p->interface_size, p->interface_size + p->n_interface_size));
+    }
+    static const char* names[] = {"hard", "soft", "none", "fixed"};
+    if (p->density_constraint < 0 || p->density_constraint > 3)
+      throw std::invalid_argument("PhaseField: Invalid density constraint.");
+    pf->setDensityConstraint(names[p->density_constraint]);
+    pf->densityConst = p->density_const;
+    if (p->fix_fluid) {
+      for (int k = 0; k < p->n_fluid; k++) pf->setFixFluid(k, p->fix_fluid[k] != 0);
+    }
+    pf->stencilXChunk = p->stencil_xchunk;
+    if (p->stencil_target_blocks > 0) pf->stencilTargetBlocks = p->stencil_target_blocks;
+    auto* h = new minim_potential;
+    h->p = std::move(pf);
+    *out = h;
+  });
+}
+
+int minim_pf_coefficients(const minim_potential* pot, double* kappa, double* kappa_p) {
+  return guard([&] {
+    need(pot, "pot");
+    auto* pf = dynamic_cast<const minim::PhaseField*>(pot->p.get());
+    if (!pf) throw std::invalid_argument("not a PhaseField potential");
+    if (!pot->initialised) throw std::runtime_error("PhaseField: call minim_potential_init first");
+    for (int k = 0; k < pf->nFluid; k++) {
+      if (kappa) kappa[k] = pf->kappa[k];
+      if (kappa_p) kappa_p[k] = pf->kappaP[k];
+    }
+  });
+}
+
+int minim_lj_create(int ndim, double sigma, double epsilon, minim_potential** out) {
+  return guard([&] {
+    need(out, "out");
+    auto* h = new minim_potential;
+    h->p = std::make_unique<minim::LjNd>(ndim, sigma, epsilon);
+    *out = h;
+  });
+}
+
+int minim_lj_set_cutoff(minim_potential* pot, double rc, const double box[3], double skin) {
+  return guard([&] {
+    need(pot, "pot");
+    need(box, "box");
+    auto* lj = dynamic_cast<minim::LjNd*>(pot->p.get());
+    if (!lj) throw std::invalid_argument("not a LjNd potential");
+    lj->setCutoff(rc, {box[0], box[1], box[2]}, skin);
+    pot->initialised = false;
+  });
+}
+
+int minim_bah_create(const minim_bah_params* p, minim_potential** out) {
+  return guard([&] {
+    need(p, "params");
+    need(out, "out");
+    auto b = std::make_unique<minim::BarAndHinge>();
+    std::vector<std::vector<int>> bl(p->n_bar), hl(p->n_hinge);
+    for (int i = 0; i < p->n_bar; i++) bl[i] = {p->bars[2 * i], p->bars[2 * i + 1]};
+    for (int i = 0; i < p->n_hinge; i++)
+      hl[i] = {p->hinges[4 * i], p->hinges[4 * i + 1], p->hinges[4 * i + 2], p->hinges[4 * i + 3]};
+    b->setBondList(bl);
+    b->setHingeList(hl);
+    std::vector<double> kb(p->bar_k, p->bar_k + p->n_bar), kh(p->hinge_k, p->hinge_k + p->n_hinge);
+    b->setRigidity(kb, kh);
+    if (p->bar_l0) b->setLength0(std::vector<double>(p->bar_l0, p->bar_l0 + p->n_bar));
+    if (p->hinge_theta0) b->setTheta0(std::vector<double>(p->hinge_theta0, p->hinge_theta0 + p->n_hinge));
+    if (p->force) {
+      if (p->force_per_node) b->setForce(std::vector<double>(p->force, p->force + 3 * (size_t)p->n_node));
+      else b->setForce(std::vector<double>(p->force, p->force + 3));
+    }
+    b->nNodeHint = p->n_node;
+    auto* h = new minim_potential;
+    h->p = std::move(b);
+    *out = h;
+  });
+}
+
+int minim_potential_set_fixed(minim_potential* pot, const int* idof, size_t n) {
+  return guard([&] {
+    need(pot, "pot");
+    if (n) need(idof, "idof");
+    pot->p->setConstraints(std::vector<int>(idof, idof + n));
+    pot->initialised = false;
+  });
+}
+
+int minim_potential_init(minim_potential* pot, const double* coords, size_t ndof) {
+  return guard([&] {
+    need(pot, "pot");
+    if (ndof) need(coords, "coords");
+    pot->p->init(std::vector<double>(coords, coords + ndof));
+    pot->initialised = true;
+    pot->ndof = ndof;
+  });
+}
+
+int minim_potential_convergence(const minim_potential* pot, double* out) {
+  return guard([&] {
+    need(pot, "pot");
+    need(out, "out");
+    *out = pot->p->convergence;
+  });
+}
+
+int minim_potential_destroy(minim_potential* pot) {
+  return guard([&] { delete pot; });
+}
+
+int minim_potential_eval(const minim_potential* pot, int precision, const void* x, size_t ndof, const void* d,
+                         double t, double* e, void* g, void* stream) {
+  return guard([&] {
+    need(pot, "pot");
+    need(x, "x");
+    check_prec(precision);
+    if (!pot->initialised) throw std::runtime_error("potential not initialised: call minim_potential_init");
+    if (ndof != pot->ndof) throw std::invalid_argument("ndof does not match the initialised potential");
+    if (d && g) throw std::invalid_argument("a gradient at a trial point (d != NULL) is not supported");
+    pot->p->evaluate(precision, x, d, t, e, g, stream);
+  });
+}
+
+int minim_potential_eval_async(const minim_potential* pot, int precision, const void* x, size_t ndof,
+                               const void* d, double t, int want_energy, double* e_dev, void* g, void* stream) {
+  return guard([&] {
+    need(pot, "pot");
+    need(x, "x");
+    check_prec(precision);
+    if (!pot->initialised) throw std::runtime_error("potential not initialised: call minim_potential_init");
+    if (ndof != pot->ndof) throw std::invalid_argument("ndof does not match the initialised potential");
+    if (d && g) throw std::invalid_argument("a gradient at a trial point (d != NULL) is not supported");
+    pot->p->evaluateAsync(precision, x, d, t, want_energy != 0, e_dev, g, stream);
+  });
+}
+
+int minim_potential_constrain(const minim_potential* pot, int precision, void* v, size_t n, void* stream) {
+  return guard([&] {
+    need(pot, "pot");
+    need(v, "v");
+    check_prec(precision);
+    if (!pot->initialised) throw std::runtime_error("potential not initialised: call minim_potential_init");
+    if (n != pot->ndof) throw std::invalid_argument("n does not match the initialised potential");
+    pot->p->applyConstraints(precision, v, n, stream);
+  });
+}
+
+int minim_dot(int precision, const void* a, const void* b, size_t n, double* out, void* stream) {
+  return guard([&] {
+    need(a, "a");
+    need(b, "b");
+    need(out, "out");
+    check_prec(precision);
+    auto& ws = minim_amd::Workspace::current();
+    const void* pa[1] = {a};
+    const void* pb[1] = {b};
+    minim_amd::launch_dots(precision, pa, pb, 1, n, ws.rs(), static_cast<hipStream_t>(stream));
+    ws.fetch(static_cast<hipStream_t>(stream), 1, out);
+  });
+}
+
+// ---------------------------------------------------------------- State
+int minim_state_create(const minim_potential* pot, const double* coords, size_t ndof, int precision,
+                       minim_state** out) {
+  return guard([&] {
+    need(pot, "pot");
+    need(out, "out");
+    if (ndof) need(coords, "coords");
+    check_prec(precision);
+    auto* h = new minim_state;
+    try {
+      h->s = std::make_unique<minim::State>(*pot->p, std::vector<double>(coords, coords + ndof), precision);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int minim_state_copy(const minim_state* s, minim_state** out) {
+  return guard([&] {
+    need(s, "state");
+    need(out, "out");
+    auto* h = new minim_state;
+    h->s = std::make_unique<minim::State>(*s->s);
+    *out = h;
+  });
+}
+
+int minim_state_destroy(minim_state* s) {
+  return guard([&] { delete s; });
+}
+
+int minim_state_energy(minim_state* s, const double* coords, double* e) {
+  return guard([&] {
+    need(s, "state");
+    need(e, "e");
+    *e = coords ? s->s->energy(std::vector<double>(coords, coords + s->s->ndof)) : s->s->energy();
+  });
+}
+
+int minim_state_gradient(minim_state* s, const double* coords, double* g) {
+  return guard([&] {
+    need(s, "state");
+    need(g, "g");
+    std::vector<double> gv = coords ? s->s->gradient(std::vector<double>(coords, coords + s->s->ndof))
+                                    : s->s->gradient();
+    std::memcpy(g, gv.data(), sizeof(double) * gv.size());
+  });
+}
+
+int minim_state_energy_gradient(minim_state* s, const double* coords, double* e, double* g) {
+  return guard([&] {
+    need(s, "state");
+    std::vector<double> gv;
+    if (coords)
+      s->s->energyGradient(std::vector<double>(coords, coords + s->s->ndof), e, g ? &gv : nullptr);
+    else
+      s->s->energyGradient(e, g ? &gv : nullptr);
+    if (g) std::memcpy(g, gv.data(), sizeof(double) * gv.size());
+  });
+}
+
+int minim_state_get_coords(minim_state* s, double* out) {
+  return guard([&] {
+    need(s, "state");
+    need(out, "out");
+    const auto& c = s->s->blockCoords();
+    std::memcpy(out, c.data(), sizeof(double) * c.size());
+  });
+}
+
+int minim_state_set_coords(minim_state* s, const double* in) {
+  return guard([&] {
+    need(s, "state");
+    need(in, "in");
+    s->s->coords(std::vector<double>(in, in + s->s->ndof));
+  });
+}
+
+int minim_state_apply_constraints(minim_state* s, double* data) {
+  return guard([&] {
+    need(s, "state");
+    need(data, "data");
+    std::vector<double> v(data, data + s->s->ndof);
+    s->s->applyConstraints(v);
+    std::memcpy(data, v.data(), sizeof(double) * v.size());
+  });
+}
+
+int minim_state_failed(minim_state* s) {
+  return guard([&] {
+    need(s, "state");
+    s->s->failed();
+  });
+}
+
+int minim_state_info(const minim_state* s, size_t* ndof, int* precision, double* convergence, int* is_failed) {
+  return guard([&] {
+    need(s, "state");
+    if (ndof) *ndof = s->s->ndof;
+    if (precision) *precision = s->s->precision();
+    if (convergence) *convergence = s->s->convergence;
+    if (is_failed) *is_failed = s->s->isFailed ? 1 : 0;
+  });
+}
+
+int minim_state_set_convergence(minim_state* s, double c) {
+  return guard([&] {
+    need(s, "state");
+    s->s->convergence = c;
+  });
+}
+
+int minim_state_set_precision(minim_state* s, int precision) {
+  return guard([&] {
+    need(s, "state");
+    check_prec(precision);
+    s->s->setPrecision(precision);
+  });
+}
+
+int minim_state_set_stream(minim_state* s, void* stream) {
+  return guard([&] {
+    need(s, "state");
+    s->s->setStream(stream);
+  });
+}
+
+void* minim_state_device_coords(minim_state* s) { return s ? s->s->deviceCoords() : nullptr; }
+
+// ---------------------------------------------------------------- minimisers
+int minim_minimiser_create(const minim_minimiser_params* p, minim_minimiser** out) {
+  return guard([&] {
+    need(p, "params");
+    need(out, "out");
+    std::unique_ptr<minim::Minimiser> m;
+    static const char* modes[] = {"state", "fp64", "fp32", "mixed"};
+    if (p->precision_mode < 0 || p->precision_mode > 3) throw std::invalid_argument("invalid precision mode");
+    if (p->kind == MINIM_LBFGS) {
+      auto l = std::make_unique<minim::Lbfgs>();
+      l->setM(p->m > 0 ? p->m : 5);
+      l->setMaxStep(p->max_step);
+      l->setInitStep(p->init_step);
+      l->setPrecision(modes[p->precision_mode], p->switch_rms);
+      m = std::move(l);
+    } else if (p->kind == MINIM_FIRE) {
+      auto f = std::make_unique<minim::Fire>();
+      f->setDtMax(p->dt_max);
+      m = std::move(f);
+    } else if (p->kind == MINIM_GRADDESCENT) {
+      auto gd = std::make_unique<minim::GradDescent>();
+      gd->setAlpha(p->alpha);
+      m = std::move(gd);
+    } else {
+      throw std::invalid_argument("unknown minimiser kind");
+    }
+    m->setMaxIter(p->max_iter);
+    m->setLinesearch(p->linesearch == MINIM_LS_NONE ? "none" : "backtracking");
+    auto* h = new minim_minimiser;
+    h->m = std::move(m);
+    *out = h;
+  });
+}
+
+int minim_minimiser_destroy(minim_minimiser* m) {
+  return guard([&] { delete m; });
+}
+
+static void fill_result(const minim_minimiser* m, minim_result* r) {
+  if (!r) return;
+  const auto& st = m->m->stats;
+  r->iterations = m->m->iter;
+  r->converged = st.converged ? 1 : 0;
+  r->n_energy = st.nEnergy;
+  r->n_gradient = st.nGradient;
+  r->n_energy_reused = st.nEnergyReused;
+  r->iter_fp32 = st.iterFP32;
+  r->iter_fp64 = st.iterFP64;
+  r->energy = st.energy;
+  r->grad_rms = st.gradRms;
+  auto* f = dynamic_cast<const minim::Fire*>(m->m.get());
+  r->dt_max = f ? f->dtMax : 0.0;
+}
+
+int minim_minimise(minim_minimiser* m, minim_state* s, minim_adjust_fn fn, void* user, double* coords_out,
+                   minim_result* r) {
+  return guard([&] {
+    need(m, "minimiser");
+    need(s, "state");
+    std::function<void(int, minim::State&)> adj = nullptr;
+    if (fn) adj = [fn, user, s](int it, minim::State&) { fn(it, s, user); };
+    std::vector<double> x = m->m->minimise(*s->s, adj);
+    if (coords_out) std::memcpy(coords_out, x.data(), sizeof(double) * x.size());
+    fill_result(m, r);
+  });
+}
+
+int minim_minimise_log(minim_minimiser* m, minim_state* s, const char* log_type, double* coords_out,
+                       minim_result* r) {
+  return guard([&] {
+    need(m, "minimiser");
+    need(s, "state");
+    need(log_type, "log_type");
+    std::vector<double> x = m->m->minimise(*s->s, std::string(log_type));
+    if (coords_out) std::memcpy(coords_out, x.data(), sizeof(double) * x.size());
+    fill_result(m, r);
+  });
+}
+
+}  // extern "C"

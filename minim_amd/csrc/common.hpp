@@ -1,0 +1,139 @@
+// common.hpp -- shared device/host helpers for the minim_amd HIP library.
+//
+// Conventions used by every kernel in this library:
+//  * vectors are flat arrays of T (double or float) resident in HBM; the
+//    PhaseField layout is AoS, index = site * nFluid + fluid, site = (x*Ny+y)*Nz+z;
+//  * every reduction accumulates in FP64, per thread, then per wavefront
+//    (64 lanes, shuffles), then per block (LDS), then across blocks in a fixed
+//    order by the last block to finish -- results are bitwise reproducible;
+//  * kernels are compiled with -ffp-contract=off: every product is rounded
+//    before it is added, as in the reference's element-wise vector operators.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace minim_amd {
+
+constexpr int kBlock = 256;          // threads per block (4 wavefronts of 64)
+constexpr int kWave = 64;
+constexpr int kMaxRedBlocks = 8192;  // upper bound on blocks of any reducing launch
+constexpr int kMaxDots = 96;         // upper bound on values reduced by one launch
+
+enum Prec : int { F64 = 0, F32 = 1 };
+
+inline size_t prec_bytes(int p) { return p == F64 ? 8 : 4; }
+
+struct HipError : std::runtime_error {
+  explicit HipError(const std::string& s) : std::runtime_error(s) {}
+};
+
+#define MINIM_HIP_CHECK(expr)                                                                  \
+  do {                                                                                         \
+    hipError_t _e = (expr);                                                                    \
+    if (_e != hipSuccess)                                                                      \
+      throw ::minim_amd::HipError(std::string("HIP error ") + hipGetErrorString(_e) + " at " + \
+                                  __FILE__ + ":" + std::to_string(__LINE__) + " (" #expr ")"); \
+  } while (0)
+
+// ---------------------------------------------------------------- reductions
+// Sum K doubles over the 64 lanes of a wavefront.
+template <int K>
+__device__ inline void wave_sum(double (&v)[K]) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] += __shfl_xor(v[k], off, kWave);
+  }
+}
+
+// Block-wide sum of K doubles; the result is valid in thread 0 only.
+// lds must hold (kBlock/kWave)*K doubles.
+template <int K>
+__device__ inline void block_sum(double (&v)[K], double* lds) {
+  wave_sum<K>(v);
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) lds[wid * K + k] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      double s = lds[k];
+      for (int w = 1; w < kBlock / kWave; w++) s += lds[w * K + k];
+      v[k] = s;
+    }
+  }
+}
+
+// Scratch for cross-block reductions: one row of K partials per block, a
+// completion counter and the final result (device).  The last block to arrive
+// sums the rows in block order, writes result[0..K) and re-arms the counter.
+struct RedScratch {
+  double* partials;   // [kMaxRedBlocks * kMaxDots]
+  double* result;     // [kMaxDots]
+  unsigned* counter;  // [1], zero between launches
+};
+
+// All threads of the (1-D) grid must call this; v is each thread's contribution.
+template <int K>
+__device__ inline void grid_sum_store(double (&v)[K], RedScratch rs) {
+  __shared__ double lds[(kBlock / kWave) * (K > 0 ? K : 1)];
+  __shared__ bool last;
+  block_sum<K>(v, lds);
+  const int nb = gridDim.x;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) rs.partials[(size_t)blockIdx.x * K + k] = v[k];
+    __threadfence();
+    unsigned prev = atomicAdd(rs.counter, 1u);
+    last = (prev == (unsigned)(nb - 1));
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  // Fixed-order final sum: thread t takes blocks t, t+kBlock, ... then a block tree.
+  double w[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) w[k] = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kBlock) {
+#pragma unroll
+    for (int k = 0; k < K; k++)
+      w[k] += __hip_atomic_load(&rs.partials[(size_t)b * K + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  block_sum<K>(w, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) rs.result[k] = w[k];
+    atomicExch(rs.counter, 0u);  // vector atomic: re-arm for the next launch
+  }
+}
+
+// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each
+// with a private L2.  Remap the hardware block id so that every XCD receives a
+// contiguous range of logical blocks: neighbouring tiles (which share halo
+// rows) then run on the same XCD and hit the same L2.  A bijection for any nb.
+constexpr int kXcd = 8;
+__device__ inline int xcd_logical_block(int b, int nb) {
+  const int q = nb / kXcd, r = nb % kXcd;
+  const int k = b % kXcd, i = b / kXcd;
+  return (k < r) ? k * (q + 1) + i : r * (q + 1) + (k - r) * q + i;
+}
+
+// Number of blocks for a grid-stride launch over n work items.
+inline int grid_blocks(size_t n, int per_thread = 1, int max_blocks = kMaxRedBlocks) {
+  size_t threads = (n + per_thread - 1) / per_thread;
+  size_t b = (threads + kBlock - 1) / kBlock;
+  if (b < 1) b = 1;
+  if (b > (size_t)max_blocks) b = max_blocks;
+  return (int)b;
+}
+
+}  // namespace minim_amd

@@ -1,0 +1,540 @@
+// lj.hip -- Lennard-Jones potential on MI355X.
+//
+// Reference semantics (LjNd::init / elementEnergyGradient,
+// src/potentials/LjNd.cpp:12-53): one element per pair i<j, E = 4 eps (lj12 -
+// lj6), g_i += dx * (-24 eps (2 lj12 - lj6) / r^2), g_j -= the same.  The
+// reference stores N(N-1)/2 heap elements and scatters into g; here one thread
+// owns particle i and gathers over all j through LDS tiles of positions, so the
+// gradient is written once per particle with no atomics and the energy is the
+// FP64 sum of per-particle halves.
+//
+// Cutoff mode (no reference counterpart): truncated-and-shifted at rc in a
+// periodic box, with a full Verlet list of radius rc+skin built from a cell
+// list; the list is rebuilt when any particle has moved more than skin/2 from
+// where the list was built.  Lists are built deterministically (cell contents
+// sorted by particle id), so energies and gradients are reproducible.
+#include <cmath>
+#include <stdexcept>
+#include <vector>
+
+#include "common.hpp"
+#include "minim/LjNd.h"
+#include "runtime.hpp"
+#include "vec.hpp"
+
+namespace minim_amd {
+namespace {
+
+template <typename T, int D>
+__device__ inline void load_pos(const T* __restrict__ x, const T* __restrict__ d, T t, int i, T (&p)[D]) {
+#pragma unroll
+  for (int k = 0; k < D; k++) {
+    T c = x[(size_t)i * D + k];
+    if (d) {
+      T s = t * d[(size_t)i * D + k];
+      c = c + s;
+    }
+    p[k] = c;
+  }
+}
+
+// All pairs, gather form.  mode bit 1 energy, bit 2 gradient.
+template <typename T, int D>
+__global__ __launch_bounds__(kBlock) void lj_allpairs_kernel(const T* __restrict__ x, const T* __restrict__ d, T t,
+                                                             int N, T sig6, T eps, int mode, T* __restrict__ g,
+                                                             RedScratch rs) {
+  __shared__ T tile[kBlock * D];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  T xi[D];
+  T gi[D];
+#pragma unroll
+  for (int k = 0; k < D; k++) gi[k] = (T)0;
+  double ei = 0.0;
+  if (i < N) load_pos<T, D>(x, d, t, i, xi);
+  for (int base = 0; base < N; base += kBlock) {
+    const int j = base + threadIdx.x;
+    if (j < N) {
+      T pj[D];
+      load_pos<T, D>(x, d, t, j, pj);
+#pragma unroll
+      for (int k = 0; k < D; k++) tile[threadIdx.x * D + k] = pj[k];
+    }
+    __syncthreads();
+    if (i < N) {
+      const int cnt = min(kBlock, N - base);
+      for (int jj = 0; jj < cnt; jj++) {
+        if (base + jj == i) continue;
+        T dx[D], r2 = (T)0;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+          dx[k] = xi[k] - tile[jj * D + k];
+          r2 += dx[k] * dx[k];
+        }
+        const T r6 = r2 * r2 * r2;
+        const T lj6 = sig6 / r6;
+        const T lj12 = lj6 * lj6;
+        if (mode & 1) ei += 0.5 * (double)((T)4 * eps * (lj12 - lj6));
+        if (mode & 2) {
+          const T f = (T)-24 * eps * ((T)2 * lj12 - lj6) / r2;
+#pragma unroll
+          for (int k = 0; k < D; k++) gi[k] += dx[k] * f;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if ((mode & 2) && i < N) {
+#pragma unroll
+    for (int k = 0; k < D; k++) g[(size_t)i * D + k] = gi[k];
+  }
+  double acc[1] = {ei};
+  grid_sum_store<1>(acc, rs);
+}
+
+// ---------------------------------------------------------------- cutoff mode
+struct BoxArgs {
+  double L[3];
+  double inv_cell[3];
+  int nc[3];
+  double rlist2;
+  double rc2;
+  double half_skin2;
+};
+
+template <typename T>
+__device__ inline T min_image(T v, T L) {
+  return v - L * rint(v / L);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void lj_cell_index_kernel(const T* __restrict__ x, int N, BoxArgs b,
+                                                               int* __restrict__ cell, int* __restrict__ count,
+                                                               double* __restrict__ xbuild) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= N) return;
+  int c[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const double v = (double)x[(size_t)i * 3 + k];
+    xbuild[(size_t)i * 3 + k] = v;
+    double w = v - b.L[k] * floor(v / b.L[k]);  // wrapped into [0, L)
+    int ci = (int)(w * b.inv_cell[k]);
+    if (ci >= b.nc[k]) ci = b.nc[k] - 1;
+    if (ci < 0) ci = 0;
+    c[k] = ci;
+  }
+  const int id = (c[0] * b.nc[1] + c[1]) * b.nc[2] + c[2];
+  cell[i] = id;
+  atomicAdd(&count[id], 1);
+}
+
+// Exclusive scan of cell counts by a single block (cell counts are small).
+__global__ __launch_bounds__(kBlock) void lj_scan_kernel(const int* __restrict__ count, int* __restrict__ start,
+                                                         int* __restrict__ cursor, int ncell) {
+  __shared__ int part[kBlock];
+  const int per = (ncell + kBlock - 1) / kBlock;
+  const int lo = threadIdx.x * per, hi = min(lo + per, ncell);
+  int s = 0;
+  for (int c = lo; c < hi; c++) s += count[c];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int k = 0; k < kBlock; k++) {
+      int v = part[k];
+      part[k] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  int run = part[threadIdx.x];
+  for (int c = lo; c < hi; c++) {
+    start[c] = run;
+    cursor[c] = 0;
+    run += count[c];
+  }
+  if (threadIdx.x == kBlock - 1) start[ncell] = run;
+}
+
+__global__ __launch_bounds__(kBlock) void lj_fill_kernel(const int* __restrict__ cell, int N,
+                                                         const int* __restrict__ start, int* __restrict__ cursor,
+                                                         int* __restrict__ order) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= N) return;
+  const int c = cell[i];
+  const int pos = start[c] + atomicAdd(&cursor[c], 1);
+  order[pos] = i;
+}
+
+// Sort each cell's members by particle id so the neighbour order is reproducible.
+__global__ __launch_bounds__(kBlock) void lj_sort_cells_kernel(const int* __restrict__ start, int* __restrict__ order,
+                                                               int ncell) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= ncell) return;
+  const int lo = start[c], hi = start[c + 1];
+  for (int a = lo + 1; a < hi; a++) {
+    int v = order[a];
+    int b = a - 1;
+    while (b >= lo && order[b] > v) {
+      order[b + 1] = order[b];
+      b--;
+    }
+    order[b + 1] = v;
+  }
+}
+
+// Full Verlet list: neighbour k of particle i stored at nbr[k*N + i].
+template <typename T>
+__global__ __launch_bounds__(kBlock) void lj_build_list_kernel(const T* __restrict__ x, int N, BoxArgs b,
+                                                               const int* __restrict__ cell,
+                                                               const int* __restrict__ start,
+                                                               const int* __restrict__ order, int maxn,
+                                                               int* __restrict__ nn, int* __restrict__ nbr,
+                                                               int* __restrict__ overflow) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= N) return;
+  const double xi0 = (double)x[(size_t)i * 3], xi1 = (double)x[(size_t)i * 3 + 1], xi2 = (double)x[(size_t)i * 3 + 2];
+  const int c = cell[i];
+  const int cz = c % b.nc[2], cy = (c / b.nc[2]) % b.nc[1], cx = c / (b.nc[1] * b.nc[2]);
+  int cnt = 0;
+  for (int dx = -1; dx <= 1; dx++) {
+    const int ax = (cx + dx + b.nc[0]) % b.nc[0];
+    for (int dy = -1; dy <= 1; dy++) {
+      const int ay = (cy + dy + b.nc[1]) % b.nc[1];
+      for (int dz = -1; dz <= 1; dz++) {
+        const int az = (cz + dz + b.nc[2]) % b.nc[2];
+        const int cc = (ax * b.nc[1] + ay) * b.nc[2] + az;
+        for (int p = start[cc]; p < start[cc + 1]; p++) {
+          const int j = order[p];
+          if (j == i) continue;
+          const double r0 = min_image(xi0 - (double)x[(size_t)j * 3], b.L[0]);
+          const double r1 = min_image(xi1 - (double)x[(size_t)j * 3 + 1], b.L[1]);
+          const double r2 = min_image(xi2 - (double)x[(size_t)j * 3 + 2], b.L[2]);
+          if (r0 * r0 + r1 * r1 + r2 * r2 < b.rlist2) {
+            if (cnt < maxn) nbr[(size_t)cnt * N + i] = j;
+            cnt++;
+          }
+        }
+      }
+    }
+  }
+  nn[i] = cnt < maxn ? cnt : maxn;
+  if (cnt > maxn) atomicMax(overflow, cnt);
+}
+
+// Small boxes (fewer than 3 cells along a side): list by brute force.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void lj_build_list_brute_kernel(const T* __restrict__ x, int N, BoxArgs b,
+                                                                     int maxn, int* __restrict__ nn,
+                                                                     int* __restrict__ nbr,
+                                                                     int* __restrict__ overflow,
+                                                                     double* __restrict__ xbuild) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= N) return;
+  double xi[3];
+  for (int k = 0; k < 3; k++) {
+    xi[k] = (double)x[(size_t)i * 3 + k];
+    xbuild[(size_t)i * 3 + k] = xi[k];
+  }
+  int cnt = 0;
+  for (int j = 0; j < N; j++) {
+    if (j == i) continue;
+    double r2 = 0;
+    for (int k = 0; k < 3; k++) {
+      const double v = min_image(xi[k] - (double)x[(size_t)j * 3 + k], b.L[k]);
+      r2 += v * v;
+    }
+    if (r2 < b.rlist2) {
+      if (cnt < maxn) nbr[(size_t)cnt * N + i] = j;
+      cnt++;
+    }
+  }
+  nn[i] = cnt < maxn ? cnt : maxn;
+  if (cnt > maxn) atomicMax(overflow, cnt);
+}
+
+// Count of particles displaced more than skin/2 from the list positions.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void lj_displacement_kernel(const T* __restrict__ x, const T* __restrict__ d,
+                                                                 T t, int N, const double* __restrict__ xbuild,
+                                                                 double lim2, RedScratch rs) {
+  double acc[1] = {0.0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
+    T p[3];
+    load_pos<T, 3>(x, d, t, i, p);
+    double r2 = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const double v = (double)p[k] - xbuild[(size_t)i * 3 + k];
+      r2 += v * v;
+    }
+    if (r2 > lim2) acc[0] += 1.0;
+  }
+  grid_sum_store<1>(acc, rs);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void lj_cut_kernel(const T* __restrict__ x, const T* __restrict__ d, T t, int N,
+                                                        BoxArgs b, T sig2, T eps, T eshift, int mode,
+                                                        const int* __restrict__ nn, const int* __restrict__ nbr,
+                                                        T* __restrict__ g, RedScratch rs) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double ei = 0.0;
+  if (i < N) {
+    T xi[3];
+    load_pos<T, 3>(x, d, t, i, xi);
+    const T L0 = (T)b.L[0], L1 = (T)b.L[1], L2 = (T)b.L[2];
+    const T rc2 = (T)b.rc2;
+    T gi[3] = {(T)0, (T)0, (T)0};
+    const int cnt = nn[i];
+    for (int k = 0; k < cnt; k++) {
+      const int j = nbr[(size_t)k * N + i];
+      T xj[3];
+      load_pos<T, 3>(x, d, t, j, xj);
+      const T r0 = min_image(xi[0] - xj[0], L0);
+      const T r1 = min_image(xi[1] - xj[1], L1);
+      const T r2v = min_image(xi[2] - xj[2], L2);
+      const T r2 = r0 * r0 + r1 * r1 + r2v * r2v;
+      if (r2 >= rc2) continue;
+      const T s2 = sig2 / r2;
+      const T lj6 = s2 * s2 * s2;
+      const T lj12 = lj6 * lj6;
+      if (mode & 1) ei += 0.5 * ((double)((T)4 * eps * (lj12 - lj6)) - (double)eshift);
+      if (mode & 2) {
+        const T f = (T)-24 * eps * ((T)2 * lj12 - lj6) / r2;
+        gi[0] += r0 * f;
+        gi[1] += r1 * f;
+        gi[2] += r2v * f;
+      }
+    }
+    if (mode & 2) {
+      g[(size_t)i * 3] = gi[0];
+      g[(size_t)i * 3 + 1] = gi[1];
+      g[(size_t)i * 3 + 2] = gi[2];
+    }
+  }
+  double acc[1] = {ei};
+  grid_sum_store<1>(acc, rs);
+}
+
+template <typename T>
+void launch_allpairs(int D, const void* x, const void* d, double t, int N, double sigma, double eps, int mode, void* g,
+                     RedScratch rs, hipStream_t s) {
+  const int nb = (N + kBlock - 1) / kBlock;
+  if (nb > kMaxRedBlocks) throw std::invalid_argument("LjNd: too many particles for the all-pairs kernel");
+  const T* xx = (const T*)x;
+  const T* dd = (const T*)d;
+  T* gg = (T*)g;
+  const T sig6 = (T)std::pow(sigma, 6);
+  switch (D) {
+    case 1: lj_allpairs_kernel<T, 1><<<nb, kBlock, 0, s>>>(xx, dd, (T)t, N, sig6, (T)eps, mode, gg, rs); break;
+    case 2: lj_allpairs_kernel<T, 2><<<nb, kBlock, 0, s>>>(xx, dd, (T)t, N, sig6, (T)eps, mode, gg, rs); break;
+    case 3: lj_allpairs_kernel<T, 3><<<nb, kBlock, 0, s>>>(xx, dd, (T)t, N, sig6, (T)eps, mode, gg, rs); break;
+    case 4: lj_allpairs_kernel<T, 4><<<nb, kBlock, 0, s>>>(xx, dd, (T)t, N, sig6, (T)eps, mode, gg, rs); break;
+    default: throw std::invalid_argument("LjNd: nDim must be 1..4 on the device path");
+  }
+  MINIM_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+}  // namespace minim_amd
+
+namespace minim {
+
+using minim_amd::DeviceBuffer;
+using minim_amd::kBlock;
+
+struct LjNd::Impl {
+  minim_amd::BoxArgs box{};
+  bool cellMode = false;
+  int maxn = 128;
+  long builds = 0;
+  int lastMax = 0;
+  bool built = false;
+  DeviceBuffer cell, count, start, cursor, order, nn, nbr, flag, xbuild;
+  int ncell = 0;
+};
+
+LjNd::LjNd(const LjNd& o)
+    : NewPotential<LjNd>(o), nDim(o.nDim), nParticle(o.nParticle), sigma(o.sigma), epsilon(o.epsilon),
+      cutoff(o.cutoff), box(o.box), skin(o.skin) {}
+
+LjNd& LjNd::operator=(const LjNd& o) {
+  NewPotential<LjNd>::operator=(o);
+  nDim = o.nDim;
+  nParticle = o.nParticle;
+  sigma = o.sigma;
+  epsilon = o.epsilon;
+  cutoff = o.cutoff;
+  box = o.box;
+  skin = o.skin;
+  impl_.reset();
+  return *this;
+}
+
+LjNd::~LjNd() = default;
+
+LjNd& LjNd::setSigma(double s) {
+  sigma = s;
+  return *this;
+}
+LjNd& LjNd::setEpsilon(double e) {
+  epsilon = e;
+  return *this;
+}
+LjNd& LjNd::setCutoff(double rc, std::array<double, 3> b, double sk) {
+  if (rc <= 0 || sk < 0) throw std::invalid_argument("LjNd: cutoff must be positive and skin non-negative.");
+  for (double v : b)
+    if (v <= 0) throw std::invalid_argument("LjNd: box lengths must be positive.");
+  cutoff = rc;
+  box = b;
+  skin = sk;
+  return *this;
+}
+
+long LjNd::listBuilds() const { return impl_ ? impl_->builds : 0; }
+int LjNd::maxNeighbours() const { return impl_ ? impl_->lastMax : 0; }
+
+// LjNd::init (src/potentials/LjNd.cpp:12-26): nParticle from the coordinate count.
+void LjNd::init(const vector<double>& coords) {
+  if (nDim < 1) throw std::invalid_argument("LjNd: nDim must be positive.");
+  if (coords.size() % nDim != 0)
+    throw std::invalid_argument("LjNd: Length of coords must be a multiple of the number of dimensions.");
+  nParticle = (int)(coords.size() / nDim);
+  uploadFixed(coords.size());
+  impl_ = std::make_shared<Impl>();
+  if (cutoff > 0) {
+    if (nDim != 3) throw std::invalid_argument("LjNd: the cutoff/periodic mode needs nDim = 3.");
+    auto& b = impl_->box;
+    const double rl = cutoff + skin;
+    for (int k = 0; k < 3; k++) {
+      if (box[k] < 2 * cutoff) throw std::invalid_argument("LjNd: box must be at least twice the cutoff.");
+      b.L[k] = box[k];
+      b.nc[k] = (int)std::floor(box[k] / rl);
+      if (b.nc[k] < 1) b.nc[k] = 1;
+      b.inv_cell[k] = b.nc[k] / box[k];
+    }
+    b.rlist2 = rl * rl;
+    b.rc2 = cutoff * cutoff;
+    b.half_skin2 = 0.25 * skin * skin;
+    impl_->cellMode = b.nc[0] >= 3 && b.nc[1] >= 3 && b.nc[2] >= 3;
+    impl_->ncell = b.nc[0] * b.nc[1] * b.nc[2];
+    const size_t N = nParticle;
+    impl_->cell.reset(sizeof(int) * N);
+    impl_->count.reset(sizeof(int) * impl_->ncell);
+    impl_->start.reset(sizeof(int) * (impl_->ncell + 1));
+    impl_->cursor.reset(sizeof(int) * impl_->ncell);
+    impl_->order.reset(sizeof(int) * N);
+    impl_->nn.reset(sizeof(int) * N);
+    impl_->nbr.reset(sizeof(int) * N * impl_->maxn);
+    impl_->flag.reset(sizeof(int));
+    impl_->xbuild.reset(sizeof(double) * 3 * N);
+  }
+}
+
+void LjNd::evaluate(int prec, const void* x, const void* d, double t, double* e, void* g, void* stream) const {
+  if (!impl_) throw std::runtime_error("LjNd: not initialised (construct a State first).");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  auto& ws = minim_amd::Workspace::current();
+  const int mode = (e ? 1 : 0) | (g ? 2 : 0);
+  if (!mode) return;
+  const int N = nParticle;
+  if (cutoff <= 0) {
+    if (prec == FP64) minim_amd::launch_allpairs<double>(nDim, x, d, t, N, sigma, epsilon, mode, g, ws.rs(), s);
+    else minim_amd::launch_allpairs<float>(nDim, x, d, t, N, sigma, epsilon, mode, g, ws.rs(), s);
+  } else {
+    Impl& I = *impl_;
+    const int nb = (N + kBlock - 1) / kBlock;
+    if (nb > minim_amd::kMaxRedBlocks) throw std::invalid_argument("LjNd: too many particles for one launch");
+    // list still valid for these positions?
+    bool rebuild = !I.built;
+    if (!rebuild) {
+      const int nbd = minim_amd::grid_blocks(N, 4, 1024);
+      if (prec == FP64)
+        minim_amd::lj_displacement_kernel<double><<<nbd, kBlock, 0, s>>>(
+            (const double*)x, (const double*)d, t, N, (const double*)I.xbuild.get(), I.box.half_skin2, ws.rs());
+      else
+        minim_amd::lj_displacement_kernel<float><<<nbd, kBlock, 0, s>>>(
+            (const float*)x, (const float*)d, (float)t, N, (const double*)I.xbuild.get(), I.box.half_skin2, ws.rs());
+      MINIM_HIP_CHECK(hipGetLastError());
+      double moved = 0;
+      ws.fetch(s, 1, &moved);
+      rebuild = moved > 0;
+    }
+    if (rebuild) {
+      // Build at the evaluated positions: materialise x + t*d when needed.
+      DeviceBuffer tmp;
+      const void* xb = x;
+      if (d) {
+        tmp.reset(3 * (size_t)N * minim_amd::prec_bytes(prec));
+        minim_amd::device_copy(tmp.get(), x, 3 * (size_t)N * minim_amd::prec_bytes(prec), s);
+        minim_amd::launch_axpy(prec, tmp.get(), d, t, 3 * (size_t)N, s);
+        xb = tmp.get();
+      }
+      for (int attempt = 0; attempt < 2; attempt++) {
+        MINIM_HIP_CHECK(hipMemsetAsync(I.flag.get(), 0, sizeof(int), s));
+        if (I.cellMode) {
+          MINIM_HIP_CHECK(hipMemsetAsync(I.count.get(), 0, sizeof(int) * I.ncell, s));
+          const int nbc = (I.ncell + kBlock - 1) / kBlock;
+          if (prec == FP64)
+            minim_amd::lj_cell_index_kernel<double><<<nb, kBlock, 0, s>>>((const double*)xb, N, I.box, (int*)I.cell.get(),
+                                                                          (int*)I.count.get(), (double*)I.xbuild.get());
+          else
+            minim_amd::lj_cell_index_kernel<float><<<nb, kBlock, 0, s>>>((const float*)xb, N, I.box, (int*)I.cell.get(),
+                                                                         (int*)I.count.get(), (double*)I.xbuild.get());
+          minim_amd::lj_scan_kernel<<<1, kBlock, 0, s>>>((const int*)I.count.get(), (int*)I.start.get(),
+                                                         (int*)I.cursor.get(), I.ncell);
+          minim_amd::lj_fill_kernel<<<nb, kBlock, 0, s>>>((const int*)I.cell.get(), N, (const int*)I.start.get(),
+                                                          (int*)I.cursor.get(), (int*)I.order.get());
+          minim_amd::lj_sort_cells_kernel<<<nbc, kBlock, 0, s>>>((const int*)I.start.get(), (int*)I.order.get(), I.ncell);
+          if (prec == FP64)
+            minim_amd::lj_build_list_kernel<double><<<nb, kBlock, 0, s>>>(
+                (const double*)xb, N, I.box, (const int*)I.cell.get(), (const int*)I.start.get(),
+                (const int*)I.order.get(), I.maxn, (int*)I.nn.get(), (int*)I.nbr.get(), (int*)I.flag.get());
+          else
+            minim_amd::lj_build_list_kernel<float><<<nb, kBlock, 0, s>>>(
+                (const float*)xb, N, I.box, (const int*)I.cell.get(), (const int*)I.start.get(),
+                (const int*)I.order.get(), I.maxn, (int*)I.nn.get(), (int*)I.nbr.get(), (int*)I.flag.get());
+        } else {
+          if (prec == FP64)
+            minim_amd::lj_build_list_brute_kernel<double><<<nb, kBlock, 0, s>>>(
+                (const double*)xb, N, I.box, I.maxn, (int*)I.nn.get(), (int*)I.nbr.get(), (int*)I.flag.get(),
+                (double*)I.xbuild.get());
+          else
+            minim_amd::lj_build_list_brute_kernel<float><<<nb, kBlock, 0, s>>>(
+                (const float*)xb, N, I.box, I.maxn, (int*)I.nn.get(), (int*)I.nbr.get(), (int*)I.flag.get(),
+                (double*)I.xbuild.get());
+        }
+        MINIM_HIP_CHECK(hipGetLastError());
+        int over = 0;
+        MINIM_HIP_CHECK(hipMemcpyAsync(&over, I.flag.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+        MINIM_HIP_CHECK(hipStreamSynchronize(s));
+        if (over == 0) break;
+        if (attempt == 1) throw std::runtime_error("LjNd: neighbour list overflow");
+        I.maxn = over + 32;
+        I.nbr.reset(sizeof(int) * (size_t)N * I.maxn);
+      }
+      I.built = true;
+      I.builds++;
+    }
+    const double src6 = std::pow(sigma / cutoff, 6);
+    const double eshift = 4 * epsilon * (src6 * src6 - src6);
+    if (prec == FP64)
+      minim_amd::lj_cut_kernel<double><<<nb, kBlock, 0, s>>>(
+          (const double*)x, (const double*)d, t, N, I.box, sigma * sigma, epsilon, eshift, mode,
+          (const int*)I.nn.get(), (const int*)I.nbr.get(), (double*)g, ws.rs());
+    else
+      minim_amd::lj_cut_kernel<float><<<nb, kBlock, 0, s>>>(
+          (const float*)x, (const float*)d, (float)t, N, I.box, (float)(sigma * sigma), (float)epsilon, (float)eshift,
+          mode, (const int*)I.nn.get(), (const int*)I.nbr.get(), (float*)g, ws.rs());
+    MINIM_HIP_CHECK(hipGetLastError());
+  }
+  if (g) applyConstraints(prec, g, (size_t)N * nDim, stream);
+  if (e) {
+    double r[1];
+    ws.fetch(s, 1, r);
+    *e = r[0];
+  }
+}
+
+}  // namespace minim

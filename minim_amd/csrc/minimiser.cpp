@@ -1,0 +1,678 @@
+// minimiser.cpp -- device-resident minimisers with the reference's semantics.
+//
+//   Minimiser::minimise  src/Minimiser.cpp:27-38 (callback, failure flag,
+//                        iter = 0..maxIter inclusive), log variant :41-69
+//   Lbfgs                src/minimisers/Lbfgs.cpp:41-133
+//   line search          src/minimisers/linesearch.cpp:8-29
+//   Fire                 src/minimisers/Fire.cpp:27-90
+//   GradDescent          src/minimisers/GradDescent.cpp:27-46
+//
+// Every vector lives on the GPU; the host only holds scalars.  Two departures
+// from the reference's *implementation* (not its results):
+//  * L-BFGS keeps the (2m+1)^2 Gram matrix of {g, s_i, y_i} on the host and
+//    runs the two-loop recursion on coefficients, so a direction costs one
+//    fused pass over the basis instead of 2m dot/axpy sweeps;
+//  * the line search's e0 (recomputed by the reference on every call,
+//    linesearch.cpp:13) is taken from the previous accepted trial, which is the
+//    energy of exactly the current coordinates; it is re-evaluated whenever
+//    that is not the case (after the unevaluated 10-failure fallback, a step
+//    without line search, or coordinates changed by the caller).
+#include <cmath>
+#include <cstdio>
+#include <deque>
+#include <iomanip>
+#include <sstream>
+#include <stdexcept>
+
+#include "minim/Minimiser.h"
+#include "minim/State.h"
+#include "runtime.hpp"
+#include "vec.hpp"
+
+namespace minim {
+
+using minim_amd::DVec;
+using minim_amd::Workspace;
+
+// ------------------------------------------------------------------ Minimiser
+Minimiser& Minimiser::setMaxIter(int m) {
+  maxIter = m;
+  return *this;
+}
+
+Minimiser& Minimiser::setLinesearch(std::string method) {
+  if (method != "backtracking" && method != "none") throw std::invalid_argument("Invalid line search method.");
+  linesearch = method;
+  return *this;
+}
+
+std::vector<double> Minimiser::minimise(State& state, std::function<void(int, State&)> adjustState) {
+  if (!state.usesThisProc) return std::vector<double>();
+  stats = MinimiseStats{};
+  init(state);
+  for (iter = 0; iter <= maxIter; iter++) {
+    if (adjustState) adjustState(iter, state);
+    if (state.isFailed) break;
+    iteration(state);
+    if (checkConvergence(state)) {
+      stats.converged = true;
+      break;
+    }
+  }
+  finish(state);
+  state.invalidateHost();
+  return state.coords();
+}
+
+std::vector<double> Minimiser::minimise(State& state, std::string logType) {
+  std::function<void(int, State&)> logFn = nullptr;
+  size_t divPos = logType.find("-");
+  if (divPos < logType.length()) {
+    std::string logFields = logType.substr(0, divPos);
+    int logIter = std::stoi(logType.substr(divPos + 1));
+    if (logIter > 0) {
+      int iterDigits = (int)std::to_string(maxIter).length();
+      logFn = [logIter, logFields, iterDigits](int i, State& s) {
+        if (i % logIter != 0) return;
+        bool logE = logFields.find('e') < logFields.length();
+        bool logG = logFields.find('g') < logFields.length();
+        double e = 0;
+        std::vector<double> g;
+        s.invalidateHost();
+        s.energyGradient(logE ? &e : nullptr, logG ? &g : nullptr);
+        std::ostringstream log;
+        log << "I: " << std::setw(iterDigits) << i;
+        if (logE) log << "  E: " << e;
+        if (logG) {
+          double gg = 0;
+          for (double v : g) gg += v * v;
+          log << "  G: " << std::sqrt(gg / g.size());
+        }
+        std::printf("%s\n", log.str().c_str());
+        std::fflush(stdout);
+      };
+    }
+  }
+  return minimise(state, logFn);
+}
+
+namespace {
+
+hipStream_t hs(const State& s) { return static_cast<hipStream_t>(s.stream()); }
+
+// Energy cache of the coordinates a minimiser last left in the State.
+struct EnergyCache {
+  bool valid = false;
+  double e = 0;
+  unsigned long version = 0;
+  void set(const State& s, double v) {
+    valid = true;
+    e = v;
+    version = s.coordsVersion();
+  }
+  bool ok(const State& s) const { return valid && version == s.coordsVersion(); }
+};
+
+// backtrackingLinesearch (linesearch.cpp:8-29) along step = tau0 * d, with d a
+// device vector.  Trials are evaluated at x + tau*d without materialising
+// them; tau halves exactly like step *= 0.5.  Returns the tau applied to x.
+double backtrack(State& s, const void* d, double tau0, double gs, EnergyCache& ec, MinimiseStats& st,
+                 bool* acceptedOut = nullptr) {
+  const double c = 0.5, shrink = 0.5;
+  double t = -c * gs;
+  double e0;
+  if (ec.ok(s)) {
+    e0 = ec.e;
+    st.nEnergyReused++;
+  } else {
+    s.pot->evaluate(s.precision(), s.deviceCoords(), nullptr, 0.0, &e0, nullptr, s.stream());
+    st.nEnergy++;
+  }
+  double tau = tau0;
+  bool accepted = false;
+  double e = 0;
+  for (int i = 0; i < 10; i++) {
+    s.pot->evaluate(s.precision(), s.deviceCoords(), d, tau, &e, nullptr, s.stream());
+    st.nEnergy++;
+    if (e0 - e >= t) {
+      accepted = true;
+      break;
+    }
+    tau = tau * shrink;
+    t = t * shrink;
+  }
+  minim_amd::launch_axpy(s.precision(), s.deviceCoords(), d, tau, s.ndof, hs(s));
+  s.invalidateHost();
+  if (accepted) {
+    ec.set(s, e);
+    st.energy = e;
+  } else {
+    ec.valid = false;  // the reference's unevaluated fallback point
+  }
+  if (acceptedOut) *acceptedOut = accepted;
+  return tau;
+}
+
+void gradient_into(State& s, void* g, MinimiseStats& st) {
+  s.pot->evaluate(s.precision(), s.deviceCoords(), nullptr, 0.0, nullptr, g, s.stream());
+  st.nGradient++;
+}
+
+double dot1(int prec, const void* a, const void* b, size_t n, hipStream_t st) {
+  auto& ws = Workspace::current();
+  const void* pa[1] = {a};
+  const void* pb[1] = {b};
+  minim_amd::launch_dots(prec, pa, pb, 1, n, ws.rs(), st);
+  double r;
+  ws.fetch(st, 1, &r);
+  return r;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ L-BFGS
+struct Lbfgs::Impl {
+  int prec = FP64;
+  size_t n = 0;
+  int m = 5;
+  DVec g, gnew;
+  std::vector<DVec> D, Y;  // m+1 slots: directions d (s = tau*d) and y
+  struct Pair {
+    int slot;
+    double tau;
+    double rho;
+  };
+  std::deque<Pair> pairs;  // oldest .. newest
+  std::vector<char> used;
+  // Gram data indexed by slot
+  double gg = 0;
+  std::vector<double> dg, yg;                  // d_a.g, y_a.g (current g)
+  std::vector<std::vector<double>> dy, yy;     // d_a.y_b, y_a.y_b
+  EnergyCache ec;
+  bool fresh = true;  // next iteration starts from a new gradient
+  int stall = 0;
+
+  void alloc(int prec_, size_t n_, int m_) {
+    prec = prec_;
+    n = n_;
+    m = m_;
+    g = DVec(n, prec);
+    gnew = DVec(n, prec);
+    D.clear();
+    Y.clear();
+    for (int i = 0; i <= m; i++) {
+      D.emplace_back(n, prec);
+      Y.emplace_back(n, prec);
+    }
+    used.assign(m + 1, 0);
+    dg.assign(m + 1, 0.0);
+    yg.assign(m + 1, 0.0);
+    dy.assign(m + 1, std::vector<double>(m + 1, 0.0));
+    yy.assign(m + 1, std::vector<double>(m + 1, 0.0));
+    pairs.clear();
+    fresh = true;
+    ec.valid = false;
+  }
+
+  int spare() const {
+    for (int i = 0; i <= m; i++)
+      if (!used[i]) return i;
+    throw std::logic_error("Lbfgs: no free history slot");
+  }
+
+  // Promote every device vector to FP64 in place (exact widening); the Gram
+  // data, accumulated in FP64 from the same values, stays valid.
+  void promote(hipStream_t s) {
+    if (prec == FP64) return;
+    auto widen = [&](DVec& v) {
+      DVec w(n, FP64);
+      minim_amd::launch_convert(FP64, w.data(), FP32, v.data(), n, s);
+      v = std::move(w);
+    };
+    widen(g);
+    gnew = DVec(n, FP64);
+    for (auto& v : D) widen(v);
+    for (auto& v : Y) widen(v);
+    MINIM_HIP_CHECK(hipStreamSynchronize(s));
+    prec = FP64;
+  }
+};
+
+Lbfgs::Lbfgs() : impl_(new Impl) {}
+Lbfgs::Lbfgs(const Lbfgs& o)
+    : NewMinimiser<Lbfgs>(o), _m(o._m), _maxStep(o._maxStep), _initStep(o._initStep), _precMode(o._precMode),
+      _switchRms(o._switchRms), impl_(new Impl) {}
+Lbfgs& Lbfgs::operator=(const Lbfgs& o) {
+  Minimiser::operator=(o);
+  _m = o._m;
+  _maxStep = o._maxStep;
+  _initStep = o._initStep;
+  _precMode = o._precMode;
+  _switchRms = o._switchRms;
+  impl_.reset(new Impl);
+  return *this;
+}
+Lbfgs::~Lbfgs() = default;
+
+Lbfgs& Lbfgs::setM(int m) {
+  if (m < 1 || m > minim_amd::kMaxM) throw std::invalid_argument("Lbfgs: m must be between 1 and 15.");
+  _m = m;
+  return *this;
+}
+Lbfgs& Lbfgs::setMaxIter(int mi) {
+  Minimiser::setMaxIter(mi);
+  return *this;
+}
+Lbfgs& Lbfgs::setMaxStep(double v) {
+  _maxStep = v;
+  return *this;
+}
+Lbfgs& Lbfgs::setInitStep(double v) {
+  _initStep = v;
+  return *this;
+}
+Lbfgs& Lbfgs::setPrecision(std::string mode, double switchRms) {
+  if (mode != "state" && mode != "fp64" && mode != "fp32" && mode != "mixed")
+    throw std::invalid_argument("Lbfgs: precision must be state, fp64, fp32 or mixed.");
+  _precMode = mode;
+  _switchRms = switchRms;
+  return *this;
+}
+
+// Lbfgs::init (Lbfgs.cpp:34-38): empty history.
+void Lbfgs::init(State& state) {
+  if (_precMode == "fp64" || _precMode == "mixed") state.setPrecision(_precMode == "mixed" ? FP32 : FP64);
+  if (_precMode == "fp32") state.setPrecision(FP32);
+  impl_->alloc(state.precision(), state.ndof, _m);
+}
+
+// Lbfgs::iteration (Lbfgs.cpp:41-81) with getDirection (:84-127).
+void Lbfgs::iteration(State& state) {
+  Impl& I = *impl_;
+  hipStream_t s = hs(state);
+  auto& ws = Workspace::current();
+  const int prec = I.prec;
+  const size_t n = I.n;
+
+  if (I.fresh) {
+    gradient_into(state, I.g.data(), stats);
+    I.gg = dot1(prec, I.g.data(), I.g.data(), n, s);
+    if (iter == 0) I.pairs.clear();
+    // new gradient: refresh its dots with every stored pair
+    if (!I.pairs.empty()) {
+      minim_amd::DirArgs a{};
+      a.basis[0] = I.g.data();
+      a.coef[0] = 1.0;
+      a.nb = 1;
+      int k = 0;
+      for (auto& p : I.pairs) a.dotv[k++] = I.D[p.slot].data();
+      for (auto& p : I.pairs) a.dotv[k++] = I.Y[p.slot].data();
+      a.ndot = k;
+      a.out = I.gnew.data();
+      a.n = n;
+      a.proj = minim_amd::SiteProj{1, 0, 0, 1, 1.0};
+      minim_amd::launch_direction(a, prec, ws.rs(), s);
+      std::vector<double> r(1 + k);
+      ws.fetch(s, 1 + k, r.data());
+      const int P = (int)I.pairs.size();
+      for (int i = 0; i < P; i++) {
+        I.dg[I.pairs[i].slot] = r[1 + i];
+        I.yg[I.pairs[i].slot] = r[1 + P + i];
+      }
+    }
+    I.fresh = false;
+  }
+
+  // ---- direction: coefficients from the two-loop recursion on the Gram matrix
+  const int P = (int)I.pairs.size();
+  const int dslot = I.spare();
+  std::vector<int> oldSlots(P);
+  for (int i = 0; i < P; i++) oldSlots[i] = I.pairs[i].slot;
+  minim_amd::DirArgs a{};
+  a.out = I.D[dslot].data();
+  a.n = n;
+  {
+    SiteProjection sp = state.pot->projection();
+    a.proj = minim_amd::SiteProj{sp.nf, sp.fixmask, sp.hard, sp.nvar, sp.inv_nvar};
+  }
+  a.dotv[0] = I.g.data();
+  for (int i = 0; i < P; i++) a.dotv[1 + i] = I.Y[I.pairs[i].slot].data();
+  a.ndot = 1 + P;
+  a.basis[0] = I.g.data();
+  a.nb = 1;
+  if (P == 0) {
+    // first step: -g * initStep/|g| (Lbfgs.cpp:91-96)
+    double gnorm = std::sqrt(I.gg);
+    a.coef[0] = (gnorm > 0) ? -(_initStep / gnorm) : -1.0;
+  } else {
+    std::vector<double> alpha(P), ag_s(P, 0.0), ay(P, 0.0);
+    double ag = -1.0;  // q = -g
+    auto sdotg = [&](int i) { return I.pairs[i].tau * I.dg[I.pairs[i].slot]; };
+    auto sdoty = [&](int i, int j) { return I.pairs[i].tau * I.dy[I.pairs[i].slot][I.pairs[j].slot]; };
+    auto ydoty = [&](int i, int j) { return I.yy[I.pairs[i].slot][I.pairs[j].slot]; };
+    for (int k = P - 1; k >= 0; k--) {  // newest -> oldest
+      double sq = ag * sdotg(k);
+      for (int q = 0; q < P; q++) sq += ay[q] * sdoty(k, q);
+      alpha[k] = I.pairs[k].rho * sq;
+      ay[k] -= alpha[k];
+    }
+    const int nw = P - 1;
+    double gamma = 1 / (I.pairs[nw].rho * ydoty(nw, nw));
+    ag *= gamma;
+    for (int q = 0; q < P; q++) ay[q] *= gamma;
+    for (int k = 0; k < P; k++) {  // oldest -> newest
+      double yr = ag * I.yg[I.pairs[k].slot];
+      for (int q = 0; q < P; q++) yr += ag_s[q] * sdoty(q, k) + ay[q] * ydoty(k, q);
+      double beta = I.pairs[k].rho * yr;
+      ag_s[k] += alpha[k] - beta;
+    }
+    a.coef[0] = ag;
+    for (int q = 0; q < P; q++) {
+      a.basis[a.nb] = I.D[I.pairs[q].slot].data();
+      a.coef[a.nb++] = ag_s[q] * I.pairs[q].tau;
+    }
+    for (int q = 0; q < P; q++) {
+      a.basis[a.nb] = I.Y[I.pairs[q].slot].data();
+      a.coef[a.nb++] = ay[q];
+    }
+  }
+  minim_amd::launch_direction(a, prec, ws.rs(), s);
+  std::vector<double> dres(1 + a.ndot);
+  ws.fetch(s, 1 + a.ndot, dres.data());
+  const double dd = dres[0], dgv = dres[1];
+
+  double tau0 = 1.0;
+  if (P > 0 && _maxStep != 0) {  // Lbfgs.cpp:121-124
+    double ss = std::sqrt(dd);
+    if (ss > _maxStep) tau0 = _maxStep / ss;
+  }
+  double gs = tau0 * dgv;  // Lbfgs.cpp:53-57
+  if (gs > 0) {
+    gs = -gs;
+    tau0 = -tau0;
+  }
+
+  double tau;
+  bool lsFailed = false;
+  if (linesearch == "backtracking") {
+    bool acc = true;
+    tau = backtrack(state, a.out, tau0, gs, I.ec, stats, &acc);
+    lsFailed = !acc;
+  } else {
+    minim_amd::launch_axpy(prec, state.deviceCoords(), a.out, tau0, n, s);
+    state.invalidateHost();
+    I.ec.valid = false;
+    tau = tau0;
+  }
+
+  // ---- new gradient and Gram update (Lbfgs.cpp:67-80)
+  gradient_into(state, I.gnew.data(), stats);
+  minim_amd::GramArgs ga{};
+  ga.gnew = I.gnew.data();
+  ga.gold = I.g.data();
+  ga.y = I.Y[dslot].data();
+  ga.n = n;
+  for (int i = 0; i < P; i++) ga.D[i] = I.D[I.pairs[i].slot].data();
+  ga.D[P] = I.D[dslot].data();
+  ga.nd = P + 1;
+  for (int i = 0; i < P; i++) ga.Y[i] = I.Y[I.pairs[i].slot].data();
+  ga.ny = P;
+  minim_amd::launch_gram(ga, prec, ws.rs(), s);
+  const int KB = minim_amd::gram_bucket(ga.nd > ga.ny ? ga.nd : ga.ny);
+  std::vector<double> r(4 * KB + 3);
+  ws.fetch(s, 4 * KB + 3, r.data());
+  auto gD = [&](int j) { return r[j]; };
+  auto gY = [&](int j) { return r[KB + j]; };
+  const double gng = r[2 * KB], gny = r[2 * KB + 1];
+  auto yD = [&](int j) { return r[2 * KB + 2 + j]; };
+  auto yY = [&](int j) { return r[3 * KB + 2 + j]; };
+  const double yyv = r[4 * KB + 2];
+
+  const double sy = tau * yD(P);
+  if (sy != 0) {
+    // store the pair (slot dslot); d_new.y_old comes from the direction pass
+    for (int i = 0; i < P; i++) {
+      const int sl = I.pairs[i].slot;
+      I.dy[dslot][sl] = dres[2 + i];
+      I.dy[sl][dslot] = yD(i);
+      I.yy[dslot][sl] = yY(i);
+      I.yy[sl][dslot] = yY(i);
+    }
+    I.dy[dslot][dslot] = yD(P);
+    I.yy[dslot][dslot] = yyv;
+    I.dg[dslot] = gD(P);
+    I.yg[dslot] = gny;
+    I.used[dslot] = 1;
+    I.pairs.push_back({dslot, tau, 1 / sy});
+    if ((int)I.pairs.size() > I.m) {
+      I.used[I.pairs.front().slot] = 0;
+      I.pairs.pop_front();
+    }
+  }
+  // dots of the old pairs with the new gradient
+  for (int i = 0; i < P; i++) {
+    I.dg[oldSlots[i]] = gD(i);
+    I.yg[oldSlots[i]] = gY(i);
+  }
+  I.gg = gng;
+  std::swap(I.g, I.gnew);
+
+  // ---- mixed precision: leave FP32 storage once it has done its job
+  if (I.prec == FP32 && _precMode == "mixed") {
+    double rms = std::sqrt(I.gg / n);
+    double sw = _switchRms > 0 ? _switchRms : 1e3 * state.convergence;
+    if (lsFailed) I.stall++;
+    else I.stall = 0;
+    if (rms < sw || I.stall >= 2) {
+      I.promote(s);
+      state.setPrecision(FP64);
+      I.ec.valid = false;
+      I.fresh = true;  // recompute g in FP64 at the same point; history kept
+      stats.iterFP32++;
+      return;
+    }
+  }
+  if (I.prec == FP32) stats.iterFP32++;
+  else stats.iterFP64++;
+}
+
+// Lbfgs::checkConvergence (Lbfgs.cpp:130-133)
+bool Lbfgs::checkConvergence(const State& state) {
+  const Impl& I = *impl_;
+  double rms = std::sqrt(I.gg / state.ndof);
+  stats.gradRms = rms;
+  if (I.fresh) return false;  // a precision switch is pending
+  return rms < state.convergence;
+}
+
+void Lbfgs::finish(State& state) {
+  if (!impl_->ec.ok(state)) {
+    double e = 0;
+    state.pot->evaluate(state.precision(), state.deviceCoords(), nullptr, 0.0, &e, nullptr, state.stream());
+    stats.energy = e;
+  }
+}
+
+// ------------------------------------------------------------------ FIRE
+struct Fire::Impl {
+  int prec = FP64;
+  size_t n = 0;
+  DVec g, v, step;
+  double gNorm = 0, a = 0.1, dt = 0;
+  int nSteps = 0;
+  EnergyCache ec;
+};
+
+Fire::Fire() : impl_(new Impl) {}
+Fire::Fire(double dtMax_) : dtMax(dtMax_), impl_(new Impl) { linesearch = "none"; }  // Fire.cpp:11-13
+Fire::Fire(const Fire& o)
+    : NewMinimiser<Fire>(o), dtMax(o.dtMax), _nMin(o._nMin), _fInc(o._fInc), _fDec(o._fDec), _fA(o._fA),
+      _aStart(o._aStart), impl_(new Impl) {}
+Fire& Fire::operator=(const Fire& o) {
+  Minimiser::operator=(o);
+  dtMax = o.dtMax;
+  impl_.reset(new Impl);
+  return *this;
+}
+Fire::~Fire() = default;
+Fire& Fire::setMaxIter(int mi) {
+  Minimiser::setMaxIter(mi);
+  return *this;
+}
+Fire& Fire::setDtMax(double d) {
+  dtMax = d;
+  return *this;
+}
+
+// Fire::init (Fire.cpp:27-38)
+void Fire::init(State& state) {
+  Impl& I = *impl_;
+  I.prec = state.precision();
+  I.n = state.ndof;
+  I.g = DVec(I.n, I.prec);
+  I.v = DVec(I.n, I.prec);
+  I.step = DVec(I.n, I.prec);
+  MINIM_HIP_CHECK(hipMemsetAsync(I.v.data(), 0, I.n * minim_amd::prec_bytes(I.prec), hs(state)));
+  I.ec.valid = false;
+  I.a = _aStart;
+  I.nSteps = 0;
+  if (dtMax != 0) return;
+  gradient_into(state, I.g.data(), stats);
+  I.gNorm = std::sqrt(dot1(I.prec, I.g.data(), I.g.data(), I.n, hs(state)));
+  if (I.gNorm != 0) {
+    dtMax = 0.1 / std::sqrt(I.gNorm);
+  } else {
+    std::printf("Warning: Unable to estimate dtMax parameter for FIRE.\n");
+    dtMax = 1;
+  }
+}
+
+// Fire::iteration (Fire.cpp:41-84)
+void Fire::iteration(State& state) {
+  Impl& I = *impl_;
+  hipStream_t s = hs(state);
+  auto& ws = Workspace::current();
+  if (iter == 0) {
+    I.dt = dtMax;
+    gradient_into(state, I.g.data(), stats);
+    I.gNorm = std::sqrt(dot1(I.prec, I.g.data(), I.g.data(), I.n, s));
+  }
+  const void* pa[2] = {I.v.data(), I.v.data()};
+  const void* pb[2] = {I.g.data(), I.v.data()};
+  minim_amd::launch_dots(I.prec, pa, pb, 2, I.n, ws.rs(), s);
+  double r[2];
+  ws.fetch(s, 2, r);
+  const double p = -r[0];
+  // velocity update with the current a and dt (Fire.cpp:50-57) ...
+  int mode;
+  double f1 = 0, f2;
+  if (p > 0) {
+    double vNorm = std::sqrt(r[1]);
+    mode = 1;
+    f1 = 1 - I.a;
+    f2 = I.a * vNorm / I.gNorm + I.dt;
+    I.nSteps++;
+  } else {
+    mode = 0;
+    f2 = -I.dt;
+    I.nSteps = 0;
+  }
+  // ... then the dt / a adaptation (:60-67), and step = dt*v with the new dt (:70)
+  if ((p > 0) && (I.nSteps > _nMin)) {
+    I.a = I.a * _fA;
+    I.dt = I.dt * _fInc;
+    if (I.dt > dtMax) I.dt = dtMax;
+  } else if (p <= 0) {
+    I.a = _aStart;
+    I.dt = I.dt * _fDec;
+  }
+  minim_amd::launch_fire_update(I.prec, mode, I.v.data(), I.g.data(), I.step.data(), f1, f2, I.dt, I.n, s);
+  state.pot->applyConstraints(I.prec, I.step.data(), I.n, state.stream());
+  if (linesearch == "backtracking") {
+    double gs = dot1(I.prec, I.g.data(), I.step.data(), I.n, s);
+    backtrack(state, I.step.data(), 1.0, gs, I.ec, stats);
+  } else {
+    minim_amd::launch_add(I.prec, state.deviceCoords(), I.step.data(), I.n, s);
+    state.invalidateHost();
+    I.ec.valid = false;
+  }
+  gradient_into(state, I.g.data(), stats);
+  I.gNorm = std::sqrt(dot1(I.prec, I.g.data(), I.g.data(), I.n, s));
+  if (I.prec == FP32) stats.iterFP32++;
+  else stats.iterFP64++;
+}
+
+// Fire::checkConvergence (Fire.cpp:87-90)
+bool Fire::checkConvergence(const State& state) {
+  double rms = impl_->gNorm / std::sqrt((double)state.ndof);
+  stats.gradRms = rms;
+  return rms < state.convergence;
+}
+
+// ------------------------------------------------------------------ GradDescent
+struct GradDescent::Impl {
+  int prec = FP64;
+  size_t n = 0;
+  DVec g, step;
+  double gg = 0;
+  EnergyCache ec;
+};
+
+GradDescent::GradDescent() : impl_(new Impl) {}
+GradDescent::GradDescent(const GradDescent& o) : NewMinimiser<GradDescent>(o), _alpha(o._alpha), impl_(new Impl) {}
+GradDescent& GradDescent::operator=(const GradDescent& o) {
+  Minimiser::operator=(o);
+  _alpha = o._alpha;
+  impl_.reset(new Impl);
+  return *this;
+}
+GradDescent::~GradDescent() = default;
+GradDescent& GradDescent::setAlpha(double al) {
+  _alpha = al;
+  return *this;
+}
+GradDescent& GradDescent::setMaxIter(int mi) {
+  Minimiser::setMaxIter(mi);
+  return *this;
+}
+
+void GradDescent::init(State& state) {
+  Impl& I = *impl_;
+  I.prec = state.precision();
+  I.n = state.ndof;
+  I.g = DVec(I.n, I.prec);
+  I.step = DVec(I.n, I.prec);
+  I.ec.valid = false;
+}
+
+// GradDescent::iteration (GradDescent.cpp:27-39)
+void GradDescent::iteration(State& state) {
+  Impl& I = *impl_;
+  hipStream_t s = hs(state);
+  gradient_into(state, I.g.data(), stats);
+  // step = -alpha * g (explicit vector, rounded as the reference's)
+  minim_amd::launch_scale_copy(I.prec, I.step.data(), I.g.data(), -_alpha, I.n, s);
+  const void* pa[2] = {I.g.data(), I.g.data()};
+  const void* pb[2] = {I.step.data(), I.g.data()};
+  auto& ws = Workspace::current();
+  minim_amd::launch_dots(I.prec, pa, pb, 2, I.n, ws.rs(), s);
+  double r[2];
+  ws.fetch(s, 2, r);
+  I.gg = r[1];
+  if (linesearch == "backtracking") {
+    backtrack(state, I.step.data(), 1.0, r[0], I.ec, stats);
+  } else {
+    minim_amd::launch_add(I.prec, state.deviceCoords(), I.step.data(), I.n, s);
+    state.invalidateHost();
+    I.ec.valid = false;
+  }
+}
+
+// GradDescent::checkConvergence (GradDescent.cpp:42-46)
+bool GradDescent::checkConvergence(const State& state) {
+  double rms = std::sqrt(impl_->gg / state.ndof);
+  stats.gradRms = rms;
+  return rms < state.convergence;
+}
+
+}  // namespace minim

@@ -1,0 +1,38 @@
+// pf.hpp -- launch interface of the PhaseField stencil kernels (pf.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace minim_amd {
+
+constexpr int kPFMaxFluid = 16;
+enum { kDensityHard = 0, kDensitySoft = 1, kDensityNone = 2, kDensityFixed = 3 };
+
+// Everything a stencil launch needs, passed by value as a kernel argument.
+struct PFArgs {
+  // canonical grid: size-1 dimensions first, Z contiguous
+  int X, Y, Z, YZ;
+  int actX, actY, actZ;
+  int nf;
+  double fb[kPFMaxFluid];  // bulk factor per fluid (kappa/16 v, or 0.5 kappa v)
+  double fg[kPFMaxFluid];  // gradient factor per fluid ((0.25|0.5) kappa' v / res^2)
+  unsigned fixmask;        // bit k set: fluid k fixed (gradient zeroed)
+  int density;             // kDensity*
+  double coef;             // soft-constraint coefficient
+  double inv_nvar;         // 1 / number of variable fluids
+  int nvar;
+  // launch shape
+  int xchunk, nxc, nblocks;
+  int xchunk_override;     // >0 forces the x-march length (autotuning)
+};
+
+// Fills YZ, act*, xchunk, nxc, nblocks from X, Y, Z.
+void pf_args_finalize(PFArgs& a, int target_blocks);
+
+// mode bits: 1 energy, 2 gradient (plus g.g), 4 evaluate at x + t*d.
+// Results: rs.result[0] = energy, rs.result[1] = g.g (device).
+void pf_launch_eval(const PFArgs& a, int prec, int mode, const void* x, const void* d, double t,
+                    void* g, RedScratch rs, hipStream_t s);
+void pf_launch_constrain(const PFArgs& a, int prec, void* v, size_t nsite, hipStream_t s);
+
+}  // namespace minim_amd

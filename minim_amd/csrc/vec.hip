@@ -1,0 +1,331 @@
+// vec.hip -- fused, HBM-streaming vector kernels of the device-resident
+// minimisers (L-BFGS direction and Gram passes, FIRE update, dots, axpy).
+//
+// The reference performs the L-BFGS two-loop recursion (src/minimisers/Lbfgs.cpp:84-127)
+// as ~2m dependent dot/axpy sweeps over full vectors with allocating
+// temporaries (include/utils/vec.hpp operators).  Here the recursion runs on the
+// (2m+1)x(2m+1) Gram matrix of the basis {g, s_i, y_i} on the host, and the
+// vectors are touched by two passes per iteration:
+//   * direction pass: d = sum_j c_j b_j (+ per-site projection), fused with the
+//     dots d.g, d.y_i and d.d that the next Gram update and the descent and
+//     max-step checks need;
+//   * Gram pass: y_new = g_new - g_old fused with every dot of g_new and y_new
+//     against the stored basis.
+// All accumulations are FP64; vectors are FP64 or FP32.
+#include "common.hpp"
+#include "vec.hpp"
+
+namespace minim_amd {
+
+namespace {
+
+// result layout: [0] = d.d, [1+j] = d.dotv[j]
+template <typename T, int NF, int K>
+__global__ __launch_bounds__(kBlock) void direction_kernel(DirArgs a, RedScratch rs) {
+  constexpr int NFM = NF > 0 ? NF : 16;
+  const int nf = NF > 0 ? NF : a.proj.nf;
+  const int NK = NF > 0 ? NF : nf;
+  const size_t nsite = a.n / nf;
+  T* __restrict__ out = static_cast<T*>(a.out);
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) acc[k] = 0.0;
+
+  for (size_t s = (size_t)blockIdx.x * kBlock + threadIdx.x; s < nsite; s += (size_t)gridDim.x * kBlock) {
+    const size_t base = s * nf;
+    double v[NFM];
+#pragma unroll
+    for (int k = 0; k < NK; k++) v[k] = 0.0;
+#pragma unroll
+    for (int j = 0; j < kMaxBasis; j++) {
+      if (j < a.nb) {
+        const T* __restrict__ b = static_cast<const T*>(a.basis[j]);
+        const double cj = a.coef[j];
+#pragma unroll
+        for (int k = 0; k < NK; k++) {
+          const double p = cj * (double)b[base + k];
+          v[k] = v[k] + p;
+        }
+      }
+    }
+    if (a.proj.fixmask || (a.proj.hard && nf > 1)) {
+#pragma unroll
+      for (int k = 0; k < NK; k++)
+        if (a.proj.fixmask & (1u << k)) v[k] = 0.0;
+      if (a.proj.hard && nf > 1 && a.proj.nvar > 0) {
+        double comp = 0.0;
+#pragma unroll
+        for (int k = 0; k < NK; k++)
+          if (!(a.proj.fixmask & (1u << k))) comp += v[k];
+        comp *= a.proj.inv_nvar;
+#pragma unroll
+        for (int k = 0; k < NK; k++)
+          if (!(a.proj.fixmask & (1u << k))) v[k] -= comp;
+      }
+    }
+    T o[NFM];
+#pragma unroll
+    for (int k = 0; k < NK; k++) {
+      o[k] = (T)v[k];
+      out[base + k] = o[k];
+      acc[0] += (double)o[k] * (double)o[k];
+    }
+#pragma unroll
+    for (int j = 0; j < K - 1; j++) {
+      if (j < a.ndot) {
+        const T* __restrict__ w = static_cast<const T*>(a.dotv[j]);
+#pragma unroll
+        for (int k = 0; k < NK; k++) acc[1 + j] += (double)o[k] * (double)w[base + k];
+      }
+    }
+  }
+  grid_sum_store<K>(acc, rs);
+}
+
+// result layout with KB = basis bucket: [0,KB) g.D ; [KB,2KB) g.Y ; 2KB g.g ;
+// 2KB+1 g.y ; [2KB+2,3KB+2) y.D ; [3KB+2,4KB+2) y.Y ; 4KB+2 y.y
+template <typename T, int KB>
+__global__ __launch_bounds__(kBlock) void gram_kernel(GramArgs a, RedScratch rs) {
+  constexpr int K = 4 * KB + 3;
+  const T* __restrict__ gn = static_cast<const T*>(a.gnew);
+  const T* __restrict__ go = static_cast<const T*>(a.gold);
+  T* __restrict__ yo = static_cast<T*>(a.y);
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) acc[k] = 0.0;
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += (size_t)gridDim.x * kBlock) {
+    const T gv = gn[i];
+    const T yv = gv - go[i];
+    yo[i] = yv;
+    const double g = (double)gv, y = (double)yv;
+#pragma unroll
+    for (int j = 0; j < KB; j++) {
+      if (j < a.nd) {
+        const double dv = (double)static_cast<const T*>(a.D[j])[i];
+        acc[j] += g * dv;
+        acc[2 * KB + 2 + j] += y * dv;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < KB; j++) {
+      if (j < a.ny) {
+        const double yy = (double)static_cast<const T*>(a.Y[j])[i];
+        acc[KB + j] += g * yy;
+        acc[3 * KB + 2 + j] += y * yy;
+      }
+    }
+    acc[2 * KB] += g * g;
+    acc[2 * KB + 1] += g * y;
+    acc[4 * KB + 2] += y * y;
+  }
+  grid_sum_store<K>(acc, rs);
+}
+
+struct DotPtrs {
+  const void* a[8];
+  const void* b[8];
+};
+
+template <typename T, int K>
+__global__ __launch_bounds__(kBlock) void dots_kernel_v(DotPtrs p, int k, size_t n, RedScratch rs) {
+  double acc[K];
+#pragma unroll
+  for (int j = 0; j < K; j++) acc[j] = 0.0;
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      if (j >= k) break;
+      acc[j] += (double)static_cast<const T*>(p.a[j])[i] * (double)static_cast<const T*>(p.b[j])[i];
+    }
+  }
+  grid_sum_store<K>(acc, rs);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void axpy_kernel(T* __restrict__ x, const T* __restrict__ d, T t, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+    const T s = t * d[i];
+    x[i] = x[i] + s;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void add_kernel(T* __restrict__ x, const T* __restrict__ st, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+    x[i] = x[i] + st[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void scale_kernel(T* __restrict__ v, T f, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+    v[i] = v[i] * f;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void scale_copy_kernel(T* __restrict__ o, const T* __restrict__ v, T f, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+    o[i] = f * v[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void zero_idx_kernel(T* __restrict__ v, const int* __restrict__ idx, size_t m) {
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < m; i += (size_t)gridDim.x * kBlock)
+    v[idx[i]] = (T)0;
+}
+
+template <typename TD, typename TS>
+__global__ __launch_bounds__(kBlock) void convert_kernel(TD* __restrict__ dst, const TS* __restrict__ src, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+    dst[i] = (TD)src[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void fire_kernel(int mode, T* __restrict__ v, const T* __restrict__ g,
+                                                      T* __restrict__ step, T f1, T f2, T dt, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+    T nv;
+    if (mode == 1) {
+      const T t1 = f1 * v[i];
+      const T t2 = f2 * g[i];
+      nv = t1 - t2;
+    } else {
+      nv = f2 * g[i];
+    }
+    v[i] = nv;
+    step[i] = dt * nv;
+  }
+}
+
+int bucket(int k) {
+  if (k <= 8) return 8;
+  if (k <= 16) return 16;
+  if (k <= 32) return 32;
+  if (k <= 64) return 64;
+  return 96;
+}
+
+template <typename T, int NF>
+void dir_nf(const DirArgs& a, RedScratch rs, hipStream_t s, int nb) {
+  switch (bucket(a.ndot + 1)) {
+    case 8: direction_kernel<T, NF, 8><<<nb, kBlock, 0, s>>>(a, rs); break;
+    case 16: direction_kernel<T, NF, 16><<<nb, kBlock, 0, s>>>(a, rs); break;
+    default: direction_kernel<T, NF, 32><<<nb, kBlock, 0, s>>>(a, rs); break;
+  }
+}
+
+template <typename T>
+void dir_t(const DirArgs& a, RedScratch rs, hipStream_t s) {
+  const int nf = a.proj.nf < 1 ? 1 : a.proj.nf;
+  const int nb = grid_blocks(a.n / nf, 2, 2048);
+  switch (nf) {
+    case 1: dir_nf<T, 1>(a, rs, s, nb); break;
+    case 2: dir_nf<T, 2>(a, rs, s, nb); break;
+    case 3: dir_nf<T, 3>(a, rs, s, nb); break;
+    case 4: dir_nf<T, 4>(a, rs, s, nb); break;
+    default: dir_nf<T, 0>(a, rs, s, nb); break;
+  }
+}
+
+template <typename T>
+void gram_t(const GramArgs& a, RedScratch rs, hipStream_t s) {
+  const int nb = grid_blocks(a.n, 4, 2048);
+  switch (gram_bucket(a.nd > a.ny ? a.nd : a.ny)) {
+    case 6: gram_kernel<T, 6><<<nb, kBlock, 0, s>>>(a, rs); break;
+    case 11: gram_kernel<T, 11><<<nb, kBlock, 0, s>>>(a, rs); break;
+    default: gram_kernel<T, 16><<<nb, kBlock, 0, s>>>(a, rs); break;
+  }
+}
+
+}  // namespace
+
+void launch_direction(const DirArgs& a, int prec, RedScratch rs, hipStream_t s) {
+  if (a.nb > kMaxBasis || a.ndot > kMaxBasis) throw std::invalid_argument("direction: basis too large");
+  if (prec == F64) dir_t<double>(a, rs, s);
+  else dir_t<float>(a, rs, s);
+  MINIM_HIP_CHECK(hipGetLastError());
+}
+
+int gram_bucket(int k) {
+  if (k <= 6) return 6;
+  if (k <= 11) return 11;
+  return 16;
+}
+
+void launch_gram(const GramArgs& a, int prec, RedScratch rs, hipStream_t s) {
+  if (a.nd > 16 || a.ny > 16) throw std::invalid_argument("gram: history too long");
+  if (prec == F64) gram_t<double>(a, rs, s);
+  else gram_t<float>(a, rs, s);
+  MINIM_HIP_CHECK(hipGetLastError());
+}
+
+void launch_dots(int prec, const void* const* a, const void* const* b, int k, size_t n, RedScratch rs,
+                 hipStream_t s) {
+  if (k < 1 || k > 8) throw std::invalid_argument("dots: 1..8 pairs");
+  DotPtrs p{};
+  for (int j = 0; j < k; j++) {
+    p.a[j] = a[j];
+    p.b[j] = b[j];
+  }
+  const int nb = grid_blocks(n, 4, 2048);
+  if (prec == F64) dots_kernel_v<double, 8><<<nb, kBlock, 0, s>>>(p, k, n, rs);
+  else dots_kernel_v<float, 8><<<nb, kBlock, 0, s>>>(p, k, n, rs);
+  MINIM_HIP_CHECK(hipGetLastError());
+}
+
+void launch_axpy(int prec, void* x, const void* d, double t, size_t n, hipStream_t s) {
+  const int nb = grid_blocks(n, 4, 4096);
+  if (prec == F64) axpy_kernel<double><<<nb, kBlock, 0, s>>>((double*)x, (const double*)d, t, n);
+  else axpy_kernel<float><<<nb, kBlock, 0, s>>>((float*)x, (const float*)d, (float)t, n);
+  MINIM_HIP_CHECK(hipGetLastError());
+}
+
+void launch_add(int prec, void* x, const void* st, size_t n, hipStream_t s) {
+  const int nb = grid_blocks(n, 4, 4096);
+  if (prec == F64) add_kernel<double><<<nb, kBlock, 0, s>>>((double*)x, (const double*)st, n);
+  else add_kernel<float><<<nb, kBlock, 0, s>>>((float*)x, (const float*)st, n);
+  MINIM_HIP_CHECK(hipGetLastError());
+}
+
+void launch_scale(int prec, void* v, double f, size_t n, hipStream_t s) {
+  const int nb = grid_blocks(n, 4, 4096);
+  if (prec == F64) scale_kernel<double><<<nb, kBlock, 0, s>>>((double*)v, f, n);
+  else scale_kernel<float><<<nb, kBlock, 0, s>>>((float*)v, (float)f, n);
+  MINIM_HIP_CHECK(hipGetLastError());
+}
+
+void launch_scale_copy(int prec, void* o, const void* v, double f, size_t n, hipStream_t s) {
+  const int nb = grid_blocks(n, 4, 4096);
+  if (prec == F64) scale_copy_kernel<double><<<nb, kBlock, 0, s>>>((double*)o, (const double*)v, f, n);
+  else scale_copy_kernel<float><<<nb, kBlock, 0, s>>>((float*)o, (const float*)v, (float)f, n);
+  MINIM_HIP_CHECK(hipGetLastError());
+}
+
+void launch_zero_idx(int prec, void* v, const int* idx, size_t m, hipStream_t s) {
+  if (m == 0) return;
+  const int nb = grid_blocks(m, 1, 1024);
+  if (prec == F64) zero_idx_kernel<double><<<nb, kBlock, 0, s>>>((double*)v, idx, m);
+  else zero_idx_kernel<float><<<nb, kBlock, 0, s>>>((float*)v, idx, m);
+  MINIM_HIP_CHECK(hipGetLastError());
+}
+
+void launch_convert(int pd, void* dst, int ps, const void* src, size_t n, hipStream_t s) {
+  const int nb = grid_blocks(n, 4, 4096);
+  if (pd == F64 && ps == F32) convert_kernel<double, float><<<nb, kBlock, 0, s>>>((double*)dst, (const float*)src, n);
+  else if (pd == F32 && ps == F64) convert_kernel<float, double><<<nb, kBlock, 0, s>>>((float*)dst, (const double*)src, n);
+  else if (pd == F64) convert_kernel<double, double><<<nb, kBlock, 0, s>>>((double*)dst, (const double*)src, n);
+  else convert_kernel<float, float><<<nb, kBlock, 0, s>>>((float*)dst, (const float*)src, n);
+  MINIM_HIP_CHECK(hipGetLastError());
+}
+
+void launch_fire_update(int prec, int mode, void* v, const void* g, void* step, double f1, double f2,
+                        double dt, size_t n, hipStream_t s) {
+  const int nb = grid_blocks(n, 4, 4096);
+  if (prec == F64)
+    fire_kernel<double><<<nb, kBlock, 0, s>>>(mode, (double*)v, (const double*)g, (double*)step, f1, f2, dt, n);
+  else
+    fire_kernel<float><<<nb, kBlock, 0, s>>>(mode, (float*)v, (const float*)g, (float*)step, (float)f1, (float)f2,
+                                             (float)dt, n);
+  MINIM_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace minim_amd

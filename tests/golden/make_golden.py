@@ -1,0 +1,177 @@
+"""Write the golden fixtures of tests/golden/ from the reference library.
+
+Runs in the build container only: it needs oracle/_ref/libminim_ref.so, the
+reference (sjavis/minim) compiled from its own sources by oracle/Makefile.
+Run single-threaded (OMP_NUM_THREADS=1), the reference's only bit-deterministic
+mode.  Inputs are drawn from numpy's default_rng with the seeds below and are
+stored in the fixtures, so nothing downstream depends on the generator.
+
+    OMP_NUM_THREADS=1 python tests/golden/make_golden.py
+"""
+import json
+import math
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+from oracle_bind import RefState, ref_available  # noqa: E402
+
+
+def pf_cases():
+    rng = np.random.default_rng(20240601)
+    specs = [
+        # grid, nfluid, surface tension, interface size, density, resolution
+        ((6, 5, 4), 1, [1.0], [1.0], "hard", 1.0),
+        ((8, 8, 1), 1, [1.0], [], "hard", 1.0),
+        ((1, 7, 1), 1, [0.7], [1.5], "hard", 1.0),
+        ((2, 3, 5), 1, [1.3], [2.0], "hard", 2.0),
+        ((4, 5, 6), 3, [1.0, 0.8, 1.2], [2.0], "hard", 1.0),
+        ((3, 4, 2), 2, [0.9], [1.2], "hard", 1.0),
+        ((4, 4, 4), 4, [1.0, 0.9, 1.1, 0.8], [1.3], "soft", 1.0),
+        ((2, 3, 2), 3, [1.0, 1.0, 1.0], [], "none", 1.0),
+        ((5, 2, 3), 3, [1.0, 0.8, 1.2], [2.0], "fixed", 0.5),
+    ]
+    out = []
+    for grid, nf, st, iw, dens, res in specs:
+        n = int(np.prod(grid)) * nf
+        if nf == 1:
+            x = rng.uniform(-1.1, 1.1, n)
+        else:
+            u = rng.uniform(0.0, 1.0, (n // nf, nf))
+            x = (u / u.sum(axis=1, keepdims=True)).reshape(-1) + rng.uniform(-0.05, 0.05, n)
+        r = RefState.phasefield(grid, x, nf, res, st, iw, dens)
+        e, g = r.energy(x), r.gradient(x)
+        out.append(dict(grid=list(grid), nfluid=nf, surface_tension=st, interface_size=iw, density=dens,
+                        resolution=res, x=x.tolist(), energy=e, gradient=g.tolist()))
+    return out
+
+
+def lbfgs_cases():
+    rng = np.random.default_rng(777)
+    out = []
+    specs = [
+        ("binary-16x16", (16, 16, 1), 1, [1.0], [1.0], 5, 20000, "backtracking", 0.0, 1.0),
+        ("binary-24x20-m3", (24, 20, 1), 1, [1.0], [2.0], 3, 20000, "backtracking", 0.0, 1.0),
+        ("binary-8x8x8-maxstep", (8, 8, 8), 1, [1.0], [1.5], 5, 20000, "backtracking", 0.3, 1.0),
+        ("ternary-6x6x6", (6, 6, 6), 3, [1.0, 0.8, 1.2], [2.0], 5, 20000, "backtracking", 0.0, 1.0),
+        ("binary-12x12-nols", (12, 12, 1), 1, [1.0], [2.0], 5, 300, "none", 0.0, 0.1),
+    ]
+    specs += [
+        ("binary-32x16-stripes", (32, 16, 1), 1, [1.0], [1.0], 5, 20000, "backtracking", 0.0, 1.0),
+        ("ternary-12x8x4-slabs", (12, 8, 4), 3, [1.0, 0.8, 1.2], [1.0], 5, 20000, "backtracking", 0.0, 1.0),
+    ]
+    for name, grid, nf, st, iw, m, maxit, ls, maxstep, init in specs:
+        n = int(np.prod(grid)) * nf
+        if "stripes" in name:
+            xg = np.arange(grid[0])
+            prof = np.where(np.sin(2 * np.pi * (xg + 0.5) / grid[0]) > 0, 1.0, -1.0)
+            x0 = (np.repeat(prof, grid[1] * grid[2]) + rng.uniform(-0.05, 0.05, n))
+        elif "slabs" in name:
+            lab = (np.arange(grid[0]) * 3 // grid[0])
+            c = np.zeros((grid[0], nf))
+            c[np.arange(grid[0]), lab] = 1.0
+            c = 0.9 * c + 0.1 / 3
+            x0 = np.repeat(c[:, None, :], grid[1] * grid[2], axis=1).reshape(-1, nf)
+            x0 = x0 + rng.uniform(-0.03, 0.03, x0.shape)
+            x0 = (x0 / x0.sum(axis=1, keepdims=True)).reshape(-1)
+        elif nf == 1:
+            x0 = rng.uniform(-0.1, 0.1, n)
+        else:
+            u = rng.uniform(0.0, 1.0, (n // nf, nf))
+            x0 = (u / u.sum(axis=1, keepdims=True)).reshape(-1)
+        r = RefState.phasefield(grid, x0, nf, 1.0, st, iw, "hard")
+        x, res = r.minimise("lbfgs", m=m, max_iter=maxit, param_a=maxstep, param_b=init, linesearch=ls)
+        out.append(dict(name=name, grid=list(grid), nfluid=nf, surface_tension=st, interface_size=iw, m=m,
+                        max_iter=maxit, linesearch=ls, max_step=maxstep, init_step=init, x0=x0.tolist(),
+                        x=x.tolist(), **res))
+    return out
+
+
+def fire_gd_cases():
+    rng = np.random.default_rng(99)
+    out = []
+    grid = (10, 10, 1)
+    x0 = rng.uniform(-0.1, 0.1, 100)
+    for kind, pa, ls, maxit in [("fire", 0.0, "backtracking", 3000), ("fire", 0.5, "none", 3000),
+                                ("graddescent", 0.1, "backtracking", 3000), ("graddescent", 0.05, "none", 3000)]:
+        r = RefState.phasefield(grid, x0, 1, 1.0, [1.0], [2.0], "hard")
+        x, res = r.minimise(kind, max_iter=maxit, param_a=pa, linesearch=ls)
+        out.append(dict(kind=kind, param=pa, linesearch=ls, max_iter=maxit, grid=list(grid), interface_size=[2.0],
+                        x0=x0.tolist(), x=x.tolist(), **res))
+    return out
+
+
+def lj_cases():
+    rng = np.random.default_rng(5)
+    out = []
+    for ndim, n in [(3, 12), (2, 9), (3, 30)]:
+        # jittered lattice: no overlapping particles
+        side = int(math.ceil(n ** (1.0 / ndim)))
+        pts = np.array(np.meshgrid(*[np.arange(side)] * ndim, indexing="ij")).reshape(ndim, -1).T[:n] * 1.15
+        x = (pts + rng.uniform(-0.1, 0.1, pts.shape)).reshape(-1)
+        r = RefState.lj(ndim, x)
+        e, g = r.energy_gradient(x)
+        out.append(dict(ndim=ndim, n=n, x=x.tolist(), energy=e, gradient=g.tolist()))
+    return out
+
+
+def sheet(nx, ny):
+    """Triangulated square lattice: nodes i*ny+j, two triangles per square."""
+    tri = []
+    for i in range(nx - 1):
+        for j in range(ny - 1):
+            a, b, c, d = i * ny + j, (i + 1) * ny + j, (i + 1) * ny + j + 1, i * ny + j + 1
+            tri.append([a, b, c])
+            tri.append([a, c, d])
+    return tri
+
+
+def bah_cases():
+    rng = np.random.default_rng(31)
+    out = []
+    nx, ny = 5, 4
+    tri = sheet(nx, ny)
+    xy = np.array([[i, j, 0.0] for i in range(nx) for j in range(ny)], dtype=float)
+    x0 = (xy + rng.normal(0, [0.05, 0.05, 0.2], xy.shape)).reshape(-1)
+    fixed = [0, 1, 2, 3 * (ny - 1), 3 * (ny - 1) + 1, 3 * (ny - 1) + 2]
+    for name, kb, kh, l0, t0, fx, force in [("rest-from-coords", 1.0, 0.1, None, None, [], None),
+                                           ("explicit-rest", 2.0, 0.3, 1.0, math.pi, fixed, [0.0, 0.0, -0.05])]:
+        r = RefState.bah(tri, x0, kb, kh, l0=l0, t0=t0, fixed_dofs=fx, force3=force)
+        el = r.bah_elements()
+        xs = x0 + rng.normal(0, 0.05, x0.shape)
+        e, g = r.energy_gradient(xs)
+        out.append(dict(name=name, tri=tri, kb=kb, kh=kh, l0=l0, t0=t0, fixed=fx, force=force, x0=x0.tolist(),
+                        x=xs.tolist(), energy=e, gradient=g.tolist(),
+                        bars=el["bars"].tolist(), bar_k=el["bar_k"].tolist(), bar_l0=el["bar_l0"].tolist(),
+                        hinges=el["hinges"].tolist(), hinge_k=el["hinge_k"].tolist(),
+                        hinge_t0=el["hinge_t0"].tolist()))
+    # BarAndHinge_test.cpp:13-56: two-triangle hinge minimised by L-BFGS
+    tri2 = [[0, 1, 2], [1, 3, 2]]
+    xk = [-1, 0, 1, 0, -math.sqrt(2), 0, 0, math.sqrt(2), 0, 1, 0, 1]
+    mins = []
+    for t0 in [1e-6, math.pi / 2, math.pi, 3 * math.pi / 2, 2 * math.pi - 1e-6]:
+        r = RefState.bah(tri2, xk, 100.0, 1.0)
+        r.set_hinge_theta0(0, t0)
+        x, res = r.minimise("lbfgs", m=5, max_iter=100000)
+        mins.append(dict(theta0=t0, x=x.tolist(), **res))
+    out.append(dict(name="two-triangle-hinge", tri=tri2, x0=xk, kb=100.0, kh=1.0, minima=mins))
+    return out
+
+
+def main():
+    if not ref_available():
+        sys.exit("oracle/_ref/libminim_ref.so is not built (make -C oracle)")
+    if os.environ.get("OMP_NUM_THREADS") != "1":
+        sys.exit("run with OMP_NUM_THREADS=1")
+    data = dict(pf=pf_cases(), lbfgs=lbfgs_cases(), fire_gd=fire_gd_cases(), lj=lj_cases(), bah=bah_cases())
+    for k, v in data.items():
+        with open(os.path.join(HERE, "ref_%s.json" % k), "w") as f:
+            json.dump(v, f)
+        print(k, len(v))
+
+
+if __name__ == "__main__":
+    main()

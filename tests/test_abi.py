@@ -1,0 +1,131 @@
+"""CPU checks of the C ABI boundary: the library loads, exports every symbol
+include/minim_hip.h declares, and host-only logic (parameter set-up, error
+mapping) behaves like the reference without touching a GPU."""
+import ctypes as C
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "minim_hip.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void\*|const char\*)\s*\*?\s*(minim_[a-z0-9_]+)\(", txt, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from minim_amd import _capi
+
+    L = _capi.load()
+    names = declared_symbols()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(_capi.SIGNATURES), "ctypes table and header disagree"
+    assert L.minim_abi_version() == 1
+
+
+def _pf_handle(grid, nf, st, iw, density=0, res=1.0, fix=None):
+    from minim_amd import _capi as A
+
+    L = A.load()
+    p = A.PFParams()
+    for i in range(3):
+        p.grid[i] = grid[i]
+    p.n_fluid = nf
+    p.resolution = res
+    sta = np.ascontiguousarray(st, dtype=np.float64)
+    iwa = np.ascontiguousarray(iw if iw else [0.0], dtype=np.float64)
+    p.surface_tension = sta.ctypes.data_as(A.dp)
+    p.n_surface_tension = sta.size
+    p.interface_size = iwa.ctypes.data_as(A.dp)
+    p.n_interface_size = len(iw)
+    p.density_constraint = density
+    p.density_const = 1.0
+    if fix is not None:
+        fx = (C.c_ubyte * nf)(*fix)
+        p.fix_fluid = fx
+    h = C.c_void_p()
+    rc = L.minim_pf_create(C.byref(p), C.byref(h))
+    return rc, h
+
+
+def test_pf_setup_matches_reference_coefficients():
+    """PhaseField::init runs on the host (no GPU needed): kappa, kappaP and the
+    convergence criterion must match the oracle/reference values."""
+    from minim_amd import _capi as A
+    from oracle_bind import OraclePF
+
+    L = A.load()
+    for grid, nf, st, iw, res in [((1, 1, 1), 1, [1.0], [], 1.0), ((5, 1, 1), 1, [1.0], [], 2.0),
+                                  ((4, 4, 4), 3, [1.0, 0.8, 1.2], [2.0], 1.0),
+                                  ((3, 3, 3), 4, [1.0, 0.9, 1.1, 0.8], [1.3], 0.7),
+                                  ((2, 2, 2), 2, [0.6], [1.4], 1.0)]:
+        rc, h = _pf_handle(grid, nf, st, iw, res=res)
+        assert rc == 0
+        n = int(np.prod(grid)) * nf
+        x = np.zeros(n)
+        assert L.minim_potential_init(h, x.ctypes.data_as(A.dp), n) == 0
+        k, kp, conv = np.zeros(nf), np.zeros(nf), C.c_double()
+        assert L.minim_pf_coefficients(h, k.ctypes.data_as(A.dp), kp.ctypes.data_as(A.dp)) == 0
+        assert L.minim_potential_convergence(h, C.byref(conv)) == 0
+        o = OraclePF(grid, nf, res, st, iw)
+        np.testing.assert_array_equal(k, o.kappa)
+        np.testing.assert_array_equal(kp, o.kappaP)
+        assert conv.value == o.convergence
+        L.minim_potential_destroy(h)
+
+
+def test_pf_errors_map_to_invalid_argument():
+    from minim_amd import _capi as A
+
+    L = A.load()
+    # wrong number of surface tensions (PhaseField.cpp:86)
+    rc, h = _pf_handle((2, 2, 2), 3, [1.0, 2.0], [])
+    assert rc == 0
+    x = np.zeros(24)
+    assert L.minim_potential_init(h, x.ctypes.data_as(A.dp), 24) == A.MINIM_EINVAL
+    assert b"surfaceTension" in L.minim_last_error()
+    # coordinates that do not match the grid (PhaseField.cpp:138-140)
+    rc, h = _pf_handle((2, 2, 2), 1, [1.0], [])
+    assert L.minim_potential_init(h, x.ctypes.data_as(A.dp), 7) == A.MINIM_EINVAL
+    assert b"does not match" in L.minim_last_error()
+    # bad density constraint name / code (PhaseField.cpp:802-815)
+    rc, h = _pf_handle((2, 2, 2), 1, [1.0], [], density=7)
+    assert rc == A.MINIM_EINVAL
+    # evaluating before init is a runtime error
+    rc, h = _pf_handle((2, 2, 2), 1, [1.0], [])
+    assert L.minim_potential_eval(h, 0, C.c_void_p(1), 8, None, 0.0, None, None, None) == A.MINIM_ERUNTIME
+
+
+def test_python_mirror_setters():
+    import minim_amd as m
+
+    pf = m.PhaseField().setNFluid(3).setGridSize([2, 3, 4]).setSurfaceTension([1, 0.8, 1.2]).setInterfaceSize(2.0)
+    assert pf.interfaceSize == [2.0, 2.0, 2.0]  # scalar setter sized by the current nFluid
+    k, kp, conv = pf.coefficients()
+    np.testing.assert_allclose(k, [0.9, 2.1, 1.5], rtol=1e-15)
+    np.testing.assert_allclose(kp, [3.6, 8.4, 6.0], rtol=1e-15)
+    assert conv == pytest.approx(1e-8)
+    with pytest.raises(ValueError):
+        m.PhaseField().setDensityConstraint("bogus")
+    with pytest.raises(ValueError):
+        m.Lbfgs().setLinesearch("bogus")
+    lb = m.Lbfgs().setMaxIter(10)
+    assert lb.maxIter == 10  # Lbfgs_test.cpp:7-10
+    with pytest.raises(ValueError):
+        m.Lbfgs().setM(99)._handle()
+
+
+def test_triangulation_lists_match_reference_order():
+    from minim_amd import triangulation_lists
+    from conftest import golden
+
+    c = golden("bah")[0]
+    bonds, hinges = triangulation_lists(c["tri"])
+    np.testing.assert_array_equal(bonds, c["bars"])
+    np.testing.assert_array_equal(hinges, c["hinges"])

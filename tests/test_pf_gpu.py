@@ -1,0 +1,165 @@
+"""PhaseField stencil on the GPU vs the oracle and the reference's golden vectors.
+
+Tolerances (from the task's correctness bar): FP64 path energy and gradient
+within 1e-12 relative (gradient norm-wise), FP32 path within 1e-5 relative,
+evaluated on identical coordinates (for FP32: the FP32-rounded ones).
+"""
+import numpy as np
+import pytest
+
+from conftest import golden
+from oracle_bind import OraclePF
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp64": 1e-12, "fp32": 1e-5}
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def make_pf(m, grid, nf, st, iw, dens="hard", res=1.0):
+    pf = m.PhaseField().setNFluid(nf).setGridSize(list(grid)).setResolution(res).setSurfaceTension(st)
+    if iw:
+        pf.setInterfaceSize(iw if len(iw) > 1 else iw[0])
+    pf.setDensityConstraint(dens)
+    return pf
+
+
+def check(m, grid, nf, st, iw, dens, x, prec, res=1.0):
+    print("case", grid, nf, dens, prec, flush=True)
+    pf = make_pf(m, grid, nf, st, iw, dens, res)
+    xs = x.astype(np.float32).astype(np.float64) if prec == "fp32" else x
+    s = m.State(pf, xs, prec)
+    o = OraclePF(tuple(grid), nf, res, st, iw, dens)
+    e, g = s.energyGradient()
+    eo, go = o.energy(xs), o.gradient(xs)
+    assert abs(e - eo) <= TOL[prec] * max(abs(eo), 1e-300), (e, eo)
+    assert rel_err(g, go) <= TOL[prec]
+    assert abs(s.energy() - eo) <= TOL[prec] * abs(eo)
+    assert rel_err(s.gradient(), go) <= TOL[prec]
+    return s
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("case", range(9))
+def test_golden_vectors(gpu, case, prec):
+    c = golden("pf")[case]
+    x = np.array(c["x"])
+    s = check(gpu, c["grid"], c["nfluid"], c["surface_tension"], c["interface_size"], c["density"], x, prec,
+              c["resolution"])
+    if prec == "fp64":
+        assert s.energy() == pytest.approx(c["energy"], rel=1e-12)
+        assert rel_err(s.gradient(), c["gradient"]) <= 1e-12
+
+
+def test_known_answers(gpu):
+    m = gpu
+    # PhaseField_test.cpp:35-42
+    s = m.State(make_pf(m, (1, 1, 1), 1, [1.0], []), [1.0])
+    assert s.energy() == 0 and s.gradient()[0] == 0
+    s.coords([0.1])
+    assert s.energy() == pytest.approx(3.0 / 16 * 1.21 * 0.81, rel=1e-15)
+    assert s.gradient()[0] == pytest.approx(-3.0 / 4 * 0.099, rel=1e-15)
+    # 3-site periodic chain, hand-derived (see test_oracle)
+    for grid in [(3, 1, 1), (1, 3, 1), (1, 1, 3)]:
+        s = m.State(make_pf(m, grid, 1, [1.0], []), [-1, 0, 1])
+        assert s.energy() == pytest.approx(3.0 / 16 + 4.5, rel=1e-15)
+        np.testing.assert_allclose(s.gradient(), [-4.5, 0, 4.5], atol=1e-15)
+    # PhaseField_test.cpp:182-200: fixed fluid -> zero gradient
+    pf = make_pf(m, (2, 2, 1), 3, [1.0], [], "none").setFixFluid(0)
+    g = m.State(pf, np.ones(12)).gradient()
+    assert np.all(g[0::3] == 0)
+
+
+SHAPES = [
+    ((17, 13, 11), 1, [1.0], [1.0], "hard"),
+    ((64, 64, 1), 1, [1.0], [1.0], "hard"),
+    ((1, 1, 300), 1, [0.8], [2.0], "hard"),
+    ((2, 9, 2), 1, [1.0], [1.5], "hard"),
+    ((32, 32, 32), 3, [1.0, 0.8, 1.2], [2.0], "hard"),
+    ((9, 300, 2), 3, [1.0, 0.8, 1.2], [2.0], "hard"),
+    ((12, 10, 8), 2, [1.0], [1.0], "hard"),
+    ((8, 8, 8), 4, [1.0, 0.9, 1.1, 0.8, 1.2, 0.7], [1.3], "soft"),
+    ((6, 7, 5), 5, [1.0] * 5, [1.0], "none"),
+    ((5, 6, 7), 3, [1.0, 0.8, 1.2], [2.0], "fixed"),
+]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("shape", range(len(SHAPES)))
+def test_random_shapes(gpu, shape, prec):
+    grid, nf, st, iw, dens = SHAPES[shape]
+    rng = np.random.default_rng(100 + shape)
+    n = int(np.prod(grid)) * nf
+    x = rng.uniform(-1.1, 1.1, n) if nf == 1 else rng.uniform(-0.1, 1.1, n)
+    if nf == 4:
+        st = [1.0, 0.9, 1.1, 0.8]
+    check(gpu, grid, nf, st, iw, dens, x, prec)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_trial_point_and_constraints(gpu, prec):
+    """Energy at x + t*d (fused) equals the energy at the materialised point,
+    and the projection kernel equals the oracle's specialisedConstraints."""
+    import torch
+
+    m = gpu
+    grid, nf = (20, 18, 16), 3
+    rng = np.random.default_rng(7)
+    n = int(np.prod(grid)) * nf
+    x0, d0, v0 = rng.uniform(0, 1, n), rng.normal(0, 0.1, n), rng.normal(0, 1, n)
+    pf = make_pf(m, grid, nf, [1.0, 0.8, 1.2], [2.0])
+    h = pf.handle(x0)
+    dt = torch.float64 if prec == "fp64" else torch.float32
+    x, d, v = (torch.tensor(a, dtype=dt, device="cuda") for a in (x0, d0, v0))
+    t = -0.375
+    e1 = m.potential_eval(h, prec, x.data_ptr(), n, d_ptr=d.data_ptr(), t=t)
+    xt = x + torch.tensor(t, dtype=dt, device="cuda") * d
+    torch.cuda.synchronize()
+    e2 = m.potential_eval(h, prec, xt.data_ptr(), n)
+    assert e1 == e2
+    m.potential_constrain(h, prec, v.data_ptr(), n)
+    torch.cuda.synchronize()
+    o = OraclePF(grid, nf, 1.0, [1.0, 0.8, 1.2], [2.0])
+    vin = v0.astype(np.float32).astype(np.float64) if prec == "fp32" else v0
+    assert rel_err(v.double().cpu().numpy(), o.constrain(vin)) <= TOL[prec]
+    # deterministic: repeated evaluation is bitwise identical
+    e3 = m.potential_eval(h, prec, x.data_ptr(), n, d_ptr=d.data_ptr(), t=t)
+    assert e3 == e1
+
+
+def test_full_size_256cubed_ternary(gpu):
+    """The bench workload (256^3 sites x 3 fluids, 50.3M DOF) at full size:
+    FP64 energy and gradient vs the oracle, FP32 vs FP64, and the hard
+    density invariant."""
+    import torch
+
+    m = gpu
+    grid, nf = (256, 256, 256), 3
+    n = 256 ** 3 * 3
+    rng = np.random.default_rng(2024)
+    u = rng.uniform(0, 1, (n // 3, 3))
+    x = (u / u.sum(axis=1, keepdims=True)).reshape(-1)
+    o = OraclePF(grid, nf, 1.0, [1.0, 0.8, 1.2], [2.0])
+    print("oracle 256^3 ...", flush=True)
+    eo = o.energy(x)
+    go = o.gradient(x)
+    pf = make_pf(m, grid, nf, [1.0, 0.8, 1.2], [2.0])
+    h = pf.handle(x)
+    xd = torch.from_numpy(x).cuda()
+    g = torch.empty_like(xd)
+    e = m.potential_eval(h, "fp64", xd.data_ptr(), n, g_ptr=g.data_ptr())
+    torch.cuda.synchronize()
+    assert abs(e - eo) <= 1e-12 * abs(eo)
+    gd = g.cpu().numpy()
+    assert rel_err(gd, go) <= 1e-12
+    assert np.abs(gd.reshape(-1, 3).sum(axis=1)).max() < 1e-12
+    x32 = xd.float()
+    g32 = torch.empty_like(x32)
+    e32 = m.potential_eval(h, "fp32", x32.data_ptr(), n, g_ptr=g32.data_ptr())
+    torch.cuda.synchronize()
+    assert abs(e32 - eo) <= 1e-5 * abs(eo)
+    assert rel_err(g32.double().cpu().numpy(), go) <= 1e-5
