@@ -66,23 +66,31 @@ int minim_lj_create(int ndim, double sigma, double epsilon, minim_potential** ou
  * has moved more than skin/2). */
 int minim_lj_set_cutoff(minim_potential* pot, double rc, const double box[3], double skin);
 
-/* BarAndHinge with explicit element lists (BarAndHinge::setBondList /
- * setHingeList / setRigidity / setLength0 / setTheta0 / setForce,
- * include/potentials/BarAndHinge.h:17-43).  bars: 2 node ids per bar;
- * hinges: 4 node ids per hinge in the reference's order (BarAndHinge.cpp:119-131).
- * k/l0/theta0 arrays of length n_bar / n_hinge; NaN entries of bar_l0 or
- * hinge_theta0 are computed from the initial coordinates (computeLength0 /
- * computeTheta0, BarAndHinge.cpp:213-248). force: NULL, a 3-vector, or 3 per node. */
+/* BarAndHinge (BarAndHinge::setTriangulation / setBondList / setHingeList /
+ * setRigidity / setLength0 / setTheta0 / setForce,
+ * include/potentials/BarAndHinge.h:17-43).  Either a triangulation
+ * (3 node ids per triangle; bars and hinges are derived in the reference's
+ * order, BarAndHinge.cpp:70-147) or explicit lists (n_bar > 0 or n_hinge > 0:
+ * 2 node ids per bar, 4 per hinge).  Per-element arrays may be NULL, in which
+ * case the scalars apply; a NaN l0 / theta0 means "from the initial
+ * coordinates" (computeLength0 / computeTheta0, BarAndHinge.cpp:213-248).
+ * force: NULL, a 3-vector, or 3 values per node (force_per_node). */
 typedef struct minim_bah_params {
   int n_node;
+  const int* triangles;
+  int n_triangle;
   int n_bar;
   const int* bars;
-  const double* bar_k;
-  const double* bar_l0;
   int n_hinge;
   const int* hinges;
+  const double* bar_k;
   const double* hinge_k;
+  double k_bar;
+  double k_hinge;
+  const double* bar_l0;
   const double* hinge_theta0;
+  double l0;
+  double theta0;
   const double* force;
   int force_per_node;
 } minim_bah_params;

@@ -26,8 +26,10 @@ class PFParams(C.Structure):
 
 
 class BaHParams(C.Structure):
-    _fields_ = [("n_node", C.c_int), ("n_bar", C.c_int), ("bars", ip), ("bar_k", dp), ("bar_l0", dp),
-                ("n_hinge", C.c_int), ("hinges", ip), ("hinge_k", dp), ("hinge_theta0", dp),
+    _fields_ = [("n_node", C.c_int), ("triangles", ip), ("n_triangle", C.c_int),
+                ("n_bar", C.c_int), ("bars", ip), ("n_hinge", C.c_int), ("hinges", ip),
+                ("bar_k", dp), ("hinge_k", dp), ("k_bar", C.c_double), ("k_hinge", C.c_double),
+                ("bar_l0", dp), ("hinge_theta0", dp), ("l0", C.c_double), ("theta0", C.c_double),
                 ("force", dp), ("force_per_node", C.c_int)]
 
 

@@ -254,50 +254,58 @@ class BarAndHinge(Potential):
         self.force = _f64(f)
         return self
 
-    def _lists(self):
-        if self.bonds is not None and self.hingesList is not None:
-            return self.bonds, self.hingesList
-        if self.triangles is None:
-            raise ValueError("BarAndHinge: The triangulation has not been set.")
-        return triangulation_lists(self.triangles)
-
     def _create(self):
         raise RuntimeError("BarAndHinge handles are created by handle()")
 
     def handle(self, coords):
+        """Explicit lists are passed as given; a triangulation is handed to the
+        library, which derives the bars and hinges in the reference's order."""
         L = A.load()
         x = _f64(coords)
         nnode = x.size // 3
-        bonds, hinges = self._lists()
-        nb, nh = len(bonds), len(hinges)
         if self.kBond is None or self.kHinge is None:
             raise ValueError("BarAndHinge requires the rigidities to be set")
-
-        def per(v, n):
-            a = np.full(n, float(v)) if np.isscalar(v) else _f64(v)
-            return a
-
         self._keep = []
-        bb = np.ascontiguousarray(bonds, dtype=np.int32).reshape(-1)
-        hh = np.ascontiguousarray(hinges, dtype=np.int32).reshape(-1)
-        kb, kh = per(self.kBond, nb), per(self.kHinge, nh)
         p = A.BaHParams()
         p.n_node = nnode
-        p.n_bar = nb
-        p.bars = bb.ctypes.data_as(A.ip)
-        p.bar_k = _dptr(kb)
-        p.n_hinge = nh
-        p.hinges = hh.ctypes.data_as(A.ip)
-        p.hinge_k = _dptr(kh)
-        self._keep += [bb, hh, kb, kh]
+        p.l0 = math.nan
+        p.theta0 = math.nan
+        explicit = self.bonds is not None and self.hingesList is not None
+        if explicit:
+            bb = np.ascontiguousarray(self.bonds, dtype=np.int32).reshape(-1)
+            hh = np.ascontiguousarray(self.hingesList, dtype=np.int32).reshape(-1)
+            p.n_bar, p.bars = bb.size // 2, bb.ctypes.data_as(A.ip)
+            p.n_hinge, p.hinges = hh.size // 4, hh.ctypes.data_as(A.ip)
+            self._keep += [bb, hh]
+        else:
+            if self.triangles is None:
+                raise ValueError("BarAndHinge: The triangulation has not been set.")
+            tt = np.ascontiguousarray(self.triangles, dtype=np.int32).reshape(-1)
+            p.triangles, p.n_triangle = tt.ctypes.data_as(A.ip), tt.size // 3
+            self._keep.append(tt)
+
+        def arr(v):
+            a = _f64(v)
+            self._keep.append(a)
+            return _dptr(a)
+
+        if np.isscalar(self.kBond) and np.isscalar(self.kHinge):
+            p.k_bar, p.k_hinge = float(self.kBond), float(self.kHinge)
+        else:
+            if not explicit:
+                raise ValueError("per-element rigidities need explicit bond and hinge lists")
+            p.bar_k = arr(np.broadcast_to(self.kBond, (p.n_bar,)))
+            p.hinge_k = arr(np.broadcast_to(self.kHinge, (p.n_hinge,)))
         if self.length0 is not None:
-            l0 = per(self.length0, nb)
-            self._keep.append(l0)
-            p.bar_l0 = _dptr(l0)
+            if np.isscalar(self.length0):
+                p.l0 = float(self.length0)
+            else:
+                p.bar_l0 = arr(self.length0)
         if self.theta0 is not None:
-            t0 = per(self.theta0, nh)
-            self._keep.append(t0)
-            p.hinge_theta0 = _dptr(t0)
+            if np.isscalar(self.theta0):
+                p.theta0 = float(self.theta0)
+            else:
+                p.hinge_theta0 = arr(self.theta0)
         if self.force is not None:
             self._keep.append(self.force)
             p.force = _dptr(self.force)

@@ -146,16 +146,33 @@ int minim_bah_create(const minim_bah_params* p, minim_potential** out) {
     need(p, "params");
     need(out, "out");
     auto b = std::make_unique<minim::BarAndHinge>();
-    std::vector<std::vector<int>> bl(p->n_bar), hl(p->n_hinge);
-    for (int i = 0; i < p->n_bar; i++) bl[i] = {p->bars[2 * i], p->bars[2 * i + 1]};
-    for (int i = 0; i < p->n_hinge; i++)
-      hl[i] = {p->hinges[4 * i], p->hinges[4 * i + 1], p->hinges[4 * i + 2], p->hinges[4 * i + 3]};
-    b->setBondList(bl);
-    b->setHingeList(hl);
-    std::vector<double> kb(p->bar_k, p->bar_k + p->n_bar), kh(p->hinge_k, p->hinge_k + p->n_hinge);
-    b->setRigidity(kb, kh);
+    if (p->n_bar > 0 || p->n_hinge > 0) {
+      if ((p->n_bar && !p->bars) || (p->n_hinge && !p->hinges)) throw std::invalid_argument("bars/hinges must not be NULL");
+      std::vector<std::vector<int>> bl(p->n_bar), hl(p->n_hinge);
+      for (int i = 0; i < p->n_bar; i++) bl[i] = {p->bars[2 * i], p->bars[2 * i + 1]};
+      for (int i = 0; i < p->n_hinge; i++)
+        hl[i] = {p->hinges[4 * i], p->hinges[4 * i + 1], p->hinges[4 * i + 2], p->hinges[4 * i + 3]};
+      b->setBondList(bl);
+      b->setHingeList(hl);
+    } else {
+      if (p->n_triangle <= 0) throw std::invalid_argument("BarAndHinge: The triangulation has not been set.");
+      need(p->triangles, "triangles");
+      std::vector<std::vector<int>> tl(p->n_triangle);
+      for (int i = 0; i < p->n_triangle; i++)
+        tl[i] = {p->triangles[3 * i], p->triangles[3 * i + 1], p->triangles[3 * i + 2]};
+      b->setTriangulation(tl);
+    }
+    if (p->bar_k || p->hinge_k) {
+      if (!p->bar_k || !p->hinge_k) throw std::invalid_argument("bar_k and hinge_k must be given together");
+      b->setRigidity(std::vector<double>(p->bar_k, p->bar_k + p->n_bar),
+                     std::vector<double>(p->hinge_k, p->hinge_k + p->n_hinge));
+    } else {
+      b->setRigidity(p->k_bar, p->k_hinge);
+    }
     if (p->bar_l0) b->setLength0(std::vector<double>(p->bar_l0, p->bar_l0 + p->n_bar));
+    else if (!std::isnan(p->l0)) b->setLength0(p->l0);
     if (p->hinge_theta0) b->setTheta0(std::vector<double>(p->hinge_theta0, p->hinge_theta0 + p->n_hinge));
+    else if (!std::isnan(p->theta0)) b->setTheta0(p->theta0);
     if (p->force) {
       if (p->force_per_node) b->setForce(std::vector<double>(p->force, p->force + 3 * (size_t)p->n_node));
       else b->setForce(std::vector<double>(p->force, p->force + 3));

@@ -1,0 +1,168 @@
+"""Lennard-Jones and bar-and-hinge potentials on the GPU vs the reference's
+golden vectors, its unit-test known answers and the oracle."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import golden
+from oracle_bind import OracleBaH, OracleLJ, OracleLJPBC, oracle_fire, oracle_lbfgs
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp64": 1e-12, "fp32": 1e-5}
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def r32(x):
+    return np.asarray(x, dtype=np.float64).astype(np.float32).astype(np.float64)
+
+
+# ---------------------------------------------------------------- Lennard-Jones
+def test_lj_dimer_known_answer(gpu):
+    # State_test.cpp:33-51 through every State entry point
+    m = gpu
+    s = m.State(m.Lj3d(), [0, 0, 0, 2, 0, 0])
+    gex = [-93.0 / 512, 0, 0, 93.0 / 512, 0, 0]
+    assert s.energy() == pytest.approx(-63.0 / 1024, rel=1e-15)
+    np.testing.assert_allclose(s.gradient(), gex, atol=1e-15)
+    e, g = s.energyGradient()
+    assert e == pytest.approx(-63.0 / 1024, rel=1e-15)
+    np.testing.assert_allclose(g, gex, atol=1e-15)
+    assert s.energy([0, 0, 0, 2, 0, 0]) == pytest.approx(-63.0 / 1024, rel=1e-15)
+    assert s.blockEnergy() == s.procEnergy() == s.allEnergy()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("case", range(3))
+def test_lj_golden_vectors(gpu, case, prec):
+    m = gpu
+    c = golden("lj")[case]
+    x = np.array(c["x"]) if prec == "fp64" else r32(c["x"])
+    s = m.State(m.LjNd(c["ndim"]), x, prec)
+    e, g = s.energyGradient()
+    o = OracleLJ(c["ndim"], c["n"])
+    eo, go = o.energy(x), o.gradient(x)
+    assert abs(e - eo) <= TOL[prec] * abs(eo)
+    assert rel_err(g, go) <= TOL[prec]
+    if prec == "fp64":
+        assert e == pytest.approx(c["energy"], rel=1e-12)
+        assert rel_err(g, c["gradient"]) <= 1e-12
+
+
+def lattice(n_side, a, jitter, rng):
+    g = np.arange(n_side) * a
+    pts = np.array(np.meshgrid(g, g, g, indexing="ij")).reshape(3, -1).T
+    return (pts + rng.uniform(-jitter, jitter, pts.shape)).reshape(-1)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("n_side", [5, 9])
+def test_lj_cutoff_vs_oracle(gpu, n_side, prec):
+    """Truncated-shifted LJ in a periodic box (cell list for 9^3, brute-force
+    list for the small 5^3 box) vs the oracle's minimum-image all-pairs sum."""
+    m = gpu
+    rng = np.random.default_rng(n_side)
+    a = 1.1
+    L = n_side * a
+    x = lattice(n_side, a, 0.15, rng)
+    x = x if prec == "fp64" else r32(x)
+    n = n_side ** 3
+    pot = m.Lj3d().setCutoff(2.5, [L, L, L], 0.3)
+    s = m.State(pot, x, prec)
+    e, g = s.energyGradient()
+    o = OracleLJPBC(n, L, 2.5)
+    assert abs(e - o.energy(x)) <= TOL[prec] * abs(o.energy(x))
+    assert rel_err(g, o.gradient(x)) <= TOL[prec]
+    # move particles beyond skin/2: the list must be rebuilt, results stay exact
+    y = x + rng.uniform(-0.3, 0.3, x.shape)
+    y = y if prec == "fp64" else r32(y)
+    e2, g2 = s.energyGradient(y)
+    assert abs(e2 - o.energy(y)) <= TOL[prec] * abs(o.energy(y))
+    assert rel_err(g2, o.gradient(y)) <= TOL[prec]
+
+
+def test_lj_fire_cluster(gpu):
+    """Fire relaxes a 27-particle cluster; same minimum as the oracle's Fire."""
+    m = gpu
+    rng = np.random.default_rng(11)
+    x0 = lattice(3, 1.12, 0.05, rng)
+    s = m.State(m.Lj3d(), x0)
+    f = m.Fire().setMaxIter(20000)
+    f.minimise(s)
+    o = OracleLJ(3, 27)
+    _, so = oracle_fire(o, x0, max_iter=20000)
+    assert f.stats["converged"] and so["converged"]
+    assert s.energy() == pytest.approx(so["energy"], rel=1e-8)
+
+
+# ---------------------------------------------------------------- BarAndHinge
+def bah_from_case(m, c, use_tri=False):
+    p = m.BarAndHinge()
+    if use_tri:
+        p.setTriangulation(c["tri"]).setRigidity(c["kb"], c["kh"])
+        if c["l0"] is not None:
+            p.setLength0(c["l0"])
+        if c["t0"] is not None:
+            p.setTheta0(c["t0"])
+    else:
+        p.setBondList(c["bars"]).setHingeList(c["hinges"]).setRigidity(c["bar_k"], c["hinge_k"])
+        p.setLength0(c["bar_l0"]).setTheta0(c["hinge_t0"])
+    if c["force"] is not None:
+        p.setForce(c["force"])
+    if c["fixed"]:
+        p.setConstraints(c["fixed"])
+    return p
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("case", range(2))
+def test_bah_golden_vectors(gpu, case, prec):
+    m = gpu
+    c = golden("bah")[case]
+    x = np.array(c["x"]) if prec == "fp64" else r32(c["x"])
+    s = m.State(bah_from_case(m, c), c["x0"], prec)
+    e, g = s.energyGradient(x)
+    nnode = len(x) // 3
+    fixed = None
+    if c["fixed"]:
+        fixed = np.zeros(3 * nnode, np.uint8)
+        fixed[c["fixed"]] = 1
+    o = OracleBaH(nnode, c["bars"], c["bar_k"], c["bar_l0"], c["hinges"], c["hinge_k"], c["hinge_t0"],
+                  force=None if c["force"] is None else np.array(c["force"]), fixed=fixed)
+    eo, go = o.energy(x), o.gradient(x)
+    assert abs(e - eo) <= TOL[prec] * abs(eo)
+    assert rel_err(g, go) <= TOL[prec]
+    if prec == "fp64":
+        assert e == pytest.approx(c["energy"], rel=1e-12)
+        assert rel_err(g, c["gradient"]) <= 1e-12
+        if c["fixed"]:
+            assert np.all(g[c["fixed"]] == 0)
+        # the same sheet built from its triangulation (O(T) list construction)
+        s2 = m.State(bah_from_case(m, c, use_tri=True), c["x0"], prec)
+        e2, g2 = s2.energyGradient(x)
+        assert e2 == pytest.approx(e, rel=1e-14)
+        assert rel_err(g2, g) <= 1e-14
+
+
+def test_bah_two_triangle_minima(gpu):
+    # BarAndHinge_test.cpp:13-56: L-BFGS minima for five rest angles, within 1e-4
+    m = gpu
+    c = golden("bah")[2]
+    dz = math.sqrt(2) / 2
+    s2 = math.sqrt(2)
+    expected = [[0, 0, 0.5 + dz, 0, -s2, 0.5 - dz, 0, s2, 0.5 - dz, 0, 0, 0.5 + dz],
+                [-1, 0, 1, 0, -s2, 0, 0, s2, 0, 1, 0, 1],
+                [-s2, 0, 0.5, 0, -s2, 0.5, 0, s2, 0.5, s2, 0, 0.5],
+                [-1, 0, 0, 0, -s2, 1, 0, s2, 1, 1, 0, 0],
+                [0, 0, 0.5 - dz, 0, -s2, 0.5 + dz, 0, s2, 0.5 + dz, 0, 0, 0.5 - dz]]
+    for mn_ref, ex in zip(c["minima"], expected):
+        p = m.BarAndHinge().setTriangulation(c["tri"]).setRigidity(c["kb"], c["kh"]).setTheta0(mn_ref["theta0"])
+        s = m.State(p, c["x0"])
+        x = m.Lbfgs().minimise(s)
+        np.testing.assert_allclose(x, ex, atol=1e-4)
+        np.testing.assert_allclose(x, mn_ref["x"], atol=1e-6)
