@@ -45,8 +45,9 @@ def time_evals(m, h, prec, x, g, e_dev, n, steps, warmup, stream, with_energy_co
     import torch
 
     with torch.cuda.stream(stream):
+        gp = None if g is None else g.data_ptr()
         for _ in range(warmup):
-            m.potential_eval_async(h, prec, x.data_ptr(), n, g_ptr=g.data_ptr(),
+            m.potential_eval_async(h, prec, x.data_ptr(), n, g_ptr=gp,
                                    e_dev_ptr=e_dev.data_ptr() if with_energy_copy else None,
                                    want_energy=True, stream=stream.cuda_stream)
         torch.cuda.synchronize()
@@ -54,7 +55,7 @@ def time_evals(m, h, prec, x, g, e_dev, n, steps, warmup, stream, with_energy_co
         t1 = torch.cuda.Event(enable_timing=True)
         t0.record(stream)
         for _ in range(steps):
-            m.potential_eval_async(h, prec, x.data_ptr(), n, g_ptr=g.data_ptr(),
+            m.potential_eval_async(h, prec, x.data_ptr(), n, g_ptr=gp,
                                    e_dev_ptr=e_dev.data_ptr() if with_energy_copy else None,
                                    want_energy=True, stream=stream.cuda_stream)
         t1.record(stream)
@@ -137,6 +138,9 @@ def main():
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--mode", choices=["full", "stencil", "converge"], default="full",
+                    help="stencil: only the evaluation timings plus a 256 MiB copy (the calibration "
+                         "kernel for rocprofv3 byte counters); converge: only time-to-converge")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the whole 256^3 field fits one MI355X); "
@@ -162,11 +166,33 @@ def main():
     e_dev = torch.zeros(1, dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
 
+    if args.mode == "converge":
+        del x64, x32, g64, g32
+        torch.cuda.empty_cache()
+        res = {mode: converge(m, x0, mode) for mode in ("fp64", "mixed")}
+        for mode, r in res.items():
+            log("%s: %.3f s, %d it (%d fp32), E %.6g -> %.6g, gRMS %.3g, nE %d nG %d" % (
+                mode, r["seconds"], r["iterations"], r["iter_fp32"], r["initial_energy"], r["final_energy"],
+                r["grad_rms"], r["n_energy"], r["n_gradient"]))
+        print(json.dumps(res), flush=True)
+        return
+
     ms32 = time_evals(m, h, "fp32", x32, g32, e_dev, n, args.steps, args.warmup, stream)
     ms32k = time_evals(m, h, "fp32", x32, g32, e_dev, n, args.steps, 1, stream, with_energy_copy=False)
     ms64 = time_evals(m, h, "fp64", x64, g64, e_dev, n, args.steps, args.warmup, stream)
     ms64k = time_evals(m, h, "fp64", x64, g64, e_dev, n, args.steps, 1, stream, with_energy_copy=False)
     log("E+G fp32 %.3f ms (kernel %.3f), fp64 %.3f ms (kernel %.3f)" % (ms32, ms32k, ms64, ms64k))
+    if args.mode == "stencil":
+        # calibration kernel for the byte counters: a plain copy of 256 MiB
+        a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        ms_e = time_evals(m, h, "fp32", x32, None, e_dev, n, args.steps, 1, stream)
+        log("E-only fp32 %.3f ms" % ms_e)
+        print(json.dumps({"fp32_eg_ms": ms32k, "fp64_eg_ms": ms64k, "fp32_e_ms": ms_e}), flush=True)
+        return
 
     sites = int(np.prod(GRID))
     bytes32 = sites * NF * (4 + 4)  # coordinates read once + gradient written once

@@ -528,6 +528,7 @@ void BarAndHinge::evaluate(int prec, const void* x, const void* d, double t, dou
     else
       minim_amd::bah_energy_kernel<float><<<nb, kBlock, 0, s>>>(a, (const float*)x, (const float*)d, (float)t, ws.rs());
     MINIM_HIP_CHECK(hipGetLastError());
+    minim_amd::launch_finalize(ws.rs(), 1, nb, s);
   }
   if (g) {
     if (d) throw std::invalid_argument("BarAndHinge: gradients at a trial point are not supported.");

@@ -72,49 +72,34 @@ __device__ inline void block_sum(double (&v)[K], double* lds) {
   }
 }
 
-// Scratch for cross-block reductions: one row of K partials per block, a
-// completion counter and the final result (device).  The last block to arrive
-// sums the rows in block order, writes result[0..K) and re-arms the counter.
+// Scratch for cross-block reductions: K rows of per-block partials and the
+// final result (device).
+//
+// Two stages, no fences and no atomics: each block of the producing kernel
+// writes its K partial sums (grid_sum_store), then a small kernel with one
+// block per value sums the partials in block order (launch_finalize).  A
+// "last block finishes the sum" scheme would need a device-scope fence in
+// every block, which on CDNA3/4 writes back the whole L2 each time.
 struct RedScratch {
-  double* partials;   // [kMaxRedBlocks * kMaxDots]
-  double* result;     // [kMaxDots]
-  unsigned* counter;  // [1], zero between launches
+  double* partials;  // [kMaxDots][kMaxRedBlocks], value k of block b at k * gridDim + b
+  double* result;    // [kMaxDots]
 };
 
 // All threads of the (1-D) grid must call this; v is each thread's contribution.
 template <int K>
 __device__ inline void grid_sum_store(double (&v)[K], RedScratch rs) {
   __shared__ double lds[(kBlock / kWave) * (K > 0 ? K : 1)];
-  __shared__ bool last;
   block_sum<K>(v, lds);
-  const int nb = gridDim.x;
   if (threadIdx.x == 0) {
+    const size_t nb = gridDim.x;
 #pragma unroll
-    for (int k = 0; k < K; k++) rs.partials[(size_t)blockIdx.x * K + k] = v[k];
-    __threadfence();
-    unsigned prev = atomicAdd(rs.counter, 1u);
-    last = (prev == (unsigned)(nb - 1));
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  // Fixed-order final sum: thread t takes blocks t, t+kBlock, ... then a block tree.
-  double w[K];
-#pragma unroll
-  for (int k = 0; k < K; k++) w[k] = 0.0;
-  for (int b = threadIdx.x; b < nb; b += kBlock) {
-#pragma unroll
-    for (int k = 0; k < K; k++)
-      w[k] += __hip_atomic_load(&rs.partials[(size_t)b * K + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  __syncthreads();
-  block_sum<K>(w, lds);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < K; k++) rs.result[k] = w[k];
-    atomicExch(rs.counter, 0u);  // vector atomic: re-arm for the next launch
+    for (int k = 0; k < K; k++) rs.partials[(size_t)k * nb + blockIdx.x] = v[k];
   }
 }
+
+// Second stage: result[k] = sum over the nb partials of value k, in a fixed
+// order (stream-ordered after the producing kernel).
+void launch_finalize(RedScratch rs, int K, int nb, hipStream_t s);
 
 // MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each
 // with a private L2.  Remap the hardware block id so that every XCD receives a

@@ -334,6 +334,7 @@ void launch_allpairs(int D, const void* x, const void* d, double t, int N, doubl
     default: throw std::invalid_argument("LjNd: nDim must be 1..4 on the device path");
   }
   MINIM_HIP_CHECK(hipGetLastError());
+  launch_finalize(rs, 1, nb, s);
 }
 
 }  // namespace
@@ -457,6 +458,7 @@ void LjNd::evaluate(int prec, const void* x, const void* d, double t, double* e,
         minim_amd::lj_displacement_kernel<float><<<nbd, kBlock, 0, s>>>(
             (const float*)x, (const float*)d, (float)t, N, (const double*)I.xbuild.get(), I.box.half_skin2, ws.rs());
       MINIM_HIP_CHECK(hipGetLastError());
+      minim_amd::launch_finalize(ws.rs(), 1, nbd, s);
       double moved = 0;
       ws.fetch(s, 1, &moved);
       rebuild = moved > 0;
@@ -528,6 +530,7 @@ void LjNd::evaluate(int prec, const void* x, const void* d, double t, double* e,
           (const float*)x, (const float*)d, (float)t, N, I.box, (float)(sigma * sigma), (float)epsilon, (float)eshift,
           mode, (const int*)I.nn.get(), (const int*)I.nbr.get(), (float*)g, ws.rs());
     MINIM_HIP_CHECK(hipGetLastError());
+    minim_amd::launch_finalize(ws.rs(), 1, nb, s);
   }
   if (g) applyConstraints(prec, g, (size_t)N * nDim, stream);
   if (e) {

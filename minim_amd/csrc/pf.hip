@@ -301,6 +301,7 @@ void pf_launch_eval(const PFArgs& a, int prec, int mode, const void* x, const vo
   else
     launch_eval_t<float>(a, mode, x, d, t, g, rs, s, a.nblocks);
   MINIM_HIP_CHECK(hipGetLastError());
+  launch_finalize(rs, 2, a.nblocks, s);
 }
 
 void pf_launch_constrain(const PFArgs& a, int prec, void* v, size_t nsite, hipStream_t s) {

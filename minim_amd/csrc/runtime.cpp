@@ -27,6 +27,27 @@ void DeviceBuffer::reset(size_t bytes) {
   }
 }
 
+namespace {
+// One block per reduced value: partials of value k are contiguous.
+__global__ __launch_bounds__(kBlock) void finalize_kernel(const double* __restrict__ partials, int nb,
+                                                          double* __restrict__ result) {
+  __shared__ double lds[kBlock / kWave];
+  const int k = blockIdx.x;
+  const double* p = partials + (size_t)k * nb;
+  double v[1] = {0.0};
+  for (int b = threadIdx.x; b < nb; b += kBlock) v[0] += p[b];
+  block_sum<1>(v, lds);
+  if (threadIdx.x == 0) result[k] = v[0];
+}
+}  // namespace
+
+void launch_finalize(RedScratch rs, int K, int nb, hipStream_t s) {
+  if (K < 1 || K > kMaxDots || nb < 1 || nb > kMaxRedBlocks)
+    throw std::invalid_argument("reduction shape out of range");
+  finalize_kernel<<<K, kBlock, 0, s>>>(rs.partials, nb, rs.result);
+  MINIM_HIP_CHECK(hipGetLastError());
+}
+
 Workspace::Workspace() {
   const size_t partial_bytes = sizeof(double) * (size_t)kMaxRedBlocks * kMaxDots;
   const size_t result_bytes = sizeof(double) * kMaxDots;
@@ -34,9 +55,6 @@ Workspace::Workspace() {
   char* base = static_cast<char*>(mem_.get());
   rs_.partials = reinterpret_cast<double*>(base);
   rs_.result = reinterpret_cast<double*>(base + partial_bytes);
-  rs_.counter = reinterpret_cast<unsigned*>(base + partial_bytes + result_bytes);
-  // the counter must start at zero; every reducing launch re-arms it
-  MINIM_HIP_CHECK(hipMemset(rs_.counter, 0, 256));
   MINIM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_), result_bytes, hipHostMallocDefault));
 }
 

@@ -206,11 +206,13 @@ int bucket(int k) {
 
 template <typename T, int NF>
 void dir_nf(const DirArgs& a, RedScratch rs, hipStream_t s, int nb) {
-  switch (bucket(a.ndot + 1)) {
+  const int K = bucket(a.ndot + 1);
+  switch (K) {
     case 8: direction_kernel<T, NF, 8><<<nb, kBlock, 0, s>>>(a, rs); break;
     case 16: direction_kernel<T, NF, 16><<<nb, kBlock, 0, s>>>(a, rs); break;
     default: direction_kernel<T, NF, 32><<<nb, kBlock, 0, s>>>(a, rs); break;
   }
+  launch_finalize(rs, K > 32 ? 32 : K, nb, s);
 }
 
 template <typename T>
@@ -229,11 +231,13 @@ void dir_t(const DirArgs& a, RedScratch rs, hipStream_t s) {
 template <typename T>
 void gram_t(const GramArgs& a, RedScratch rs, hipStream_t s) {
   const int nb = grid_blocks(a.n, 4, 2048);
-  switch (gram_bucket(a.nd > a.ny ? a.nd : a.ny)) {
+  const int KB = gram_bucket(a.nd > a.ny ? a.nd : a.ny);
+  switch (KB) {
     case 6: gram_kernel<T, 6><<<nb, kBlock, 0, s>>>(a, rs); break;
     case 11: gram_kernel<T, 11><<<nb, kBlock, 0, s>>>(a, rs); break;
     default: gram_kernel<T, 16><<<nb, kBlock, 0, s>>>(a, rs); break;
   }
+  launch_finalize(rs, 4 * KB + 3, nb, s);
 }
 
 }  // namespace
@@ -270,6 +274,7 @@ void launch_dots(int prec, const void* const* a, const void* const* b, int k, si
   if (prec == F64) dots_kernel_v<double, 8><<<nb, kBlock, 0, s>>>(p, k, n, rs);
   else dots_kernel_v<float, 8><<<nb, kBlock, 0, s>>>(p, k, n, rs);
   MINIM_HIP_CHECK(hipGetLastError());
+  launch_finalize(rs, k, nb, s);
 }
 
 void launch_axpy(int prec, void* x, const void* d, double t, size_t n, hipStream_t s) {
