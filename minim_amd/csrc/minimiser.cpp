@@ -338,6 +338,7 @@ void Lbfgs::iteration(State& state) {
   a.dotv[0] = I.g.data();
   for (int i = 0; i < P; i++) a.dotv[1 + i] = I.Y[I.pairs[i].slot].data();
   a.ndot = 1 + P;
+  a.pairs = P;  // basis = [g, d_0.., y_0..], dotv = [g, y_0..]
   a.basis[0] = I.g.data();
   a.nb = 1;
   if (P == 0) {
@@ -418,7 +419,7 @@ void Lbfgs::iteration(State& state) {
   for (int i = 0; i < P; i++) ga.Y[i] = I.Y[I.pairs[i].slot].data();
   ga.ny = P;
   minim_amd::launch_gram(ga, prec, ws.rs(), s);
-  const int KB = minim_amd::gram_bucket(ga.nd > ga.ny ? ga.nd : ga.ny);
+  const int KB = minim_amd::gram_kb(ga.nd, ga.ny);
   std::vector<double> r(4 * KB + 3);
   ws.fetch(s, 4 * KB + 3, r.data());
   auto gD = [&](int j) { return r[j]; };

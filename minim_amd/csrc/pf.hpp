@@ -21,7 +21,9 @@ struct PFArgs {
   double coef;             // soft-constraint coefficient
   double inv_nvar;         // 1 / number of variable fluids
   int nvar;
-  // launch shape
+  // launch shape: TZ x TY tiles of the (y,z) plane, ntile of them, each
+  // marching over nxc chunks of xchunk planes
+  int TZ, TY, ntz, nty, ntile;
   int xchunk, nxc, nblocks;
   int xchunk_override;     // >0 forces the x-march length (autotuning)
 };

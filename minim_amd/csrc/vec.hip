@@ -121,6 +121,142 @@ __global__ __launch_bounds__(kBlock) void gram_kernel(GramArgs a, RedScratch rs)
   grid_sum_store<K>(acc, rs);
 }
 
+// ---- compile-time-P fast paths: every load unconditional and issued together
+
+// L-BFGS direction with P stored pairs, sites of NF components:
+// d = c0 g + sum_q c(1+q) d_q + sum_q c(1+P+q) y_q, projected; dots d.d, d.g, d.y_q.
+template <typename T, int NF, int P>
+__global__ __launch_bounds__(kBlock) void direction_fast_kernel(DirArgs a, RedScratch rs) {
+  constexpr int NB = 1 + 2 * P;
+  constexpr int K = 2 + P;
+  const size_t nsite = a.n / NF;
+  T* __restrict__ out = static_cast<T*>(a.out);
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) acc[k] = 0.0;
+  for (size_t s = (size_t)blockIdx.x * kBlock + threadIdx.x; s < nsite; s += (size_t)gridDim.x * kBlock) {
+    const size_t base = s * NF;
+    T b[NB][NF];
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      const T* __restrict__ src = static_cast<const T*>(a.basis[j]);
+#pragma unroll
+      for (int k = 0; k < NF; k++) b[j][k] = src[base + k];
+    }
+    double v[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) v[k] = 0.0;
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      const double cj = a.coef[j];
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        const double p = cj * (double)b[j][k];
+        v[k] = v[k] + p;
+      }
+    }
+    if (NF > 1 && (a.proj.fixmask || a.proj.hard)) {
+#pragma unroll
+      for (int k = 0; k < NF; k++)
+        if (a.proj.fixmask & (1u << k)) v[k] = 0.0;
+      if (a.proj.hard && a.proj.nvar > 0) {
+        double comp = 0.0;
+#pragma unroll
+        for (int k = 0; k < NF; k++)
+          if (!(a.proj.fixmask & (1u << k))) comp += v[k];
+        comp *= a.proj.inv_nvar;
+#pragma unroll
+        for (int k = 0; k < NF; k++)
+          if (!(a.proj.fixmask & (1u << k))) v[k] -= comp;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      const T o = (T)v[k];
+      out[base + k] = o;
+      const double od = (double)o;
+      acc[0] += od * od;
+      acc[1] += od * (double)b[0][k];
+#pragma unroll
+      for (int q = 0; q < P; q++) acc[2 + q] += od * (double)b[1 + P + q][k];
+    }
+  }
+  grid_sum_store<K>(acc, rs);
+}
+
+// Gram pass with P old pairs: D = [d_0..d_{P-1}, d_new], Y = [y_0..y_{P-1}];
+// result layout of GramArgs with KB = P + 1.
+template <typename T, int P>
+__global__ __launch_bounds__(kBlock) void gram_fast_kernel(GramArgs a, RedScratch rs) {
+  constexpr int KB = P + 1;
+  constexpr int K = 4 * KB + 3;
+  const T* __restrict__ gn = static_cast<const T*>(a.gnew);
+  const T* __restrict__ go = static_cast<const T*>(a.gold);
+  T* __restrict__ yo = static_cast<T*>(a.y);
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) acc[k] = 0.0;
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += (size_t)gridDim.x * kBlock) {
+    T dv[P + 1], yv[P > 0 ? P : 1];
+    const T gv = gn[i];
+    const T gov = go[i];
+#pragma unroll
+    for (int j = 0; j <= P; j++) dv[j] = static_cast<const T*>(a.D[j])[i];
+#pragma unroll
+    for (int j = 0; j < P; j++) yv[j] = static_cast<const T*>(a.Y[j])[i];
+    const T ynew = gv - gov;
+    yo[i] = ynew;
+    const double g = (double)gv, y = (double)ynew;
+#pragma unroll
+    for (int j = 0; j <= P; j++) {
+      acc[j] += g * (double)dv[j];
+      acc[2 * KB + 2 + j] += y * (double)dv[j];
+    }
+#pragma unroll
+    for (int j = 0; j < P; j++) {
+      acc[KB + j] += g * (double)yv[j];
+      acc[3 * KB + 2 + j] += y * (double)yv[j];
+    }
+    acc[2 * KB] += g * g;
+    acc[2 * KB + 1] += g * y;
+    acc[4 * KB + 2] += y * y;
+  }
+  grid_sum_store<K>(acc, rs);
+}
+
+template <typename T, int NF>
+bool dir_fast_nf(const DirArgs& a, RedScratch rs, hipStream_t s, int nb) {
+  switch (a.pairs) {
+#define MINIM_DIR_CASE(P)                                                   \
+  case P:                                                                   \
+    direction_fast_kernel<T, NF, P><<<nb, kBlock, 0, s>>>(a, rs);           \
+    launch_finalize(rs, 2 + P, nb, s);                                      \
+    return true;
+    MINIM_DIR_CASE(0) MINIM_DIR_CASE(1) MINIM_DIR_CASE(2) MINIM_DIR_CASE(3) MINIM_DIR_CASE(4)
+    MINIM_DIR_CASE(5) MINIM_DIR_CASE(6) MINIM_DIR_CASE(7) MINIM_DIR_CASE(8) MINIM_DIR_CASE(9)
+    MINIM_DIR_CASE(10)
+#undef MINIM_DIR_CASE
+  }
+  return false;
+}
+
+template <typename T>
+bool gram_fast(const GramArgs& a, RedScratch rs, hipStream_t s, int nb) {
+  if (a.nd != a.ny + 1) return false;
+  switch (a.ny) {
+#define MINIM_GRAM_CASE(P)                                                  \
+  case P:                                                                   \
+    gram_fast_kernel<T, P><<<nb, kBlock, 0, s>>>(a, rs);                    \
+    launch_finalize(rs, 4 * (P + 1) + 3, nb, s);                            \
+    return true;
+    MINIM_GRAM_CASE(0) MINIM_GRAM_CASE(1) MINIM_GRAM_CASE(2) MINIM_GRAM_CASE(3) MINIM_GRAM_CASE(4)
+    MINIM_GRAM_CASE(5) MINIM_GRAM_CASE(6) MINIM_GRAM_CASE(7) MINIM_GRAM_CASE(8) MINIM_GRAM_CASE(9)
+    MINIM_GRAM_CASE(10)
+#undef MINIM_GRAM_CASE
+  }
+  return false;
+}
+
 struct DotPtrs {
   const void* a[8];
   const void* b[8];
@@ -219,6 +355,10 @@ template <typename T>
 void dir_t(const DirArgs& a, RedScratch rs, hipStream_t s) {
   const int nf = a.proj.nf < 1 ? 1 : a.proj.nf;
   const int nb = grid_blocks(a.n / nf, 2, 2048);
+  if (a.pairs >= 0 && a.pairs <= kFastPairs) {
+    if (nf == 1 && dir_fast_nf<T, 1>(a, rs, s, nb)) return;
+    if (nf == 3 && dir_fast_nf<T, 3>(a, rs, s, nb)) return;
+  }
   switch (nf) {
     case 1: dir_nf<T, 1>(a, rs, s, nb); break;
     case 2: dir_nf<T, 2>(a, rs, s, nb); break;
@@ -231,6 +371,7 @@ void dir_t(const DirArgs& a, RedScratch rs, hipStream_t s) {
 template <typename T>
 void gram_t(const GramArgs& a, RedScratch rs, hipStream_t s) {
   const int nb = grid_blocks(a.n, 4, 2048);
+  if (a.ny <= kFastPairs && gram_fast<T>(a, rs, s, nb)) return;
   const int KB = gram_bucket(a.nd > a.ny ? a.nd : a.ny);
   switch (KB) {
     case 6: gram_kernel<T, 6><<<nb, kBlock, 0, s>>>(a, rs); break;
@@ -253,6 +394,11 @@ int gram_bucket(int k) {
   if (k <= 6) return 6;
   if (k <= 11) return 11;
   return 16;
+}
+
+int gram_kb(int nd, int ny) {
+  if (nd == ny + 1 && ny <= kFastPairs) return ny + 1;
+  return gram_bucket(nd > ny ? nd : ny);
 }
 
 void launch_gram(const GramArgs& a, int prec, RedScratch rs, hipStream_t s) {

@@ -31,7 +31,12 @@ struct DirArgs {
   void* out;
   size_t n;
   SiteProj proj;
+  // >= 0 when the arguments follow the L-BFGS layout with P = pairs stored:
+  // basis = [g, d_0..d_{P-1}, y_0..y_{P-1}], dotv = [g, y_0..y_{P-1}]; selects
+  // kernels whose loads are all unconditional (compile-time P).
+  int pairs = -1;
 };
+constexpr int kFastPairs = 10;  // largest P with a compile-time kernel
 void launch_direction(const DirArgs& a, int prec, RedScratch rs, hipStream_t s);
 
 // y = gnew - gold, then dots written to rs.result with KB = gram_bucket(max(nd, ny)):
@@ -48,6 +53,8 @@ struct GramArgs {
   size_t n;
 };
 int gram_bucket(int k);
+// KB of the result layout above for a launch with these counts
+int gram_kb(int nd, int ny);
 void launch_gram(const GramArgs& a, int prec, RedScratch rs, hipStream_t s);
 
 // dots a[j].b[j] for j < k -> rs.result[0..k)
