@@ -39,6 +39,9 @@ class Minimiser {
 
   std::vector<double> minimise(State& state, std::function<void(int, State&)> adjustState = nullptr);
   std::vector<double> minimise(State& state, std::string logType);
+  // minimise() without the final download: the result stays in the State's
+  // device coordinates.
+  void run(State& state, std::function<void(int, State&)> adjustState = nullptr);
 
   virtual void init(State& state) {}
   virtual void iteration(State& state) = 0;
@@ -72,7 +75,9 @@ class Lbfgs : public NewMinimiser<Lbfgs> {
   Lbfgs& setInitStep(double initStep);
   // "state" (use the State's precision), "fp64", "fp32" or "mixed": FP32
   // storage until sqrt(g.g/ndof) < switchRms (or the FP32 line search stalls),
-  // then FP64 storage to the requested convergence.
+  // then FP64 coordinates and arithmetic to the requested convergence.  In the
+  // second phase the gradient, direction and history vectors stay in FP32 when
+  // the potential supportsMixed(), else they are widened to FP64.
   Lbfgs& setPrecision(std::string mode, double switchRms = 0);
 
   void init(State& state) override;

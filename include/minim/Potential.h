@@ -58,6 +58,12 @@ class Potential {
   // e_dev without synchronising the stream.
   virtual void evaluateAsync(int prec, const void* x, const void* d, double t, bool wantE, double* e_dev,
                              void* g, void* stream) const;
+  // Mixed storage: FP64 coordinates x and FP64 arithmetic, with the direction
+  // d and the gradient g held in FP32.  Potentials that implement it return
+  // true from supportsMixed(); the default evaluateMixed throws
+  // std::invalid_argument.
+  virtual bool supportsMixed() const { return false; }
+  virtual void evaluateMixed(const void* x, const void* d, double t, double* e, void* g, void* stream) const;
   // Potential::applyConstraints (src/Potential.cpp:102-115) on a device vector.
   virtual void applyConstraints(int prec, void* v, size_t n, void* stream) const;
   // The projection applyConstraints performs, for fused kernels.  Potentials

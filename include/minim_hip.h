@@ -116,6 +116,12 @@ int minim_potential_eval(const minim_potential* pot, int precision, const void* 
  * want_energy) is written to the device double e_dev, in stream order. */
 int minim_potential_eval_async(const minim_potential* pot, int precision, const void* x, size_t ndof,
                                const void* d, double t, int want_energy, double* e_dev, void* g, void* stream);
+/* As minim_potential_eval with FP64 coordinates x and FP64 arithmetic, but the
+ * direction d and the gradient g in FP32 (the second phase of the mixed L-BFGS
+ * schedule).  MINIM_ERR_INVALID for potentials without mixed support
+ * (PhaseField with nFluid <= 4 has it). */
+int minim_potential_eval_mixed(const minim_potential* pot, const void* x, size_t ndof, const void* d, double t,
+                               double* e, void* g, void* stream);
 /* Potential::applyConstraints on a device vector (src/Potential.cpp:102-115,
  * PhaseField::specialisedConstraints PhaseField.cpp:580-612). */
 int minim_potential_constrain(const minim_potential* pot, int precision, void* v, size_t n, void* stream);
@@ -191,7 +197,8 @@ typedef void (*minim_adjust_fn)(int iter, minim_state* state, void* user);
 int minim_minimiser_create(const minim_minimiser_params* p, minim_minimiser** out);
 int minim_minimiser_destroy(minim_minimiser* m);
 /* Minimiser::minimise(state, adjustState) (src/Minimiser.cpp:27-38); coords_out
- * (host, ndof) receives state.coords() and may be NULL. */
+ * (host, ndof) receives state.coords().  With coords_out NULL nothing is
+ * copied to the host: the result stays in the State's device coordinates. */
 int minim_minimise(minim_minimiser* m, minim_state* s, minim_adjust_fn fn, void* user, double* coords_out,
                    minim_result* r);
 /* Minimiser::minimise(state, "<fields>-<N>") (src/Minimiser.cpp:41-69). */

@@ -501,12 +501,13 @@ class Minimiser:
                           n_gradient=r.n_gradient, n_energy_reused=r.n_energy_reused, iter_fp32=r.iter_fp32,
                           iter_fp64=r.iter_fp64, energy=r.energy, grad_rms=r.grad_rms, dt_max=r.dt_max)
 
-    def minimise(self, state, adjust=None):
+    def minimise(self, state, adjust=None, download=True):
         """Minimiser::minimise(state, adjustState) / minimise(state, "<fields>-<N>")
-        (src/Minimiser.cpp:27-69)."""
+        (src/Minimiser.cpp:27-69).  With download=False the result stays in the
+        State's device coordinates and None is returned."""
         L = A.load()
         h = self._handle()
-        out = np.zeros(state.ndof)
+        out = np.zeros(state.ndof) if download or isinstance(adjust, str) else None
         r = A.Result()
         if isinstance(adjust, str):
             A.check(L.minim_minimise_log(h.ptr, state.ptr, adjust.encode(), _dptr(out), C.byref(r)))
@@ -516,7 +517,8 @@ class Minimiser:
                 def _cb(it, sp, user):
                     adjust(it, state)
                 cb = A.ADJUST_FN(_cb)
-            A.check(L.minim_minimise(h.ptr, state.ptr, cb, None, _dptr(out), C.byref(r)))
+            A.check(L.minim_minimise(h.ptr, state.ptr, cb, None, _dptr(out) if out is not None else None,
+                                     C.byref(r)))
         self._finish(r)
         return out
 
@@ -589,8 +591,8 @@ class Fire(Minimiser):
         p.dt_max = self.dtMax
         return p
 
-    def minimise(self, state, adjust=None):
-        out = super().minimise(state, adjust)
+    def minimise(self, state, adjust=None, download=True):
+        out = super().minimise(state, adjust, download)
         self.dtMax = self.stats["dt_max"]  # Fire::init stores its estimate (Fire.cpp:29-37)
         return out
 
@@ -638,6 +640,14 @@ def potential_eval_async(handle, precision, x_ptr, ndof, g_ptr=None, e_dev_ptr=N
     we = (e_dev_ptr is not None) if want_energy is None else bool(want_energy)
     A.check(A.load().minim_potential_eval_async(handle.ptr, _PREC[precision], x_ptr, ndof, d_ptr, float(t),
                                                 1 if we else 0, e_dev_ptr, g_ptr, stream))
+
+
+def potential_eval_mixed(handle, x_ptr, ndof, g_ptr=None, want_energy=True, d_ptr=None, t=0.0, stream=None):
+    """Evaluation with FP64 coordinates and FP32 direction/gradient vectors."""
+    e = C.c_double()
+    A.check(A.load().minim_potential_eval_mixed(handle.ptr, x_ptr, ndof, d_ptr, float(t),
+                                                C.byref(e) if want_energy else None, g_ptr, stream))
+    return e.value if want_energy else None
 
 
 def potential_constrain(handle, precision, v_ptr, n, stream=None):

@@ -241,6 +241,19 @@ int minim_potential_eval_async(const minim_potential* pot, int precision, const 
   });
 }
 
+int minim_potential_eval_mixed(const minim_potential* pot, const void* x, size_t ndof, const void* d, double t,
+                               double* e, void* g, void* stream) {
+  return guard([&] {
+    need(pot, "pot");
+    need(x, "x");
+    if (!pot->initialised) throw std::runtime_error("potential not initialised: call minim_potential_init");
+    if (ndof != pot->ndof) throw std::invalid_argument("ndof does not match the initialised potential");
+    if (d && g) throw std::invalid_argument("a gradient at a trial point (d != NULL) is not supported");
+    if (!pot->p->supportsMixed()) throw std::invalid_argument("this potential has no mixed FP64/FP32 evaluation");
+    pot->p->evaluateMixed(x, d, t, e, g, stream);
+  });
+}
+
 int minim_potential_constrain(const minim_potential* pot, int precision, void* v, size_t n, void* stream) {
   return guard([&] {
     need(pot, "pot");
@@ -458,8 +471,12 @@ int minim_minimise(minim_minimiser* m, minim_state* s, minim_adjust_fn fn, void*
     need(s, "state");
     std::function<void(int, minim::State&)> adj = nullptr;
     if (fn) adj = [fn, user, s](int it, minim::State&) { fn(it, s, user); };
-    std::vector<double> x = m->m->minimise(*s->s, adj);
-    if (coords_out) std::memcpy(coords_out, x.data(), sizeof(double) * x.size());
+    if (coords_out) {
+      std::vector<double> x = m->m->minimise(*s->s, adj);
+      std::memcpy(coords_out, x.data(), sizeof(double) * x.size());
+    } else {
+      m->m->run(*s->s, adj);  // result stays in the State's device coordinates
+    }
     fill_result(m, r);
   });
 }

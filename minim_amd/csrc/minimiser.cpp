@@ -48,6 +48,12 @@ Minimiser& Minimiser::setLinesearch(std::string method) {
 
 std::vector<double> Minimiser::minimise(State& state, std::function<void(int, State&)> adjustState) {
   if (!state.usesThisProc) return std::vector<double>();
+  run(state, adjustState);
+  return state.coords();
+}
+
+void Minimiser::run(State& state, std::function<void(int, State&)> adjustState) {
+  if (!state.usesThisProc) return;
   stats = MinimiseStats{};
   init(state);
   for (iter = 0; iter <= maxIter; iter++) {
@@ -61,7 +67,6 @@ std::vector<double> Minimiser::minimise(State& state, std::function<void(int, St
   }
   finish(state);
   state.invalidateHost();
-  return state.coords();
 }
 
 std::vector<double> Minimiser::minimise(State& state, std::string logType) {
@@ -113,10 +118,20 @@ struct EnergyCache {
   bool ok(const State& s) const { return valid && version == s.coordsVersion(); }
 };
 
+// Evaluation at x (+ t*d) with the vectors d and g in precision pv: the
+// State's own precision, or FP32 vectors against FP64 coordinates.
+void eval_at(State& s, int pv, const void* d, double t, double* e, void* g) {
+  if (pv == s.precision())
+    s.pot->evaluate(pv, s.deviceCoords(), d, t, e, g, s.stream());
+  else
+    s.pot->evaluateMixed(s.deviceCoords(), d, t, e, g, s.stream());
+}
+
 // backtrackingLinesearch (linesearch.cpp:8-29) along step = tau0 * d, with d a
-// device vector.  Trials are evaluated at x + tau*d without materialising
-// them; tau halves exactly like step *= 0.5.  Returns the tau applied to x.
-double backtrack(State& s, const void* d, double tau0, double gs, EnergyCache& ec, MinimiseStats& st,
+// device vector of precision pv.  Trials are evaluated at x + tau*d without
+// materialising them; tau halves exactly like step *= 0.5.  Returns the tau
+// applied to x.
+double backtrack(State& s, int pv, const void* d, double tau0, double gs, EnergyCache& ec, MinimiseStats& st,
                  bool* acceptedOut = nullptr) {
   const double c = 0.5, shrink = 0.5;
   double t = -c * gs;
@@ -132,7 +147,7 @@ double backtrack(State& s, const void* d, double tau0, double gs, EnergyCache& e
   bool accepted = false;
   double e = 0;
   for (int i = 0; i < 10; i++) {
-    s.pot->evaluate(s.precision(), s.deviceCoords(), d, tau, &e, nullptr, s.stream());
+    eval_at(s, pv, d, tau, &e, nullptr);
     st.nEnergy++;
     if (e0 - e >= t) {
       accepted = true;
@@ -141,7 +156,7 @@ double backtrack(State& s, const void* d, double tau0, double gs, EnergyCache& e
     tau = tau * shrink;
     t = t * shrink;
   }
-  minim_amd::launch_axpy(s.precision(), s.deviceCoords(), d, tau, s.ndof, hs(s));
+  minim_amd::launch_axpy_mixed(s.precision(), s.deviceCoords(), pv, d, tau, s.ndof, hs(s));
   s.invalidateHost();
   if (accepted) {
     ec.set(s, e);
@@ -153,8 +168,8 @@ double backtrack(State& s, const void* d, double tau0, double gs, EnergyCache& e
   return tau;
 }
 
-void gradient_into(State& s, void* g, MinimiseStats& st) {
-  s.pot->evaluate(s.precision(), s.deviceCoords(), nullptr, 0.0, nullptr, g, s.stream());
+void gradient_into(State& s, int pv, void* g, MinimiseStats& st) {
+  eval_at(s, pv, nullptr, 0.0, nullptr, g);
   st.nGradient++;
 }
 
@@ -172,7 +187,7 @@ double dot1(int prec, const void* a, const void* b, size_t n, hipStream_t st) {
 
 // ------------------------------------------------------------------ L-BFGS
 struct Lbfgs::Impl {
-  int prec = FP64;
+  int prec = FP64;  // precision of every vector below (g, D, Y)
   size_t n = 0;
   int m = 5;
   DVec g, gnew;
@@ -192,17 +207,25 @@ struct Lbfgs::Impl {
   bool fresh = true;  // next iteration starts from a new gradient
   int stall = 0;
 
+  // Empty history; device buffers are kept from the previous call when the
+  // sizes and precision match (a minimise on the same State allocates
+  // nothing).
   void alloc(int prec_, size_t n_, int m_) {
+    const bool reuse = prec_ == prec && n_ == n && m_ == m && (int)D.size() == m_ + 1 && g.n == n_ && g.prec == prec_;
     prec = prec_;
     n = n_;
     m = m_;
-    g = DVec(n, prec);
-    gnew = DVec(n, prec);
-    D.clear();
-    Y.clear();
-    for (int i = 0; i <= m; i++) {
-      D.emplace_back(n, prec);
-      Y.emplace_back(n, prec);
+    if (!reuse) {
+      D.clear();
+      Y.clear();
+      g = DVec();
+      gnew = DVec();
+      g = DVec(n, prec);
+      gnew = DVec(n, prec);
+      for (int i = 0; i <= m; i++) {
+        D.emplace_back(n, prec);
+        Y.emplace_back(n, prec);
+      }
     }
     used.assign(m + 1, 0);
     dg.assign(m + 1, 0.0);
@@ -211,6 +234,7 @@ struct Lbfgs::Impl {
     yy.assign(m + 1, std::vector<double>(m + 1, 0.0));
     pairs.clear();
     fresh = true;
+    stall = 0;
     ec.valid = false;
   }
 
@@ -227,13 +251,14 @@ struct Lbfgs::Impl {
     auto widen = [&](DVec& v) {
       DVec w(n, FP64);
       minim_amd::launch_convert(FP64, w.data(), FP32, v.data(), n, s);
+      MINIM_HIP_CHECK(hipStreamSynchronize(s));
       v = std::move(w);
     };
     widen(g);
+    gnew = DVec();
     gnew = DVec(n, FP64);
     for (auto& v : D) widen(v);
     for (auto& v : Y) widen(v);
-    MINIM_HIP_CHECK(hipStreamSynchronize(s));
     prec = FP64;
   }
 };
@@ -295,7 +320,7 @@ void Lbfgs::iteration(State& state) {
   const size_t n = I.n;
 
   if (I.fresh) {
-    gradient_into(state, I.g.data(), stats);
+    gradient_into(state, prec, I.g.data(), stats);
     I.gg = dot1(prec, I.g.data(), I.g.data(), n, s);
     if (iter == 0) I.pairs.clear();
     // new gradient: refresh its dots with every stored pair
@@ -397,17 +422,17 @@ void Lbfgs::iteration(State& state) {
   bool lsFailed = false;
   if (linesearch == "backtracking") {
     bool acc = true;
-    tau = backtrack(state, a.out, tau0, gs, I.ec, stats, &acc);
+    tau = backtrack(state, prec, a.out, tau0, gs, I.ec, stats, &acc);
     lsFailed = !acc;
   } else {
-    minim_amd::launch_axpy(prec, state.deviceCoords(), a.out, tau0, n, s);
+    minim_amd::launch_axpy_mixed(state.precision(), state.deviceCoords(), prec, a.out, tau0, n, s);
     state.invalidateHost();
     I.ec.valid = false;
     tau = tau0;
   }
 
   // ---- new gradient and Gram update (Lbfgs.cpp:67-80)
-  gradient_into(state, I.gnew.data(), stats);
+  gradient_into(state, prec, I.gnew.data(), stats);
   minim_amd::GramArgs ga{};
   ga.gnew = I.gnew.data();
   ga.gold = I.g.data();
@@ -458,22 +483,25 @@ void Lbfgs::iteration(State& state) {
   I.gg = gng;
   std::swap(I.g, I.gnew);
 
-  // ---- mixed precision: leave FP32 storage once it has done its job
-  if (I.prec == FP32 && _precMode == "mixed") {
+  // ---- mixed precision: leave FP32 coordinates once they have done their job
+  if (state.precision() == FP32 && _precMode == "mixed") {
     double rms = std::sqrt(I.gg / n);
     double sw = _switchRms > 0 ? _switchRms : 1e3 * state.convergence;
     if (lsFailed) I.stall++;
     else I.stall = 0;
     if (rms < sw || I.stall >= 2) {
-      I.promote(s);
+      // x widens exactly to FP64; the vectors stay FP32 when the potential
+      // evaluates FP64 coordinates against them, else widen too.  Both keep
+      // the Gram data valid.
+      if (!state.pot->supportsMixed()) I.promote(s);
       state.setPrecision(FP64);
       I.ec.valid = false;
-      I.fresh = true;  // recompute g in FP64 at the same point; history kept
+      I.fresh = true;  // recompute g from FP64 coordinates at the same point; history kept
       stats.iterFP32++;
       return;
     }
   }
-  if (I.prec == FP32) stats.iterFP32++;
+  if (state.precision() == FP32) stats.iterFP32++;
   else stats.iterFP64++;
 }
 
@@ -538,7 +566,7 @@ void Fire::init(State& state) {
   I.a = _aStart;
   I.nSteps = 0;
   if (dtMax != 0) return;
-  gradient_into(state, I.g.data(), stats);
+  gradient_into(state, I.prec, I.g.data(), stats);
   I.gNorm = std::sqrt(dot1(I.prec, I.g.data(), I.g.data(), I.n, hs(state)));
   if (I.gNorm != 0) {
     dtMax = 0.1 / std::sqrt(I.gNorm);
@@ -555,7 +583,7 @@ void Fire::iteration(State& state) {
   auto& ws = Workspace::current();
   if (iter == 0) {
     I.dt = dtMax;
-    gradient_into(state, I.g.data(), stats);
+    gradient_into(state, I.prec, I.g.data(), stats);
     I.gNorm = std::sqrt(dot1(I.prec, I.g.data(), I.g.data(), I.n, s));
   }
   const void* pa[2] = {I.v.data(), I.v.data()};
@@ -591,13 +619,13 @@ void Fire::iteration(State& state) {
   state.pot->applyConstraints(I.prec, I.step.data(), I.n, state.stream());
   if (linesearch == "backtracking") {
     double gs = dot1(I.prec, I.g.data(), I.step.data(), I.n, s);
-    backtrack(state, I.step.data(), 1.0, gs, I.ec, stats);
+    backtrack(state, I.prec, I.step.data(), 1.0, gs, I.ec, stats);
   } else {
     minim_amd::launch_add(I.prec, state.deviceCoords(), I.step.data(), I.n, s);
     state.invalidateHost();
     I.ec.valid = false;
   }
-  gradient_into(state, I.g.data(), stats);
+  gradient_into(state, I.prec, I.g.data(), stats);
   I.gNorm = std::sqrt(dot1(I.prec, I.g.data(), I.g.data(), I.n, s));
   if (I.prec == FP32) stats.iterFP32++;
   else stats.iterFP64++;
@@ -650,7 +678,7 @@ void GradDescent::init(State& state) {
 void GradDescent::iteration(State& state) {
   Impl& I = *impl_;
   hipStream_t s = hs(state);
-  gradient_into(state, I.g.data(), stats);
+  gradient_into(state, I.prec, I.g.data(), stats);
   // step = -alpha * g (explicit vector, rounded as the reference's)
   minim_amd::launch_scale_copy(I.prec, I.step.data(), I.g.data(), -_alpha, I.n, s);
   const void* pa[2] = {I.g.data(), I.g.data()};
@@ -661,7 +689,7 @@ void GradDescent::iteration(State& state) {
   ws.fetch(s, 2, r);
   I.gg = r[1];
   if (linesearch == "backtracking") {
-    backtrack(state, I.step.data(), 1.0, r[0], I.ec, stats);
+    backtrack(state, I.prec, I.step.data(), 1.0, r[0], I.ec, stats);
   } else {
     minim_amd::launch_add(I.prec, state.deviceCoords(), I.step.data(), I.n, s);
     state.invalidateHost();

@@ -38,8 +38,10 @@ struct NFM {
   static constexpr int value = NF > 0 ? NF : kPFMaxFluid;
 };
 
-template <typename T, int NF, bool DIR>
-__device__ inline void load_site(const T* __restrict__ x, const T* __restrict__ d, T t, size_t site, int nf,
+// T: precision of the coordinates and of the arithmetic; TV: precision of the
+// direction and gradient vectors (T = double, TV = float in the mixed path).
+template <typename T, typename TV, int NF, bool DIR>
+__device__ inline void load_site(const T* __restrict__ x, const TV* __restrict__ d, T t, size_t site, int nf,
                                  T (&v)[NFM<NF>::value]) {
   const size_t base = site * (size_t)nf;
   const int NK = NF > 0 ? NF : nf;
@@ -47,7 +49,7 @@ __device__ inline void load_site(const T* __restrict__ x, const T* __restrict__ 
   for (int k = 0; k < NK; k++) {
     T c = x[base + k];
     if (DIR) {
-      T s = t * d[base + k];  // trial point x + t*d, rounded as x + (t*d)
+      T s = t * (T)d[base + k];  // trial point x + t*d, rounded as x + (t*d)
       c = c + s;
     }
     v[k] = c;
@@ -128,9 +130,9 @@ __device__ inline double site_eg(const PFArgs& a, int nf, const T (&fb)[NFM<NF>:
 
 // MODE bits: 1 = energy, 2 = gradient (+ g.g), 4 = trial direction.
 // U = planes per march step.
-template <typename T, int NF, int MODE, int U>
+template <typename T, typename TV, int NF, int MODE, int U>
 __global__ __launch_bounds__(kBlock) void pf_kernel(PFArgs a, const T* __restrict__ x,
-                                                    const T* __restrict__ d, T t, T* __restrict__ g,
+                                                    const TV* __restrict__ d, T t, TV* __restrict__ g,
                                                     RedScratch rs) {
   constexpr bool WE = MODE & 1, WG = MODE & 2, DIR = MODE & 4;
   constexpr int M = NFM<NF>::value;
@@ -167,21 +169,21 @@ __global__ __launch_bounds__(kBlock) void pf_kernel(PFArgs a, const T* __restric
 
     // col[0] = plane xi-1, col[1] = plane xi, col[2..U+1] = planes xi+1..xi+U
     T col[U + 2][M];
-    load_site<T, NF, DIR>(x, d, t, (size_t)wrapx(x0 - 1) * YZ + p, nf, col[0]);
-    load_site<T, NF, DIR>(x, d, t, (size_t)x0 * YZ + p, nf, col[1]);
+    load_site<T, TV, NF, DIR>(x, d, t, (size_t)wrapx(x0 - 1) * YZ + p, nf, col[0]);
+    load_site<T, TV, NF, DIR>(x, d, t, (size_t)x0 * YZ + p, nf, col[1]);
 
     int xi = x0;
     for (; xi + U <= x1; xi += U) {
       T ym[U][M], yp[U][M], zm[U][M], zp[U][M];
 #pragma unroll
-      for (int u = 0; u < U; u++) load_site<T, NF, DIR>(x, d, t, (size_t)wrapx(xi + 1 + u) * YZ + p, nf, col[2 + u]);
+      for (int u = 0; u < U; u++) load_site<T, TV, NF, DIR>(x, d, t, (size_t)wrapx(xi + 1 + u) * YZ + p, nf, col[2 + u]);
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const size_t plane = (size_t)(xi + u) * YZ;
-        load_site<T, NF, DIR>(x, d, t, plane + pym, nf, ym[u]);
-        load_site<T, NF, DIR>(x, d, t, plane + pyp, nf, yp[u]);
-        load_site<T, NF, DIR>(x, d, t, plane + pzm, nf, zm[u]);
-        load_site<T, NF, DIR>(x, d, t, plane + pzp, nf, zp[u]);
+        load_site<T, TV, NF, DIR>(x, d, t, plane + pym, nf, ym[u]);
+        load_site<T, TV, NF, DIR>(x, d, t, plane + pyp, nf, yp[u]);
+        load_site<T, TV, NF, DIR>(x, d, t, plane + pzm, nf, zm[u]);
+        load_site<T, TV, NF, DIR>(x, d, t, plane + pzp, nf, zp[u]);
       }
 #pragma unroll
       for (int u = 0; u < U; u++) {
@@ -193,8 +195,9 @@ __global__ __launch_bounds__(kBlock) void pf_kernel(PFArgs a, const T* __restric
           const size_t base = ((size_t)(xi + u) * YZ + p) * (size_t)nf;
 #pragma unroll
           for (int k = 0; k < NK; k++) {
-            g[base + k] = gv[k];
-            acc[1] += (double)gv[k] * (double)gv[k];
+            const TV gs = (TV)gv[k];
+            g[base + k] = gs;
+            acc[1] += (double)gs * (double)gs;
           }
         }
       }
@@ -207,12 +210,12 @@ __global__ __launch_bounds__(kBlock) void pf_kernel(PFArgs a, const T* __restric
     // remaining planes one at a time
     for (; xi < x1; xi++) {
       T ym[M], yp[M], zm[M], zp[M];
-      load_site<T, NF, DIR>(x, d, t, (size_t)wrapx(xi + 1) * YZ + p, nf, col[2]);
+      load_site<T, TV, NF, DIR>(x, d, t, (size_t)wrapx(xi + 1) * YZ + p, nf, col[2]);
       const size_t plane = (size_t)xi * YZ;
-      load_site<T, NF, DIR>(x, d, t, plane + pym, nf, ym);
-      load_site<T, NF, DIR>(x, d, t, plane + pyp, nf, yp);
-      load_site<T, NF, DIR>(x, d, t, plane + pzm, nf, zm);
-      load_site<T, NF, DIR>(x, d, t, plane + pzp, nf, zp);
+      load_site<T, TV, NF, DIR>(x, d, t, plane + pym, nf, ym);
+      load_site<T, TV, NF, DIR>(x, d, t, plane + pyp, nf, yp);
+      load_site<T, TV, NF, DIR>(x, d, t, plane + pzm, nf, zm);
+      load_site<T, TV, NF, DIR>(x, d, t, plane + pzp, nf, zp);
       T gv[M];
       const double es =
           site_eg<T, NF, WE, WG>(a, nf, fb, fg, coef, inv_nvar, col[1], col[0], col[2], ym, yp, zm, zp, gv);
@@ -221,8 +224,9 @@ __global__ __launch_bounds__(kBlock) void pf_kernel(PFArgs a, const T* __restric
         const size_t base = (plane + p) * (size_t)nf;
 #pragma unroll
         for (int k = 0; k < NK; k++) {
-          g[base + k] = gv[k];
-          acc[1] += (double)gv[k] * (double)gv[k];
+          const TV gs = (TV)gv[k];
+          g[base + k] = gs;
+          acc[1] += (double)gs * (double)gs;
         }
       }
 #pragma unroll
@@ -273,32 +277,35 @@ constexpr int march_unroll() {
   return NF == 0 ? 1 : (sizeof(T) * NF <= 12 ? 4 : (sizeof(T) * NF <= 24 ? 2 : 1));
 }
 
-template <typename T, int NF>
+template <typename T, typename TV, int NF>
 void launch_eval_nf(const PFArgs& a, int mode, const void* x, const void* d, double t, void* g,
                     RedScratch rs, hipStream_t s, int nblocks) {
   constexpr int U = march_unroll<T, NF>();
   const T* xx = static_cast<const T*>(x);
-  const T* dd = static_cast<const T*>(d);
-  T* gg = static_cast<T*>(g);
+  const TV* dd = static_cast<const TV*>(d);
+  TV* gg = static_cast<TV*>(g);
   const T tt = (T)t;
   switch (mode) {
-    case 1: pf_kernel<T, NF, 1, U><<<nblocks, kBlock, 0, s>>>(a, xx, dd, tt, gg, rs); break;
-    case 5: pf_kernel<T, NF, 5, U><<<nblocks, kBlock, 0, s>>>(a, xx, dd, tt, gg, rs); break;
-    case 2: pf_kernel<T, NF, 2, U><<<nblocks, kBlock, 0, s>>>(a, xx, dd, tt, gg, rs); break;
-    case 3: pf_kernel<T, NF, 3, U><<<nblocks, kBlock, 0, s>>>(a, xx, dd, tt, gg, rs); break;
+    case 1: pf_kernel<T, TV, NF, 1, U><<<nblocks, kBlock, 0, s>>>(a, xx, dd, tt, gg, rs); break;
+    case 5: pf_kernel<T, TV, NF, 5, U><<<nblocks, kBlock, 0, s>>>(a, xx, dd, tt, gg, rs); break;
+    case 2: pf_kernel<T, TV, NF, 2, U><<<nblocks, kBlock, 0, s>>>(a, xx, dd, tt, gg, rs); break;
+    case 3: pf_kernel<T, TV, NF, 3, U><<<nblocks, kBlock, 0, s>>>(a, xx, dd, tt, gg, rs); break;
     default: throw std::invalid_argument("pf: unsupported evaluation mode");
   }
 }
 
-template <typename T>
+template <typename T, typename TV>
 void launch_eval_t(const PFArgs& a, int mode, const void* x, const void* d, double t, void* g,
                    RedScratch rs, hipStream_t s, int nblocks) {
   switch (a.nf) {
-    case 1: launch_eval_nf<T, 1>(a, mode, x, d, t, g, rs, s, nblocks); break;
-    case 2: launch_eval_nf<T, 2>(a, mode, x, d, t, g, rs, s, nblocks); break;
-    case 3: launch_eval_nf<T, 3>(a, mode, x, d, t, g, rs, s, nblocks); break;
-    case 4: launch_eval_nf<T, 4>(a, mode, x, d, t, g, rs, s, nblocks); break;
-    default: launch_eval_nf<T, 0>(a, mode, x, d, t, g, rs, s, nblocks); break;
+    case 1: launch_eval_nf<T, TV, 1>(a, mode, x, d, t, g, rs, s, nblocks); break;
+    case 2: launch_eval_nf<T, TV, 2>(a, mode, x, d, t, g, rs, s, nblocks); break;
+    case 3: launch_eval_nf<T, TV, 3>(a, mode, x, d, t, g, rs, s, nblocks); break;
+    case 4: launch_eval_nf<T, TV, 4>(a, mode, x, d, t, g, rs, s, nblocks); break;
+    default:
+      if (sizeof(T) != sizeof(TV)) throw std::invalid_argument("PhaseField: mixed precision needs nFluid <= 4");
+      launch_eval_nf<T, T, 0>(a, mode, x, d, t, g, rs, s, nblocks);
+      break;
   }
 }
 
@@ -352,12 +359,16 @@ void pf_args_finalize(PFArgs& a, int target_blocks) {
   a.nblocks = a.ntile * a.nxc;
 }
 
-void pf_launch_eval(const PFArgs& a, int prec, int mode, const void* x, const void* d, double t,
+void pf_launch_eval(const PFArgs& a, int prec, int pv, int mode, const void* x, const void* d, double t,
                     void* g, RedScratch rs, hipStream_t s) {
-  if (prec == F64)
-    launch_eval_t<double>(a, mode, x, d, t, g, rs, s, a.nblocks);
+  if (prec == F64 && pv == F64)
+    launch_eval_t<double, double>(a, mode, x, d, t, g, rs, s, a.nblocks);
+  else if (prec == F32 && pv == F32)
+    launch_eval_t<float, float>(a, mode, x, d, t, g, rs, s, a.nblocks);
+  else if (prec == F64 && pv == F32)
+    launch_eval_t<double, float>(a, mode, x, d, t, g, rs, s, a.nblocks);
   else
-    launch_eval_t<float>(a, mode, x, d, t, g, rs, s, a.nblocks);
+    throw std::invalid_argument("PhaseField: FP32 coordinates with FP64 vectors are not supported");
   MINIM_HIP_CHECK(hipGetLastError());
   launch_finalize(rs, 2, a.nblocks, s);
 }

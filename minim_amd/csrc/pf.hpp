@@ -32,8 +32,9 @@ struct PFArgs {
 void pf_args_finalize(PFArgs& a, int target_blocks);
 
 // mode bits: 1 energy, 2 gradient (plus g.g), 4 evaluate at x + t*d.
-// Results: rs.result[0] = energy, rs.result[1] = g.g (device).
-void pf_launch_eval(const PFArgs& a, int prec, int mode, const void* x, const void* d, double t,
+// prec: coordinates (and arithmetic); pv: the d and g vectors (prec, or FP32
+// with FP64 coordinates).  Results: rs.result[0] = energy, [1] = g.g (device).
+void pf_launch_eval(const PFArgs& a, int prec, int pv, int mode, const void* x, const void* d, double t,
                     void* g, RedScratch rs, hipStream_t s);
 void pf_launch_constrain(const PFArgs& a, int prec, void* v, size_t nsite, hipStream_t s);
 

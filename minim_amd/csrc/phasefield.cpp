@@ -195,9 +195,9 @@ void PhaseField::init(const vector<double>& coords) {
   args_ = a;
 }
 
-void PhaseField::evaluate(int prec, const void* x, const void* d, double t, double* e, void* g,
-                          void* stream) const {
-  const PFArgs& a = args_of(args_);
+namespace {
+void pf_eval_sync(const PFArgs& a, int px, int pv, const void* x, const void* d, double t, double* e, void* g,
+                  void* stream) {
   int mode = (e ? 1 : 0) | (g ? 2 : 0) | (d ? 4 : 0);
   if (mode == 0) return;
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -205,12 +205,22 @@ void PhaseField::evaluate(int prec, const void* x, const void* d, double t, doub
   if ((mode & 4) && (mode & 2)) {
     throw std::invalid_argument("PhaseField: gradients at a trial point are not supported.");
   }
-  minim_amd::pf_launch_eval(a, prec, mode, x, d, t, g, ws.rs(), s);
+  minim_amd::pf_launch_eval(a, px, pv, mode, x, d, t, g, ws.rs(), s);
   if (e) {
     double r[1];
     ws.fetch(s, 1, r);
     *e = r[0];
   }
+}
+}  // namespace
+
+void PhaseField::evaluate(int prec, const void* x, const void* d, double t, double* e, void* g,
+                          void* stream) const {
+  pf_eval_sync(args_of(args_), prec, prec, x, d, t, e, g, stream);
+}
+
+void PhaseField::evaluateMixed(const void* x, const void* d, double t, double* e, void* g, void* stream) const {
+  pf_eval_sync(args_of(args_), FP64, FP32, x, d, t, e, g, stream);
 }
 
 void PhaseField::evaluateAsync(int prec, const void* x, const void* d, double t, bool wantE, double* e_dev, void* g,
@@ -221,7 +231,7 @@ void PhaseField::evaluateAsync(int prec, const void* x, const void* d, double t,
   if ((mode & 4) && (mode & 2)) throw std::invalid_argument("PhaseField: gradients at a trial point are not supported.");
   hipStream_t s = static_cast<hipStream_t>(stream);
   auto& ws = minim_amd::Workspace::current();
-  minim_amd::pf_launch_eval(a, prec, mode, x, d, t, g, ws.rs(), s);
+  minim_amd::pf_launch_eval(a, prec, prec, mode, x, d, t, g, ws.rs(), s);
   if (wantE && e_dev)
     MINIM_HIP_CHECK(hipMemcpyAsync(e_dev, ws.device_result(), sizeof(double), hipMemcpyDeviceToDevice, s));
 }

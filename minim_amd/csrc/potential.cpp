@@ -45,6 +45,10 @@ void Potential::applyConstraints(int prec, void* v, size_t n, void* stream) cons
                                static_cast<hipStream_t>(stream));
 }
 
+void Potential::evaluateMixed(const void*, const void*, double, double*, void*, void*) const {
+  throw std::invalid_argument("Potential: mixed FP64/FP32 storage is not supported by this potential.");
+}
+
 void Potential::evaluateAsync(int prec, const void* x, const void* d, double t, bool wantE, double* e_dev, void* g,
                               void* stream) const {
   double e = 0;

@@ -277,10 +277,10 @@ __global__ __launch_bounds__(kBlock) void dots_kernel_v(DotPtrs p, int k, size_t
   grid_sum_store<K>(acc, rs);
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void axpy_kernel(T* __restrict__ x, const T* __restrict__ d, T t, size_t n) {
+template <typename T, typename TV>
+__global__ __launch_bounds__(kBlock) void axpy_kernel(T* __restrict__ x, const TV* __restrict__ d, T t, size_t n) {
   for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
-    const T s = t * d[i];
+    const T s = t * (T)d[i];
     x[i] = x[i] + s;
   }
 }
@@ -424,9 +424,15 @@ void launch_dots(int prec, const void* const* a, const void* const* b, int k, si
 }
 
 void launch_axpy(int prec, void* x, const void* d, double t, size_t n, hipStream_t s) {
+  launch_axpy_mixed(prec, x, prec, d, t, n, s);
+}
+
+void launch_axpy_mixed(int px, void* x, int pv, const void* d, double t, size_t n, hipStream_t s) {
   const int nb = grid_blocks(n, 4, 4096);
-  if (prec == F64) axpy_kernel<double><<<nb, kBlock, 0, s>>>((double*)x, (const double*)d, t, n);
-  else axpy_kernel<float><<<nb, kBlock, 0, s>>>((float*)x, (const float*)d, (float)t, n);
+  if (px == F64 && pv == F64) axpy_kernel<double, double><<<nb, kBlock, 0, s>>>((double*)x, (const double*)d, t, n);
+  else if (px == F32 && pv == F32) axpy_kernel<float, float><<<nb, kBlock, 0, s>>>((float*)x, (const float*)d, (float)t, n);
+  else if (px == F64 && pv == F32) axpy_kernel<double, float><<<nb, kBlock, 0, s>>>((double*)x, (const float*)d, t, n);
+  else throw std::invalid_argument("axpy: FP32 coordinates with an FP64 direction");
   MINIM_HIP_CHECK(hipGetLastError());
 }
 

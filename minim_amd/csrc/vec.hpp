@@ -63,6 +63,9 @@ void launch_dots(int prec, const void* const* a, const void* const* b, int k, si
 
 // x = x + t*d (element-wise, product rounded first)
 void launch_axpy(int prec, void* x, const void* d, double t, size_t n, hipStream_t s);
+// as launch_axpy with x in precision px and d in pv (pv == px, or FP32 d
+// widened exactly into FP64 x)
+void launch_axpy_mixed(int px, void* x, int pv, const void* d, double t, size_t n, hipStream_t s);
 
 // dst = src converted between precisions (prec_dst, prec_src)
 void launch_convert(int prec_dst, void* dst, int prec_src, const void* src, size_t n, hipStream_t s);

@@ -131,6 +131,64 @@ def test_trial_point_and_constraints(gpu, prec):
     assert e3 == e1
 
 
+MIXED_SHAPES = [
+    ((12, 10, 9), 1, [1.0], [1.5], "none"),
+    ((11, 1, 13), 2, [1.0], [2.0], "hard"),
+    ((20, 18, 16), 3, [1.0, 0.8, 1.2], [2.0], "hard"),
+    ((8, 8, 8), 4, [1.0, 0.9, 1.1, 0.8, 1.2, 0.7], [1.3], "soft"),
+    ((5, 6, 7), 3, [1.0, 0.8, 1.2], [2.0], "fixed"),
+]
+
+
+@pytest.mark.parametrize("shape", range(len(MIXED_SHAPES)))
+def test_mixed_storage_matches_fp64(gpu, shape):
+    """FP64 coordinates with FP32 direction/gradient storage: the arithmetic is
+    the FP64 kernel's, so the gradient is exactly the FP64 gradient rounded to
+    FP32, and energies (at x and at x + t*d) equal the FP64 evaluation with the
+    FP32 direction widened."""
+    import torch
+
+    m = gpu
+    grid, nf, st, iw, dens = MIXED_SHAPES[shape]
+    print("case mixed", grid, nf, dens, flush=True)
+    rng = np.random.default_rng(300 + shape)
+    n = int(np.prod(grid)) * nf
+    x0 = rng.uniform(-1.1, 1.1, n) if nf == 1 else rng.uniform(-0.1, 1.1, n)
+    pf = make_pf(m, grid, nf, st, iw, dens)
+    h = pf.handle(x0)
+    x = torch.tensor(x0, dtype=torch.float64, device="cuda")
+    d32 = torch.tensor(rng.normal(0, 0.1, n), dtype=torch.float32, device="cuda")
+    d64 = d32.double()
+    g64 = torch.empty(n, dtype=torch.float64, device="cuda")
+    g32 = torch.empty(n, dtype=torch.float32, device="cuda")
+    e64 = m.potential_eval(h, "fp64", x.data_ptr(), n, g_ptr=g64.data_ptr())
+    emx = m.potential_eval_mixed(h, x.data_ptr(), n, g_ptr=g32.data_ptr())
+    torch.cuda.synchronize()
+    assert emx == e64
+    assert torch.equal(g32, g64.float())
+    t = -0.625
+    assert m.potential_eval_mixed(h, x.data_ptr(), n, d_ptr=d32.data_ptr(), t=t) == m.potential_eval(
+        h, "fp64", x.data_ptr(), n, d_ptr=d64.data_ptr(), t=t)
+    # against the oracle too (FP64 bar)
+    o = OraclePF(grid, nf, 1.0, st, iw, dens)
+    assert abs(emx - o.energy(x0)) <= TOL["fp64"] * abs(o.energy(x0))
+    assert rel_err(g32.double().cpu().numpy(), o.gradient(x0)) <= 1e-7
+
+
+def test_mixed_storage_rejects_five_fluids(gpu):
+    import torch
+
+    m = gpu
+    grid, nf = (4, 5, 6), 5
+    n = int(np.prod(grid)) * nf
+    pf = make_pf(m, grid, nf, [1.0] * 5, [1.0], "none")
+    h = pf.handle(np.full(n, 0.2))
+    x = torch.full((n,), 0.2, dtype=torch.float64, device="cuda")
+    g = torch.empty(n, dtype=torch.float32, device="cuda")
+    with pytest.raises(ValueError):
+        m.potential_eval_mixed(h, x.data_ptr(), n, g_ptr=g.data_ptr())
+
+
 def test_full_size_256cubed_ternary(gpu):
     """The bench workload (256^3 sites x 3 fluids, 50.3M DOF) at full size:
     FP64 energy and gradient vs the oracle, FP32 vs FP64, and the hard
