@@ -251,7 +251,7 @@ def main():
         "fp64_ms_per_step": ms64,
         "roofline": {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach32 / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
-                     "kernel": "pf_kernel<float,3,E+G>", "kernel_ms": ms32k, "algorithmic_bytes": bytes32,
+                     "kernel": "pf_tile_kernel<float,float,3,hard,E+G>", "kernel_ms": ms32k, "algorithmic_bytes": bytes32,
                      "fp64": {"achieved": ach64, "frac": ach64 / HBM_PEAK_GBS, "kernel_ms": ms64k,
                               "algorithmic_bytes": bytes64}},
         "time_to_converge": ttc,

@@ -58,9 +58,10 @@ class PhaseField : public NewPotential<PhaseField> {
   vector<double> kappaP;
   int nGrid = 0;
 
-  // Launch tuning: x-march length of the stencil (0 = automatic), target block count.
+  // Launch tuning: x-march length of the stencil and target block count
+  // (0 = automatic for both).
   int stencilXChunk = 0;
-  int stencilTargetBlocks = 2048;
+  int stencilTargetBlocks = 0;
 
  private:
   void setDefaults();

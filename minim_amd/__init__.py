@@ -9,7 +9,7 @@ missing library raises ImportError.
 """
 from . import _capi
 from .api import (BarAndHinge, Fire, GradDescent, Lbfgs, Lj2d, Lj3d, LjNd, Minimiser, PhaseField, Potential,
-                  State, dot, potential_constrain, potential_eval, potential_eval_async,
+                  State, dot, potential_constrain, potential_eval, potential_eval_async, potential_eval_mixed,
                   triangulation_lists)
 
 
@@ -19,5 +19,5 @@ def __getattr__(name):
     raise AttributeError(name)
 
 __all__ = ["BarAndHinge", "Fire", "GradDescent", "Lbfgs", "Lj2d", "Lj3d", "LjNd", "Minimiser", "PhaseField",
-           "Potential", "State", "dot", "potential_constrain", "potential_eval", "potential_eval_async",
+           "Potential", "State", "dot", "potential_constrain", "potential_eval", "potential_eval_async", "potential_eval_mixed",
            "triangulation_lists", "lib"]

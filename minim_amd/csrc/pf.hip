@@ -7,155 +7,356 @@
 // gradient (src/State.cpp:48-56).
 //
 // The reference scatters +-factor*(c_i - c_j) into three gradient entries per
-// site, fluid and direction with OpenMP atomics.  With a uniform node volume the
-// scatter sums to the atomic-free gather form
+// site, fluid and direction with OpenMP atomics.  With a uniform node volume
+// the scatter sums to the atomic-free gather form
 //     g_i = bulk'(c_i) + 2 f * sum_d (2 c_i - c_{i-e_d} - c_{i+e_d})
-//     E   = sum_i [ bulk(c_i) + f * sum_d 0.5((c_i-c_{i-e_d})^2 + (c_i-c_{i+e_d})^2) ]
-// so every site is produced by exactly one thread and written once.
+// and the gradient energy, 0.5 f sum_i sum_d ((c_i-c_{i-e_d})^2 + (c_i-c_{i+e_d})^2),
+// counts every bond twice, so it equals f * sum_i sum_d (c_i - c_{i+e_d})^2:
+// each site adds only its three forward bonds.  Every site is produced by
+// exactly one thread and written once.  Dimensions of size 1 need no branch:
+// both "neighbours" are the site itself and contribute exact zeros, as the
+// reference's skip of such dimensions does (PhaseField.cpp:245).
 //
-// Work decomposition.  The grid is canonicalised (size-1 dimensions moved to
-// the front, which keeps the linear index) to X x Y x Z with Z contiguous.  A
-// block owns a TZ x TY tile of the (y,z) plane (64 x 4 when Z >= 64, so that
-// y neighbours are mostly loaded by the same block) and marches along a chunk
-// of x planes; each thread owns one (y,z) column.  The march advances U planes
-// per step: all U new column values and the 4U y/z neighbours are issued
-// before anything is waited on, so every thread keeps U sites of HBM traffic
-// in flight.  x neighbours stay in registers.  A dimension of size 1 needs no
-// branch: its two "neighbours" are the site itself and contribute exact zeros,
-// as the reference's skip of such dimensions does (PhaseField.cpp:245).
-// Blocks are ordered x-chunk major and spread over the 8 XCDs in contiguous
-// ranges, so all tiles of an x slab (which share halo rows) and consecutive
-// slabs (which share one halo plane) meet in the same L2.
+// Tiled kernel (nFluid <= 4), see pf_tile.hpp for the decomposition.  Each
+// block streams the x planes of a 64 x 4R (z, y) tile through a two-plane LDS
+// ring: while plane i is computed from LDS (y/z neighbours) and registers
+// (x neighbours, own column), the loads of plane i+2 are in flight; one
+// LDS-only barrier per plane (no wait on outstanding global loads or stores)
+// separates the reads of a buffer from its refill.  Loads and stores are one
+// global_load/store_dwordxN per site at a 32-bit offset from the plane base.
+// FP32 gradients are written with streaming (nt) stores.  The block order is
+// x-chunk major and the blocks of a chunk are kept on one XCD, so the halo
+// rows shared by neighbouring tiles meet in one L2.
+//
+// Generic kernel (nFluid > 4): one thread per (y, z) column, marching x with
+// the y/z neighbours read through the L1.
 #include "common.hpp"
 #include "pf.hpp"
+#include "pf_tile.hpp"
 
 namespace minim_amd {
 
 namespace {
 
-template <int NF>
-struct NFM {
-  static constexpr int value = NF > 0 ? NF : kPFMaxFluid;
+// Barrier that orders LDS traffic only: waits for this wave's LDS operations,
+// not for its outstanding global loads and stores.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+template <typename T>
+__device__ __forceinline__ T fmad(T a, T b, T c) {
+  return fma(a, b, c);
+}
+
+enum { DM_NONE = 0, DM_HARD = 1, DM_SOFT = 2 };
+
+// ---------------------------------------------------------------- tiled kernel
+template <typename T, int NF>
+struct SiteV {
+  T v[NF];
 };
 
-// T: precision of the coordinates and of the arithmetic; TV: precision of the
-// direction and gradient vectors (T = double, TV = float in the mixed path).
-template <typename T, typename TV, int NF, bool DIR>
-__device__ inline void load_site(const T* __restrict__ x, const TV* __restrict__ d, T t, size_t site, int nf,
-                                 T (&v)[NFM<NF>::value]) {
-  const size_t base = site * (size_t)nf;
-  const int NK = NF > 0 ? NF : nf;
+// One site at a 32-bit byte offset from a uniform base (one dwordxN access).
+template <typename T, int NF>
+__device__ __forceinline__ void ld_site(const T* __restrict__ base, unsigned boff, T (&v)[NF]) {
+  const SiteV<T, NF> s = *reinterpret_cast<const SiteV<T, NF>*>(reinterpret_cast<const char*>(base) + boff);
 #pragma unroll
-  for (int k = 0; k < NK; k++) {
+  for (int k = 0; k < NF; k++) v[k] = s.v[k];
+}
+
+template <typename T, int NF>
+__device__ __forceinline__ void st_site(T* __restrict__ base, unsigned boff, const T (&v)[NF]) {
+  if (sizeof(T) == 4) {
+    // streaming: the gradient is not re-read by this kernel
+    T* p = reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff);
+#pragma unroll
+    for (int k = 0; k < NF; k++) __builtin_nontemporal_store(v[k], p + k);
+  } else {
+    SiteV<T, NF> s;
+#pragma unroll
+    for (int k = 0; k < NF; k++) s.v[k] = v[k];
+    *reinterpret_cast<SiteV<T, NF>*>(reinterpret_cast<char*>(base) + boff) = s;
+  }
+}
+
+// Per-precision coefficient tables of PFArgs.
+template <typename T>
+struct PFC;
+template <>
+struct PFC<double> {
+  __device__ static double fb(const PFArgs& a, int k) { return a.fb[k]; }
+  __device__ static double fbg(const PFArgs& a, int k) { return a.fbg[k]; }
+  __device__ static double fg(const PFArgs& a, int k) { return a.fg[k]; }
+  __device__ static double fg2(const PFArgs& a, int k) { return a.fg2[k]; }
+  __device__ static double coef(const PFArgs& a) { return a.coef; }
+  __device__ static double coef2(const PFArgs& a) { return a.coef2; }
+  __device__ static double inv_nvar(const PFArgs& a) { return a.inv_nvar; }
+};
+template <>
+struct PFC<float> {
+  __device__ static float fb(const PFArgs& a, int k) { return a.fb_f[k]; }
+  __device__ static float fbg(const PFArgs& a, int k) { return a.fbg_f[k]; }
+  __device__ static float fg(const PFArgs& a, int k) { return a.fg_f[k]; }
+  __device__ static float fg2(const PFArgs& a, int k) { return a.fg2_f[k]; }
+  __device__ static float coef(const PFArgs& a) { return a.coef_f; }
+  __device__ static float coef2(const PFArgs& a) { return a.coef2_f; }
+  __device__ static float inv_nvar(const PFArgs& a) { return a.inv_nvar_f; }
+};
+
+// Energy (forward bonds) and constrained gradient of one site.
+template <typename T, int NF, int DM, bool WE, bool WG>
+__device__ __forceinline__ void site_tile(const PFArgs& a, const T (&c)[NF], const T (&xm)[NF], const T (&xp)[NF],
+                                          const T (&ym)[NF], const T (&yp)[NF], const T (&zm)[NF],
+                                          const T (&zp)[NF], T& e, T (&g)[NF]) {
+  using C = PFC<T>;
+  T rho = (T)-1;
+  e = (T)0;
+#pragma unroll
+  for (int k = 0; k < NF; k++) {
+    const T ck = c[k];
+    const T dxm = ck - xm[k], dxp = ck - xp[k];
+    const T dym = ck - ym[k], dyp = ck - yp[k];
+    const T dzm = ck - zm[k], dzp = ck - zp[k];
+    const T a2 = ck - (T)1;
+    if (WE) {
+      // bulk: (kappa/16) v (c+1)^2 (c-1)^2 or 0.5 kappa v c^2 (c-1)^2 (PhaseField.cpp:373-385)
+      T be;
+      if (NF == 1) {
+        const T a1 = ck + (T)1;
+        be = C::fb(a, k) * ((a1 * a1) * (a2 * a2));
+      } else {
+        be = C::fb(a, k) * ((ck * ck) * (a2 * a2));
+      }
+      T sq = dxp * dxp;
+      sq = fmad(dyp, dyp, sq);
+      sq = fmad(dzp, dzp, sq);
+      e = e + be;
+      e = fmad(C::fg(a, k), sq, e);
+    }
+    if (WG) {
+      const T lap = ((dxm + dxp) + (dym + dyp)) + (dzm + dzp);
+      T bg;
+      if (NF == 1) bg = C::fbg(a, k) * (ck * fmad(ck, ck, (T)-1));
+      else bg = C::fbg(a, k) * ((ck * a2) * fmad((T)2, ck, (T)-1));
+      g[k] = fmad(C::fg2(a, k), lap, bg);
+    }
+    if (DM == DM_SOFT) rho = rho + ck;
+  }
+  if (DM == DM_SOFT) {
+    // densityConstraintEnergy (PhaseField.cpp:457-473)
+    if (WE) e = fmad(C::coef(a), rho * rho, e);
+    if (WG) {
+      const T gr = C::coef2(a) * rho;
+#pragma unroll
+      for (int k = 0; k < NF; k++) g[k] = g[k] + gr;
+    }
+  }
+  if (WG) {
+    // specialisedConstraints: fixed fluids zeroed, then the hard density projection
+    if (a.fixmask) {
+#pragma unroll
+      for (int k = 0; k < NF; k++)
+        if (a.fixmask & (1u << k)) g[k] = (T)0;
+    }
+    if (DM == DM_HARD) {
+      T comp = g[0];
+#pragma unroll
+      for (int k = 1; k < NF; k++) comp = comp + g[k];  // fixed entries are zero
+      comp = comp * C::inv_nvar(a);
+#pragma unroll
+      for (int k = 0; k < NF; k++)
+        if (!(a.fixmask & (1u << k))) g[k] = g[k] - comp;
+    }
+  }
+}
+
+// Plane data of the R own rows and the halo site, as loaded (x, and d for a
+// trial point).
+template <typename T, typename TV, int NF, int R, bool DIR>
+struct PlaneIn {
+  T x[R][NF];
+  T hx[NF];
+  TV d[DIR ? R : 1][NF];
+  TV hd[NF];
+};
+
+template <typename T, typename TV, int NF, int R, bool DIR>
+__device__ __forceinline__ void load_plane(const T* __restrict__ xp, const TV* __restrict__ dp,
+                                           const unsigned (&ox)[R], const unsigned (&od)[R], unsigned hox,
+                                           unsigned hod, PlaneIn<T, TV, NF, R, DIR>& p) {
+#pragma unroll
+  for (int r = 0; r < R; r++) ld_site<T, NF>(xp, ox[r], p.x[r]);
+  ld_site<T, NF>(xp, hox, p.hx);
+  if constexpr (DIR) {
+#pragma unroll
+    for (int r = 0; r < R; r++) ld_site<TV, NF>(dp, od[r], p.d[r]);
+    ld_site<TV, NF>(dp, hod, p.hd);
+  }
+}
+
+// Coordinates of a loaded plane: x, or the trial point x + (t*d).
+template <typename T, typename TV, int NF, int R, bool DIR>
+__device__ __forceinline__ void resolve_plane(PlaneIn<T, TV, NF, R, DIR>& p, T t) {
+  if constexpr (DIR) {
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        const T s = t * (T)p.d[r][k];
+        p.x[r][k] = p.x[r][k] + s;
+      }
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      const T s = t * (T)p.hd[k];
+      p.hx[k] = p.hx[k] + s;
+    }
+  }
+}
+
+// MODE bits: 1 = energy, 2 = gradient (+ g.g), 4 = trial point x + t*d.
+template <typename T, typename TV, int NF, int DM, int MODE, int R>
+__global__ __launch_bounds__(kBlock) void pf_tile_kernel(PFArgs a, TileGeom G, const T* __restrict__ x,
+                                                         const TV* __restrict__ d, T t, TV* __restrict__ g,
+                                                         RedScratch rs) {
+  constexpr bool WE = MODE & 1, WG = MODE & 2, DIR = MODE & 4;
+  constexpr int LH = kTileWaves * R + 2, LW = kTileZ + 2;
+  __shared__ T lds[2][LH][LW][NF];
+
+  TileThread th;
+  tile_thread(G, xcd_logical_block(blockIdx.x, gridDim.x), threadIdx.x, th);
+  const size_t plane = (size_t)G.Y * G.Z * NF;  // elements per x plane
+  const int X = G.X;
+  auto wrapx = [X](int v) { return v >= X ? v - X : (v < 0 ? v + X : v); };
+  unsigned ox[R], od[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    ox[r] = th.site[r] * (unsigned)(NF * sizeof(T));
+    od[r] = th.site[r] * (unsigned)(NF * sizeof(TV));
+  }
+  const unsigned hox = th.hsite * (unsigned)(NF * sizeof(T));
+  const unsigned hod = th.hsite * (unsigned)(NF * sizeof(TV));
+  const bool halo = th.hrow >= 0;
+  const int x0 = th.x0, x1 = th.x1;
+
+  T A[R][NF], B[R][NF], Cn[R][NF];  // planes i-1, i, i+1 of the own rows
+  {
+    PlaneIn<T, TV, NF, R, DIR> pm, p0, p1;
+    const size_t sm = (size_t)wrapx(x0 - 1) * plane, s0 = (size_t)x0 * plane, s1 = (size_t)wrapx(x0 + 1) * plane;
+    load_plane<T, TV, NF, R, DIR>(x + sm, d + sm, ox, od, hox, hod, pm);
+    load_plane<T, TV, NF, R, DIR>(x + s0, d + s0, ox, od, hox, hod, p0);
+    load_plane<T, TV, NF, R, DIR>(x + s1, d + s1, ox, od, hox, hod, p1);
+    resolve_plane<T, TV, NF, R, DIR>(pm, t);
+    resolve_plane<T, TV, NF, R, DIR>(p0, t);
+    resolve_plane<T, TV, NF, R, DIR>(p1, t);
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        A[r][k] = pm.x[r][k];
+        B[r][k] = p0.x[r][k];
+        Cn[r][k] = p1.x[r][k];
+        lds[0][th.lrow0 + r][th.lcol][k] = p0.x[r][k];
+        lds[1][th.lrow0 + r][th.lcol][k] = p1.x[r][k];
+      }
+    if (halo)
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        lds[0][th.hrow][th.hcol][k] = p0.hx[k];
+        lds[1][th.hrow][th.hcol][k] = p1.hx[k];
+      }
+  }
+  __syncthreads();
+
+  double acc[2] = {0.0, 0.0};  // energy, g.g
+#pragma nounroll
+  for (int i = x0; i < x1; i++) {
+    const int b = (i - x0) & 1;
+    // plane i+2 in flight while plane i is computed (wrapped: always a valid plane)
+    PlaneIn<T, TV, NF, R, DIR> q;
+    const size_t sq = (size_t)wrapx(i + 2) * plane;
+    load_plane<T, TV, NF, R, DIR>(x + sq, d + sq, ox, od, hox, hod, q);
+
+    TV* gp = g + (size_t)i * plane;
+    T eacc = (T)0, gacc = (T)0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      T ym[NF], yp[NF], zm[NF], zp[NF], gv[NF], es;
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        ym[k] = (r > 0) ? B[r > 0 ? r - 1 : 0][k] : lds[b][th.lrow0 + r - 1][th.lcol][k];
+        yp[k] = (r < R - 1) ? B[r < R - 1 ? r + 1 : 0][k] : lds[b][th.lrow0 + r + 1][th.lcol][k];
+        zm[k] = lds[b][th.lrow0 + r][th.lcol - 1][k];
+        zp[k] = lds[b][th.lrow0 + r][th.lcol + 1][k];
+      }
+      site_tile<T, NF, DM, WE, WG>(a, B[r], A[r], Cn[r], ym, yp, zm, zp, es, gv);
+      if (th.own & (1u << r)) {
+        if (WE) eacc = eacc + es;
+        if (WG) {
+          TV gs[NF];
+#pragma unroll
+          for (int k = 0; k < NF; k++) {
+            gs[k] = (TV)gv[k];
+            gacc = fmad((T)gs[k], (T)gs[k], gacc);
+          }
+          st_site<TV, NF>(gp, od[r], gs);
+        }
+      }
+    }
+    if (WE) acc[0] += (double)eacc;
+    if (WG) acc[1] += (double)gacc;
+
+    lds_barrier();  // every wave is done reading buffer b
+    resolve_plane<T, TV, NF, R, DIR>(q, t);
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        lds[b][th.lrow0 + r][th.lcol][k] = q.x[r][k];
+        A[r][k] = B[r][k];
+        B[r][k] = Cn[r][k];
+        Cn[r][k] = q.x[r][k];
+      }
+    if (halo)
+#pragma unroll
+      for (int k = 0; k < NF; k++) lds[b][th.hrow][th.hcol][k] = q.hx[k];
+  }
+  grid_sum_store<2>(acc, rs);
+}
+
+// Rows per thread: the three plane copies of the own rows plus the plane in
+// flight must stay in registers.
+template <typename T, int NF>
+constexpr int tile_rows() {
+  return sizeof(T) * NF <= 24 ? 4 : 2;
+}
+
+// ---------------------------------------------------------------- generic kernel (nFluid > 4)
+template <typename T>
+__device__ inline void load_site_g(const T* __restrict__ x, const T* __restrict__ d, T t, size_t site, int nf,
+                                   bool dir, T (&v)[kPFMaxFluid]) {
+  const size_t base = site * (size_t)nf;
+  for (int k = 0; k < nf; k++) {
     T c = x[base + k];
-    if (DIR) {
-      T s = t * (T)d[base + k];  // trial point x + t*d, rounded as x + (t*d)
+    if (dir) {
+      const T s = t * d[base + k];
       c = c + s;
     }
     v[k] = c;
   }
 }
 
-// Energy and gradient of one site from its own value c, x neighbours (cm, cp)
-// and y/z neighbours.  Returns the site energy in FP64; writes gv.
-template <typename T, int NF, bool WE, bool WG>
-__device__ inline double site_eg(const PFArgs& a, int nf, const T (&fb)[NFM<NF>::value],
-                                 const T (&fg)[NFM<NF>::value], T coef, T inv_nvar, const T* c, const T* cm,
-                                 const T* cp, const T* ym, const T* yp, const T* zm, const T* zp,
-                                 T (&gv)[NFM<NF>::value]) {
-  const int NK = NF > 0 ? NF : nf;
-  T rho = (T)-1;
-  double esite = 0.0;
-#pragma unroll
-  for (int k = 0; k < NK; k++) {
-    const T ck = c[k];
-    T bulk_e, bulk_g;
-    if (NF == 1 || (NF == 0 && nf == 1)) {
-      // (kappa/16) v (c+1)^2 (c-1)^2 and its derivative (PhaseField.cpp:373-378)
-      const T a1 = ck + (T)1, a2 = ck - (T)1;
-      bulk_e = fb[k] * (a1 * a1) * (a2 * a2);
-      bulk_g = fb[k] * (T)4 * ck * (ck * ck - (T)1);
-    } else {
-      // 0.5 kappa v c^2 (c-1)^2 and its derivative (PhaseField.cpp:380-385)
-      const T a2 = ck - (T)1;
-      bulk_e = fb[k] * (ck * ck) * (a2 * a2);
-      bulk_g = fb[k] * (T)2 * ck * a2 * ((T)2 * ck - (T)1);
-    }
-    const T gxm = ck - cm[k], gxp = ck - cp[k];
-    const T gym = ck - ym[k], gyp = ck - yp[k];
-    const T gzm = ck - zm[k], gzp = ck - zp[k];
-    if (WE) {
-      T grad2 = (T)0.5 * (gxm * gxm + gxp * gxp);
-      grad2 += (T)0.5 * (gym * gym + gyp * gyp);
-      grad2 += (T)0.5 * (gzm * gzm + gzp * gzp);
-      esite += (double)bulk_e + (double)(fg[k] * grad2);
-    }
-    if (WG) {
-      T lap = gxm + gxp;
-      lap += gym + gyp;
-      lap += gzm + gzp;
-      gv[k] = bulk_g + (T)2 * fg[k] * lap;
-    }
-    rho += ck;
-  }
-  if (a.density == kDensitySoft && nf > 1) {
-    // densityConstraintEnergy (PhaseField.cpp:457-473)
-    if (WE) esite += (double)(coef * (rho * rho));
-    if (WG) {
-      const T gr = (T)2 * coef * rho;
-#pragma unroll
-      for (int k = 0; k < NK; k++) gv[k] += gr;
-    }
-  }
-  if (WG) {
-    // specialisedConstraints: fixed fluids, then the hard density projection
-    if (a.fixmask) {
-#pragma unroll
-      for (int k = 0; k < NK; k++)
-        if (a.fixmask & (1u << k)) gv[k] = (T)0;
-    }
-    if (a.density == kDensityHard && nf > 1 && a.nvar > 0) {
-      T comp = (T)0;
-#pragma unroll
-      for (int k = 0; k < NK; k++)
-        if (!(a.fixmask & (1u << k))) comp += gv[k];
-      comp *= inv_nvar;
-#pragma unroll
-      for (int k = 0; k < NK; k++)
-        if (!(a.fixmask & (1u << k))) gv[k] -= comp;
-    }
-  }
-  return esite;
-}
-
-// MODE bits: 1 = energy, 2 = gradient (+ g.g), 4 = trial direction.
-// U = planes per march step.
-template <typename T, typename TV, int NF, int MODE, int U>
-__global__ __launch_bounds__(kBlock) void pf_kernel(PFArgs a, const T* __restrict__ x,
-                                                    const TV* __restrict__ d, T t, TV* __restrict__ g,
-                                                    RedScratch rs) {
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void pf_generic_kernel(PFArgs a, const T* __restrict__ x,
+                                                            const T* __restrict__ d, T t, T* __restrict__ g,
+                                                            RedScratch rs) {
   constexpr bool WE = MODE & 1, WG = MODE & 2, DIR = MODE & 4;
-  constexpr int M = NFM<NF>::value;
-  const int nf = NF > 0 ? NF : a.nf;
-  const int NK = NF > 0 ? NF : nf;
-
+  const int nf = a.nf;
   const int L = xcd_logical_block(blockIdx.x, gridDim.x);
   const int xc = L / a.ntile, tile = L - xc * a.ntile;
   const int ty_tile = tile / a.ntz, tz_tile = tile - ty_tile * a.ntz;
   const int ly = threadIdx.x / a.TZ, lz = threadIdx.x - ly * a.TZ;
   const int y = ty_tile * a.TY + ly, z = tz_tile * a.TZ + lz;
-
-  double acc[2] = {0.0, 0.0};  // energy, g.g
-
+  double acc[2] = {0.0, 0.0};
   if (ly < a.TY && y < a.Y && z < a.Z) {
-    T fb[M], fg[M];
-#pragma unroll
-    for (int k = 0; k < NK; k++) {
-      fb[k] = (T)a.fb[k];
-      fg[k] = (T)a.fg[k];
-    }
-    const T coef = (T)a.coef;
-    const T inv_nvar = (T)a.inv_nvar;
     const int p = y * a.Z + z;
     const int pzm = (z == 0) ? p + a.Z - 1 : p - 1;
     const int pzp = (z == a.Z - 1) ? p - (a.Z - 1) : p + 1;
@@ -163,87 +364,81 @@ __global__ __launch_bounds__(kBlock) void pf_kernel(PFArgs a, const T* __restric
     const int pyp = (y == a.Y - 1) ? p - (a.Y - 1) * a.Z : p + a.Z;
     const size_t YZ = (size_t)a.YZ;
     const int X = a.X;
-    const int x0 = xc * a.xchunk;
-    const int x1 = min(x0 + a.xchunk, X);
     auto wrapx = [X](int v) { return v >= X ? v - X : (v < 0 ? v + X : v); };
-
-    // col[0] = plane xi-1, col[1] = plane xi, col[2..U+1] = planes xi+1..xi+U
-    T col[U + 2][M];
-    load_site<T, TV, NF, DIR>(x, d, t, (size_t)wrapx(x0 - 1) * YZ + p, nf, col[0]);
-    load_site<T, TV, NF, DIR>(x, d, t, (size_t)x0 * YZ + p, nf, col[1]);
-
-    int xi = x0;
-    for (; xi + U <= x1; xi += U) {
-      T ym[U][M], yp[U][M], zm[U][M], zp[U][M];
-#pragma unroll
-      for (int u = 0; u < U; u++) load_site<T, TV, NF, DIR>(x, d, t, (size_t)wrapx(xi + 1 + u) * YZ + p, nf, col[2 + u]);
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const size_t plane = (size_t)(xi + u) * YZ;
-        load_site<T, TV, NF, DIR>(x, d, t, plane + pym, nf, ym[u]);
-        load_site<T, TV, NF, DIR>(x, d, t, plane + pyp, nf, yp[u]);
-        load_site<T, TV, NF, DIR>(x, d, t, plane + pzm, nf, zm[u]);
-        load_site<T, TV, NF, DIR>(x, d, t, plane + pzp, nf, zp[u]);
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        T gv[M];
-        const double es = site_eg<T, NF, WE, WG>(a, nf, fb, fg, coef, inv_nvar, col[1 + u], col[u], col[2 + u],
-                                                 ym[u], yp[u], zm[u], zp[u], gv);
-        if (WE) acc[0] += es;
+    const int x0 = xc * a.xchunk, x1 = min(x0 + a.xchunk, X);
+    T cm[kPFMaxFluid], c[kPFMaxFluid], cp[kPFMaxFluid];
+    load_site_g<T>(x, d, t, (size_t)wrapx(x0 - 1) * YZ + p, nf, DIR, cm);
+    load_site_g<T>(x, d, t, (size_t)x0 * YZ + p, nf, DIR, c);
+    for (int xi = x0; xi < x1; xi++) {
+      T ym[kPFMaxFluid], yp[kPFMaxFluid], zm[kPFMaxFluid], zp[kPFMaxFluid];
+      const size_t pl = (size_t)xi * YZ;
+      load_site_g<T>(x, d, t, (size_t)wrapx(xi + 1) * YZ + p, nf, DIR, cp);
+      load_site_g<T>(x, d, t, pl + pym, nf, DIR, ym);
+      load_site_g<T>(x, d, t, pl + pyp, nf, DIR, yp);
+      load_site_g<T>(x, d, t, pl + pzm, nf, DIR, zm);
+      load_site_g<T>(x, d, t, pl + pzp, nf, DIR, zp);
+      T gv[kPFMaxFluid];
+      T rho = (T)-1;
+      T es = (T)0;
+      for (int k = 0; k < nf; k++) {
+        const T ck = c[k];
+        const T a2 = ck - (T)1;
+        const T dxm = ck - cm[k], dxp = ck - cp[k], dym = ck - ym[k], dyp = ck - yp[k];
+        const T dzm = ck - zm[k], dzp = ck - zp[k];
+        if (WE) {
+          const T be = PFC<T>::fb(a, k) * ((ck * ck) * (a2 * a2));
+          T sq = dxp * dxp;
+          sq = fmad(dyp, dyp, sq);
+          sq = fmad(dzp, dzp, sq);
+          es = es + be;
+          es = fmad(PFC<T>::fg(a, k), sq, es);
+        }
         if (WG) {
-          const size_t base = ((size_t)(xi + u) * YZ + p) * (size_t)nf;
-#pragma unroll
-          for (int k = 0; k < NK; k++) {
-            const TV gs = (TV)gv[k];
-            g[base + k] = gs;
-            acc[1] += (double)gs * (double)gs;
-          }
+          const T lap = ((dxm + dxp) + (dym + dyp)) + (dzm + dzp);
+          const T bg = PFC<T>::fbg(a, k) * ((ck * a2) * fmad((T)2, ck, (T)-1));
+          gv[k] = fmad(PFC<T>::fg2(a, k), lap, bg);
         }
+        rho = rho + ck;
       }
-#pragma unroll
-      for (int k = 0; k < NK; k++) {
-        col[0][k] = col[U][k];
-        col[1][k] = col[U + 1][k];
+      if (a.density == kDensitySoft) {
+        if (WE) es = fmad(PFC<T>::coef(a), rho * rho, es);
+        if (WG)
+          for (int k = 0; k < nf; k++) gv[k] = gv[k] + PFC<T>::coef2(a) * rho;
       }
-    }
-    // remaining planes one at a time
-    for (; xi < x1; xi++) {
-      T ym[M], yp[M], zm[M], zp[M];
-      load_site<T, TV, NF, DIR>(x, d, t, (size_t)wrapx(xi + 1) * YZ + p, nf, col[2]);
-      const size_t plane = (size_t)xi * YZ;
-      load_site<T, TV, NF, DIR>(x, d, t, plane + pym, nf, ym);
-      load_site<T, TV, NF, DIR>(x, d, t, plane + pyp, nf, yp);
-      load_site<T, TV, NF, DIR>(x, d, t, plane + pzm, nf, zm);
-      load_site<T, TV, NF, DIR>(x, d, t, plane + pzp, nf, zp);
-      T gv[M];
-      const double es =
-          site_eg<T, NF, WE, WG>(a, nf, fb, fg, coef, inv_nvar, col[1], col[0], col[2], ym, yp, zm, zp, gv);
-      if (WE) acc[0] += es;
+      if (WE) acc[0] += (double)es;
       if (WG) {
-        const size_t base = (plane + p) * (size_t)nf;
-#pragma unroll
-        for (int k = 0; k < NK; k++) {
-          const TV gs = (TV)gv[k];
-          g[base + k] = gs;
-          acc[1] += (double)gs * (double)gs;
+        for (int k = 0; k < nf; k++)
+          if (a.fixmask & (1u << k)) gv[k] = (T)0;
+        if (a.density == kDensityHard && a.nvar > 0) {
+          T comp = (T)0;
+          for (int k = 0; k < nf; k++) comp = comp + gv[k];
+          comp = comp * PFC<T>::inv_nvar(a);
+          for (int k = 0; k < nf; k++)
+            if (!(a.fixmask & (1u << k))) gv[k] = gv[k] - comp;
         }
+        const size_t base = (pl + p) * (size_t)nf;
+        T gacc = (T)0;
+        for (int k = 0; k < nf; k++) {
+          g[base + k] = gv[k];
+          gacc = fmad(gv[k], gv[k], gacc);
+        }
+        acc[1] += (double)gacc;
       }
-#pragma unroll
-      for (int k = 0; k < NK; k++) {
-        col[0][k] = col[1][k];
-        col[1][k] = col[2][k];
+      for (int k = 0; k < nf; k++) {
+        cm[k] = c[k];
+        c[k] = cp[k];
       }
     }
   }
   grid_sum_store<2>(acc, rs);
 }
 
+// ---------------------------------------------------------------- constraints
 // Hard density projection / fixed-fluid zeroing of an arbitrary vector
 // (Potential::applyConstraints -> PhaseField::specialisedConstraints).
 template <typename T, int NF>
 __global__ __launch_bounds__(kBlock) void pf_constrain_kernel(PFArgs a, T* __restrict__ v, size_t nsite) {
-  constexpr int M = NFM<NF>::value;
+  constexpr int M = NF > 0 ? NF : kPFMaxFluid;
   const int nf = NF > 0 ? NF : a.nf;
   const int NK = NF > 0 ? NF : nf;
   const T inv_nvar = (T)a.inv_nvar;
@@ -270,42 +465,94 @@ __global__ __launch_bounds__(kBlock) void pf_constrain_kernel(PFArgs a, T* __res
   }
 }
 
-// Planes per march step: enough HBM loads in flight per thread while keeping
-// the register file within reach of 4+ waves per SIMD.
-template <typename T, int NF>
-constexpr int march_unroll() {
-  return NF == 0 ? 1 : (sizeof(T) * NF <= 12 ? 4 : (sizeof(T) * NF <= 24 ? 2 : 1));
+// ---------------------------------------------------------------- launches
+constexpr int kTileTargetBlocks = 512;  // two blocks of 4 waves per CU
+
+int density_mode(const PFArgs& a) {
+  if (a.nf > 1 && a.density == kDensityHard && a.nvar > 0) return DM_HARD;
+  if (a.nf > 1 && a.density == kDensitySoft) return DM_SOFT;
+  return DM_NONE;
 }
 
 template <typename T, typename TV, int NF>
-void launch_eval_nf(const PFArgs& a, int mode, const void* x, const void* d, double t, void* g,
-                    RedScratch rs, hipStream_t s, int nblocks) {
-  constexpr int U = march_unroll<T, NF>();
-  const T* xx = static_cast<const T*>(x);
-  const TV* dd = static_cast<const TV*>(d);
-  TV* gg = static_cast<TV*>(g);
-  const T tt = (T)t;
-  switch (mode) {
-    case 1: pf_kernel<T, TV, NF, 1, U><<<nblocks, kBlock, 0, s>>>(a, xx, dd, tt, gg, rs); break;
-    case 5: pf_kernel<T, TV, NF, 5, U><<<nblocks, kBlock, 0, s>>>(a, xx, dd, tt, gg, rs); break;
-    case 2: pf_kernel<T, TV, NF, 2, U><<<nblocks, kBlock, 0, s>>>(a, xx, dd, tt, gg, rs); break;
-    case 3: pf_kernel<T, TV, NF, 3, U><<<nblocks, kBlock, 0, s>>>(a, xx, dd, tt, gg, rs); break;
-    default: throw std::invalid_argument("pf: unsupported evaluation mode");
+TileGeom tile_geom_for(const PFArgs& a) {
+  TileGeom G;
+  const int target = a.target_blocks > 0 ? a.target_blocks : kTileTargetBlocks;
+  if (!tile_geom(a.X, a.Y, a.Z, tile_rows<T, NF>(), a.xchunk_override, target, kMaxRedBlocks, G))
+    throw std::invalid_argument("PhaseField: grid too large for one launch");
+  if ((double)a.Y * a.Z * NF * sizeof(T) >= 4294967296.0)
+    throw std::invalid_argument("PhaseField: one x plane of the grid exceeds 4 GiB");
+  return G;
+}
+
+template <typename T, typename TV, int NF, int DM, int MODE>
+void launch_tile(const PFArgs& a, const TileGeom& G, const void* x, const void* d, double t, void* g, RedScratch rs,
+                 hipStream_t s) {
+  constexpr int R = tile_rows<T, NF>();
+  // without a direction d is never read; without a gradient g is never written
+  const TV* dd = static_cast<const TV*>((MODE & 4) ? d : x);
+  TV* gg = static_cast<TV*>((MODE & 2) ? g : const_cast<void*>(x));
+  pf_tile_kernel<T, TV, NF, DM, MODE, R><<<G.nblocks, kBlock, 0, s>>>(a, G, static_cast<const T*>(x), dd, (T)t, gg,
+                                                                      rs);
+}
+
+template <typename T, typename TV, int NF, int MODE>
+void launch_tile_dm(const PFArgs& a, const TileGeom& G, const void* x, const void* d, double t, void* g,
+                    RedScratch rs, hipStream_t s) {
+  const int dm = density_mode(a);
+  if constexpr (NF == 1) {
+    launch_tile<T, TV, NF, DM_NONE, MODE>(a, G, x, d, t, g, rs, s);
+  } else {
+    if (dm == DM_SOFT) {
+      launch_tile<T, TV, NF, DM_SOFT, MODE>(a, G, x, d, t, g, rs, s);
+    } else if constexpr ((MODE & 2) != 0) {
+      if (dm == DM_HARD) launch_tile<T, TV, NF, DM_HARD, MODE>(a, G, x, d, t, g, rs, s);
+      else launch_tile<T, TV, NF, DM_NONE, MODE>(a, G, x, d, t, g, rs, s);
+    } else {
+      launch_tile<T, TV, NF, DM_NONE, MODE>(a, G, x, d, t, g, rs, s);  // the projection acts on g only
+    }
   }
 }
 
+template <typename T, typename TV, int NF>
+int launch_tile_nf(const PFArgs& a, int mode, const void* x, const void* d, double t, void* g, RedScratch rs,
+                   hipStream_t s) {
+  const TileGeom G = tile_geom_for<T, TV, NF>(a);
+  switch (mode) {
+    case 1: launch_tile_dm<T, T, NF, 1>(a, G, x, d, t, g, rs, s); break;  // no vector is touched
+    case 2: launch_tile_dm<T, TV, NF, 2>(a, G, x, d, t, g, rs, s); break;
+    case 3: launch_tile_dm<T, TV, NF, 3>(a, G, x, d, t, g, rs, s); break;
+    case 5: launch_tile_dm<T, TV, NF, 5>(a, G, x, d, t, g, rs, s); break;
+    default: throw std::invalid_argument("pf: unsupported evaluation mode");
+  }
+  return G.nblocks;
+}
+
 template <typename T, typename TV>
-void launch_eval_t(const PFArgs& a, int mode, const void* x, const void* d, double t, void* g,
-                   RedScratch rs, hipStream_t s, int nblocks) {
+int launch_eval_t(const PFArgs& a, int mode, const void* x, const void* d, double t, void* g, RedScratch rs,
+                  hipStream_t s) {
   switch (a.nf) {
-    case 1: launch_eval_nf<T, TV, 1>(a, mode, x, d, t, g, rs, s, nblocks); break;
-    case 2: launch_eval_nf<T, TV, 2>(a, mode, x, d, t, g, rs, s, nblocks); break;
-    case 3: launch_eval_nf<T, TV, 3>(a, mode, x, d, t, g, rs, s, nblocks); break;
-    case 4: launch_eval_nf<T, TV, 4>(a, mode, x, d, t, g, rs, s, nblocks); break;
-    default:
-      if (sizeof(T) != sizeof(TV)) throw std::invalid_argument("PhaseField: mixed precision needs nFluid <= 4");
-      launch_eval_nf<T, T, 0>(a, mode, x, d, t, g, rs, s, nblocks);
-      break;
+    case 1: return launch_tile_nf<T, TV, 1>(a, mode, x, d, t, g, rs, s);
+    case 2: return launch_tile_nf<T, TV, 2>(a, mode, x, d, t, g, rs, s);
+    case 3: return launch_tile_nf<T, TV, 3>(a, mode, x, d, t, g, rs, s);
+    case 4: return launch_tile_nf<T, TV, 4>(a, mode, x, d, t, g, rs, s);
+    default: break;
+  }
+  if constexpr (sizeof(T) != sizeof(TV)) {
+    throw std::invalid_argument("PhaseField: mixed precision needs nFluid <= 4");
+  } else {
+    const T* xx = static_cast<const T*>(x);
+    const T* dd = static_cast<const T*>(d);
+    T* gg = static_cast<T*>(g);
+    const T tt = (T)t;
+    switch (mode) {
+      case 1: pf_generic_kernel<T, 1><<<a.nblocks, kBlock, 0, s>>>(a, xx, dd, tt, gg, rs); break;
+      case 2: pf_generic_kernel<T, 2><<<a.nblocks, kBlock, 0, s>>>(a, xx, dd, tt, gg, rs); break;
+      case 3: pf_generic_kernel<T, 3><<<a.nblocks, kBlock, 0, s>>>(a, xx, dd, tt, gg, rs); break;
+      case 5: pf_generic_kernel<T, 5><<<a.nblocks, kBlock, 0, s>>>(a, xx, dd, tt, gg, rs); break;
+      default: throw std::invalid_argument("pf: unsupported evaluation mode");
+    }
+    return a.nblocks;
   }
 }
 
@@ -329,8 +576,21 @@ void pf_args_finalize(PFArgs& a, int target_blocks) {
   a.actX = a.X > 1;
   a.actY = a.Y > 1;
   a.actZ = a.Z > 1;
-  // (y,z) tile of one block: 64 z x 4 y when rows are long enough, else whole
-  // rows stacked up to the block size
+  a.target_blocks = target_blocks;
+  for (int k = 0; k < kPFMaxFluid; k++) {
+    a.fbg[k] = (a.nf == 1 ? 4.0 : 2.0) * a.fb[k];
+    a.fg2[k] = 2.0 * a.fg[k];
+    a.fb_f[k] = (float)a.fb[k];
+    a.fbg_f[k] = (float)a.fbg[k];
+    a.fg_f[k] = (float)a.fg[k];
+    a.fg2_f[k] = (float)a.fg2[k];
+  }
+  a.coef2 = 2.0 * a.coef;
+  a.coef_f = (float)a.coef;
+  a.coef2_f = (float)a.coef2;
+  a.inv_nvar_f = (float)a.inv_nvar;
+  // launch shape of the generic kernel: (y,z) tiles of 64 z x 4 y (or whole
+  // rows stacked up to the block size), each marching over an x chunk
   a.TZ = a.Z >= 64 ? 64 : a.Z;
   a.TY = kBlock / a.TZ;
   if (a.TY > a.Y) a.TY = a.Y;
@@ -339,8 +599,6 @@ void pf_args_finalize(PFArgs& a, int target_blocks) {
   a.ntile = a.ntz * a.nty;
   int xchunk = a.xchunk_override > 0 ? a.xchunk_override : 1;
   if (a.xchunk_override <= 0) {
-    // Longest x-march that still gives ~target_blocks blocks (a chunk re-reads
-    // two halo planes, so longer chunks cost less traffic).
     const long want = target_blocks > 0 ? target_blocks : 2048;
     long nxc = (want + a.ntile - 1) / a.ntile;
     if (nxc < 1) nxc = 1;
@@ -351,8 +609,9 @@ void pf_args_finalize(PFArgs& a, int target_blocks) {
   if (xchunk > a.X) xchunk = a.X;
   a.xchunk = xchunk;
   a.nxc = (a.X + xchunk - 1) / xchunk;
-  while ((long)a.ntile * a.nxc > kMaxRedBlocks) {
+  while ((long)a.ntile * a.nxc > kMaxRedBlocks && a.xchunk < a.X) {
     a.xchunk *= 2;
+    if (a.xchunk > a.X) a.xchunk = a.X;
     a.nxc = (a.X + a.xchunk - 1) / a.xchunk;
   }
   if ((long)a.ntile * a.nxc > kMaxRedBlocks) throw std::invalid_argument("PhaseField: grid too large for one launch");
@@ -361,16 +620,17 @@ void pf_args_finalize(PFArgs& a, int target_blocks) {
 
 void pf_launch_eval(const PFArgs& a, int prec, int pv, int mode, const void* x, const void* d, double t,
                     void* g, RedScratch rs, hipStream_t s) {
+  int nb;
   if (prec == F64 && pv == F64)
-    launch_eval_t<double, double>(a, mode, x, d, t, g, rs, s, a.nblocks);
+    nb = launch_eval_t<double, double>(a, mode, x, d, t, g, rs, s);
   else if (prec == F32 && pv == F32)
-    launch_eval_t<float, float>(a, mode, x, d, t, g, rs, s, a.nblocks);
+    nb = launch_eval_t<float, float>(a, mode, x, d, t, g, rs, s);
   else if (prec == F64 && pv == F32)
-    launch_eval_t<double, float>(a, mode, x, d, t, g, rs, s, a.nblocks);
+    nb = launch_eval_t<double, float>(a, mode, x, d, t, g, rs, s);
   else
     throw std::invalid_argument("PhaseField: FP32 coordinates with FP64 vectors are not supported");
   MINIM_HIP_CHECK(hipGetLastError());
-  launch_finalize(rs, 2, a.nblocks, s);
+  launch_finalize(rs, 2, nb, s);
 }
 
 void pf_launch_constrain(const PFArgs& a, int prec, void* v, size_t nsite, hipStream_t s) {

@@ -16,6 +16,11 @@ struct PFArgs {
   int nf;
   double fb[kPFMaxFluid];  // bulk factor per fluid (kappa/16 v, or 0.5 kappa v)
   double fg[kPFMaxFluid];  // gradient factor per fluid ((0.25|0.5) kappa' v / res^2)
+  // derived constants of the tiled kernel, FP64 and rounded to FP32:
+  // fbg = 4 fb (one fluid) or 2 fb, fg2 = 2 fg, coef2 = 2 coef
+  double fbg[kPFMaxFluid], fg2[kPFMaxFluid], coef2;
+  float fb_f[kPFMaxFluid], fbg_f[kPFMaxFluid], fg_f[kPFMaxFluid], fg2_f[kPFMaxFluid];
+  float coef_f, coef2_f, inv_nvar_f;
   unsigned fixmask;        // bit k set: fluid k fixed (gradient zeroed)
   int density;             // kDensity*
   double coef;             // soft-constraint coefficient
@@ -26,9 +31,11 @@ struct PFArgs {
   int TZ, TY, ntz, nty, ntile;
   int xchunk, nxc, nblocks;
   int xchunk_override;     // >0 forces the x-march length (autotuning)
+  int target_blocks;       // >0 overrides the launch-size heuristic
 };
 
-// Fills YZ, act*, xchunk, nxc, nblocks from X, Y, Z.
+// Fills YZ, act*, the derived constants and the launch shape of the generic
+// kernel from X, Y, Z, nf, fb, fg, coef, inv_nvar.
 void pf_args_finalize(PFArgs& a, int target_blocks);
 
 // mode bits: 1 energy, 2 gradient (plus g.g), 4 evaluate at x + t*d.

@@ -135,7 +135,7 @@ MIXED_SHAPES = [
     ((12, 10, 9), 1, [1.0], [1.5], "none"),
     ((11, 1, 13), 2, [1.0], [2.0], "hard"),
     ((20, 18, 16), 3, [1.0, 0.8, 1.2], [2.0], "hard"),
-    ((8, 8, 8), 4, [1.0, 0.9, 1.1, 0.8, 1.2, 0.7], [1.3], "soft"),
+    ((8, 8, 8), 4, [1.0, 0.9, 1.1, 0.8], [1.3], "soft"),
     ((5, 6, 7), 3, [1.0, 0.8, 1.2], [2.0], "fixed"),
 ]
 
