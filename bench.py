@@ -184,20 +184,27 @@ def main():
         print(json.dumps(res), flush=True)
         return
 
+    # whole evaluation (stencil kernel + cross-block reduction + energy copy) per step
     ms32 = time_evals(m, h, "fp32", x32, g32, e_dev, n, args.steps, args.warmup, stream)
-    ms32k = time_evals(m, h, "fp32", x32, g32, e_dev, n, args.steps, 1, stream, with_energy_copy=False)
     ms64 = time_evals(m, h, "fp64", x64, g64, e_dev, n, args.steps, args.warmup, stream)
-    ms64k = time_evals(m, h, "fp64", x64, g64, e_dev, n, args.steps, 1, stream, with_energy_copy=False)
+    # the stencil kernel alone: HIP events around back-to-back launches on the stream
+    ms32k = m.potential_time_kernel(h, "fp32", x32.data_ptr(), n, g_ptr=g32.data_ptr(), reps=args.steps,
+                                    stream=stream.cuda_stream)
+    ms64k = m.potential_time_kernel(h, "fp64", x64.data_ptr(), n, g_ptr=g64.data_ptr(), reps=args.steps,
+                                    stream=stream.cuda_stream)
     log("E+G fp32 %.3f ms (kernel %.3f), fp64 %.3f ms (kernel %.3f)" % (ms32, ms32k, ms64, ms64k))
     if args.mode == "stencil":
-        # calibration kernel for the byte counters: a plain copy of 256 MiB
-        a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        # calibration kernel for the byte counters: a plain copy of 1 GiB into
+        # another 1 GiB (far beyond the on-chip caches), 1 GiB read + 1 GiB
+        # written per launch
+        a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)
         for _ in range(3):
             b.copy_(a)
         torch.cuda.synchronize()
-        ms_e = time_evals(m, h, "fp32", x32, None, e_dev, n, args.steps, 1, stream)
-        log("E-only fp32 %.3f ms" % ms_e)
+        del a, b
+        ms_e = m.potential_time_kernel(h, "fp32", x32.data_ptr(), n, reps=args.steps, stream=stream.cuda_stream)
+        log("E-only fp32 kernel %.3f ms" % ms_e)
         print(json.dumps({"fp32_eg_ms": ms32k, "fp64_eg_ms": ms64k, "fp32_e_ms": ms_e}), flush=True)
         return
 

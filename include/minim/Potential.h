@@ -63,6 +63,11 @@ class Potential {
   // true from supportsMixed(); the default evaluateMixed throws
   // std::invalid_argument.
   virtual bool supportsMixed() const { return false; }
+  // Benchmark support: average duration (ms, HIP events on the stream) of
+  // reps back-to-back launches of the evaluation kernel alone, without the
+  // cross-block reduction that follows it.  The default throws
+  // std::invalid_argument.
+  virtual double timeKernel(int prec, const void* x, bool wantE, void* g, int reps, void* stream) const;
   virtual void evaluateMixed(const void* x, const void* d, double t, double* e, void* g, void* stream) const;
   // Potential::applyConstraints (src/Potential.cpp:102-115) on a device vector.
   virtual void applyConstraints(int prec, void* v, size_t n, void* stream) const;

@@ -122,6 +122,12 @@ int minim_potential_eval_async(const minim_potential* pot, int precision, const 
  * (PhaseField with nFluid <= 4 has it). */
 int minim_potential_eval_mixed(const minim_potential* pot, const void* x, size_t ndof, const void* d, double t,
                                double* e, void* g, void* stream);
+/* Benchmark support (no reference counterpart): average duration in ms,
+ * measured with HIP events on stream, of reps back-to-back launches of the
+ * evaluation kernel alone (energy when want_energy, gradient into g when g
+ * != NULL), without the cross-block reduction.  PhaseField only. */
+int minim_potential_time_kernel(const minim_potential* pot, int precision, const void* x, size_t ndof,
+                                int want_energy, void* g, int reps, void* stream, double* ms_per_launch);
 /* Potential::applyConstraints on a device vector (src/Potential.cpp:102-115,
  * PhaseField::specialisedConstraints PhaseField.cpp:580-612). */
 int minim_potential_constrain(const minim_potential* pot, int precision, void* v, size_t n, void* stream);

@@ -650,5 +650,13 @@ def potential_eval_mixed(handle, x_ptr, ndof, g_ptr=None, want_energy=True, d_pt
     return e.value if want_energy else None
 
 
+def potential_time_kernel(handle, precision, x_ptr, ndof, g_ptr=None, want_energy=True, reps=20, stream=None):
+    """Average ms per launch of the evaluation kernel alone (HIP events)."""
+    ms = C.c_double()
+    A.check(A.load().minim_potential_time_kernel(handle.ptr, _PREC[precision], x_ptr, ndof, 1 if want_energy else 0,
+                                                 g_ptr, int(reps), stream, C.byref(ms)))
+    return ms.value
+
+
 def potential_constrain(handle, precision, v_ptr, n, stream=None):
     A.check(A.load().minim_potential_constrain(handle.ptr, _PREC[precision], v_ptr, n, stream))

@@ -254,6 +254,19 @@ int minim_potential_eval_mixed(const minim_potential* pot, const void* x, size_t
   });
 }
 
+int minim_potential_time_kernel(const minim_potential* pot, int precision, const void* x, size_t ndof,
+                                int want_energy, void* g, int reps, void* stream, double* ms_per_launch) {
+  return guard([&] {
+    need(pot, "pot");
+    need(x, "x");
+    need(ms_per_launch, "ms_per_launch");
+    check_prec(precision);
+    if (!pot->initialised) throw std::runtime_error("potential not initialised: call minim_potential_init");
+    if (ndof != pot->ndof) throw std::invalid_argument("ndof does not match the initialised potential");
+    *ms_per_launch = pot->p->timeKernel(precision, x, want_energy != 0, g, reps, stream);
+  });
+}
+
 int minim_potential_constrain(const minim_potential* pot, int precision, void* v, size_t n, void* stream) {
   return guard([&] {
     need(pot, "pot");

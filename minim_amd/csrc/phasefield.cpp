@@ -6,6 +6,7 @@
 // stencil of pf.hip.
 #include <cmath>
 #include <stdexcept>
+#include <vector>
 
 #include "minim/PhaseField.h"
 #include "pf.hpp"
@@ -217,6 +218,34 @@ void pf_eval_sync(const PFArgs& a, int px, int pv, const void* x, const void* d,
 void PhaseField::evaluate(int prec, const void* x, const void* d, double t, double* e, void* g,
                           void* stream) const {
   pf_eval_sync(args_of(args_), prec, prec, x, d, t, e, g, stream);
+}
+
+double PhaseField::timeKernel(int prec, const void* x, bool wantE, void* g, int reps, void* stream) const {
+  const PFArgs& a = args_of(args_);
+  const int mode = (wantE ? 1 : 0) | (g ? 2 : 0);
+  if (mode == 0 || reps < 1) throw std::invalid_argument("PhaseField: nothing to time");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  auto& ws = minim_amd::Workspace::current();
+  minim_amd::pf_launch_eval(a, prec, prec, mode, x, nullptr, 0.0, g, ws.rs(), s);  // warm-up
+  // evaluations as the minimisers issue them (stencil, then the reduction),
+  // with a pair of events around each stencil launch
+  std::vector<hipEvent_t> ev(2 * (size_t)reps);
+  for (auto& e : ev) MINIM_HIP_CHECK(hipEventCreate(&e));
+  for (int i = 0; i < reps; i++) {
+    MINIM_HIP_CHECK(hipEventRecord(ev[2 * i], s));
+    minim_amd::pf_launch_eval(a, prec, prec, mode, x, nullptr, 0.0, g, ws.rs(), s, false);
+    MINIM_HIP_CHECK(hipEventRecord(ev[2 * i + 1], s));
+    minim_amd::launch_finalize(ws.rs(), 2, minim_amd::pf_blocks(a, prec, prec, mode), s);
+  }
+  MINIM_HIP_CHECK(hipEventSynchronize(ev.back()));
+  double total = 0;
+  for (int i = 0; i < reps; i++) {
+    float ms = 0;
+    MINIM_HIP_CHECK(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+    total += ms;
+  }
+  for (auto& e : ev) MINIM_HIP_CHECK(hipEventDestroy(e));
+  return total / reps;
 }
 
 void PhaseField::evaluateMixed(const void* x, const void* d, double t, double* e, void* g, void* stream) const {

@@ -45,6 +45,10 @@ void Potential::applyConstraints(int prec, void* v, size_t n, void* stream) cons
                                static_cast<hipStream_t>(stream));
 }
 
+double Potential::timeKernel(int, const void*, bool, void*, int, void*) const {
+  throw std::invalid_argument("Potential: kernel timing is not available for this potential.");
+}
+
 void Potential::evaluateMixed(const void*, const void*, double, double*, void*, void*) const {
   throw std::invalid_argument("Potential: mixed FP64/FP32 storage is not supported by this potential.");
 }
