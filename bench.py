@@ -20,6 +20,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
+sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
 
 METRIC = "energy+grad evals/s and time-to-converge, 256³ 3-fluid PF: mixed vs FP64"
 HBM_PEAK_GBS = 8000.0
@@ -85,6 +86,44 @@ def converge(m, x0, mode, max_iter=20000, repeats=2):
     st = dict(mn.stats)
     st.update(seconds=times[-1], seconds_cold=times[0], initial_energy=e0, final_energy=s.energy())
     return st
+
+
+def cfg1_leg(m, with_cpu=True):
+    """The reference's CPU-runnable case: a 128x128x1 binary field (phi ~ U[-0.1, 0.1]
+    from mt19937_64, sigma = w = 1), L-BFGS m=5 to the reference tolerance.  GPU
+    time-to-converge (FP64 and mixed; warm buffers, result left on the device)
+    beside the reference library's minimise on the host cores."""
+    import torch
+    from mt64 import cfg1_input
+
+    x0 = cfg1_input()
+    pf = m.PhaseField().setNFluid(1).setGridSize([128, 128, 1]).setSurfaceTension(1.0).setInterfaceSize(1.0)
+    out = {}
+    for mode in ("fp64", "mixed"):
+        mn = m.Lbfgs().setM(5).setMaxIter(20000).setPrecision(mode)
+        times = []
+        for r in range(2):
+            s = None
+            s = m.State(pf, x0, "fp64")
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            mn.minimise(s, download=(r == 0))
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t)
+        out[mode] = dict(seconds=times[-1], seconds_cold=times[0], iterations=mn.stats["iterations"],
+                         converged=mn.stats["converged"], energy=s.energy())
+    if with_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS") or min(os.cpu_count() or 1, 16))
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        from oracle_bind import RefState, ref_available
+
+        if ref_available():
+            r = RefState.phasefield((128, 128, 1), x0, 1, 1.0, [1.0], [1.0], "hard")
+            t = time.perf_counter()
+            _, res = r.minimise("lbfgs", m=5, max_iter=20000)
+            out["reference_cpu"] = dict(seconds=time.perf_counter() - t, iterations=res["iterations"],
+                                        energy=res["energy"], cores=threads, kind="reference")
+    return out
 
 
 def cpu_baseline(x, budget_s=12.0):
@@ -208,6 +247,19 @@ def main():
         print(json.dumps({"fp32_eg_ms": ms32k, "fp64_eg_ms": ms64k, "fp32_e_ms": ms_e}), flush=True)
         return
 
+    # PCIe-inclusive rate: State::energyGradient(coords) with host FP64 vectors in
+    # and out, as a caller of the reference's host API would use it
+    host_ms = {}
+    for prec in ("fp32", "fp64"):
+        s = m.State(pf, x0, prec)
+        s.energyGradient(x0)
+        t = time.perf_counter()
+        for _ in range(3):
+            s.energyGradient(x0)
+        host_ms[prec] = (time.perf_counter() - t) / 3 * 1e3
+        del s
+    log("host-boundary E+G (upload + eval + download): fp32 %.1f ms, fp64 %.1f ms" % (host_ms["fp32"], host_ms["fp64"]))
+
     sites = int(np.prod(GRID))
     bytes32 = sites * NF * (4 + 4)  # coordinates read once + gradient written once
     bytes64 = sites * NF * (8 + 8)
@@ -235,6 +287,9 @@ def main():
             tolerance="gradient RMS < 1e-8 (reference convergence for sigma_mean=1, res=1); energies compared "
                       "with the absolute floor 1e-8*E_initial since the minimum is E~0")
 
+    log("cfg1 (128x128x1 binary) time-to-converge ...")
+    cfg1 = cfg1_leg(m, with_cpu=not args.no_cpu)
+    log("  cfg1: " + json.dumps(cfg1))
     cpu = None if args.no_cpu else cpu_baseline(x0, args.cpu_budget)
 
     out = {
@@ -261,7 +316,11 @@ def main():
                      "kernel": "pf_tile_kernel<float,float,3,hard,E+G>", "kernel_ms": ms32k, "algorithmic_bytes": bytes32,
                      "fp64": {"achieved": ach64, "frac": ach64 / HBM_PEAK_GBS, "kernel_ms": ms64k,
                               "algorithmic_bytes": bytes64}},
+        "host_boundary": {"fp32_ms_per_eval": host_ms["fp32"], "fp64_ms_per_eval": host_ms["fp64"],
+                          "what": "State::energyGradient(coords) with host FP64 coordinates in and host FP64 "
+                                  "gradient out (PCIe transfers included); not the headline value"},
         "time_to_converge": ttc,
+        "cfg1_time_to_converge": cfg1,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -633,19 +633,6 @@ void pf_launch_eval(const PFArgs& a, int prec, int pv, int mode, const void* x, 
   if (finalize) launch_finalize(rs, 2, nb, s);
 }
 
-int pf_blocks(const PFArgs& a, int prec, int pv, int mode) {
-  (void)mode;
-  if (a.nf > 4) return a.nblocks;
-  const bool f32 = prec == F32;
-  (void)pv;
-  switch (a.nf) {
-    case 1: return f32 ? tile_geom_for<float, float, 1>(a).nblocks : tile_geom_for<double, double, 1>(a).nblocks;
-    case 2: return f32 ? tile_geom_for<float, float, 2>(a).nblocks : tile_geom_for<double, double, 2>(a).nblocks;
-    case 3: return f32 ? tile_geom_for<float, float, 3>(a).nblocks : tile_geom_for<double, double, 3>(a).nblocks;
-    default: return f32 ? tile_geom_for<float, float, 4>(a).nblocks : tile_geom_for<double, double, 4>(a).nblocks;
-  }
-}
-
 void pf_launch_constrain(const PFArgs& a, int prec, void* v, size_t nsite, hipStream_t s) {
   if (prec == F64)
     launch_constrain_t<double>(a, v, nsite, s);

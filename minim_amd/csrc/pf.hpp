@@ -44,7 +44,5 @@ void pf_args_finalize(PFArgs& a, int target_blocks);
 void pf_launch_eval(const PFArgs& a, int prec, int pv, int mode, const void* x, const void* d, double t,
                     void* g, RedScratch rs, hipStream_t s, bool finalize = true);
 void pf_launch_constrain(const PFArgs& a, int prec, void* v, size_t nsite, hipStream_t s);
-// Number of blocks (partials to reduce) of the evaluation launch.
-int pf_blocks(const PFArgs& a, int prec, int pv, int mode);
 
 }  // namespace minim_amd

@@ -226,26 +226,19 @@ double PhaseField::timeKernel(int prec, const void* x, bool wantE, void* g, int 
   if (mode == 0 || reps < 1) throw std::invalid_argument("PhaseField: nothing to time");
   hipStream_t s = static_cast<hipStream_t>(stream);
   auto& ws = minim_amd::Workspace::current();
-  minim_amd::pf_launch_eval(a, prec, prec, mode, x, nullptr, 0.0, g, ws.rs(), s);  // warm-up
-  // evaluations as the minimisers issue them (stencil, then the reduction),
-  // with a pair of events around each stencil launch
-  std::vector<hipEvent_t> ev(2 * (size_t)reps);
-  for (auto& e : ev) MINIM_HIP_CHECK(hipEventCreate(&e));
-  for (int i = 0; i < reps; i++) {
-    MINIM_HIP_CHECK(hipEventRecord(ev[2 * i], s));
-    minim_amd::pf_launch_eval(a, prec, prec, mode, x, nullptr, 0.0, g, ws.rs(), s, false);
-    MINIM_HIP_CHECK(hipEventRecord(ev[2 * i + 1], s));
-    minim_amd::launch_finalize(ws.rs(), 2, minim_amd::pf_blocks(a, prec, prec, mode), s);
-  }
-  MINIM_HIP_CHECK(hipEventSynchronize(ev.back()));
-  double total = 0;
-  for (int i = 0; i < reps; i++) {
-    float ms = 0;
-    MINIM_HIP_CHECK(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
-    total += ms;
-  }
-  for (auto& e : ev) MINIM_HIP_CHECK(hipEventDestroy(e));
-  return total / reps;
+  minim_amd::pf_launch_eval(a, prec, prec, mode, x, nullptr, 0.0, g, ws.rs(), s, false);  // warm-up
+  hipEvent_t e0, e1;
+  MINIM_HIP_CHECK(hipEventCreate(&e0));
+  MINIM_HIP_CHECK(hipEventCreate(&e1));
+  MINIM_HIP_CHECK(hipEventRecord(e0, s));
+  for (int i = 0; i < reps; i++) minim_amd::pf_launch_eval(a, prec, prec, mode, x, nullptr, 0.0, g, ws.rs(), s, false);
+  MINIM_HIP_CHECK(hipEventRecord(e1, s));
+  MINIM_HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0;
+  MINIM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  MINIM_HIP_CHECK(hipEventDestroy(e0));
+  MINIM_HIP_CHECK(hipEventDestroy(e1));
+  return (double)ms / reps;
 }
 
 void PhaseField::evaluateMixed(const void* x, const void* d, double t, double* e, void* g, void* stream) const {

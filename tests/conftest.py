@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 sys.path.insert(0, HERE)
 sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(HERE, "golden"))
 
 
 def pytest_configure(config):

@@ -17,6 +17,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, HERE)
 from oracle_bind import RefState, ref_available  # noqa: E402
 
 
@@ -161,12 +162,30 @@ def bah_cases():
     return out
 
 
+def cfg1_cases():
+    """The benchmark's CPU-runnable case: 128x128x1 binary field, phi ~ U[-0.1, 0.1]
+    from mt19937_64 (mt64.cfg1_input), sigma = w = 1, L-BFGS m=5 to the
+    reference tolerance.  The inputs are regenerated from the seed, so only
+    the results are stored."""
+    from mt64 import cfg1_input
+
+    x0 = cfg1_input()
+    r = RefState.phasefield((128, 128, 1), x0, 1, 1.0, [1.0], [1.0], "hard")
+    e0 = r.energy(x0)
+    x, res = r.minimise("lbfgs", m=5, max_iter=20000)
+    return [dict(name="cfg1-128x128-binary", grid=[128, 128, 1], nfluid=1, surface_tension=[1.0],
+                 interface_size=[1.0], seed=20240, lo=-0.1, hi=0.1, m=5, max_iter=20000, initial_energy=e0,
+                 x_sum=float(x.sum()), x_sumsq=float((x * x).sum()), x_min=float(x.min()), x_max=float(x.max()),
+                 **res)]
+
+
 def main():
     if not ref_available():
         sys.exit("oracle/_ref/libminim_ref.so is not built (make -C oracle)")
     if os.environ.get("OMP_NUM_THREADS") != "1":
         sys.exit("run with OMP_NUM_THREADS=1")
-    data = dict(pf=pf_cases(), lbfgs=lbfgs_cases(), fire_gd=fire_gd_cases(), lj=lj_cases(), bah=bah_cases())
+    data = dict(pf=pf_cases(), lbfgs=lbfgs_cases(), fire_gd=fire_gd_cases(), lj=lj_cases(), bah=bah_cases(),
+                cfg1=cfg1_cases())
     for k, v in data.items():
         with open(os.path.join(HERE, "ref_%s.json" % k), "w") as f:
             json.dump(v, f)

@@ -166,6 +166,27 @@ def test_log_minimise(gpu, capfd):
     assert len(lines) >= 2 and "E:" in lines[0] and "G:" in lines[0]
 
 
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_cfg1_lbfgs_vs_reference(gpu, mode):
+    """The 128x128x1 binary benchmark case (phi ~ U[-0.1, 0.1] from mt19937_64):
+    the device L-BFGS reaches the reference's converged state."""
+    from mt64 import uniform
+
+    m = gpu
+    c = golden("cfg1")[0]
+    x0 = uniform(c["seed"], int(np.prod(c["grid"])), c["lo"], c["hi"])
+    s = m.State(make_pf(m, c["grid"], 1, c["surface_tension"], c["interface_size"]), x0)
+    assert abs(s.energy() - c["initial_energy"]) <= 1e-12 * c["initial_energy"]
+    mn = m.Lbfgs().setM(c["m"]).setMaxIter(c["max_iter"]).setPrecision(mode)
+    x = mn.minimise(s)
+    st = mn.stats
+    print(mode, st, flush=True)
+    assert st["converged"] and st["grad_rms"] < s.convergence
+    assert energy_ok(s.energy(), c["energy"], c["initial_energy"])
+    assert abs(float(x.sum()) - c["x_sum"]) <= 1e-6 * abs(c["x_sum"])
+    assert abs(st["iterations"] - c["iterations"]) <= max(10, 0.3 * c["iterations"])
+
+
 def test_lbfgs_vs_oracle_ternary_medium(gpu):
     """A 24^3 ternary field (41k DOF): converged energy vs the oracle's FP64 L-BFGS."""
     m = gpu

@@ -139,6 +139,37 @@ def test_lbfgs_golden_trajectory(case):
     assert st["energy"] == c["energy"]
 
 
+def test_mt19937_64_known_answers():
+    """The input generator reproduces C++ std::mt19937_64 (the standard's
+    required 10000th output for seed 5489, and libstdc++'s first outputs for
+    the benchmark seed)."""
+    from mt64 import MT19937_64, cfg1_input
+
+    g = MT19937_64(5489)
+    v = [g.next() for _ in range(10000)]
+    assert v[-1] == 9981545732273789042
+    h = MT19937_64(20240)
+    assert [h.next() for _ in range(3)] == [2090563417169947710, 6791348416396944384, 8999006810243764886]
+    x = cfg1_input()
+    assert x.shape == (16384,) and x.min() >= -0.1 and x.max() < 0.1
+
+
+def test_cfg1_lbfgs_golden():
+    """128x128x1 binary field, L-BFGS m=5 to the reference tolerance: the
+    oracle reproduces the reference's run bit for bit."""
+    from mt64 import uniform
+
+    c = golden("cfg1")[0]
+    x0 = uniform(c["seed"], int(np.prod(c["grid"])), c["lo"], c["hi"])
+    o = OraclePF(tuple(c["grid"]), 1, 1.0, c["surface_tension"], c["interface_size"])
+    assert o.energy(x0) == c["initial_energy"]
+    x, st = oracle_lbfgs(o, x0, m=c["m"], max_iter=c["max_iter"])
+    assert st["iterations"] == c["iterations"] and st["converged"]
+    assert st["n_energy"] == c["n_energy"] and st["n_gradient"] == c["n_gradient"]
+    assert st["energy"] == c["energy"]
+    assert float(x.sum()) == c["x_sum"]
+
+
 @pytest.mark.parametrize("case", range(4))
 def test_fire_graddescent_golden_trajectory(case):
     c = golden("fire_gd")[case]
