@@ -225,7 +225,6 @@ __global__ __launch_bounds__(kBlock) void pf_tile_kernel(PFArgs a, TileGeom G, c
   tile_thread(G, xcd_logical_block(blockIdx.x, gridDim.x), threadIdx.x, th);
   const size_t plane = (size_t)G.Y * G.Z * NF;  // elements per x plane
   const int X = G.X;
-  auto wrapx = [X](int v) { return v >= X ? v - X : (v < 0 ? v + X : v); };
   unsigned ox[R], od[R];
 #pragma unroll
   for (int r = 0; r < R; r++) {
@@ -240,7 +239,8 @@ __global__ __launch_bounds__(kBlock) void pf_tile_kernel(PFArgs a, TileGeom G, c
   T A[R][NF], B[R][NF], Cn[R][NF];  // planes i-1, i, i+1 of the own rows
   {
     PlaneIn<T, TV, NF, R, DIR> pm, p0, p1;
-    const size_t sm = (size_t)wrapx(x0 - 1) * plane, s0 = (size_t)x0 * plane, s1 = (size_t)wrapx(x0 + 1) * plane;
+    const size_t sm = (size_t)tile_plane(x0 - 1, X) * plane, s0 = (size_t)tile_plane(x0, X) * plane,
+                 s1 = (size_t)tile_plane(x0 + 1, X) * plane;
     load_plane<T, TV, NF, R, DIR>(x + sm, d + sm, ox, od, hox, hod, pm);
     load_plane<T, TV, NF, R, DIR>(x + s0, d + s0, ox, od, hox, hod, p0);
     load_plane<T, TV, NF, R, DIR>(x + s1, d + s1, ox, od, hox, hod, p1);
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(kBlock) void pf_tile_kernel(PFArgs a, TileGeom G, c
     const int b = (i - x0) & 1;
     // plane i+2 in flight while plane i is computed (wrapped: always a valid plane)
     PlaneIn<T, TV, NF, R, DIR> q;
-    const size_t sq = (size_t)wrapx(i + 2) * plane;
+    const size_t sq = (size_t)tile_plane(i + 2, X) * plane;
     load_plane<T, TV, NF, R, DIR>(x + sq, d + sq, ox, od, hox, hod, q);
 
     TV* gp = g + (size_t)i * plane;

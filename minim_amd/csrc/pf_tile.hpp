@@ -53,6 +53,19 @@ MINIM_HD inline int wrap_mod(int v, int n) {
   return r < 0 ? r + n : r;
 }
 
+// x plane of a (possibly out-of-range) plane index v, periodic in X.  The
+// kernel reads planes x0-1 .. x1+1 of its chunk, so v lies in [-1, X+1]; with
+// X = 1 or 2 that is more than one period away.
+MINIM_HD inline int tile_plane(int v, int X) { return wrap_mod(v, X); }
+
+// Every x plane the tiled kernel reads for chunk [x0, x1): the three planes
+// x0-1, x0, x0+1 of its prologue and plane i+2 fetched while plane i is
+// computed, for i in [x0, x1).
+MINIM_HD inline int tile_planes_read(int x0, int x1) { return 3 + (x1 - x0); }
+MINIM_HD inline int tile_plane_read(int x0, int k, int X) {
+  return k < 3 ? tile_plane(x0 - 1 + k, X) : tile_plane(x0 + 2 + (k - 3), X);
+}
+
 // L: logical block index (x-chunk major), t: thread index in the block.
 MINIM_HD inline void tile_thread(const TileGeom& G, int L, int t, TileThread& o) {
   const int xc = L / G.ntile, tile = L - xc * G.ntile;

@@ -3,6 +3,7 @@
 //
 // For every grid shape and rows-per-thread value below it replays the
 // thread -> site mapping of every block and checks that
+//  * every x plane the block reads (tile_plane_read) lies inside the grid;
 //  * every site of every x plane is owned (stored and summed) exactly once;
 //  * every in-plane index a thread loads lies inside the grid;
 //  * every LDS cell a thread reads for its y/z neighbours was filled, for the
@@ -35,6 +36,20 @@ static int check_shape(int X, int Y, int Z, int R, int xchunk, int target) {
       if (o.x0 < 0 || o.x1 > X || o.x0 >= o.x1) {
         std::printf("block %d: bad x range [%d,%d)\n", L, o.x0, o.x1);
         return 1;
+      }
+      // every plane the kernel reads is a plane of the grid, and the planes
+      // around each computed plane are its periodic x neighbours
+      for (int k = 0; k < tile_planes_read(o.x0, o.x1); k++) {
+        const int p = tile_plane_read(o.x0, k, X);
+        if (p < 0 || p >= X) {
+          if (bad < 5) std::printf("shape %dx%dx%d block %d: plane %d read (k=%d)\n", X, Y, Z, L, p, k);
+          bad++;
+        }
+      }
+      for (int i = o.x0; i < o.x1; i++) {
+        if (tile_plane(i - 1, X) != (i - 1 + X) % X || tile_plane(i + 1, X) != (i + 1) % X ||
+            tile_plane(i + 2, X) != (i + 2) % X)
+          bad++;
       }
       for (int r = 0; r < R; r++) {
         if ((long)o.site[r] >= nsite) bad++;
@@ -89,6 +104,7 @@ static int check_shape(int X, int Y, int Z, int R, int xchunk, int target) {
 
 int main() {
   const int shapes[][3] = {{1, 1, 1},   {1, 1, 7},    {1, 5, 3},    {3, 1, 1},    {2, 2, 2},    {4, 17, 65},
+                           {1, 128, 128}, {2, 64, 64}, {2, 1, 1},
                            {5, 16, 64}, {7, 33, 130}, {1, 40, 200}, {9, 3, 129},  {6, 64, 64},  {3, 70, 63},
                            {2, 1, 300}, {11, 9, 1},   {4, 100, 5},  {16, 32, 256}};
   int fails = 0, n = 0;
