@@ -22,6 +22,7 @@ struct MinimiseStats {
   int iterFP64 = 0;
   bool converged = false;
   double gradRms = 0;    // sqrt(g.g/ndof) of the last gradient
+  double gradMax = 0;    // Fire with setMaxForceConvergence: largest per-particle force norm
   double energy = 0;     // last accepted energy (FP64 accumulation)
 };
 
@@ -108,12 +109,20 @@ class Fire : public NewMinimiser<Fire> {
   double dtMax = 0;
   Fire& setMaxIter(int maxIter) override;
   Fire& setDtMax(double dtMax);
+  // Options without a reference counterpart (both off reproduce the
+  // reference): the initial time step (0 = dtMax, Fire.cpp:43), and
+  // convergence on the largest per-particle force norm over groups of dim
+  // DOFs (0 = RMS gradient, Fire.cpp:87-90).
+  Fire& setDt0(double dt0);
+  Fire& setMaxForceConvergence(int dim);
 
   void init(State& state) override;
   void iteration(State& state) override;
   bool checkConvergence(const State& state) override;
 
  private:
+  double _dt0 = 0;
+  int _maxForceDim = 0;
   int _nMin = 5;
   double _fInc = 1.1;
   double _fDec = 0.5;

@@ -183,6 +183,9 @@ typedef struct minim_minimiser_params {
   double switch_rms; /* mixed: gradient RMS at which FP32 storage is left (0 = 1e3*convergence) */
   double dt_max;     /* Fire::setDtMax (0 = estimate, Fire.cpp:29-37) */
   double alpha;      /* GradDescent::setAlpha */
+  double dt0;        /* Fire::setDt0: initial time step (0 = dt_max, the reference) */
+  int max_force_dim; /* Fire::setMaxForceConvergence: converge on the largest norm over groups of
+                        this many DOFs (0 = RMS gradient, the reference) */
 } minim_minimiser_params;
 
 typedef struct minim_result {
@@ -196,6 +199,7 @@ typedef struct minim_result {
   double energy;
   double grad_rms;
   double dt_max; /* Fire: dtMax after init */
+  double grad_max; /* Fire with max_force_dim: largest per-group norm of the last gradient */
 } minim_result;
 
 typedef void (*minim_adjust_fn)(int iter, minim_state* state, void* user);

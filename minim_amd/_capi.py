@@ -36,14 +36,15 @@ class BaHParams(C.Structure):
 class MinimiserParams(C.Structure):
     _fields_ = [("kind", C.c_int), ("max_iter", C.c_int), ("linesearch", C.c_int), ("m", C.c_int),
                 ("max_step", C.c_double), ("init_step", C.c_double), ("precision_mode", C.c_int),
-                ("switch_rms", C.c_double), ("dt_max", C.c_double), ("alpha", C.c_double)]
+                ("switch_rms", C.c_double), ("dt_max", C.c_double), ("alpha", C.c_double),
+                ("dt0", C.c_double), ("max_force_dim", C.c_int)]
 
 
 class Result(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("n_energy", C.c_long),
                 ("n_gradient", C.c_long), ("n_energy_reused", C.c_long), ("iter_fp32", C.c_int),
                 ("iter_fp64", C.c_int), ("energy", C.c_double), ("grad_rms", C.c_double),
-                ("dt_max", C.c_double)]
+                ("dt_max", C.c_double), ("grad_max", C.c_double)]
 
 
 ADJUST_FN = C.CFUNCTYPE(None, C.c_int, vp, vp)
@@ -91,6 +92,9 @@ SIGNATURES = {
 _lib = None
 
 
+ABI_VERSION = 2  # layout of the parameter/result structs below
+
+
 def load():
     """Load libminim_amd.so.  There is no fallback: a missing library is an error."""
     global _lib
@@ -103,6 +107,9 @@ def load():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
+        if L.minim_abi_version() != ABI_VERSION:
+            raise ImportError("minim_amd: %s has ABI version %d, this package expects %d; rebuild it"
+                              % (LIB_PATH, L.minim_abi_version(), ABI_VERSION))
         _lib = L
     return _lib
 

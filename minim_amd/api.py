@@ -499,7 +499,8 @@ class Minimiser:
         self.iter = r.iterations
         self.stats = dict(iterations=r.iterations, converged=bool(r.converged), n_energy=r.n_energy,
                           n_gradient=r.n_gradient, n_energy_reused=r.n_energy_reused, iter_fp32=r.iter_fp32,
-                          iter_fp64=r.iter_fp64, energy=r.energy, grad_rms=r.grad_rms, dt_max=r.dt_max)
+                          iter_fp64=r.iter_fp64, energy=r.energy, grad_rms=r.grad_rms, dt_max=r.dt_max,
+                          grad_max=r.grad_max)
 
     def minimise(self, state, adjust=None, download=True):
         """Minimiser::minimise(state, adjustState) / minimise(state, "<fields>-<N>")
@@ -586,9 +587,23 @@ class Fire(Minimiser):
         self._h = None
         return self
 
+    def setDt0(self, v):
+        """Initial time step (no reference counterpart; 0 = dtMax as in Fire.cpp:43)."""
+        self.dt0 = float(v)
+        self._h = None
+        return self
+
+    def setMaxForceConvergence(self, dim):
+        """Converge on the largest per-particle force norm (groups of dim DOFs) instead of the RMS."""
+        self.maxForceDim = int(dim)
+        self._h = None
+        return self
+
     def _params(self):
         p = super()._params()
         p.dt_max = self.dtMax
+        p.dt0 = getattr(self, "dt0", 0.0)
+        p.max_force_dim = getattr(self, "maxForceDim", 0)
         return p
 
     def minimise(self, state, adjust=None, download=True):

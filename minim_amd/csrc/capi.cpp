@@ -70,7 +70,7 @@ void check_prec(int prec) {
 extern "C" {
 
 const char* minim_last_error(void) { return g_err.c_str(); }
-int minim_abi_version(void) { return 1; }
+int minim_abi_version(void) { return 2; }
 
 // ---------------------------------------------------------------- potentials
 int minim_pf_create(const minim_pf_params* p, minim_potential** out) {
@@ -441,6 +441,8 @@ int minim_minimiser_create(const minim_minimiser_params* p, minim_minimiser** ou
     } else if (p->kind == MINIM_FIRE) {
       auto f = std::make_unique<minim::Fire>();
       f->setDtMax(p->dt_max);
+      f->setDt0(p->dt0);
+      f->setMaxForceConvergence(p->max_force_dim);
       m = std::move(f);
     } else if (p->kind == MINIM_GRADDESCENT) {
       auto gd = std::make_unique<minim::GradDescent>();
@@ -475,6 +477,7 @@ static void fill_result(const minim_minimiser* m, minim_result* r) {
   r->grad_rms = st.gradRms;
   auto* f = dynamic_cast<const minim::Fire*>(m->m.get());
   r->dt_max = f ? f->dtMax : 0.0;
+  r->grad_max = st.gradMax;
 }
 
 int minim_minimise(minim_minimiser* m, minim_state* s, minim_adjust_fn fn, void* user, double* coords_out,

@@ -527,7 +527,7 @@ struct Fire::Impl {
   int prec = FP64;
   size_t n = 0;
   DVec g, v, step;
-  double gNorm = 0, a = 0.1, dt = 0;
+  double gNorm = 0, gMax = 0, a = 0.1, dt = 0;
   int nSteps = 0;
   EnergyCache ec;
 };
@@ -535,11 +535,13 @@ struct Fire::Impl {
 Fire::Fire() : impl_(new Impl) {}
 Fire::Fire(double dtMax_) : dtMax(dtMax_), impl_(new Impl) { linesearch = "none"; }  // Fire.cpp:11-13
 Fire::Fire(const Fire& o)
-    : NewMinimiser<Fire>(o), dtMax(o.dtMax), _nMin(o._nMin), _fInc(o._fInc), _fDec(o._fDec), _fA(o._fA),
-      _aStart(o._aStart), impl_(new Impl) {}
+    : NewMinimiser<Fire>(o), dtMax(o.dtMax), _dt0(o._dt0), _maxForceDim(o._maxForceDim), _nMin(o._nMin),
+      _fInc(o._fInc), _fDec(o._fDec), _fA(o._fA), _aStart(o._aStart), impl_(new Impl) {}
 Fire& Fire::operator=(const Fire& o) {
   Minimiser::operator=(o);
   dtMax = o.dtMax;
+  _dt0 = o._dt0;
+  _maxForceDim = o._maxForceDim;
   impl_.reset(new Impl);
   return *this;
 }
@@ -552,6 +554,26 @@ Fire& Fire::setDtMax(double d) {
   dtMax = d;
   return *this;
 }
+Fire& Fire::setDt0(double d) {
+  if (d < 0) throw std::invalid_argument("Fire: dt0 must not be negative.");
+  _dt0 = d;
+  return *this;
+}
+Fire& Fire::setMaxForceConvergence(int dim) {
+  if (dim < 0) throw std::invalid_argument("Fire: the force group size must not be negative.");
+  _maxForceDim = dim;
+  return *this;
+}
+
+namespace {
+double group_norm_max(int prec, const void* v, size_t n, int dim, hipStream_t s) {
+  auto& ws = Workspace::current();
+  minim_amd::launch_group_norm_max(prec, v, n, dim, ws.rs(), s);
+  double r;
+  ws.fetch(s, 1, &r);
+  return r;
+}
+}  // namespace
 
 // Fire::init (Fire.cpp:27-38)
 void Fire::init(State& state) {
@@ -582,7 +604,7 @@ void Fire::iteration(State& state) {
   hipStream_t s = hs(state);
   auto& ws = Workspace::current();
   if (iter == 0) {
-    I.dt = dtMax;
+    I.dt = _dt0 > 0 ? _dt0 : dtMax;
     gradient_into(state, I.prec, I.g.data(), stats);
     I.gNorm = std::sqrt(dot1(I.prec, I.g.data(), I.g.data(), I.n, s));
   }
@@ -627,14 +649,19 @@ void Fire::iteration(State& state) {
   }
   gradient_into(state, I.prec, I.g.data(), stats);
   I.gNorm = std::sqrt(dot1(I.prec, I.g.data(), I.g.data(), I.n, s));
+  if (_maxForceDim > 0) I.gMax = group_norm_max(I.prec, I.g.data(), I.n, _maxForceDim, s);
   if (I.prec == FP32) stats.iterFP32++;
   else stats.iterFP64++;
 }
 
-// Fire::checkConvergence (Fire.cpp:87-90)
+// Fire::checkConvergence (Fire.cpp:87-90), or the largest per-particle force
 bool Fire::checkConvergence(const State& state) {
   double rms = impl_->gNorm / std::sqrt((double)state.ndof);
   stats.gradRms = rms;
+  if (_maxForceDim > 0) {
+    stats.gradMax = impl_->gMax;
+    return impl_->gMax < state.convergence;
+  }
   return rms < state.convergence;
 }
 

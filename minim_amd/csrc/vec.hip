@@ -332,6 +332,45 @@ __global__ __launch_bounds__(kBlock) void fire_kernel(int mode, T* __restrict__ 
   }
 }
 
+// NaN-propagating maximum
+__device__ inline double nan_max(double a, double b) { return (b > a || b != b) ? b : a; }
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void group_norm_max_kernel(const T* __restrict__ v, size_t ngroup, int dim,
+                                                                 double* __restrict__ partial) {
+  double m = 0.0;
+  for (size_t q = (size_t)blockIdx.x * kBlock + threadIdx.x; q < ngroup; q += (size_t)gridDim.x * kBlock) {
+    double s = 0.0;
+    for (int k = 0; k < dim; k++) {
+      const double c = (double)v[q * dim + k];
+      s += c * c;
+    }
+    m = nan_max(m, s);
+  }
+  for (int off = kWave / 2; off > 0; off >>= 1) m = nan_max(m, __shfl_xor(m, off, kWave));
+  __shared__ double lds[kBlock / kWave];
+  if (threadIdx.x % kWave == 0) lds[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / kWave; w++) m = nan_max(m, lds[w]);
+    partial[blockIdx.x] = m;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void max_sqrt_finalize_kernel(const double* __restrict__ partial, int nb,
+                                                                   double* __restrict__ result) {
+  double m = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kBlock) m = nan_max(m, partial[b]);
+  for (int off = kWave / 2; off > 0; off >>= 1) m = nan_max(m, __shfl_xor(m, off, kWave));
+  __shared__ double lds[kBlock / kWave];
+  if (threadIdx.x % kWave == 0) lds[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / kWave; w++) m = nan_max(m, lds[w]);
+    result[0] = sqrt(m);
+  }
+}
+
 int bucket(int k) {
   if (k <= 8) return 8;
   if (k <= 16) return 16;
@@ -471,6 +510,17 @@ void launch_convert(int pd, void* dst, int ps, const void* src, size_t n, hipStr
   else if (pd == F32 && ps == F64) convert_kernel<float, double><<<nb, kBlock, 0, s>>>((float*)dst, (const double*)src, n);
   else if (pd == F64) convert_kernel<double, double><<<nb, kBlock, 0, s>>>((double*)dst, (const double*)src, n);
   else convert_kernel<float, float><<<nb, kBlock, 0, s>>>((float*)dst, (const float*)src, n);
+  MINIM_HIP_CHECK(hipGetLastError());
+}
+
+void launch_group_norm_max(int prec, const void* v, size_t n, int dim, RedScratch rs, hipStream_t s) {
+  if (dim < 1) throw std::invalid_argument("group norm: dim must be positive");
+  const size_t ngroup = n / (size_t)dim;
+  const int nb = grid_blocks(ngroup, 4, 2048);
+  if (prec == F64) group_norm_max_kernel<double><<<nb, kBlock, 0, s>>>((const double*)v, ngroup, dim, rs.partials);
+  else group_norm_max_kernel<float><<<nb, kBlock, 0, s>>>((const float*)v, ngroup, dim, rs.partials);
+  MINIM_HIP_CHECK(hipGetLastError());
+  max_sqrt_finalize_kernel<<<1, kBlock, 0, s>>>(rs.partials, nb, rs.result);
   MINIM_HIP_CHECK(hipGetLastError());
 }
 

@@ -84,6 +84,10 @@ void launch_scale_copy(int prec, void* out, const void* in, double f, size_t n, 
 // v *= f (element-wise)
 void launch_scale(int prec, void* v, double f, size_t n, hipStream_t s);
 
+// rs.result[0] = the largest Euclidean norm over groups of dim consecutive
+// entries of v (NaN if any entry is NaN)
+void launch_group_norm_max(int prec, const void* v, size_t n, int dim, RedScratch rs, hipStream_t s);
+
 // zero the entries listed in idx (single-DOF constraints, Potential.cpp:104)
 void launch_zero_idx(int prec, void* v, const int* idx, size_t nidx, hipStream_t s);
 

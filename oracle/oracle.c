@@ -414,8 +414,29 @@ int orc_lbfgs_minimise(const orc_pot* pot, double* x, int m, int max_iter, doubl
 /* Fire::init (src/minimisers/Fire.cpp:27-38), Fire::iteration (:41-84),
  * Fire::checkConvergence (:87-90); constants include/minimisers/Fire.h:24-28.
  * dt_max == 0 requests the reference's estimate 0.1/sqrt(|g|). */
+/* Largest Euclidean norm over groups of dim consecutive entries (the largest
+ * per-particle force when g is a gradient of dim-dimensional particles). */
+static double group_norm_max(const double* g, size_t n, int dim) {
+  double m = 0;
+  for (size_t q = 0; q + (size_t)dim <= n; q += (size_t)dim) {
+    double s = 0;
+    for (int k = 0; k < dim; k++) s += g[q + k] * g[q + k];
+    if (s > m || s != s) m = s;
+  }
+  return sqrt(m);
+}
+
 int orc_fire_minimise(const orc_pot* pot, double* x, double dt_max, int max_iter,
                       int ls, double convergence, orc_stats* st, double* dt_max_out) {
+  return orc_fire_minimise_ex(pot, x, dt_max, 0.0, 0, max_iter, ls, convergence, st, dt_max_out);
+}
+
+/* Fire with two options the reference does not have (both off reproduce it):
+ * dt0 > 0 starts from that time step instead of dtMax (Fire.cpp:43), and
+ * max_force_dim > 0 converges on the largest per-particle force norm over
+ * groups of that many DOFs instead of the RMS gradient (Fire.cpp:87-90). */
+int orc_fire_minimise_ex(const orc_pot* pot, double* x, double dt_max, double dt0, int max_force_dim,
+                         int max_iter, int ls, double convergence, orc_stats* st, double* dt_max_out) {
   const int nMin = 5;
   const double fInc = 1.1, fDec = 0.5, fA = 0.99, aStart = 0.1;
   size_t n = pot->ndof;
@@ -435,7 +456,7 @@ int orc_fire_minimise(const orc_pot* pot, double* x, double dt_max, int max_iter
   int iter, conv = 0;
   for (iter = 0; iter <= max_iter; iter++) {
     if (iter == 0) {
-      dt = dt_max;
+      dt = dt0 > 0 ? dt0 : dt_max;
       st_gradient(&s, x, g);
       gNorm = sqrt(orc_dot(g, g, n));
     }
@@ -473,7 +494,8 @@ int orc_fire_minimise(const orc_pot* pot, double* x, double dt_max, int max_iter
     }
     st_gradient(&s, x, g);
     gNorm = sqrt(orc_dot(g, g, n));
-    if (gNorm / sqrt((double)n) < convergence) {
+    const double crit = max_force_dim > 0 ? group_norm_max(g, n, max_force_dim) : gNorm / sqrt((double)n);
+    if (crit < convergence) {
       conv = 1;
       break;
     }
@@ -617,6 +639,17 @@ void orc_ljpbc_energy_gradient(const orc_ljpbc* lj, const double* x, double* e, 
     }
   }
   if (e) *e = etot;
+}
+
+static void ljpbc_eg_cb(const void* ctx, const double* x, double* e, double* g) {
+  orc_ljpbc_energy_gradient((const orc_ljpbc*)ctx, x, e, g);
+}
+void orc_ljpbc_potential(const orc_ljpbc* lj, orc_pot* out) {
+  out->ctx = lj;
+  out->ndof = (size_t)lj->nparticle * 3;
+  out->eg = ljpbc_eg_cb;
+  out->constrain = NULL;
+  out->convergence = 1e-6;
 }
 
 /* ------------------------------------------------------------------------- */

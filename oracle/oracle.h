@@ -75,6 +75,11 @@ int orc_lbfgs_minimise(const orc_pot* pot, double* x, int m, int max_iter, doubl
                        double init_step, int linesearch, double convergence, orc_stats* st);
 int orc_fire_minimise(const orc_pot* pot, double* x, double dt_max, int max_iter,
                       int linesearch, double convergence, orc_stats* st, double* dt_max_out);
+/* orc_fire_minimise with an initial time step dt0 (0 = dt_max, the reference)
+ * and, for max_force_dim > 0, convergence on the largest per-particle force
+ * norm over groups of max_force_dim DOFs (0 = RMS gradient, the reference). */
+int orc_fire_minimise_ex(const orc_pot* pot, double* x, double dt_max, double dt0, int max_force_dim,
+                         int max_iter, int linesearch, double convergence, orc_stats* st, double* dt_max_out);
 int orc_graddescent_minimise(const orc_pot* pot, double* x, double alpha, int max_iter,
                              int linesearch, double convergence, orc_stats* st);
 
@@ -95,6 +100,7 @@ typedef struct orc_ljpbc {
   double sigma, epsilon, rc, box;
 } orc_ljpbc;
 void orc_ljpbc_energy_gradient(const orc_ljpbc* lj, const double* x, double* e, double* g);
+void orc_ljpbc_potential(const orc_ljpbc* lj, orc_pot* out);
 
 /* ---- BarAndHinge element kernels (bars, hinges, external z force) ---- */
 typedef struct orc_bah {

@@ -81,6 +81,9 @@ def lib():
         L.orc_lj_energy_gradient.argtypes = [C.POINTER(OrcLJ), _dp, _dp, _dp]
         L.orc_lj_potential.argtypes = [C.POINTER(OrcLJ), C.c_size_t, C.POINTER(OrcPot)]
         L.orc_ljpbc_energy_gradient.argtypes = [C.POINTER(OrcLJPBC), _dp, _dp, _dp]
+        L.orc_ljpbc_potential.argtypes = [C.POINTER(OrcLJPBC), C.POINTER(OrcPot)]
+        L.orc_fire_minimise_ex.argtypes = [C.POINTER(OrcPot), _dp, C.c_double, C.c_double, C.c_int, C.c_int,
+                                           C.c_int, C.c_double, C.POINTER(OrcStats), _dp]
         L.orc_bah_energy_gradient.argtypes = [C.POINTER(OrcBaH), _dp, _dp, _dp]
         L.orc_bah_potential.argtypes = [C.POINTER(OrcBaH), C.POINTER(OrcPot)]
         _lib = L
@@ -156,13 +159,18 @@ def oracle_lbfgs(pot_holder, x0, m=5, max_iter=100000, max_step=0.0, init_step=1
     return x, _stats(st)
 
 
-def oracle_fire(pot_holder, x0, dt_max=0.0, max_iter=100000, linesearch="backtracking", convergence=None):
+def oracle_fire(pot_holder, x0, dt_max=0.0, max_iter=100000, linesearch="backtracking", convergence=None,
+                dt0=0.0, max_force_dim=0):
     x = np.array(x0, dtype=np.float64)
     st = OrcStats()
     dto = C.c_double(0)
     conv = pot_holder.pot.convergence if convergence is None else convergence
-    lib().orc_fire_minimise(C.byref(pot_holder.pot), _d(x), dt_max, max_iter,
-                            0 if linesearch == "backtracking" else 1, conv, C.byref(st), C.byref(dto))
+    if dt0 == 0.0 and max_force_dim == 0:
+        lib().orc_fire_minimise(C.byref(pot_holder.pot), _d(x), dt_max, max_iter,
+                                0 if linesearch == "backtracking" else 1, conv, C.byref(st), C.byref(dto))
+    else:
+        lib().orc_fire_minimise_ex(C.byref(pot_holder.pot), _d(x), dt_max, dt0, max_force_dim, max_iter,
+                                   0 if linesearch == "backtracking" else 1, conv, C.byref(st), C.byref(dto))
     out = _stats(st)
     out["dt_max"] = dto.value
     return x, out
@@ -207,6 +215,9 @@ class OracleLJ:
 class OracleLJPBC:
     def __init__(self, nparticle, box, rc=2.5, sigma=1.0, epsilon=1.0):
         self.lj = OrcLJPBC(nparticle, sigma, epsilon, rc, box)
+        self.ndof = 3 * nparticle
+        self.pot = OrcPot()
+        lib().orc_ljpbc_potential(C.byref(self.lj), C.byref(self.pot))
 
     def energy(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)

@@ -184,6 +184,34 @@ def test_fire_graddescent_golden_trajectory(case):
     np.testing.assert_array_equal(x, np.array(c["x"]))
 
 
+def test_fire_dt0_and_max_force_options():
+    """orc_fire_minimise_ex: with both options off it is orc_fire_minimise bit for
+    bit; with dt0 the first step is -0.5 dt0^2 g (v = -dt0 g, then dt halves,
+    Fire.cpp:50-67); with max-force convergence the largest per-particle force
+    of the result is below the tolerance."""
+    import ctypes as C
+
+    from oracle_bind import OrcStats, _d, lib
+
+    rng = np.random.default_rng(8)
+    L = 5 * 1.1
+    base = np.stack(np.meshgrid(*[np.arange(5) * 1.1] * 3, indexing="ij"), -1).reshape(-1, 3)
+    x0 = (base + rng.uniform(-0.08, 0.08, base.shape)).reshape(-1)
+    o = OracleLJPBC(125, L, 2.5)
+    xa, sa = oracle_fire(o, x0, dt_max=0.01, max_iter=300, linesearch="none", convergence=1e-6)
+    xb = np.array(x0)
+    st = OrcStats()
+    dto = C.c_double(0)
+    lib().orc_fire_minimise_ex(C.byref(o.pot), _d(xb), 0.01, 0.0, 0, 300, 1, 1e-6, C.byref(st), C.byref(dto))
+    np.testing.assert_array_equal(xa, xb)
+    x1, _ = oracle_fire(o, x0, dt_max=0.02, dt0=0.002, max_force_dim=3, max_iter=0, linesearch="none")
+    np.testing.assert_allclose(x1, x0 - 0.5 * 0.002 ** 2 * o.gradient(x0), rtol=1e-15, atol=1e-15)
+    xc, sc = oracle_fire(o, x0, dt_max=0.02, dt0=0.002, max_force_dim=3, max_iter=20000, linesearch="none",
+                         convergence=1e-6)
+    assert sc["converged"]
+    assert np.sqrt((o.gradient(xc).reshape(-1, 3) ** 2).sum(axis=1)).max() < 1e-6
+
+
 def test_fire_first_step_known_answer():
     # Fire_test.cpp:25-40 on a PhaseField state: with dtMax estimated as
     # 0.1/sqrt(|g|), the first step is exactly (0.5 dtMax) * (-dtMax * g).

@@ -100,6 +100,52 @@ def test_lj_fire_cluster(gpu):
     assert s.energy() == pytest.approx(so["energy"], rel=1e-8)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_fire_dt0_first_step(gpu, prec):
+    """Fire with an initial time step: iteration 0 has v = 0, so p = 0, v = -dt0 g
+    and dt halves before the step (Fire.cpp:50-67): x1 = x0 - 0.5 dt0^2 g(x0)."""
+    m = gpu
+    rng = np.random.default_rng(5)
+    n_side, a = 5, 1.1
+    L = n_side * a
+    x0 = lattice(n_side, a, 0.1, rng)
+    x0 = x0 if prec == "fp64" else r32(x0)
+    pot = m.Lj3d().setCutoff(2.5, [L, L, L], 0.3)
+    s = m.State(pot, x0, prec)
+    g0 = s.gradient()
+    f = m.Fire(0.02).setDt0(0.002).setMaxIter(0)
+    x1 = f.minimise(s)
+    np.testing.assert_allclose(x1, x0 - 0.5 * 0.002 ** 2 * g0, rtol=TOL[prec], atol=TOL[prec] * np.abs(x0).max())
+
+
+def test_fire_max_force_lj_cutoff_vs_oracle(gpu):
+    """The LJ benchmark's minimiser settings on a small periodic box: Fire with
+    dt0 0.002, dtMax 0.02, no line search, converged when the largest
+    per-particle force is below 1e-6; same minimum as the oracle's Fire on the
+    minimum-image all-pairs potential."""
+    m = gpu
+    rng = np.random.default_rng(31)
+    n_side, a = 6, (1 / 0.85) ** (1 / 3)  # density 0.85
+    L = n_side * a
+    x0 = lattice(n_side, a, 0.08, rng)
+    n = n_side ** 3
+    pot = m.Lj3d().setCutoff(2.5, [L, L, L], 0.3)
+    s = m.State(pot, x0)
+    s.convergence = 1e-6
+    f = m.Fire(0.02).setDt0(0.002).setMaxForceConvergence(3).setMaxIter(50000)
+    x = f.minimise(s)
+    o = OracleLJPBC(n, L, 2.5)
+    xo, so = oracle_fire(o, x0, dt_max=0.02, dt0=0.002, max_force_dim=3, max_iter=50000, linesearch="none",
+                         convergence=1e-6)
+    print(f.stats, so, flush=True)
+    assert f.stats["converged"] and so["converged"]
+    assert f.stats["grad_max"] < 1e-6
+    gmax = np.sqrt((o.gradient(x).reshape(-1, 3) ** 2).sum(axis=1)).max()
+    assert gmax < 2e-6
+    assert s.energy() == pytest.approx(so["energy"], rel=1e-8)
+    assert abs(f.stats["iterations"] - so["iterations"]) <= max(20, 0.2 * so["iterations"])
+
+
 # ---------------------------------------------------------------- BarAndHinge
 def bah_from_case(m, c, use_tri=False):
     p = m.BarAndHinge()
