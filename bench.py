@@ -126,6 +126,167 @@ def cfg1_leg(m, with_cpu=True):
     return out
 
 
+def time_device_evals(m, h, prec, x, g, n, steps, warmup):
+    """ms per whole evaluation (kernels, reductions, energy copy) on device inputs."""
+    import torch
+
+    stream = torch.cuda.Stream()
+    e_dev = torch.zeros(1, dtype=torch.float64, device="cuda")
+    return time_evals(m, h, prec, x, g, e_dev, n, steps, warmup, stream)
+
+
+def lj_leg(m, n_side=64, density=0.85, fire_max_iter=20000, ref_side=10):
+    """Lennard-Jones at the benchmark size: n_side^3 particles at number density
+    0.85 in a periodic cube, truncated-shifted at rc = 2.5 sigma with a 0.3 sigma
+    skin.  Particles start on a simple cubic lattice jittered by U[-0.1, 0.1]
+    (seeded); uniform-random placement puts pairs at r -> 0, whose forces
+    (~1e26) no time-stepping minimiser can integrate.  Reports the evaluation
+    rate (FP64, FP32) and FIRE (dt0 0.002, dtMax 0.02, no line search) to a
+    largest per-particle force below 1e-6.  The reference's LjNd has no cutoff
+    or box; beside it, the reference-semantics all-pairs potential is timed on
+    the GPU and by the reference library on a ref_side^3 cluster."""
+    import numpy as np
+    import torch
+
+    N = n_side ** 3
+    a = (1.0 / density) ** (1.0 / 3.0)
+    L = n_side * a
+    rng = np.random.default_rng(2025)
+    grid = np.stack(np.meshgrid(*[np.arange(n_side) * a] * 3, indexing="ij"), -1).reshape(-1, 3)
+    x0 = (grid + rng.uniform(-0.1, 0.1, grid.shape)).reshape(-1)
+    pot = m.Lj3d().setCutoff(2.5, [L, L, L], 0.3)
+    out = dict(workload="LJ %d particles, density %.2f, periodic cube L=%.3f, rc 2.5, skin 0.3" % (N, density, L))
+    list_n = density * 4.0 / 3.0 * np.pi * 2.8 ** 3  # neighbours within rc + skin (full list)
+    for prec, dt in (("fp64", torch.float64), ("fp32", torch.float32)):
+        h = pot.handle(x0)
+        x = torch.tensor(x0, dtype=dt, device="cuda")
+        g = torch.empty_like(x)
+        ms = time_device_evals(m, h, prec, x, g, 3 * N, 50, 5)
+        b = x.element_size()
+        alg = N * 3 * b * 2 + 4 * list_n * N  # positions in, gradient out, neighbour list
+        out[prec] = dict(eval_ms=ms, evals_per_s=1e3 / ms, algorithmic_bytes=alg,
+                         achieved_GBs=alg / (ms * 1e-3) / 1e9, frac_of_hbm_peak=alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS)
+        del h, x, g
+    s = m.State(pot, x0, "fp64")
+    s.convergence = 1e-6
+    f = m.Fire(0.02).setDt0(0.002).setMaxForceConvergence(3).setMaxIter(fire_max_iter)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    f.minimise(s, download=False)
+    torch.cuda.synchronize()
+    out["fire"] = dict(seconds=time.perf_counter() - t, iterations=f.stats["iterations"],
+                       converged=f.stats["converged"], max_force=f.stats["grad_max"], energy=s.energy(),
+                       max_iter=fire_max_iter)
+    # reference semantics (all pairs, no cutoff) at ref_n particles: GPU beside the reference library
+    rn = ref_side ** 3
+    ra = 1.12
+    rg = np.stack(np.meshgrid(*[np.arange(ref_side) * ra] * 3, indexing="ij"), -1).reshape(-1, 3)
+    xr = (rg + rng.uniform(-0.05, 0.05, rg.shape)).reshape(-1)
+    hr = m.Lj3d().handle(xr)
+    xt = torch.tensor(xr, dtype=torch.float64, device="cuda")
+    gt = torch.empty_like(xt)
+    ms = time_device_evals(m, hr, "fp64", xt, gt, xr.size, 20, 3)
+    pairs = rn * (rn - 1) / 2
+    out["all_pairs_reference_semantics"] = dict(particles=rn, gpu_eval_ms=ms, gpu_pairs_per_s=pairs / (ms * 1e-3))
+    from oracle_bind import RefState, ref_available
+
+    if ref_available():
+        r = RefState.lj(3, xr)
+        t = time.perf_counter()
+        r.energy_gradient(xr)
+        el = time.perf_counter() - t
+        out["all_pairs_reference_semantics"].update(
+            reference_cpu_eval_s=el, reference_cpu_pairs_per_s=pairs / el,
+            reference_cores=int(os.environ.get("OMP_NUM_THREADS") or 1))
+    return out
+
+
+def sheet_triangles(nx, ny):
+    """Triangulated square lattice: nodes i*ny + j, triangles (a, b, c), (a, c, d)
+    per square (a = (i,j), b = (i+1,j), c = (i+1,j+1), d = (i,j+1))."""
+    import numpy as np
+
+    i, j = np.meshgrid(np.arange(nx - 1), np.arange(ny - 1), indexing="ij")
+    a = (i * ny + j).reshape(-1)
+    b = a + ny
+    c = b + 1
+    d = a + 1
+    return np.stack([np.stack([a, b, c], 1), np.stack([a, c, d], 1)], 1).reshape(-1, 3).astype(np.int32)
+
+
+def bah_leg(m, nside=1024, max_iter=2000, ref_side=48):
+    """Bar-and-hinge sheet at the benchmark size: nside x nside nodes, two
+    triangles per lattice square (2(nside-1)^2 triangles).  Nodes sit on an
+    equilateral triangular lattice (node (i,j) at (i - j/2, j sqrt(3)/2)) so every
+    bar has rest length 1 (setLength0(1.0)) and the sheet is flat (setTheta0(pi));
+    k_bar = 1, k_hinge = 0.01.  x is compressed by 0.95, the nodes of the two
+    x edges are clamped (all their DOFs fixed), z += N(0, 0.01^2) (seeded).
+    Reports the evaluation rate (FP64, FP32) and L-BFGS m=10 from that start;
+    beside it, the reference library's evaluation on a ref_side^2 sheet."""
+    import numpy as np
+    import torch
+
+    tri = sheet_triangles(nside, nside)
+    i, j = np.meshgrid(np.arange(nside), np.arange(nside), indexing="ij")
+    rng = np.random.default_rng(77)
+    xyz = np.stack([0.95 * (i - 0.5 * j), (np.sqrt(3) / 2) * j, rng.normal(0, 0.01, i.shape)], -1).reshape(-1, 3)
+    x0 = xyz.reshape(-1)
+    edge = np.where((i == 0) | (i == nside - 1), 1, 0).reshape(-1).nonzero()[0]
+    fixed = np.stack([3 * edge, 3 * edge + 1, 3 * edge + 2], 1).reshape(-1)
+    pot = m.BarAndHinge().setTriangulation(tri).setRigidity(1.0, 0.01).setLength0(1.0).setTheta0(np.pi)
+    pot.setConstraints(fixed.tolist())
+    nnode = nside * nside
+    ntri = tri.shape[0]
+
+    def counts(n):  # bars = lattice edges + diagonals; hinges = interior edges
+        nb = 2 * n * (n - 1) + (n - 1) ** 2
+        return nb, nb - 4 * (n - 1)
+
+    nbar, nhinge = counts(nside)
+    out = dict(workload="bar-and-hinge sheet %dx%d nodes, %d triangles" % (nside, nside, ntri))
+    t = time.perf_counter()
+    h = pot.handle(x0)
+    out["setup_s"] = time.perf_counter() - t
+    for prec, dt in (("fp64", torch.float64), ("fp32", torch.float32)):
+        x = torch.tensor(x0, dtype=dt, device="cuda")
+        g = torch.empty_like(x)
+        ms = time_device_evals(m, h, prec, x, g, x0.size, 20, 3)
+        b = x.element_size()
+        # node coordinates in, gradient out, bar (2 idx + k, l0) and hinge (4 idx + k, theta0) streams
+        alg = nnode * 3 * b * 2 + nbar * (8 + 2 * b) + nhinge * (16 + 2 * b)
+        out[prec] = dict(eval_ms=ms, evals_per_s=1e3 / ms, algorithmic_bytes=alg,
+                         achieved_GBs=alg / (ms * 1e-3) / 1e9, frac_of_hbm_peak=alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS)
+        del x, g
+    s = m.State(pot, x0, "fp64")
+    lb = m.Lbfgs().setM(10).setMaxIter(max_iter)
+    e0 = s.energy()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    lb.minimise(s, download=False)
+    torch.cuda.synchronize()
+    out["lbfgs_m10"] = dict(seconds=time.perf_counter() - t, iterations=lb.stats["iterations"],
+                            converged=lb.stats["converged"], grad_rms=lb.stats["grad_rms"], initial_energy=e0,
+                            energy=s.energy(), max_iter=max_iter)
+    from oracle_bind import RefState, ref_available
+
+    if ref_available():
+        rt = sheet_triangles(ref_side, ref_side)
+        ri, rj = np.meshgrid(np.arange(ref_side), np.arange(ref_side), indexing="ij")
+        rx = np.stack([0.95 * (ri - 0.5 * rj), (np.sqrt(3) / 2) * rj, rng.normal(0, 0.01, ri.shape)], -1).reshape(-1)
+        r = RefState.bah(rt, rx, 1.0, 0.01, l0=1.0, t0=np.pi)
+        n, t = 0, time.perf_counter()
+        while n < 20 and time.perf_counter() - t < 5:
+            r.energy_gradient(rx)
+            n += 1
+        el = (time.perf_counter() - t) / n
+        rb, rh = counts(ref_side)
+        out["reference_cpu"] = dict(nodes=ref_side * ref_side, triangles=rt.shape[0], eval_s=el,
+                                    elements_per_s=(rb + rh) / el,
+                                    cores=int(os.environ.get("OMP_NUM_THREADS") or 1))
+        out["gpu_elements_per_s_fp64"] = (nbar + nhinge) / (out["fp64"]["eval_ms"] * 1e-3)
+    return out
+
+
 def cpu_baseline(x, budget_s=12.0):
     """The reference library (oracle/_ref, built from its own sources) timed on
     the host cores: State construction excluded, then as many
@@ -184,7 +345,9 @@ def main():
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--mode", choices=["full", "stencil", "converge"], default="full",
+    ap.add_argument("--fire-iters", type=int, default=20000, help="particles mode: FIRE iteration cap")
+    ap.add_argument("--lbfgs-iters", type=int, default=2000, help="particles mode: sheet L-BFGS iteration cap")
+    ap.add_argument("--mode", choices=["full", "stencil", "converge", "particles"], default="full",
                     help="stencil: only the evaluation timings plus a 256 MiB copy (the calibration "
                          "kernel for rocprofv3 byte counters); converge: only time-to-converge")
     args = ap.parse_args()
@@ -211,6 +374,14 @@ def main():
     g32 = torch.empty_like(x32)
     e_dev = torch.zeros(1, dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
+
+    if args.mode == "particles":
+        del x64, x32, g64, g32
+        torch.cuda.empty_cache()
+        res = dict(lj=lj_leg(m, fire_max_iter=args.fire_iters), bah=bah_leg(m, max_iter=args.lbfgs_iters))
+        log(json.dumps(res, indent=1))
+        print(json.dumps(res), flush=True)
+        return
 
     if args.mode == "converge":
         del x64, x32, g64, g32

@@ -76,15 +76,10 @@ struct HingeGeo {
   T b2m, n1sq, n2sq, dtheta;
 };
 
+// Hinge geometry from the four node positions and the rest angle.
 template <typename T>
-__device__ inline void hinge_geo(const T* __restrict__ x, const T* __restrict__ d, T t, const BaHArgs& a, int h,
-                                 HingeGeo<T>& G) {
-  const int* nd = a.hinge + 4 * (size_t)h;
-  T x1[3], x2[3], x3[3], x4[3];
-  ld3(x, d, t, nd[0], x1);
-  ld3(x, d, t, nd[1], x2);
-  ld3(x, d, t, nd[2], x3);
-  ld3(x, d, t, nd[3], x4);
+__device__ inline void hinge_geo_at(const T (&x1)[3], const T (&x2)[3], const T (&x3)[3], const T (&x4)[3], T t0,
+                                    HingeGeo<T>& G) {
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     G.b1[k] = x2[k] - x1[k];
@@ -109,36 +104,106 @@ __device__ inline void hinge_geo(const T* __restrict__ x, const T* __restrict__ 
     const T at = atan2(yv, dot3(G.n1, G.n2));
     theta = (yv >= 0) ? at : (T)2 * pi + at;
   }
-  T dth = theta - (T)a.ht0[h];
+  T dth = theta - t0;
   dth = fmod(dth + pi, (T)2 * pi) - pi;
   G.dtheta = dth;
 }
 
+// Node positions and rest angle of hinge h (one int4 of node ids).
+template <typename T>
+struct HingeIn {
+  T x1[3], x2[3], x3[3], x4[3];
+  T t0, k;
+};
+
+template <typename T>
+__device__ inline void hinge_load(const T* __restrict__ x, const T* __restrict__ d, T t, const BaHArgs& a, int h,
+                                  HingeIn<T>& H) {
+  const int4 nd = reinterpret_cast<const int4*>(a.hinge)[h];
+  ld3(x, d, t, nd.x, H.x1);
+  ld3(x, d, t, nd.y, H.x2);
+  ld3(x, d, t, nd.z, H.x3);
+  ld3(x, d, t, nd.w, H.x4);
+  H.t0 = (T)a.ht0[h];
+  H.k = (T)a.hk[h];
+}
+
+// Per-node gradient contribution of a hinge (BarAndHinge.cpp:336-354) for
+// the node at position loc (0..3) of the hinge.
+template <typename T>
+__device__ inline void hinge_grad_at(const HingeGeo<T>& G, T k, int loc, T (&gn)[3]) {
+  const T gf = k * (G.dtheta);
+  const T f1 = G.b2m / G.n1sq, f2 = G.b2m / G.n2sq;
+  T n1h[3], n2h[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    n1h[c] = f1 * G.n1[c];
+    n2h[c] = f2 * G.n2[c];
+  }
+  const T bb = G.b2m * G.b2m;
+  const T skew1 = -dot3(G.b1, G.b2) / bb;
+  const T skew2 = -dot3(G.b3, G.b2) / bb;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    T v;
+    if (loc == 0) v = -gf * n1h[c];
+    else if (loc == 1) v = gf * (((T)1 - skew1) * n1h[c] - skew2 * n2h[c]);
+    else if (loc == 2) v = -gf * (((T)1 - skew2) * n2h[c] - skew1 * n1h[c]);
+    else v = gf * n2h[c];
+    gn[c] += v;
+  }
+}
+
+// Energy: bars, then hinges, then the external force, each a grid-stride
+// loop that takes U elements per step and issues all their loads first.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void bah_energy_kernel(BaHArgs a, const T* __restrict__ x,
                                                             const T* __restrict__ d, T t, RedScratch rs) {
+  constexpr int U = 4;
   double acc[1] = {0.0};
-  const size_t tot = (size_t)a.nbar + a.nhinge + a.nnode;
-  for (size_t w = (size_t)blockIdx.x * kBlock + threadIdx.x; w < tot; w += (size_t)gridDim.x * kBlock) {
-    if (w < (size_t)a.nbar) {
-      const int b = (int)w;
-      T p1[3], p2[3];
-      ld3(x, d, t, a.bar[2 * b], p1);
-      ld3(x, d, t, a.bar[2 * b + 1], p2);
-      T dx[3] = {p1[0] - p2[0], p1[1] - p2[1], p1[2] - p2[2]};
+  const size_t stride = (size_t)gridDim.x * kBlock;
+  const size_t tid = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  for (size_t w0 = tid; w0 < (size_t)a.nbar; w0 += U * stride) {
+    T p1[U][3], p2[U][3], k[U], l0[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const size_t w = w0 + u * stride;
+      const int b = (int)(w < (size_t)a.nbar ? w : (size_t)a.nbar - 1);
+      const int2 nd = reinterpret_cast<const int2*>(a.bar)[b];
+      ld3(x, d, t, nd.x, p1[u]);
+      ld3(x, d, t, nd.y, p2[u]);
+      k[u] = (T)a.bk[b];
+      l0[u] = (T)a.bl0[b];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (w0 + u * stride >= (size_t)a.nbar) break;
+      T dx[3] = {p1[u][0] - p2[u][0], p1[u][1] - p2[u][1], p1[u][2] - p2[u][2]};
       const T l = sqrt(dot3(dx, dx));
-      const T dl = l - (T)a.bl0[b];
-      acc[0] += (double)((T)0.5 * (T)a.bk[b] * (dl * dl));
-    } else if (w < (size_t)a.nbar + a.nhinge) {
-      const int h = (int)(w - a.nbar);
+      const T dl = l - l0[u];
+      acc[0] += (double)((T)0.5 * k[u] * (dl * dl));
+    }
+  }
+  for (size_t w0 = tid; w0 < (size_t)a.nhinge; w0 += U * stride) {
+    HingeIn<T> H[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const size_t w = w0 + u * stride;
+      hinge_load(x, d, t, a, (int)(w < (size_t)a.nhinge ? w : (size_t)a.nhinge - 1), H[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (w0 + u * stride >= (size_t)a.nhinge) break;
       HingeGeo<T> G;
-      hinge_geo(x, d, t, a, h, G);
-      acc[0] += (double)((T)0.5 * (T)a.hk[h] * (G.dtheta * G.dtheta));
-    } else if (a.fz) {
-      const int n = (int)(w - a.nbar - a.nhinge);
+      hinge_geo_at(H[u].x1, H[u].x2, H[u].x3, H[u].x4, H[u].t0, G);
+      acc[0] += (double)((T)0.5 * H[u].k * (G.dtheta * G.dtheta));
+    }
+  }
+  if (a.fz) {
+    for (size_t n = tid; n < (size_t)a.nnode; n += stride) {
       if (a.nodeActive[n]) {
         T p[3];
-        ld3(x, d, t, n, p);
+        ld3(x, d, t, (int)n, p);
         acc[0] -= a.fz[n] * (double)p[2];
       }
     }
@@ -146,49 +211,59 @@ __global__ __launch_bounds__(kBlock) void bah_energy_kernel(BaHArgs a, const T* 
   grid_sum_store<1>(acc, rs);
 }
 
+// Gradient, gathered per node over its incident bars and hinges (CSR in
+// element order), U elements at a time with all their loads issued first;
+// contributions are added in list order.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void bah_gradient_kernel(BaHArgs a, const T* __restrict__ x,
                                                               T* __restrict__ g) {
+  constexpr int U = 4;
   for (int n = blockIdx.x * kBlock + threadIdx.x; n < a.nnode; n += gridDim.x * kBlock) {
     T gn[3] = {(T)0, (T)0, (T)0};
-    for (int q = a.bptr[n]; q < a.bptr[n + 1]; q++) {
-      const int code = a.bidx[q];
-      const int b = code >> 1, loc = code & 1;
-      T p1[3], p2[3];
-      ld3<T>(x, nullptr, (T)0, a.bar[2 * b], p1);
-      ld3<T>(x, nullptr, (T)0, a.bar[2 * b + 1], p2);
-      T dx[3] = {p1[0] - p2[0], p1[1] - p2[1], p1[2] - p2[2]};
-      const T l = sqrt(dot3(dx, dx));
-      const T dl = l - (T)a.bl0[b];
-      const T gf = (T)2 * (T)a.bk[b] * dl / l;
-      const T sg = loc == 0 ? (T)1 : (T)-1;
+    const int qb0 = a.bptr[n], qb1 = a.bptr[n + 1];
+    for (int q0 = qb0; q0 < qb1; q0 += U) {
+      T p1[U][3], p2[U][3], k[U], l0[U];
+      int loc[U];
 #pragma unroll
-      for (int k = 0; k < 3; k++) gn[k] += sg * (gf * dx[k]);
-    }
-    for (int q = a.hptr[n]; q < a.hptr[n + 1]; q++) {
-      const int code = a.hidx[q];
-      const int h = code >> 2, loc = code & 3;
-      HingeGeo<T> G;
-      hinge_geo<T>(x, nullptr, (T)0, a, h, G);
-      const T gf = (T)a.hk[h] * (G.dtheta);
-      const T f1 = G.b2m / G.n1sq, f2 = G.b2m / G.n2sq;
-      T n1h[3], n2h[3];
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        n1h[k] = f1 * G.n1[k];
-        n2h[k] = f2 * G.n2[k];
+      for (int u = 0; u < U; u++) {
+        const int code = a.bidx[min(q0 + u, qb1 - 1)];
+        const int b = code >> 1;
+        loc[u] = code & 1;
+        const int2 nd = reinterpret_cast<const int2*>(a.bar)[b];
+        ld3<T>(x, nullptr, (T)0, nd.x, p1[u]);
+        ld3<T>(x, nullptr, (T)0, nd.y, p2[u]);
+        k[u] = (T)a.bk[b];
+        l0[u] = (T)a.bl0[b];
       }
-      const T bb = G.b2m * G.b2m;
-      const T skew1 = -dot3(G.b1, G.b2) / bb;
-      const T skew2 = -dot3(G.b3, G.b2) / bb;
 #pragma unroll
-      for (int k = 0; k < 3; k++) {
-        T c;
-        if (loc == 0) c = -gf * n1h[k];
-        else if (loc == 1) c = gf * (((T)1 - skew1) * n1h[k] - skew2 * n2h[k]);
-        else if (loc == 2) c = -gf * (((T)1 - skew2) * n2h[k] - skew1 * n1h[k]);
-        else c = gf * n2h[k];
-        gn[k] += c;
+      for (int u = 0; u < U; u++) {
+        if (q0 + u >= qb1) break;
+        T dx[3] = {p1[u][0] - p2[u][0], p1[u][1] - p2[u][1], p1[u][2] - p2[u][2]};
+        const T l = sqrt(dot3(dx, dx));
+        const T dl = l - l0[u];
+        // stretching gradient 2 k (l - l0) / l * dx (BarAndHinge.cpp:282)
+        const T gf = (T)2 * k[u] * dl / l;
+        const T sg = loc[u] == 0 ? (T)1 : (T)-1;
+#pragma unroll
+        for (int c = 0; c < 3; c++) gn[c] += sg * (gf * dx[c]);
+      }
+    }
+    const int qh0 = a.hptr[n], qh1 = a.hptr[n + 1];
+    for (int q0 = qh0; q0 < qh1; q0 += U) {
+      HingeIn<T> H[U];
+      int loc[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int code = a.hidx[min(q0 + u, qh1 - 1)];
+        loc[u] = code & 3;
+        hinge_load<T>(x, nullptr, (T)0, a, code >> 2, H[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (q0 + u >= qh1) break;
+        HingeGeo<T> G;
+        hinge_geo_at(H[u].x1, H[u].x2, H[u].x3, H[u].x4, H[u].t0, G);
+        hinge_grad_at(G, H[u].k, loc[u], gn);
       }
     }
     if (a.fz && a.nodeActive[n]) gn[2] -= (T)a.fz[n];

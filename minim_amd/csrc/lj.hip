@@ -273,11 +273,17 @@ __global__ __launch_bounds__(kBlock) void lj_displacement_kernel(const T* __rest
   grid_sum_store<1>(acc, rs);
 }
 
+// One thread per particle over its Verlet list, in chunks of U neighbours whose
+// indices and positions are all requested before any is used (the list rows
+// of a chunk are clamped to the last entry, and the repeats are skipped), so
+// the gathers overlap instead of forming one dependent chain per neighbour.
+// Pairs are summed in list order.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void lj_cut_kernel(const T* __restrict__ x, const T* __restrict__ d, T t, int N,
                                                         BoxArgs b, T sig2, T eps, T eshift, int mode,
                                                         const int* __restrict__ nn, const int* __restrict__ nbr,
                                                         T* __restrict__ g, RedScratch rs) {
+  constexpr int U = 8;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   double ei = 0.0;
   if (i < N) {
@@ -287,24 +293,31 @@ __global__ __launch_bounds__(kBlock) void lj_cut_kernel(const T* __restrict__ x,
     const T rc2 = (T)b.rc2;
     T gi[3] = {(T)0, (T)0, (T)0};
     const int cnt = nn[i];
-    for (int k = 0; k < cnt; k++) {
-      const int j = nbr[(size_t)k * N + i];
-      T xj[3];
-      load_pos<T, 3>(x, d, t, j, xj);
-      const T r0 = min_image(xi[0] - xj[0], L0);
-      const T r1 = min_image(xi[1] - xj[1], L1);
-      const T r2v = min_image(xi[2] - xj[2], L2);
-      const T r2 = r0 * r0 + r1 * r1 + r2v * r2v;
-      if (r2 >= rc2) continue;
-      const T s2 = sig2 / r2;
-      const T lj6 = s2 * s2 * s2;
-      const T lj12 = lj6 * lj6;
-      if (mode & 1) ei += 0.5 * ((double)((T)4 * eps * (lj12 - lj6)) - (double)eshift);
-      if (mode & 2) {
-        const T f = (T)-24 * eps * ((T)2 * lj12 - lj6) / r2;
-        gi[0] += r0 * f;
-        gi[1] += r1 * f;
-        gi[2] += r2v * f;
+    for (int k0 = 0; k0 < cnt; k0 += U) {
+      int jj[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) jj[u] = nbr[(size_t)min(k0 + u, cnt - 1) * N + i];
+      T xj[U][3];
+#pragma unroll
+      for (int u = 0; u < U; u++) load_pos<T, 3>(x, d, t, jj[u], xj[u]);
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (k0 + u >= cnt) break;
+        const T r0 = min_image(xi[0] - xj[u][0], L0);
+        const T r1 = min_image(xi[1] - xj[u][1], L1);
+        const T r2v = min_image(xi[2] - xj[u][2], L2);
+        const T r2 = r0 * r0 + r1 * r1 + r2v * r2v;
+        if (r2 >= rc2) continue;
+        const T s2 = sig2 / r2;
+        const T lj6 = s2 * s2 * s2;
+        const T lj12 = lj6 * lj6;
+        if (mode & 1) ei += 0.5 * ((double)((T)4 * eps * (lj12 - lj6)) - (double)eshift);
+        if (mode & 2) {
+          const T f = (T)-24 * eps * ((T)2 * lj12 - lj6) / r2;
+          gi[0] += r0 * f;
+          gi[1] += r1 * f;
+          gi[2] += r2v * f;
+        }
       }
     }
     if (mode & 2) {
