@@ -287,6 +287,91 @@ def bah_leg(m, nside=1024, max_iter=2000, ref_side=48):
     return out
 
 
+def batch_leg(m, n_inst=512, workers=8, seed=4040, cpu_budget=10.0):
+    """Many independent fields: n_inst 2-D binary PhaseField instances with side
+    s_j = round(exp(U[ln 32, ln 512])), phi ~ U[-0.1, 0.1] and interface width
+    w_j ~ U[1, 3] (all seeded), each minimised by L-BFGS m=5 to the reference
+    tolerance.  The instances are spread over `workers` host threads, each with
+    its own HIP stream, State objects and (reused) minimiser buffers; they are
+    set up before the timed region and ordered largest first.  Reports the
+    aggregate evaluation rate and the p50/p99 per-instance time-to-converge,
+    and the reference library on a bounded sample of the same instances."""
+    import concurrent.futures as cf
+    import threading
+
+    import numpy as np
+    import torch
+
+    rng = np.random.default_rng(seed)
+    sides = np.rint(np.exp(rng.uniform(np.log(32), np.log(512), n_inst))).astype(int)
+    widths = rng.uniform(1.0, 3.0, n_inst)
+    seeds = rng.integers(0, 2 ** 62, n_inst)
+
+    def inputs(j):
+        return np.random.default_rng(int(seeds[j])).uniform(-0.1, 0.1, int(sides[j]) ** 2)
+
+    def make_pf(j):
+        s = int(sides[j])
+        return m.PhaseField().setNFluid(1).setGridSize([s, s, 1]).setSurfaceTension(1.0).setInterfaceSize(
+            float(widths[j]))
+
+    order = sorted(range(n_inst), key=lambda j: -sides[j])
+    t = time.perf_counter()
+    states = {j: m.State(make_pf(j), inputs(j), "fp64") for j in order}
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t
+    tls = threading.local()
+
+    def run(j):
+        if not hasattr(tls, "stream"):
+            torch.cuda.set_device(0)
+            tls.stream = torch.cuda.Stream()
+            tls.lb = m.Lbfgs().setM(5).setMaxIter(20000)
+        s = states[j]
+        s.setStream(tls.stream.cuda_stream)
+        t0 = time.perf_counter()
+        tls.lb.minimise(s, download=False)
+        st = tls.lb.stats
+        return dict(seconds=time.perf_counter() - t0, iterations=st["iterations"], converged=st["converged"],
+                    evals=st["n_energy"] + st["n_gradient"])
+
+    # largest first: each worker allocates its minimiser buffers once, on its first instance
+    with cf.ThreadPoolExecutor(workers) as ex:
+        t = time.perf_counter()
+        res = list(ex.map(run, order))
+        wall = time.perf_counter() - t
+    secs = np.array([r["seconds"] for r in res])
+    evals = sum(r["evals"] for r in res)
+    out = dict(workload="%d 2-D binary PhaseField instances, sides %d..%d, %d sites in total; L-BFGS m=5 each"
+                        % (n_inst, sides.min(), sides.max(), int((sides ** 2).sum())),
+               workers=workers, setup_s=setup_s, wall_s=wall, evals=evals, evals_per_s=evals / wall,
+               instances_per_s=n_inst / wall, converged=int(sum(r["converged"] for r in res)),
+               p50_s=float(np.percentile(secs, 50)), p99_s=float(np.percentile(secs, 99)),
+               iterations_total=int(sum(r["iterations"] for r in res)))
+    del states
+    from oracle_bind import RefState, ref_available
+
+    if ref_available() and cpu_budget > 0:
+        threads = int(os.environ.get("OMP_NUM_THREADS") or min(os.cpu_count() or 1, 16))
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        done, el, sites_done = 0, 0.0, 0
+        for j in range(n_inst):
+            x0 = inputs(j)
+            r = RefState.phasefield((int(sides[j]), int(sides[j]), 1), x0, 1, 1.0, [1.0], [float(widths[j])], "hard")
+            t = time.perf_counter()
+            r.minimise("lbfgs", m=5, max_iter=20000)
+            el += time.perf_counter() - t
+            done += 1
+            sites_done += int(sides[j]) ** 2
+            if el > cpu_budget:
+                break
+        out["reference_cpu"] = dict(kind="reference", cores=threads, instances=done, sites=sites_done, seconds=el,
+                                    instances_per_s=done / el,
+                                    sample="the first %d instances in generation order, minimised one after "
+                                           "another (reference library, OpenMP %d threads)" % (done, threads))
+    return out
+
+
 def cpu_baseline(x, budget_s=12.0):
     """The reference library (oracle/_ref, built from its own sources) timed on
     the host cores: State construction excluded, then as many
@@ -347,7 +432,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--fire-iters", type=int, default=20000, help="particles mode: FIRE iteration cap")
     ap.add_argument("--lbfgs-iters", type=int, default=2000, help="particles mode: sheet L-BFGS iteration cap")
-    ap.add_argument("--mode", choices=["full", "stencil", "converge", "particles"], default="full",
+    ap.add_argument("--mode", choices=["full", "stencil", "converge", "particles", "batch"], default="full",
                     help="stencil: only the evaluation timings plus a 256 MiB copy (the calibration "
                          "kernel for rocprofv3 byte counters); converge: only time-to-converge")
     args = ap.parse_args()
@@ -374,6 +459,14 @@ def main():
     g32 = torch.empty_like(x32)
     e_dev = torch.zeros(1, dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
+
+    if args.mode == "batch":
+        del x64, x32, g64, g32
+        torch.cuda.empty_cache()
+        res = batch_leg(m, cpu_budget=0 if args.no_cpu else 10.0)
+        log(json.dumps(res, indent=1))
+        print(json.dumps(res), flush=True)
+        return
 
     if args.mode == "particles":
         del x64, x32, g64, g32

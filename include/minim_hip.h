@@ -10,8 +10,9 @@
  *
  * Device vectors are flat arrays of double (MINIM_FP64) or float (MINIM_FP32)
  * in GPU memory.  Streams are hipStream_t passed as void* (NULL = default).
- * Calls that reduce (energies, dots) share per-device scratch memory and must
- * not overlap on one GPU.
+ * Calls that reduce (energies, dots, minimisers) use scratch memory private to
+ * the calling host thread and device: independent problems may run
+ * concurrently from several host threads, each on its own stream.
  */
 #ifndef MINIM_HIP_H
 #define MINIM_HIP_H

@@ -207,11 +207,12 @@ struct Lbfgs::Impl {
   bool fresh = true;  // next iteration starts from a new gradient
   int stall = 0;
 
-  // Empty history; device buffers are kept from the previous call when the
-  // sizes and precision match (a minimise on the same State allocates
-  // nothing).
+  // Empty history; device buffers are kept from the previous call when they
+  // are large enough and of the same precision (a minimise on the same or a
+  // smaller problem allocates nothing; vector lengths come from n, not from
+  // the buffer sizes).
   void alloc(int prec_, size_t n_, int m_) {
-    const bool reuse = prec_ == prec && n_ == n && m_ == m && (int)D.size() == m_ + 1 && g.n == n_ && g.prec == prec_;
+    const bool reuse = prec_ == prec && m_ == m && (int)D.size() == m_ + 1 && g.n >= n_ && g.prec == prec_;
     prec = prec_;
     n = n_;
     m = m_;

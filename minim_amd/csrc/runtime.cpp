@@ -58,12 +58,12 @@ Workspace::Workspace() {
   MINIM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_), result_bytes, hipHostMallocDefault));
 }
 
+// One workspace per host thread and device: reductions issued from different
+// host threads (each on its own stream) never share scratch memory.
 Workspace& Workspace::current() {
-  static std::mutex mu;
-  static std::map<int, std::unique_ptr<Workspace>> per_device;
+  thread_local std::map<int, std::unique_ptr<Workspace>> per_device;
   int dev = 0;
   MINIM_HIP_CHECK(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> lock(mu);
   auto it = per_device.find(dev);
   if (it == per_device.end()) {
     it = per_device.emplace(dev, std::unique_ptr<Workspace>(new Workspace())).first;

@@ -48,8 +48,9 @@ struct DVec {
   void* data() const { return buf.get(); }
 };
 
-// Per-device reduction scratch plus a pinned host landing area.  Launches that
-// reduce share it, so calls on one device must be issued from one host thread.
+// Reduction scratch plus a pinned host landing area, one per host thread and
+// device.  Launches that reduce from one host thread share it (they are
+// ordered on that thread's streams); other host threads have their own.
 class Workspace {
  public:
   static Workspace& current();  // for the current HIP device
