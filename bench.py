@@ -554,6 +554,19 @@ def main():
     log("cfg1 (128x128x1 binary) time-to-converge ...")
     cfg1 = cfg1_leg(m, with_cpu=not args.no_cpu)
     log("  cfg1: " + json.dumps(cfg1))
+
+    # the other workloads of the path, each beside the reference library where it can run them;
+    # a failure here is recorded in the output and does not hide the headline line
+    others = {}
+    for name, fn in (("lj_periodic_262144", lambda: lj_leg(m, fire_max_iter=args.fire_iters)),
+                     ("bar_and_hinge_1024x1024", lambda: bah_leg(m, max_iter=args.lbfgs_iters)),
+                     ("batch_512_instances", lambda: batch_leg(m, cpu_budget=0 if args.no_cpu else 10.0))):
+        log("%s ..." % name)
+        try:
+            others[name] = fn()
+        except Exception as exc:  # noqa: BLE001
+            others[name] = {"error": "%s: %s" % (type(exc).__name__, exc)}
+        log("  %s: %s" % (name, json.dumps(others[name])))
     cpu = None if args.no_cpu else cpu_baseline(x0, args.cpu_budget)
 
     out = {
@@ -585,6 +598,7 @@ def main():
                                   "gradient out (PCIe transfers included); not the headline value"},
         "time_to_converge": ttc,
         "cfg1_time_to_converge": cfg1,
+        "other_workloads": others,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -48,6 +48,8 @@ class PhaseField : public NewPotential<PhaseField> {
                      void* stream) const override;
   bool supportsMixed() const override { return nFluid <= 4; }
   void evaluateMixed(const void* x, const void* d, double t, double* e, void* g, void* stream) const override;
+  void evaluateMixedAsync(const void* x, const void* d, double t, bool wantE, double* e_dev, void* g,
+                          void* stream) const override;
   double timeKernel(int prec, const void* x, bool wantE, void* g, int reps, void* stream) const override;
   void applyConstraints(int prec, void* v, size_t n, void* stream) const override;
   SiteProjection projection() const override;

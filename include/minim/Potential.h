@@ -63,6 +63,11 @@ class Potential {
   // true from supportsMixed(); the default evaluateMixed throws
   // std::invalid_argument.
   virtual bool supportsMixed() const { return false; }
+  // As evaluateMixed, without a host synchronisation (energy, when wantE, to
+  // the device double e_dev; with e_dev NULL it is left where evaluateAsync
+  // leaves it).
+  virtual void evaluateMixedAsync(const void* x, const void* d, double t, bool wantE, double* e_dev, void* g,
+                                  void* stream) const;
   // Benchmark support: average duration (ms, HIP events on the stream) of
   // reps back-to-back launches of the evaluation kernel alone, without the
   // cross-block reduction that follows it.  The default throws

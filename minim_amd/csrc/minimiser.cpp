@@ -127,12 +127,22 @@ void eval_at(State& s, int pv, const void* d, double t, double* e, void* g) {
     s.pot->evaluateMixed(s.deviceCoords(), d, t, e, g, s.stream());
 }
 
+// Energy at x + t*d launched without a host synchronisation; the value is
+// left in the calling thread's reduction result slot 0.
+void energy_async_at(State& s, int pv, const void* d, double t) {
+  if (pv == s.precision())
+    s.pot->evaluateAsync(pv, s.deviceCoords(), d, t, true, nullptr, nullptr, s.stream());
+  else
+    s.pot->evaluateMixedAsync(s.deviceCoords(), d, t, true, nullptr, nullptr, s.stream());
+}
+
 // backtrackingLinesearch (linesearch.cpp:8-29) along step = tau0 * d, with d a
 // device vector of precision pv.  Trials are evaluated at x + tau*d without
-// materialising them; tau halves exactly like step *= 0.5.  Returns the tau
+// materialising them; tau halves exactly like step *= 0.5.  firstTrial, when
+// given, is the energy already evaluated at x + tau0*d.  Returns the tau
 // applied to x.
 double backtrack(State& s, int pv, const void* d, double tau0, double gs, EnergyCache& ec, MinimiseStats& st,
-                 bool* acceptedOut = nullptr) {
+                 bool* acceptedOut = nullptr, const double* firstTrial = nullptr) {
   const double c = 0.5, shrink = 0.5;
   double t = -c * gs;
   double e0;
@@ -147,8 +157,12 @@ double backtrack(State& s, int pv, const void* d, double tau0, double gs, Energy
   bool accepted = false;
   double e = 0;
   for (int i = 0; i < 10; i++) {
-    eval_at(s, pv, d, tau, &e, nullptr);
-    st.nEnergy++;
+    if (i == 0 && firstTrial) {
+      e = *firstTrial;
+    } else {
+      eval_at(s, pv, d, tau, &e, nullptr);
+      st.nEnergy++;
+    }
     if (e0 - e >= t) {
       accepted = true;
       break;
@@ -404,8 +418,25 @@ void Lbfgs::iteration(State& state) {
     }
   }
   minim_amd::launch_direction(a, prec, ws.rs(), s);
+  // Without a maxStep cap the line search starts at x + d (tau = 1) unless d
+  // must be flipped (rare).  Launch that trial energy now, after stashing the
+  // direction pass's dots, so that one host synchronisation returns both.
   std::vector<double> dres(1 + a.ndot);
-  ws.fetch(s, 1 + a.ndot, dres.data());
+  const bool spec = linesearch == "backtracking" && (P == 0 || _maxStep == 0);
+  double eSpec = 0;
+  if (spec) {
+    constexpr int kStash = 48;  // result slots [48, 48 + 1 + ndot) hold the direction dots
+    double* res = ws.device_result();
+    MINIM_HIP_CHECK(hipMemcpyAsync(res + kStash, res, sizeof(double) * (1 + a.ndot), hipMemcpyDeviceToDevice, s));
+    energy_async_at(state, prec, a.out, 1.0);
+    std::vector<double> r(kStash + 1 + a.ndot);
+    ws.fetch(s, kStash + 1 + a.ndot, r.data());
+    eSpec = r[0];
+    for (int i = 0; i <= a.ndot; i++) dres[i] = r[kStash + i];
+    stats.nEnergy++;
+  } else {
+    ws.fetch(s, 1 + a.ndot, dres.data());
+  }
   const double dd = dres[0], dgv = dres[1];
 
   double tau0 = 1.0;
@@ -423,7 +454,7 @@ void Lbfgs::iteration(State& state) {
   bool lsFailed = false;
   if (linesearch == "backtracking") {
     bool acc = true;
-    tau = backtrack(state, prec, a.out, tau0, gs, I.ec, stats, &acc);
+    tau = backtrack(state, prec, a.out, tau0, gs, I.ec, stats, &acc, (spec && tau0 == 1.0) ? &eSpec : nullptr);
     lsFailed = !acc;
   } else {
     minim_amd::launch_axpy_mixed(state.precision(), state.deviceCoords(), prec, a.out, tau0, n, s);

@@ -245,17 +245,28 @@ void PhaseField::evaluateMixed(const void* x, const void* d, double t, double* e
   pf_eval_sync(args_of(args_), FP64, FP32, x, d, t, e, g, stream);
 }
 
-void PhaseField::evaluateAsync(int prec, const void* x, const void* d, double t, bool wantE, double* e_dev, void* g,
-                               void* stream) const {
-  const PFArgs& a = args_of(args_);
+namespace {
+void pf_eval_async(const PFArgs& a, int prec, int pv, const void* x, const void* d, double t, bool wantE,
+                   double* e_dev, void* g, void* stream) {
   int mode = (wantE ? 1 : 0) | (g ? 2 : 0) | (d ? 4 : 0);
   if (mode == 0) return;
   if ((mode & 4) && (mode & 2)) throw std::invalid_argument("PhaseField: gradients at a trial point are not supported.");
   hipStream_t s = static_cast<hipStream_t>(stream);
   auto& ws = minim_amd::Workspace::current();
-  minim_amd::pf_launch_eval(a, prec, prec, mode, x, d, t, g, ws.rs(), s);
+  minim_amd::pf_launch_eval(a, prec, pv, mode, x, d, t, g, ws.rs(), s);
   if (wantE && e_dev)
     MINIM_HIP_CHECK(hipMemcpyAsync(e_dev, ws.device_result(), sizeof(double), hipMemcpyDeviceToDevice, s));
+}
+}  // namespace
+
+void PhaseField::evaluateAsync(int prec, const void* x, const void* d, double t, bool wantE, double* e_dev, void* g,
+                               void* stream) const {
+  pf_eval_async(args_of(args_), prec, prec, x, d, t, wantE, e_dev, g, stream);
+}
+
+void PhaseField::evaluateMixedAsync(const void* x, const void* d, double t, bool wantE, double* e_dev, void* g,
+                                    void* stream) const {
+  pf_eval_async(args_of(args_), FP64, FP32, x, d, t, wantE, e_dev, g, stream);
 }
 
 void PhaseField::applyConstraints(int prec, void* v, size_t n, void* stream) const {

@@ -49,6 +49,10 @@ double Potential::timeKernel(int, const void*, bool, void*, int, void*) const {
   throw std::invalid_argument("Potential: kernel timing is not available for this potential.");
 }
 
+void Potential::evaluateMixedAsync(const void*, const void*, double, bool, double*, void*, void*) const {
+  throw std::invalid_argument("Potential: mixed FP64/FP32 storage is not supported by this potential.");
+}
+
 void Potential::evaluateMixed(const void*, const void*, double, double*, void*, void*) const {
   throw std::invalid_argument("Potential: mixed FP64/FP32 storage is not supported by this potential.");
 }
