@@ -17,6 +17,9 @@
  *                       coordinates); n-step convergence on quadratics.
  *   orc_fire_minimise   Fire_test.cpp:25-40 exact first step; convergence KAT
  *                       Fire_test.cpp:43-52; bit-exact vs oracle/_ref.
+ *   orc_fire_minimise_ex  options without a reference counterpart: off, it is
+ *                       orc_fire_minimise bit for bit; dt0 gives the closed-form
+ *                       first step -0.5 dt0^2 g; the max-force criterion holds at exit.
  *   orc_graddescent_*   bit-exact vs oracle/_ref.
  *   orc_lj_*            State_test.cpp:33-51 dimer KAT; vs oracle/_ref; Newton's 3rd law.
  *   orc_ljpbc_*         NO reference implementation exists (the reference LjNd has
