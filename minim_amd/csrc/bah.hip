@@ -162,7 +162,8 @@ __global__ __launch_bounds__(kBlock) void bah_energy_kernel(BaHArgs a, const T* 
   constexpr int U = 4;
   double acc[1] = {0.0};
   const size_t stride = (size_t)gridDim.x * kBlock;
-  const size_t tid = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  // contiguous ranges of blocks per XCD: neighbouring elements share nodes in one L2
+  const size_t tid = (size_t)xcd_logical_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   for (size_t w0 = tid; w0 < (size_t)a.nbar; w0 += U * stride) {
     T p1[U][3], p2[U][3], k[U], l0[U];
 #pragma unroll
@@ -218,7 +219,8 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void bah_gradient_kernel(BaHArgs a, const T* __restrict__ x,
                                                               T* __restrict__ g) {
   constexpr int U = 4;
-  for (int n = blockIdx.x * kBlock + threadIdx.x; n < a.nnode; n += gridDim.x * kBlock) {
+  for (int n = xcd_logical_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; n < a.nnode;
+       n += gridDim.x * kBlock) {
     T gn[3] = {(T)0, (T)0, (T)0};
     const int qb0 = a.bptr[n], qb1 = a.bptr[n + 1];
     for (int q0 = qb0; q0 < qb1; q0 += U) {

@@ -253,42 +253,34 @@ __global__ __launch_bounds__(kBlock) void lj_build_list_brute_kernel(const T* __
   if (cnt > maxn) atomicMax(overflow, cnt);
 }
 
-// Count of particles displaced more than skin/2 from the list positions.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void lj_displacement_kernel(const T* __restrict__ x, const T* __restrict__ d,
-                                                                 T t, int N, const double* __restrict__ xbuild,
-                                                                 double lim2, RedScratch rs) {
-  double acc[1] = {0.0};
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < N; i += gridDim.x * kBlock) {
-    T p[3];
-    load_pos<T, 3>(x, d, t, i, p);
-    double r2 = 0;
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      const double v = (double)p[k] - xbuild[(size_t)i * 3 + k];
-      r2 += v * v;
-    }
-    if (r2 > lim2) acc[0] += 1.0;
-  }
-  grid_sum_store<1>(acc, rs);
-}
-
 // One thread per particle over its Verlet list, in chunks of U neighbours whose
 // indices and positions are all requested before any is used (the list rows
 // of a chunk are clamped to the last entry, and the repeats are skipped), so
 // the gathers overlap instead of forming one dependent chain per neighbour.
-// Pairs are summed in list order.
+// Pairs are summed in list order.  The same pass counts the particles that
+// have moved more than skin/2 since the list was built (reduction slot 1):
+// when any has, the caller rebuilds the list and evaluates again.  Blocks are
+// spread over the XCDs in contiguous ranges, so each L2 holds the positions
+// of one spatially coherent range of particles.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void lj_cut_kernel(const T* __restrict__ x, const T* __restrict__ d, T t, int N,
                                                         BoxArgs b, T sig2, T eps, T eshift, int mode,
                                                         const int* __restrict__ nn, const int* __restrict__ nbr,
-                                                        T* __restrict__ g, RedScratch rs) {
+                                                        const double* __restrict__ xbuild, T* __restrict__ g,
+                                                        RedScratch rs) {
   constexpr int U = 8;
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  double ei = 0.0;
+  const int i = xcd_logical_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  double ei = 0.0, moved = 0.0;
   if (i < N) {
     T xi[3];
     load_pos<T, 3>(x, d, t, i, xi);
+    double dr2 = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const double v = (double)xi[k] - xbuild[(size_t)i * 3 + k];
+      dr2 += v * v;
+    }
+    if (dr2 > b.half_skin2) moved = 1.0;
     const T L0 = (T)b.L[0], L1 = (T)b.L[1], L2 = (T)b.L[2];
     const T rc2 = (T)b.rc2;
     T gi[3] = {(T)0, (T)0, (T)0};
@@ -326,8 +318,8 @@ __global__ __launch_bounds__(kBlock) void lj_cut_kernel(const T* __restrict__ x,
       g[(size_t)i * 3 + 2] = gi[2];
     }
   }
-  double acc[1] = {ei};
-  grid_sum_store<1>(acc, rs);
+  double acc[2] = {ei, moved};
+  grid_sum_store<2>(acc, rs);
 }
 
 template <typename T>
@@ -460,23 +452,7 @@ void LjNd::evaluate(int prec, const void* x, const void* d, double t, double* e,
     Impl& I = *impl_;
     const int nb = (N + kBlock - 1) / kBlock;
     if (nb > minim_amd::kMaxRedBlocks) throw std::invalid_argument("LjNd: too many particles for one launch");
-    // list still valid for these positions?
-    bool rebuild = !I.built;
-    if (!rebuild) {
-      const int nbd = minim_amd::grid_blocks(N, 4, 1024);
-      if (prec == FP64)
-        minim_amd::lj_displacement_kernel<double><<<nbd, kBlock, 0, s>>>(
-            (const double*)x, (const double*)d, t, N, (const double*)I.xbuild.get(), I.box.half_skin2, ws.rs());
-      else
-        minim_amd::lj_displacement_kernel<float><<<nbd, kBlock, 0, s>>>(
-            (const float*)x, (const float*)d, (float)t, N, (const double*)I.xbuild.get(), I.box.half_skin2, ws.rs());
-      MINIM_HIP_CHECK(hipGetLastError());
-      minim_amd::launch_finalize(ws.rs(), 1, nbd, s);
-      double moved = 0;
-      ws.fetch(s, 1, &moved);
-      rebuild = moved > 0;
-    }
-    if (rebuild) {
+    auto rebuild_list = [&]() {
       // Build at the evaluated positions: materialise x + t*d when needed.
       DeviceBuffer tmp;
       const void* xb = x;
@@ -531,19 +507,34 @@ void LjNd::evaluate(int prec, const void* x, const void* d, double t, double* e,
       }
       I.built = true;
       I.builds++;
-    }
+    };
     const double src6 = std::pow(sigma / cutoff, 6);
     const double eshift = 4 * epsilon * (src6 * src6 - src6);
-    if (prec == FP64)
-      minim_amd::lj_cut_kernel<double><<<nb, kBlock, 0, s>>>(
-          (const double*)x, (const double*)d, t, N, I.box, sigma * sigma, epsilon, eshift, mode,
-          (const int*)I.nn.get(), (const int*)I.nbr.get(), (double*)g, ws.rs());
-    else
-      minim_amd::lj_cut_kernel<float><<<nb, kBlock, 0, s>>>(
-          (const float*)x, (const float*)d, (float)t, N, I.box, (float)(sigma * sigma), (float)epsilon, (float)eshift,
-          mode, (const int*)I.nn.get(), (const int*)I.nbr.get(), (float*)g, ws.rs());
-    MINIM_HIP_CHECK(hipGetLastError());
-    minim_amd::launch_finalize(ws.rs(), 1, nb, s);
+    auto launch_force = [&]() {
+      if (prec == FP64)
+        minim_amd::lj_cut_kernel<double><<<nb, kBlock, 0, s>>>(
+            (const double*)x, (const double*)d, t, N, I.box, sigma * sigma, epsilon, eshift, mode,
+            (const int*)I.nn.get(), (const int*)I.nbr.get(), (const double*)I.xbuild.get(), (double*)g, ws.rs());
+      else
+        minim_amd::lj_cut_kernel<float><<<nb, kBlock, 0, s>>>(
+            (const float*)x, (const float*)d, (float)t, N, I.box, (float)(sigma * sigma), (float)epsilon,
+            (float)eshift, mode, (const int*)I.nn.get(), (const int*)I.nbr.get(), (const double*)I.xbuild.get(),
+            (float*)g, ws.rs());
+      MINIM_HIP_CHECK(hipGetLastError());
+      minim_amd::launch_finalize(ws.rs(), 2, nb, s);
+    };
+    if (!I.built) rebuild_list();
+    launch_force();
+    double r[2];
+    ws.fetch(s, 2, r);  // energy, particles moved more than skin/2 since the list was built
+    if (r[1] > 0) {
+      rebuild_list();
+      launch_force();
+      ws.fetch(s, 2, r);
+    }
+    if (g) applyConstraints(prec, g, (size_t)N * nDim, stream);
+    if (e) *e = r[0];
+    return;
   }
   if (g) applyConstraints(prec, g, (size_t)N * nDim, stream);
   if (e) {
