@@ -104,9 +104,14 @@ __device__ inline void hinge_geo_at(const T (&x1)[3], const T (&x2)[3], const T 
     const T at = atan2(yv, dot3(G.n1, G.n2));
     theta = (yv >= 0) ? at : (T)2 * pi + at;
   }
-  T dth = theta - t0;
-  dth = fmod(dth + pi, (T)2 * pi) - pi;
-  G.dtheta = dth;
+  // dtheta = fmod(theta - t0 + pi, 2 pi) - pi (BarAndHinge.cpp:324).  With
+  // |a| < 2b, fmod(a, b) is a, a - b or a + b, each exact (Sterbenz), so the
+  // library call is needed only for rest angles far outside [0, 2 pi].
+  const T a = (theta - t0) + pi, b2 = (T)2 * pi;
+  T r;
+  if (a < (T)2 * b2 && a > -(T)2 * b2) r = a >= b2 ? a - b2 : (a <= -b2 ? a + b2 : a);
+  else r = fmod(a, b2);
+  G.dtheta = r - pi;
 }
 
 // Node positions and rest angle of hinge h (one int4 of node ids).

@@ -94,6 +94,7 @@ __global__ __launch_bounds__(kBlock) void lj_allpairs_kernel(const T* __restrict
 // ---------------------------------------------------------------- cutoff mode
 struct BoxArgs {
   double L[3];
+  double invL[3];
   double inv_cell[3];
   int nc[3];
   double rlist2;
@@ -101,9 +102,12 @@ struct BoxArgs {
   double half_skin2;
 };
 
+// Minimum image v - L rint(v / L), with v / L taken as v * (1/L): the two
+// can round to different integers only when |v| is within an ulp of L/2,
+// where both images are far beyond any cutoff.
 template <typename T>
-__device__ inline T min_image(T v, T L) {
-  return v - L * rint(v / L);
+__device__ inline T min_image(T v, T L, T invL) {
+  return v - L * rint(v * invL);
 }
 
 template <typename T>
@@ -207,9 +211,9 @@ __global__ __launch_bounds__(kBlock) void lj_build_list_kernel(const T* __restri
         for (int p = start[cc]; p < start[cc + 1]; p++) {
           const int j = order[p];
           if (j == i) continue;
-          const double r0 = min_image(xi0 - (double)x[(size_t)j * 3], b.L[0]);
-          const double r1 = min_image(xi1 - (double)x[(size_t)j * 3 + 1], b.L[1]);
-          const double r2 = min_image(xi2 - (double)x[(size_t)j * 3 + 2], b.L[2]);
+          const double r0 = min_image(xi0 - (double)x[(size_t)j * 3], b.L[0], b.invL[0]);
+          const double r1 = min_image(xi1 - (double)x[(size_t)j * 3 + 1], b.L[1], b.invL[1]);
+          const double r2 = min_image(xi2 - (double)x[(size_t)j * 3 + 2], b.L[2], b.invL[2]);
           if (r0 * r0 + r1 * r1 + r2 * r2 < b.rlist2) {
             if (cnt < maxn) nbr[(size_t)cnt * N + i] = j;
             cnt++;
@@ -241,7 +245,7 @@ __global__ __launch_bounds__(kBlock) void lj_build_list_brute_kernel(const T* __
     if (j == i) continue;
     double r2 = 0;
     for (int k = 0; k < 3; k++) {
-      const double v = min_image(xi[k] - (double)x[(size_t)j * 3 + k], b.L[k]);
+      const double v = min_image(xi[k] - (double)x[(size_t)j * 3 + k], b.L[k], b.invL[k]);
       r2 += v * v;
     }
     if (r2 < b.rlist2) {
@@ -283,6 +287,7 @@ __global__ __launch_bounds__(kBlock) void lj_cut_kernel(const T* __restrict__ x,
     if (dr2 > b.half_skin2) moved = 1.0;
     const T L0 = (T)b.L[0], L1 = (T)b.L[1], L2 = (T)b.L[2];
     const T rc2 = (T)b.rc2;
+    const T iL0 = (T)b.invL[0], iL1 = (T)b.invL[1], iL2 = (T)b.invL[2];
     T gi[3] = {(T)0, (T)0, (T)0};
     const int cnt = nn[i];
     for (int k0 = 0; k0 < cnt; k0 += U) {
@@ -295,17 +300,18 @@ __global__ __launch_bounds__(kBlock) void lj_cut_kernel(const T* __restrict__ x,
 #pragma unroll
       for (int u = 0; u < U; u++) {
         if (k0 + u >= cnt) break;
-        const T r0 = min_image(xi[0] - xj[u][0], L0);
-        const T r1 = min_image(xi[1] - xj[u][1], L1);
-        const T r2v = min_image(xi[2] - xj[u][2], L2);
+        const T r0 = min_image(xi[0] - xj[u][0], L0, iL0);
+        const T r1 = min_image(xi[1] - xj[u][1], L1, iL1);
+        const T r2v = min_image(xi[2] - xj[u][2], L2, iL2);
         const T r2 = r0 * r0 + r1 * r1 + r2v * r2v;
         if (r2 >= rc2) continue;
-        const T s2 = sig2 / r2;
+        const T ir2 = (T)1 / r2;  // one division per pair
+        const T s2 = sig2 * ir2;
         const T lj6 = s2 * s2 * s2;
         const T lj12 = lj6 * lj6;
         if (mode & 1) ei += 0.5 * ((double)((T)4 * eps * (lj12 - lj6)) - (double)eshift);
         if (mode & 2) {
-          const T f = (T)-24 * eps * ((T)2 * lj12 - lj6) / r2;
+          const T f = (T)-24 * eps * ((T)2 * lj12 - lj6) * ir2;
           gi[0] += r0 * f;
           gi[1] += r1 * f;
           gi[2] += r2v * f;
@@ -421,6 +427,7 @@ void LjNd::init(const vector<double>& coords) {
       b.inv_cell[k] = b.nc[k] / box[k];
     }
     b.rlist2 = rl * rl;
+    for (int k = 0; k < 3; k++) b.invL[k] = 1.0 / box[k];
     b.rc2 = cutoff * cutoff;
     b.half_skin2 = 0.25 * skin * skin;
     impl_->cellMode = b.nc[0] >= 3 && b.nc[1] >= 3 && b.nc[2] >= 3;
