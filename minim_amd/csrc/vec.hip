@@ -409,7 +409,9 @@ void dir_t(const DirArgs& a, RedScratch rs, hipStream_t s) {
 
 template <typename T>
 void gram_t(const GramArgs& a, RedScratch rs, hipStream_t s) {
-  const int nb = grid_blocks(a.n, 4, 2048);
+  // 1024 blocks (4 per CU): the fused y/Gram pass streams 2P+3 vectors at once
+  // and reaches a higher rate with fewer, longer-running blocks than with 2048
+  const int nb = grid_blocks(a.n, 4, 1024);
   if (a.ny <= kFastPairs && gram_fast<T>(a, rs, s, nb)) return;
   const int KB = gram_bucket(a.nd > a.ny ? a.nd : a.ny);
   switch (KB) {

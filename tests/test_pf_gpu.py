@@ -131,6 +131,50 @@ def test_trial_point_and_constraints(gpu, prec):
     assert e3 == e1
 
 
+MODE_SHAPES = [
+    ((1, 70, 130), 1, [1.0], [1.5], "none"),
+    ((3, 17, 65), 2, [1.0], [2.0], "hard"),
+    ((5, 33, 64), 3, [1.0, 0.8, 1.2], [2.0], "hard"),
+    ((2, 9, 200), 3, [1.0, 0.8, 1.2], [1.5], "soft"),
+    ((4, 20, 40), 4, [1.0, 0.9, 1.1, 0.8], [1.3], "fixed"),
+    ((3, 5, 7), 6, [1.0] * 6, [1.0], "hard"),
+]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("shape", range(len(MODE_SHAPES)))
+def test_evaluation_modes_agree(gpu, shape, prec):
+    """Every launch mode of the stencil on shapes whose tiles overhang the grid:
+    gradient-only equals the energy+gradient launch's gradient and energy-only
+    its energy, bit for bit; the energy at a trial point x + t*d equals the
+    energy at the materialised point, bit for bit."""
+    import torch
+
+    m = gpu
+    grid, nf, st, iw, dens = MODE_SHAPES[shape]
+    print("case modes", grid, nf, dens, prec, flush=True)
+    rng = np.random.default_rng(500 + shape)
+    n = int(np.prod(grid)) * nf
+    x0 = rng.uniform(-1.1, 1.1, n) if nf == 1 else rng.uniform(-0.1, 1.1, n)
+    h = make_pf(m, grid, nf, st, iw, dens).handle(x0)
+    dt = torch.float64 if prec == "fp64" else torch.float32
+    x = torch.tensor(x0, dtype=dt, device="cuda")
+    d = torch.tensor(rng.normal(0, 0.2, n), dtype=dt, device="cuda")
+    g1 = torch.empty_like(x)
+    g2 = torch.empty_like(x)
+    e1 = m.potential_eval(h, prec, x.data_ptr(), n, g_ptr=g1.data_ptr())
+    m.potential_eval(h, prec, x.data_ptr(), n, g_ptr=g2.data_ptr(), want_energy=False)
+    e3 = m.potential_eval(h, prec, x.data_ptr(), n)
+    torch.cuda.synchronize()
+    assert torch.equal(g1, g2)
+    assert e1 == e3
+    t = 0.8125
+    et = m.potential_eval(h, prec, x.data_ptr(), n, d_ptr=d.data_ptr(), t=t)
+    xt = x + torch.tensor(t, dtype=dt, device="cuda") * d
+    torch.cuda.synchronize()
+    assert et == m.potential_eval(h, prec, xt.data_ptr(), n)
+
+
 MIXED_SHAPES = [
     ((12, 10, 9), 1, [1.0], [1.5], "none"),
     ((11, 1, 13), 2, [1.0], [2.0], "hard"),
