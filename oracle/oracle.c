@@ -243,6 +243,501 @@ void orc_pf_potential(const orc_pf* pf, orc_pot* out) {
 }
 
 /* ------------------------------------------------------------------------- */
+/* PhaseField: solid nodes, wetting, pressure, force, confinement, volume    */
+/* ------------------------------------------------------------------------- */
+
+/* PhaseField::getType (src/potentials/PhaseField.cpp:899-958).  The 26
+ * neighbours are counted by how many of their offsets are non-zero: faces (1),
+ * edges (2) and corners (3); "one side" (type 4 against 5) asks whether the
+ * solid face's axis has no solid edge in the plane across it (the reference's
+ * neighbour groups 10-13, 7/8/15/16 and 6/9/14/17).  Offsets wrap periodically
+ * (getIdx, :44-49), so a dimension of size 1 or 2 can count one site twice, as
+ * in the reference. */
+int orc_pf_site_types(const int grid[3], const unsigned char* solid, int* type) {
+  const int n0 = grid[0], n1 = grid[1], n2 = grid[2];
+  for (int x = 0; x < n0; x++)
+    for (int y = 0; y < n1; y++)
+      for (int z = 0; z < n2; z++) {
+        const int i = (x * n1 + y) * n2 + z;
+        if (solid[i]) {
+          type[i] = -1;
+          continue;
+        }
+        int nface = 0, nedge = 0, ncorner = 0;
+        int face_axis[3] = {0, 0, 0};     /* a solid face along axis a */
+        int edge_across[3] = {0, 0, 0};   /* a solid edge with zero offset along axis a */
+        for (int dx = -1; dx <= 1; dx++)
+          for (int dy = -1; dy <= 1; dy++)
+            for (int dz = -1; dz <= 1; dz++) {
+              const int nz = (dx != 0) + (dy != 0) + (dz != 0);
+              if (nz == 0) continue;
+              const int j = (((x + dx + n0) % n0) * n1 + (y + dy + n1) % n1) * n2 + (z + dz + n2) % n2;
+              if (!solid[j]) continue;
+              if (nz == 1) {
+                nface++;
+                face_axis[dx != 0 ? 0 : (dy != 0 ? 1 : 2)] = 1;
+              } else if (nz == 2) {
+                nedge++;
+                if (dx == 0) edge_across[0] = 1;
+                if (dy == 0) edge_across[1] = 1;
+                if (dz == 0) edge_across[2] = 1;
+              } else {
+                ncorner++;
+              }
+            }
+        int t = -2;
+        if (nface + nedge + ncorner == 0) {
+          t = 0;
+        } else if (nface == 0) {
+          if (nedge == 0 && ncorner == 1) t = 1;
+          else if (nedge == 1 && ncorner <= 2) t = 2;
+          else if (nedge == 2 && ncorner <= 3) t = 3;
+        } else if (nface == 1) {
+          int one = 0;
+          for (int a = 0; a < 3; a++)
+            if (face_axis[a] && !edge_across[a]) one = 1;
+          t = one ? 4 : 5;
+        } else if (nface == 2) {
+          t = 6;
+        } else if (nface == 3) {
+          t = 7;
+        }
+        if (t == -2) return 1; /* "Undefined surface type" (:957) */
+        type[i] = t;
+      }
+  return 0;
+}
+
+void orc_pfx_free(orc_pfx* p) {
+  free(p->kappa);
+  free(p->kappaP);
+  free(p->solid);
+  free(p->type);
+  free(p->node_vol);
+  free(p->surface_area);
+  free(p->contact_angle);
+  free(p->ff_init);
+  memset(p, 0, sizeof(*p));
+}
+
+/* assignFluidCoefficients (:78-134) with per-site tensions, init (:137-158),
+ * initLocal (:161-235) on one rank, setDefaults (:961-967), checkArraySizes
+ * (:970-995). */
+int orc_pfx_init(orc_pfx* p, const int grid[3], int nfluid, double resolution,
+                 const double* surface_tension, int n_st, const double* interface_size, int n_iw,
+                 int density, double density_const, const unsigned char* fix_fluid,
+                 const unsigned char* solid, const double* contact_angle, int n_ca,
+                 const double* pressure, const double* force, const double* confinement,
+                 int volume_fixed, double vol_const, const double* volume, const double* x0) {
+  memset(p, 0, sizeof(*p));
+  if (nfluid < 1 || nfluid > ORC_MAXF) return 1;
+  for (int d = 0; d < 3; d++)
+    if (grid[d] < 1) return 1;
+  const size_t ng = (size_t)grid[0] * grid[1] * grid[2];
+  const int nF = nfluid, np = nfluid;
+  p->ngrid = ng;
+  orc_pf* b = &p->pf;
+  for (int d = 0; d < 3; d++) b->grid[d] = grid[d];
+  b->nfluid = nF;
+  b->resolution = resolution;
+  b->density = density;
+  b->density_const = density_const;
+  const int nk = (n_st == (int)(ng * np)) ? (int)ng : 1; /* :79 */
+  p->nkgrid = nk;
+  /* checkArraySizes: interfaceSize has nfluid entries after setDefaults (the
+   * scalar setter, n_iw == 1, fills nfluid copies) */
+  double iw1[ORC_MAXF];
+  if (n_iw == 0) {
+    for (int i = 0; i < nF; i++) iw1[i] = resolution;
+  } else if (n_iw == 1) {
+    for (int i = 0; i < nF; i++) iw1[i] = interface_size[0];
+  } else if (n_iw == nF) {
+    for (int i = 0; i < nF; i++) iw1[i] = interface_size[i];
+  } else {
+    return 3;
+  }
+  if (n_st != 1 && n_st != nk * np) return 2; /* :84-87 */
+  const size_t nst = (size_t)nk * np;
+  double* st = (double*)malloc(sizeof(double) * nst);
+  for (size_t i = 0; i < nst; i++) st[i] = (n_st == 1) ? surface_tension[0] : surface_tension[i];
+  double sum = 0;
+  for (size_t i = 0; i < nst; i++) sum += st[i];
+  b->st_mean = sum / nst; /* :88 */
+  p->kappa = (double*)malloc(sizeof(double) * nst);
+  p->kappaP = (double*)malloc(sizeof(double) * nst);
+  for (int ig = 0; ig < nk; ig++) {
+    const double* s = st + (size_t)ig * np;
+    double* ka = p->kappa + (size_t)ig * np;
+    double* kp = p->kappaP + (size_t)ig * np;
+    if (nF <= 2) {
+      for (int i = 0; i < np; i++) {
+        ka[i] = (3 * s[i]) / iw1[i];
+        kp[i] = (3 * s[i]) * iw1[i];
+      }
+    } else {
+      double ks[ORC_MAXF], kps[ORC_MAXF];
+      for (int i = 0; i < np; i++) {
+        ks[i] = (6 * s[i]) / iw1[i];
+        kps[i] = (6 * s[i]) * iw1[i];
+      }
+      const int l = nF - 1;
+      ka[0] = 0.5 * (ks[0] + ks[1] - ks[l]);
+      ka[1] = 0.5 * (ks[0] - ks[1] + ks[l]);
+      ka[2] = 0.5 * (-ks[0] + ks[1] + ks[l]);
+      kp[0] = 0.5 * (kps[0] + kps[1] - kps[l]);
+      kp[1] = 0.5 * (kps[0] - kps[1] + kps[l]);
+      kp[2] = 0.5 * (-kps[0] + kps[1] + kps[l]);
+      for (int i = 3; i < nF; i++) {
+        ka[i] = ks[i - 1] - ka[0];
+        kp[i] = kps[i - 1] - kp[0];
+      }
+    }
+  }
+  free(st);
+  for (int i = 0; i < nF && i < ORC_MAXF; i++) {
+    b->kappa[i] = p->kappa[i];
+    b->kappaP[i] = p->kappaP[i];
+  }
+  b->convergence = 1e-8 * b->st_mean * pow(resolution, 2); /* :146 */
+  for (int i = 0; i < nF; i++) b->fix[i] = fix_fluid ? fix_fluid[i] : 0;
+  if (density == ORC_DENSITY_FIXED) b->fix[nF - 1] = 1;
+  if (contact_angle && n_ca != (int)(ng * nF)) {
+    orc_pfx_free(p);
+    return 4; /* "Invalid size of contactAngle array." :977 */
+  }
+
+  /* forces (:149-157): magnitude and unit direction per fluid */
+  if (force) {
+    for (int i = 0; i < nF; i++) {
+      const double* f = force + 3 * i;
+      double r = 0;
+      for (int d = 0; d < 3; d++) r += f[d] * f[d];
+      p->fmag[i] = sqrt(r);
+      for (int d = 0; d < 3; d++) p->fnorm[i][d] = f[d] / p->fmag[i];
+      if (p->fmag[i] != 0) p->any_force = 1; /* vec::any(fMag) */
+    }
+  }
+  for (int i = 0; i < nF; i++) {
+    p->pressure[i] = pressure ? pressure[i] : 0;
+    p->conf[i] = confinement ? confinement[i] : 0;
+  }
+
+  p->solid = (unsigned char*)calloc(ng, 1);
+  if (solid) memcpy(p->solid, solid, ng);
+  p->type = (int*)malloc(sizeof(int) * ng);
+  if (orc_pf_site_types(grid, p->solid, p->type)) {
+    orc_pfx_free(p);
+    return 5;
+  }
+  /* node volume and solid surface area per site (:194-208) */
+  p->node_vol = (double*)malloc(sizeof(double) * ng);
+  p->surface_area = (double*)malloc(sizeof(double) * ng);
+  for (size_t i = 0; i < ng; i++) {
+    const int t = p->type[i];
+    double v = 0, a = 0;
+    if (t == 0) {
+      v = 1;
+    } else if (t > 0) {
+      v = t / 8.0;
+      if (t == 2 || t == 4 || t == 6) a = 1;
+      if (t == 1 || t == 7) a = 0.75;
+      if (t == 3 || t == 5) a = 1.25;
+    }
+    p->node_vol[i] = v * pow(resolution, 3);
+    p->surface_area[i] = a * pow(resolution, 2);
+  }
+  if (contact_angle) {
+    p->contact_angle = (double*)malloc(sizeof(double) * n_ca);
+    memcpy(p->contact_angle, contact_angle, sizeof(double) * n_ca);
+  }
+
+  /* fixed-volume target and sum of squared node volumes (:216-231) */
+  p->volume_fixed = volume_fixed;
+  p->vol_const = vol_const;
+  if (volume) {
+    p->has_volume = 1;
+    for (int i = 0; i < nF; i++) p->volume[i] = volume[i];
+  } else if (volume_fixed) {
+    p->has_volume = 1;
+    for (int i = 0; i < nF; i++) p->volume[i] = 0;
+    for (size_t ig = 0; ig < ng; ig++)
+      for (int i = 0; i < nF; i++) p->volume[i] += x0[i + nF * ig] * p->node_vol[ig];
+  }
+  /* the reference sets totalVolume2 only when it derives the target itself;
+   * with an explicit target it is left unset there.  It is always set here. */
+  p->total_volume2 = 0;
+  for (size_t ig = 0; ig < ng; ig++) p->total_volume2 += p->node_vol[ig] * p->node_vol[ig];
+
+  int any_conf = 0;
+  for (int i = 0; i < nF; i++)
+    if (p->conf[i] != 0) any_conf = 1;
+  if (any_conf) {
+    p->ff_init = (double*)malloc(sizeof(double) * ng * nF);
+    memcpy(p->ff_init, x0, sizeof(double) * ng * nF);
+  }
+  return 0;
+}
+
+/* The terms of one site, each adding to its own energy/gradient pair (the
+ * reference's per-site calls in energyGradient, :679-694, and
+ * energyComponents, :720-729).  t[c] selects term c. */
+enum { PFX_FLUID, PFX_SURFACE, PFX_PRESSURE, PFX_DENSITY, PFX_FORCE, PFX_CONF, PFX_NTERM };
+
+static void pfx_site(const orc_pfx* p, const double* x, int ix, int iy, int iz, double** e, double** g) {
+  const orc_pf* b = &p->pf;
+  const int nF = b->nfluid;
+  const int* n = b->grid;
+  const double res = b->resolution;
+  const int iGrid = pf_idx(b, ix, iy, iz);
+  const double v = p->node_vol[iGrid];
+  int nb[6];
+  nb[0] = pf_idx(b, ix - 1, iy, iz);
+  nb[1] = pf_idx(b, ix + 1, iy, iz);
+  nb[2] = pf_idx(b, ix, iy - 1, iz);
+  nb[3] = pf_idx(b, ix, iy + 1, iz);
+  nb[4] = pf_idx(b, ix, iy, iz - 1);
+  nb[5] = pf_idx(b, ix, iy, iz + 1);
+
+  /* fluidEnergy (:365-393) with phaseGradient (:238-288) */
+  if (e[PFX_FLUID] || g[PFX_FLUID]) {
+    double* E = e[PFX_FLUID];
+    double* G = g[PFX_FLUID];
+    for (int iF = 0; iF < nF; iF++) {
+      const int iDof = iGrid * nF + iF;
+      const double c = x[iDof];
+      const int iK = (p->nkgrid == 1) ? iF : iGrid * nF + iF;
+      if (nF == 1) {
+        double factor = p->kappa[iK] / 16 * v;
+        if (E) *E += factor * pow(c + 1, 2) * pow(c - 1, 2);
+        if (G) G[iDof] += factor * 4 * c * (c * c - 1);
+      } else {
+        double factor = 0.5 * p->kappa[iK] * v;
+        if (E) *E += factor * pow(c, 2) * pow(c - 1, 2);
+        if (G) G[iDof] += factor * 2 * c * (c - 1) * (2 * c - 1);
+      }
+      double factor = (nF == 1) ? 0.25 * p->kappaP[iK] * v : 0.5 * p->kappaP[iK] * v;
+      factor = factor / pow(res, 2);
+      double grad2 = 0;
+      for (int d = 0; d < 3; d++) {
+        if (n[d] == 1) continue;
+        const int mg = nb[2 * d], pg = nb[2 * d + 1];
+        const int im = mg * nF + iF, ip = pg * nF + iF;
+        if (!p->solid[mg] && !p->solid[pg]) {
+          const double gm = c - x[im], gp = c - x[ip];
+          if (E) grad2 += 0.5 * (pow(gm, 2) + pow(gp, 2));
+          if (G) {
+            G[iDof] += factor * (gm + gp);
+            G[im] -= factor * gm;
+            G[ip] -= factor * gp;
+          }
+        } else if (!p->solid[pg]) {
+          const double gp = c - x[ip];
+          if (E) grad2 += pow(gp, 2);
+          if (G) {
+            G[iDof] += factor * 2 * gp;
+            G[ip] -= factor * 2 * gp;
+          }
+        } else if (!p->solid[mg]) {
+          const double gm = c - x[im];
+          if (E) grad2 += pow(gm, 2);
+          if (G) {
+            G[iDof] += factor * 2 * gm;
+            G[im] -= factor * 2 * gm;
+          }
+        }
+      }
+      if (E) *E += factor * grad2;
+    }
+  }
+
+  /* surfaceEnergy (:476-487): binary fluids only */
+  if ((e[PFX_SURFACE] || g[PFX_SURFACE]) && p->surface_area[iGrid] != 0 && p->contact_angle &&
+      p->contact_angle[iGrid] != 90 && nF == 1) {
+    const double w = 1 / sqrt(2.0) * cos(p->contact_angle[iGrid] * 3.1415926536 / 180);
+    const double phi = x[iGrid];
+    if (e[PFX_SURFACE]) *e[PFX_SURFACE] += w * (pow(phi, 3) / 3 - phi - 2.0 / 3) * p->surface_area[iGrid];
+    if (g[PFX_SURFACE]) g[PFX_SURFACE][iGrid] += w * (pow(phi, 2) - 1) * p->surface_area[iGrid];
+  }
+
+  /* pressureEnergy (:396-421) */
+  if (e[PFX_PRESSURE] || g[PFX_PRESSURE]) {
+    double* E = e[PFX_PRESSURE];
+    double* G = g[PFX_PRESSURE];
+    for (int iF = 0; iF < nF; iF++) {
+      if (p->pressure[iF] == 0 || b->fix[iF]) continue;
+      if (nF == 1) {
+        const double vol = 0.5 * (x[iGrid] + 1) * v;
+        if (E) *E -= p->pressure[iF] * vol;
+        if (G) G[iGrid] -= 0.5 * p->pressure[iF] * v;
+      } else {
+        const int iDof = iGrid * nF + iF;
+        const double vol = x[iDof] * v;
+        if (E) *E -= p->pressure[iF] * vol;
+        if (G) G[iDof] -= p->pressure[iF] * v;
+      }
+    }
+  }
+
+  /* densityConstraintEnergy (:424-440) */
+  if ((e[PFX_DENSITY] || g[PFX_DENSITY]) && nF > 1 && b->density == ORC_DENSITY_SOFT) {
+    const double coef = b->density_const * b->st_mean * pow(res, 2);
+    double rho = -1;
+    for (int iF = 0; iF < nF; iF++) rho += x[iGrid * nF + iF];
+    if (e[PFX_DENSITY]) *e[PFX_DENSITY] += coef * pow(rho, 2);
+    if (g[PFX_DENSITY])
+      for (int iF = 0; iF < nF; iF++) g[PFX_DENSITY][iGrid * nF + iF] += 2 * coef * rho;
+  }
+
+  /* forceEnergy (:490-518).  Without a gradient the reference returns after
+   * the first fluid that contributes (:511), so an energy-only call counts
+   * one fluid; this is kept. */
+  if ((e[PFX_FORCE] || g[PFX_FORCE]) && !p->solid[iGrid] && p->any_force) {
+    const double coord[3] = {ix - (n[0] - 1) / 2.0, iy - (n[1] - 1) / 2.0, iz - (n[2] - 1) / 2.0};
+    for (int iF = 0; iF < nF; iF++) {
+      if (p->fmag[iF] == 0) continue;
+      const int iDof = iGrid * nF + iF;
+      const double c = (nF == 1) ? 0.5 * (1 + x[iDof]) : x[iDof];
+      if (c < 0.01) continue;
+      double dot = 0;
+      for (int d = 0; d < 3; d++) dot += coord[d] * p->fnorm[iF][d];
+      const double h = -dot * res;
+      const double ePerConc = v * p->fmag[iF] * h;
+      if (e[PFX_FORCE]) *e[PFX_FORCE] += c * ePerConc;
+      if (!g[PFX_FORCE]) break;
+      if (nF == 1) g[PFX_FORCE][iDof] += 0.5 * ePerConc;
+      else g[PFX_FORCE][iDof] += ePerConc;
+    }
+  }
+
+  /* ffConfinementEnergy (:521-537) */
+  if (e[PFX_CONF] || g[PFX_CONF]) {
+    for (int iF = 0; iF < nF; iF++) {
+      if (p->conf[iF] == 0) continue;
+      const int iDof = iGrid * nF + iF;
+      const double coef = p->conf[iF] * b->st_mean * pow(res, 2);
+      const double c = x[iDof], c0 = p->ff_init[iDof];
+      if ((c0 - 0.5) * (c - 0.5) < 0) {
+        if (e[PFX_CONF]) *e[PFX_CONF] += coef * pow(c - 0.5, 2);
+        if (g[PFX_CONF]) g[PFX_CONF][iDof] += coef * 2 * (c - 0.5);
+      }
+    }
+  }
+}
+
+/* PhaseField::energyGradient (:662-703) on one thread: every term adds to one
+ * energy and one gradient; State::basicEG zeroes both first. */
+void orc_pfx_energy_gradient(const orc_pfx* p, const double* x, double* e, double* g) {
+  const int* n = p->pf.grid;
+  const size_t nd = p->ngrid * p->pf.nfluid;
+  double eLocal = 0;
+  double* E[PFX_NTERM];
+  double* G[PFX_NTERM];
+  for (int t = 0; t < PFX_NTERM; t++) {
+    E[t] = e ? &eLocal : NULL;
+    G[t] = g;
+  }
+  if (g) memset(g, 0, sizeof(double) * nd);
+  for (int ix = 0; ix < n[0]; ix++)
+    for (int iy = 0; iy < n[1]; iy++)
+      for (int iz = 0; iz < n[2]; iz++) pfx_site(p, x, ix, iy, iz, E, G);
+  if (e) *e = 0 + eLocal;
+}
+
+/* specialisedConstraints (:580-659): fixed fluids, hard density projection,
+ * then removal of the volume-changing component of every variable fluid. */
+void orc_pfx_constraints(const orc_pfx* p, double* data) {
+  orc_pf_constraints(&p->pf, data);
+  if (!p->volume_fixed) return;
+  const int nF = p->pf.nfluid;
+  int var[ORC_MAXF], nvar = 0;
+  for (int iF = 0; iF < nF; iF++)
+    if (!p->pf.fix[iF]) var[nvar++] = iF;
+  const int hard = p->pf.density == ORC_DENSITY_HARD;
+  double comp[ORC_MAXF] = {0};
+  for (size_t ig = 0; ig < p->ngrid; ig++) {
+    const double v = p->node_vol[ig];
+    for (int a = 0; a < nvar; a++) {
+      comp[var[a]] += data[ig * nF + var[a]] * v;
+      if (hard)
+        for (int b2 = 0; b2 < nvar; b2++) comp[var[a]] -= data[ig * nF + var[b2]] * v / nvar;
+    }
+  }
+  for (int a = 0; a < nvar; a++) comp[var[a]] = comp[var[a]] / p->total_volume2;
+  for (size_t ig = 0; ig < p->ngrid; ig++) {
+    const double v = p->node_vol[ig];
+    for (int a = 0; a < nvar; a++) {
+      data[ig * nF + var[a]] -= comp[var[a]] * v;
+      if (hard)
+        for (int b2 = 0; b2 < nvar; b2++) data[ig * nF + var[b2]] += comp[var[a]] * v / nvar;
+    }
+  }
+}
+
+/* PhaseField::energyComponents (:706-744) with volumeConstraintEnergy
+ * (:540-577) on one rank. */
+void orc_pfx_components(const orc_pfx* p, const double* x, double out[14]) {
+  const int* n = p->pf.grid;
+  const int nF = p->pf.nfluid;
+  const size_t nd = p->ngrid * nF;
+  double ev[PFX_NTERM + 1] = {0};
+  double* gv[PFX_NTERM + 1];
+  for (int t = 0; t <= PFX_NTERM; t++) gv[t] = (double*)calloc(nd, sizeof(double));
+  double* E[PFX_NTERM];
+  for (int t = 0; t < PFX_NTERM; t++) E[t] = &ev[t];
+  for (int ix = 0; ix < n[0]; ix++)
+    for (int iy = 0; iy < n[1]; iy++)
+      for (int iz = 0; iz < n[2]; iz++) pfx_site(p, x, ix, iy, iz, E, gv);
+  if (p->volume_fixed) {
+    double vol[ORC_MAXF] = {0}, diff[ORC_MAXF] = {0};
+    for (size_t ig = 0; ig < p->ngrid; ig++) {
+      if (nF == 1) {
+        vol[0] += 0.5 * (x[ig] + 1) * p->node_vol[ig];
+      } else {
+        for (int iF = 0; iF < nF; iF++) vol[iF] += x[ig * nF + iF] * p->node_vol[ig];
+      }
+    }
+    for (int iF = 0; iF < nF; iF++) {
+      if (p->pf.fix[iF] || p->volume[iF] < 0) continue;
+      diff[iF] = vol[iF] - p->volume[iF];
+    }
+    const double coef = p->vol_const * p->pf.st_mean / pow(p->pf.resolution, 4);
+    double* g = gv[PFX_NTERM];
+    for (int iF = 0; iF < nF; iF++) {
+      if (diff[iF] == 0) continue;
+      ev[PFX_NTERM] += coef * pow(diff[iF], 2) / 1;
+      for (size_t ig = 0; ig < p->ngrid; ig++) {
+        const size_t iDof = ig * nF + iF;
+        double w = 4 * x[iDof] * (1 - x[iDof]);
+        if (w < 0.0) w = 0.0;
+        g[iDof] += coef * diff[iF] * p->node_vol[ig] * w;
+      }
+    }
+  }
+  for (int t = 0; t <= PFX_NTERM; t++) {
+    double s = 0;
+    for (size_t i = 0; i < nd; i++) s += gv[t][i] * gv[t][i];
+    out[2 * t] = ev[t];
+    out[2 * t + 1] = sqrt(s);
+    free(gv[t]);
+  }
+}
+
+static void pfx_eg_cb(const void* ctx, const double* x, double* e, double* g) {
+  orc_pfx_energy_gradient((const orc_pfx*)ctx, x, e, g);
+}
+static void pfx_con_cb(const void* ctx, const double* x, double* data) {
+  (void)x;
+  orc_pfx_constraints((const orc_pfx*)ctx, data);
+}
+void orc_pfx_potential(const orc_pfx* p, orc_pot* out) {
+  out->ctx = p;
+  out->ndof = p->ngrid * p->pf.nfluid;
+  out->eg = pfx_eg_cb;
+  out->constrain = pfx_con_cb;
+  out->convergence = p->pf.convergence;
+}
+
+/* ------------------------------------------------------------------------- */
 /* State-level helpers                                                       */
 /* ------------------------------------------------------------------------- */
 

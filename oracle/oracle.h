@@ -56,6 +56,56 @@ void orc_pf_energy_gradient(const orc_pf* pf, const double* x, double* e, double
 void orc_pf_constraints(const orc_pf* pf, double* data);
 void orc_pf_potential(const orc_pf* pf, orc_pot* out);
 
+/* ---- PhaseField with the reference's remaining terms: solid nodes (node
+ * volumes, one-sided gradients), wetting, pressure, external force,
+ * confinement, per-site surface tensions, the volume projection and
+ * energyComponents.  Arrays are copied; free with orc_pfx_free. ---- */
+typedef struct orc_pfx {
+  orc_pf pf;              /* grid, fluids, density mode, fixed fluids, convergence */
+  size_t ngrid;
+  int nkgrid;             /* 1, or ngrid with per-site surface tensions */
+  double* kappa;          /* nkgrid * nfluid */
+  double* kappaP;
+  unsigned char* solid;   /* ngrid */
+  int* type;              /* ngrid: -1 solid, 0 bulk, 1..7 surface types */
+  double* node_vol;       /* ngrid */
+  double* surface_area;   /* ngrid */
+  double* contact_angle;  /* ngrid * nfluid, or NULL */
+  double pressure[ORC_MAXF];
+  int any_force;
+  double fmag[ORC_MAXF], fnorm[ORC_MAXF][3];
+  double conf[ORC_MAXF];
+  double* ff_init;        /* initial coordinates when any confinement, else NULL */
+  int volume_fixed;
+  double vol_const;
+  double volume[ORC_MAXF];
+  int has_volume;
+  double total_volume2;
+} orc_pfx;
+
+/* n_st: 1, nfluid, or ngrid*nfluid (per-site tensions).  solid, contact_angle
+ * (n_ca entries), pressure (nfluid), force (nfluid x 3, already expanded per
+ * fluid), confinement (nfluid) and volume (nfluid) may be NULL.  x0 is the
+ * initial state (node volumes of the fixed-volume target, confinement).
+ * Returns 0, or nonzero where the reference throws. */
+int orc_pfx_init(orc_pfx* p, const int grid[3], int nfluid, double resolution,
+                 const double* surface_tension, int n_st, const double* interface_size, int n_iw,
+                 int density, double density_const, const unsigned char* fix_fluid,
+                 const unsigned char* solid, const double* contact_angle, int n_ca,
+                 const double* pressure, const double* force, const double* confinement,
+                 int volume_fixed, double vol_const, const double* volume, const double* x0);
+void orc_pfx_free(orc_pfx* p);
+void orc_pfx_energy_gradient(const orc_pfx* p, const double* x, double* e, double* g);
+void orc_pfx_constraints(const orc_pfx* p, double* data);
+/* PhaseField::energyComponents: out[2*c] = energy, out[2*c+1] = |gradient| for
+ * c = fluid, surface, pressure, density constraint, force, confinement,
+ * volume constraint. */
+void orc_pfx_components(const orc_pfx* p, const double* x, double out[14]);
+void orc_pfx_potential(const orc_pfx* p, orc_pot* out);
+/* PhaseField::getType (PhaseField.cpp:899-958) of every site; returns 0, or
+ * 1 where the reference throws "Undefined surface type". */
+int orc_pf_site_types(const int grid[3], const unsigned char* solid, int* type);
+
 /* ---- reductions (CommGrid::dotProduct, single rank) ---- */
 double orc_dot(const double* a, const double* b, size_t n);
 

@@ -262,6 +262,114 @@ int ref_bah_set_hinge_theta0(void* h, int ih, double t0) {
   return 1;
 }
 
+// ---- PhaseField with its remaining terms (solid, wetting, pressure, force,
+// confinement, volume) set through the reference's own setters ----
+struct PFExtOpts {
+  const unsigned char* solid;  // ngrid, or NULL
+  const double* contact_angle;
+  int n_ca;
+  const double* pressure;  // nfluid, or NULL
+  const double* force3;    // 3, or NULL
+  const int* force_fluids;
+  int n_force_fluids;
+  const double* confinement;  // nfluid, or NULL
+  int volume_fixed;
+  double vol_const;
+  const double* volume;  // nfluid, or NULL
+  const unsigned char* fix_fluid;  // nfluid, or NULL
+  double density_const;
+};
+
+static CountingPF make_pf_ext(const int* grid, int nfluid, double res, const double* st, int nst, const double* iw,
+                              int niw, int density, const PFExtOpts* o) {
+  CountingPF pot = make_pf(grid, nfluid, res, st, nst, iw, niw, density);
+  const size_t ng = (size_t)grid[0] * grid[1] * grid[2];
+  pot.densityConst = o->density_const;
+  if (o->solid) pot.setSolid(vector<char>(o->solid, o->solid + ng));
+  if (o->contact_angle) pot.setContactAngle(vector<double>(o->contact_angle, o->contact_angle + o->n_ca));
+  if (o->pressure) pot.setPressure(vector<double>(o->pressure, o->pressure + nfluid));
+  if (o->force3)
+    pot.setForce(vector<double>(o->force3, o->force3 + 3),
+                 vector<int>(o->force_fluids, o->force_fluids + o->n_force_fluids));
+  if (o->confinement) pot.setConfinement(vector<double>(o->confinement, o->confinement + nfluid));
+  if (o->volume) pot.setVolume(vector<double>(o->volume, o->volume + nfluid), o->vol_const);
+  else if (o->volume_fixed) pot.setVolumeFixed(true, o->vol_const);
+  if (o->fix_fluid)
+    for (int i = 0; i < nfluid; i++)
+      if (o->fix_fluid[i]) pot.setFixFluid(i);
+  return pot;
+}
+
+void* ref_pf_state_create_ex(const int* grid, int nfluid, double res, const double* st, int nst, const double* iw,
+                             int niw, int density, const double* x, const PFExtOpts* o) {
+  try {
+    CountingPF pot = make_pf_ext(grid, nfluid, res, st, nst, iw, niw, density, o);
+    size_t n = (size_t)grid[0] * grid[1] * grid[2] * nfluid;
+    return new State(pot, vector<double>(x, x + n));
+  } catch (...) {
+    return nullptr;
+  }
+}
+
+// PhaseField::energyComponents at the state's coordinates: out[2c], out[2c+1]
+// = energy and gradient norm of fluid, surface, pressure, density constraint,
+// force, confinement, volume constraint.
+int ref_pf_components(void* h, double* out) {
+  State* s = static_cast<State*>(h);
+  try {
+    const PhaseField& p = static_cast<const PhaseField&>(*s->pot);
+    auto m = p.energyComponents(s->coords(), *s->comm);
+    const char* names[7] = {"fluid", "surface", "pressure", "density constraint", "force", "confinement",
+                            "volume constraint"};
+    for (int c = 0; c < 7; c++) {
+      out[2 * c] = m.at(names[c])[0];
+      out[2 * c + 1] = m.at(names[c])[1];
+    }
+  } catch (...) {
+    return 1;
+  }
+  return 0;
+}
+
+// Node volumes and surface areas after initLocal.
+int ref_pf_geometry(void* h, double* node_vol, double* surface_area) {
+  State* s = static_cast<State*>(h);
+  const PhaseField& p = static_cast<const PhaseField&>(*s->pot);
+  std::memcpy(node_vol, p.nodeVol.data(), sizeof(double) * p.nodeVol.size());
+  std::memcpy(surface_area, p.surfaceArea.data(), sizeof(double) * p.surfaceArea.size());
+  return 0;
+}
+
+// State::applyConstraints (src/State.cpp:321-323) on a vector of ndof entries.
+int ref_state_constraints(void* h, double* data) {
+  State* s = static_cast<State*>(h);
+  try {
+    vector<double> d(data, data + s->ndof);
+    s->applyConstraints(d);
+    std::memcpy(data, d.data(), sizeof(double) * d.size());
+  } catch (...) {
+    return 1;
+  }
+  return 0;
+}
+
+// PhaseField::diffuseSolid (PhaseField.cpp:1005-1035) of a potential built
+// like ref_pf_state_create_ex; out has ngrid * nfluid entries.
+int ref_pf_diffuse_solid(const int* grid, int nfluid, double res, const double* st, int nst, const double* iw,
+                         int niw, int density, const PFExtOpts* o, const unsigned char* solid_in, int ifluid,
+                         int two_step, double* out) {
+  try {
+    CountingPF pot = make_pf_ext(grid, nfluid, res, st, nst, iw, niw, density, o);
+    const size_t ng = (size_t)grid[0] * grid[1] * grid[2];
+    PhaseField base(pot);
+    vector<double> r = base.diffuseSolid(vector<char>(solid_in, solid_in + ng), ifluid, two_step != 0);
+    std::memcpy(out, r.data(), sizeof(double) * r.size());
+  } catch (...) {
+    return 1;
+  }
+  return 0;
+}
+
 void ref_state_destroy(void* h) { delete static_cast<State*>(h); }
 long ref_state_ndof(void* h) { return (long)static_cast<State*>(h)->ndof; }
 

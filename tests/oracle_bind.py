@@ -35,6 +35,16 @@ class OrcPF(C.Structure):
                 ("fix", C.c_ubyte * ORC_MAXF), ("convergence", C.c_double)]
 
 
+class OrcPFX(C.Structure):
+    _fields_ = [("pf", OrcPF), ("ngrid", C.c_size_t), ("nkgrid", C.c_int), ("kappa", _dp), ("kappaP", _dp),
+                ("solid", C.POINTER(C.c_ubyte)), ("type", _ip), ("node_vol", _dp), ("surface_area", _dp),
+                ("contact_angle", _dp), ("pressure", C.c_double * ORC_MAXF), ("any_force", C.c_int),
+                ("fmag", C.c_double * ORC_MAXF), ("fnorm", (C.c_double * 3) * ORC_MAXF),
+                ("conf", C.c_double * ORC_MAXF), ("ff_init", _dp), ("volume_fixed", C.c_int),
+                ("vol_const", C.c_double), ("volume", C.c_double * ORC_MAXF), ("has_volume", C.c_int),
+                ("total_volume2", C.c_double)]
+
+
 class OrcStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("n_energy", C.c_long),
                 ("n_gradient", C.c_long), ("energy", C.c_double), ("grad_rms", C.c_double)]
@@ -70,6 +80,16 @@ def lib():
         L.orc_pf_energy_gradient.argtypes = [C.POINTER(OrcPF), _dp, _dp, _dp]
         L.orc_pf_constraints.argtypes = [C.POINTER(OrcPF), _dp]
         L.orc_pf_potential.argtypes = [C.POINTER(OrcPF), C.POINTER(OrcPot)]
+        _ub = C.POINTER(C.c_ubyte)
+        L.orc_pfx_init.argtypes = [C.POINTER(OrcPFX), _ip, C.c_int, C.c_double, _dp, C.c_int, _dp, C.c_int,
+                                   C.c_int, C.c_double, _ub, _ub, _dp, C.c_int, _dp, _dp, _dp, C.c_int,
+                                   C.c_double, _dp, _dp]
+        L.orc_pfx_free.argtypes = [C.POINTER(OrcPFX)]
+        L.orc_pfx_energy_gradient.argtypes = [C.POINTER(OrcPFX), _dp, _dp, _dp]
+        L.orc_pfx_constraints.argtypes = [C.POINTER(OrcPFX), _dp]
+        L.orc_pfx_components.argtypes = [C.POINTER(OrcPFX), _dp, _dp]
+        L.orc_pfx_potential.argtypes = [C.POINTER(OrcPFX), C.POINTER(OrcPot)]
+        L.orc_pf_site_types.argtypes = [_ip, _ub, _ip]
         L.orc_dot.argtypes = [_dp, _dp, C.c_size_t]
         L.orc_dot.restype = C.c_double
         L.orc_lbfgs_minimise.argtypes = [C.POINTER(OrcPot), _dp, C.c_int, C.c_int, C.c_double, C.c_double,
@@ -142,6 +162,133 @@ class OraclePF:
         data = np.array(data, dtype=np.float64)
         lib().orc_pf_constraints(C.byref(self.pf), _d(data))
         return data
+
+
+COMPONENTS = ("fluid", "surface", "pressure", "density constraint", "force", "confinement", "volume constraint")
+
+
+def expand_force(nfluid, force3, fluids=()):
+    """PhaseField::setForce (PhaseField.cpp:876-887): one force for every fluid
+    (one fluid, or no list), else only for the listed fluids."""
+    f = np.zeros((nfluid, 3))
+    if nfluid == 1 or len(fluids) == 0:
+        f[:] = force3
+    else:
+        for i in fluids:
+            f[i] = force3
+    return f
+
+
+def _opt(a, dt):
+    return None if a is None else np.ascontiguousarray(a, dtype=dt)
+
+
+class OraclePFX:
+    """CPU restatement of PhaseField with solid nodes, wetting, pressure, force,
+    confinement, per-site surface tensions and the volume projection."""
+
+    def __init__(self, grid, x0, nfluid=1, resolution=1.0, surface_tension=(1.0,), interface_size=(),
+                 density="hard", density_const=1.0, fix_fluid=None, solid=None, contact_angle=None,
+                 pressure=None, force=None, force_fluids=(), confinement=None, volume_fixed=False,
+                 vol_const=0.01, volume=None):
+        L = lib()
+        self.p = OrcPFX()
+        self.nfluid = nfluid
+        self._keep = dict(st=np.ascontiguousarray(surface_tension, dtype=np.float64),
+                          iw=np.ascontiguousarray(interface_size if len(interface_size) else [0.0], dtype=np.float64),
+                          x0=np.ascontiguousarray(x0, dtype=np.float64),
+                          solid=_opt(solid, np.uint8), ca=_opt(contact_angle, np.float64),
+                          pres=_opt(pressure, np.float64),
+                          force=None if force is None else np.ascontiguousarray(
+                              expand_force(nfluid, force, force_fluids)).reshape(-1),
+                          conf=_opt(confinement, np.float64), vol=_opt(volume, np.float64))
+        k = self._keep
+        fix = None
+        if fix_fluid is not None:
+            fix = (C.c_ubyte * nfluid)(*[1 if f else 0 for f in fix_fluid])
+
+        def dp(a):
+            return None if a is None else _d(a)
+
+        ub = None if k["solid"] is None else k["solid"].ctypes.data_as(C.POINTER(C.c_ubyte))
+        rc = L.orc_pfx_init(C.byref(self.p), (C.c_int * 3)(*grid), nfluid, resolution, _d(k["st"]), len(k["st"]),
+                            _d(k["iw"]), len(interface_size), DENSITY[density], density_const, fix, ub,
+                            dp(k["ca"]), 0 if k["ca"] is None else k["ca"].size, dp(k["pres"]), dp(k["force"]),
+                            dp(k["conf"]), 1 if volume_fixed else 0, vol_const, dp(k["vol"]), _d(k["x0"]))
+        if rc != 0:
+            raise ValueError("PhaseField: invalid parameters (code %d)" % rc)
+        self.ndof = int(np.prod(grid)) * nfluid
+        self.ngrid = int(np.prod(grid))
+        self.pot = OrcPot()
+        L.orc_pfx_potential(C.byref(self.p), C.byref(self.pot))
+
+    def __del__(self):
+        try:
+            lib().orc_pfx_free(C.byref(self.p))
+        except Exception:
+            pass
+
+    @property
+    def convergence(self):
+        return self.p.pf.convergence
+
+    @property
+    def node_vol(self):
+        return np.ctypeslib.as_array(self.p.node_vol, shape=(self.ngrid,)).copy()
+
+    @property
+    def surface_area(self):
+        return np.ctypeslib.as_array(self.p.surface_area, shape=(self.ngrid,)).copy()
+
+    @property
+    def site_type(self):
+        return np.ctypeslib.as_array(self.p.type, shape=(self.ngrid,)).copy()
+
+    @property
+    def volume(self):
+        return np.array(self.p.volume[: self.nfluid])
+
+    def energy(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        e = C.c_double(0)
+        lib().orc_pfx_energy_gradient(C.byref(self.p), _d(x), C.byref(e), None)
+        return e.value
+
+    def energy_gradient(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        g = np.zeros_like(x)
+        e = C.c_double(0)
+        lib().orc_pfx_energy_gradient(C.byref(self.p), _d(x), C.byref(e), _d(g))
+        return e.value, g
+
+    def gradient(self, x, constrained=True):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        g = np.zeros_like(x)
+        lib().orc_pfx_energy_gradient(C.byref(self.p), _d(x), None, _d(g))
+        if constrained:
+            lib().orc_pfx_constraints(C.byref(self.p), _d(g))
+        return g
+
+    def constrain(self, data):
+        data = np.array(data, dtype=np.float64)
+        lib().orc_pfx_constraints(C.byref(self.p), _d(data))
+        return data
+
+    def components(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(14)
+        lib().orc_pfx_components(C.byref(self.p), _d(x), _d(out))
+        return {n: [out[2 * i], out[2 * i + 1]] for i, n in enumerate(COMPONENTS)}
+
+
+def oracle_site_types(grid, solid):
+    s = np.ascontiguousarray(solid, dtype=np.uint8)
+    t = np.zeros(s.size, np.int32)
+    rc = lib().orc_pf_site_types((C.c_int * 3)(*grid), s.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                 t.ctypes.data_as(_ip))
+    if rc != 0:
+        raise RuntimeError("PhaseField: Undefined surface type")
+    return t
 
 
 def _stats(st):
@@ -281,6 +428,48 @@ class OracleBaH:
 
 
 # ---------------------------------------------------------------- reference (_ref)
+class RefPFExtOpts(C.Structure):
+    _fields_ = [("solid", C.POINTER(C.c_ubyte)), ("contact_angle", _dp), ("n_ca", C.c_int), ("pressure", _dp),
+                ("force3", _dp), ("force_fluids", _ip), ("n_force_fluids", C.c_int), ("confinement", _dp),
+                ("volume_fixed", C.c_int), ("vol_const", C.c_double), ("volume", _dp),
+                ("fix_fluid", C.POINTER(C.c_ubyte)), ("density_const", C.c_double)]
+
+
+def _ref_opts(nfluid, keep, solid=None, contact_angle=None, pressure=None, force=None, force_fluids=(),
+              confinement=None, volume_fixed=False, vol_const=0.01, volume=None, fix_fluid=None,
+              density_const=1.0):
+    o = RefPFExtOpts()
+    if solid is not None:
+        keep["solid"] = np.ascontiguousarray(solid, dtype=np.uint8)
+        o.solid = keep["solid"].ctypes.data_as(C.POINTER(C.c_ubyte))
+    if contact_angle is not None:
+        keep["ca"] = np.ascontiguousarray(contact_angle, dtype=np.float64)
+        o.contact_angle = _d(keep["ca"])
+        o.n_ca = keep["ca"].size
+    if pressure is not None:
+        keep["p"] = np.ascontiguousarray(pressure, dtype=np.float64)
+        o.pressure = _d(keep["p"])
+    if force is not None:
+        keep["f"] = np.ascontiguousarray(force, dtype=np.float64)
+        keep["ff"] = np.ascontiguousarray(list(force_fluids) or [0], dtype=np.int32)
+        o.force3 = _d(keep["f"])
+        o.force_fluids = keep["ff"].ctypes.data_as(_ip)
+        o.n_force_fluids = len(force_fluids)
+    if confinement is not None:
+        keep["c"] = np.ascontiguousarray(confinement, dtype=np.float64)
+        o.confinement = _d(keep["c"])
+    o.volume_fixed = 1 if volume_fixed else 0
+    o.vol_const = vol_const
+    if volume is not None:
+        keep["v"] = np.ascontiguousarray(volume, dtype=np.float64)
+        o.volume = _d(keep["v"])
+    if fix_fluid is not None:
+        keep["fix"] = np.ascontiguousarray([1 if f else 0 for f in fix_fluid], dtype=np.uint8)
+        o.fix_fluid = keep["fix"].ctypes.data_as(C.POINTER(C.c_ubyte))
+    o.density_const = density_const
+    return o
+
+
 class RefResult(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("n_energy", C.c_long),
                 ("n_gradient", C.c_long), ("energy", C.c_double), ("grad_rms", C.c_double)]
@@ -310,6 +499,14 @@ def reflib():
                                            C.c_double, _ip, C.c_int, _dp, _dp, C.c_int]
         L.ref_bah_elements.argtypes = [C.c_void_p, _ip, _ip, _dp, _dp, _ip, _ip, _dp, _dp, C.c_int]
         L.ref_bah_set_hinge_theta0.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.ref_pf_state_create_ex.restype = C.c_void_p
+        L.ref_pf_state_create_ex.argtypes = [_ip, C.c_int, C.c_double, _dp, C.c_int, _dp, C.c_int, C.c_int, _dp,
+                                             C.POINTER(RefPFExtOpts)]
+        L.ref_pf_components.argtypes = [C.c_void_p, _dp]
+        L.ref_pf_geometry.argtypes = [C.c_void_p, _dp, _dp]
+        L.ref_state_constraints.argtypes = [C.c_void_p, _dp]
+        L.ref_pf_diffuse_solid.argtypes = [_ip, C.c_int, C.c_double, _dp, C.c_int, _dp, C.c_int, C.c_int,
+                                           C.POINTER(RefPFExtOpts), C.POINTER(C.c_ubyte), C.c_int, C.c_int, _dp]
         L.ref_state_destroy.argtypes = [C.c_void_p]
         L.ref_state_ndof.argtypes = [C.c_void_p]
         L.ref_state_ndof.restype = C.c_long
@@ -340,6 +537,35 @@ class RefState:
         h = reflib().ref_pf_state_create((C.c_int * 3)(*grid), nfluid, resolution, _d(st), len(st), _d(iw),
                                          len(interface_size), DENSITY[density], _d(x))
         return RefState(h)
+
+    @staticmethod
+    def phasefield_ext(grid, x, nfluid=1, resolution=1.0, surface_tension=(1.0,), interface_size=(),
+                       density="hard", **opts):
+        keep = {}
+        o = _ref_opts(nfluid, keep, **opts)
+        st = np.ascontiguousarray(surface_tension, dtype=np.float64)
+        iw = np.ascontiguousarray(interface_size if len(interface_size) else [0.0], dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        h = reflib().ref_pf_state_create_ex((C.c_int * 3)(*grid), nfluid, resolution, _d(st), len(st), _d(iw),
+                                            len(interface_size), DENSITY[density], _d(x), C.byref(o))
+        r = RefState(h)
+        r.ngrid = int(np.prod(grid))
+        return r
+
+    def components(self):
+        out = np.zeros(14)
+        assert reflib().ref_pf_components(self.h, _d(out)) == 0
+        return {n: [out[2 * i], out[2 * i + 1]] for i, n in enumerate(COMPONENTS)}
+
+    def geometry(self):
+        nv, sa = np.zeros(self.ngrid), np.zeros(self.ngrid)
+        reflib().ref_pf_geometry(self.h, _d(nv), _d(sa))
+        return nv, sa
+
+    def constrain(self, data):
+        d = np.array(data, dtype=np.float64)
+        assert reflib().ref_state_constraints(self.h, _d(d)) == 0
+        return d
 
     @staticmethod
     def lj(ndim, x, sigma=1.0, epsilon=1.0):
@@ -403,3 +629,20 @@ class RefState:
         assert rc == 0
         return x, dict(iterations=r.iterations, converged=bool(r.converged), n_energy=r.n_energy,
                        n_gradient=r.n_gradient, energy=r.energy, grad_rms=r.grad_rms)
+
+
+def ref_diffuse_solid(grid, solid_in, nfluid=2, ifluid=0, two_step=False, resolution=1.0, surface_tension=(1.0,),
+                      interface_size=(), density="hard", **opts):
+    """PhaseField::diffuseSolid of a potential configured like phasefield_ext."""
+    keep = {}
+    o = _ref_opts(nfluid, keep, **opts)
+    st = np.ascontiguousarray(surface_tension, dtype=np.float64)
+    iw = np.ascontiguousarray(interface_size if len(interface_size) else [0.0], dtype=np.float64)
+    s = np.ascontiguousarray(solid_in, dtype=np.uint8)
+    out = np.zeros(int(np.prod(grid)) * nfluid)
+    rc = reflib().ref_pf_diffuse_solid((C.c_int * 3)(*grid), nfluid, resolution, _d(st), len(st), _d(iw),
+                                       len(interface_size), DENSITY[density], C.byref(o),
+                                       s.ctypes.data_as(C.POINTER(C.c_ubyte)), ifluid, 1 if two_step else 0,
+                                       _d(out))
+    assert rc == 0
+    return out

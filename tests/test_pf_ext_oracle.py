@@ -1,0 +1,407 @@
+"""Pins the oracle's PhaseField terms beyond bulk and gradient energies (solid
+nodes, wetting, pressure, external force, confinement, per-site surface
+tensions, the fixed-volume projection and energyComponents) before the GPU
+tests trust it.
+
+Checks: the reference's unit-test known answers (test/unit/PhaseField_test.cpp
+:31-68, 71-89, 92-98, 101-107, 130-171, 181-197), the live reference build when
+present (energies, constrained gradients, projections, components, node
+volumes, minimiser trajectories), and what the mathematics fixes: finite
+differences of the energy, the closed form of each term, and that the volume
+projection removes exactly the volume change.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle_bind import OraclePFX, RefState, oracle_lbfgs, oracle_site_types, ref_available, ref_diffuse_solid
+
+needs_ref = pytest.mark.skipif(not ref_available(), reason="oracle/_ref (reference build) not present")
+
+
+def near(a, b, tol=1e-6):
+    # ArraysNear / EXPECT_FLOAT_EQ of the reference tests: single-precision closeness
+    np.testing.assert_allclose(np.asarray(a, float), np.asarray(b, float), rtol=tol, atol=tol)
+
+
+# ---------------------------------------------------------------- known answers
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_solid_walls_known_answer(axis):
+    # PhaseField_test.cpp:44-57: gradient of 1 over the two half-fluid nodes
+    grid = [1, 1, 1]
+    grid[axis] = 5
+    x = [0, -1, 0, 1, 0]
+    o = OraclePFX(grid, x, solid=[1, 0, 0, 0, 1])
+    near(o.energy(x), 3.0 / 16 + 2 * (3.0 / 4))
+    near(o.gradient(x), [0, -1.5, 0, 1.5, 0])
+    # :58-68: periodic boundary through a solid node
+    grid = [1, 1, 1]
+    grid[axis] = 3
+    x = [-1, 0, 1]
+    o = OraclePFX(grid, x, solid=[0, 1, 0])
+    near(o.energy(x), 3.0 / 4 * 4)
+    near(o.gradient(x), [-3, 0, 3])
+
+
+def test_external_force_known_answer():
+    # PhaseField_test.cpp:71-89
+    ones = [1.0] * 8
+    for f, g in (([-2, 0, 0], [-.5] * 4 + [.5] * 4),
+                 ([0, -2, 0], [-.5, -.5, .5, .5, -.5, -.5, .5, .5]),
+                 ([0, 0, -2], [-.5, .5, -.5, .5, -.5, .5, -.5, .5])):
+        o = OraclePFX((2, 2, 2), ones, force=f)
+        near(o.energy(ones), 0)
+        near(o.gradient(ones), g)
+    x = [-1.0] * 4 + [1.0] * 4
+    o = OraclePFX((2, 2, 2), x, surface_tension=[0.0], force=[-4, 0, 0])
+    near(o.energy(x), 8)
+    near(o.gradient(x), [0] * 4 + [1] * 4)
+
+
+def test_surface_energy_known_answer():
+    # PhaseField_test.cpp:92-98
+    x = [0.0, 0.5]
+    o = OraclePFX((2, 1, 1), x, surface_tension=[0.0], contact_angle=[90, 60], solid=[1, 0])
+    near(o.energy(x), 0.5 / math.sqrt(2.0) * (-27.0 / 24))
+    near(o.gradient(x), [0, 0.5 / math.sqrt(2.0) * (-0.75)])
+
+
+def test_pressure_known_answer():
+    # PhaseField_test.cpp:101-107
+    x = [1.0] * 6
+    o = OraclePFX((6, 1, 1), x, solid=[1, 0, 0, 0, 0, 1], pressure=[10])
+    near(o.energy(x), -30)
+    near(o.gradient(x), [0, -2.5, -5, -5, -2.5, 0])
+
+
+def test_resolution_known_answers():
+    # PhaseField_test.cpp:130-171 (resolution 2)
+    x = [0, -1, 0, 1, 0]
+    o = OraclePFX((5, 1, 1), x, resolution=2.0, solid=[1, 0, 0, 0, 1])
+    near(o.energy(x), 1.5 / 16 * 8 + 2 * (6.0 / 4) * 2)
+    near(o.gradient(x), [0, -6, 0, 6, 0])
+    x = [-1.0] * 4 + [1.0] * 4
+    o = OraclePFX((2, 2, 2), x, resolution=2.0, surface_tension=[0.0], solid=[0] * 8, force=[-4, 0, 0])
+    near(o.energy(x), 128)
+    near(o.gradient(x), [0] * 4 + [16] * 4)
+    x = [0.0, 0.5]
+    o = OraclePFX((2, 1, 1), x, resolution=2.0, surface_tension=[0.0], solid=[1, 0], contact_angle=[90, 60],
+                  force=[0, 0, 0])
+    near(o.energy(x), 0.5 / math.sqrt(2.0) * (-27.0 / 24) * 4)
+    near(o.gradient(x), [0, 0.5 / math.sqrt(2.0) * (-0.75) * 4])
+    x = [1.0] * 6
+    o = OraclePFX((6, 1, 1), x, resolution=2.0, surface_tension=[0.0], solid=[1, 0, 0, 0, 0, 1],
+                  contact_angle=[90] * 6, pressure=[10], force=[0, 0, 0])
+    near(o.energy(x), -240)
+    near(o.gradient(x), [0, -20, -40, -40, -20, 0])
+
+
+def test_fix_fluid_known_answer():
+    # PhaseField_test.cpp:181-197
+    x = [1.0] * 12
+    o = OraclePFX((2, 2, 1), x, nfluid=3, density="none", fix_fluid=[1, 0, 0])
+    assert np.all(o.gradient(x)[0::3] == 0)
+
+
+# ---------------------------------------------------------------- site types
+def test_site_types_by_hand():
+    # a wall at x = 0 of a 4x4x4 grid: its neighbours see one solid face and
+    # solid edges only along the wall (type 4, volume 1/2, area 1)
+    s = np.zeros((4, 4, 4), np.uint8)
+    s[0] = 1
+    t = oracle_site_types((4, 4, 4), s.reshape(-1)).reshape(4, 4, 4)
+    assert np.all(t[0] == -1) and np.all(t[1] == 4) and np.all(t[3] == 4) and np.all(t[2] == 0)
+    # one solid site: faces see type 6? no -- one face and no edge across it -> 4;
+    # edge neighbours see a single edge -> 2; corner neighbours a single corner -> 1
+    s = np.zeros((5, 5, 5), np.uint8)
+    s[2, 2, 2] = 1
+    t = oracle_site_types((5, 5, 5), s.reshape(-1)).reshape(5, 5, 5)
+    assert t[1, 2, 2] == 4 and t[2, 1, 2] == 4 and t[2, 2, 3] == 4
+    assert t[1, 1, 2] == 2 and t[2, 3, 1] == 2
+    assert t[1, 1, 1] == 1 and t[3, 3, 1] == 1
+    assert t[0, 2, 2] == 0
+    # inner corner of an L-shaped wall: two solid faces -> 6; three -> 7
+    s = np.zeros((5, 5, 5), np.uint8)
+    s[0] = 1
+    s[:, 0] = 1
+    t = oracle_site_types((5, 5, 5), s.reshape(-1)).reshape(5, 5, 5)
+    assert t[1, 1, 2] == 6
+    s[:, :, 0] = 1
+    t = oracle_site_types((5, 5, 5), s.reshape(-1)).reshape(5, 5, 5)
+    assert t[1, 1, 1] == 7
+    # two solid edges meeting only diagonally with three corners is undefined
+    s = np.zeros((5, 5, 5), np.uint8)
+    s[1, 1, 2] = s[3, 3, 2] = s[1, 3, 2] = 1
+    with pytest.raises(RuntimeError):
+        oracle_site_types((5, 5, 5), s.reshape(-1))
+
+
+def structured_solid(grid, rng, kind):
+    """Walls, a channel, a pillar on a base or a box, along the dimensions of
+    size >= 3 (so that fluid sites remain)."""
+    s = np.zeros(grid, np.uint8)
+    ax = [d for d in range(3) if grid[d] >= 3]
+    if not ax:
+        return s.reshape(-1)
+
+    def sl(d, a, b):
+        idx = [slice(None)] * 3
+        idx[d] = slice(a, b)
+        return tuple(idx)
+
+    if kind == "wall":
+        s[sl(ax[0], 0, 1)] = 1
+    elif kind == "channel":
+        d = ax[-1]
+        s[sl(d, 0, 1)] = 1
+        s[sl(d, grid[d] - 1, grid[d])] = 1
+    elif kind == "pillar" and len(ax) >= 2:
+        a, b = ax[0], ax[1]
+        s[sl(b, 0, 1)] = 1
+        idx = [slice(None)] * 3
+        idx[a] = slice(0, max(1, grid[a] // 3))
+        idx[b] = slice(0, max(2, 2 * grid[b] // 3))
+        s[tuple(idx)] = 1
+    elif kind == "box" and len(ax) == 3:
+        s[1:3, 1:3, 1:3] = 1
+    else:
+        s[sl(ax[0], 0, 1)] = 1
+    return s.reshape(-1)
+
+
+@needs_ref
+@pytest.mark.parametrize("kind", ["wall", "channel", "pillar", "box"])
+@pytest.mark.parametrize("grid", [(6, 7, 5), (8, 6, 1), (1, 9, 7), (5, 2, 4)])
+def test_node_volumes_vs_reference(grid, kind):
+    rng = np.random.default_rng(3)
+    solid = structured_solid(grid, rng, kind)
+    x = np.zeros(int(np.prod(grid)))
+    try:
+        o = OraclePFX(grid, x, solid=solid, resolution=0.5)
+    except ValueError:
+        with pytest.raises(ValueError):
+            RefState.phasefield_ext(grid, x, resolution=0.5, solid=solid)
+        return
+    r = RefState.phasefield_ext(grid, x, resolution=0.5, solid=solid)
+    nv, sa = r.geometry()
+    assert np.array_equal(o.node_vol, nv)
+    assert np.array_equal(o.surface_area, sa)
+
+
+# ---------------------------------------------------------------- vs the reference
+def random_case(seed):
+    """A small PhaseField with a random selection of the extra terms."""
+    rng = np.random.default_rng(seed)
+    grids = [(6, 5, 4), (7, 6, 1), (1, 8, 6), (4, 4, 4), (5, 2, 6), (3, 6, 5)]
+    grid = grids[seed % len(grids)]
+    nf = [1, 2, 3, 3, 4][seed % 5]
+    ng = int(np.prod(grid))
+    opts = {}
+    if rng.random() < 0.7:
+        opts["solid"] = structured_solid(grid, rng, ["wall", "channel", "pillar", "box"][rng.integers(4)])
+    if nf == 1 and rng.random() < 0.6:
+        opts["contact_angle"] = rng.uniform(20, 160, ng)
+    if rng.random() < 0.6:
+        opts["pressure"] = rng.uniform(-2, 2, nf) * (rng.random(nf) < 0.7)
+    if rng.random() < 0.6:
+        opts["force"] = rng.uniform(-1, 1, 3)
+        if nf > 1 and rng.random() < 0.5:
+            opts["force_fluids"] = [int(rng.integers(nf))]
+    if rng.random() < 0.5:
+        opts["confinement"] = rng.uniform(0, 3, nf) * (rng.random(nf) < 0.7)
+    if nf > 1 and rng.random() < 0.5:
+        opts["volume_fixed"] = True
+        opts["vol_const"] = float(rng.uniform(0.005, 0.05))
+    if nf > 1 and rng.random() < 0.4:
+        fix = [False] * nf
+        fix[int(rng.integers(nf))] = True
+        opts["fix_fluid"] = fix
+    dens = ["hard", "soft", "none", "hard"][rng.integers(4)] if nf > 1 else "hard"
+    st = list(rng.uniform(0.5, 1.5, nf if nf > 1 else 1))
+    if rng.random() < 0.3:
+        st = list(rng.uniform(0.5, 1.5, ng * nf))  # per-site tensions
+    iw = [float(rng.uniform(1.0, 2.5))]
+    res = float(rng.choice([1.0, 0.5, 2.0]))
+    x0 = rng.uniform(-1.1, 1.1, ng) if nf == 1 else rng.uniform(-0.1, 1.1, ng * nf)
+    if "solid" in opts and nf > 1:
+        x0.reshape(ng, nf)[opts["solid"] == 1] = 0
+    return dict(grid=grid, nfluid=nf, resolution=res, surface_tension=st, interface_size=iw, density=dens,
+                opts=opts, x0=x0, rng=rng)
+
+
+def make_pair(c):
+    """Oracle and reference for one case; (None, None) when both reject it (an
+    undefined surface type of the solid layout)."""
+    try:
+        o = OraclePFX(c["grid"], c["x0"], nfluid=c["nfluid"], resolution=c["resolution"],
+                      surface_tension=c["surface_tension"], interface_size=c["interface_size"],
+                      density=c["density"], **c["opts"])
+    except ValueError:
+        with pytest.raises(ValueError):
+            RefState.phasefield_ext(c["grid"], c["x0"], c["nfluid"], c["resolution"], c["surface_tension"],
+                                    c["interface_size"], c["density"], **c["opts"])
+        return None, None
+    o = OraclePFX(c["grid"], c["x0"], nfluid=c["nfluid"], resolution=c["resolution"],
+                  surface_tension=c["surface_tension"], interface_size=c["interface_size"], density=c["density"],
+                  **c["opts"])
+    r = RefState.phasefield_ext(c["grid"], c["x0"], c["nfluid"], c["resolution"], c["surface_tension"],
+                                c["interface_size"], c["density"], **c["opts"])
+    return o, r
+
+
+def close(a, b, rel=1e-13):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    scale = max(1.0, float(np.max(np.abs(b))) if b.size else 1.0)
+    assert np.max(np.abs(a - b)) <= rel * scale, (np.max(np.abs(a - b)), scale)
+
+
+@needs_ref
+@pytest.mark.parametrize("seed", range(40))
+def test_extended_terms_vs_reference(seed):
+    c = random_case(seed)
+    o, r = make_pair(c)
+    if o is None:
+        return
+    rng = c["rng"]
+    for x in (c["x0"], c["x0"] + rng.normal(0, 0.05, c["x0"].size)):
+        # energy-only and gradient-only calls, as the minimisers make them
+        close(o.energy(x), r.energy(x))
+        close(o.gradient(x), r.gradient(x))
+        e, g = r.energy_gradient(x)
+        eo, go = o.energy_gradient(x)
+        close(eo, e)
+        # State::energyGradient applies the constraints to g
+        close(o.constrain(go), g)
+    v = rng.normal(0, 1, c["x0"].size)
+    close(o.constrain(v), r.constrain(v))
+    oc, rc = o.components(c["x0"]), r.components()
+    for k in rc:
+        close(oc[k], rc[k], 1e-12)
+
+
+@needs_ref
+def test_force_energy_only_counts_one_fluid():
+    # PhaseField.cpp:511: without a gradient forceEnergy returns after the first
+    # contributing fluid, so E-only and E+G energies differ with two forced fluids
+    grid = (3, 3, 3)
+    x = np.full(27 * 2, 0.5)
+    kw = dict(nfluid=2, surface_tension=[1.0], interface_size=[1.0], density="none")
+    o = OraclePFX(grid, x, force=[0, 0, -1.0], **kw)
+    r = RefState.phasefield_ext(grid, x, force=[0, 0, -1.0], **kw)
+    assert o.energy(x) == r.energy(x)
+    assert o.energy_gradient(x)[0] == r.energy_gradient(x)[0]
+    x2 = x.copy()
+    x2[1::2] = 0.3
+    assert o.energy(x2) != o.energy_gradient(x2)[0]
+
+
+@needs_ref
+@pytest.mark.parametrize("seed", [1, 4, 7])
+def test_lbfgs_with_extended_terms_vs_reference(seed):
+    o = None
+    while o is None:  # the next seed whose solid layout the reference accepts
+        c = random_case(100 + seed)
+        c["opts"].pop("contact_angle", None)
+        o, r = make_pair(c)
+        seed += 1000
+    xo, so = oracle_lbfgs(o, c["x0"], m=5, max_iter=60, linesearch="none")
+    xr, sr = r.minimise("lbfgs", m=5, max_iter=60, linesearch="none")
+    assert so["iterations"] == sr["iterations"]
+    close(xo, xr, 1e-9)
+    close(so["energy"], sr["energy"], 1e-10)
+
+
+# ---------------------------------------------------------------- mathematics
+@pytest.mark.parametrize("seed", range(12))
+def test_extended_gradient_is_energy_derivative(seed):
+    c = random_case(200 + seed)
+    c["opts"].pop("confinement", None)  # its switch at c = 0.5 is not differentiable
+    o = OraclePFX(c["grid"], c["x0"], nfluid=c["nfluid"], resolution=c["resolution"],
+                  surface_tension=c["surface_tension"], interface_size=c["interface_size"], density=c["density"],
+                  **{k: v for k, v in c["opts"].items() if k not in ("volume_fixed", "vol_const")})
+    x = c["x0"]
+    # the force term skips c < 0.01 (a step): keep every concentration above it
+    x = np.where(np.abs(x - 0.01) < 0.02, 0.2, x) if c["nfluid"] > 1 else np.where(np.abs(x + 0.98) < 0.04, 0.3, x)
+    e0, g = o.energy_gradient(x)
+    rng = np.random.default_rng(seed)
+    for i in rng.choice(x.size, size=min(8, x.size), replace=False):
+        h = 1e-6
+        xp, xm = x.copy(), x.copy()
+        xp[i] += h
+        xm[i] -= h
+        fd = (o.energy_gradient(xp)[0] - o.energy_gradient(xm)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), (i, fd, g[i])
+
+
+def test_pressure_and_confinement_closed_form():
+    # pressure: E = -sum_k p_k c_k v; confinement: strength * mean tension *
+    # res^2 * (c-0.5)^2 where c crossed 0.5 since the start (the components
+    # isolate each term)
+    grid = (4, 3, 2)
+    rng = np.random.default_rng(9)
+    x0 = rng.uniform(0, 1, 24 * 2)
+    kw = dict(nfluid=2, surface_tension=[1.2, 0.8], interface_size=[1.0], density="none", resolution=0.5)
+    x = rng.uniform(0, 1, 48)
+    o = OraclePFX(grid, x0, pressure=[1.5, -0.5], **kw)
+    v = 0.5 ** 3
+    want = -(1.5 * v * x[0::2].sum() - 0.5 * v * x[1::2].sum())
+    assert math.isclose(o.components(x)["pressure"][0], want, rel_tol=1e-13)
+    assert math.isclose(o.components(x)["pressure"][1], v * math.sqrt(24 * (1.5 ** 2 + 0.5 ** 2)), rel_tol=1e-13)
+    o = OraclePFX(grid, x0, confinement=[2.0, 0.0], **kw)
+    flip = (x0[0::2] - 0.5) * (x[0::2] - 0.5) < 0
+    coef = 2.0 * 1.0 * 0.25
+    want = coef * np.sum((x[0::2][flip] - 0.5) ** 2)
+    assert flip.sum() > 3
+    assert math.isclose(o.components(x)["confinement"][0], want, rel_tol=1e-13)
+    e_all = o.energy(x)
+    comp = o.components(x)
+    assert math.isclose(e_all, sum(comp[k][0] for k in comp), rel_tol=1e-13)
+
+
+@pytest.mark.parametrize("density", ["hard", "none"])
+def test_volume_projection_removes_volume_change(density):
+    grid = (6, 5, 4)
+    rng = np.random.default_rng(11)
+    solid = structured_solid(grid, rng, "channel")
+    nf = 3
+    x0 = rng.uniform(0, 1, 120 * nf)
+    o = OraclePFX(grid, x0, nfluid=nf, surface_tension=[1.0, 0.8, 1.2], interface_size=[2.0], density=density,
+                  solid=solid, volume_fixed=True, fix_fluid=[True, False, False])
+    v = o.constrain(rng.normal(0, 1, x0.size)).reshape(-1, nf)
+    nv = o.node_vol
+    assert np.all(v[:, 0] == 0)
+    for k in (1, 2):
+        assert abs(np.dot(v[:, k], nv)) < 1e-12 * np.abs(v).sum()
+    if density == "hard":
+        assert np.max(np.abs(v.sum(axis=1))) < 1e-14 * 10
+    # a projection: applying it twice changes nothing beyond rounding
+    v2 = o.constrain(v.reshape(-1)).reshape(-1, nf)
+    assert np.max(np.abs(v2 - v)) < 1e-14 * np.abs(v).max() * 10
+
+
+@needs_ref
+@pytest.mark.parametrize("two_step", [False, True])
+def test_diffuse_solid_vs_reference(two_step):
+    # PhaseField::diffuseSolid (PhaseField.cpp:1005-1035): 100 L-BFGS steps
+    # without line search on a confined copy of the potential
+    grid = (8, 7, 1)
+    rng = np.random.default_rng(5)
+    solid = structured_solid(grid, rng, "channel")
+    sub = np.zeros(56, np.uint8)
+    sub[(np.arange(56) // 7) < 3] = 1
+    kw = dict(nfluid=3, surface_tension=[1.0, 0.8, 1.2], interface_size=[1.0], density="hard")
+    ref = ref_diffuse_solid(grid, sub, ifluid=0, two_step=two_step, solid=solid, **kw)
+    # the same procedure on the oracle
+    init = np.zeros(56 * 3)
+    init[0::3] = sub
+    conf = np.zeros(3)
+    conf[0] = 100
+    o = OraclePFX(grid, init, nfluid=3, surface_tension=kw["surface_tension"], interface_size=[1.0],
+                  density="none", solid=solid, confinement=conf)
+    x, _ = oracle_lbfgs(o, init, m=5, max_iter=100, linesearch="none", convergence=1e-8 * 1.0)
+    if two_step:
+        o = OraclePFX(grid, x, nfluid=3, surface_tension=kw["surface_tension"], interface_size=[1.0],
+                      density="none", solid=solid, confinement=conf)
+        x, _ = oracle_lbfgs(o, x, m=5, max_iter=100, linesearch="none", convergence=1e-8 * 1.0)
+    close(x, ref, 1e-9)
