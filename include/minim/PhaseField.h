@@ -1,9 +1,12 @@
 // PhaseField.h -- device PhaseField potential, mirroring the reference's
-// PhaseField (include/potentials/PhaseField.h:13-123) for bulk, gradient and
-// density-constraint terms on a periodic grid without solid nodes.
+// PhaseField (include/potentials/PhaseField.h:13-123): bulk, gradient and
+// density-constraint terms, solid nodes with wetting, pressure, fixed volumes,
+// external forces, confinement and per-site surface tensions on a periodic grid.
 #ifndef MINIM_AMD_PHASEFIELD_H
 #define MINIM_AMD_PHASEFIELD_H
 
+#include <functional>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -30,15 +33,45 @@ class PhaseField : public NewPotential<PhaseField> {
   PhaseField& setInterfaceSize(vector<double> interfaceSize);
   PhaseField& setSurfaceTension(double surfaceTension);
   PhaseField& setSurfaceTension(vector<double> surfaceTension);
+  // per-site tensions: fn(x, y, z) gives the nParams values of a site
+  PhaseField& setSurfaceTension(std::function<vector<double>(int, int, int)> surfaceTensionFn);
 
   // Density constraint: "hard"/"gradient" (default), "soft"/"energy penalty", "none", "fixed"
   int densityConstraint = 0;
   double densityConst = 1;
   PhaseField& setDensityConstraint(std::string method);
 
-  // Frozen fluids (gradient zeroed)
+  // Total volume / pressure constraints (PhaseField.h:37-44)
+  bool volumeFixed = false;
+  double volConst = 0.01;
+  vector<double> pressure;
+  vector<double> volume;
+  PhaseField& setPressure(vector<double> pressure);
+  PhaseField& setVolume(vector<double> volume, double volConst = 0.01);
+  PhaseField& setVolumeFixed(bool volumeFixed, double volConst = 0.01);
+
+  // Solid nodes (PhaseField.h:47-54)
+  vector<char> solid;
+  vector<double> contactAngle;
+  PhaseField& setSolid(vector<char> solid);
+  PhaseField& setSolid(std::function<bool(int, int, int)> solidFn);
+  PhaseField& setContactAngle(double contactAngle);
+  PhaseField& setContactAngle(vector<double> contactAngle);
+  PhaseField& setContactAngle(std::function<double(int, int, int)> contactAngleFn);
+
+  // External force (PhaseField.h:57-58)
+  vector<vector<double>> force;
+  PhaseField& setForce(vector<double> force, vector<int> iFluid = {});
+
+  // Frozen fluids (gradient zeroed) and the diffuse-solid method (PhaseField.h:61-68)
   vector<char> fixFluid;
+  vector<double> confinementStrength;
   PhaseField& setFixFluid(int iFluid, bool fix = true);
+  PhaseField& setConfinement(vector<double> strength);
+  vector<double> diffuseSolid(vector<char> solid, int iFluid = 0, bool twoStep = false);
+  static vector<double> diffuseSolid(vector<char> solid, PhaseField potential, int iFluid = 0, bool twoStep = false);
+  static vector<double> diffuseSolid(vector<char> solid, vector<int> gridSize, int nFluid = 2, int iFluid = 0,
+                                     bool twoStep = false);
 
   // Overrides
   void init(const vector<double>& coords) override;
@@ -46,25 +79,38 @@ class PhaseField : public NewPotential<PhaseField> {
                 void* stream) const override;
   void evaluateAsync(int prec, const void* x, const void* d, double t, bool wantE, double* e_dev, void* g,
                      void* stream) const override;
-  bool supportsMixed() const override { return nFluid <= 4; }
+  bool supportsMixed() const override;
   void evaluateMixed(const void* x, const void* d, double t, double* e, void* g, void* stream) const override;
   void evaluateMixedAsync(const void* x, const void* d, double t, bool wantE, double* e_dev, void* g,
                           void* stream) const override;
   double timeKernel(int prec, const void* x, bool wantE, void* g, int reps, void* stream) const override;
   void applyConstraints(int prec, void* v, size_t n, void* stream) const override;
   SiteProjection projection() const override;
+  bool hasGlobalConstraints() const override { return volumeFixed; }
+
+  // PhaseField::energyComponents (PhaseField.cpp:706-744): energy and gradient
+  // norm (unconstrained) of each term at the given coordinates, computed on
+  // the GPU in FP64.
+  std::map<std::string, vector<double>> energyComponents(const vector<double>& coords) const;
 
   // Read only (after init)
   int nParams = 0;
   double surfaceTensionMean = 0;
-  vector<double> kappa;
+  vector<double> kappa;   // nParams, or nGrid * nParams with per-site tensions
   vector<double> kappaP;
   int nGrid = 0;
+  vector<double> nodeVol;      // per site, filled when solid nodes are present
+  vector<double> surfaceArea;  // (uniform otherwise: resolution^3 and 0)
+  double totalVolume2 = 0;     // sum of squared node volumes (fixed volume)
+  vector<double> fMag;
+  vector2d<double> fNorm;
 
   // Launch tuning: x-march length of the stencil and target block count
-  // (0 = automatic for both).
+  // (0 = automatic for both); stencilKernel 1 forces the general kernel
+  // (solid-aware, every term) where the tiled one would run.
   int stencilXChunk = 0;
   int stencilTargetBlocks = 0;
+  int stencilKernel = 0;
 
  private:
   void setDefaults();
@@ -72,6 +118,11 @@ class PhaseField : public NewPotential<PhaseField> {
   void assignFluidCoefficients();
   std::shared_ptr<void> args_;  // PFArgs
 };
+
+// PhaseField::getType (PhaseField.cpp:899-958) of every site of a periodic
+// grid: -1 solid, 0 bulk fluid, 1..7 surface types.  Throws
+// std::runtime_error where the reference finds an undefined surface type.
+std::vector<int> pf_site_types(const int grid[3], const char* solid);
 
 }  // namespace minim
 

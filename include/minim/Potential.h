@@ -80,6 +80,10 @@ class Potential {
   // whose constraints are only fixed DOFs report nf = 1, fixmask = 0: their
   // basis vectors already hold exact zeros there.
   virtual SiteProjection projection() const { return SiteProjection{}; }
+  // True when applyConstraints also couples all DOFs (PhaseField's fixed
+  // total volumes, PhaseField.cpp:614-658): minimisers then apply it to each
+  // step in addition to the per-site projection.
+  virtual bool hasGlobalConstraints() const { return false; }
 
  protected:
   std::shared_ptr<void> fixedIdxDev_;  // device int[fixedDofs.size()]

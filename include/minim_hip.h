@@ -53,8 +53,38 @@ typedef struct minim_pf_params {
   const unsigned char* fix_fluid; /* NULL or n_fluid flags (setFixFluid) */
   int stencil_xchunk;             /* 0 = automatic x-march length */
   int stencil_target_blocks;      /* 0 = automatic */
+  /* The remaining terms (PhaseField.h:36-68, PhaseField.cpp:818-895); NULL / 0
+   * leaves a setter uncalled.  n_surface_tension may also be
+   * ngrid * n_fluid: per-site tensions. */
+  const unsigned char* solid;     /* ngrid flags: setSolid */
+  const double* contact_angle;    /* setContactAngle(vector) */
+  size_t n_contact_angle;
+  const double* pressure;         /* n_fluid: setPressure */
+  const double* force;            /* 3: setForce(force, force_fluids) */
+  const int* force_fluids;
+  int n_force_fluids;
+  const double* confinement;      /* n_fluid: setConfinement */
+  int volume_fixed;               /* setVolumeFixed(true, vol_const) ... */
+  double vol_const;
+  const double* volume;           /* ... or setVolume(volume, vol_const): n_fluid */
+  int stencil_kernel;             /* 0 automatic, 1 the general (solid-aware) kernel */
 } minim_pf_params;
 int minim_pf_create(const minim_pf_params* p, minim_potential** out);
+
+/* PhaseField::energyComponents (PhaseField.cpp:706-744) at host coordinates
+ * (initialised potential): out[2c], out[2c+1] = energy and unconstrained
+ * gradient norm of c = fluid, surface, pressure, density constraint, force,
+ * confinement, volume constraint. */
+int minim_pf_energy_components(const minim_potential* pot, const double* coords, size_t ndof, double* out);
+/* nodeVol / surfaceArea after init (PhaseField.cpp:190-214), ngrid each (host). */
+int minim_pf_node_volumes(const minim_potential* pot, double* node_vol, double* surface_area);
+/* PhaseField::diffuseSolid(solid, iFluid, twoStep) (PhaseField.cpp:998-1035)
+ * on a copy of the configured potential; out: ngrid * n_fluid host doubles. */
+int minim_pf_diffuse_solid(const minim_potential* pot, const unsigned char* solid, int i_fluid, int two_step,
+                           double* out);
+/* PhaseField::getType (PhaseField.cpp:899-958) of every site: type[ngrid]
+ * (-1 solid, 0 bulk, 1..7).  Host only. */
+int minim_pf_site_types(const int grid[3], const unsigned char* solid, int* type);
 
 /* kappa / kappaP per fluid after init (PhaseField::assignFluidCoefficients, PhaseField.cpp:78-134). */
 int minim_pf_coefficients(const minim_potential* pot, double* kappa, double* kappa_p);

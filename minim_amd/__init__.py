@@ -10,7 +10,7 @@ missing library raises ImportError.
 from . import _capi
 from .api import (BarAndHinge, Fire, GradDescent, Lbfgs, Lj2d, Lj3d, LjNd, Minimiser, PhaseField, Potential,
                   State, dot, potential_constrain, potential_eval, potential_eval_async, potential_eval_mixed,
-                  potential_time_kernel, triangulation_lists)
+                  potential_time_kernel, site_types, triangulation_lists)
 
 
 def __getattr__(name):

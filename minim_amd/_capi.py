@@ -22,7 +22,11 @@ class PFParams(C.Structure):
                 ("interface_size", dp), ("n_interface_size", C.c_int),
                 ("density_constraint", C.c_int), ("density_const", C.c_double),
                 ("fix_fluid", C.POINTER(C.c_ubyte)), ("stencil_xchunk", C.c_int),
-                ("stencil_target_blocks", C.c_int)]
+                ("stencil_target_blocks", C.c_int),
+                ("solid", C.POINTER(C.c_ubyte)), ("contact_angle", dp), ("n_contact_angle", C.c_size_t),
+                ("pressure", dp), ("force", dp), ("force_fluids", ip), ("n_force_fluids", C.c_int),
+                ("confinement", dp), ("volume_fixed", C.c_int), ("vol_const", C.c_double), ("volume", dp),
+                ("stencil_kernel", C.c_int)]
 
 
 class BaHParams(C.Structure):
@@ -55,6 +59,10 @@ SIGNATURES = {
     "minim_abi_version": (C.c_int, []),
     "minim_pf_create": (C.c_int, [C.POINTER(PFParams), C.POINTER(vp)]),
     "minim_pf_coefficients": (C.c_int, [vp, dp, dp]),
+    "minim_pf_energy_components": (C.c_int, [vp, dp, C.c_size_t, dp]),
+    "minim_pf_node_volumes": (C.c_int, [vp, dp, dp]),
+    "minim_pf_diffuse_solid": (C.c_int, [vp, C.POINTER(C.c_ubyte), C.c_int, C.c_int, dp]),
+    "minim_pf_site_types": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_ubyte), C.POINTER(C.c_int)]),
     "minim_lj_create": (C.c_int, [C.c_int, C.c_double, C.c_double, C.POINTER(vp)]),
     "minim_lj_set_cutoff": (C.c_int, [vp, C.c_double, dp, C.c_double]),
     "minim_bah_create": (C.c_int, [C.POINTER(BaHParams), C.POINTER(vp)]),
@@ -92,7 +100,7 @@ SIGNATURES = {
 _lib = None
 
 
-ABI_VERSION = 2  # layout of the parameter/result structs below
+ABI_VERSION = 3  # layout of the parameter/result structs below
 
 
 def load():

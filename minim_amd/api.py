@@ -70,7 +70,9 @@ class Potential:
 
 
 class PhaseField(Potential):
-    """include/potentials/PhaseField.h:13-123 (bulk, gradient, density constraint, fixed fluids)."""
+    """include/potentials/PhaseField.h:13-123: bulk, gradient and density terms,
+    fixed fluids, solid nodes and wetting, pressure, fixed volumes, external
+    forces, confinement and per-site surface tensions."""
 
     potentialType = 2
 
@@ -86,6 +88,16 @@ class PhaseField(Potential):
         self.fixFluid = []
         self.stencilXChunk = 0
         self.stencilTargetBlocks = 0
+        self.stencilKernel = 0
+        self.pressure = []
+        self.volumeFixed = False
+        self.volConst = 0.01
+        self.volume = []
+        self.solid = []
+        self.contactAngle = []
+        self.force = None
+        self.forceFluids = []
+        self.confinementStrength = []
 
     def setNFluid(self, n):
         self.nFluid = int(n)
@@ -105,7 +117,62 @@ class PhaseField(Potential):
         return self
 
     def setSurfaceTension(self, s):
-        self.surfaceTension = [float(s)] * self.nFluid if np.isscalar(s) else [float(v) for v in s]
+        """A value, a vector (nParams, or nParams per site), or fn(x, y, z) ->
+        the site's nParams values (PhaseField.cpp:769-797)."""
+        if callable(s):
+            self.surfaceTension = [float(v) for site in self._sites() for v in s(*site)]
+        else:
+            self.surfaceTension = [float(s)] * self.nFluid if np.isscalar(s) else [float(v) for v in s]
+        return self
+
+    def _sites(self):
+        if self.gridSize is None:
+            raise ValueError("PhaseField: set the grid size first.")
+        nx, ny, nz = self.gridSize
+        return [(x, y, z) for x in range(nx) for y in range(ny) for z in range(nz)]
+
+    def setPressure(self, p):
+        self.pressure = [float(v) for v in p]
+        return self
+
+    def setVolume(self, volume, volConst=0.01):
+        self.volumeFixed = True
+        self.volConst = float(volConst)
+        self.volume = [float(v) for v in volume]
+        return self
+
+    def setVolumeFixed(self, volumeFixed, volConst=0.01):
+        self.volumeFixed = bool(volumeFixed)
+        self.volConst = float(volConst)
+        return self
+
+    def setSolid(self, solid):
+        """Flags per site, or fn(x, y, z) -> bool (PhaseField.cpp:836-852)."""
+        if callable(solid):
+            self.solid = [1 if solid(*site) else 0 for site in self._sites()]
+        else:
+            self.solid = [1 if v else 0 for v in np.asarray(solid).reshape(-1)]
+        return self
+
+    def setContactAngle(self, a):
+        """A value for every site, a vector, or fn(x, y, z) (PhaseField.cpp:854-873)."""
+        if callable(a):
+            self.contactAngle = [float(a(*site)) for site in self._sites()]
+        elif np.isscalar(a):
+            self.contactAngle = [float(a)] * int(np.prod(self.gridSize))
+        else:
+            self.contactAngle = [float(v) for v in a]
+        return self
+
+    def setForce(self, force, iFluid=()):
+        if len(force) != 3:
+            raise ValueError("PhaseField: Invalid size of force array.")
+        self.force = [float(v) for v in force]
+        self.forceFluids = [int(i) for i in iFluid]
+        return self
+
+    def setConfinement(self, strength):
+        self.confinementStrength = [float(v) for v in strength]
         return self
 
     def setDensityConstraint(self, method):
@@ -144,9 +211,71 @@ class PhaseField(Potential):
             p.fix_fluid = self._fx
         p.stencil_xchunk = self.stencilXChunk
         p.stencil_target_blocks = self.stencilTargetBlocks
+        p.stencil_kernel = self.stencilKernel
+        if len(self.solid):
+            self._solid = (C.c_ubyte * len(self.solid))(*self.solid)
+            p.solid = self._solid
+        if len(self.contactAngle):
+            self._ca = _f64(self.contactAngle)
+            p.contact_angle = _dptr(self._ca)
+            p.n_contact_angle = self._ca.size
+        if len(self.pressure):
+            if len(self.pressure) != self.nFluid:
+                raise ValueError("PhaseField: Invalid size of pressure array.")
+            self._pres = _f64(self.pressure)
+            p.pressure = _dptr(self._pres)
+        if self.force is not None:
+            self._force = _f64(self.force)
+            p.force = _dptr(self._force)
+            self._ff = np.ascontiguousarray(self.forceFluids or [0], dtype=np.int32)
+            p.force_fluids = self._ff.ctypes.data_as(A.ip)
+            p.n_force_fluids = len(self.forceFluids)
+        if len(self.confinementStrength):
+            if len(self.confinementStrength) != self.nFluid:
+                raise ValueError("PhaseField: Invalid size of confinement strength array.")
+            self._conf = _f64(self.confinementStrength)
+            p.confinement = _dptr(self._conf)
+        p.volume_fixed = 1 if self.volumeFixed else 0
+        p.vol_const = self.volConst
+        if len(self.volume):
+            if len(self.volume) != self.nFluid:
+                raise ValueError("PhaseField: Invalid size of volume array.")
+            self._vol = _f64(self.volume)
+            p.volume = _dptr(self._vol)
         h = C.c_void_p()
         A.check(A.load().minim_pf_create(C.byref(p), C.byref(h)))
         return _Handle(h.value, A.load().minim_potential_destroy)
+
+    COMPONENTS = ("fluid", "surface", "pressure", "density constraint", "force", "confinement", "volume constraint")
+
+    def energyComponents(self, coords, init_coords=None):
+        """PhaseField::energyComponents (PhaseField.cpp:706-744): {name: [energy,
+        |gradient|]} at coords, for the potential initialised at init_coords
+        (default: coords)."""
+        x = _f64(coords)
+        h = self.handle(x if init_coords is None else init_coords)
+        out = np.zeros(14)
+        A.check(A.load().minim_pf_energy_components(h.ptr, _dptr(x), x.size, _dptr(out)))
+        return {n: [out[2 * i], out[2 * i + 1]] for i, n in enumerate(self.COMPONENTS)}
+
+    def nodeVolumes(self, coords=None):
+        """(nodeVol, surfaceArea) per site after init (PhaseField.cpp:190-214)."""
+        ng = int(np.prod(self.gridSize))
+        h = self.handle(np.zeros(ng * self.nFluid) if coords is None else coords)
+        nv, sa = np.zeros(ng), np.zeros(ng)
+        A.check(A.load().minim_pf_node_volumes(h.ptr, _dptr(nv), _dptr(sa)))
+        return nv, sa
+
+    def diffuseSolid(self, solid, iFluid=0, twoStep=False):
+        """PhaseField::diffuseSolid (PhaseField.cpp:998-1035) on a copy of this
+        potential."""
+        s = np.ascontiguousarray(np.asarray(solid).reshape(-1), dtype=np.uint8)
+        out = np.zeros(int(np.prod(self.gridSize)) * self.nFluid)
+        h = self._create()
+        A.check(A.load().minim_pf_diffuse_solid(h.ptr, s.ctypes.data_as(C.POINTER(C.c_ubyte)), int(iFluid),
+                                                1 if twoStep else 0, _dptr(out)))
+        return out
+
 
     def coefficients(self, coords=None):
         """(kappa, kappaP, convergence) as PhaseField::init computes them."""
@@ -158,6 +287,16 @@ class PhaseField(Potential):
         conv = C.c_double()
         A.check(A.load().minim_potential_convergence(h.ptr, C.byref(conv)))
         return k, kp, conv.value
+
+
+def site_types(grid, solid):
+    """PhaseField::getType of every site (host): -1 solid, 0 bulk, 1..7."""
+    s = np.ascontiguousarray(np.asarray(solid).reshape(-1), dtype=np.uint8)
+    g = (C.c_int * 3)(*[int(v) for v in grid])
+    t = np.zeros(s.size, np.int32)
+    A.check(A.load().minim_pf_site_types(g, s.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                         t.ctypes.data_as(C.POINTER(C.c_int))))
+    return t
 
 
 class LjNd(Potential):

@@ -70,7 +70,7 @@ void check_prec(int prec) {
 extern "C" {
 
 const char* minim_last_error(void) { return g_err.c_str(); }
-int minim_abi_version(void) { return 2; }
+int minim_abi_version(void) { return 3; }
 
 // ---------------------------------------------------------------- potentials
 int minim_pf_create(const minim_pf_params* p, minim_potential** out) {
@@ -102,6 +102,22 @@ int minim_pf_create(const minim_pf_params* p, minim_potential** out) {
     }
     pf->stencilXChunk = p->stencil_xchunk;
     if (p->stencil_target_blocks > 0) pf->stencilTargetBlocks = p->stencil_target_blocks;
+    pf->stencilKernel = p->stencil_kernel;
+    const long ng = (long)p->grid[0] * p->grid[1] * p->grid[2];
+    if (p->solid) pf->setSolid(std::vector<char>(p->solid, p->solid + ng));
+    if (p->contact_angle) pf->setContactAngle(std::vector<double>(p->contact_angle, p->contact_angle + p->n_contact_angle));
+    if (p->pressure) pf->setPressure(std::vector<double>(p->pressure, p->pressure + p->n_fluid));
+    if (p->force) {
+      std::vector<int> fl;
+      if (p->n_force_fluids > 0) {
+        need(p->force_fluids, "force_fluids");
+        fl.assign(p->force_fluids, p->force_fluids + p->n_force_fluids);
+      }
+      pf->setForce(std::vector<double>(p->force, p->force + 3), fl);
+    }
+    if (p->confinement) pf->setConfinement(std::vector<double>(p->confinement, p->confinement + p->n_fluid));
+    if (p->volume) pf->setVolume(std::vector<double>(p->volume, p->volume + p->n_fluid), p->vol_const);
+    else if (p->volume_fixed) pf->setVolumeFixed(true, p->vol_const);
     auto* h = new minim_potential;
     h->p = std::move(pf);
     *out = h;
@@ -118,6 +134,70 @@ int minim_pf_coefficients(const minim_potential* pot, double* kappa, double* kap
       if (kappa) kappa[k] = pf->kappa[k];
       if (kappa_p) kappa_p[k] = pf->kappaP[k];
     }
+  });
+}
+
+namespace {
+const minim::PhaseField& pf_of(const minim_potential* pot, bool initialised) {
+  need(pot, "pot");
+  auto* pf = dynamic_cast<const minim::PhaseField*>(pot->p.get());
+  if (!pf) throw std::invalid_argument("not a PhaseField potential");
+  if (initialised && !pot->initialised) throw std::runtime_error("PhaseField: call minim_potential_init first");
+  return *pf;
+}
+}  // namespace
+
+int minim_pf_energy_components(const minim_potential* pot, const double* coords, size_t ndof, double* out) {
+  return guard([&] {
+    const minim::PhaseField& pf = pf_of(pot, true);
+    need(coords, "coords");
+    need(out, "out");
+    auto m = pf.energyComponents(std::vector<double>(coords, coords + ndof));
+    static const char* names[7] = {"fluid", "surface", "pressure", "density constraint", "force", "confinement",
+                                   "volume constraint"};
+    for (int c = 0; c < 7; c++) {
+      out[2 * c] = m.at(names[c])[0];
+      out[2 * c + 1] = m.at(names[c])[1];
+    }
+  });
+}
+
+int minim_pf_node_volumes(const minim_potential* pot, double* node_vol, double* surface_area) {
+  return guard([&] {
+    const minim::PhaseField& pf = pf_of(pot, true);
+    const size_t ng = (size_t)pf.nGrid;
+    for (size_t i = 0; i < ng; i++) {
+      if (node_vol) node_vol[i] = pf.nodeVol.empty() ? std::pow(pf.resolution, 3) : pf.nodeVol[i];
+      if (surface_area) surface_area[i] = pf.surfaceArea.empty() ? 0.0 : pf.surfaceArea[i];
+    }
+  });
+}
+
+int minim_pf_diffuse_solid(const minim_potential* pot, const unsigned char* solid, int i_fluid, int two_step,
+                           double* out) {
+  return guard([&] {
+    const minim::PhaseField& pf = pf_of(pot, false);
+    need(solid, "solid");
+    need(out, "out");
+    if (pf.gridSize.size() != 3) throw std::invalid_argument("PhaseField: gridSize must have three entries.");
+    const size_t ng = (size_t)pf.gridSize[0] * pf.gridSize[1] * pf.gridSize[2];
+    std::vector<double> r =
+        minim::PhaseField::diffuseSolid(std::vector<char>(solid, solid + ng), pf, i_fluid, two_step != 0);
+    std::memcpy(out, r.data(), sizeof(double) * r.size());
+  });
+}
+
+int minim_pf_site_types(const int grid[3], const unsigned char* solid, int* type) {
+  return guard([&] {
+    need(grid, "grid");
+    need(solid, "solid");
+    need(type, "type");
+    for (int d = 0; d < 3; d++)
+      if (grid[d] < 1) throw std::invalid_argument("PhaseField: grid sizes must be positive.");
+    const size_t ng = (size_t)grid[0] * grid[1] * grid[2];
+    std::vector<char> s(solid, solid + ng);
+    std::vector<int> t = minim::pf_site_types(grid, s.data());
+    std::memcpy(type, t.data(), sizeof(int) * ng);
   });
 }
 

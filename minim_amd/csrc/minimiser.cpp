@@ -418,6 +418,10 @@ void Lbfgs::iteration(State& state) {
     }
   }
   minim_amd::launch_direction(a, prec, ws.rs(), s);
+  // Constraints that couple all DOFs (fixed volumes) act on the step too
+  // (Lbfgs.cpp:51); d lies in their subspace up to rounding, so the direction
+  // pass's dots stay valid.  The projection leaves result[0, 80) untouched.
+  if (state.pot->hasGlobalConstraints()) state.pot->applyConstraints(prec, a.out, n, s);
   // Without a maxStep cap the line search starts at x + d (tau = 1) unless d
   // must be flipped (rare).  Launch that trial energy now, after stashing the
   // direction pass's dots, so that one host synchronisation returns both.

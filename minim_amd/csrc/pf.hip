@@ -621,7 +621,11 @@ void pf_args_finalize(PFArgs& a, int target_blocks) {
 void pf_launch_eval(const PFArgs& a, int prec, int pv, int mode, const void* x, const void* d, double t,
                     void* g, RedScratch rs, hipStream_t s, bool finalize) {
   int nb;
-  if (prec == F64 && pv == F64)
+  if (a.use_ext) {
+    PFCall c{};
+    c.terms = kTermsEnergy;
+    nb = pf_launch_ext(a, c, prec, pv, mode, x, d, t, g, rs, s);
+  } else if (prec == F64 && pv == F64)
     nb = launch_eval_t<double, double>(a, mode, x, d, t, g, rs, s);
   else if (prec == F32 && pv == F32)
     nb = launch_eval_t<float, float>(a, mode, x, d, t, g, rs, s);
@@ -630,15 +634,20 @@ void pf_launch_eval(const PFArgs& a, int prec, int pv, int mode, const void* x, 
   else
     throw std::invalid_argument("PhaseField: FP32 coordinates with FP64 vectors are not supported");
   MINIM_HIP_CHECK(hipGetLastError());
-  if (finalize) launch_finalize(rs, 2, nb, s);
+  if (finalize) {
+    launch_finalize(rs, 2, nb, s);
+    // fixed volumes: the global part of the gradient projection; g.g anew
+    if ((mode & 2) && a.volfix) pf_launch_volume_projection(a, pv, g, (size_t)a.X * a.YZ, true, rs, s);
+  }
 }
 
-void pf_launch_constrain(const PFArgs& a, int prec, void* v, size_t nsite, hipStream_t s) {
+void pf_launch_constrain(const PFArgs& a, int prec, void* v, size_t nsite, RedScratch rs, hipStream_t s) {
   if (prec == F64)
     launch_constrain_t<double>(a, v, nsite, s);
   else
     launch_constrain_t<float>(a, v, nsite, s);
   MINIM_HIP_CHECK(hipGetLastError());
+  if (a.volfix) pf_launch_volume_projection(a, prec, v, nsite, false, rs, s);
 }
 
 }  // namespace minim_amd

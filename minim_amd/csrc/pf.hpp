@@ -8,6 +8,48 @@ namespace minim_amd {
 constexpr int kPFMaxFluid = 16;
 enum { kDensityHard = 0, kDensitySoft = 1, kDensityNone = 2, kDensityFixed = 3 };
 
+// Site kinds of the general kernel: 0 = solid, 1 + the reference's surface
+// type 0..7 (PhaseField::getType, PhaseField.cpp:899-958).
+constexpr int kPFKinds = 9;
+
+// Per-site code (uint16): bits 0-3 the kind; for canonical dimension j, bits
+// 4+4j..5+4j and 6+4j..7+4j weight the bond to the minus and the plus
+// neighbour: 0, 1 (one half) or 2 (one).  A bond to a solid site weighs 0;
+// the opposite bond of a site whose other neighbour is solid weighs 1
+// (phaseGradient's one-sided branches, PhaseField.cpp:265-286).
+constexpr int kPFCodeShift = 4;
+
+// Constants of the general kernel, resident in device memory.
+struct PFExt {
+  double B[kPFMaxFluid][kPFKinds];  // bulk factor (kappa/16 | 0.5 kappa) * node volume
+  double F[kPFMaxFluid][kPFKinds];  // gradient factor (0.25 | 0.5) kappa' * node volume / res^2
+  double vol[kPFKinds];             // node volume
+  double area[kPFKinds];            // solid surface area
+  double pres[kPFMaxFluid];         // pressure; 0 when unset or the fluid is fixed
+  double fmag[kPFMaxFluid];         // force magnitude
+  double fn[kPFMaxFluid][3];        // force direction in canonical dimensions
+  double fc[3];                     // grid centre (N - 1) / 2 in canonical dimensions
+  double res;
+  double conf[kPFMaxFluid];         // confinement strength * mean tension * res^2
+};
+
+// Term selection of one launch of the general kernel.
+enum {
+  kTermFluid = 1,
+  kTermSurface = 2,
+  kTermPressure = 4,
+  kTermDensity = 8,
+  kTermForce = 16,
+  kTermConfinement = 32,
+  kTermVolume = 64,
+  kTermsEnergy = 63  // everything PhaseField::energyGradient adds
+};
+struct PFCall {
+  unsigned terms;
+  int raw;                   // 1: no constraints applied to the gradient
+  double volg[kPFMaxFluid];  // volume-constraint factor volCoef * volDiff per fluid
+};
+
 // Everything a stencil launch needs, passed by value as a kernel argument.
 struct PFArgs {
   // canonical grid: size-1 dimensions first, Z contiguous
@@ -32,6 +74,19 @@ struct PFArgs {
   int xchunk, nxc, nblocks;
   int xchunk_override;     // >0 forces the x-march length (autotuning)
   int target_blocks;       // >0 overrides the launch-size heuristic
+  // general kernel: used when any solid node or extra term is set (or forced)
+  int use_ext;
+  const PFExt* ext;        // device
+  const uint16_t* code;    // per-site code, or NULL: ucode at every site
+  unsigned ucode;
+  const double* wet;       // per-site wetting parameter (binary fluids), or NULL
+  const int8_t* csign;     // per-DOF sign of (initial c - 0.5), or NULL (no confinement)
+  const double* Bs;        // per-DOF bulk factors with per-site tensions, or NULL
+  const double* Fs;        // per-DOF gradient factors with per-site tensions, or NULL
+  int any_pres, any_force;
+  // fixed-volume projection after the per-site one
+  int volfix;
+  double tv2;              // sum of squared node volumes
 };
 
 // Fills YZ, act*, the derived constants and the launch shape of the generic
@@ -41,8 +96,27 @@ void pf_args_finalize(PFArgs& a, int target_blocks);
 // mode bits: 1 energy, 2 gradient (plus g.g), 4 evaluate at x + t*d.
 // prec: coordinates (and arithmetic); pv: the d and g vectors (prec, or FP32
 // with FP64 coordinates).  Results: rs.result[0] = energy, [1] = g.g (device).
+// Runs the general kernel when a.use_ext; with fixed volumes and a gradient
+// the volume projection follows (not when finalize is false: timing only).
 void pf_launch_eval(const PFArgs& a, int prec, int pv, int mode, const void* x, const void* d, double t,
                     void* g, RedScratch rs, hipStream_t s, bool finalize = true);
-void pf_launch_constrain(const PFArgs& a, int prec, void* v, size_t nsite, hipStream_t s);
+// Potential::applyConstraints: fixed fluids, hard density projection, then
+// (fixed volumes) the global projection, which uses rs as
+// pf_launch_volume_projection does.
+void pf_launch_constrain(const PFArgs& a, int prec, void* v, size_t nsite, RedScratch rs, hipStream_t s);
+
+// General kernel (pf_ext.hip): the terms selected by c, gradient projected
+// per site unless c.raw.  Results as pf_launch_eval; returns the block count.
+int pf_launch_ext(const PFArgs& a, const PFCall& c, int prec, int pv, int mode, const void* x, const void* d,
+                  double t, void* g, RedScratch rs, hipStream_t s);
+// Fixed-volume projection of v (specialisedConstraints, PhaseField.cpp:614-658),
+// after the per-site one.  Uses rs.partials and rs.result[80, 96); with norm,
+// rs.result[1] = v.v afterwards.
+constexpr int kPFVolSlot = 80;
+void pf_launch_volume_projection(const PFArgs& a, int prec, void* v, size_t nsite, bool norm, RedScratch rs,
+                                 hipStream_t s);
+// Fluid volumes (volumeConstraintEnergy, PhaseField.cpp:540-553):
+// rs.result[k] = sum_i c_ik v_i (binary fluids: (c+1)/2 v).
+void pf_launch_volumes(const PFArgs& a, int prec, const void* x, size_t nsite, RedScratch rs, hipStream_t s);
 
 }  // namespace minim_amd

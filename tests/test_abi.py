@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(L, n), n
     assert set(names) == set(_capi.SIGNATURES), "ctypes table and header disagree"
-    assert L.minim_abi_version() == 2
+    assert L.minim_abi_version() == 3
 
 
 def _pf_handle(grid, nf, st, iw, density=0, res=1.0, fix=None):
@@ -129,3 +129,30 @@ def test_triangulation_lists_match_reference_order():
     bonds, hinges = triangulation_lists(c["tri"])
     np.testing.assert_array_equal(bonds, c["bars"])
     np.testing.assert_array_equal(hinges, c["hinges"])
+
+
+def test_site_types_match_oracle():
+    """The library's host classification of solid layouts (PhaseField::getType)
+    equals the oracle's, site for site, and rejects the same layouts."""
+    import numpy as np
+    from minim_amd import site_types
+    from oracle_bind import oracle_site_types
+    from test_pf_ext_oracle import structured_solid
+
+    rng = np.random.default_rng(4)
+    for grid in [(6, 7, 5), (8, 6, 1), (1, 9, 7), (5, 2, 4), (4, 4, 4), (3, 1, 1)]:
+        for kind in ["wall", "channel", "pillar", "box"]:
+            s = structured_solid(grid, rng, kind)
+            assert np.array_equal(site_types(grid, s), oracle_site_types(grid, s))
+    # random sparse solids: both accept or both reject
+    for seed in range(60):
+        r = np.random.default_rng(seed)
+        grid = (5, 6, 4)
+        s = (r.random(120) < 0.03).astype(np.uint8)
+        try:
+            want = oracle_site_types(grid, s)
+        except RuntimeError:
+            with pytest.raises(Exception, match="Undefined surface type"):
+                site_types(grid, s)
+            continue
+        assert np.array_equal(site_types(grid, s), want)
