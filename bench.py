@@ -135,6 +135,79 @@ def time_device_evals(m, h, prec, x, g, n, steps, warmup):
     return time_evals(m, h, prec, x, g, e_dev, n, steps, warmup, stream)
 
 
+def pillar_layout(n):
+    """A pillar array on a base (the pf-pillar-collapse geometry of the reference's
+    benchmarks, test/benchmark/pf-pillar-collapse/main.cpp:42-60, on an n^3
+    grid): solid for y < n/16 and, for y < n/2, in square pillars of side n/4
+    on a period of n/2 in x and z.  Three fluids: the diffuse solid (fixed), a
+    liquid above y = 3n/4 and gas elsewhere, with a pressure on the liquid."""
+    import numpy as np
+
+    x, y, z = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    pillar = ((x % (n // 2)) < n // 4) & ((z % (n // 2)) < n // 4) & (y < n // 2)
+    solid = ((y < n // 16) | pillar).reshape(-1).astype(np.uint8)
+    liquid = (y > 3 * n // 4).reshape(-1)
+    rng = np.random.default_rng(11)
+    c = np.zeros((n ** 3, 3))
+    c[:, 0] = solid
+    c[:, 1] = np.where(liquid, 1.0, 0.0) * (1 - solid)
+    c[:, 2] = np.where(liquid, 0.0, 1.0) * (1 - solid)
+    c += rng.uniform(-0.02, 0.02, c.shape) * (1 - solid)[:, None]
+    return solid, c.reshape(-1)
+
+
+def pf_general_leg(m, n=256, steps=20, warmup=3, ref_n=48):
+    """The general PhaseField kernel (solid nodes, pressure, a fixed fluid; the
+    tiled kernel covers neither) at 256^3 x 3 fluids: kernel and whole-evaluation
+    times in FP32 and FP64 against the HBM roofline, and the reference library's
+    evaluation of the same layout at ref_n^3 on the host cores."""
+    import numpy as np
+    import torch
+
+    solid, x0 = pillar_layout(n)
+    pf = (m.PhaseField().setNFluid(3).setGridSize([n, n, n]).setSurfaceTension(SIGMA).setInterfaceSize(WIDTH)
+          .setSolid(solid).setFixFluid(0).setPressure([0.0, 0.05, 0.0]))
+    nd = x0.size
+    t0 = time.perf_counter()
+    h = pf.handle(x0)
+    setup_s = time.perf_counter() - t0
+    stream = torch.cuda.Stream()
+    out = dict(workload="PhaseField %d^3 x 3 fluids, pillar array on a base (%.1f%% solid), fixed solid fluid, "
+                        "pressure on the liquid, hard density constraint" % (n, 100.0 * solid.mean()),
+               setup_s=setup_s)
+    sites = n ** 3
+    for prec, b in (("fp32", 4), ("fp64", 8)):
+        dt = torch.float32 if prec == "fp32" else torch.float64
+        x = torch.from_numpy(x0).to("cuda", dt)
+        g = torch.empty_like(x)
+        ms = time_device_evals(m, h, prec, x, g, nd, steps, warmup)
+        msk = m.potential_time_kernel(h, prec, x.data_ptr(), nd, g_ptr=g.data_ptr(), reps=steps,
+                                      stream=stream.cuda_stream)
+        alg = sites * 3 * (b + b) + sites * 2  # coordinates in, gradient out, 2-byte site code
+        out[prec] = dict(eval_ms=ms, evals_per_s=1e3 / ms, kernel_ms=msk, algorithmic_bytes=alg,
+                         achieved_GBs=alg / (msk * 1e-3) / 1e9, frac_of_hbm_peak=alg / (msk * 1e-3) / 1e9 / HBM_PEAK_GBS)
+        del x, g
+        torch.cuda.empty_cache()
+    from oracle_bind import RefState, ref_available
+
+    if ref_available():
+        threads = int(os.environ.get("OMP_NUM_THREADS") or min(os.cpu_count() or 1, 16))
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        rs, rx = pillar_layout(ref_n)
+        r = RefState.phasefield_ext((ref_n,) * 3, rx, 3, 1.0, SIGMA, [WIDTH], "hard", solid=rs,
+                                    fix_fluid=[True, False, False], pressure=[0.0, 0.05, 0.0])
+        r.energy_gradient(rx)
+        t = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            r.energy_gradient(rx)
+        dt_ref = (time.perf_counter() - t) / reps
+        out["reference_cpu"] = dict(grid=[ref_n] * 3, eval_s=dt_ref, sites_per_s=ref_n ** 3 / dt_ref, cores=threads,
+                                    kind="reference")
+        out["gpu_sites_per_s_fp64"] = sites / (out["fp64"]["eval_ms"] * 1e-3)
+    return out
+
+
 def lj_leg(m, n_side=64, density=0.85, fire_max_iter=20000, ref_side=10):
     """Lennard-Jones at the benchmark size: n_side^3 particles at number density
     0.85 in a periodic cube, truncated-shifted at rc = 2.5 sigma with a 0.3 sigma
@@ -432,7 +505,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--fire-iters", type=int, default=20000, help="particles mode: FIRE iteration cap")
     ap.add_argument("--lbfgs-iters", type=int, default=2000, help="particles mode: sheet L-BFGS iteration cap")
-    ap.add_argument("--mode", choices=["full", "stencil", "converge", "particles", "batch"], default="full",
+    ap.add_argument("--mode", choices=["full", "stencil", "converge", "particles", "batch", "general"],
+                    default="full",
                     help="stencil: only the evaluation timings plus a 256 MiB copy (the calibration "
                          "kernel for rocprofv3 byte counters); converge: only time-to-converge")
     args = ap.parse_args()
@@ -464,6 +538,14 @@ def main():
         del x64, x32, g64, g32
         torch.cuda.empty_cache()
         res = batch_leg(m, cpu_budget=0 if args.no_cpu else 10.0)
+        log(json.dumps(res, indent=1))
+        print(json.dumps(res), flush=True)
+        return
+
+    if args.mode == "general":
+        del x64, x32, g64, g32
+        torch.cuda.empty_cache()
+        res = pf_general_leg(m, steps=args.steps, warmup=args.warmup)
         log(json.dumps(res, indent=1))
         print(json.dumps(res), flush=True)
         return
@@ -558,7 +640,8 @@ def main():
     # the other workloads of the path, each beside the reference library where it can run them;
     # a failure here is recorded in the output and does not hide the headline line
     others = {}
-    for name, fn in (("lj_periodic_262144", lambda: lj_leg(m, fire_max_iter=args.fire_iters)),
+    for name, fn in (("pf_general_256cubed_pillars", lambda: pf_general_leg(m, steps=args.steps, warmup=args.warmup)),
+                     ("lj_periodic_262144", lambda: lj_leg(m, fire_max_iter=args.fire_iters)),
                      ("bar_and_hinge_1024x1024", lambda: bah_leg(m, max_iter=args.lbfgs_iters)),
                      ("batch_512_instances", lambda: batch_leg(m, cpu_budget=0 if args.no_cpu else 10.0))):
         log("%s ..." % name)

@@ -53,21 +53,46 @@ struct Site {
   unsigned q;
 };
 
-template <typename T, typename TV, int M, bool PS, bool DIR>
+// The nf values of one site as one vector access (dwordx3 for three FP32
+// fluids) when the fluid count is a compile-time constant.
+template <typename T, int N>
+struct SiteVec {
+  T v[N];
+};
+
+template <typename T, typename TV, int NF, int M, bool PS, bool DIR>
 __device__ __forceinline__ void load_site(const PFArgs& a, const T* __restrict__ x, const TV* __restrict__ d, T t,
                                           size_t site, int nf, Site<T, M, PS>& s) {
   const size_t base = site * (size_t)nf;
+  if constexpr (NF > 0) {
+    const SiteVec<T, NF> xv = *reinterpret_cast<const SiteVec<T, NF>*>(x + base);
 #pragma unroll
-  for (int k = 0; k < M; k++) {
-    if (k < nf) {
-      T c = x[base + k];
-      if constexpr (DIR) {
-        const T st = t * (T)d[base + k];
-        c = c + st;
+    for (int k = 0; k < NF; k++) s.c[k] = xv.v[k];
+    if constexpr (DIR) {
+      const SiteVec<TV, NF> dv = *reinterpret_cast<const SiteVec<TV, NF>*>(d + base);
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        const T st = t * (T)dv.v[k];
+        s.c[k] = s.c[k] + st;
       }
-      s.c[k] = c;
-      if constexpr (PS) s.F[k] = (T)a.Fs[base + k];
     }
+  } else {
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+      if (k < nf) {
+        T c = x[base + k];
+        if constexpr (DIR) {
+          const T st = t * (T)d[base + k];
+          c = c + st;
+        }
+        s.c[k] = c;
+      }
+    }
+  }
+  if constexpr (PS) {
+#pragma unroll
+    for (int k = 0; k < M; k++)
+      if (k < nf) s.F[k] = (T)a.Fs[base + k];
   }
   s.q = a.code ? (unsigned)a.code[site] : a.ucode;
 }
@@ -112,17 +137,17 @@ __global__ __launch_bounds__(kBlock) void pf_ext_kernel(PFArgs a, PFCall cl, con
     const PFExt* E = a.ext;
     const T res = (T)E->res;
     Site<T, M, PS> sm, sc, sp;
-    load_site<T, TV, M, PS, DIR>(a, x, d, t, (size_t)wrapx(x0 - 1) * YZ + p, nf, sm);
-    load_site<T, TV, M, PS, DIR>(a, x, d, t, (size_t)x0 * YZ + p, nf, sc);
+    load_site<T, TV, NF, M, PS, DIR>(a, x, d, t, (size_t)wrapx(x0 - 1) * YZ + p, nf, sm);
+    load_site<T, TV, NF, M, PS, DIR>(a, x, d, t, (size_t)x0 * YZ + p, nf, sc);
     for (int xi = x0; xi < x1; xi++) {
       const size_t pl = (size_t)xi * YZ;
       const size_t site = pl + p;
-      load_site<T, TV, M, PS, DIR>(a, x, d, t, (size_t)wrapx(xi + 1) * YZ + p, nf, sp);
+      load_site<T, TV, NF, M, PS, DIR>(a, x, d, t, (size_t)wrapx(xi + 1) * YZ + p, nf, sp);
       Site<T, M, PS> nb[4];  // y-, y+, z-, z+
-      load_site<T, TV, M, PS, DIR>(a, x, d, t, pl + pym, nf, nb[0]);
-      load_site<T, TV, M, PS, DIR>(a, x, d, t, pl + pyp, nf, nb[1]);
-      load_site<T, TV, M, PS, DIR>(a, x, d, t, pl + pzm, nf, nb[2]);
-      load_site<T, TV, M, PS, DIR>(a, x, d, t, pl + pzp, nf, nb[3]);
+      load_site<T, TV, NF, M, PS, DIR>(a, x, d, t, pl + pym, nf, nb[0]);
+      load_site<T, TV, NF, M, PS, DIR>(a, x, d, t, pl + pyp, nf, nb[1]);
+      load_site<T, TV, NF, M, PS, DIR>(a, x, d, t, pl + pzm, nf, nb[2]);
+      load_site<T, TV, NF, M, PS, DIR>(a, x, d, t, pl + pzp, nf, nb[3]);
       const Site<T, M, PS>* mside[3] = {&sm, &nb[0], &nb[2]};
       const Site<T, M, PS>* pside[3] = {&sp, &nb[1], &nb[3]};
       const int kind = site_kind(sc.q);
@@ -280,12 +305,28 @@ __global__ __launch_bounds__(kBlock) void pf_ext_kernel(PFArgs a, PFCall cl, con
         }
         const size_t base = site * (size_t)nf;
         T gacc = (T)0;
+        if constexpr (NF > 0) {
+          SiteVec<TV, NF> o;
 #pragma unroll
-        for (int k = 0; k < M; k++) {
-          if (k >= nf) continue;
-          const TV gs = (TV)gv[k];
-          g[base + k] = gs;
-          gacc = fmad((T)gs, (T)gs, gacc);
+          for (int k = 0; k < NF; k++) {
+            o.v[k] = (TV)gv[k];
+            gacc = fmad((T)o.v[k], (T)o.v[k], gacc);
+          }
+          if constexpr (sizeof(TV) == 4) {
+            // streaming: the gradient is not re-read by this kernel
+#pragma unroll
+            for (int k = 0; k < NF; k++) __builtin_nontemporal_store(o.v[k], g + base + k);
+          } else {
+            *reinterpret_cast<SiteVec<TV, NF>*>(g + base) = o;
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < M; k++) {
+            if (k >= nf) continue;
+            const TV gs = (TV)gv[k];
+            g[base + k] = gs;
+            gacc = fmad((T)gs, (T)gs, gacc);
+          }
         }
         acc[1] += (double)gacc;
       }

@@ -6,7 +6,7 @@ Run single-threaded (OMP_NUM_THREADS=1), the reference's only bit-deterministic
 mode.  Inputs are drawn from numpy's default_rng with the seeds below and are
 stored in the fixtures, so nothing downstream depends on the generator.
 
-    OMP_NUM_THREADS=1 python tests/golden/make_golden.py
+    OMP_NUM_THREADS=1 python tests/golden/make_golden.py [fixture ...]
 """
 import json
 import math
@@ -179,13 +179,91 @@ def cfg1_cases():
                  **res)]
 
 
+def pf_ext_cases():
+    """PhaseField with solid nodes, wetting, pressure, forces, confinement,
+    per-site tensions and fixed volumes, set through the reference's setters.
+    Stored per case: the options, the initial state x0 (which fixes the volume
+    target and the confinement), an evaluation point x, the energy from an
+    energy-only call and from an energy+gradient call (they differ when several
+    fluids feel the force, PhaseField.cpp:511), the constrained gradient, a
+    random vector and its projection, and energyComponents at x."""
+    rng = np.random.default_rng(31337)
+
+    def solid_of(grid, kind):
+        s = np.zeros(grid, np.uint8)
+        if kind == "wall-x":
+            s[0] = 1
+        elif kind == "channel-z":
+            s[:, :, 0] = 1
+            s[:, :, -1] = 1
+        elif kind == "pillar":
+            s[:, 0] = 1
+            s[:2, :4] = 1
+        elif kind == "box":
+            s[1:3, 1:3, 1:3] = 1
+        return s.reshape(-1)
+
+    specs = [
+        # grid, nfluid, resolution, surface tension (None: per site), interface size, density, options
+        ((8, 7, 1), 1, 0.5, [1.0], [1.0], "hard",
+         dict(solid="wall-x", contact_angle=True, pressure=[1.5], force=[0.3, -0.2, 0.0])),
+        ((6, 5, 4), 2, 1.0, [0.9], [1.2], "hard",
+         dict(solid="channel-z", pressure=[0.5, -0.3], confinement=[2.0, 0.0], volume_fixed=True, vol_const=0.02)),
+        ((5, 6, 4), 3, 1.0, [1.0, 0.8, 1.2], [2.0], "hard",
+         dict(solid="pillar", fix_fluid=[True, False, False], pressure=[0.0, 0.2, 0.0])),
+        ((4, 4, 4), 3, 1.0, None, [1.5], "soft",
+         dict(solid="box", force=[0.0, 0.0, -0.7], force_fluids=[1])),
+        ((6, 1, 5), 4, 2.0, [1.0, 0.9, 1.1, 0.8], [1.3], "none",
+         dict(solid="wall-x", confinement=[0.0, 1.5, 0.0, 0.5], volume=[3.0, 2.0, 1.5, 0.5], vol_const=0.05)),
+        ((1, 9, 7), 1, 1.0, None, [1.0], "hard",
+         dict(contact_angle=True, force=[0.0, 0.4, 0.1])),
+    ]
+    out = []
+    for grid, nf, res, st, iw, dens, spec in specs:
+        ng = int(np.prod(grid))
+        opts = {}
+        if "solid" in spec:
+            opts["solid"] = solid_of(grid, spec["solid"])
+        if spec.get("contact_angle"):
+            opts["contact_angle"] = rng.uniform(30, 150, ng * nf)
+        for k in ("pressure", "force", "force_fluids", "confinement", "volume", "vol_const", "volume_fixed",
+                  "fix_fluid"):
+            if k in spec:
+                opts[k] = spec[k]
+        if st is None:
+            st = list(rng.uniform(0.6, 1.4, ng * nf))
+        x0 = rng.uniform(-1.0, 1.0, ng) if nf == 1 else rng.uniform(0.0, 1.0, ng * nf)
+        x = x0 + rng.normal(0, 0.05, x0.size)
+        r = RefState.phasefield_ext(grid, x0, nf, res, st, iw, dens, **opts)
+        e_only = r.energy(x)
+        e_eg, _ = r.energy_gradient(x)
+        g = r.gradient(x)
+        v = rng.normal(0, 1, x0.size)
+        pv = r.constrain(v)
+        r2 = RefState.phasefield_ext(grid, x, nf, res, st, iw, dens, **opts)
+        comp = r2.components()  # at x (the volume target and confinement then come from x)
+        if "volume" in opts:
+            # with an explicit volume target the reference's projection divides
+            # by totalVolume2, which it leaves unset (PhaseField.cpp:216-231):
+            # its constrained vectors are not a reference result
+            g, pv = None, None
+        jopts = {k: (np.asarray(v).tolist() if hasattr(v, "__len__") else v) for k, v in opts.items()}
+        out.append(dict(grid=list(grid), nfluid=nf, resolution=res, surface_tension=list(st), interface_size=iw,
+                        density=dens, opts=jopts, x0=x0.tolist(), x=x.tolist(), energy=e_only, energy_eg=e_eg,
+                        gradient=None if g is None else g.tolist(), v=v.tolist(),
+                        v_constrained=None if pv is None else pv.tolist(), components=comp))
+    return out
+
+
 def main():
     if not ref_available():
         sys.exit("oracle/_ref/libminim_ref.so is not built (make -C oracle)")
     if os.environ.get("OMP_NUM_THREADS") != "1":
         sys.exit("run with OMP_NUM_THREADS=1")
-    data = dict(pf=pf_cases(), lbfgs=lbfgs_cases(), fire_gd=fire_gd_cases(), lj=lj_cases(), bah=bah_cases(),
-                cfg1=cfg1_cases())
+    makers = dict(pf=pf_cases, lbfgs=lbfgs_cases, fire_gd=fire_gd_cases, lj=lj_cases, bah=bah_cases, cfg1=cfg1_cases,
+                  pf_ext=pf_ext_cases)
+    only = sys.argv[1:] or list(makers)  # e.g. "make_golden.py pf_ext" writes that fixture alone
+    data = {k: makers[k]() for k in only}
     for k, v in data.items():
         with open(os.path.join(HERE, "ref_%s.json" % k), "w") as f:
             json.dump(v, f)

@@ -194,6 +194,8 @@ class OraclePFX:
         L = lib()
         self.p = OrcPFX()
         self.nfluid = nfluid
+        if volume is not None:
+            volume_fixed = True  # setVolume fixes the volumes (PhaseField.cpp:823-828)
         self._keep = dict(st=np.ascontiguousarray(surface_tension, dtype=np.float64),
                           iw=np.ascontiguousarray(interface_size if len(interface_size) else [0.0], dtype=np.float64),
                           x0=np.ascontiguousarray(x0, dtype=np.float64),

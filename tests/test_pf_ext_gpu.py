@@ -12,6 +12,7 @@ import math
 import numpy as np
 import pytest
 
+from conftest import golden
 from oracle_bind import OraclePFX, oracle_lbfgs
 from test_pf_ext_oracle import random_case, structured_solid
 
@@ -42,7 +43,9 @@ def device_pf(m, c):
         pf.setForce(list(o["force"]), o.get("force_fluids", ()))
     if "confinement" in o:
         pf.setConfinement(list(o["confinement"]))
-    if o.get("volume_fixed"):
+    if o.get("volume") is not None:
+        pf.setVolume(list(o["volume"]), o.get("vol_const", 0.01))
+    elif o.get("volume_fixed"):
         pf.setVolumeFixed(True, o.get("vol_const", 0.01))
     if "fix_fluid" in o:
         for i, f in enumerate(o["fix_fluid"]):
@@ -178,6 +181,46 @@ def test_general_kernel_modes_agree(gpu, prec):
         torch.cuda.synchronize()
         assert em == e1
         assert torch.equal(gm, g1.float())
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("case", range(6))
+def test_golden_vectors(gpu, case, prec):
+    """The library against the reference's outputs (tests/golden/ref_pf_ext.json)."""
+    import torch
+
+    m = gpu
+    c = golden("pf_ext")[case]
+    print("case golden-ext", case, c["grid"], c["nfluid"], sorted(c["opts"]), prec, flush=True)
+    x0, x = np.array(c["x0"]), np.array(c["x"])
+    xs = x.astype(np.float32).astype(np.float64) if prec == "fp32" else x
+    s = m.State(device_pf(m, c), x0, prec)
+    s.coords(xs)
+    tol = TOL[prec]
+    if prec == "fp64":
+        es = max(1.0, sum(abs(v[0]) for v in c["components"].values()))
+        assert abs(s.energy() - c["energy"]) <= tol * es
+        assert abs(s.energyGradient()[0] - c["energy_eg"]) <= tol * es
+        pf = device_pf(m, c)
+        if c["gradient"] is not None:  # absent where the reference's projection reads an unset value
+            assert rel_err(s.gradient(), c["gradient"]) <= tol
+            h = pf.handle(x0)
+            v = torch.tensor(c["v"], dtype=torch.float64, device="cuda")
+            m.potential_constrain(h, "fp64", v.data_ptr(), v.numel())
+            torch.cuda.synchronize()
+            assert rel_err(v.cpu().numpy(), c["v_constrained"]) <= tol
+        comp = pf.energyComponents(x)
+        for k, want in c["components"].items():
+            assert abs(comp[k][0] - want[0]) <= tol * max(1.0, abs(want[0])), k
+            assert abs(comp[k][1] - want[1]) <= tol * max(1.0, abs(want[1])), k
+    else:
+        # FP32 against the oracle at the FP32-rounded point
+        o = OraclePFX(tuple(c["grid"]), x0, nfluid=c["nfluid"], resolution=c["resolution"],
+                      surface_tension=c["surface_tension"], interface_size=c["interface_size"], density=c["density"],
+                      **c["opts"])
+        es = energy_scale(o, xs)
+        assert abs(s.energy() - o.energy(xs)) <= tol * es
+        assert rel_err(s.gradient(), o.gradient(xs)) <= tol
 
 
 # ---------------------------------------------------------------- known answers

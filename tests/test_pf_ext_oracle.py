@@ -15,6 +15,7 @@ import math
 import numpy as np
 import pytest
 
+from conftest import golden
 from oracle_bind import OraclePFX, RefState, oracle_lbfgs, oracle_site_types, ref_available, ref_diffuse_solid
 
 needs_ref = pytest.mark.skipif(not ref_available(), reason="oracle/_ref (reference build) not present")
@@ -405,3 +406,26 @@ def test_diffuse_solid_vs_reference(two_step):
                       density="none", solid=solid, confinement=conf)
         x, _ = oracle_lbfgs(o, x, m=5, max_iter=100, linesearch="none", convergence=1e-8 * 1.0)
     close(x, ref, 1e-9)
+
+
+# ---------------------------------------------------------------- golden vectors
+def fixture_oracle(c, at):
+    return OraclePFX(tuple(c["grid"]), np.array(at), nfluid=c["nfluid"], resolution=c["resolution"],
+                     surface_tension=c["surface_tension"], interface_size=c["interface_size"], density=c["density"],
+                     **c["opts"])
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_extended_terms_vs_golden(case):
+    """The oracle against the reference's outputs stored in tests/golden/ref_pf_ext.json."""
+    c = golden("pf_ext")[case]
+    o = fixture_oracle(c, c["x0"])
+    x = np.array(c["x"])
+    close(o.energy(x), c["energy"])
+    close(o.energy_gradient(x)[0], c["energy_eg"])
+    if c["gradient"] is not None:  # absent where the reference's projection reads an unset value
+        close(o.gradient(x), c["gradient"])
+        close(o.constrain(c["v"]), c["v_constrained"])
+    comp = fixture_oracle(c, c["x"]).components(x)
+    for k, v in c["components"].items():
+        close(comp[k], v, 1e-12)
