@@ -18,6 +18,11 @@ constexpr int kPFKinds = 9;
 // the opposite bond of a site whose other neighbour is solid weighs 1
 // (phaseGradient's one-sided branches, PhaseField.cpp:265-286).
 constexpr int kPFCodeShift = 4;
+// Forward-bond code (uint32) of the tiled general kernel: bits 0-3 the kind;
+// for canonical dimension j, 6 bits at 4+6j hold 16 w(i, i+e_j), the weight
+// (v_i h_i->j + v_j h_j->i) / res^3 of the bond to the plus neighbour (0..32).
+constexpr int kPFWeightShift = 4;
+constexpr int kPFWeightBits = 6;
 
 // Constants of the general kernel, resident in device memory.
 struct PFExt {
@@ -77,8 +82,11 @@ struct PFArgs {
   // general kernel: used when any solid node or extra term is set (or forced)
   int use_ext;
   const PFExt* ext;        // device
+  const PFExt* ext_host;   // the same constants in host memory (launch set-up)
   const uint16_t* code;    // per-site code, or NULL: ucode at every site
   unsigned ucode;
+  const uint32_t* wcode;   // per-site forward-bond code, or NULL: wucode at every site
+  unsigned wucode;
   const double* wet;       // per-site wetting parameter (binary fluids), or NULL
   const int8_t* csign;     // per-DOF sign of (initial c - 0.5), or NULL (no confinement)
   const double* Bs;        // per-DOF bulk factors with per-site tensions, or NULL

@@ -22,11 +22,20 @@
 // so each thread reads its site's and its six neighbours' codes and factors
 // and writes its own gradient entries once: no atomics.
 //
-// One thread per (y, z) column marching over an x chunk, as pf.hip's generic
-// kernel; the y/z neighbours come through the L1/L2.  The factor tables by
-// site kind live in LDS.
+// Two kernels.  The tiled one (pf_ext_tile_kernel) follows pf.hip's tiled
+// stencil: x planes of a 64 x 4R tile stream through a two-plane LDS ring with
+// the site codes beside the concentrations, one vector load per site.  With
+// spatially uniform tensions the node volume enters every factor as the
+// ratio v/res^3 of the site kind (0, k/8 or 1, exact), so a bond's weight
+//     w = (v_i/res^3) h_i->j + (v_j/res^3) h_j->i
+// is the same for every fluid and g_i,k = bulk'_k + 2 f_k sum_j w (c_i - c_j)
+// with the bulk-site factor f_k: six weights per site serve all fluids.  The
+// generic kernel (pf_ext_kernel: one thread per (y, z) column, neighbours
+// through the L1/L2, factor tables by kind in LDS) covers per-site tensions,
+// more than four fluids and the single-term launches of energyComponents.
 #include "common.hpp"
 #include "pf.hpp"
+#include "pf_tile.hpp"
 
 namespace minim_amd {
 
@@ -337,6 +346,403 @@ __global__ __launch_bounds__(kBlock) void pf_ext_kernel(PFArgs a, PFCall cl, con
   grid_sum_store<2>(acc, rs);
 }
 
+// ---------------------------------------------------------------- tiled kernel
+// Barrier that orders LDS traffic only (no wait on outstanding global accesses).
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// Node volume of a site kind relative to a bulk site: 0 (solid), 1 (bulk) or
+// type/8 (surface types 1..7), all exact.
+template <typename T>
+__device__ __forceinline__ T vrel_of(unsigned q) {
+  const int k = (int)(q & 15u);
+  return k >= 2 ? (T)(k - 1) * (T)0.125 : (T)k;
+}
+
+template <typename T>
+struct PFF;  // bulk-site factors of the tiled stencil by precision
+template <>
+struct PFF<double> {
+  __device__ static double fb(const PFArgs& a, int k) { return a.fb[k]; }
+  __device__ static double fbg(const PFArgs& a, int k) { return a.fbg[k]; }
+  __device__ static double fg(const PFArgs& a, int k) { return a.fg[k]; }
+  __device__ static double fg2(const PFArgs& a, int k) { return a.fg2[k]; }
+};
+template <>
+struct PFF<float> {
+  __device__ static float fb(const PFArgs& a, int k) { return a.fb_f[k]; }
+  __device__ static float fbg(const PFArgs& a, int k) { return a.fbg_f[k]; }
+  __device__ static float fg(const PFArgs& a, int k) { return a.fg_f[k]; }
+  __device__ static float fg2(const PFArgs& a, int k) { return a.fg2_f[k]; }
+};
+
+// The constants of the extra terms, passed by value so that they sit in the
+// kernel-argument segment (scalar loads, uniform registers) in the arithmetic
+// precision.
+template <typename T>
+struct ExtK {
+  T pres[kPFMaxFluid];
+  T conf[kPFMaxFluid];
+  T fmag[kPFMaxFluid];
+  T fn[kPFMaxFluid][3];
+  T fc[3];
+  T area[kPFKinds];
+  T v1, res;
+};
+
+template <typename T>
+ExtK<T> ext_constants(const PFExt& E) {
+  ExtK<T> k{};
+  for (int i = 0; i < kPFMaxFluid; i++) {
+    k.pres[i] = (T)E.pres[i];
+    k.conf[i] = (T)E.conf[i];
+    k.fmag[i] = (T)E.fmag[i];
+    for (int j = 0; j < 3; j++) k.fn[i][j] = (T)E.fn[i][j];
+  }
+  for (int j = 0; j < 3; j++) k.fc[j] = (T)E.fc[j];
+  for (int i = 0; i < kPFKinds; i++) k.area[i] = (T)E.area[i];
+  k.v1 = (T)E.vol[1];
+  k.res = (T)E.res;
+  return k;
+}
+
+// Concentrations (x, or x + t*d) and forward-bond codes of the R own rows and
+// the halo site of one x plane.
+template <typename T, typename TV, int NF, int R, bool DIR>
+struct ExtPlane {
+  T x[R][NF];
+  T hx[NF];
+  TV d[DIR ? R : 1][NF];
+  TV hd[NF];
+  unsigned q[R];
+  unsigned hq;
+};
+
+template <typename T, int NF>
+__device__ __forceinline__ void ld_vec(const T* __restrict__ base, unsigned boff, T (&v)[NF]) {
+  const SiteVec<T, NF> s = *reinterpret_cast<const SiteVec<T, NF>*>(reinterpret_cast<const char*>(base) + boff);
+#pragma unroll
+  for (int k = 0; k < NF; k++) v[k] = s.v[k];
+}
+
+template <typename T, typename TV, int NF, int R, bool DIR>
+__device__ __forceinline__ void load_ext_plane(const PFArgs& a, size_t pl, const T* __restrict__ x,
+                                               const TV* __restrict__ d, const unsigned (&site)[R], unsigned hsite,
+                                               ExtPlane<T, TV, NF, R, DIR>& p) {
+  const T* xp = x + pl * NF;
+#pragma unroll
+  for (int r = 0; r < R; r++) ld_vec<T, NF>(xp, site[r] * (unsigned)(NF * sizeof(T)), p.x[r]);
+  ld_vec<T, NF>(xp, hsite * (unsigned)(NF * sizeof(T)), p.hx);
+  if constexpr (DIR) {
+    const TV* dp = d + pl * NF;
+#pragma unroll
+    for (int r = 0; r < R; r++) ld_vec<TV, NF>(dp, site[r] * (unsigned)(NF * sizeof(TV)), p.d[r]);
+    ld_vec<TV, NF>(dp, hsite * (unsigned)(NF * sizeof(TV)), p.hd);
+  }
+  if (a.wcode) {
+    const uint32_t* qp = a.wcode + pl;
+#pragma unroll
+    for (int r = 0; r < R; r++) p.q[r] = qp[site[r]];
+    p.hq = qp[hsite];
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; r++) p.q[r] = a.wucode;
+    p.hq = a.wucode;
+  }
+}
+
+template <typename T, typename TV, int NF, int R, bool DIR>
+__device__ __forceinline__ void resolve_ext_plane(ExtPlane<T, TV, NF, R, DIR>& p, T t) {
+  if constexpr (DIR) {
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        const T st = t * (T)p.d[r][k];
+        p.x[r][k] = p.x[r][k] + st;
+      }
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      const T st = t * (T)p.hd[k];
+      p.hx[k] = p.hx[k] + st;
+    }
+  }
+}
+
+// Weight of the forward bond along canonical dimension j from a forward-bond code.
+template <typename T>
+__device__ __forceinline__ T fwd_w(unsigned q, int j) {
+  return (T)((q >> (kPFWeightShift + kPFWeightBits * j)) & ((1u << kPFWeightBits) - 1u)) * (T)0.0625;
+}
+
+// Energy (the site's bulk term and its three forward bonds) and constrained
+// gradient of one site from its six neighbours, its forward-bond code qc and
+// the codes of its minus neighbours (whose forward bonds end here).  FULL adds
+// the force, confinement and wetting terms.
+template <typename T, int NF, bool WE, bool WG, bool FULL>
+__device__ __forceinline__ void site_ext(const PFArgs& a, const ExtK<T>& E, const T (&c)[NF], const T (&xm)[NF],
+                                         const T (&xp)[NF], const T (&ym)[NF], const T (&yp)[NF], const T (&zm)[NF],
+                                         const T (&zp)[NF], unsigned qc, unsigned qxm, unsigned qym, unsigned qzm,
+                                         int xi, int y, int z, size_t site, T& e, T (&g)[NF]) {
+  using C = PFF<T>;
+  const T vc = vrel_of<T>(qc);
+  const T wfx = fwd_w<T>(qc, 0), wfy = fwd_w<T>(qc, 1), wfz = fwd_w<T>(qc, 2);
+  const T wbx = fwd_w<T>(qxm, 0), wby = fwd_w<T>(qym, 1), wbz = fwd_w<T>(qzm, 2);
+  e = (T)0;
+#pragma unroll
+  for (int k = 0; k < NF; k++) {
+    const T ck = c[k];
+    const T a2 = ck - (T)1;
+    const T dxm = ck - xm[k], dxp = ck - xp[k];
+    const T dym = ck - ym[k], dyp = ck - yp[k];
+    const T dzm = ck - zm[k], dzp = ck - zp[k];
+    if (WE) {
+      T be;
+      if (NF == 1) {
+        const T a1 = ck + (T)1;
+        be = (C::fb(a, k) * vc) * ((a1 * a1) * (a2 * a2));
+      } else {
+        be = (C::fb(a, k) * vc) * ((ck * ck) * (a2 * a2));
+      }
+      T sq = (wfx * dxp) * dxp;
+      sq = fmad(wfy * dyp, dyp, sq);
+      sq = fmad(wfz * dzp, dzp, sq);
+      e = e + be;
+      e = fmad(C::fg(a, k), sq, e);
+    }
+    if (WG) {
+      T lap = wfx * dxp;
+      lap = fmad(wbx, dxm, lap);
+      lap = fmad(wfy, dyp, lap);
+      lap = fmad(wby, dym, lap);
+      lap = fmad(wfz, dzp, lap);
+      lap = fmad(wbz, dzm, lap);
+      T bg;
+      if (NF == 1) bg = (C::fbg(a, k) * vc) * (ck * fmad(ck, ck, (T)-1));
+      else bg = (C::fbg(a, k) * vc) * ((ck * a2) * fmad((T)2, ck, (T)-1));
+      g[k] = fmad(C::fg2(a, k), lap, bg);
+    }
+  }
+  const T v = vc * E.v1;
+  if (a.any_pres) {
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      const T pk = E.pres[k];
+      if (pk == (T)0) continue;
+      if (NF == 1) {
+        if (WE) e = e - pk * (((T)0.5 * (c[k] + (T)1)) * v);
+        if (WG) g[k] = g[k] - ((T)0.5 * pk) * v;
+      } else {
+        if (WE) e = e - pk * (c[k] * v);
+        if (WG) g[k] = g[k] - pk * v;
+      }
+    }
+  }
+  if (NF > 1 && a.density == kDensitySoft) {
+    T rho = (T)-1;
+#pragma unroll
+    for (int k = 0; k < NF; k++) rho = rho + c[k];
+    const T coef = (T)a.coef;
+    if (WE) e = e + coef * (rho * rho);
+    if (WG) {
+      const T gr = ((T)2 * coef) * rho;
+#pragma unroll
+      for (int k = 0; k < NF; k++) g[k] = g[k] + gr;
+    }
+  }
+  if constexpr (FULL) {
+    if (a.csign) {
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        const T coef = E.conf[k];
+        if (coef == (T)0) continue;
+        const int s0 = a.csign[site * NF + k];
+        const T u = c[k] - (T)0.5;
+        if ((s0 > 0 && u < (T)0) || (s0 < 0 && u > (T)0)) {
+          if (WE) e = e + coef * (u * u);
+          if (WG) g[k] = g[k] + (coef * (T)2) * u;
+        }
+      }
+    }
+    if (NF == 1 && a.wet) {
+      const T w = (T)a.wet[site];
+      if (w != (T)0) {
+        const T phi = c[0], ar = E.area[qc & 15u];
+        if (WE) e = e + (w * ((((phi * phi) * phi) / (T)3 - phi) - (T)2 / (T)3)) * ar;
+        if (WG) g[0] = g[0] + (w * (phi * phi - (T)1)) * ar;
+      }
+    }
+    if (a.any_force && (qc & 15u) != 0u) {
+      const T co[3] = {(T)xi - E.fc[0], (T)y - E.fc[1], (T)z - E.fc[2]};
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        const T fm = E.fmag[k];
+        if (fm == (T)0) continue;
+        const T cv = (NF == 1) ? (T)0.5 * ((T)1 + c[k]) : c[k];
+        if (cv < (T)0.01) continue;
+        T dot = (T)0;
+        dot = dot + co[0] * E.fn[k][0];
+        dot = dot + co[1] * E.fn[k][1];
+        dot = dot + co[2] * E.fn[k][2];
+        const T h = -dot * E.res;
+        const T epc = (v * fm) * h;
+        if (WE) e = e + cv * epc;
+        if (!WG) break;
+        g[k] = g[k] + ((NF == 1) ? (T)0.5 * epc : epc);
+      }
+    }
+  }
+  if (WG) {
+    if (a.fixmask) {
+#pragma unroll
+      for (int k = 0; k < NF; k++)
+        if (a.fixmask & (1u << k)) g[k] = (T)0;
+    }
+    if (NF > 1 && a.density == kDensityHard && a.nvar > 0) {
+      T comp = g[0];
+#pragma unroll
+      for (int k = 1; k < NF; k++) comp = comp + g[k];
+      comp = comp * (T)a.inv_nvar;
+#pragma unroll
+      for (int k = 0; k < NF; k++)
+        if (!(a.fixmask & (1u << k))) g[k] = g[k] - comp;
+    }
+  }
+}
+
+// MODE bits: 1 = energy, 2 = gradient (+ g.g), 4 = trial point x + t*d.
+// FULL: the force, confinement and wetting terms are compiled in.
+template <typename T, typename TV, int NF, int MODE, int R, bool FULL>
+__global__ __launch_bounds__(kBlock) void pf_ext_tile_kernel(PFArgs a, ExtK<T> K, TileGeom G, const T* __restrict__ x,
+                                                             const TV* __restrict__ d, T t, TV* __restrict__ g,
+                                                             RedScratch rs) {
+  constexpr bool WE = MODE & 1, WG = MODE & 2, DIR = MODE & 4;
+  constexpr int LH = kTileWaves * R + 2, LW = kTileZ + 2;
+  __shared__ T lds[2][LH][LW][NF];
+  __shared__ unsigned lq[2][LH][LW];
+
+  TileThread th;
+  tile_thread(G, xcd_logical_block(blockIdx.x, gridDim.x), threadIdx.x, th);
+  const size_t YZ = (size_t)G.Y * G.Z;
+  const int X = G.X;
+  unsigned site[R];
+  int yy[R], zz[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    site[r] = th.site[r];
+    yy[r] = (int)(th.site[r] / (unsigned)G.Z);
+    zz[r] = (int)(th.site[r] - (unsigned)yy[r] * (unsigned)G.Z);
+  }
+  const bool halo = th.hrow >= 0;
+  const int x0 = th.x0, x1 = th.x1;
+
+  T A[R][NF], B[R][NF], Cn[R][NF];
+  unsigned qA[R], qB[R];  // forward-bond codes of planes i-1 and i
+  {
+    ExtPlane<T, TV, NF, R, DIR> pm, p0, p1;
+    load_ext_plane<T, TV, NF, R, DIR>(a, (size_t)tile_plane(x0 - 1, X) * YZ, x, d, site, th.hsite, pm);
+    load_ext_plane<T, TV, NF, R, DIR>(a, (size_t)tile_plane(x0, X) * YZ, x, d, site, th.hsite, p0);
+    load_ext_plane<T, TV, NF, R, DIR>(a, (size_t)tile_plane(x0 + 1, X) * YZ, x, d, site, th.hsite, p1);
+    resolve_ext_plane<T, TV, NF, R, DIR>(pm, t);
+    resolve_ext_plane<T, TV, NF, R, DIR>(p0, t);
+    resolve_ext_plane<T, TV, NF, R, DIR>(p1, t);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        A[r][k] = pm.x[r][k];
+        B[r][k] = p0.x[r][k];
+        Cn[r][k] = p1.x[r][k];
+        lds[0][th.lrow0 + r][th.lcol][k] = p0.x[r][k];
+        lds[1][th.lrow0 + r][th.lcol][k] = p1.x[r][k];
+      }
+      qA[r] = pm.q[r];
+      qB[r] = p0.q[r];
+      lq[0][th.lrow0 + r][th.lcol] = p0.q[r];
+      lq[1][th.lrow0 + r][th.lcol] = p1.q[r];
+    }
+    if (halo) {
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        lds[0][th.hrow][th.hcol][k] = p0.hx[k];
+        lds[1][th.hrow][th.hcol][k] = p1.hx[k];
+      }
+      lq[0][th.hrow][th.hcol] = p0.hq;
+      lq[1][th.hrow][th.hcol] = p1.hq;
+    }
+  }
+  __syncthreads();
+
+  double acc[2] = {0.0, 0.0};
+#pragma nounroll
+  for (int i = x0; i < x1; i++) {
+    const int b = (i - x0) & 1;
+    ExtPlane<T, TV, NF, R, DIR> q;
+    load_ext_plane<T, TV, NF, R, DIR>(a, (size_t)tile_plane(i + 2, X) * YZ, x, d, site, th.hsite, q);
+
+    const size_t pl = (size_t)i * YZ;
+    T eacc = (T)0, gacc = (T)0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      T ym[NF], yp[NF], zm[NF], zp[NF], gv[NF], es;
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        ym[k] = (r > 0) ? B[r > 0 ? r - 1 : 0][k] : lds[b][th.lrow0 + r - 1][th.lcol][k];
+        yp[k] = (r < R - 1) ? B[r < R - 1 ? r + 1 : 0][k] : lds[b][th.lrow0 + r + 1][th.lcol][k];
+        zm[k] = lds[b][th.lrow0 + r][th.lcol - 1][k];
+        zp[k] = lds[b][th.lrow0 + r][th.lcol + 1][k];
+      }
+      const unsigned qym = (r > 0) ? qB[r > 0 ? r - 1 : 0] : lq[b][th.lrow0 + r - 1][th.lcol];
+      const unsigned qzm = lq[b][th.lrow0 + r][th.lcol - 1];
+      site_ext<T, NF, WE, WG, FULL>(a, K, B[r], A[r], Cn[r], ym, yp, zm, zp, qB[r], qA[r], qym, qzm, i, yy[r], zz[r],
+                                    pl + site[r], es, gv);
+      if (th.own & (1u << r)) {
+        if (WE) eacc = eacc + es;
+        if (WG) {
+          SiteVec<TV, NF> o;
+#pragma unroll
+          for (int k = 0; k < NF; k++) {
+            o.v[k] = (TV)gv[k];
+            gacc = fmad((T)o.v[k], (T)o.v[k], gacc);
+          }
+          TV* gp = g + (pl + site[r]) * NF;
+          if constexpr (sizeof(TV) == 4) {
+#pragma unroll
+            for (int k = 0; k < NF; k++) __builtin_nontemporal_store(o.v[k], gp + k);
+          } else {
+            *reinterpret_cast<SiteVec<TV, NF>*>(gp) = o;
+          }
+        }
+      }
+    }
+    if (WE) acc[0] += (double)eacc;
+    if (WG) acc[1] += (double)gacc;
+
+    lds_barrier();  // every wave is done reading buffer b
+    resolve_ext_plane<T, TV, NF, R, DIR>(q, t);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      // plane i+1's codes were stored in buffer b^1 last iteration; keep them in registers too
+      const unsigned q1 = lq[b ^ 1][th.lrow0 + r][th.lcol];
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        lds[b][th.lrow0 + r][th.lcol][k] = q.x[r][k];
+        A[r][k] = B[r][k];
+        B[r][k] = Cn[r][k];
+        Cn[r][k] = q.x[r][k];
+      }
+      lq[b][th.lrow0 + r][th.lcol] = q.q[r];
+      qA[r] = qB[r];
+      qB[r] = q1;
+    }
+    if (halo) {
+#pragma unroll
+      for (int k = 0; k < NF; k++) lds[b][th.hrow][th.hcol][k] = q.hx[k];
+      lq[b][th.hrow][th.hcol] = q.hq;
+    }
+  }
+  grid_sum_store<2>(acc, rs);
+}
+
 // ---------------------------------------------------------------- volumes
 __device__ __forceinline__ double site_vol(const PFArgs& a, size_t s) {
   const unsigned q = a.code ? (unsigned)a.code[s] : a.ucode;
@@ -454,12 +860,65 @@ void ext_nf(const PFArgs& a, const PFCall& c, int mode, const void* x, const voi
   }
 }
 
+constexpr int kExtTileTarget = 512;
+
+// Rows per thread of the tiled kernel: three plane copies of the own rows and
+// the plane in flight stay in registers.
+template <typename T, int NF>
+constexpr int ext_rows() {
+  return sizeof(T) * NF <= 12 ? 4 : 2;
+}
+
+template <typename T, typename TV, int NF, bool FULL>
+int ext_tile_full(const PFArgs& a, int mode, const void* x, const void* d, double t, void* g, RedScratch rs,
+                  hipStream_t s) {
+  constexpr int R = ext_rows<T, NF>();
+  TileGeom G;
+  const int target = a.target_blocks > 0 ? a.target_blocks : kExtTileTarget;
+  if (!tile_geom(a.X, a.Y, a.Z, R, a.xchunk_override, target, kMaxRedBlocks, G))
+    throw std::invalid_argument("PhaseField: grid too large for one launch");
+  const T* xx = static_cast<const T*>(x);
+  const TV* dd = static_cast<const TV*>((mode & 4) ? d : x);
+  TV* gg = static_cast<TV*>((mode & 2) ? g : const_cast<void*>(x));
+  const T tt = (T)t;
+  const ExtK<T> K = ext_constants<T>(*a.ext_host);
+  switch (mode) {
+    case 1:
+      pf_ext_tile_kernel<T, T, NF, 1, R, FULL><<<G.nblocks, kBlock, 0, s>>>(a, K, G, xx, (const T*)xx, tt, (T*)nullptr, rs);
+      break;
+    case 2: pf_ext_tile_kernel<T, TV, NF, 2, R, FULL><<<G.nblocks, kBlock, 0, s>>>(a, K, G, xx, dd, tt, gg, rs); break;
+    case 3: pf_ext_tile_kernel<T, TV, NF, 3, R, FULL><<<G.nblocks, kBlock, 0, s>>>(a, K, G, xx, dd, tt, gg, rs); break;
+    case 5: pf_ext_tile_kernel<T, TV, NF, 5, R, FULL><<<G.nblocks, kBlock, 0, s>>>(a, K, G, xx, dd, tt, gg, rs); break;
+    default: throw std::invalid_argument("pf: unsupported evaluation mode");
+  }
+  return G.nblocks;
+}
+
+template <typename T, typename TV, int NF>
+int ext_tile_nf(const PFArgs& a, int mode, const void* x, const void* d, double t, void* g, RedScratch rs,
+                hipStream_t s) {
+  if (a.any_force || a.csign || a.wet) return ext_tile_full<T, TV, NF, true>(a, mode, x, d, t, g, rs, s);
+  return ext_tile_full<T, TV, NF, false>(a, mode, x, d, t, g, rs, s);
+}
+
 template <typename T, typename TV>
-void ext_t(const PFArgs& a, const PFCall& c, int mode, const void* x, const void* d, double t, void* g,
-           RedScratch rs, hipStream_t s) {
+int ext_t(const PFArgs& a, const PFCall& c, int mode, const void* x, const void* d, double t, void* g,
+          RedScratch rs, hipStream_t s) {
+  // the tiled kernel: every term of energyGradient, uniform tensions, up to
+  // four fluids, planes addressable with 32-bit byte offsets
+  const bool tiled = !a.Fs && c.terms == kTermsEnergy && !c.raw && a.nf <= 4 &&
+                     (double)a.Y * a.Z * a.nf * sizeof(T) < 4294967296.0;
+  if (tiled) {
+    switch (a.nf) {
+      case 1: return ext_tile_nf<T, TV, 1>(a, mode, x, d, t, g, rs, s);
+      case 2: return ext_tile_nf<T, TV, 2>(a, mode, x, d, t, g, rs, s);
+      case 3: return ext_tile_nf<T, TV, 3>(a, mode, x, d, t, g, rs, s);
+      default: return ext_tile_nf<T, TV, 4>(a, mode, x, d, t, g, rs, s);
+    }
+  }
   if (a.Fs) {  // per-site tensions: runtime fluid count
     ext_nf<T, TV, 0, true>(a, c, mode, x, d, t, g, rs, s);
-    return;
+    return a.nblocks;
   }
   switch (a.nf) {
     case 1: ext_nf<T, TV, 1, false>(a, c, mode, x, d, t, g, rs, s); break;
@@ -468,6 +927,7 @@ void ext_t(const PFArgs& a, const PFCall& c, int mode, const void* x, const void
     case 4: ext_nf<T, TV, 4, false>(a, c, mode, x, d, t, g, rs, s); break;
     default: ext_nf<T, TV, 0, false>(a, c, mode, x, d, t, g, rs, s); break;
   }
+  return a.nblocks;
 }
 
 template <typename T>
@@ -522,16 +982,17 @@ void volumes_t(const PFArgs& a, const void* x, size_t nsite, RedScratch rs, hipS
 int pf_launch_ext(const PFArgs& a, const PFCall& c, int prec, int pv, int mode, const void* x, const void* d,
                   double t, void* g, RedScratch rs, hipStream_t s) {
   if (!a.ext) throw std::runtime_error("PhaseField: general kernel constants missing");
+  int nb;
   if (prec == F64 && pv == F64)
-    ext_t<double, double>(a, c, mode, x, d, t, g, rs, s);
+    nb = ext_t<double, double>(a, c, mode, x, d, t, g, rs, s);
   else if (prec == F32 && pv == F32)
-    ext_t<float, float>(a, c, mode, x, d, t, g, rs, s);
+    nb = ext_t<float, float>(a, c, mode, x, d, t, g, rs, s);
   else if (prec == F64 && pv == F32)
-    ext_t<double, float>(a, c, mode, x, d, t, g, rs, s);
+    nb = ext_t<double, float>(a, c, mode, x, d, t, g, rs, s);
   else
     throw std::invalid_argument("PhaseField: FP32 coordinates with FP64 vectors are not supported");
   MINIM_HIP_CHECK(hipGetLastError());
-  return a.nblocks;
+  return nb;
 }
 
 void pf_launch_volume_projection(const PFArgs& a, int prec, void* v, size_t nsite, bool norm, RedScratch rs,

@@ -30,9 +30,10 @@ struct PFHolder {
   PFArgs a;
   minim_amd::PFExt ext_h{};
   std::vector<uint16_t> code_h;
+  std::vector<uint32_t> wcode_h;
   std::vector<double> wet_h, bs_h, fs_h;
   std::vector<int8_t> csign_h;
-  minim_amd::DeviceBuffer ext, code, wet, csign, bs, fs;
+  minim_amd::DeviceBuffer ext, code, wcode, wet, csign, bs, fs;
   std::once_flag uploaded;
 };
 
@@ -53,6 +54,7 @@ const PFArgs& args_of(const std::shared_ptr<void>& p) {
     MINIM_HIP_CHECK(hipMemcpy(h->ext.get(), &h->ext_h, sizeof(h->ext_h), hipMemcpyHostToDevice));
     h->a.ext = static_cast<const minim_amd::PFExt*>(h->ext.get());
     h->a.code = to_device(h->code, h->code_h);
+    h->a.wcode = to_device(h->wcode, h->wcode_h);
     h->a.wet = to_device(h->wet, h->wet_h);
     h->a.csign = to_device(h->csign, h->csign_h);
     h->a.Bs = to_device(h->bs, h->bs_h);
@@ -461,13 +463,40 @@ void PhaseField::init(const vector<double>& coords) {
           }
           code[i] = (uint16_t)q;
         }
+    // forward-bond weights for the tiled kernel: w(i, i+e) = (v_i h_i->i+e +
+    // v_{i+e} h_{i+e}->i) / res^3 in sixteenths (v/res^3 in eighths, h in halves)
+    std::vector<uint32_t> wcode(ng);
+    auto v8 = [](unsigned q) { const unsigned k = q & 15u; return k >= 2 ? k - 1 : 8 * k; };
+    auto hq = [](unsigned q, int j, int side) { return (q >> (minim_amd::kPFCodeShift + 4 * j + 2 * side)) & 3u; };
+    for (int x = 0; x < n0; x++)
+      for (int y = 0; y < n1; y++)
+        for (int z = 0; z < n2; z++) {
+          const size_t i = ((size_t)x * n1 + y) * n2 + z;
+          uint32_t w = (uint32_t)kinds[i];
+          for (int j = 0; j < 3; j++) {
+            const int dd = cmap[j];
+            if (dd < 0) continue;
+            int cp[3] = {x, y, z};
+            cp[dd] = (cp[dd] + 1) % gridSize[dd];
+            const size_t ip = ((size_t)cp[0] * n1 + cp[1]) * n2 + cp[2];
+            const unsigned w16 = v8(code[i]) * hq(code[i], j, 1) + v8(code[ip]) * hq(code[ip], j, 0);
+            w |= w16 << (minim_amd::kPFWeightShift + minim_amd::kPFWeightBits * j);
+          }
+          wcode[i] = w;
+        }
     h->code_h = std::move(code);
+    h->wcode_h = std::move(wcode);
   }
   {
     unsigned q = 1;  // bulk site, both bonds one half along every dimension of size > 1
+    uint32_t w = 1;  // and forward-bond weight one
     for (int j = 0; j < 3; j++)
-      if (cmap[j] >= 0) q |= (1u << (minim_amd::kPFCodeShift + 4 * j)) | (1u << (minim_amd::kPFCodeShift + 4 * j + 2));
+      if (cmap[j] >= 0) {
+        q |= (1u << (minim_amd::kPFCodeShift + 4 * j)) | (1u << (minim_amd::kPFCodeShift + 4 * j + 2));
+        w |= 16u << (minim_amd::kPFWeightShift + minim_amd::kPFWeightBits * j);
+      }
     A.ucode = q;
+    A.wucode = w;
   }
   auto volAt = [&](size_t i) { return anySolid ? nodeVol[i] : E.vol[1]; };
 
@@ -535,7 +564,7 @@ void PhaseField::init(const vector<double>& coords) {
     A.volfix = 1;
     A.tv2 = totalVolume2;
   }
-  h->ext_h = E;
+  A.ext_host = &h->ext_h;
   A.use_ext = (anySolid || anyWet || A.any_pres || anyForce || anyConf || perSite || stencilKernel == 1) ? 1 : 0;
   args_ = h;
 }
