@@ -183,7 +183,7 @@ def pf_general_leg(m, n=256, steps=20, warmup=3, ref_n=48):
         ms = time_device_evals(m, h, prec, x, g, nd, steps, warmup)
         msk = m.potential_time_kernel(h, prec, x.data_ptr(), nd, g_ptr=g.data_ptr(), reps=steps,
                                       stream=stream.cuda_stream)
-        alg = sites * 3 * (b + b) + sites * 2  # coordinates in, gradient out, 2-byte site code
+        alg = sites * 3 * (b + b) + sites * 4  # coordinates in, gradient out, 4-byte forward-bond code
         out[prec] = dict(eval_ms=ms, evals_per_s=1e3 / ms, kernel_ms=msk, algorithmic_bytes=alg,
                          achieved_GBs=alg / (msk * 1e-3) / 1e9, frac_of_hbm_peak=alg / (msk * 1e-3) / 1e9 / HBM_PEAK_GBS)
         del x, g
