@@ -63,19 +63,13 @@ __device__ __forceinline__ void ld_site(const T* __restrict__ base, unsigned bof
   for (int k = 0; k < NF; k++) v[k] = s.v[k];
 }
 
+// Streaming (nt) stores: the gradient is not re-read by this kernel (FP32
+// 4.2 -> 6.9 TB/s in a copy test; FP64 E+G 174 -> 169 us at 256^3 x 3).
 template <typename T, int NF>
 __device__ __forceinline__ void st_site(T* __restrict__ base, unsigned boff, const T (&v)[NF]) {
-  if (sizeof(T) == 4) {
-    // streaming: the gradient is not re-read by this kernel
-    T* p = reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff);
+  T* p = reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff);
 #pragma unroll
-    for (int k = 0; k < NF; k++) __builtin_nontemporal_store(v[k], p + k);
-  } else {
-    SiteV<T, NF> s;
-#pragma unroll
-    for (int k = 0; k < NF; k++) s.v[k] = v[k];
-    *reinterpret_cast<SiteV<T, NF>*>(reinterpret_cast<char*>(base) + boff) = s;
-  }
+  for (int k = 0; k < NF; k++) __builtin_nontemporal_store(v[k], p + k);
 }
 
 // Per-precision coefficient tables of PFArgs.

@@ -704,13 +704,9 @@ __global__ __launch_bounds__(kBlock) void pf_ext_tile_kernel(PFArgs a, ExtK<T> K
             o.v[k] = (TV)gv[k];
             gacc = fmad((T)o.v[k], (T)o.v[k], gacc);
           }
-          TV* gp = g + (pl + site[r]) * NF;
-          if constexpr (sizeof(TV) == 4) {
+          TV* gp = g + (pl + site[r]) * NF;  // streaming stores, as the plain stencil's
 #pragma unroll
-            for (int k = 0; k < NF; k++) __builtin_nontemporal_store(o.v[k], gp + k);
-          } else {
-            *reinterpret_cast<SiteVec<TV, NF>*>(gp) = o;
-          }
+          for (int k = 0; k < NF; k++) __builtin_nontemporal_store(o.v[k], gp + k);
         }
       }
     }
