@@ -738,6 +738,433 @@ void orc_pfx_potential(const orc_pfx* p, orc_pot* out) {
 }
 
 /* ------------------------------------------------------------------------- */
+/* PhaseFieldUnstructured: the element form of the phase field               */
+/* ------------------------------------------------------------------------- */
+
+enum { PFU_FLUID = 0, PFU_FLUID_ALL = 1, PFU_PRESSURE = 3, PFU_DENSITY = 4, PFU_SURFACE = 5, PFU_FORCE = 6,
+       PFU_CONF = 7 };
+
+void orc_pfu_free(orc_pfu* p) {
+  free(p->node_vol);
+  free(p->el_type);
+  free(p->el_idof);
+  free(p->el_par);
+  memset(p, 0, sizeof(*p));
+}
+
+static void pfu_push(orc_pfu* p, size_t* cap, int type, const int* idof, int n, const double* par, int npar) {
+  if (p->nel == *cap) {
+    *cap = *cap ? 2 * *cap : 1024;
+    p->el_type = (int*)realloc(p->el_type, sizeof(int) * *cap);
+    p->el_idof = (int*)realloc(p->el_idof, sizeof(int) * ORC_PFU_NIDOF * *cap);
+    p->el_par = (double*)realloc(p->el_par, sizeof(double) * 3 * *cap);
+  }
+  p->el_type[p->nel] = type;
+  for (int i = 0; i < ORC_PFU_NIDOF; i++) p->el_idof[p->nel * ORC_PFU_NIDOF + i] = i < n ? idof[i] : -1;
+  for (int i = 0; i < 3; i++) p->el_par[p->nel * 3 + i] = i < npar ? par[i] : 0;
+  p->nel++;
+}
+
+/* PhaseFieldUnstructured::assignFluidCoefficients (PhaseFieldUnstructured.cpp:72-107),
+ * init (:110-249), initLocal on one rank (:252-268), setDefaults, checkArraySizes. */
+int orc_pfu_init(orc_pfu* p, const int grid[3], int nfluid, double resolution, const double* surface_tension,
+                 int n_st, const double* interface_size, int n_iw, int density, double density_const,
+                 const unsigned char* fix_fluid, const unsigned char* solid, const double* contact_angle, int n_ca,
+                 const double* pressure, const double* force, const double* confinement, int volume_fixed,
+                 double vol_const, const double* volume, const double* x0) {
+  memset(p, 0, sizeof(*p));
+  if (nfluid < 1 || nfluid > ORC_MAXF) return 1;
+  for (int d = 0; d < 3; d++) {
+    if (grid[d] < 1) return 1;
+    p->grid[d] = grid[d];
+  }
+  const int nF = nfluid;
+  const size_t ng = (size_t)grid[0] * grid[1] * grid[2];
+  p->ngrid = ng;
+  p->nfluid = nF;
+  p->resolution = resolution;
+  p->density = density;
+  p->density_const = density_const;
+  /* the soft constraint's element reads three DOFs of each site
+     (PhaseFieldUnstructured.cpp:628): with two fluids that reads past the element */
+  if (nF == 2 && density == ORC_DENSITY_SOFT) return 6;
+  double iw[ORC_MAXF], st[ORC_MAXF];
+  if (n_iw == 0) {
+    for (int i = 0; i < nF; i++) iw[i] = resolution;
+  } else if (n_iw == 1) {
+    for (int i = 0; i < nF; i++) iw[i] = interface_size[0];
+  } else if (n_iw == nF) {
+    for (int i = 0; i < nF; i++) iw[i] = interface_size[i];
+  } else {
+    return 3;
+  }
+  if (n_st == 1) {
+    for (int i = 0; i < nF; i++) st[i] = surface_tension[0];
+  } else if (n_st == nF) {
+    for (int i = 0; i < nF; i++) st[i] = surface_tension[i];
+  } else {
+    return 2;
+  }
+  double sum = 0;
+  for (int i = 0; i < nF; i++) sum += st[i];
+  p->st_mean = sum / nF;
+  if (nF <= 2) {
+    for (int i = 0; i < nF; i++) {
+      p->kappa[i] = 3 * st[i] / iw[i];
+      p->kappaP[i] = 3 * st[i] * iw[i];
+    }
+  } else {
+    double ks[ORC_MAXF], kps[ORC_MAXF];
+    for (int i = 0; i < nF; i++) {
+      ks[i] = 6 * st[i] / iw[i];
+      kps[i] = 6 * st[i] * iw[i];
+    }
+    const int l = nF - 1;
+    p->kappa[0] = 0.5 * (ks[0] + ks[1] - ks[l]);
+    p->kappa[1] = 0.5 * (ks[0] - ks[1] + ks[l]);
+    p->kappa[2] = 0.5 * (-ks[0] + ks[1] + ks[l]);
+    p->kappaP[0] = 0.5 * (kps[0] + kps[1] - kps[l]);
+    p->kappaP[1] = 0.5 * (kps[0] - kps[1] + kps[l]);
+    p->kappaP[2] = 0.5 * (-kps[0] + kps[1] + kps[l]);
+    for (int i = 3; i < nF; i++) { /* :103-104, unlike PhaseField.cpp:128-129 */
+      p->kappa[i] = 0.5 * (-ks[0] - ks[1] + ks[l]) + ks[i - 1];
+      p->kappaP[i] = 0.5 * (-kps[0] - kps[1] + kps[l]) + kps[i - 1];
+    }
+  }
+  p->convergence = 1e-8 * p->st_mean * pow(resolution, 2);
+  for (int i = 0; i < nF; i++) p->fix[i] = fix_fluid ? fix_fluid[i] : 0;
+  if (density == ORC_DENSITY_FIXED) p->fix[nF - 1] = 1;
+  if (contact_angle && n_ca != (int)(ng * nF)) return 4;
+
+  double fmag[ORC_MAXF] = {0}, fnorm[ORC_MAXF][3];
+  if (force) {
+    for (int i = 0; i < nF; i++) {
+      double r = 0;
+      for (int d = 0; d < 3; d++) r += force[3 * i + d] * force[3 * i + d];
+      fmag[i] = sqrt(r);
+      for (int d = 0; d < 3; d++) fnorm[i][d] = force[3 * i + d] / fmag[i];
+    }
+  }
+  unsigned char* sol = (unsigned char*)calloc(ng, 1);
+  if (solid) memcpy(sol, solid, ng);
+  int* type = (int*)malloc(sizeof(int) * ng);
+  if (orc_pf_site_types(grid, sol, type)) {
+    free(sol);
+    free(type);
+    return 5;
+  }
+  p->node_vol = (double*)malloc(sizeof(double) * ng);
+  double* area = (double*)malloc(sizeof(double) * ng);
+  for (size_t i = 0; i < ng; i++) {
+    const int t = type[i];
+    double v = 0, a = 0;
+    if (t == 0) {
+      v = 1;
+    } else if (t > 0) {
+      v = t / 8.0;
+      if (t == 2 || t == 4 || t == 6) a = 1;
+      if (t == 1 || t == 7) a = 0.75;
+      if (t == 3 || t == 5) a = 1.25;
+    }
+    p->node_vol[i] = v * pow(resolution, 3);
+    area[i] = a * pow(resolution, 2);
+  }
+  p->volume_fixed = volume_fixed || volume != NULL;
+  p->vol_const = vol_const;
+  if (volume) {
+    for (int i = 0; i < nF; i++) p->volume[i] = volume[i];
+  } else if (p->volume_fixed) {
+    for (size_t ig = 0; ig < ng; ig++)
+      for (int i = 0; i < nF; i++) p->volume[i] += x0[i + nF * ig] * p->node_vol[ig];
+  }
+
+  /* elements, site by site (:153-228) */
+  size_t cap = 0;
+  const int n0 = grid[0], n1 = grid[1], n2 = grid[2];
+  static const int off[6][3] = {{-1, 0, 0}, {0, -1, 0}, {0, 0, -1}, {0, 0, 1}, {0, 1, 0}, {1, 0, 0}};
+  for (int x = 0; x < n0; x++)
+    for (int y = 0; y < n1; y++)
+      for (int z = 0; z < n2; z++) {
+        const int iGrid = (x * n1 + y) * n2 + z;
+        if (sol[iGrid]) continue;
+        int nodes[7];
+        nodes[0] = iGrid;
+        for (int k = 0; k < 6; k++) {
+          const int j = (((x + off[k][0] + n0) % n0) * n1 + (y + off[k][1] + n1) % n1) * n2 + (z + off[k][2] + n2) % n2;
+          nodes[k + 1] = sol[j] ? iGrid : j;
+        }
+        const double vol = p->node_vol[iGrid];
+        if (density == ORC_DENSITY_FIXED) {
+          int idof[7];
+          for (int k = 0; k < 7; k++) idof[k] = nodes[k] * nF;
+          pfu_push(p, &cap, PFU_FLUID_ALL, idof, 7, &vol, 1);
+        } else {
+          for (int f = 0; f < nF; f++) {
+            int idof[7];
+            for (int k = 0; k < 7; k++) idof[k] = nodes[k] * nF + f;
+            const double par[2] = {vol, (double)f};
+            pfu_push(p, &cap, PFU_FLUID, idof, 7, par, 2);
+          }
+        }
+        for (int f = 0; f < nF; f++) {
+          if (!pressure || pressure[f] == 0) continue;
+          const int idof = iGrid * nF + f;
+          const double par = pressure[f];
+          pfu_push(p, &cap, PFU_PRESSURE, &idof, 1, &par, 1);
+        }
+        if (nF > 1 && density == ORC_DENSITY_SOFT) {
+          int idof[ORC_PFU_NIDOF];
+          const int n = nF < ORC_PFU_NIDOF ? nF : ORC_PFU_NIDOF;
+          for (int f = 0; f < n; f++) idof[f] = iGrid * nF + f;
+          pfu_push(p, &cap, PFU_DENSITY, idof, n, NULL, 0);
+        }
+        if (area[iGrid] > 0 && contact_angle && contact_angle[iGrid] != 90) {
+          const double w = 1 / sqrt(2.0) * cos(contact_angle[iGrid] * 3.1415926536 / 180);
+          for (int f = 0; f < nF; f++) {
+            const int idof = iGrid * nF + f;
+            const double par[2] = {area[iGrid], w};
+            pfu_push(p, &cap, PFU_SURFACE, &idof, 1, par, 2);
+          }
+        }
+        for (int f = 0; f < nF; f++) {
+          if (!(fmag[f] > 0)) continue;
+          const double coord[3] = {x - (n0 - 1) / 2.0, y - (n1 - 1) / 2.0, z - (n2 - 1) / 2.0};
+          double dot = 0;
+          for (int d = 0; d < 3; d++) dot += coord[d] * fnorm[f][d];
+          const double h = -dot * resolution;
+          const double par[3] = {vol, fmag[f], h};
+          const int idof = iGrid * nF + f;
+          pfu_push(p, &cap, PFU_FORCE, &idof, 1, par, 3);
+        }
+        for (int f = 0; f < nF; f++) {
+          if (!confinement || confinement[f] == 0) continue;
+          const int idof = iGrid * nF + f;
+          const double par[2] = {confinement[f], x0[idof]};
+          pfu_push(p, &cap, PFU_CONF, &idof, 1, par, 2);
+        }
+      }
+  free(sol);
+  free(type);
+  free(area);
+  return 0;
+}
+
+/* phaseGradient of the element form (PhaseFieldUnstructured.cpp:311-405): a
+ * neighbour replaced by the site itself marks a solid one. */
+static void pfu_phase_gradient(const double* x, const int* idof, double factor, double* e, double* g) {
+  static const int mp[3][2] = {{1, 6}, {2, 5}, {3, 4}};
+  const double c1 = x[idof[0]];
+  double grad2 = 0;
+  for (int d = 0; d < 3; d++) {
+    const int im = idof[mp[d][0]], ip = idof[mp[d][1]];
+    if (im != idof[0] && ip != idof[0]) {
+      const double gradm = c1 - x[im], gradp = c1 - x[ip];
+      if (e) grad2 += 0.5 * (pow(gradm, 2) + pow(gradp, 2));
+      if (g) {
+        g[idof[0]] += factor * (gradm + gradp);
+        g[im] -= factor * gradm;
+        g[ip] -= factor * gradp;
+      }
+    } else if (ip != idof[0]) {
+      const double gradp = c1 - x[ip];
+      if (e) grad2 += pow(gradp, 2);
+      if (g) {
+        g[idof[0]] += factor * 2 * gradp;
+        g[ip] -= factor * 2 * gradp;
+      }
+    } else if (im != idof[0]) {
+      const double gradm = c1 - x[im];
+      if (e) grad2 += pow(gradm, 2);
+      if (g) {
+        g[idof[0]] += factor * 2 * gradm;
+        g[im] -= factor * 2 * gradm;
+      }
+    }
+  }
+  if (e) *e += factor * grad2;
+}
+
+/* elementEnergyGradient (:563-590) of one element */
+static void pfu_element(const orc_pfu* p, const double* x, size_t ie, double* e, double* g) {
+  const int nF = p->nfluid;
+  const int* idof = p->el_idof + ie * ORC_PFU_NIDOF;
+  const double* par = p->el_par + ie * 3;
+  const double res = p->resolution;
+  switch (p->el_type[ie]) {
+    case PFU_FLUID: { /* fluidEnergy (:409-428) */
+      const double vol = par[0];
+      const int f = (int)par[1];
+      const double c = x[idof[0]];
+      if (nF == 1) {
+        const double factor = p->kappa[0] / 16 * vol;
+        if (e) *e += factor * pow(c + 1, 2) * pow(c - 1, 2);
+        if (g) g[idof[0]] += factor * 4 * c * (c * c - 1);
+      } else {
+        const double factor = 0.5 * p->kappa[f] * vol;
+        if (e) *e += factor * pow(c, 2) * pow(c - 1, 2);
+        if (g) g[idof[0]] += factor * 2 * c * (c - 1) * (2 * c - 1);
+      }
+      double factor = (nF == 1) ? 0.25 * p->kappaP[0] * vol : 0.5 * p->kappaP[f] * vol;
+      factor = factor / pow(res, 2);
+      pfu_phase_gradient(x, idof, factor, e, g);
+      break;
+    }
+    case PFU_FLUID_ALL: { /* fluidEnergyAll (:431-478) */
+      const double vol = par[0];
+      double cN[7], gN[7] = {0};
+      static const int iota[7] = {0, 1, 2, 3, 4, 5, 6};
+      for (int i = 0; i < 7; i++) cN[i] = 1;
+      for (int f = 0; f < nF - 1; f++)
+        for (int i = 0; i < 7; i++) cN[i] -= x[idof[i] + f];
+      const double bfN = 0.5 * p->kappa[nF - 1] * vol;
+      const double gfN = 0.5 * p->kappaP[nF - 1] * vol; /* not divided by res^2 (:444) */
+      if (e) *e += bfN * pow(cN[0], 2) * pow(cN[0] - 1, 2);
+      if (g) {
+        gN[0] += bfN * 2 * cN[0] * (cN[0] - 1) * (2 * cN[0] - 1);
+        pfu_phase_gradient(cN, iota, gfN, e, gN);
+      } else {
+        pfu_phase_gradient(cN, iota, gfN, e, NULL);
+      }
+      for (int f = 0; f < nF - 1; f++) {
+        int id[7];
+        for (int i = 0; i < 7; i++) id[i] = idof[i] + f;
+        const double c = x[id[0]];
+        const double bf = 0.5 * p->kappa[f] * vol;
+        if (e) *e += bf * pow(c, 2) * pow(c - 1, 2);
+        if (g) g[id[0]] += bf * 2 * c * (c - 1) * (2 * c - 1);
+        double gf = 0.5 * p->kappaP[f] * vol;
+        gf = gf / pow(res, 2);
+        pfu_phase_gradient(x, id, gf, e, g);
+        if (g)
+          for (int i = 0; i < 7; i++) g[id[i]] -= gN[i];
+      }
+      break;
+    }
+    case PFU_PRESSURE: { /* pressureEnergy (:508-525) */
+      const int iDof = idof[0];
+      const double pr = par[0];
+      const double v = p->node_vol[iDof / nF];
+      if (nF == 1) {
+        const double volume = 0.5 * (x[iDof] + 1) * v;
+        if (e) *e -= pr * volume;
+        if (g) g[iDof] -= 0.5 * pr * v;
+      } else {
+        const double volume = x[iDof] * v;
+        if (e) *e -= pr * volume;
+        if (g) g[iDof] -= pr * v;
+      }
+      break;
+    }
+    case PFU_DENSITY: { /* densityConstraintEnergy (:528-541): three DOFs */
+      if (nF == 1) break;
+      const double coef = p->density_const * p->st_mean * pow(res, 2);
+      const double rho = x[idof[0]] + x[idof[1]] + x[idof[2]] - 1;
+      if (e) *e += coef * pow(rho, 2);
+      if (g) {
+        g[idof[0]] += 2 * coef * rho;
+        g[idof[1]] += 2 * coef * rho;
+        g[idof[2]] += 2 * coef * rho;
+      }
+      break;
+    }
+    case PFU_SURFACE: /* surfaceEnergy (:544-553) */
+      if (nF == 1) {
+        const double phi = x[idof[0]];
+        if (e) *e += par[1] * (pow(phi, 3) / 3 - phi - 2.0 / 3) * par[0];
+        if (g) g[idof[0]] += par[1] * (pow(phi, 2) - 1) * par[0];
+      }
+      break;
+    case PFU_FORCE: { /* forceEnergy (:556-575): every fluid, also without a gradient */
+      const double c = (nF == 1) ? 0.5 * (1 + x[idof[0]]) : x[idof[0]];
+      if (c < 0.01) break;
+      const double vol = par[0], f = par[1], h = par[2];
+      if (e) *e += c * vol * f * h;
+      if (g) {
+        if (nF == 1) g[idof[0]] += 0.5 * vol * f * h;
+        else g[idof[0]] += vol * f * h;
+      }
+      break;
+    }
+    case PFU_CONF: { /* ffConfinementEnergy (:578-592) */
+      const double c = x[idof[0]];
+      const double coef = par[0] * p->st_mean * pow(res, 2);
+      const double c0 = par[1];
+      if ((c0 - 0.5) * (c - 0.5) < 0) {
+        if (e) *e += coef * pow(c - 0.5, 2);
+        if (g) g[idof[0]] += coef * 2 * (c - 0.5);
+      }
+      break;
+    }
+  }
+}
+
+/* State's elementEG (src/State.cpp:58-79) without the constraints: e set, g
+ * zeroed, every element in order, then blockEnergyGradient (:271-297). */
+void orc_pfu_energy_gradient(const orc_pfu* p, const double* x, double* e, double* g) {
+  const int nF = p->nfluid;
+  const size_t nd = p->ngrid * nF;
+  if (e) *e = 0;
+  if (g) memset(g, 0, sizeof(double) * nd);
+  for (size_t ie = 0; ie < p->nel; ie++) pfu_element(p, x, ie, e, g);
+  if (!p->volume_fixed) return;
+  double vol[ORC_MAXF] = {0};
+  for (size_t iDof = 0; iDof < nd; iDof++) {
+    const double v = p->node_vol[iDof / nF];
+    if (nF == 1) vol[0] += 0.5 * (x[iDof] + 1) * v;
+    else vol[iDof % nF] += x[iDof] * v;
+  }
+  const double coef = p->vol_const * p->st_mean / pow(p->resolution, 4);
+  for (int f = 0; f < nF; f++) {
+    const double diff = vol[f] - p->volume[f];
+    if (e) *e += coef * pow(diff, 2) / 1;
+    if (!g) continue;
+    for (size_t ig = 0; ig < p->ngrid; ig++) {
+      const size_t iDof = ig * nF + f;
+      const double w0 = 4 * x[iDof] * (1 - x[iDof]);
+      const double w = w0 > 0.0 ? w0 : 0.0;
+      g[iDof] += coef * diff * p->node_vol[ig] * w;
+    }
+  }
+}
+
+/* Potential::applyConstraints (src/Potential.cpp:102-115) with the constraints
+ * PhaseFieldUnstructured::init sets (:231-247): per site the variable fluids'
+ * sum (hard density constraint), then the fixed fluids' DOFs. */
+void orc_pfu_constraints(const orc_pfu* p, double* data) {
+  const int nF = p->nfluid;
+  int var[ORC_MAXF], nvar = 0;
+  for (int f = 0; f < nF; f++)
+    if (!p->fix[f]) var[nvar++] = f;
+  if (nF > 1 && p->density == ORC_DENSITY_HARD) {
+    for (size_t ig = 0; ig < p->ngrid; ig++) {
+      double nn = 0, gn = 0;
+      for (int a = 0; a < nvar; a++) nn += 1.0 * 1.0;
+      for (int a = 0; a < nvar; a++) gn += data[ig * nF + var[a]] * 1.0;
+      for (int a = 0; a < nvar; a++) data[ig * nF + var[a]] -= gn * 1.0 / nn;
+    }
+  }
+  for (int f = 0; f < nF; f++) {
+    if (!p->fix[f]) continue;
+    for (size_t ig = 0; ig < p->ngrid; ig++) data[ig * nF + f] = 0;
+  }
+}
+
+static void pfu_eg_cb(const void* ctx, const double* x, double* e, double* g) {
+  orc_pfu_energy_gradient((const orc_pfu*)ctx, x, e, g);
+}
+static void pfu_con_cb(const void* ctx, const double* x, double* data) {
+  (void)x;
+  orc_pfu_constraints((const orc_pfu*)ctx, data);
+}
+void orc_pfu_potential(const orc_pfu* p, orc_pot* out) {
+  out->ctx = p;
+  out->ndof = p->ngrid * p->nfluid;
+  out->eg = pfu_eg_cb;
+  out->constrain = pfu_con_cb;
+  out->convergence = p->convergence;
+}
+
+/* ------------------------------------------------------------------------- */
 /* State-level helpers                                                       */
 /* ------------------------------------------------------------------------- */
 

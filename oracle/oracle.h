@@ -102,6 +102,44 @@ void orc_pfx_constraints(const orc_pfx* p, double* data);
  * volume constraint. */
 void orc_pfx_components(const orc_pfx* p, const double* x, double out[14]);
 void orc_pfx_potential(const orc_pfx* p, orc_pot* out);
+/* ---- PhaseFieldUnstructured: the same physics as an element list (one
+ * seven-site element per site and fluid, or per site in the "fixed" density
+ * mode, plus one-DOF pressure, surface, force and confinement elements and a
+ * soft-density element), the volume constraint as an energy term, and the
+ * constraints as Potential::setConstraints groups. ---- */
+#define ORC_PFU_NIDOF 16
+typedef struct orc_pfu {
+  int grid[3];
+  int nfluid;
+  double resolution;
+  int density;
+  double density_const;
+  double kappa[ORC_MAXF], kappaP[ORC_MAXF];
+  double st_mean, convergence;
+  unsigned char fix[ORC_MAXF];
+  size_t ngrid;
+  double* node_vol;  /* ngrid */
+  int volume_fixed;
+  double vol_const;
+  double volume[ORC_MAXF];
+  size_t nel;
+  int* el_type;      /* nel */
+  int* el_idof;      /* nel * ORC_PFU_NIDOF, unused slots -1 */
+  double* el_par;    /* nel * 3 */
+} orc_pfu;
+/* Arguments as orc_pfx_init (per-site tensions excepted).  Returns nonzero
+ * where the reference throws, and 6 for the soft constraint with two fluids,
+ * whose element the reference reads past. */
+int orc_pfu_init(orc_pfu* p, const int grid[3], int nfluid, double resolution, const double* surface_tension,
+                 int n_st, const double* interface_size, int n_iw, int density, double density_const,
+                 const unsigned char* fix_fluid, const unsigned char* solid, const double* contact_angle, int n_ca,
+                 const double* pressure, const double* force, const double* confinement, int volume_fixed,
+                 double vol_const, const double* volume, const double* x0);
+void orc_pfu_free(orc_pfu* p);
+void orc_pfu_energy_gradient(const orc_pfu* p, const double* x, double* e, double* g);
+void orc_pfu_constraints(const orc_pfu* p, double* data);
+void orc_pfu_potential(const orc_pfu* p, orc_pot* out);
+
 /* PhaseField::getType (PhaseField.cpp:899-958) of every site; returns 0, or
  * 1 where the reference throws "Undefined surface type". */
 int orc_pf_site_types(const int grid[3], const unsigned char* solid, int* type);

@@ -20,6 +20,7 @@
 #include "potentials/BarAndHinge.h"
 #include "potentials/LjNd.h"
 #include "potentials/PhaseField.h"
+#include "potentials/PhaseFieldUnstructured.h"
 
 using namespace minim;
 using std::vector;
@@ -369,6 +370,42 @@ int ref_pf_diffuse_solid(const int* grid, int nfluid, double res, const double* 
   }
   return 0;
 }
+
+// ---- PhaseFieldUnstructured through its own setters ----
+void* ref_pfu_state_create_ex(const int* grid, int nfluid, double res, const double* st, int nst, const double* iw,
+                              int niw, int density, const double* x, const PFExtOpts* o) {
+  try {
+    PhaseFieldUnstructured pot;
+    pot.setNFluid(nfluid);
+    pot.setGridSize({grid[0], grid[1], grid[2]});
+    pot.setResolution(res);
+    pot.setSurfaceTension(vector<double>(st, st + nst));
+    if (niw == 1) pot.setInterfaceSize(iw[0]);
+    else if (niw > 1) pot.setInterfaceSize(vector<double>(iw, iw + niw));
+    pot.setDensityConstraint(density_name(density));
+    const size_t ng = (size_t)grid[0] * grid[1] * grid[2];
+    pot.densityConst = o->density_const;
+    if (o->solid) pot.setSolid(vector<char>(o->solid, o->solid + ng));
+    if (o->contact_angle) pot.setContactAngle(vector<double>(o->contact_angle, o->contact_angle + o->n_ca));
+    if (o->pressure) pot.setPressure(vector<double>(o->pressure, o->pressure + nfluid));
+    if (o->force3)
+      pot.setForce(vector<double>(o->force3, o->force3 + 3),
+                   vector<int>(o->force_fluids, o->force_fluids + o->n_force_fluids));
+    if (o->confinement) pot.setConfinement(vector<double>(o->confinement, o->confinement + nfluid));
+    if (o->volume) pot.setVolume(vector<double>(o->volume, o->volume + nfluid), o->vol_const);
+    else if (o->volume_fixed) pot.setVolumeFixed(true, o->vol_const);
+    if (o->fix_fluid)
+      for (int i = 0; i < nfluid; i++)
+        if (o->fix_fluid[i]) pot.setFixFluid(i);
+    size_t n = ng * nfluid;
+    return new State(pot, vector<double>(x, x + n));
+  } catch (...) {
+    return nullptr;
+  }
+}
+
+// Number of energy elements of a state's potential.
+long ref_state_nelements(void* h) { return (long)static_cast<State*>(h)->pot->elements.size(); }
 
 void ref_state_destroy(void* h) { delete static_cast<State*>(h); }
 long ref_state_ndof(void* h) { return (long)static_cast<State*>(h)->ndof; }

@@ -45,6 +45,18 @@ class OrcPFX(C.Structure):
                 ("total_volume2", C.c_double)]
 
 
+ORC_PFU_NIDOF = 16
+
+
+class OrcPFU(C.Structure):
+    _fields_ = [("grid", C.c_int * 3), ("nfluid", C.c_int), ("resolution", C.c_double), ("density", C.c_int),
+                ("density_const", C.c_double), ("kappa", C.c_double * ORC_MAXF), ("kappaP", C.c_double * ORC_MAXF),
+                ("st_mean", C.c_double), ("convergence", C.c_double), ("fix", C.c_ubyte * ORC_MAXF),
+                ("ngrid", C.c_size_t), ("node_vol", _dp), ("volume_fixed", C.c_int), ("vol_const", C.c_double),
+                ("volume", C.c_double * ORC_MAXF), ("nel", C.c_size_t), ("el_type", _ip), ("el_idof", _ip),
+                ("el_par", _dp)]
+
+
 class OrcStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("n_energy", C.c_long),
                 ("n_gradient", C.c_long), ("energy", C.c_double), ("grad_rms", C.c_double)]
@@ -90,6 +102,13 @@ def lib():
         L.orc_pfx_components.argtypes = [C.POINTER(OrcPFX), _dp, _dp]
         L.orc_pfx_potential.argtypes = [C.POINTER(OrcPFX), C.POINTER(OrcPot)]
         L.orc_pf_site_types.argtypes = [_ip, _ub, _ip]
+        L.orc_pfu_init.argtypes = [C.POINTER(OrcPFU), _ip, C.c_int, C.c_double, _dp, C.c_int, _dp, C.c_int,
+                                   C.c_int, C.c_double, _ub, _ub, _dp, C.c_int, _dp, _dp, _dp, C.c_int,
+                                   C.c_double, _dp, _dp]
+        L.orc_pfu_free.argtypes = [C.POINTER(OrcPFU)]
+        L.orc_pfu_energy_gradient.argtypes = [C.POINTER(OrcPFU), _dp, _dp, _dp]
+        L.orc_pfu_constraints.argtypes = [C.POINTER(OrcPFU), _dp]
+        L.orc_pfu_potential.argtypes = [C.POINTER(OrcPFU), C.POINTER(OrcPot)]
         L.orc_dot.argtypes = [_dp, _dp, C.c_size_t]
         L.orc_dot.restype = C.c_double
         L.orc_lbfgs_minimise.argtypes = [C.POINTER(OrcPot), _dp, C.c_int, C.c_int, C.c_double, C.c_double,
@@ -281,6 +300,92 @@ class OraclePFX:
         out = np.zeros(14)
         lib().orc_pfx_components(C.byref(self.p), _d(x), _d(out))
         return {n: [out[2 * i], out[2 * i + 1]] for i, n in enumerate(COMPONENTS)}
+
+
+class OraclePFU:
+    """CPU restatement of PhaseFieldUnstructured (src/potentials/PhaseFieldUnstructured.cpp)."""
+
+    def __init__(self, grid, x0, nfluid=1, resolution=1.0, surface_tension=(1.0,), interface_size=(),
+                 density="hard", density_const=1.0, fix_fluid=None, solid=None, contact_angle=None,
+                 pressure=None, force=None, force_fluids=(), confinement=None, volume_fixed=False,
+                 vol_const=0.01, volume=None):
+        L = lib()
+        self.p = OrcPFU()
+        self.nfluid = nfluid
+        self._keep = dict(st=np.ascontiguousarray(surface_tension, dtype=np.float64),
+                          iw=np.ascontiguousarray(interface_size if len(interface_size) else [0.0], dtype=np.float64),
+                          x0=np.ascontiguousarray(x0, dtype=np.float64),
+                          solid=_opt(solid, np.uint8), ca=_opt(contact_angle, np.float64),
+                          pres=_opt(pressure, np.float64),
+                          force=None if force is None else np.ascontiguousarray(
+                              expand_force(nfluid, force, force_fluids)).reshape(-1),
+                          conf=_opt(confinement, np.float64), vol=_opt(volume, np.float64))
+        k = self._keep
+        fix = None
+        if fix_fluid is not None:
+            fix = (C.c_ubyte * nfluid)(*[1 if f else 0 for f in fix_fluid])
+
+        def dp(a):
+            return None if a is None else _d(a)
+
+        ub = None if k["solid"] is None else k["solid"].ctypes.data_as(C.POINTER(C.c_ubyte))
+        rc = L.orc_pfu_init(C.byref(self.p), (C.c_int * 3)(*grid), nfluid, resolution, _d(k["st"]), len(k["st"]),
+                            _d(k["iw"]), len(interface_size), DENSITY[density], density_const, fix, ub,
+                            dp(k["ca"]), 0 if k["ca"] is None else k["ca"].size, dp(k["pres"]), dp(k["force"]),
+                            dp(k["conf"]), 1 if volume_fixed else 0, vol_const, dp(k["vol"]), _d(k["x0"]))
+        if rc != 0:
+            raise ValueError("PhaseFieldUnstructured: invalid parameters (code %d)" % rc)
+        self.ndof = int(np.prod(grid)) * nfluid
+        self.pot = OrcPot()
+        L.orc_pfu_potential(C.byref(self.p), C.byref(self.pot))
+
+    def __del__(self):
+        try:
+            lib().orc_pfu_free(C.byref(self.p))
+        except Exception:
+            pass
+
+    @property
+    def convergence(self):
+        return self.p.convergence
+
+    @property
+    def nelements(self):
+        return self.p.nel
+
+    @property
+    def kappa(self):
+        return np.array(self.p.kappa[: self.nfluid])
+
+    @property
+    def volume(self):
+        return np.array(self.p.volume[: self.nfluid])
+
+    def energy(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        e = C.c_double(0)
+        lib().orc_pfu_energy_gradient(C.byref(self.p), _d(x), C.byref(e), None)
+        return e.value
+
+    def energy_gradient(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        g = np.zeros_like(x)
+        e = C.c_double(0)
+        lib().orc_pfu_energy_gradient(C.byref(self.p), _d(x), C.byref(e), _d(g))
+        return e.value, g
+
+    def gradient(self, x, constrained=True):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        g = np.zeros_like(x)
+        lib().orc_pfu_energy_gradient(C.byref(self.p), _d(x), None, _d(g))
+        if constrained:
+            lib().orc_pfu_constraints(C.byref(self.p), _d(g))
+        return g
+
+    def constrain(self, data):
+        data = np.array(data, dtype=np.float64)
+        lib().orc_pfu_constraints(C.byref(self.p), _d(data))
+        return data
 
 
 def oracle_site_types(grid, solid):
@@ -509,6 +614,11 @@ def reflib():
         L.ref_state_constraints.argtypes = [C.c_void_p, _dp]
         L.ref_pf_diffuse_solid.argtypes = [_ip, C.c_int, C.c_double, _dp, C.c_int, _dp, C.c_int, C.c_int,
                                            C.POINTER(RefPFExtOpts), C.POINTER(C.c_ubyte), C.c_int, C.c_int, _dp]
+        L.ref_pfu_state_create_ex.restype = C.c_void_p
+        L.ref_pfu_state_create_ex.argtypes = [_ip, C.c_int, C.c_double, _dp, C.c_int, _dp, C.c_int, C.c_int, _dp,
+                                              C.POINTER(RefPFExtOpts)]
+        L.ref_state_nelements.argtypes = [C.c_void_p]
+        L.ref_state_nelements.restype = C.c_long
         L.ref_state_destroy.argtypes = [C.c_void_p]
         L.ref_state_ndof.argtypes = [C.c_void_p]
         L.ref_state_ndof.restype = C.c_long
@@ -553,6 +663,23 @@ class RefState:
         r = RefState(h)
         r.ngrid = int(np.prod(grid))
         return r
+
+    @staticmethod
+    def phasefield_unstructured(grid, x, nfluid=1, resolution=1.0, surface_tension=(1.0,), interface_size=(),
+                                density="hard", **opts):
+        keep = {}
+        o = _ref_opts(nfluid, keep, **opts)
+        st = np.ascontiguousarray(surface_tension, dtype=np.float64)
+        iw = np.ascontiguousarray(interface_size if len(interface_size) else [0.0], dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        h = reflib().ref_pfu_state_create_ex((C.c_int * 3)(*grid), nfluid, resolution, _d(st), len(st), _d(iw),
+                                             len(interface_size), DENSITY[density], _d(x), C.byref(o))
+        r = RefState(h)
+        r.ngrid = int(np.prod(grid))
+        return r
+
+    def nelements(self):
+        return reflib().ref_state_nelements(self.h)
 
     def components(self):
         out = np.zeros(14)
