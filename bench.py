@@ -208,6 +208,66 @@ def pf_general_leg(m, n=256, steps=20, warmup=3, ref_n=48):
     return out
 
 
+PFU_ELEMENT_BYTES = 1 + 7 * 4 + 3 * 8  # type, seven DOF indices, three FP64 parameters
+
+
+def pfu_leg(m, x0, steps=20, warmup=3, ref_n=48):
+    """The element form (PhaseFieldUnstructured) on the headline workload:
+    256^3 x 3 fluids, one seven-site element per site and fluid.  Whole
+    evaluations and the element kernel alone in FP32 and FP64 against the HBM
+    roofline, the energy beside the gather form's (the same sum arranged
+    differently), and the reference library's element form at ref_n^3 on the
+    host cores."""
+    import numpy as np
+    import torch
+
+    pf = m.PhaseFieldUnstructured().setNFluid(NF).setGridSize(list(GRID)).setSurfaceTension(SIGMA)
+    pf.setInterfaceSize(WIDTH)
+    t0 = time.perf_counter()
+    h = pf.handle(x0)
+    setup_s = time.perf_counter() - t0
+    nd = x0.size
+    nel = nd  # one gradient element per site and fluid; no other term is set
+    stream = torch.cuda.Stream()
+    out = dict(workload="PhaseFieldUnstructured %s x %d fluids (the headline input), %d elements of %d bytes"
+                        % ("x".join(map(str, GRID)), NF, nel, PFU_ELEMENT_BYTES), setup_s=setup_s)
+    gather = m.PhaseField().setNFluid(NF).setGridSize(list(GRID)).setSurfaceTension(SIGMA).setInterfaceSize(WIDTH)
+    hg = gather.handle(x0)
+    for prec, b in (("fp32", 4), ("fp64", 8)):
+        dt = torch.float32 if prec == "fp32" else torch.float64
+        x = torch.from_numpy(x0).to("cuda", dt)
+        g = torch.empty_like(x)
+        ms = time_device_evals(m, h, prec, x, g, nd, steps, warmup)
+        msk = m.potential_time_kernel(h, prec, x.data_ptr(), nd, g_ptr=g.data_ptr(), reps=steps,
+                                      stream=stream.cuda_stream)
+        e_el = m.potential_eval(h, prec, x.data_ptr(), nd)
+        e_ga = m.potential_eval(hg, prec, x.data_ptr(), nd)
+        alg = nel * PFU_ELEMENT_BYTES + nd * (b + b)  # element list, coordinates in, gradient out
+        out[prec] = dict(eval_ms=ms, evals_per_s=1e3 / ms, kernel_ms=msk, algorithmic_bytes=alg,
+                         achieved_GBs=alg / (msk * 1e-3) / 1e9, frac_of_hbm_peak=alg / (msk * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         energy=e_el, gather_form_energy=e_ga, energy_rel_diff=abs(e_el - e_ga) / abs(e_ga))
+        del x, g
+        torch.cuda.empty_cache()
+    del h, hg
+    from oracle_bind import RefState, ref_available
+
+    if ref_available():
+        threads = int(os.environ.get("OMP_NUM_THREADS") or min(os.cpu_count() or 1, 16))
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        rx = make_input((ref_n,) * 3)
+        r = RefState.phasefield_unstructured((ref_n,) * 3, rx, NF, 1.0, SIGMA, [WIDTH], "hard")
+        r.energy_gradient(rx)
+        t = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            r.energy_gradient(rx)
+        dt_ref = (time.perf_counter() - t) / reps
+        out["reference_cpu"] = dict(grid=[ref_n] * 3, eval_s=dt_ref, sites_per_s=ref_n ** 3 / dt_ref, cores=threads,
+                                    kind="reference")
+        out["gpu_sites_per_s_fp64"] = int(np.prod(GRID)) / (out["fp64"]["eval_ms"] * 1e-3)
+    return out
+
+
 def lj_leg(m, n_side=64, density=0.85, fire_max_iter=20000, ref_side=10):
     """Lennard-Jones at the benchmark size: n_side^3 particles at number density
     0.85 in a periodic cube, truncated-shifted at rc = 2.5 sigma with a 0.3 sigma
@@ -505,7 +565,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--fire-iters", type=int, default=20000, help="particles mode: FIRE iteration cap")
     ap.add_argument("--lbfgs-iters", type=int, default=2000, help="particles mode: sheet L-BFGS iteration cap")
-    ap.add_argument("--mode", choices=["full", "stencil", "converge", "particles", "batch", "general"],
+    ap.add_argument("--mode", choices=["full", "stencil", "converge", "particles", "batch", "general", "elements"],
                     default="full",
                     help="stencil: only the evaluation timings plus a 256 MiB copy (the calibration "
                          "kernel for rocprofv3 byte counters); converge: only time-to-converge")
@@ -546,6 +606,14 @@ def main():
         del x64, x32, g64, g32
         torch.cuda.empty_cache()
         res = pf_general_leg(m, steps=args.steps, warmup=args.warmup)
+        log(json.dumps(res, indent=1))
+        print(json.dumps(res), flush=True)
+        return
+
+    if args.mode == "elements":
+        del x64, x32, g64, g32
+        torch.cuda.empty_cache()
+        res = pfu_leg(m, x0, steps=args.steps, warmup=args.warmup)
         log(json.dumps(res, indent=1))
         print(json.dumps(res), flush=True)
         return
@@ -641,6 +709,7 @@ def main():
     # a failure here is recorded in the output and does not hide the headline line
     others = {}
     for name, fn in (("pf_general_256cubed_pillars", lambda: pf_general_leg(m, steps=args.steps, warmup=args.warmup)),
+                     ("pf_element_form_256cubed", lambda: pfu_leg(m, x0, steps=args.steps, warmup=args.warmup)),
                      ("lj_periodic_262144", lambda: lj_leg(m, fire_max_iter=args.fire_iters)),
                      ("bar_and_hinge_1024x1024", lambda: bah_leg(m, max_iter=args.lbfgs_iters)),
                      ("batch_512_instances", lambda: batch_leg(m, cpu_budget=0 if args.no_cpu else 10.0))):

@@ -86,8 +86,23 @@ int minim_pf_diffuse_solid(const minim_potential* pot, const unsigned char* soli
  * (-1 solid, 0 bulk, 1..7).  Host only. */
 int minim_pf_site_types(const int grid[3], const unsigned char* solid, int* type);
 
-/* kappa / kappaP per fluid after init (PhaseField::assignFluidCoefficients, PhaseField.cpp:78-134). */
+/* kappa / kappaP per fluid after init (PhaseField::assignFluidCoefficients, PhaseField.cpp:78-134;
+ * PhaseFieldUnstructured::assignFluidCoefficients, PhaseFieldUnstructured.cpp:71-107). */
 int minim_pf_coefficients(const minim_potential* pot, double* kappa, double* kappa_p);
+
+/* PhaseFieldUnstructured (include/potentials/PhaseFieldUnstructured.h:13-104):
+ * the element form of the same physics, built from the same parameters
+ * (PhaseFieldUnstructured.cpp:110-277, setters :727-856).  Differences from
+ * minim_pf_create: surface tensions are 1 or n_fluid values; the stencil_*
+ * fields are ignored; the fixed volume is an energy term, not a projection;
+ * the soft density constraint needs at least three fluids (its element reads
+ * three concentrations per site, :528-541).  minim_pf_coefficients,
+ * minim_pf_node_volumes (node volumes only) and minim_pf_diffuse_solid accept
+ * the handle; minim_pf_energy_components does not (the class has none). */
+int minim_pfu_create(const minim_pf_params* p, minim_potential** out);
+/* Number of energy elements init built (Potential::elements of the reference,
+ * include/Potential.h:90). */
+int minim_pfu_n_elements(const minim_potential* pot, size_t* n);
 
 /* LjNd(nDim, sigma, epsilon) (include/potentials/LjNd.h:12-44): all pairs, no cutoff. */
 int minim_lj_create(int ndim, double sigma, double epsilon, minim_potential** out);

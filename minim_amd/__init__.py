@@ -8,8 +8,8 @@ the tests and the benchmark.  The library is loaded on first use (so that
 missing library raises ImportError.
 """
 from . import _capi
-from .api import (BarAndHinge, Fire, GradDescent, Lbfgs, Lj2d, Lj3d, LjNd, Minimiser, PhaseField, Potential,
-                  State, dot, potential_constrain, potential_eval, potential_eval_async, potential_eval_mixed,
+from .api import (BarAndHinge, Fire, GradDescent, Lbfgs, Lj2d, Lj3d, LjNd, Minimiser, PhaseField,
+                  PhaseFieldUnstructured, Potential, State, dot, potential_constrain, potential_eval, potential_eval_async, potential_eval_mixed,
                   potential_time_kernel, site_types, triangulation_lists)
 
 
@@ -19,5 +19,5 @@ def __getattr__(name):
     raise AttributeError(name)
 
 __all__ = ["BarAndHinge", "Fire", "GradDescent", "Lbfgs", "Lj2d", "Lj3d", "LjNd", "Minimiser", "PhaseField",
-           "Potential", "State", "dot", "potential_constrain", "potential_eval", "potential_eval_async", "potential_eval_mixed",
-           "potential_time_kernel", "triangulation_lists", "lib"]
+           "PhaseFieldUnstructured", "Potential", "State", "dot", "potential_constrain", "potential_eval", "potential_eval_async", "potential_eval_mixed",
+           "potential_time_kernel", "site_types", "triangulation_lists", "lib"]

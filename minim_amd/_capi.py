@@ -59,6 +59,8 @@ SIGNATURES = {
     "minim_abi_version": (C.c_int, []),
     "minim_pf_create": (C.c_int, [C.POINTER(PFParams), C.POINTER(vp)]),
     "minim_pf_coefficients": (C.c_int, [vp, dp, dp]),
+    "minim_pfu_create": (C.c_int, [C.POINTER(PFParams), C.POINTER(vp)]),
+    "minim_pfu_n_elements": (C.c_int, [vp, C.POINTER(C.c_size_t)]),
     "minim_pf_energy_components": (C.c_int, [vp, dp, C.c_size_t, dp]),
     "minim_pf_node_volumes": (C.c_int, [vp, dp, dp]),
     "minim_pf_diffuse_solid": (C.c_int, [vp, C.POINTER(C.c_ubyte), C.c_int, C.c_int, dp]),
@@ -100,7 +102,7 @@ SIGNATURES = {
 _lib = None
 
 
-ABI_VERSION = 3  # layout of the parameter/result structs below
+ABI_VERSION = 4  # layout of the parameter/result structs below
 
 
 def load():

@@ -188,7 +188,8 @@ class PhaseField(Potential):
         self.fixFluid[iFluid] = bool(fix)
         return self
 
-    def _create(self):
+    def _params(self):
+        """The setters' values as minim_pf_params (arrays kept alive on self)."""
         if self.gridSize is None or len(self.gridSize) != 3:
             raise ValueError("PhaseField: gridSize must have three entries.")
         p = A.PFParams()
@@ -242,6 +243,10 @@ class PhaseField(Potential):
                 raise ValueError("PhaseField: Invalid size of volume array.")
             self._vol = _f64(self.volume)
             p.volume = _dptr(self._vol)
+        return p
+
+    def _create(self):
+        p = self._params()
         h = C.c_void_p()
         A.check(A.load().minim_pf_create(C.byref(p), C.byref(h)))
         return _Handle(h.value, A.load().minim_potential_destroy)
@@ -287,6 +292,46 @@ class PhaseField(Potential):
         conv = C.c_double()
         A.check(A.load().minim_potential_convergence(h.ptr, C.byref(conv)))
         return k, kp, conv.value
+
+
+class PhaseFieldUnstructured(PhaseField):
+    """include/potentials/PhaseFieldUnstructured.h:13-104: the element form of
+    the phase field (one element per site and fluid plus one-DOF pressure,
+    surface, force and confinement elements), with the same setters as
+    PhaseField except per-site surface tensions.  The fixed volume is an
+    energy term (PhaseFieldUnstructured.cpp:271-297)."""
+
+    potentialType = 1
+
+    def setSurfaceTension(self, s):
+        if callable(s):
+            raise ValueError("PhaseFieldUnstructured: per-site surface tensions are not supported")
+        return super().setSurfaceTension(s)
+
+    def _create(self):
+        p = self._params()
+        h = C.c_void_p()
+        A.check(A.load().minim_pfu_create(C.byref(p), C.byref(h)))
+        return _Handle(h.value, A.load().minim_potential_destroy)
+
+    def energyComponents(self, coords, init_coords=None):
+        raise NotImplementedError("PhaseFieldUnstructured has no energyComponents")
+
+    def nodeVolumes(self, coords=None):
+        """nodeVol per site after init (PhaseFieldUnstructured.cpp:252-268)."""
+        ng = int(np.prod(self.gridSize))
+        h = self.handle(np.zeros(ng * self.nFluid) if coords is None else coords)
+        nv = np.zeros(ng)
+        A.check(A.load().minim_pf_node_volumes(h.ptr, _dptr(nv), None))
+        return nv
+
+    def nElements(self, coords=None):
+        """Number of energy elements init builds."""
+        ng = int(np.prod(self.gridSize))
+        h = self.handle(np.zeros(ng * self.nFluid) if coords is None else coords)
+        n = C.c_size_t()
+        A.check(A.load().minim_pfu_n_elements(h.ptr, C.byref(n)))
+        return n.value
 
 
 def site_types(grid, solid):

@@ -14,6 +14,7 @@
 #include "minim/BarAndHinge.h"
 #include "minim/Minimiser.h"
 #include "minim/PhaseField.h"
+#include "minim/PhaseFieldUnstructured.h"
 #include "minim/State.h"
 #include "runtime.hpp"
 #include "vec.hpp"
@@ -65,12 +66,55 @@ void check_prec(int prec) {
   if (prec != MINIM_FP64 && prec != MINIM_FP32) throw std::invalid_argument("precision must be MINIM_FP64 or MINIM_FP32");
 }
 
+// The setters both phase-field classes share (PhaseField.cpp:747-900,
+// PhaseFieldUnstructured.cpp:727-856), called in the order a user would.
+template <typename PF>
+void apply_pf_params(const minim_pf_params* p, PF* pf) {
+  pf->setNFluid(p->n_fluid);
+  pf->setGridSize({p->grid[0], p->grid[1], p->grid[2]});
+  pf->setResolution(p->resolution);
+  if (p->n_surface_tension > 0) {
+    need(p->surface_tension, "surface_tension");
+    pf->setSurfaceTension(std::vector<double>(p->surface_tension, p->surface_tension + p->n_surface_tension));
+  }
+  if (p->n_interface_size == 1) {
+    need(p->interface_size, "interface_size");
+    pf->setInterfaceSize(p->interface_size[0]);
+  } else if (p->n_interface_size > 1) {
+    need(p->interface_size, "interface_size");
+    pf->setInterfaceSize(std::vector<double>(p->interface_size, p->interface_size + p->n_interface_size));
+  }
+  static const char* names[] = {"hard", "soft", "none", "fixed"};
+  if (p->density_constraint < 0 || p->density_constraint > 3)
+    throw std::invalid_argument("Invalid density constraint.");
+  pf->setDensityConstraint(names[p->density_constraint]);
+  pf->densityConst = p->density_const;
+  if (p->fix_fluid) {
+    for (int k = 0; k < p->n_fluid; k++) pf->setFixFluid(k, p->fix_fluid[k] != 0);
+  }
+  const long ng = (long)p->grid[0] * p->grid[1] * p->grid[2];
+  if (p->solid) pf->setSolid(std::vector<char>(p->solid, p->solid + ng));
+  if (p->contact_angle) pf->setContactAngle(std::vector<double>(p->contact_angle, p->contact_angle + p->n_contact_angle));
+  if (p->pressure) pf->setPressure(std::vector<double>(p->pressure, p->pressure + p->n_fluid));
+  if (p->force) {
+    std::vector<int> fl;
+    if (p->n_force_fluids > 0) {
+      need(p->force_fluids, "force_fluids");
+      fl.assign(p->force_fluids, p->force_fluids + p->n_force_fluids);
+    }
+    pf->setForce(std::vector<double>(p->force, p->force + 3), fl);
+  }
+  if (p->confinement) pf->setConfinement(std::vector<double>(p->confinement, p->confinement + p->n_fluid));
+  if (p->volume) pf->setVolume(std::vector<double>(p->volume, p->volume + p->n_fluid), p->vol_const);
+  else if (p->volume_fixed) pf->setVolumeFixed(true, p->vol_const);
+}
+
 }  // namespace
 
 extern "C" {
 
 const char* minim_last_error(void) { return g_err.c_str(); }
-int minim_abi_version(void) { return 3; }
+int minim_abi_version(void) { return 4; }
 
 // ---------------------------------------------------------------- potentials
 int minim_pf_create(const minim_pf_params* p, minim_potential** out) {
@@ -78,61 +122,57 @@ int minim_pf_create(const minim_pf_params* p, minim_potential** out) {
     need(p, "params");
     need(out, "out");
     auto pf = std::make_unique<minim::PhaseField>();
-    pf->setNFluid(p->n_fluid);
-    pf->setGridSize({p->grid[0], p->grid[1], p->grid[2]});
-    pf->setResolution(p->resolution);
-    if (p->n_surface_tension > 0) {
-      need(p->surface_tension, "surface_tension");
-      pf->setSurfaceTension(std::vector<double>(p->surface_tension, p->surface_tension + p->n_surface_tension));
-    }
-    if (p->n_interface_size == 1) {
-      need(p->interface_size, "interface_size");
-      pf->setInterfaceSize(p->interface_size[0]);
-    } else if (p->n_interface_size > 1) {
-      need(p->interface_size, "interface_size");
-      pf->setInterfaceSize(std::vector<double>(p->interface_size, p->interface_size + p->n_interface_size));
-    }
-    static const char* names[] = {"hard", "soft", "none", "fixed"};
-    if (p->density_constraint < 0 || p->density_constraint > 3)
-      throw std::invalid_argument("PhaseField: Invalid density constraint.");
-    pf->setDensityConstraint(names[p->density_constraint]);
-    pf->densityConst = p->density_const;
-    if (p->fix_fluid) {
-      for (int k = 0; k < p->n_fluid; k++) pf->setFixFluid(k, p->fix_fluid[k] != 0);
-    }
+    apply_pf_params(p, pf.get());
     pf->stencilXChunk = p->stencil_xchunk;
     if (p->stencil_target_blocks > 0) pf->stencilTargetBlocks = p->stencil_target_blocks;
     pf->stencilKernel = p->stencil_kernel;
-    const long ng = (long)p->grid[0] * p->grid[1] * p->grid[2];
-    if (p->solid) pf->setSolid(std::vector<char>(p->solid, p->solid + ng));
-    if (p->contact_angle) pf->setContactAngle(std::vector<double>(p->contact_angle, p->contact_angle + p->n_contact_angle));
-    if (p->pressure) pf->setPressure(std::vector<double>(p->pressure, p->pressure + p->n_fluid));
-    if (p->force) {
-      std::vector<int> fl;
-      if (p->n_force_fluids > 0) {
-        need(p->force_fluids, "force_fluids");
-        fl.assign(p->force_fluids, p->force_fluids + p->n_force_fluids);
-      }
-      pf->setForce(std::vector<double>(p->force, p->force + 3), fl);
-    }
-    if (p->confinement) pf->setConfinement(std::vector<double>(p->confinement, p->confinement + p->n_fluid));
-    if (p->volume) pf->setVolume(std::vector<double>(p->volume, p->volume + p->n_fluid), p->vol_const);
-    else if (p->volume_fixed) pf->setVolumeFixed(true, p->vol_const);
     auto* h = new minim_potential;
     h->p = std::move(pf);
     *out = h;
   });
 }
 
+int minim_pfu_create(const minim_pf_params* p, minim_potential** out) {
+  return guard([&] {
+    need(p, "params");
+    need(out, "out");
+    auto pf = std::make_unique<minim::PhaseFieldUnstructured>();
+    apply_pf_params(p, pf.get());
+    auto* h = new minim_potential;
+    h->p = std::move(pf);
+    *out = h;
+  });
+}
+
+int minim_pfu_n_elements(const minim_potential* pot, size_t* n) {
+  return guard([&] {
+    need(pot, "pot");
+    need(n, "n");
+    auto* pf = dynamic_cast<const minim::PhaseFieldUnstructured*>(pot->p.get());
+    if (!pf) throw std::invalid_argument("not a PhaseFieldUnstructured potential");
+    if (!pot->initialised) throw std::runtime_error("PhaseFieldUnstructured: call minim_potential_init first");
+    *n = pf->nElements;
+  });
+}
+
 int minim_pf_coefficients(const minim_potential* pot, double* kappa, double* kappa_p) {
   return guard([&] {
     need(pot, "pot");
-    auto* pf = dynamic_cast<const minim::PhaseField*>(pot->p.get());
-    if (!pf) throw std::invalid_argument("not a PhaseField potential");
+    const std::vector<double>* k = nullptr;
+    const std::vector<double>* kp = nullptr;
+    if (auto* pf = dynamic_cast<const minim::PhaseField*>(pot->p.get())) {
+      k = &pf->kappa;
+      kp = &pf->kappaP;
+    } else if (auto* pu = dynamic_cast<const minim::PhaseFieldUnstructured*>(pot->p.get())) {
+      k = &pu->kappa;
+      kp = &pu->kappaP;
+    } else {
+      throw std::invalid_argument("not a PhaseField potential");
+    }
     if (!pot->initialised) throw std::runtime_error("PhaseField: call minim_potential_init first");
-    for (int k = 0; k < pf->nFluid; k++) {
-      if (kappa) kappa[k] = pf->kappa[k];
-      if (kappa_p) kappa_p[k] = pf->kappaP[k];
+    for (size_t i = 0; i < k->size(); i++) {
+      if (kappa) kappa[i] = (*k)[i];
+      if (kappa_p) kappa_p[i] = (*kp)[i];
     }
   });
 }
@@ -164,6 +204,16 @@ int minim_pf_energy_components(const minim_potential* pot, const double* coords,
 
 int minim_pf_node_volumes(const minim_potential* pot, double* node_vol, double* surface_area) {
   return guard([&] {
+    need(pot, "pot");
+    if (auto* pu = dynamic_cast<const minim::PhaseFieldUnstructured*>(pot->p.get())) {
+      // the element form keeps node volumes per DOF and no surface areas
+      // (PhaseFieldUnstructured.cpp:252-268)
+      if (!pot->initialised) throw std::runtime_error("PhaseField: call minim_potential_init first");
+      if (surface_area) throw std::invalid_argument("PhaseFieldUnstructured keeps no surface areas");
+      for (size_t i = 0; i < (size_t)pu->nGrid; i++)
+        if (node_vol) node_vol[i] = pu->nodeVol[i * pu->nFluid];
+      return;
+    }
     const minim::PhaseField& pf = pf_of(pot, true);
     const size_t ng = (size_t)pf.nGrid;
     for (size_t i = 0; i < ng; i++) {
@@ -176,13 +226,21 @@ int minim_pf_node_volumes(const minim_potential* pot, double* node_vol, double* 
 int minim_pf_diffuse_solid(const minim_potential* pot, const unsigned char* solid, int i_fluid, int two_step,
                            double* out) {
   return guard([&] {
-    const minim::PhaseField& pf = pf_of(pot, false);
+    need(pot, "pot");
     need(solid, "solid");
     need(out, "out");
-    if (pf.gridSize.size() != 3) throw std::invalid_argument("PhaseField: gridSize must have three entries.");
-    const size_t ng = (size_t)pf.gridSize[0] * pf.gridSize[1] * pf.gridSize[2];
-    std::vector<double> r =
-        minim::PhaseField::diffuseSolid(std::vector<char>(solid, solid + ng), pf, i_fluid, two_step != 0);
+    std::vector<double> r;
+    if (auto* pu = dynamic_cast<const minim::PhaseFieldUnstructured*>(pot->p.get())) {
+      if (pu->gridSize.size() != 3) throw std::invalid_argument("PhaseField: gridSize must have three entries.");
+      const size_t ng = (size_t)pu->gridSize[0] * pu->gridSize[1] * pu->gridSize[2];
+      r = minim::PhaseFieldUnstructured::diffuseSolid(std::vector<char>(solid, solid + ng), *pu, i_fluid,
+                                                       two_step != 0);
+    } else {
+      const minim::PhaseField& pf = pf_of(pot, false);
+      if (pf.gridSize.size() != 3) throw std::invalid_argument("PhaseField: gridSize must have three entries.");
+      const size_t ng = (size_t)pf.gridSize[0] * pf.gridSize[1] * pf.gridSize[2];
+      r = minim::PhaseField::diffuseSolid(std::vector<char>(solid, solid + ng), pf, i_fluid, two_step != 0);
+    }
     std::memcpy(out, r.data(), sizeof(double) * r.size());
   });
 }

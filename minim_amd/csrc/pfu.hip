@@ -1,0 +1,366 @@
+// pfu.hip -- the element form of the phase field (PhaseFieldUnstructured,
+// src/potentials/PhaseFieldUnstructured.cpp) on MI355X (gfx950).
+//
+// The reference evaluates a list of elements one after another
+// (State::elementEG, src/State.cpp:58-79): a seven-site gradient element per
+// site and fluid (or per site for the "fixed" density mode, whose last fluid
+// is implicit), one-DOF pressure, surface, force and confinement elements and
+// a soft-density element per site, then the volume term over the whole field
+// (blockEnergyGradient, :271-297).  Here one thread evaluates one element of
+// the list the host built (in the reference's order and with its parameters),
+// accumulates its energy in FP64 for a deterministic two-stage reduction, and
+// adds its gradient contributions with hardware atomics (global_atomic_add):
+// the element form is kept as it is, including the reference's choices in
+// each element, so that it serves as the independent cross-check of the
+// gather-form kernels it is in the reference.
+#include "common.hpp"
+#include "pfu.hpp"
+
+namespace minim_amd {
+
+namespace {
+
+template <typename T, typename TV, bool DIR>
+struct Coords {
+  const T* __restrict__ x;
+  const TV* __restrict__ d;
+  T t;
+  __device__ __forceinline__ T operator()(int i) const {
+    T c = x[i];
+    if constexpr (DIR) {
+      const T s = t * (T)d[i];
+      c = c + s;
+    }
+    return c;
+  }
+};
+
+// The gradient term of one seven-site element (phaseGradient,
+// PhaseFieldUnstructured.cpp:311-515): node 0 is the site, nodes (1,6), (2,5),
+// (3,4) its x, y, z neighbours; self[i] marks a neighbour that is the site
+// itself (the reference's stand-in for a solid neighbour).  Contributions to
+// the nodes' gradients go to gl.
+template <typename T, bool WE, bool WG>
+__device__ __forceinline__ void phase7(const T (&c)[kPFUNodes], const bool (&self)[kPFUNodes], T factor, T& e,
+                                       T (&gl)[kPFUNodes]) {
+  const int mp[3][2] = {{1, 6}, {2, 5}, {3, 4}};
+  T grad2 = (T)0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const int m = mp[d][0], p = mp[d][1];
+    if (!self[m] && !self[p]) {
+      const T gm = c[0] - c[m], gp = c[0] - c[p];
+      if (WE) grad2 = grad2 + (T)0.5 * (gm * gm + gp * gp);
+      if (WG) {
+        gl[0] = gl[0] + factor * (gm + gp);
+        gl[m] = gl[m] - factor * gm;
+        gl[p] = gl[p] - factor * gp;
+      }
+    } else if (!self[p]) {
+      const T gp = c[0] - c[p];
+      if (WE) grad2 = grad2 + gp * gp;
+      if (WG) {
+        gl[0] = gl[0] + (factor * (T)2) * gp;
+        gl[p] = gl[p] - (factor * (T)2) * gp;
+      }
+    } else if (!self[m]) {
+      const T gm = c[0] - c[m];
+      if (WE) grad2 = grad2 + gm * gm;
+      if (WG) {
+        gl[0] = gl[0] + (factor * (T)2) * gm;
+        gl[m] = gl[m] - (factor * (T)2) * gm;
+      }
+    }
+  }
+  if (WE) e = e + factor * grad2;
+}
+
+template <typename TV, typename T>
+__device__ __forceinline__ void gadd(TV* g, int i, T v) {
+  unsafeAtomicAdd(g + i, (TV)v);
+}
+
+// MODE bits: 1 = energy, 2 = gradient, 4 = at x + t*d.
+template <typename T, typename TV, int MODE>
+__global__ __launch_bounds__(kBlock) void pfu_element_kernel(PFUArgs u, const T* __restrict__ x,
+                                                             const TV* __restrict__ d, T t, TV* __restrict__ g,
+                                                             RedScratch rs) {
+  constexpr bool WE = MODE & 1, WG = MODE & 2, DIR = MODE & 4;
+  const Coords<T, TV, DIR> cx{x, d, t};
+  const int nf = u.nf;
+  double acc[2] = {0.0, 0.0};
+  for (size_t ie = (size_t)blockIdx.x * kBlock + threadIdx.x; ie < u.nel; ie += (size_t)gridDim.x * kBlock) {
+    const int type = u.type[ie];
+    const int32_t* id = u.idof + ie * kPFUNodes;
+    const double* par = u.par + ie * 3;
+    T e = (T)0;
+    switch (type) {
+      case kPFUFluid: {  // fluidEnergy (:518-538)
+        const T vol = (T)par[0];
+        const int f = (int)par[1];
+        int n[kPFUNodes];
+        T c[kPFUNodes], gl[kPFUNodes];
+        bool self[kPFUNodes];
+#pragma unroll
+        for (int i = 0; i < kPFUNodes; i++) {
+          n[i] = id[i];
+          c[i] = cx(n[i]);
+          gl[i] = (T)0;
+          self[i] = (i > 0) && n[i] == n[0];
+        }
+        const T c0 = c[0];
+        T factor;
+        if (nf == 1) {
+          factor = ((T)u.kappa[0] / (T)16) * vol;
+          if (WE) e = e + (factor * ((c0 + (T)1) * (c0 + (T)1))) * ((c0 - (T)1) * (c0 - (T)1));
+          if (WG) gl[0] = gl[0] + ((factor * (T)4) * c0) * (c0 * c0 - (T)1);
+        } else {
+          factor = ((T)0.5 * (T)u.kappa[f]) * vol;
+          if (WE) e = e + (factor * (c0 * c0)) * ((c0 - (T)1) * (c0 - (T)1));
+          if (WG) gl[0] = gl[0] + (((factor * (T)2) * c0) * (c0 - (T)1)) * ((T)2 * c0 - (T)1);
+        }
+        T gfac = (nf == 1) ? ((T)0.25 * (T)u.kappaP[0]) * vol : ((T)0.5 * (T)u.kappaP[f]) * vol;
+        gfac = gfac / (T)u.res2;
+        phase7<T, WE, WG>(c, self, gfac, e, gl);
+        if (WG) {
+#pragma unroll
+          for (int i = 0; i < kPFUNodes; i++)
+            if (gl[i] != (T)0) gadd(g, n[i], gl[i]);
+        }
+        break;
+      }
+      case kPFUFluidAll: {  // fluidEnergyAll (:541-586): the last fluid is 1 - the others
+        const T vol = (T)par[0];
+        int n[kPFUNodes];
+        T cN[kPFUNodes], gN[kPFUNodes];
+        bool none[kPFUNodes];
+#pragma unroll
+        for (int i = 0; i < kPFUNodes; i++) {
+          n[i] = id[i];
+          cN[i] = (T)1;
+          gN[i] = (T)0;
+          none[i] = false;  // the reference passes distinct indices here: no node counts as solid
+        }
+        for (int f = 0; f < nf - 1; f++)
+#pragma unroll
+          for (int i = 0; i < kPFUNodes; i++) cN[i] = cN[i] - cx(n[i] + f);
+        const T bfN = ((T)0.5 * (T)u.kappa[nf - 1]) * vol;
+        const T gfN = ((T)0.5 * (T)u.kappaP[nf - 1]) * vol;  // not divided by res^2 (:555)
+        if (WE) e = e + (bfN * (cN[0] * cN[0])) * ((cN[0] - (T)1) * (cN[0] - (T)1));
+        if (WG) gN[0] = gN[0] + (((bfN * (T)2) * cN[0]) * (cN[0] - (T)1)) * ((T)2 * cN[0] - (T)1);
+        phase7<T, WE, WG>(cN, none, gfN, e, gN);
+        for (int f = 0; f < nf - 1; f++) {
+          T c[kPFUNodes], gl[kPFUNodes];
+          bool self[kPFUNodes];
+#pragma unroll
+          for (int i = 0; i < kPFUNodes; i++) {
+            c[i] = cx(n[i] + f);
+            gl[i] = (T)0;
+            self[i] = (i > 0) && n[i] == n[0];
+          }
+          const T c0 = c[0];
+          const T bf = ((T)0.5 * (T)u.kappa[f]) * vol;
+          if (WE) e = e + (bf * (c0 * c0)) * ((c0 - (T)1) * (c0 - (T)1));
+          if (WG) gl[0] = gl[0] + (((bf * (T)2) * c0) * (c0 - (T)1)) * ((T)2 * c0 - (T)1);
+          T gf = ((T)0.5 * (T)u.kappaP[f]) * vol;
+          gf = gf / (T)u.res2;
+          phase7<T, WE, WG>(c, self, gf, e, gl);
+          if (WG) {
+#pragma unroll
+            for (int i = 0; i < kPFUNodes; i++) {
+              const T v = gl[i] - gN[i];
+              if (v != (T)0) gadd(g, n[i] + f, v);
+            }
+          }
+        }
+        break;
+      }
+      case kPFUPressure: {  // pressureEnergy (:613-628)
+        const int i = id[0];
+        const T pr = (T)u.pressure[i % nf];
+        const T v = (T)u.node_vol[i / nf];
+        const T c = cx(i);
+        if (nf == 1) {
+          if (WE) e = e - pr * (((T)0.5 * (c + (T)1)) * v);
+          if (WG) gadd(g, i, -(((T)0.5 * pr) * v));
+        } else {
+          if (WE) e = e - pr * (c * v);
+          if (WG) gadd(g, i, -(pr * v));
+        }
+        break;
+      }
+      case kPFUDensity: {  // densityConstraintEnergy (:631-642): three DOFs of the site
+        if (nf == 1) break;
+        const T coef = (T)u.coef_density;
+        const T rho = ((cx(id[0]) + cx(id[1])) + cx(id[2])) - (T)1;
+        if (WE) e = e + coef * (rho * rho);
+        if (WG) {
+          const T gr = ((T)2 * coef) * rho;
+          gadd(g, id[0], gr);
+          gadd(g, id[1], gr);
+          gadd(g, id[2], gr);
+        }
+        break;
+      }
+      case kPFUSurface: {  // surfaceEnergy (:645-654): binary fluids only
+        if (nf != 1) break;
+        const T area = (T)par[0], w = (T)par[1];
+        const T phi = cx(id[0]);
+        if (WE) e = e + (w * ((((phi * phi) * phi) / (T)3 - phi) - (T)2 / (T)3)) * area;
+        if (WG) gadd(g, id[0], (w * (phi * phi - (T)1)) * area);
+        break;
+      }
+      case kPFUForce: {  // forceEnergy (:657-676)
+        const T c = (nf == 1) ? (T)0.5 * ((T)1 + cx(id[0])) : cx(id[0]);
+        if (c < (T)0.01) break;
+        const T vol = (T)par[0], f = (T)par[1], h = (T)par[2];
+        if (WE) e = e + ((c * vol) * f) * h;
+        if (WG) gadd(g, id[0], (nf == 1) ? ((((T)0.5 * vol) * f) * h) : ((vol * f) * h));
+        break;
+      }
+      case kPFUConfinement: {  // ffConfinementEnergy (:679-692)
+        const T c = cx(id[0]);
+        const T coef = ((T)par[0] * (T)u.st_mean) * (T)u.res2;
+        const T c0 = (T)par[1];
+        if ((c0 - (T)0.5) * (c - (T)0.5) < (T)0) {
+          if (WE) e = e + coef * ((c - (T)0.5) * (c - (T)0.5));
+          if (WG) gadd(g, id[0], (coef * (T)2) * (c - (T)0.5));
+        }
+        break;
+      }
+      default:
+        break;
+    }
+    if (WE) acc[0] += (double)e;
+  }
+  grid_sum_store<2>(acc, rs);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void pfu_zero_kernel(T* __restrict__ v, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) v[i] = (T)0;
+}
+
+// Fluid volumes for the volume term (:284-293).
+template <typename T, typename TV, bool DIR>
+__global__ __launch_bounds__(kBlock) void pfu_volumes_kernel(PFUArgs u, const T* __restrict__ x,
+                                                             const TV* __restrict__ d, T t, RedScratch rs) {
+  const Coords<T, TV, DIR> cx{x, d, t};
+  const int nf = u.nf;
+  double acc[kPFMaxFluid];
+#pragma unroll
+  for (int k = 0; k < kPFMaxFluid; k++) acc[k] = 0.0;
+  for (size_t s = (size_t)blockIdx.x * kBlock + threadIdx.x; s < u.ngrid; s += (size_t)gridDim.x * kBlock) {
+    const double v = u.node_vol[s];
+#pragma unroll
+    for (int k = 0; k < kPFMaxFluid; k++) {
+      if (k >= nf) continue;
+      const double c = (double)cx((int)(s * nf + k));
+      acc[k] += (nf == 1) ? (0.5 * (c + 1)) * v : c * v;
+    }
+  }
+  grid_sum_store<kPFMaxFluid>(acc, rs);
+}
+
+// volCoef * volDiff per fluid, and (with the energy) the term's energy added to
+// the elements' (:295-300).  One thread.
+__global__ void pfu_volfix_kernel(PFUArgs u, const double* __restrict__ vol, double* __restrict__ energy,
+                                  int want_energy) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double e = want_energy ? energy[0] : 0.0;
+  for (int k = 0; k < u.nf; k++) {
+    const double diff = vol[k] - u.volume[k];
+    e += u.vol_coef * (diff * diff) / 1;
+    u.volg[k] = u.vol_coef * diff;
+  }
+  if (want_energy) energy[0] = e;
+}
+
+// The volume term's gradient, weighted towards interfaces (:301-306).
+template <typename T, typename TV, bool DIR>
+__global__ __launch_bounds__(kBlock) void pfu_volgrad_kernel(PFUArgs u, const T* __restrict__ x,
+                                                             const TV* __restrict__ d, T t, TV* __restrict__ g) {
+  const Coords<T, TV, DIR> cx{x, d, t};
+  const size_t nd = u.ngrid * u.nf;
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < nd; i += (size_t)gridDim.x * kBlock) {
+    const int k = (int)(i % u.nf);
+    const T c = cx((int)i);
+    T w = ((T)4 * c) * ((T)1 - c);
+    if (w < (T)0) w = (T)0;
+    const T add = (((T)u.volg[k] * (T)u.node_vol[i / u.nf]) * w);
+    g[i] = (TV)((T)g[i] + add);
+  }
+}
+
+template <typename T>
+void eval_t(const PFUArgs& u, int mode, const void* x, const void* d, double t, void* g, RedScratch rs,
+            hipStream_t s) {
+  const T* xx = static_cast<const T*>(x);
+  const T* dd = static_cast<const T*>((mode & 4) ? d : x);
+  T* gg = static_cast<T*>(g);
+  const T tt = (T)t;
+  const size_t nd = u.ngrid * u.nf;
+  if (mode & 2) pfu_zero_kernel<T><<<grid_blocks(nd, 4, 4096), kBlock, 0, s>>>(gg, nd);
+  const int nb = grid_blocks(u.nel, 1, 4096);
+  switch (mode) {
+    case 1: pfu_element_kernel<T, T, 1><<<nb, kBlock, 0, s>>>(u, xx, dd, tt, gg, rs); break;
+    case 2: pfu_element_kernel<T, T, 2><<<nb, kBlock, 0, s>>>(u, xx, dd, tt, gg, rs); break;
+    case 3: pfu_element_kernel<T, T, 3><<<nb, kBlock, 0, s>>>(u, xx, dd, tt, gg, rs); break;
+    case 5: pfu_element_kernel<T, T, 5><<<nb, kBlock, 0, s>>>(u, xx, dd, tt, gg, rs); break;
+    default: throw std::invalid_argument("pfu: unsupported evaluation mode");
+  }
+  launch_finalize(rs, 2, nb, s);
+  if (u.volfix) {
+    RedScratch slot{rs.partials, rs.result + kPFVolSlot};
+    const int nbv = grid_blocks(u.ngrid, 4, 2048);
+    if (mode & 4)
+      pfu_volumes_kernel<T, T, true><<<nbv, kBlock, 0, s>>>(u, xx, dd, tt, slot);
+    else
+      pfu_volumes_kernel<T, T, false><<<nbv, kBlock, 0, s>>>(u, xx, dd, tt, slot);
+    launch_finalize(slot, kPFMaxFluid, nbv, s);
+    pfu_volfix_kernel<<<1, 64, 0, s>>>(u, rs.result + kPFVolSlot, rs.result, (mode & 1) ? 1 : 0);
+    if (mode & 2) pfu_volgrad_kernel<T, T, false><<<grid_blocks(nd, 4, 4096), kBlock, 0, s>>>(u, xx, dd, tt, gg);
+  }
+}
+
+}  // namespace
+
+void pfu_launch_eval(const PFUArgs& u, int prec, int mode, const void* x, const void* d, double t, void* g,
+                     RedScratch rs, hipStream_t s) {
+  if ((mode & 4) && (mode & 2)) throw std::invalid_argument("pfu: gradients at a trial point are not supported");
+  if (u.ngrid * (size_t)u.nf >= (size_t)1 << 31) throw std::invalid_argument("pfu: more than 2^31 DOFs");
+  if (prec == F64) eval_t<double>(u, mode, x, d, t, g, rs, s);
+  else eval_t<float>(u, mode, x, d, t, g, rs, s);
+  MINIM_HIP_CHECK(hipGetLastError());
+}
+
+double pfu_time_elements(const PFUArgs& u, int prec, int mode, const void* x, void* g, int reps, RedScratch rs,
+                         hipStream_t s) {
+  const int nb = grid_blocks(u.nel, 1, 4096);
+  auto launch = [&]() {
+    if (prec == F64) {
+      const double* xx = static_cast<const double*>(x);
+      if (mode == 1) pfu_element_kernel<double, double, 1><<<nb, kBlock, 0, s>>>(u, xx, xx, 0.0, (double*)g, rs);
+      else pfu_element_kernel<double, double, 3><<<nb, kBlock, 0, s>>>(u, xx, xx, 0.0, (double*)g, rs);
+    } else {
+      const float* xx = static_cast<const float*>(x);
+      if (mode == 1) pfu_element_kernel<float, float, 1><<<nb, kBlock, 0, s>>>(u, xx, xx, 0.0f, (float*)g, rs);
+      else pfu_element_kernel<float, float, 3><<<nb, kBlock, 0, s>>>(u, xx, xx, 0.0f, (float*)g, rs);
+    }
+  };
+  launch();
+  hipEvent_t e0, e1;
+  MINIM_HIP_CHECK(hipEventCreate(&e0));
+  MINIM_HIP_CHECK(hipEventCreate(&e1));
+  MINIM_HIP_CHECK(hipEventRecord(e0, s));
+  for (int i = 0; i < reps; i++) launch();
+  MINIM_HIP_CHECK(hipEventRecord(e1, s));
+  MINIM_HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0;
+  MINIM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  MINIM_HIP_CHECK(hipEventDestroy(e0));
+  MINIM_HIP_CHECK(hipEventDestroy(e1));
+  return (double)ms / reps;
+}
+
+}  // namespace minim_amd

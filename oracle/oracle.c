@@ -765,8 +765,8 @@ static void pfu_push(orc_pfu* p, size_t* cap, int type, const int* idof, int n, 
   p->nel++;
 }
 
-/* PhaseFieldUnstructured::assignFluidCoefficients (PhaseFieldUnstructured.cpp:72-107),
- * init (:110-249), initLocal on one rank (:252-268), setDefaults, checkArraySizes. */
+/* PhaseFieldUnstructured::assignFluidCoefficients (PhaseFieldUnstructured.cpp:71-107),
+ * init (:110-257), initLocal on one rank (:260-277), setDefaults, checkArraySizes. */
 int orc_pfu_init(orc_pfu* p, const int grid[3], int nfluid, double resolution, const double* surface_tension,
                  int n_st, const double* interface_size, int n_iw, int density, double density_const,
                  const unsigned char* fix_fluid, const unsigned char* solid, const double* contact_angle, int n_ca,
@@ -786,7 +786,7 @@ int orc_pfu_init(orc_pfu* p, const int grid[3], int nfluid, double resolution, c
   p->density = density;
   p->density_const = density_const;
   /* the soft constraint's element reads three DOFs of each site
-     (PhaseFieldUnstructured.cpp:628): with two fluids that reads past the element */
+     (PhaseFieldUnstructured.cpp:635): with two fluids that reads past the element */
   if (nF == 2 && density == ORC_DENSITY_SOFT) return 6;
   double iw[ORC_MAXF], st[ORC_MAXF];
   if (n_iw == 0) {
@@ -878,7 +878,7 @@ int orc_pfu_init(orc_pfu* p, const int grid[3], int nfluid, double resolution, c
       for (int i = 0; i < nF; i++) p->volume[i] += x0[i + nF * ig] * p->node_vol[ig];
   }
 
-  /* elements, site by site (:153-228) */
+  /* elements, site by site (:157-240) */
   size_t cap = 0;
   const int n0 = grid[0], n1 = grid[1], n2 = grid[2];
   static const int off[6][3] = {{-1, 0, 0}, {0, -1, 0}, {0, 0, -1}, {0, 0, 1}, {0, 1, 0}, {1, 0, 0}};
@@ -949,7 +949,7 @@ int orc_pfu_init(orc_pfu* p, const int grid[3], int nfluid, double resolution, c
   return 0;
 }
 
-/* phaseGradient of the element form (PhaseFieldUnstructured.cpp:311-405): a
+/* phaseGradient of the element form (PhaseFieldUnstructured.cpp:311-515): a
  * neighbour replaced by the site itself marks a solid one. */
 static void pfu_phase_gradient(const double* x, const int* idof, double factor, double* e, double* g) {
   static const int mp[3][2] = {{1, 6}, {2, 5}, {3, 4}};
@@ -984,14 +984,14 @@ static void pfu_phase_gradient(const double* x, const int* idof, double factor, 
   if (e) *e += factor * grad2;
 }
 
-/* elementEnergyGradient (:563-590) of one element */
+/* elementEnergyGradient (:695-724) of one element */
 static void pfu_element(const orc_pfu* p, const double* x, size_t ie, double* e, double* g) {
   const int nF = p->nfluid;
   const int* idof = p->el_idof + ie * ORC_PFU_NIDOF;
   const double* par = p->el_par + ie * 3;
   const double res = p->resolution;
   switch (p->el_type[ie]) {
-    case PFU_FLUID: { /* fluidEnergy (:409-428) */
+    case PFU_FLUID: { /* fluidEnergy (:518-538) */
       const double vol = par[0];
       const int f = (int)par[1];
       const double c = x[idof[0]];
@@ -1009,7 +1009,7 @@ static void pfu_element(const orc_pfu* p, const double* x, size_t ie, double* e,
       pfu_phase_gradient(x, idof, factor, e, g);
       break;
     }
-    case PFU_FLUID_ALL: { /* fluidEnergyAll (:431-478) */
+    case PFU_FLUID_ALL: { /* fluidEnergyAll (:541-586) */
       const double vol = par[0];
       double cN[7], gN[7] = {0};
       static const int iota[7] = {0, 1, 2, 3, 4, 5, 6};
@@ -1017,7 +1017,7 @@ static void pfu_element(const orc_pfu* p, const double* x, size_t ie, double* e,
       for (int f = 0; f < nF - 1; f++)
         for (int i = 0; i < 7; i++) cN[i] -= x[idof[i] + f];
       const double bfN = 0.5 * p->kappa[nF - 1] * vol;
-      const double gfN = 0.5 * p->kappaP[nF - 1] * vol; /* not divided by res^2 (:444) */
+      const double gfN = 0.5 * p->kappaP[nF - 1] * vol; /* not divided by res^2 (:555) */
       if (e) *e += bfN * pow(cN[0], 2) * pow(cN[0] - 1, 2);
       if (g) {
         gN[0] += bfN * 2 * cN[0] * (cN[0] - 1) * (2 * cN[0] - 1);
@@ -1040,7 +1040,7 @@ static void pfu_element(const orc_pfu* p, const double* x, size_t ie, double* e,
       }
       break;
     }
-    case PFU_PRESSURE: { /* pressureEnergy (:508-525) */
+    case PFU_PRESSURE: { /* pressureEnergy (:613-628) */
       const int iDof = idof[0];
       const double pr = par[0];
       const double v = p->node_vol[iDof / nF];
@@ -1055,7 +1055,7 @@ static void pfu_element(const orc_pfu* p, const double* x, size_t ie, double* e,
       }
       break;
     }
-    case PFU_DENSITY: { /* densityConstraintEnergy (:528-541): three DOFs */
+    case PFU_DENSITY: { /* densityConstraintEnergy (:631-642): three DOFs */
       if (nF == 1) break;
       const double coef = p->density_const * p->st_mean * pow(res, 2);
       const double rho = x[idof[0]] + x[idof[1]] + x[idof[2]] - 1;
@@ -1067,14 +1067,14 @@ static void pfu_element(const orc_pfu* p, const double* x, size_t ie, double* e,
       }
       break;
     }
-    case PFU_SURFACE: /* surfaceEnergy (:544-553) */
+    case PFU_SURFACE: /* surfaceEnergy (:645-654) */
       if (nF == 1) {
         const double phi = x[idof[0]];
         if (e) *e += par[1] * (pow(phi, 3) / 3 - phi - 2.0 / 3) * par[0];
         if (g) g[idof[0]] += par[1] * (pow(phi, 2) - 1) * par[0];
       }
       break;
-    case PFU_FORCE: { /* forceEnergy (:556-575): every fluid, also without a gradient */
+    case PFU_FORCE: { /* forceEnergy (:657-676): every fluid, also without a gradient */
       const double c = (nF == 1) ? 0.5 * (1 + x[idof[0]]) : x[idof[0]];
       if (c < 0.01) break;
       const double vol = par[0], f = par[1], h = par[2];
@@ -1085,7 +1085,7 @@ static void pfu_element(const orc_pfu* p, const double* x, size_t ie, double* e,
       }
       break;
     }
-    case PFU_CONF: { /* ffConfinementEnergy (:578-592) */
+    case PFU_CONF: { /* ffConfinementEnergy (:679-692) */
       const double c = x[idof[0]];
       const double coef = par[0] * p->st_mean * pow(res, 2);
       const double c0 = par[1];
@@ -1099,7 +1099,7 @@ static void pfu_element(const orc_pfu* p, const double* x, size_t ie, double* e,
 }
 
 /* State's elementEG (src/State.cpp:58-79) without the constraints: e set, g
- * zeroed, every element in order, then blockEnergyGradient (:271-297). */
+ * zeroed, every element in order, then blockEnergyGradient (:280-308). */
 void orc_pfu_energy_gradient(const orc_pfu* p, const double* x, double* e, double* g) {
   const int nF = p->nfluid;
   const size_t nd = p->ngrid * nF;
@@ -1128,7 +1128,7 @@ void orc_pfu_energy_gradient(const orc_pfu* p, const double* x, double* e, doubl
 }
 
 /* Potential::applyConstraints (src/Potential.cpp:102-115) with the constraints
- * PhaseFieldUnstructured::init sets (:231-247): per site the variable fluids'
+ * PhaseFieldUnstructured::init sets (:242-256): per site the variable fluids'
  * sum (hard density constraint), then the fixed fluids' DOFs. */
 void orc_pfu_constraints(const orc_pfu* p, double* data) {
   const int nF = p->nfluid;

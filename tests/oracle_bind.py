@@ -358,6 +358,10 @@ class OraclePFU:
         return np.array(self.p.kappa[: self.nfluid])
 
     @property
+    def node_vol(self):
+        return np.ctypeslib.as_array(self.p.node_vol, shape=(self.p.ngrid,)).copy()
+
+    @property
     def volume(self):
         return np.array(self.p.volume[: self.nfluid])
 

@@ -26,10 +26,11 @@ def rel_err(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-def device_pf(m, c):
-    """The library's PhaseField configured like a random_case."""
+def device_pf(m, c, cls="PhaseField"):
+    """The library's PhaseField (or another class with its setters, by name)
+    configured like a random_case."""
     o = c["opts"]
-    pf = (m.PhaseField().setNFluid(c["nfluid"]).setGridSize(list(c["grid"])).setResolution(c["resolution"])
+    pf = (getattr(m, cls)().setNFluid(c["nfluid"]).setGridSize(list(c["grid"])).setResolution(c["resolution"])
           .setSurfaceTension(c["surface_tension"]))
     pf.setInterfaceSize(c["interface_size"][0])
     pf.setDensityConstraint(c["density"])

@@ -1,7 +1,7 @@
 """Pins the oracle's PhaseFieldUnstructured (the element form of the phase
 field, src/potentials/PhaseFieldUnstructured.cpp) before the GPU tests trust it:
 the reference's unit-test known answers (test/unit/PhaseFieldUnstructured_test.cpp
-:27-66, 101-128, 131-178), the reference build on random layouts (energies,
+:27-80, 104-129, 132-180 where no elements are erased), the reference build on random layouts (energies,
 gradients, projections, element counts, L-BFGS trajectories), and finite
 differences of the energy.
 """
@@ -20,7 +20,7 @@ def near(a, b, tol=1e-6):
 
 @pytest.mark.parametrize("axis", [0, 1, 2])
 def test_bulk_known_answers(axis):
-    # PhaseFieldUnstructured_test.cpp:31-66
+    # PhaseFieldUnstructured_test.cpp:27-65
     o = OraclePFU((1, 1, 1), [0.1])
     assert o.kappa[0] == 3
     near(o.energy([0.1]), 3.0 / 16 * 1.21 * 0.81)
@@ -39,8 +39,18 @@ def test_bulk_known_answers(axis):
     near(o.gradient(x), [-3, 0, 3])
 
 
+def test_force_known_answers():
+    # :68-80
+    x = [1.0] * 8
+    for f, g in (([-2, 0, 0], [-.5] * 4 + [.5] * 4), ([0, -2, 0], [-.5, -.5, .5, .5, -.5, -.5, .5, .5]),
+                 ([0, 0, -2], [-.5, .5, -.5, .5, -.5, .5, -.5, .5])):
+        o = OraclePFU((2, 2, 2), x, force=f)
+        near(o.energy(x), 0)
+        near(o.gradient(x), g)
+
+
 def test_pressure_and_volume_known_answers():
-    # :101-128
+    # :104-129
     x = [1.0] * 6
     o = OraclePFU((6, 1, 1), x, solid=[1, 0, 0, 0, 0, 1], pressure=[10])
     near(o.energy(x), -30)
@@ -54,7 +64,7 @@ def test_pressure_and_volume_known_answers():
 
 
 def test_resolution_known_answers():
-    # :131-178 (the cases that do not erase elements)
+    # :132-180 (the cases that do not erase elements)
     x = [0, -1, 0, 1, 0]
     o = OraclePFU((5, 1, 1), x, resolution=2.0, solid=[1, 0, 0, 0, 1])
     assert o.kappa[0] == 1.5
@@ -137,7 +147,7 @@ def test_element_form_gradient_is_energy_derivative(seed):
     c = unstructured_case(400 + seed)
     c["opts"].pop("confinement", None)
     # the volume term's gradient is weighted towards interfaces, not the
-    # derivative of its energy (PhaseFieldUnstructured.cpp:290-295)
+    # derivative of its energy (PhaseFieldUnstructured.cpp:301-306)
     c["opts"].pop("volume_fixed", None)
     try:
         o = OraclePFU(c["grid"], c["x0"], nfluid=c["nfluid"], resolution=c["resolution"],
