@@ -90,6 +90,7 @@ class PhaseFieldUnstructured : public NewPotential<PhaseFieldUnstructured> {
   vector<double> nodeVol;  // per DOF, as the reference's initLocal stores it
   vector<int> fluidType;   // per DOF
   size_t nElements = 0;    // energy elements built by init
+  size_t listBytes = 0;    // device bytes of the element list (records, parameters, chunk and site tables)
 
  private:
   void setDefaults();

@@ -100,9 +100,11 @@ int minim_pf_coefficients(const minim_potential* pot, double* kappa, double* kap
  * minim_pf_node_volumes (node volumes only) and minim_pf_diffuse_solid accept
  * the handle; minim_pf_energy_components does not (the class has none). */
 int minim_pfu_create(const minim_pf_params* p, minim_potential** out);
-/* Number of energy elements init built (Potential::elements of the reference,
- * include/Potential.h:90). */
-int minim_pfu_n_elements(const minim_potential* pot, size_t* n);
+/* Number of energy elements init built (the size of the reference's
+ * Potential::elements, include/Potential.h:90) and the device bytes of the
+ * list as this library stores it (16-byte records, parameters, chunk and
+ * site tables); either pointer may be NULL. */
+int minim_pfu_element_list(const minim_potential* pot, size_t* n_elements, size_t* device_bytes);
 
 /* LjNd(nDim, sigma, epsilon) (include/potentials/LjNd.h:12-44): all pairs, no cutoff. */
 int minim_lj_create(int ndim, double sigma, double epsilon, minim_potential** out);

@@ -60,7 +60,7 @@ SIGNATURES = {
     "minim_pf_create": (C.c_int, [C.POINTER(PFParams), C.POINTER(vp)]),
     "minim_pf_coefficients": (C.c_int, [vp, dp, dp]),
     "minim_pfu_create": (C.c_int, [C.POINTER(PFParams), C.POINTER(vp)]),
-    "minim_pfu_n_elements": (C.c_int, [vp, C.POINTER(C.c_size_t)]),
+    "minim_pfu_element_list": (C.c_int, [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "minim_pf_energy_components": (C.c_int, [vp, dp, C.c_size_t, dp]),
     "minim_pf_node_volumes": (C.c_int, [vp, dp, dp]),
     "minim_pf_diffuse_solid": (C.c_int, [vp, C.POINTER(C.c_ubyte), C.c_int, C.c_int, dp]),

@@ -327,11 +327,16 @@ class PhaseFieldUnstructured(PhaseField):
 
     def nElements(self, coords=None):
         """Number of energy elements init builds."""
-        ng = int(np.prod(self.gridSize))
-        h = self.handle(np.zeros(ng * self.nFluid) if coords is None else coords)
-        n = C.c_size_t()
-        A.check(A.load().minim_pfu_n_elements(h.ptr, C.byref(n)))
-        return n.value
+        return self.elementList(coords)[0]
+
+    def elementList(self, coords=None, handle=None):
+        """(number of elements, device bytes of the list) after init."""
+        if handle is None:
+            ng = int(np.prod(self.gridSize))
+            handle = self.handle(np.zeros(ng * self.nFluid) if coords is None else coords)
+        n, b = C.c_size_t(), C.c_size_t()
+        A.check(A.load().minim_pfu_element_list(handle.ptr, C.byref(n), C.byref(b)))
+        return n.value, b.value
 
 
 def site_types(grid, solid):

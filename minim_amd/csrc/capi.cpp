@@ -144,14 +144,14 @@ int minim_pfu_create(const minim_pf_params* p, minim_potential** out) {
   });
 }
 
-int minim_pfu_n_elements(const minim_potential* pot, size_t* n) {
+int minim_pfu_element_list(const minim_potential* pot, size_t* n_elements, size_t* device_bytes) {
   return guard([&] {
     need(pot, "pot");
-    need(n, "n");
     auto* pf = dynamic_cast<const minim::PhaseFieldUnstructured*>(pot->p.get());
     if (!pf) throw std::invalid_argument("not a PhaseFieldUnstructured potential");
     if (!pot->initialised) throw std::runtime_error("PhaseFieldUnstructured: call minim_potential_init first");
-    *n = pf->nElements;
+    if (n_elements) *n_elements = pf->nElements;
+    if (device_bytes) *device_bytes = pf->listBytes;
   });
 }
 

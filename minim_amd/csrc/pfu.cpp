@@ -2,6 +2,7 @@
 // defaults and checks (src/potentials/PhaseFieldUnstructured.cpp:71-107,
 // :110-268, :594-978), the element list built in the reference's order, and
 // the launches of pfu.hip.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <mutex>
@@ -20,15 +21,181 @@ namespace minim {
 
 namespace {
 
-// Launch arguments, the host element list until the first launch uploads it,
-// and the device memory.
+// The element list in the reference's order (PhaseFieldUnstructured::init,
+// :157-240): type, DOF indices (unused slots -1) and parameters.
+struct RefElements {
+  std::vector<int8_t> type;
+  std::vector<int32_t> idof;  // kPFUNodes per element
+  std::vector<double> par;    // 3 per element
+  void push(int t, std::initializer_list<int32_t> ids, std::initializer_list<double> p) {
+    type.push_back((int8_t)t);
+    size_t k = 0;
+    for (int32_t i : ids) {
+      idof.push_back(i);
+      k++;
+    }
+    for (; k < (size_t)minim_amd::kPFUNodes; k++) idof.push_back(-1);
+    k = 0;
+    for (double v : p) {
+      par.push_back(v);
+      k++;
+    }
+    for (; k < 3; k++) par.push_back(0.0);
+  }
+  size_t size() const { return type.size(); }
+};
+
+// The device form of the list, on the host until the first launch uploads it:
+// elements sorted (stably) into chunks by the tile of their first node, each
+// chunk's sorted table of the sites its elements touch (flagged where another
+// chunk touches the site too), and 16-byte element records naming their
+// nodes by table position.
+struct PFUPacked {
+  std::vector<uint32_t> rec;  // 4 words per element
+  std::vector<uint32_t> pidx;
+  std::vector<double> par;
+  std::vector<uint32_t> chunk_el, chunk_site, sites;
+  // sites not exclusive to one chunk, and the site-table positions of each
+  std::vector<uint32_t> gsite, gpos_off, gpos;
+};
+
+// Largest tile of tx x ty x tz sites whose sites plus face neighbours hold at
+// most kPFUChunkDofs DOFs (fewer counted in a direction the tile spans).
+void choose_tile(const int n[3], int nf, int t[3]) {
+  long best = 0, best_touch = 0;
+  t[0] = t[1] = t[2] = 1;
+  const long cap = minim_amd::kPFUChunkDofs / nf;
+  for (int tz = 1; tz <= std::min(n[2], 64); tz++)
+    for (int ty = 1; ty <= std::min(n[1], 64); ty++)
+      for (int tx = 1; tx <= std::min(n[0], 64); tx++) {
+        const long v = (long)tx * ty * tz;
+        const long touch = v + (n[0] > tx ? 2L * ty * tz : 0) + (n[1] > ty ? 2L * tx * tz : 0) +
+                           (n[2] > tz ? 2L * tx * ty : 0);
+        if (touch > cap) break;
+        if (v > best || (v == best && touch < best_touch)) {
+          best = v;
+          best_touch = touch;
+          t[0] = tx;
+          t[1] = ty;
+          t[2] = tz;
+        }
+      }
+}
+
+// Packs the list for tiles of t[] sites; false when a chunk touches more than
+// kPFUChunkDofs DOFs.
+bool pack_elements(const RefElements& el, const int n[3], int nf, const int t[3], PFUPacked& out) {
+  const size_t nel = el.size();
+  const size_t ng = (size_t)n[0] * n[1] * n[2];
+  const int nt[3] = {(n[0] + t[0] - 1) / t[0], (n[1] + t[1] - 1) / t[1], (n[2] + t[2] - 1) / t[2]};
+  const size_t ntile = (size_t)nt[0] * nt[1] * nt[2];
+  auto tile_of = [&](size_t site) {
+    const size_t z = site % n[2], y = (site / n[2]) % n[1], x = site / ((size_t)n[1] * n[2]);
+    return ((x / t[0]) * nt[1] + y / t[1]) * nt[2] + z / t[2];
+  };
+  // counting sort of the elements by the tile of their first node
+  std::vector<uint32_t> start(ntile + 1, 0);
+  std::vector<uint32_t> key(nel);
+  for (size_t e = 0; e < nel; e++) {
+    key[e] = (uint32_t)tile_of((size_t)el.idof[e * minim_amd::kPFUNodes] / nf);
+    start[key[e] + 1]++;
+  }
+  for (size_t i = 0; i < ntile; i++) start[i + 1] += start[i];
+  std::vector<uint32_t> order(nel);
+  {
+    std::vector<uint32_t> pos(start.begin(), start.end() - 1);
+    for (size_t e = 0; e < nel; e++) order[pos[key[e]]++] = (uint32_t)e;
+  }
+  std::vector<uint32_t>().swap(key);
+  std::vector<int32_t> stamp(ng, -1);
+  std::vector<uint16_t> local(ng, 0);
+  std::vector<uint8_t> touch(ng, 0);
+  out = PFUPacked{};
+  out.rec.resize(4 * nel);
+  out.chunk_el.push_back(0);
+  out.chunk_site.push_back(0);
+  std::vector<uint32_t> list;
+  auto nnodes = [&](int type) {
+    return (type == minim_amd::kPFUFluid || type == minim_amd::kPFUFluidAll) ? minim_amd::kPFUNodes : 1;
+  };
+  size_t pos = 0;
+  for (size_t tl = 0; tl < ntile; tl++) {
+    if (start[tl] == start[tl + 1]) continue;
+    const int32_t c = (int32_t)(out.chunk_el.size() - 1);
+    list.clear();
+    for (uint32_t k = start[tl]; k < start[tl + 1]; k++) {
+      const size_t e = order[k];
+      const int nn = nnodes(el.type[e]);
+      for (int i = 0; i < nn; i++) {
+        const size_t site = (size_t)el.idof[e * minim_amd::kPFUNodes + i] / nf;
+        if (stamp[site] != c) {
+          stamp[site] = c;
+          list.push_back((uint32_t)site);
+        }
+      }
+    }
+    if (list.size() * (size_t)nf > (size_t)minim_amd::kPFUChunkDofs) return false;
+    std::sort(list.begin(), list.end());
+    for (size_t i = 0; i < list.size(); i++) {
+      local[list[i]] = (uint16_t)i;
+      if (touch[list[i]] < 2) touch[list[i]]++;
+    }
+    for (uint32_t k = start[tl]; k < start[tl + 1]; k++, pos++) {
+      const size_t e = order[k];
+      const int type = el.type[e];
+      const int nn = nnodes(type);
+      const int32_t* id = &el.idof[e * minim_amd::kPFUNodes];
+      uint32_t l[minim_amd::kPFUNodes];
+      for (int i = 0; i < minim_amd::kPFUNodes; i++) l[i] = local[(size_t)id[i < nn ? i : 0] / nf];
+      const uint32_t fluid = (uint32_t)(id[0] % nf);
+      uint32_t* r = &out.rec[4 * pos];
+      r[0] = l[0] | (l[1] << 16);
+      r[1] = l[2] | (l[3] << 16);
+      r[2] = l[4] | (l[5] << 16);
+      r[3] = l[6] | ((uint32_t)type << 16) | (fluid << 24);
+      if (type == minim_amd::kPFUSurface || type == minim_amd::kPFUForce || type == minim_amd::kPFUConfinement) {
+        if (out.pidx.empty()) out.pidx.assign(nel, 0);
+        out.pidx[pos] = (uint32_t)(out.par.size() / 3);
+        out.par.insert(out.par.end(), &el.par[3 * e], &el.par[3 * e] + 3);
+      }
+    }
+    out.sites.insert(out.sites.end(), list.begin(), list.end());
+    out.chunk_el.push_back((uint32_t)pos);
+    out.chunk_site.push_back((uint32_t)out.sites.size());
+  }
+  for (uint32_t& en : out.sites)
+    if (touch[en] > 1) en |= minim_amd::kPFUShared;
+  // the gather pass: every site but the exclusive ones (untouched sites too,
+  // whose gradient it sets to zero), with its table positions in chunk order
+  std::vector<uint32_t>().swap(order);
+  std::vector<int32_t>& gidx = stamp;  // reused: site -> gather index, or -1
+  out.gpos_off.push_back(0);
+  for (size_t site = 0; site < ng; site++) {
+    gidx[site] = -1;
+    if (touch[site] == 1) continue;
+    gidx[site] = (int32_t)out.gsite.size();
+    out.gsite.push_back((uint32_t)site);
+    out.gpos_off.push_back(0);
+  }
+  for (uint32_t en : out.sites)
+    if (en & minim_amd::kPFUShared) out.gpos_off[gidx[en & ~minim_amd::kPFUShared] + 1]++;
+  for (size_t j = 0; j < out.gsite.size(); j++) out.gpos_off[j + 1] += out.gpos_off[j];
+  out.gpos.resize(out.gpos_off.back());
+  std::vector<uint32_t> fill(out.gpos_off.begin(), out.gpos_off.end() - 1);
+  for (size_t p = 0; p < out.sites.size(); p++)
+    if (out.sites[p] & minim_amd::kPFUShared) out.gpos[fill[gidx[out.sites[p] & ~minim_amd::kPFUShared]]++] = (uint32_t)p;
+  return true;
+}
+
+// Launch arguments, the packed element list until the first launch uploads
+// it, and the device memory.
 struct PFUHolder {
   minim_amd::PFUArgs u{};
   minim_amd::PFArgs con{};  // the per-site projection (hard density, fixed fluids)
-  std::vector<int8_t> type_h;
-  std::vector<int32_t> idof_h;
-  std::vector<double> par_h, vol_h, target_h;
-  minim_amd::DeviceBuffer type, idof, par, vol, target, volg;
+  PFUPacked pk;
+  std::vector<double> vol_h, target_h;
+  minim_amd::DeviceBuffer rec, pidx, par, chunk_el, chunk_site, sites, gsite, gpos_off, gpos, scratch, vol, target,
+      volg;
   std::once_flag uploaded;
 };
 
@@ -51,9 +218,19 @@ PFUHolder& holder_peek(const std::shared_ptr<void>& p) {
 PFUHolder& holder_of(const std::shared_ptr<void>& p) {
   PFUHolder* h = &holder_peek(p);
   std::call_once(h->uploaded, [h] {
-    h->u.type = upload_vec(h->type, h->type_h);
-    h->u.idof = upload_vec(h->idof, h->idof_h);
-    h->u.par = upload_vec(h->par, h->par_h);
+    h->u.rec = reinterpret_cast<const uint4*>(upload_vec(h->rec, h->pk.rec));
+    h->u.pidx = upload_vec(h->pidx, h->pk.pidx);
+    h->u.par = upload_vec(h->par, h->pk.par);
+    h->u.chunk_el = upload_vec(h->chunk_el, h->pk.chunk_el);
+    h->u.chunk_site = upload_vec(h->chunk_site, h->pk.chunk_site);
+    const size_t nslot = h->pk.sites.size() * (size_t)h->u.nf;
+    h->u.sites = upload_vec(h->sites, h->pk.sites);
+    h->u.ngather = h->pk.gsite.size();
+    h->u.gsite = upload_vec(h->gsite, h->pk.gsite);
+    h->u.gpos_off = upload_vec(h->gpos_off, h->pk.gpos_off);
+    h->u.gpos = upload_vec(h->gpos, h->pk.gpos);
+    h->scratch.reset(sizeof(double) * (nslot ? nslot : 1));
+    h->u.scratch = h->scratch.get();
     h->u.node_vol = upload_vec(h->vol, h->vol_h);
     h->u.volume = upload_vec(h->target, h->target_h);
     h->volg.reset(sizeof(double) * minim_amd::kPFMaxFluid);
@@ -277,20 +454,9 @@ void PhaseFieldUnstructured::init(const vector<double>& coords) {
   }
 
   auto h = std::make_shared<PFUHolder>();
-  auto push = [&](int t, std::initializer_list<int32_t> ids, std::initializer_list<double> par) {
-    h->type_h.push_back((int8_t)t);
-    size_t k = 0;
-    for (int32_t i : ids) {
-      h->idof_h.push_back(i);
-      k++;
-    }
-    for (; k < (size_t)minim_amd::kPFUNodes; k++) h->idof_h.push_back(-1);
-    k = 0;
-    for (double p : par) {
-      h->par_h.push_back(p);
-      k++;
-    }
-    for (; k < 3; k++) h->par_h.push_back(0.0);
+  RefElements el;
+  auto push = [&el](int t, std::initializer_list<int32_t> ids, std::initializer_list<double> par) {
+    el.push(t, ids, par);
   };
   const int n0 = gridSize[0], n1 = gridSize[1], n2 = gridSize[2];
   static const int off[6][3] = {{-1, 0, 0}, {0, -1, 0}, {0, 0, -1}, {0, 0, 1}, {0, 1, 0}, {1, 0, 0}};
@@ -339,8 +505,23 @@ void PhaseFieldUnstructured::init(const vector<double>& coords) {
           if (confinementStrength[f] != 0)
             push(minim_amd::kPFUConfinement, {iGrid * nF + f}, {confinementStrength[f], coords[iGrid * nF + f]});
       }
-  nElements = h->type_h.size();
+  nElements = el.size();
   if (ng * (size_t)nF >= ((size_t)1 << 31)) throw std::invalid_argument("PhaseFieldUnstructured: more than 2^31 DOFs");
+  if (nElements >= ((size_t)1 << 32)) throw std::invalid_argument("PhaseFieldUnstructured: more than 2^32 elements");
+  {
+    // the device list: chunks of tiles, shrunk until every chunk fits its LDS accumulator
+    int t[3];
+    choose_tile(gridSize.data(), nF, t);
+    while (!pack_elements(el, gridSize.data(), nF, t, h->pk)) {
+      int* big = std::max_element(t, t + 3);
+      if (*big == 1) throw std::runtime_error("PhaseFieldUnstructured: an element touches too many sites");
+      *big = (*big + 1) / 2;
+    }
+    h->u.nchunk = (int)(h->pk.chunk_el.size() - 1);
+    const PFUPacked& pk = h->pk;
+    listBytes = 4 * (pk.rec.size() + pk.pidx.size() + pk.chunk_el.size() + pk.chunk_site.size() + pk.sites.size()) +
+                8 * pk.par.size();
+  }
 
   // initLocal: node volume and fluid of every DOF
   nodeVol.assign(ng * nF, 0.0);

@@ -6,13 +6,17 @@
 // site and fluid (or per site for the "fixed" density mode, whose last fluid
 // is implicit), one-DOF pressure, surface, force and confinement elements and
 // a soft-density element per site, then the volume term over the whole field
-// (blockEnergyGradient, :271-297).  Here one thread evaluates one element of
-// the list the host built (in the reference's order and with its parameters),
-// accumulates its energy in FP64 for a deterministic two-stage reduction, and
-// adds its gradient contributions with hardware atomics (global_atomic_add):
-// the element form is kept as it is, including the reference's choices in
-// each element, so that it serves as the independent cross-check of the
-// gather-form kernels it is in the reference.
+// (blockEnergyGradient, :280-308).  Here the host groups the elements into
+// chunks of nearby sites; a block takes one chunk at a time, one thread per
+// element, accumulates the energy in FP64 for a deterministic two-stage
+// reduction and the gradient contributions in an FP64 LDS accumulator
+// (ds_add_f64), then stores the gradient of every site only its chunk touches
+// and writes the partial sums of the sites other chunks also touch to a
+// scratch slot; a second pass sums those partials per site in chunk order.
+// No global atomics; the LDS adds of one chunk land in the order its threads
+// reach them, so the gradient's last bits may vary from launch to launch.  Each element keeps the reference's formula,
+// including its choices, so that the element form serves as the independent
+// cross-check of the gather-form kernels it is in the reference.
 #include "common.hpp"
 #include "pfu.hpp"
 
@@ -75,170 +79,198 @@ __device__ __forceinline__ void phase7(const T (&c)[kPFUNodes], const bool (&sel
   if (WE) e = e + factor * grad2;
 }
 
-template <typename TV, typename T>
-__device__ __forceinline__ void gadd(TV* g, int i, T v) {
-  unsafeAtomicAdd(g + i, (TV)v);
-}
-
 // MODE bits: 1 = energy, 2 = gradient, 4 = at x + t*d.
 template <typename T, typename TV, int MODE>
-__global__ __launch_bounds__(kBlock) void pfu_element_kernel(PFUArgs u, const T* __restrict__ x,
-                                                             const TV* __restrict__ d, T t, TV* __restrict__ g,
-                                                             RedScratch rs) {
+__global__ __launch_bounds__(kBlock) void pfu_chunk_kernel(PFUArgs u, const T* __restrict__ x,
+                                                           const TV* __restrict__ d, T t, TV* __restrict__ g,
+                                                           RedScratch rs) {
   constexpr bool WE = MODE & 1, WG = MODE & 2, DIR = MODE & 4;
+  __shared__ uint32_t ssite[kPFUChunkDofs];
+  __shared__ double sacc[WG ? kPFUChunkDofs : 1];
   const Coords<T, TV, DIR> cx{x, d, t};
   const int nf = u.nf;
+  // gradient contribution v to fluid k of local site l
+  auto add = [&](int l, int k, T v) {
+    if (WG && v != (T)0) unsafeAtomicAdd(&sacc[l * nf + k], (double)v);
+  };
+  auto gsite = [&](int l) { return (int)(ssite[l] & ~kPFUShared); };
   double acc[2] = {0.0, 0.0};
-  for (size_t ie = (size_t)blockIdx.x * kBlock + threadIdx.x; ie < u.nel; ie += (size_t)gridDim.x * kBlock) {
-    const int type = u.type[ie];
-    const int32_t* id = u.idof + ie * kPFUNodes;
-    const double* par = u.par + ie * 3;
-    T e = (T)0;
-    switch (type) {
-      case kPFUFluid: {  // fluidEnergy (:518-538)
-        const T vol = (T)par[0];
-        const int f = (int)par[1];
-        int n[kPFUNodes];
-        T c[kPFUNodes], gl[kPFUNodes];
-        bool self[kPFUNodes];
-#pragma unroll
-        for (int i = 0; i < kPFUNodes; i++) {
-          n[i] = id[i];
-          c[i] = cx(n[i]);
-          gl[i] = (T)0;
-          self[i] = (i > 0) && n[i] == n[0];
-        }
-        const T c0 = c[0];
-        T factor;
-        if (nf == 1) {
-          factor = ((T)u.kappa[0] / (T)16) * vol;
-          if (WE) e = e + (factor * ((c0 + (T)1) * (c0 + (T)1))) * ((c0 - (T)1) * (c0 - (T)1));
-          if (WG) gl[0] = gl[0] + ((factor * (T)4) * c0) * (c0 * c0 - (T)1);
-        } else {
-          factor = ((T)0.5 * (T)u.kappa[f]) * vol;
-          if (WE) e = e + (factor * (c0 * c0)) * ((c0 - (T)1) * (c0 - (T)1));
-          if (WG) gl[0] = gl[0] + (((factor * (T)2) * c0) * (c0 - (T)1)) * ((T)2 * c0 - (T)1);
-        }
-        T gfac = (nf == 1) ? ((T)0.25 * (T)u.kappaP[0]) * vol : ((T)0.5 * (T)u.kappaP[f]) * vol;
-        gfac = gfac / (T)u.res2;
-        phase7<T, WE, WG>(c, self, gfac, e, gl);
-        if (WG) {
-#pragma unroll
-          for (int i = 0; i < kPFUNodes; i++)
-            if (gl[i] != (T)0) gadd(g, n[i], gl[i]);
-        }
-        break;
-      }
-      case kPFUFluidAll: {  // fluidEnergyAll (:541-586): the last fluid is 1 - the others
-        const T vol = (T)par[0];
-        int n[kPFUNodes];
-        T cN[kPFUNodes], gN[kPFUNodes];
-        bool none[kPFUNodes];
-#pragma unroll
-        for (int i = 0; i < kPFUNodes; i++) {
-          n[i] = id[i];
-          cN[i] = (T)1;
-          gN[i] = (T)0;
-          none[i] = false;  // the reference passes distinct indices here: no node counts as solid
-        }
-        for (int f = 0; f < nf - 1; f++)
-#pragma unroll
-          for (int i = 0; i < kPFUNodes; i++) cN[i] = cN[i] - cx(n[i] + f);
-        const T bfN = ((T)0.5 * (T)u.kappa[nf - 1]) * vol;
-        const T gfN = ((T)0.5 * (T)u.kappaP[nf - 1]) * vol;  // not divided by res^2 (:555)
-        if (WE) e = e + (bfN * (cN[0] * cN[0])) * ((cN[0] - (T)1) * (cN[0] - (T)1));
-        if (WG) gN[0] = gN[0] + (((bfN * (T)2) * cN[0]) * (cN[0] - (T)1)) * ((T)2 * cN[0] - (T)1);
-        phase7<T, WE, WG>(cN, none, gfN, e, gN);
-        for (int f = 0; f < nf - 1; f++) {
+  for (int c = blockIdx.x; c < u.nchunk; c += gridDim.x) {
+    const uint32_t s0 = u.chunk_site[c];
+    const int ns = (int)(u.chunk_site[c + 1] - s0);
+    for (int i = threadIdx.x; i < ns; i += kBlock) ssite[i] = u.sites[s0 + i];
+    if (WG)
+      for (int i = threadIdx.x; i < ns * nf; i += kBlock) sacc[i] = 0.0;
+    __syncthreads();
+    const uint32_t e1 = u.chunk_el[c + 1];
+    for (uint32_t ie = u.chunk_el[c] + threadIdx.x; ie < e1; ie += kBlock) {
+      const uint4 r = u.rec[ie];
+      const int loc[kPFUNodes] = {(int)(r.x & 0xffffu), (int)(r.x >> 16), (int)(r.y & 0xffffu), (int)(r.y >> 16),
+                                  (int)(r.z & 0xffffu), (int)(r.z >> 16), (int)(r.w & 0xffffu)};
+      const int type = (int)((r.w >> 16) & 0xffu), f = (int)(r.w >> 24);
+      const int site0 = gsite(loc[0]);
+      T e = (T)0;
+      switch (type) {
+        case kPFUFluid: {  // fluidEnergy (:518-538)
+          const T vol = (T)u.node_vol[site0];
           T c[kPFUNodes], gl[kPFUNodes];
           bool self[kPFUNodes];
 #pragma unroll
           for (int i = 0; i < kPFUNodes; i++) {
-            c[i] = cx(n[i] + f);
+            c[i] = cx(gsite(loc[i]) * nf + f);
             gl[i] = (T)0;
-            self[i] = (i > 0) && n[i] == n[0];
+            self[i] = (i > 0) && loc[i] == loc[0];
           }
           const T c0 = c[0];
-          const T bf = ((T)0.5 * (T)u.kappa[f]) * vol;
-          if (WE) e = e + (bf * (c0 * c0)) * ((c0 - (T)1) * (c0 - (T)1));
-          if (WG) gl[0] = gl[0] + (((bf * (T)2) * c0) * (c0 - (T)1)) * ((T)2 * c0 - (T)1);
-          T gf = ((T)0.5 * (T)u.kappaP[f]) * vol;
-          gf = gf / (T)u.res2;
-          phase7<T, WE, WG>(c, self, gf, e, gl);
-          if (WG) {
+          if (nf == 1) {
+            const T factor = ((T)u.kappa[0] / (T)16) * vol;
+            if (WE) e = e + (factor * ((c0 + (T)1) * (c0 + (T)1))) * ((c0 - (T)1) * (c0 - (T)1));
+            if (WG) gl[0] = gl[0] + ((factor * (T)4) * c0) * (c0 * c0 - (T)1);
+          } else {
+            const T factor = ((T)0.5 * (T)u.kappa[f]) * vol;
+            if (WE) e = e + (factor * (c0 * c0)) * ((c0 - (T)1) * (c0 - (T)1));
+            if (WG) gl[0] = gl[0] + (((factor * (T)2) * c0) * (c0 - (T)1)) * ((T)2 * c0 - (T)1);
+          }
+          T gfac = (nf == 1) ? ((T)0.25 * (T)u.kappaP[0]) * vol : ((T)0.5 * (T)u.kappaP[f]) * vol;
+          gfac = gfac / (T)u.res2;
+          phase7<T, WE, WG>(c, self, gfac, e, gl);
+#pragma unroll
+          for (int i = 0; i < kPFUNodes; i++) add(loc[i], f, gl[i]);
+          break;
+        }
+        case kPFUFluidAll: {  // fluidEnergyAll (:541-586): the last fluid is 1 - the others
+          const T vol = (T)u.node_vol[site0];
+          int sn[kPFUNodes];
+          T cN[kPFUNodes], gN[kPFUNodes];
+          bool none[kPFUNodes];
+#pragma unroll
+          for (int i = 0; i < kPFUNodes; i++) {
+            sn[i] = gsite(loc[i]) * nf;
+            cN[i] = (T)1;
+            gN[i] = (T)0;
+            none[i] = false;  // the reference passes distinct indices here: no node counts as solid
+          }
+          for (int k = 0; k < nf - 1; k++)
+#pragma unroll
+            for (int i = 0; i < kPFUNodes; i++) cN[i] = cN[i] - cx(sn[i] + k);
+          const T bfN = ((T)0.5 * (T)u.kappa[nf - 1]) * vol;
+          const T gfN = ((T)0.5 * (T)u.kappaP[nf - 1]) * vol;  // not divided by res^2 (:555)
+          if (WE) e = e + (bfN * (cN[0] * cN[0])) * ((cN[0] - (T)1) * (cN[0] - (T)1));
+          if (WG) gN[0] = gN[0] + (((bfN * (T)2) * cN[0]) * (cN[0] - (T)1)) * ((T)2 * cN[0] - (T)1);
+          phase7<T, WE, WG>(cN, none, gfN, e, gN);
+          for (int k = 0; k < nf - 1; k++) {
+            T c[kPFUNodes], gl[kPFUNodes];
+            bool self[kPFUNodes];
 #pragma unroll
             for (int i = 0; i < kPFUNodes; i++) {
-              const T v = gl[i] - gN[i];
-              if (v != (T)0) gadd(g, n[i] + f, v);
+              c[i] = cx(sn[i] + k);
+              gl[i] = (T)0;
+              self[i] = (i > 0) && loc[i] == loc[0];
             }
+            const T c0 = c[0];
+            const T bf = ((T)0.5 * (T)u.kappa[k]) * vol;
+            if (WE) e = e + (bf * (c0 * c0)) * ((c0 - (T)1) * (c0 - (T)1));
+            if (WG) gl[0] = gl[0] + (((bf * (T)2) * c0) * (c0 - (T)1)) * ((T)2 * c0 - (T)1);
+            T gf = ((T)0.5 * (T)u.kappaP[k]) * vol;
+            gf = gf / (T)u.res2;
+            phase7<T, WE, WG>(c, self, gf, e, gl);
+#pragma unroll
+            for (int i = 0; i < kPFUNodes; i++) add(loc[i], k, gl[i] - gN[i]);
           }
+          break;
         }
-        break;
-      }
-      case kPFUPressure: {  // pressureEnergy (:613-628)
-        const int i = id[0];
-        const T pr = (T)u.pressure[i % nf];
-        const T v = (T)u.node_vol[i / nf];
-        const T c = cx(i);
-        if (nf == 1) {
-          if (WE) e = e - pr * (((T)0.5 * (c + (T)1)) * v);
-          if (WG) gadd(g, i, -(((T)0.5 * pr) * v));
-        } else {
-          if (WE) e = e - pr * (c * v);
-          if (WG) gadd(g, i, -(pr * v));
+        case kPFUPressure: {  // pressureEnergy (:613-628)
+          const T pr = (T)u.pressure[f];
+          const T v = (T)u.node_vol[site0];
+          const T c = cx(site0 * nf + f);
+          if (nf == 1) {
+            if (WE) e = e - pr * (((T)0.5 * (c + (T)1)) * v);
+            add(loc[0], f, -(((T)0.5 * pr) * v));
+          } else {
+            if (WE) e = e - pr * (c * v);
+            add(loc[0], f, -(pr * v));
+          }
+          break;
         }
-        break;
-      }
-      case kPFUDensity: {  // densityConstraintEnergy (:631-642): three DOFs of the site
-        if (nf == 1) break;
-        const T coef = (T)u.coef_density;
-        const T rho = ((cx(id[0]) + cx(id[1])) + cx(id[2])) - (T)1;
-        if (WE) e = e + coef * (rho * rho);
-        if (WG) {
+        case kPFUDensity: {  // densityConstraintEnergy (:631-642): three DOFs of the site
+          if (nf == 1) break;
+          const T coef = (T)u.coef_density;
+          const T rho = ((cx(site0 * nf) + cx(site0 * nf + 1)) + cx(site0 * nf + 2)) - (T)1;
+          if (WE) e = e + coef * (rho * rho);
           const T gr = ((T)2 * coef) * rho;
-          gadd(g, id[0], gr);
-          gadd(g, id[1], gr);
-          gadd(g, id[2], gr);
+          add(loc[0], 0, gr);
+          add(loc[0], 1, gr);
+          add(loc[0], 2, gr);
+          break;
         }
-        break;
-      }
-      case kPFUSurface: {  // surfaceEnergy (:645-654): binary fluids only
-        if (nf != 1) break;
-        const T area = (T)par[0], w = (T)par[1];
-        const T phi = cx(id[0]);
-        if (WE) e = e + (w * ((((phi * phi) * phi) / (T)3 - phi) - (T)2 / (T)3)) * area;
-        if (WG) gadd(g, id[0], (w * (phi * phi - (T)1)) * area);
-        break;
-      }
-      case kPFUForce: {  // forceEnergy (:657-676)
-        const T c = (nf == 1) ? (T)0.5 * ((T)1 + cx(id[0])) : cx(id[0]);
-        if (c < (T)0.01) break;
-        const T vol = (T)par[0], f = (T)par[1], h = (T)par[2];
-        if (WE) e = e + ((c * vol) * f) * h;
-        if (WG) gadd(g, id[0], (nf == 1) ? ((((T)0.5 * vol) * f) * h) : ((vol * f) * h));
-        break;
-      }
-      case kPFUConfinement: {  // ffConfinementEnergy (:679-692)
-        const T c = cx(id[0]);
-        const T coef = ((T)par[0] * (T)u.st_mean) * (T)u.res2;
-        const T c0 = (T)par[1];
-        if ((c0 - (T)0.5) * (c - (T)0.5) < (T)0) {
-          if (WE) e = e + coef * ((c - (T)0.5) * (c - (T)0.5));
-          if (WG) gadd(g, id[0], (coef * (T)2) * (c - (T)0.5));
+        case kPFUSurface: {  // surfaceEnergy (:645-654): binary fluids only
+          if (nf != 1) break;
+          const double* p = u.par + 3 * (size_t)u.pidx[ie];
+          const T area = (T)p[0], w = (T)p[1];
+          const T phi = cx(site0 * nf + f);
+          if (WE) e = e + (w * ((((phi * phi) * phi) / (T)3 - phi) - (T)2 / (T)3)) * area;
+          add(loc[0], f, (w * (phi * phi - (T)1)) * area);
+          break;
         }
-        break;
+        case kPFUForce: {  // forceEnergy (:657-676)
+          const T c = (nf == 1) ? (T)0.5 * ((T)1 + cx(site0 * nf + f)) : cx(site0 * nf + f);
+          if (c < (T)0.01) break;
+          const double* p = u.par + 3 * (size_t)u.pidx[ie];
+          const T vol = (T)p[0], fm = (T)p[1], h = (T)p[2];
+          if (WE) e = e + ((c * vol) * fm) * h;
+          add(loc[0], f, (nf == 1) ? ((((T)0.5 * vol) * fm) * h) : ((vol * fm) * h));
+          break;
+        }
+        case kPFUConfinement: {  // ffConfinementEnergy (:679-692)
+          const T c = cx(site0 * nf + f);
+          const double* p = u.par + 3 * (size_t)u.pidx[ie];
+          const T coef = ((T)p[0] * (T)u.st_mean) * (T)u.res2;
+          const T c0 = (T)p[1];
+          if ((c0 - (T)0.5) * (c - (T)0.5) < (T)0) {
+            if (WE) e = e + coef * ((c - (T)0.5) * (c - (T)0.5));
+            add(loc[0], f, (coef * (T)2) * (c - (T)0.5));
+          }
+          break;
+        }
+        default:
+          break;
       }
-      default:
-        break;
+      if (WE) acc[0] += (double)e;
     }
-    if (WE) acc[0] += (double)e;
+    __syncthreads();
+    if (WG) {
+      // sites only this chunk touches: the gradient; the others: partials
+      TV* __restrict__ part = static_cast<TV*>(u.scratch) + (size_t)s0 * nf;
+      for (int i = threadIdx.x; i < ns * nf; i += kBlock) {
+        const int l = i / nf;
+        const uint32_t en = ssite[l];
+        const TV v = (TV)sacc[i];
+        if (en & kPFUShared)
+          part[i] = v;
+        else
+          g[(size_t)(en & ~kPFUShared) * nf + (i - l * nf)] = v;
+      }
+      __syncthreads();
+    }
   }
   grid_sum_store<2>(acc, rs);
 }
 
+// The gradient of the sites that are not exclusive to one chunk: the sum of
+// their partials in chunk order (zero for a site no element touches).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void pfu_zero_kernel(T* __restrict__ v, size_t n) {
-  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) v[i] = (T)0;
+__global__ __launch_bounds__(kBlock) void pfu_gather_kernel(PFUArgs u, T* __restrict__ g) {
+  const int nf = u.nf;
+  const T* __restrict__ part = static_cast<const T*>(u.scratch);
+  const size_t n = u.ngather * nf;
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+    const size_t j = i / nf;
+    const int k = (int)(i - j * nf);
+    const uint32_t p0 = u.gpos_off[j], p1 = u.gpos_off[j + 1];
+    double sum = 0.0;
+    for (uint32_t p = p0; p < p1; p++) sum += (double)part[(size_t)u.gpos[p] * nf + k];
+    g[(size_t)u.gsite[j] * nf + k] = (T)sum;
+  }
 }
 
 // Fluid volumes for the volume term (:284-293).
@@ -292,6 +324,10 @@ __global__ __launch_bounds__(kBlock) void pfu_volgrad_kernel(PFUArgs u, const T*
   }
 }
 
+// Blocks of a chunk-kernel launch: each loops over chunks (at most
+// kMaxRedBlocks blocks write reduction partials).
+int chunk_blocks(const PFUArgs& u) { return u.nchunk < 1 ? 1 : (u.nchunk < 2048 ? u.nchunk : 2048); }
+
 template <typename T>
 void eval_t(const PFUArgs& u, int mode, const void* x, const void* d, double t, void* g, RedScratch rs,
             hipStream_t s) {
@@ -300,15 +336,16 @@ void eval_t(const PFUArgs& u, int mode, const void* x, const void* d, double t, 
   T* gg = static_cast<T*>(g);
   const T tt = (T)t;
   const size_t nd = u.ngrid * u.nf;
-  if (mode & 2) pfu_zero_kernel<T><<<grid_blocks(nd, 4, 4096), kBlock, 0, s>>>(gg, nd);
-  const int nb = grid_blocks(u.nel, 1, 4096);
+  const int nb = chunk_blocks(u);
   switch (mode) {
-    case 1: pfu_element_kernel<T, T, 1><<<nb, kBlock, 0, s>>>(u, xx, dd, tt, gg, rs); break;
-    case 2: pfu_element_kernel<T, T, 2><<<nb, kBlock, 0, s>>>(u, xx, dd, tt, gg, rs); break;
-    case 3: pfu_element_kernel<T, T, 3><<<nb, kBlock, 0, s>>>(u, xx, dd, tt, gg, rs); break;
-    case 5: pfu_element_kernel<T, T, 5><<<nb, kBlock, 0, s>>>(u, xx, dd, tt, gg, rs); break;
+    case 1: pfu_chunk_kernel<T, T, 1><<<nb, kBlock, 0, s>>>(u, xx, dd, tt, gg, rs); break;
+    case 2: pfu_chunk_kernel<T, T, 2><<<nb, kBlock, 0, s>>>(u, xx, dd, tt, gg, rs); break;
+    case 3: pfu_chunk_kernel<T, T, 3><<<nb, kBlock, 0, s>>>(u, xx, dd, tt, gg, rs); break;
+    case 5: pfu_chunk_kernel<T, T, 5><<<nb, kBlock, 0, s>>>(u, xx, dd, tt, gg, rs); break;
     default: throw std::invalid_argument("pfu: unsupported evaluation mode");
   }
+  if ((mode & 2) && u.ngather)
+    pfu_gather_kernel<T><<<grid_blocks(u.ngather * u.nf, 1, 8192), kBlock, 0, s>>>(u, gg);
   launch_finalize(rs, 2, nb, s);
   if (u.volfix) {
     RedScratch slot{rs.partials, rs.result + kPFVolSlot};
@@ -336,16 +373,26 @@ void pfu_launch_eval(const PFUArgs& u, int prec, int mode, const void* x, const 
 
 double pfu_time_elements(const PFUArgs& u, int prec, int mode, const void* x, void* g, int reps, RedScratch rs,
                          hipStream_t s) {
-  const int nb = grid_blocks(u.nel, 1, 4096);
+  const int nb = chunk_blocks(u);
   auto launch = [&]() {
     if (prec == F64) {
       const double* xx = static_cast<const double*>(x);
-      if (mode == 1) pfu_element_kernel<double, double, 1><<<nb, kBlock, 0, s>>>(u, xx, xx, 0.0, (double*)g, rs);
-      else pfu_element_kernel<double, double, 3><<<nb, kBlock, 0, s>>>(u, xx, xx, 0.0, (double*)g, rs);
+      if (mode == 1) {
+        pfu_chunk_kernel<double, double, 1><<<nb, kBlock, 0, s>>>(u, xx, xx, 0.0, (double*)g, rs);
+      } else {
+        pfu_chunk_kernel<double, double, 3><<<nb, kBlock, 0, s>>>(u, xx, xx, 0.0, (double*)g, rs);
+        if (u.ngather)
+          pfu_gather_kernel<double><<<grid_blocks(u.ngather * u.nf, 1, 8192), kBlock, 0, s>>>(u, (double*)g);
+      }
     } else {
       const float* xx = static_cast<const float*>(x);
-      if (mode == 1) pfu_element_kernel<float, float, 1><<<nb, kBlock, 0, s>>>(u, xx, xx, 0.0f, (float*)g, rs);
-      else pfu_element_kernel<float, float, 3><<<nb, kBlock, 0, s>>>(u, xx, xx, 0.0f, (float*)g, rs);
+      if (mode == 1) {
+        pfu_chunk_kernel<float, float, 1><<<nb, kBlock, 0, s>>>(u, xx, xx, 0.0f, (float*)g, rs);
+      } else {
+        pfu_chunk_kernel<float, float, 3><<<nb, kBlock, 0, s>>>(u, xx, xx, 0.0f, (float*)g, rs);
+        if (u.ngather)
+          pfu_gather_kernel<float><<<grid_blocks(u.ngather * u.nf, 1, 8192), kBlock, 0, s>>>(u, (float*)g);
+      }
     }
   };
   launch();

@@ -2,8 +2,9 @@
 against the oracle, the reference's unit-test known answers
 (test/unit/PhaseFieldUnstructured_test.cpp) and the gather-form PhaseField.
 
-The gradient is accumulated with atomics, so its summation order (and, in the
-last bits, its value) varies from launch to launch.  Tolerances: FP64 within
+Within a chunk of elements the gradient is accumulated with LDS atomics, so
+its summation order (and, in the last bits, its value) varies from launch to
+launch.  Tolerances: FP64 within
 1e-12 relative (gradients norm-wise; energies relative to the sum of the
 magnitudes of the energy terms), FP32 within 1e-5, on identical (for FP32:
 FP32-rounded) coordinates.
