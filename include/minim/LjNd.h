@@ -42,6 +42,7 @@ class LjNd : public NewPotential<LjNd> {
   void init(const vector<double>& coords) override;
   void evaluate(int prec, const void* x, const void* d, double t, double* e, void* g,
                 void* stream) const override;
+  double timeKernel(int prec, const void* x, bool wantE, void* g, int reps, void* stream) const override;
 
   // Statistics of the neighbour list (cutoff mode).
   long listBuilds() const;

@@ -257,27 +257,59 @@ __global__ __launch_bounds__(kBlock) void lj_build_list_brute_kernel(const T* __
   if (cnt > maxn) atomicMax(overflow, cnt);
 }
 
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  using type = float4;
+};
+template <>
+struct Vec4<double> {
+  using type = double4;
+};
+
+// Positions at the evaluated point (x, or x + t*d) padded to four components,
+// so that the force pass reads a neighbour with one vector access.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void lj_pad_kernel(const T* __restrict__ x, const T* __restrict__ d, T t, int N,
+                                                        typename Vec4<T>::type* __restrict__ xp) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= N) return;
+  T p[3];
+  load_pos<T, 3>(x, d, t, i, p);
+  typename Vec4<T>::type v;
+  v.x = p[0];
+  v.y = p[1];
+  v.z = p[2];
+  v.w = (T)0;
+  xp[i] = v;
+}
+
 // One thread per particle over its Verlet list, in chunks of U neighbours whose
 // indices and positions are all requested before any is used (the list rows
 // of a chunk are clamped to the last entry, and the repeats are skipped), so
 // the gathers overlap instead of forming one dependent chain per neighbour.
+// Positions come padded (lj_pad_kernel): one vector load per neighbour.
 // Pairs are summed in list order.  The same pass counts the particles that
 // have moved more than skin/2 since the list was built (reduction slot 1):
 // when any has, the caller rebuilds the list and evaluates again.  Blocks are
 // spread over the XCDs in contiguous ranges, so each L2 holds the positions
-// of one spatially coherent range of particles.
+// of one spatially coherent range of particles.  (Two to eight lanes per
+// particle, each taking every P-th entry, measured no faster at 262,144
+// particles.)
 template <typename T>
-__global__ __launch_bounds__(kBlock) void lj_cut_kernel(const T* __restrict__ x, const T* __restrict__ d, T t, int N,
+__global__ __launch_bounds__(kBlock) void lj_cut_kernel(const typename Vec4<T>::type* __restrict__ xp, int N,
                                                         BoxArgs b, T sig2, T eps, T eshift, int mode,
                                                         const int* __restrict__ nn, const int* __restrict__ nbr,
                                                         const double* __restrict__ xbuild, T* __restrict__ g,
                                                         RedScratch rs) {
   constexpr int U = 8;
+  using V = typename Vec4<T>::type;
   const int i = xcd_logical_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   double ei = 0.0, moved = 0.0;
   if (i < N) {
-    T xi[3];
-    load_pos<T, 3>(x, d, t, i, xi);
+    const V vi = xp[i];
+    const T xi[3] = {vi.x, vi.y, vi.z};
     double dr2 = 0;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -294,15 +326,15 @@ __global__ __launch_bounds__(kBlock) void lj_cut_kernel(const T* __restrict__ x,
       int jj[U];
 #pragma unroll
       for (int u = 0; u < U; u++) jj[u] = nbr[(size_t)min(k0 + u, cnt - 1) * N + i];
-      T xj[U][3];
+      V xj[U];
 #pragma unroll
-      for (int u = 0; u < U; u++) load_pos<T, 3>(x, d, t, jj[u], xj[u]);
+      for (int u = 0; u < U; u++) xj[u] = xp[jj[u]];
 #pragma unroll
       for (int u = 0; u < U; u++) {
         if (k0 + u >= cnt) break;
-        const T r0 = min_image(xi[0] - xj[u][0], L0, iL0);
-        const T r1 = min_image(xi[1] - xj[u][1], L1, iL1);
-        const T r2v = min_image(xi[2] - xj[u][2], L2, iL2);
+        const T r0 = min_image(xi[0] - xj[u].x, L0, iL0);
+        const T r1 = min_image(xi[1] - xj[u].y, L1, iL1);
+        const T r2v = min_image(xi[2] - xj[u].z, L2, iL2);
         const T r2 = r0 * r0 + r1 * r1 + r2v * r2v;
         if (r2 >= rc2) continue;
         const T ir2 = (T)1 / r2;  // one division per pair
@@ -363,7 +395,7 @@ struct LjNd::Impl {
   long builds = 0;
   int lastMax = 0;
   bool built = false;
-  DeviceBuffer cell, count, start, cursor, order, nn, nbr, flag, xbuild;
+  DeviceBuffer cell, count, start, cursor, order, nn, nbr, flag, xbuild, xpad;
   int ncell = 0;
 };
 
@@ -442,8 +474,92 @@ void LjNd::init(const vector<double>& coords) {
     impl_->nbr.reset(sizeof(int) * N * impl_->maxn);
     impl_->flag.reset(sizeof(int));
     impl_->xbuild.reset(sizeof(double) * 3 * N);
+    impl_->xpad.reset(sizeof(double) * 4 * N);
   }
 }
+
+namespace {
+int cut_blocks(int N) { return (N + kBlock - 1) / kBlock; }
+
+void launch_cut(const LjNd::Impl& I, int prec, const void* x, const void* d, double t, int N, double sigma,
+                double epsilon, double eshift, int mode, void* g, minim_amd::RedScratch rs, hipStream_t s) {
+  const int nb = cut_blocks(N);
+  if (prec == FP64) {
+    auto* xp = (double4*)I.xpad.get();
+    minim_amd::lj_pad_kernel<double><<<nb, kBlock, 0, s>>>((const double*)x, (const double*)d, t, N, xp);
+    minim_amd::lj_cut_kernel<double><<<nb, kBlock, 0, s>>>(xp, N, I.box, sigma * sigma, epsilon, eshift, mode,
+                                                           (const int*)I.nn.get(), (const int*)I.nbr.get(),
+                                                           (const double*)I.xbuild.get(), (double*)g, rs);
+  } else {
+    auto* xp = (float4*)I.xpad.get();
+    minim_amd::lj_pad_kernel<float><<<nb, kBlock, 0, s>>>((const float*)x, (const float*)d, (float)t, N, xp);
+    minim_amd::lj_cut_kernel<float><<<nb, kBlock, 0, s>>>(xp, N, I.box, (float)(sigma * sigma), (float)epsilon,
+                                                          (float)eshift, mode, (const int*)I.nn.get(),
+                                                          (const int*)I.nbr.get(), (const double*)I.xbuild.get(),
+                                                          (float*)g, rs);
+  }
+  MINIM_HIP_CHECK(hipGetLastError());
+}
+
+// (Re)builds the Verlet list at the evaluated positions x (+ t*d).
+void rebuild_list(LjNd::Impl& I, int prec, const void* x, const void* d, double t, int N, hipStream_t s) {
+  const int nb = (N + kBlock - 1) / kBlock;
+  // Build at the evaluated positions: materialise x + t*d when needed.
+  DeviceBuffer tmp;
+  const void* xb = x;
+  if (d) {
+    tmp.reset(3 * (size_t)N * minim_amd::prec_bytes(prec));
+    minim_amd::device_copy(tmp.get(), x, 3 * (size_t)N * minim_amd::prec_bytes(prec), s);
+    minim_amd::launch_axpy(prec, tmp.get(), d, t, 3 * (size_t)N, s);
+    xb = tmp.get();
+  }
+  for (int attempt = 0; attempt < 2; attempt++) {
+    MINIM_HIP_CHECK(hipMemsetAsync(I.flag.get(), 0, sizeof(int), s));
+    if (I.cellMode) {
+      MINIM_HIP_CHECK(hipMemsetAsync(I.count.get(), 0, sizeof(int) * I.ncell, s));
+      const int nbc = (I.ncell + kBlock - 1) / kBlock;
+      if (prec == FP64)
+        minim_amd::lj_cell_index_kernel<double><<<nb, kBlock, 0, s>>>((const double*)xb, N, I.box, (int*)I.cell.get(),
+                                                                      (int*)I.count.get(), (double*)I.xbuild.get());
+      else
+        minim_amd::lj_cell_index_kernel<float><<<nb, kBlock, 0, s>>>((const float*)xb, N, I.box, (int*)I.cell.get(),
+                                                                     (int*)I.count.get(), (double*)I.xbuild.get());
+      minim_amd::lj_scan_kernel<<<1, kBlock, 0, s>>>((const int*)I.count.get(), (int*)I.start.get(),
+                                                     (int*)I.cursor.get(), I.ncell);
+      minim_amd::lj_fill_kernel<<<nb, kBlock, 0, s>>>((const int*)I.cell.get(), N, (const int*)I.start.get(),
+                                                      (int*)I.cursor.get(), (int*)I.order.get());
+      minim_amd::lj_sort_cells_kernel<<<nbc, kBlock, 0, s>>>((const int*)I.start.get(), (int*)I.order.get(), I.ncell);
+      if (prec == FP64)
+        minim_amd::lj_build_list_kernel<double><<<nb, kBlock, 0, s>>>(
+            (const double*)xb, N, I.box, (const int*)I.cell.get(), (const int*)I.start.get(),
+            (const int*)I.order.get(), I.maxn, (int*)I.nn.get(), (int*)I.nbr.get(), (int*)I.flag.get());
+      else
+        minim_amd::lj_build_list_kernel<float><<<nb, kBlock, 0, s>>>(
+            (const float*)xb, N, I.box, (const int*)I.cell.get(), (const int*)I.start.get(),
+            (const int*)I.order.get(), I.maxn, (int*)I.nn.get(), (int*)I.nbr.get(), (int*)I.flag.get());
+    } else {
+      if (prec == FP64)
+        minim_amd::lj_build_list_brute_kernel<double><<<nb, kBlock, 0, s>>>(
+            (const double*)xb, N, I.box, I.maxn, (int*)I.nn.get(), (int*)I.nbr.get(), (int*)I.flag.get(),
+            (double*)I.xbuild.get());
+      else
+        minim_amd::lj_build_list_brute_kernel<float><<<nb, kBlock, 0, s>>>(
+            (const float*)xb, N, I.box, I.maxn, (int*)I.nn.get(), (int*)I.nbr.get(), (int*)I.flag.get(),
+            (double*)I.xbuild.get());
+    }
+    MINIM_HIP_CHECK(hipGetLastError());
+    int over = 0;
+    MINIM_HIP_CHECK(hipMemcpyAsync(&over, I.flag.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    MINIM_HIP_CHECK(hipStreamSynchronize(s));
+    if (over == 0) break;
+    if (attempt == 1) throw std::runtime_error("LjNd: neighbour list overflow");
+    I.maxn = over + 32;
+    I.nbr.reset(sizeof(int) * (size_t)N * I.maxn);
+  }
+  I.built = true;
+  I.builds++;
+}
+}  // namespace
 
 void LjNd::evaluate(int prec, const void* x, const void* d, double t, double* e, void* g, void* stream) const {
   if (!impl_) throw std::runtime_error("LjNd: not initialised (construct a State first).");
@@ -457,85 +573,20 @@ void LjNd::evaluate(int prec, const void* x, const void* d, double t, double* e,
     else minim_amd::launch_allpairs<float>(nDim, x, d, t, N, sigma, epsilon, mode, g, ws.rs(), s);
   } else {
     Impl& I = *impl_;
-    const int nb = (N + kBlock - 1) / kBlock;
-    if (nb > minim_amd::kMaxRedBlocks) throw std::invalid_argument("LjNd: too many particles for one launch");
-    auto rebuild_list = [&]() {
-      // Build at the evaluated positions: materialise x + t*d when needed.
-      DeviceBuffer tmp;
-      const void* xb = x;
-      if (d) {
-        tmp.reset(3 * (size_t)N * minim_amd::prec_bytes(prec));
-        minim_amd::device_copy(tmp.get(), x, 3 * (size_t)N * minim_amd::prec_bytes(prec), s);
-        minim_amd::launch_axpy(prec, tmp.get(), d, t, 3 * (size_t)N, s);
-        xb = tmp.get();
-      }
-      for (int attempt = 0; attempt < 2; attempt++) {
-        MINIM_HIP_CHECK(hipMemsetAsync(I.flag.get(), 0, sizeof(int), s));
-        if (I.cellMode) {
-          MINIM_HIP_CHECK(hipMemsetAsync(I.count.get(), 0, sizeof(int) * I.ncell, s));
-          const int nbc = (I.ncell + kBlock - 1) / kBlock;
-          if (prec == FP64)
-            minim_amd::lj_cell_index_kernel<double><<<nb, kBlock, 0, s>>>((const double*)xb, N, I.box, (int*)I.cell.get(),
-                                                                          (int*)I.count.get(), (double*)I.xbuild.get());
-          else
-            minim_amd::lj_cell_index_kernel<float><<<nb, kBlock, 0, s>>>((const float*)xb, N, I.box, (int*)I.cell.get(),
-                                                                         (int*)I.count.get(), (double*)I.xbuild.get());
-          minim_amd::lj_scan_kernel<<<1, kBlock, 0, s>>>((const int*)I.count.get(), (int*)I.start.get(),
-                                                         (int*)I.cursor.get(), I.ncell);
-          minim_amd::lj_fill_kernel<<<nb, kBlock, 0, s>>>((const int*)I.cell.get(), N, (const int*)I.start.get(),
-                                                          (int*)I.cursor.get(), (int*)I.order.get());
-          minim_amd::lj_sort_cells_kernel<<<nbc, kBlock, 0, s>>>((const int*)I.start.get(), (int*)I.order.get(), I.ncell);
-          if (prec == FP64)
-            minim_amd::lj_build_list_kernel<double><<<nb, kBlock, 0, s>>>(
-                (const double*)xb, N, I.box, (const int*)I.cell.get(), (const int*)I.start.get(),
-                (const int*)I.order.get(), I.maxn, (int*)I.nn.get(), (int*)I.nbr.get(), (int*)I.flag.get());
-          else
-            minim_amd::lj_build_list_kernel<float><<<nb, kBlock, 0, s>>>(
-                (const float*)xb, N, I.box, (const int*)I.cell.get(), (const int*)I.start.get(),
-                (const int*)I.order.get(), I.maxn, (int*)I.nn.get(), (int*)I.nbr.get(), (int*)I.flag.get());
-        } else {
-          if (prec == FP64)
-            minim_amd::lj_build_list_brute_kernel<double><<<nb, kBlock, 0, s>>>(
-                (const double*)xb, N, I.box, I.maxn, (int*)I.nn.get(), (int*)I.nbr.get(), (int*)I.flag.get(),
-                (double*)I.xbuild.get());
-          else
-            minim_amd::lj_build_list_brute_kernel<float><<<nb, kBlock, 0, s>>>(
-                (const float*)xb, N, I.box, I.maxn, (int*)I.nn.get(), (int*)I.nbr.get(), (int*)I.flag.get(),
-                (double*)I.xbuild.get());
-        }
-        MINIM_HIP_CHECK(hipGetLastError());
-        int over = 0;
-        MINIM_HIP_CHECK(hipMemcpyAsync(&over, I.flag.get(), sizeof(int), hipMemcpyDeviceToHost, s));
-        MINIM_HIP_CHECK(hipStreamSynchronize(s));
-        if (over == 0) break;
-        if (attempt == 1) throw std::runtime_error("LjNd: neighbour list overflow");
-        I.maxn = over + 32;
-        I.nbr.reset(sizeof(int) * (size_t)N * I.maxn);
-      }
-      I.built = true;
-      I.builds++;
-    };
+    if ((N + kBlock - 1) / kBlock > minim_amd::kMaxRedBlocks)
+      throw std::invalid_argument("LjNd: too many particles for one launch");
     const double src6 = std::pow(sigma / cutoff, 6);
     const double eshift = 4 * epsilon * (src6 * src6 - src6);
     auto launch_force = [&]() {
-      if (prec == FP64)
-        minim_amd::lj_cut_kernel<double><<<nb, kBlock, 0, s>>>(
-            (const double*)x, (const double*)d, t, N, I.box, sigma * sigma, epsilon, eshift, mode,
-            (const int*)I.nn.get(), (const int*)I.nbr.get(), (const double*)I.xbuild.get(), (double*)g, ws.rs());
-      else
-        minim_amd::lj_cut_kernel<float><<<nb, kBlock, 0, s>>>(
-            (const float*)x, (const float*)d, (float)t, N, I.box, (float)(sigma * sigma), (float)epsilon,
-            (float)eshift, mode, (const int*)I.nn.get(), (const int*)I.nbr.get(), (const double*)I.xbuild.get(),
-            (float*)g, ws.rs());
-      MINIM_HIP_CHECK(hipGetLastError());
-      minim_amd::launch_finalize(ws.rs(), 2, nb, s);
+      launch_cut(I, prec, x, d, t, N, sigma, epsilon, eshift, mode, g, ws.rs(), s);
+      minim_amd::launch_finalize(ws.rs(), 2, cut_blocks(N), s);
     };
-    if (!I.built) rebuild_list();
+    if (!I.built) rebuild_list(I, prec, x, d, t, N, s);
     launch_force();
     double r[2];
     ws.fetch(s, 2, r);  // energy, particles moved more than skin/2 since the list was built
     if (r[1] > 0) {
-      rebuild_list();
+      rebuild_list(I, prec, x, d, t, N, s);
       launch_force();
       ws.fetch(s, 2, r);
     }
@@ -549,6 +600,43 @@ void LjNd::evaluate(int prec, const void* x, const void* d, double t, double* e,
     ws.fetch(s, 1, r);
     *e = r[0];
   }
+}
+
+// Average duration (ms) of reps launches of the force kernel (cutoff mode:
+// the list is built first when it does not exist yet; all-pairs mode: the
+// kernel and its reduction).
+double LjNd::timeKernel(int prec, const void* x, bool wantE, void* g, int reps, void* stream) const {
+  if (!impl_) throw std::runtime_error("LjNd: not initialised (construct a State first).");
+  const int mode = (wantE ? 1 : 0) | (g ? 2 : 0);
+  if (mode == 0 || reps < 1) throw std::invalid_argument("LjNd: nothing to time");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  auto& ws = minim_amd::Workspace::current();
+  const int N = nParticle;
+  const double src6 = cutoff > 0 ? std::pow(sigma / cutoff, 6) : 0.0;
+  const double eshift = 4 * epsilon * (src6 * src6 - src6);
+  auto launch = [&]() {
+    if (cutoff > 0) {
+      launch_cut(*impl_, prec, x, nullptr, 0.0, N, sigma, epsilon, eshift, mode, g, ws.rs(), s);
+    } else if (prec == FP64) {
+      minim_amd::launch_allpairs<double>(nDim, x, nullptr, 0.0, N, sigma, epsilon, mode, g, ws.rs(), s);
+    } else {
+      minim_amd::launch_allpairs<float>(nDim, x, nullptr, 0.0, N, sigma, epsilon, mode, g, ws.rs(), s);
+    }
+  };
+  if (cutoff > 0 && !impl_->built) rebuild_list(*impl_, prec, x, nullptr, 0.0, N, s);
+  launch();
+  hipEvent_t e0, e1;
+  MINIM_HIP_CHECK(hipEventCreate(&e0));
+  MINIM_HIP_CHECK(hipEventCreate(&e1));
+  MINIM_HIP_CHECK(hipEventRecord(e0, s));
+  for (int i = 0; i < reps; i++) launch();
+  MINIM_HIP_CHECK(hipEventRecord(e1, s));
+  MINIM_HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0;
+  MINIM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  MINIM_HIP_CHECK(hipEventDestroy(e0));
+  MINIM_HIP_CHECK(hipEventDestroy(e1));
+  return (double)ms / reps;
 }
 
 }  // namespace minim
