@@ -29,6 +29,7 @@ class BarAndHinge : public NewPotential<BarAndHinge> {
   void init(const vector<double>& coords) override;
   void evaluate(int prec, const void* x, const void* d, double t, double* e, void* g,
                 void* stream) const override;
+  double timeKernel(int prec, const void* x, bool wantE, void* g, int reps, void* stream) const override;
 
   // Element lists after init, in the reference's element order.
   vector2d<int> bondList() const { return bonds_; }

@@ -596,37 +596,69 @@ void BarAndHinge::upload() {
   dev_ = dev;
 }
 
-void BarAndHinge::evaluate(int prec, const void* x, const void* d, double t, double* e, void* g,
-                           void* stream) const {
-  if (!dev_) throw std::runtime_error("BarAndHinge: not initialised (construct a State first).");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  auto& ws = minim_amd::Workspace::current();
-  const auto& a = dev_->args;
-  if (e) {
+namespace {
+// Energy kernel (into rs.result[0] when wantE) and gradient kernel (when g).
+void launch_bah(const minim_amd::BaHArgs& a, int prec, const void* x, const void* d, double t, bool wantE, void* g,
+                minim_amd::RedScratch rs, hipStream_t s) {
+  if (wantE) {
     const size_t tot = (size_t)a.nbar + a.nhinge + a.nnode;
     const int nb = minim_amd::grid_blocks(tot, 4, 2048);
     if (prec == FP64)
-      minim_amd::bah_energy_kernel<double><<<nb, kBlock, 0, s>>>(a, (const double*)x, (const double*)d, t, ws.rs());
+      minim_amd::bah_energy_kernel<double><<<nb, kBlock, 0, s>>>(a, (const double*)x, (const double*)d, t, rs);
     else
-      minim_amd::bah_energy_kernel<float><<<nb, kBlock, 0, s>>>(a, (const float*)x, (const float*)d, (float)t, ws.rs());
+      minim_amd::bah_energy_kernel<float><<<nb, kBlock, 0, s>>>(a, (const float*)x, (const float*)d, (float)t, rs);
     MINIM_HIP_CHECK(hipGetLastError());
-    minim_amd::launch_finalize(ws.rs(), 1, nb, s);
+    minim_amd::launch_finalize(rs, 1, nb, s);
   }
   if (g) {
-    if (d) throw std::invalid_argument("BarAndHinge: gradients at a trial point are not supported.");
     const int nb = minim_amd::grid_blocks(a.nnode, 1, 1 << 20);
     if (prec == FP64)
       minim_amd::bah_gradient_kernel<double><<<nb, kBlock, 0, s>>>(a, (const double*)x, (double*)g);
     else
       minim_amd::bah_gradient_kernel<float><<<nb, kBlock, 0, s>>>(a, (const float*)x, (float*)g);
     MINIM_HIP_CHECK(hipGetLastError());
-    applyConstraints(prec, g, 3 * (size_t)a.nnode, stream);
+  }
+}
+}  // namespace
+
+void BarAndHinge::evaluate(int prec, const void* x, const void* d, double t, double* e, void* g,
+                           void* stream) const {
+  if (!dev_) throw std::runtime_error("BarAndHinge: not initialised (construct a State first).");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  auto& ws = minim_amd::Workspace::current();
+  const auto& a = dev_->args;
+  if (g && d) throw std::invalid_argument("BarAndHinge: gradients at a trial point are not supported.");
+  if (e || g) {
+    launch_bah(dev_->args, prec, x, d, t, e != nullptr, g, ws.rs(), s);
+    if (g) applyConstraints(prec, g, 3 * (size_t)a.nnode, stream);
   }
   if (e) {
     double r[1];
     ws.fetch(s, 1, r);
     *e = r[0];
   }
+}
+
+// Average duration (ms) of reps evaluations on the device (energy kernel with
+// its reduction, gradient kernel; no constraints, no host copy).
+double BarAndHinge::timeKernel(int prec, const void* x, bool wantE, void* g, int reps, void* stream) const {
+  if (!dev_) throw std::runtime_error("BarAndHinge: not initialised (construct a State first).");
+  if ((!wantE && !g) || reps < 1) throw std::invalid_argument("BarAndHinge: nothing to time");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  auto& ws = minim_amd::Workspace::current();
+  launch_bah(dev_->args, prec, x, nullptr, 0.0, wantE, g, ws.rs(), s);
+  hipEvent_t e0, e1;
+  MINIM_HIP_CHECK(hipEventCreate(&e0));
+  MINIM_HIP_CHECK(hipEventCreate(&e1));
+  MINIM_HIP_CHECK(hipEventRecord(e0, s));
+  for (int i = 0; i < reps; i++) launch_bah(dev_->args, prec, x, nullptr, 0.0, wantE, g, ws.rs(), s);
+  MINIM_HIP_CHECK(hipEventRecord(e1, s));
+  MINIM_HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0;
+  MINIM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  MINIM_HIP_CHECK(hipEventDestroy(e0));
+  MINIM_HIP_CHECK(hipEventDestroy(e1));
+  return (double)ms / reps;
 }
 
 }  // namespace minim
