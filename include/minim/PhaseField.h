@@ -80,6 +80,9 @@ class PhaseField : public NewPotential<PhaseField> {
   void evaluateAsync(int prec, const void* x, const void* d, double t, bool wantE, double* e_dev, void* g,
                      void* stream) const override;
   bool supportsMixed() const override;
+  bool supportsStep() const override;
+  void evaluateStepAsync(int px, int pv, const void* x, const void* d, double t, void* xout, void* g,
+                         void* stream) const override;
   void evaluateMixed(const void* x, const void* d, double t, double* e, void* g, void* stream) const override;
   void evaluateMixedAsync(const void* x, const void* d, double t, bool wantE, double* e_dev, void* g,
                           void* stream) const override;

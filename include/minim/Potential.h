@@ -80,6 +80,15 @@ class Potential {
   // whose constraints are only fixed DOFs report nf = 1, fixmask = 0: their
   // basis vectors already hold exact zeros there.
   virtual SiteProjection projection() const { return SiteProjection{}; }
+  // Fused accepted step: the energy (left on the device where evaluateAsync
+  // leaves it) and the constrained gradient g at x + t*d, with x + t*d also
+  // written to xout (precision px, like x); d and g have precision pv (px, or
+  // FP32 against FP64 coordinates).  Potentials that implement it return true
+  // from supportsStep(); the default evaluateStepAsync throws
+  // std::invalid_argument.
+  virtual bool supportsStep() const { return false; }
+  virtual void evaluateStepAsync(int px, int pv, const void* x, const void* d, double t, void* xout, void* g,
+                                 void* stream) const;
   // True when applyConstraints also couples all DOFs (PhaseField's fixed
   // total volumes, PhaseField.cpp:614-658): minimisers then apply it to each
   // step in addition to the per-site projection.

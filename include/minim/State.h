@@ -75,6 +75,11 @@ class State {
   void invalidateHost() { hostValid_ = false; }
   // incremented whenever coordinates are replaced from outside a minimiser
   unsigned long coordsVersion() const { return version_; }
+  // A device buffer of the coordinates' size and precision, and the exchange
+  // of the coordinates with such a buffer (a minimiser's accepted step written
+  // by a fused evaluation; deviceCoords() changes).
+  std::shared_ptr<void> newCoordsBuffer() const;
+  void swapDeviceCoords(std::shared_ptr<void>& buffer);
 
  private:
   int prec_ = FP64;

@@ -170,6 +170,15 @@ int minim_potential_eval_async(const minim_potential* pot, int precision, const 
  * (PhaseField with nFluid <= 4 has it). */
 int minim_potential_eval_mixed(const minim_potential* pot, const void* x, size_t ndof, const void* d, double t,
                                double* e, void* g, void* stream);
+/* The fused accepted step of this library's L-BFGS line search (no
+ * reference counterpart: it replaces the reference's separate trial energy,
+ * `x += step` and gradient, linesearch.cpp:13-25 and Lbfgs.cpp:67): energy
+ * (host, may be NULL) and constrained gradient g at x + t*d, with x + t*d
+ * written to x_out (precision_x, like x; d and g in precision_v).
+ * MINIM_EINVAL for potentials without it (PhaseField on the tiled kernel has
+ * it: nFluid <= 4, no solid nodes or extra terms).  Synchronises the stream. */
+int minim_potential_eval_step(const minim_potential* pot, int precision_x, int precision_v, const void* x,
+                              size_t ndof, const void* d, double t, void* x_out, double* e, void* g, void* stream);
 /* Benchmark support (no reference counterpart): average duration in ms,
  * measured with HIP events on stream, of reps back-to-back launches of the
  * evaluation kernel alone (energy when want_energy, gradient into g when g

@@ -75,6 +75,7 @@ SIGNATURES = {
     "minim_potential_eval": (C.c_int, [vp, C.c_int, vp, C.c_size_t, vp, C.c_double, dp, vp, vp]),
     "minim_potential_eval_async": (C.c_int, [vp, C.c_int, vp, C.c_size_t, vp, C.c_double, C.c_int, vp, vp, vp]),
     "minim_potential_eval_mixed": (C.c_int, [vp, vp, C.c_size_t, vp, C.c_double, dp, vp, vp]),
+    "minim_potential_eval_step": (C.c_int, [vp, C.c_int, C.c_int, vp, C.c_size_t, vp, C.c_double, vp, dp, vp, vp]),
     "minim_potential_time_kernel": (C.c_int, [vp, C.c_int, vp, C.c_size_t, C.c_int, vp, C.c_int, vp, dp]),
     "minim_potential_constrain": (C.c_int, [vp, C.c_int, vp, C.c_size_t, vp]),
     "minim_dot": (C.c_int, [C.c_int, vp, vp, C.c_size_t, dp, vp]),

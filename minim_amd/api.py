@@ -854,6 +854,14 @@ def potential_eval_mixed(handle, x_ptr, ndof, g_ptr=None, want_energy=True, d_pt
     return e.value if want_energy else None
 
 
+def potential_eval_step(handle, precision_x, precision_v, x_ptr, ndof, d_ptr, t, xout_ptr, g_ptr, stream=None):
+    """Fused accepted step: energy and gradient at x + t*d, x + t*d written to xout."""
+    e = C.c_double()
+    A.check(A.load().minim_potential_eval_step(handle.ptr, _PREC[precision_x], _PREC[precision_v], x_ptr, ndof, d_ptr,
+                                               float(t), xout_ptr, C.byref(e), g_ptr, stream))
+    return e.value
+
+
 def potential_time_kernel(handle, precision, x_ptr, ndof, g_ptr=None, want_energy=True, reps=20, stream=None):
     """Average ms per launch of the evaluation kernel alone (HIP events)."""
     ms = C.c_double()

@@ -392,6 +392,30 @@ int minim_potential_eval_mixed(const minim_potential* pot, const void* x, size_t
   });
 }
 
+int minim_potential_eval_step(const minim_potential* pot, int precision_x, int precision_v, const void* x,
+                              size_t ndof, const void* d, double t, void* x_out, double* e, void* g, void* stream) {
+  return guard([&] {
+    need(pot, "pot");
+    need(x, "x");
+    need(d, "d");
+    need(x_out, "x_out");
+    need(g, "g");
+    check_prec(precision_x);
+    check_prec(precision_v);
+    if (!pot->initialised) throw std::runtime_error("potential not initialised: call minim_potential_init");
+    if (ndof != pot->ndof) throw std::invalid_argument("ndof does not match the initialised potential");
+    if (!pot->p->supportsStep()) throw std::invalid_argument("this potential has no fused step evaluation");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    pot->p->evaluateStepAsync(precision_x, precision_v, x, d, t, x_out, g, stream);
+    if (e) {
+      auto& ws = minim_amd::Workspace::current();
+      ws.fetch(s, 1, e);
+    } else {
+      MINIM_HIP_CHECK(hipStreamSynchronize(s));
+    }
+  });
+}
+
 int minim_potential_time_kernel(const minim_potential* pot, int precision, const void* x, size_t ndof,
                                 int want_energy, void* g, int reps, void* stream, double* ms_per_launch) {
   return guard([&] {

@@ -139,10 +139,13 @@ void energy_async_at(State& s, int pv, const void* d, double t) {
 // backtrackingLinesearch (linesearch.cpp:8-29) along step = tau0 * d, with d a
 // device vector of precision pv.  Trials are evaluated at x + tau*d without
 // materialising them; tau halves exactly like step *= 0.5.  firstTrial, when
-// given, is the energy already evaluated at x + tau0*d.  Returns the tau
-// applied to x.
+// given, is the energy already evaluated at x + tau0*d; xTrial, when given,
+// holds that point itself (written by a fused evaluation): if the first trial
+// is accepted it becomes the coordinates instead of an axpy, and *usedTrial
+// says so.  Returns the tau applied to x.
 double backtrack(State& s, int pv, const void* d, double tau0, double gs, EnergyCache& ec, MinimiseStats& st,
-                 bool* acceptedOut = nullptr, const double* firstTrial = nullptr) {
+                 bool* acceptedOut = nullptr, const double* firstTrial = nullptr,
+                 std::shared_ptr<void>* xTrial = nullptr, bool* usedTrial = nullptr) {
   const double c = 0.5, shrink = 0.5;
   double t = -c * gs;
   double e0;
@@ -154,7 +157,7 @@ double backtrack(State& s, int pv, const void* d, double tau0, double gs, Energy
     st.nEnergy++;
   }
   double tau = tau0;
-  bool accepted = false;
+  bool accepted = false, fromTrial = false;
   double e = 0;
   for (int i = 0; i < 10; i++) {
     if (i == 0 && firstTrial) {
@@ -165,12 +168,17 @@ double backtrack(State& s, int pv, const void* d, double tau0, double gs, Energy
     }
     if (e0 - e >= t) {
       accepted = true;
+      fromTrial = i == 0 && firstTrial && xTrial;
       break;
     }
     tau = tau * shrink;
     t = t * shrink;
   }
-  minim_amd::launch_axpy_mixed(s.precision(), s.deviceCoords(), pv, d, tau, s.ndof, hs(s));
+  if (fromTrial)
+    s.swapDeviceCoords(*xTrial);  // x + tau0*d, bit for bit what the axpy below computes
+  else
+    minim_amd::launch_axpy_mixed(s.precision(), s.deviceCoords(), pv, d, tau, s.ndof, hs(s));
+  if (usedTrial) *usedTrial = fromTrial;
   s.invalidateHost();
   if (accepted) {
     ec.set(s, e);
@@ -220,6 +228,7 @@ struct Lbfgs::Impl {
   EnergyCache ec;
   bool fresh = true;  // next iteration starts from a new gradient
   int stall = 0;
+  std::shared_ptr<void> xtrial;  // coordinates-sized buffer for the fused first trial
 
   // Empty history; device buffers are kept from the previous call when they
   // are large enough and of the same precision (a minimise on the same or a
@@ -425,14 +434,28 @@ void Lbfgs::iteration(State& state) {
   // Without a maxStep cap the line search starts at x + d (tau = 1) unless d
   // must be flipped (rare).  Launch that trial energy now, after stashing the
   // direction pass's dots, so that one host synchronisation returns both.
+  // When the potential can, that trial is the fused step: energy and gradient
+  // at x + d, and x + d itself into a spare buffer, in one pass; if the trial
+  // is accepted the spare becomes the coordinates and the gradient is there.
   std::vector<double> dres(1 + a.ndot);
   const bool spec = linesearch == "backtracking" && (P == 0 || _maxStep == 0);
+  const bool fused = spec && state.pot->supportsStep();
   double eSpec = 0;
   if (spec) {
     constexpr int kStash = 48;  // result slots [48, 48 + 1 + ndot) hold the direction dots
     double* res = ws.device_result();
     MINIM_HIP_CHECK(hipMemcpyAsync(res + kStash, res, sizeof(double) * (1 + a.ndot), hipMemcpyDeviceToDevice, s));
-    energy_async_at(state, prec, a.out, 1.0);
+    if (fused) {
+      const auto* xt = static_cast<const DVec*>(I.xtrial.get());
+      if (!xt || xt->n != n || xt->prec != state.precision()) {
+        I.xtrial = state.newCoordsBuffer();
+        xt = static_cast<const DVec*>(I.xtrial.get());
+      }
+      state.pot->evaluateStepAsync(state.precision(), prec, state.deviceCoords(), a.out, 1.0, xt->data(),
+                                   I.gnew.data(), s);
+    } else {
+      energy_async_at(state, prec, a.out, 1.0);
+    }
     std::vector<double> r(kStash + 1 + a.ndot);
     ws.fetch(s, kStash + 1 + a.ndot, r.data());
     eSpec = r[0];
@@ -455,10 +478,12 @@ void Lbfgs::iteration(State& state) {
   }
 
   double tau;
-  bool lsFailed = false;
+  bool lsFailed = false, haveGradient = false;
   if (linesearch == "backtracking") {
     bool acc = true;
-    tau = backtrack(state, prec, a.out, tau0, gs, I.ec, stats, &acc, (spec && tau0 == 1.0) ? &eSpec : nullptr);
+    const bool first = spec && tau0 == 1.0;
+    tau = backtrack(state, prec, a.out, tau0, gs, I.ec, stats, &acc, first ? &eSpec : nullptr,
+                    (first && fused) ? &I.xtrial : nullptr, &haveGradient);
     lsFailed = !acc;
   } else {
     minim_amd::launch_axpy_mixed(state.precision(), state.deviceCoords(), prec, a.out, tau0, n, s);
@@ -468,7 +493,10 @@ void Lbfgs::iteration(State& state) {
   }
 
   // ---- new gradient and Gram update (Lbfgs.cpp:67-80)
-  gradient_into(state, prec, I.gnew.data(), stats);
+  if (haveGradient)
+    stats.nGradient++;  // computed with the accepted trial point
+  else
+    gradient_into(state, prec, I.gnew.data(), stats);
   minim_amd::GramArgs ga{};
   ga.gnew = I.gnew.data();
   ga.gold = I.g.data();

@@ -206,12 +206,14 @@ __device__ __forceinline__ void resolve_plane(PlaneIn<T, TV, NF, R, DIR>& p, T t
   }
 }
 
-// MODE bits: 1 = energy, 2 = gradient (+ g.g), 4 = trial point x + t*d.
+// MODE bits: 1 = energy, 2 = gradient (+ g.g), 4 = trial point x + t*d,
+// 8 = the trial point also written to xo (an accepted line-search step and
+// the gradient there in one pass).
 template <typename T, typename TV, int NF, int DM, int MODE, int R>
 __global__ __launch_bounds__(kBlock) void pf_tile_kernel(PFArgs a, TileGeom G, const T* __restrict__ x,
                                                          const TV* __restrict__ d, T t, TV* __restrict__ g,
-                                                         RedScratch rs) {
-  constexpr bool WE = MODE & 1, WG = MODE & 2, DIR = MODE & 4;
+                                                         RedScratch rs, T* __restrict__ xo) {
+  constexpr bool WE = MODE & 1, WG = MODE & 2, DIR = MODE & 4, XW = (MODE & 8) && DIR;
   constexpr int LH = kTileWaves * R + 2, LW = kTileZ + 2;
   __shared__ T lds[2][LH][LW][NF];
 
@@ -270,6 +272,7 @@ __global__ __launch_bounds__(kBlock) void pf_tile_kernel(PFArgs a, TileGeom G, c
     load_plane<T, TV, NF, R, DIR>(x + sq, d + sq, ox, od, hox, hod, q);
 
     TV* gp = g + (size_t)i * plane;
+    T* xop = xo + (size_t)i * plane;
     T eacc = (T)0, gacc = (T)0;
 #pragma unroll
     for (int r = 0; r < R; r++) {
@@ -283,6 +286,7 @@ __global__ __launch_bounds__(kBlock) void pf_tile_kernel(PFArgs a, TileGeom G, c
       }
       site_tile<T, NF, DM, WE, WG>(a, B[r], A[r], Cn[r], ym, yp, zm, zp, es, gv);
       if (th.own & (1u << r)) {
+        if (XW) st_site<T, NF>(xop, ox[r], B[r]);
         if (WE) eacc = eacc + es;
         if (WG) {
           TV gs[NF];
@@ -481,42 +485,45 @@ TileGeom tile_geom_for(const PFArgs& a) {
 
 template <typename T, typename TV, int NF, int DM, int MODE>
 void launch_tile(const PFArgs& a, const TileGeom& G, const void* x, const void* d, double t, void* g, RedScratch rs,
-                 hipStream_t s) {
+                 hipStream_t s, void* xo) {
   constexpr int R = tile_rows<T, NF>();
-  // without a direction d is never read; without a gradient g is never written
+  // without a direction d is never read; without a gradient g is never written;
+  // without mode 8 xo is never written
   const TV* dd = static_cast<const TV*>((MODE & 4) ? d : x);
   TV* gg = static_cast<TV*>((MODE & 2) ? g : const_cast<void*>(x));
+  T* xx = static_cast<T*>((MODE & 8) ? xo : const_cast<void*>(x));
   pf_tile_kernel<T, TV, NF, DM, MODE, R><<<G.nblocks, kBlock, 0, s>>>(a, G, static_cast<const T*>(x), dd, (T)t, gg,
-                                                                      rs);
+                                                                      rs, xx);
 }
 
 template <typename T, typename TV, int NF, int MODE>
 void launch_tile_dm(const PFArgs& a, const TileGeom& G, const void* x, const void* d, double t, void* g,
-                    RedScratch rs, hipStream_t s) {
+                    RedScratch rs, hipStream_t s, void* xo) {
   const int dm = density_mode(a);
   if constexpr (NF == 1) {
-    launch_tile<T, TV, NF, DM_NONE, MODE>(a, G, x, d, t, g, rs, s);
+    launch_tile<T, TV, NF, DM_NONE, MODE>(a, G, x, d, t, g, rs, s, xo);
   } else {
     if (dm == DM_SOFT) {
-      launch_tile<T, TV, NF, DM_SOFT, MODE>(a, G, x, d, t, g, rs, s);
+      launch_tile<T, TV, NF, DM_SOFT, MODE>(a, G, x, d, t, g, rs, s, xo);
     } else if constexpr ((MODE & 2) != 0) {
-      if (dm == DM_HARD) launch_tile<T, TV, NF, DM_HARD, MODE>(a, G, x, d, t, g, rs, s);
-      else launch_tile<T, TV, NF, DM_NONE, MODE>(a, G, x, d, t, g, rs, s);
+      if (dm == DM_HARD) launch_tile<T, TV, NF, DM_HARD, MODE>(a, G, x, d, t, g, rs, s, xo);
+      else launch_tile<T, TV, NF, DM_NONE, MODE>(a, G, x, d, t, g, rs, s, xo);
     } else {
-      launch_tile<T, TV, NF, DM_NONE, MODE>(a, G, x, d, t, g, rs, s);  // the projection acts on g only
+      launch_tile<T, TV, NF, DM_NONE, MODE>(a, G, x, d, t, g, rs, s, xo);  // the projection acts on g only
     }
   }
 }
 
 template <typename T, typename TV, int NF>
 int launch_tile_nf(const PFArgs& a, int mode, const void* x, const void* d, double t, void* g, RedScratch rs,
-                   hipStream_t s) {
+                   hipStream_t s, void* xo) {
   const TileGeom G = tile_geom_for<T, TV, NF>(a);
   switch (mode) {
-    case 1: launch_tile_dm<T, T, NF, 1>(a, G, x, d, t, g, rs, s); break;  // no vector is touched
-    case 2: launch_tile_dm<T, TV, NF, 2>(a, G, x, d, t, g, rs, s); break;
-    case 3: launch_tile_dm<T, TV, NF, 3>(a, G, x, d, t, g, rs, s); break;
-    case 5: launch_tile_dm<T, TV, NF, 5>(a, G, x, d, t, g, rs, s); break;
+    case 1: launch_tile_dm<T, T, NF, 1>(a, G, x, d, t, g, rs, s, xo); break;  // no vector is touched
+    case 2: launch_tile_dm<T, TV, NF, 2>(a, G, x, d, t, g, rs, s, xo); break;
+    case 3: launch_tile_dm<T, TV, NF, 3>(a, G, x, d, t, g, rs, s, xo); break;
+    case 5: launch_tile_dm<T, TV, NF, 5>(a, G, x, d, t, g, rs, s, xo); break;
+    case 15: launch_tile_dm<T, TV, NF, 15>(a, G, x, d, t, g, rs, s, xo); break;
     default: throw std::invalid_argument("pf: unsupported evaluation mode");
   }
   return G.nblocks;
@@ -524,14 +531,15 @@ int launch_tile_nf(const PFArgs& a, int mode, const void* x, const void* d, doub
 
 template <typename T, typename TV>
 int launch_eval_t(const PFArgs& a, int mode, const void* x, const void* d, double t, void* g, RedScratch rs,
-                  hipStream_t s) {
+                  hipStream_t s, void* xo) {
   switch (a.nf) {
-    case 1: return launch_tile_nf<T, TV, 1>(a, mode, x, d, t, g, rs, s);
-    case 2: return launch_tile_nf<T, TV, 2>(a, mode, x, d, t, g, rs, s);
-    case 3: return launch_tile_nf<T, TV, 3>(a, mode, x, d, t, g, rs, s);
-    case 4: return launch_tile_nf<T, TV, 4>(a, mode, x, d, t, g, rs, s);
+    case 1: return launch_tile_nf<T, TV, 1>(a, mode, x, d, t, g, rs, s, xo);
+    case 2: return launch_tile_nf<T, TV, 2>(a, mode, x, d, t, g, rs, s, xo);
+    case 3: return launch_tile_nf<T, TV, 3>(a, mode, x, d, t, g, rs, s, xo);
+    case 4: return launch_tile_nf<T, TV, 4>(a, mode, x, d, t, g, rs, s, xo);
     default: break;
   }
+  if (mode & 8) throw std::invalid_argument("PhaseField: the fused step needs nFluid <= 4");
   if constexpr (sizeof(T) != sizeof(TV)) {
     throw std::invalid_argument("PhaseField: mixed precision needs nFluid <= 4");
   } else {
@@ -613,18 +621,24 @@ void pf_args_finalize(PFArgs& a, int target_blocks) {
 }
 
 void pf_launch_eval(const PFArgs& a, int prec, int pv, int mode, const void* x, const void* d, double t,
-                    void* g, RedScratch rs, hipStream_t s, bool finalize) {
+                    void* g, RedScratch rs, hipStream_t s, bool finalize, void* xout) {
   int nb;
+  if (xout) {
+    if (mode != 7 || a.use_ext || a.volfix)
+      throw std::invalid_argument("PhaseField: the fused step needs energy, gradient and a direction, and the "
+                                  "tiled kernel");
+    mode |= 8;
+  }
   if (a.use_ext) {
     PFCall c{};
     c.terms = kTermsEnergy;
     nb = pf_launch_ext(a, c, prec, pv, mode, x, d, t, g, rs, s);
   } else if (prec == F64 && pv == F64)
-    nb = launch_eval_t<double, double>(a, mode, x, d, t, g, rs, s);
+    nb = launch_eval_t<double, double>(a, mode, x, d, t, g, rs, s, xout);
   else if (prec == F32 && pv == F32)
-    nb = launch_eval_t<float, float>(a, mode, x, d, t, g, rs, s);
+    nb = launch_eval_t<float, float>(a, mode, x, d, t, g, rs, s, xout);
   else if (prec == F64 && pv == F32)
-    nb = launch_eval_t<double, float>(a, mode, x, d, t, g, rs, s);
+    nb = launch_eval_t<double, float>(a, mode, x, d, t, g, rs, s, xout);
   else
     throw std::invalid_argument("PhaseField: FP32 coordinates with FP64 vectors are not supported");
   MINIM_HIP_CHECK(hipGetLastError());

@@ -106,8 +106,10 @@ void pf_args_finalize(PFArgs& a, int target_blocks);
 // with FP64 coordinates).  Results: rs.result[0] = energy, [1] = g.g (device).
 // Runs the general kernel when a.use_ext; with fixed volumes and a gradient
 // the volume projection follows (not when finalize is false: timing only).
+// With xout (mode 7, tiled kernel only), the point x + t*d is also written to
+// xout (precision prec): an accepted step and its gradient in one pass.
 void pf_launch_eval(const PFArgs& a, int prec, int pv, int mode, const void* x, const void* d, double t,
-                    void* g, RedScratch rs, hipStream_t s, bool finalize = true);
+                    void* g, RedScratch rs, hipStream_t s, bool finalize = true, void* xout = nullptr);
 // Potential::applyConstraints: fixed fluids, hard density projection, then
 // (fixed volumes) the global projection, which uses rs as
 // pf_launch_volume_projection does.

@@ -647,6 +647,22 @@ void PhaseField::evaluateMixedAsync(const void* x, const void* d, double t, bool
   pf_eval_async(args_of(args_), FP64, FP32, x, d, t, wantE, e_dev, g, stream);
 }
 
+// The tiled kernel with the trial point written out (no general kernel, no
+// fixed volumes: their projection needs the whole gradient first).
+bool PhaseField::supportsStep() const {
+  if (!args_) return false;
+  const PFArgs& a = args_of(args_);
+  return !a.use_ext && !a.volfix && a.nf <= 4;
+}
+
+void PhaseField::evaluateStepAsync(int px, int pv, const void* x, const void* d, double t, void* xout, void* g,
+                                   void* stream) const {
+  if (!supportsStep()) throw std::invalid_argument("PhaseField: no fused step for this configuration");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  auto& ws = minim_amd::Workspace::current();
+  minim_amd::pf_launch_eval(args_of(args_), px, pv, 7, x, d, t, g, ws.rs(), s, true, xout);
+}
+
 void PhaseField::applyConstraints(int prec, void* v, size_t n, void* stream) const {
   const PFArgs& a = args_of(args_);
   auto& ws = minim_amd::Workspace::current();

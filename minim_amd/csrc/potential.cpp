@@ -45,6 +45,10 @@ void Potential::applyConstraints(int prec, void* v, size_t n, void* stream) cons
                                static_cast<hipStream_t>(stream));
 }
 
+void Potential::evaluateStepAsync(int, int, const void*, const void*, double, void*, void*, void*) const {
+  throw std::invalid_argument("this potential has no fused step evaluation");
+}
+
 double Potential::timeKernel(int, const void*, bool, void*, int, void*) const {
   throw std::invalid_argument("Potential: kernel timing is not available for this potential.");
 }

@@ -67,6 +67,15 @@ State::~State() = default;
 
 void* State::deviceCoords() const { return dv(x_).data(); }
 
+std::shared_ptr<void> State::newCoordsBuffer() const { return make_dvec(ndof, prec_); }
+
+void State::swapDeviceCoords(std::shared_ptr<void>& buffer) {
+  if (!buffer || dv(buffer).n != dv(x_).n || dv(buffer).prec != dv(x_).prec)
+    throw std::invalid_argument("State: coordinate buffer of another size or precision");
+  std::swap(x_, buffer);
+  hostValid_ = false;
+}
+
 void State::setPrecision(int precision) {
   if (precision == prec_) return;
   if (precision != FP64 && precision != FP32) throw std::invalid_argument("State: precision must be FP64 or FP32.");

@@ -219,6 +219,60 @@ def test_mixed_storage_matches_fp64(gpu, shape):
     assert rel_err(g32.double().cpu().numpy(), o.gradient(x0)) <= 1e-7
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
+@pytest.mark.parametrize("shape", range(len(MIXED_SHAPES)))
+def test_fused_step_equals_separate_passes(gpu, shape, prec):
+    """The L-BFGS line search's fused accepted step (one pass: energy and
+    gradient at x + t*d, and x + t*d written out) equals, bit for bit, the
+    separate trial energy, the axpy and the gradient pass at the materialised
+    point it replaces."""
+    import torch
+
+    m = gpu
+    grid, nf, st, iw, dens = MIXED_SHAPES[shape]
+    print("case fused", grid, nf, dens, prec, flush=True)
+    rng = np.random.default_rng(700 + shape)
+    n = int(np.prod(grid)) * nf
+    x0 = rng.uniform(-1.1, 1.1, n) if nf == 1 else rng.uniform(-0.1, 1.1, n)
+    h = make_pf(m, grid, nf, st, iw, dens).handle(x0)
+    px = "fp32" if prec == "fp32" else "fp64"
+    pv = "fp64" if prec == "fp64" else "fp32"
+    tx = torch.float32 if px == "fp32" else torch.float64
+    tv = torch.float32 if pv == "fp32" else torch.float64
+    x = torch.tensor(x0, dtype=tx, device="cuda")
+    d = torch.tensor(rng.normal(0, 0.1, n), dtype=tv, device="cuda")
+    t = 0.75
+    xo = torch.empty_like(x)
+    g = torch.empty(n, dtype=tv, device="cuda")
+    e = m.potential_eval_step(h, px, pv, x.data_ptr(), n, d.data_ptr(), t, xo.data_ptr(), g.data_ptr())
+    xt = x + torch.tensor(t, dtype=tx, device="cuda") * d.to(tx)
+    gref = torch.empty(n, dtype=tv, device="cuda")
+    if prec == "mixed":
+        eref = m.potential_eval_mixed(h, x.data_ptr(), n, d_ptr=d.data_ptr(), t=t)
+        m.potential_eval_mixed(h, xt.data_ptr(), n, g_ptr=gref.data_ptr(), want_energy=False)
+    else:
+        eref = m.potential_eval(h, px, x.data_ptr(), n, d_ptr=d.data_ptr(), t=t)
+        m.potential_eval(h, px, xt.data_ptr(), n, g_ptr=gref.data_ptr(), want_energy=False)
+    torch.cuda.synchronize()
+    assert torch.equal(xo, xt)
+    assert torch.equal(g, gref)
+    assert e == eref
+
+
+def test_fused_step_rejects_general_kernel(gpu):
+    import torch
+
+    m = gpu
+    grid, nf = (6, 5, 4), 2
+    n = int(np.prod(grid)) * nf
+    pf = make_pf(m, grid, nf, [1.0], [1.0], "hard")
+    pf.stencilKernel = 1
+    h = pf.handle(np.full(n, 0.5))
+    x = torch.full((n,), 0.5, dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError):
+        m.potential_eval_step(h, "fp64", "fp64", x.data_ptr(), n, x.data_ptr(), 1.0, x.data_ptr(), x.data_ptr())
+
+
 def test_mixed_storage_rejects_five_fluids(gpu):
     import torch
 
