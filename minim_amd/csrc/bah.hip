@@ -76,10 +76,10 @@ struct HingeGeo {
   T b2m, n1sq, n2sq, dtheta;
 };
 
-// Hinge geometry from the four node positions and the rest angle.
+// Hinge geometry from the four node positions, without the angle.
 template <typename T>
-__device__ inline void hinge_geo_at(const T (&x1)[3], const T (&x2)[3], const T (&x3)[3], const T (&x4)[3], T t0,
-                                    HingeGeo<T>& G) {
+__device__ inline void hinge_geo_vec(const T (&x1)[3], const T (&x2)[3], const T (&x3)[3], const T (&x4)[3],
+                                     HingeGeo<T>& G) {
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     G.b1[k] = x2[k] - x1[k];
@@ -91,6 +91,11 @@ __device__ inline void hinge_geo_at(const T (&x1)[3], const T (&x2)[3], const T 
   cross(G.b2, G.b3, G.n2);
   G.n1sq = dot3(G.n1, G.n1);
   G.n2sq = dot3(G.n2, G.n2);
+}
+
+// The bending angle's deviation from the rest angle t0.
+template <typename T>
+__device__ inline void hinge_dtheta(HingeGeo<T>& G, T t0) {
   const T n1b3 = dot3(G.n1, G.b3);
   const T pi = (T)3.14159265358979323846;
   T theta;
@@ -112,6 +117,14 @@ __device__ inline void hinge_geo_at(const T (&x1)[3], const T (&x2)[3], const T 
   if (a < (T)2 * b2 && a > -(T)2 * b2) r = a >= b2 ? a - b2 : (a <= -b2 ? a + b2 : a);
   else r = fmod(a, b2);
   G.dtheta = r - pi;
+}
+
+// Hinge geometry from the four node positions and the rest angle.
+template <typename T>
+__device__ inline void hinge_geo_at(const T (&x1)[3], const T (&x2)[3], const T (&x3)[3], const T (&x4)[3], T t0,
+                                    HingeGeo<T>& G) {
+  hinge_geo_vec(x1, x2, x3, x4, G);
+  hinge_dtheta(G, t0);
 }
 
 // Node positions and rest angle of hinge h (one int4 of node ids).
@@ -161,9 +174,12 @@ __device__ inline void hinge_grad_at(const HingeGeo<T>& G, T k, int loc, T (&gn)
 
 // Energy: bars, then hinges, then the external force, each a grid-stride
 // loop that takes U elements per step and issues all their loads first.
-template <typename T>
+// With SD, each hinge's angle deviation also goes to dth (for the gradient
+// kernel, which then needs no inverse trigonometric function).
+template <typename T, bool SD>
 __global__ __launch_bounds__(kBlock) void bah_energy_kernel(BaHArgs a, const T* __restrict__ x,
-                                                            const T* __restrict__ d, T t, RedScratch rs) {
+                                                            const T* __restrict__ d, T t, RedScratch rs,
+                                                            T* __restrict__ dth) {
   constexpr int U = 4;
   double acc[1] = {0.0};
   const size_t stride = (size_t)gridDim.x * kBlock;
@@ -203,6 +219,7 @@ __global__ __launch_bounds__(kBlock) void bah_energy_kernel(BaHArgs a, const T* 
       HingeGeo<T> G;
       hinge_geo_at(H[u].x1, H[u].x2, H[u].x3, H[u].x4, H[u].t0, G);
       acc[0] += (double)((T)0.5 * H[u].k * (G.dtheta * G.dtheta));
+      if (SD) dth[w0 + u * stride] = G.dtheta;
     }
   }
   if (a.fz) {
@@ -217,12 +234,26 @@ __global__ __launch_bounds__(kBlock) void bah_energy_kernel(BaHArgs a, const T* 
   grid_sum_store<1>(acc, rs);
 }
 
+// The angle deviation of every hinge (gradient without energy).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bah_dtheta_kernel(BaHArgs a, const T* __restrict__ x,
+                                                            T* __restrict__ dth) {
+  for (size_t h = (size_t)blockIdx.x * kBlock + threadIdx.x; h < (size_t)a.nhinge; h += (size_t)gridDim.x * kBlock) {
+    HingeIn<T> H;
+    hinge_load<T>(x, nullptr, (T)0, a, (int)h, H);
+    HingeGeo<T> G;
+    hinge_geo_at(H.x1, H.x2, H.x3, H.x4, H.t0, G);
+    dth[h] = G.dtheta;
+  }
+}
+
 // Gradient, gathered per node over its incident bars and hinges (CSR in
 // element order), U elements at a time with all their loads issued first;
-// contributions are added in list order.
+// contributions are added in list order.  Hinge angle deviations come from
+// dth (computed once per hinge by the energy or the dtheta kernel).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void bah_gradient_kernel(BaHArgs a, const T* __restrict__ x,
-                                                              T* __restrict__ g) {
+                                                              const T* __restrict__ dth, T* __restrict__ g) {
   constexpr int U = 4;
   for (int n = xcd_logical_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; n < a.nnode;
        n += gridDim.x * kBlock) {
@@ -259,17 +290,20 @@ __global__ __launch_bounds__(kBlock) void bah_gradient_kernel(BaHArgs a, const T
     for (int q0 = qh0; q0 < qh1; q0 += U) {
       HingeIn<T> H[U];
       int loc[U];
+      T dt[U];
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int code = a.hidx[min(q0 + u, qh1 - 1)];
         loc[u] = code & 3;
         hinge_load<T>(x, nullptr, (T)0, a, code >> 2, H[u]);
+        dt[u] = dth[code >> 2];
       }
 #pragma unroll
       for (int u = 0; u < U; u++) {
         if (q0 + u >= qh1) break;
         HingeGeo<T> G;
-        hinge_geo_at(H[u].x1, H[u].x2, H[u].x3, H[u].x4, H[u].t0, G);
+        hinge_geo_vec(H[u].x1, H[u].x2, H[u].x3, H[u].x4, G);
+        G.dtheta = dt[u];
         hinge_grad_at(G, H[u].k, loc[u], gn);
       }
     }
@@ -290,6 +324,7 @@ using minim_amd::kBlock;
 struct BarAndHinge::Dev {
   minim_amd::BaHArgs args{};
   DeviceBuffer mem;
+  DeviceBuffer dth;  // per-hinge angle deviation at the last gradient evaluation (FP64-sized)
   std::vector<int> activeBars, activeHinges;  // element index -> original bond/hinge index
 };
 
@@ -593,31 +628,42 @@ void BarAndHinge::upload() {
   a.hidx = (const int*)(base + o_hidx);
   a.fz = fz.empty() ? nullptr : (const double*)(base + o_fz);
   a.nodeActive = (const unsigned char*)(base + o_act);
+  dev->dth.reset(sizeof(double) * (size_t)(nh > 0 ? nh : 1));
   dev_ = dev;
 }
 
 namespace {
-// Energy kernel (into rs.result[0] when wantE) and gradient kernel (when g).
-void launch_bah(const minim_amd::BaHArgs& a, int prec, const void* x, const void* d, double t, bool wantE, void* g,
+// Energy kernel (into rs.result[0] when wantE) and gradient kernel (when g;
+// the hinge angles come from the energy kernel, or from the dtheta kernel
+// without an energy).
+void launch_bah(const BarAndHinge::Dev& dv, int prec, const void* x, const void* d, double t, bool wantE, void* g,
                 minim_amd::RedScratch rs, hipStream_t s) {
-  if (wantE) {
-    const size_t tot = (size_t)a.nbar + a.nhinge + a.nnode;
-    const int nb = minim_amd::grid_blocks(tot, 4, 2048);
-    if (prec == FP64)
-      minim_amd::bah_energy_kernel<double><<<nb, kBlock, 0, s>>>(a, (const double*)x, (const double*)d, t, rs);
-    else
-      minim_amd::bah_energy_kernel<float><<<nb, kBlock, 0, s>>>(a, (const float*)x, (const float*)d, (float)t, rs);
-    MINIM_HIP_CHECK(hipGetLastError());
-    minim_amd::launch_finalize(rs, 1, nb, s);
-  }
-  if (g) {
-    const int nb = minim_amd::grid_blocks(a.nnode, 1, 1 << 20);
-    if (prec == FP64)
-      minim_amd::bah_gradient_kernel<double><<<nb, kBlock, 0, s>>>(a, (const double*)x, (double*)g);
-    else
-      minim_amd::bah_gradient_kernel<float><<<nb, kBlock, 0, s>>>(a, (const float*)x, (float*)g);
-    MINIM_HIP_CHECK(hipGetLastError());
-  }
+  const auto& a = dv.args;
+  auto go = [&](auto tag) {
+    using T = decltype(tag);
+    T* dth = static_cast<T*>(dv.dth.get());
+    const T* xx = (const T*)x;
+    if (wantE) {
+      const size_t tot = (size_t)a.nbar + a.nhinge + a.nnode;
+      const int nb = minim_amd::grid_blocks(tot, 4, 2048);
+      if (g)
+        minim_amd::bah_energy_kernel<T, true><<<nb, kBlock, 0, s>>>(a, xx, (const T*)d, (T)t, rs, dth);
+      else
+        minim_amd::bah_energy_kernel<T, false><<<nb, kBlock, 0, s>>>(a, xx, (const T*)d, (T)t, rs, dth);
+      MINIM_HIP_CHECK(hipGetLastError());
+      minim_amd::launch_finalize(rs, 1, nb, s);
+    } else if (g && a.nhinge > 0) {
+      minim_amd::bah_dtheta_kernel<T><<<minim_amd::grid_blocks(a.nhinge, 1, 1 << 20), kBlock, 0, s>>>(a, xx, dth);
+      MINIM_HIP_CHECK(hipGetLastError());
+    }
+    if (g) {
+      const int nb = minim_amd::grid_blocks(a.nnode, 1, 1 << 20);
+      minim_amd::bah_gradient_kernel<T><<<nb, kBlock, 0, s>>>(a, xx, dth, (T*)g);
+      MINIM_HIP_CHECK(hipGetLastError());
+    }
+  };
+  if (prec == FP64) go(0.0);
+  else go(0.0f);
 }
 }  // namespace
 
@@ -629,7 +675,7 @@ void BarAndHinge::evaluate(int prec, const void* x, const void* d, double t, dou
   const auto& a = dev_->args;
   if (g && d) throw std::invalid_argument("BarAndHinge: gradients at a trial point are not supported.");
   if (e || g) {
-    launch_bah(dev_->args, prec, x, d, t, e != nullptr, g, ws.rs(), s);
+    launch_bah(*dev_, prec, x, d, t, e != nullptr, g, ws.rs(), s);
     if (g) applyConstraints(prec, g, 3 * (size_t)a.nnode, stream);
   }
   if (e) {
@@ -646,12 +692,12 @@ double BarAndHinge::timeKernel(int prec, const void* x, bool wantE, void* g, int
   if ((!wantE && !g) || reps < 1) throw std::invalid_argument("BarAndHinge: nothing to time");
   hipStream_t s = static_cast<hipStream_t>(stream);
   auto& ws = minim_amd::Workspace::current();
-  launch_bah(dev_->args, prec, x, nullptr, 0.0, wantE, g, ws.rs(), s);
+  launch_bah(*dev_, prec, x, nullptr, 0.0, wantE, g, ws.rs(), s);
   hipEvent_t e0, e1;
   MINIM_HIP_CHECK(hipEventCreate(&e0));
   MINIM_HIP_CHECK(hipEventCreate(&e1));
   MINIM_HIP_CHECK(hipEventRecord(e0, s));
-  for (int i = 0; i < reps; i++) launch_bah(dev_->args, prec, x, nullptr, 0.0, wantE, g, ws.rs(), s);
+  for (int i = 0; i < reps; i++) launch_bah(*dev_, prec, x, nullptr, 0.0, wantE, g, ws.rs(), s);
   MINIM_HIP_CHECK(hipEventRecord(e1, s));
   MINIM_HIP_CHECK(hipEventSynchronize(e1));
   float ms = 0;
