@@ -7,7 +7,8 @@
  *
  * Serial FP64, written to follow the reference's operation order so that with
  * -ffp-contract=off it agrees bit for bit with a single-thread build of the
- * reference (oracle/_ref, built from /root/reference by oracle/Makefile).
+ * reference (oracle/_ref, built from the reference's own sources by
+ * oracle/Makefile).
  *
  * Parity pins (see tests/test_oracle.py):
  *   orc_pf_*            reference unit-test known answers (PhaseField_test.cpp:31-68,
@@ -29,6 +30,19 @@
  *   orc_bah_*           BarAndHinge_test.cpp:13-56 minima; vs oracle/_ref; the
  *                       reference's 2x stretching-gradient factor (BarAndHinge.cpp:282)
  *                       checked against a finite difference of the energy.
+ *   orc_pfx_*           PhaseField's remaining terms: PhaseField_test.cpp:44-197 known
+ *                       answers; vs oracle/_ref on 40 random layouts (energies,
+ *                       constrained gradients, projections, energyComponents, node
+ *                       volumes and surface areas; 1e-13), L-BFGS trajectories and
+ *                       diffuseSolid vs oracle/_ref; finite differences; closed forms of
+ *                       pressure and confinement; zero volume change and idempotence of
+ *                       the volume projection (tests/test_pf_ext_oracle.py).
+ *   orc_pf_site_types   site for site vs the reference build's getType, including the
+ *                       layouts it rejects (tests/test_pf_ext_oracle.py).
+ *   orc_pfu_*           PhaseFieldUnstructured_test.cpp:27-80,104-129,132-180 known
+ *                       answers; vs oracle/_ref on 40 random layouts (element counts,
+ *                       energies, gradients, projections; 1e-13) and L-BFGS
+ *                       trajectories; finite differences (tests/test_pfu_oracle.py).
  */
 #include "oracle.h"
 

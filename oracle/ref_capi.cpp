@@ -1,5 +1,5 @@
 // ref_capi.cpp -- a C-ABI shim over the reference library (sjavis/minim),
-// compiled together with the reference's own sources from /root/reference by
+// compiled together with the reference's own sources from the reference source tree by
 // oracle/Makefile into oracle/_ref/libminim_ref.so.
 //
 // TEST INFRASTRUCTURE ONLY: used by tests/ to pin the oracle restatement and by
