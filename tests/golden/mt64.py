@@ -1,6 +1,6 @@
 """64-bit Mersenne Twister (MT19937-64, Matsumoto & Nishimura 2004) and the
 documented bit -> double map used for the synthetic benchmark inputs, so that
-the GPU box, the oracle and the golden generator draw identical inputs without
+the GPU runs, the oracle and the golden generator draw identical inputs without
 depending on a library's distribution code.
 
     u = (x >> 11) * 2**-53        # 53 random bits -> [0, 1)
