@@ -218,7 +218,9 @@ int minim_state_info(const minim_state* s, size_t* ndof, int* precision, double*
 int minim_state_set_convergence(minim_state* s, double convergence);
 int minim_state_set_precision(minim_state* s, int precision);
 int minim_state_set_stream(minim_state* s, void* stream);
-/* Device address of the coordinates (ndof entries of the State's precision). */
+/* Device address of the coordinates (ndof entries of the State's precision).
+ * Valid until the next minimise or precision change on this state: L-BFGS's
+ * fused accepted step exchanges the coordinates with a buffer of its own. */
 void* minim_state_device_coords(minim_state* s);
 
 /* ------------------------------------------------------------------ minimisers */
