@@ -594,6 +594,9 @@ struct Fire::Impl {
   double gNorm = 0, gMax = 0, a = 0.1, dt = 0;
   int nSteps = 0;
   EnergyCache ec;
+  // v.g and v.v for the next iteration, fetched with the previous one's g.g
+  bool haveDots = false;
+  double vg = 0, vv = 0;
 };
 
 Fire::Fire() : impl_(new Impl) {}
@@ -649,6 +652,7 @@ void Fire::init(State& state) {
   I.step = DVec(I.n, I.prec);
   MINIM_HIP_CHECK(hipMemsetAsync(I.v.data(), 0, I.n * minim_amd::prec_bytes(I.prec), hs(state)));
   I.ec.valid = false;
+  I.haveDots = false;
   I.a = _aStart;
   I.nSteps = 0;
   if (dtMax != 0) return;
@@ -672,11 +676,17 @@ void Fire::iteration(State& state) {
     gradient_into(state, I.prec, I.g.data(), stats);
     I.gNorm = std::sqrt(dot1(I.prec, I.g.data(), I.g.data(), I.n, s));
   }
-  const void* pa[2] = {I.v.data(), I.v.data()};
-  const void* pb[2] = {I.g.data(), I.v.data()};
-  minim_amd::launch_dots(I.prec, pa, pb, 2, I.n, ws.rs(), s);
   double r[2];
-  ws.fetch(s, 2, r);
+  if (I.haveDots) {
+    r[0] = I.vg;
+    r[1] = I.vv;
+  } else {
+    const void* pa[2] = {I.v.data(), I.v.data()};
+    const void* pb[2] = {I.g.data(), I.v.data()};
+    minim_amd::launch_dots(I.prec, pa, pb, 2, I.n, ws.rs(), s);
+    ws.fetch(s, 2, r);
+  }
+  I.haveDots = false;
   const double p = -r[0];
   // velocity update with the current a and dt (Fire.cpp:50-57) ...
   int mode;
@@ -712,8 +722,29 @@ void Fire::iteration(State& state) {
     I.ec.valid = false;
   }
   gradient_into(state, I.prec, I.g.data(), stats);
-  I.gNorm = std::sqrt(dot1(I.prec, I.g.data(), I.g.data(), I.n, s));
-  if (_maxForceDim > 0) I.gMax = group_norm_max(I.prec, I.g.data(), I.n, _maxForceDim, s);
+  // g.g now and the next iteration's v.g and v.v (v is already updated), with
+  // the largest per-particle force, in one host synchronisation
+  {
+    const void* pa[3] = {I.g.data(), I.v.data(), I.v.data()};
+    const void* pb[3] = {I.g.data(), I.g.data(), I.v.data()};
+    minim_amd::launch_dots(I.prec, pa, pb, 3, I.n, ws.rs(), s);
+    constexpr int kStash = 48;
+    double* res = ws.device_result();
+    double q[kStash + 3];
+    int nfetch = 3;
+    if (_maxForceDim > 0) {
+      MINIM_HIP_CHECK(hipMemcpyAsync(res + kStash, res, sizeof(double) * 3, hipMemcpyDeviceToDevice, s));
+      minim_amd::launch_group_norm_max(I.prec, I.g.data(), I.n, _maxForceDim, ws.rs(), s);
+      nfetch = kStash + 3;
+    }
+    ws.fetch(s, nfetch, q);
+    const double* d3 = _maxForceDim > 0 ? q + kStash : q;
+    I.gNorm = std::sqrt(d3[0]);
+    I.vg = d3[1];
+    I.vv = d3[2];
+    I.haveDots = true;
+    if (_maxForceDim > 0) I.gMax = q[0];
+  }
   if (I.prec == FP32) stats.iterFP32++;
   else stats.iterFP64++;
 }
