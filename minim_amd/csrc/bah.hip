@@ -24,6 +24,7 @@
 #include "common.hpp"
 #include "minim/BarAndHinge.h"
 #include "runtime.hpp"
+#include "vec.hpp"
 
 namespace minim_amd {
 namespace {
@@ -324,7 +325,8 @@ using minim_amd::kBlock;
 struct BarAndHinge::Dev {
   minim_amd::BaHArgs args{};
   DeviceBuffer mem;
-  DeviceBuffer dth;  // per-hinge angle deviation at the last gradient evaluation (FP64-sized)
+  DeviceBuffer dth;     // per-hinge angle deviation at the last gradient evaluation (FP64-sized)
+  DeviceBuffer xtrial;  // node coordinates at a line-search trial point (FP64-sized)
   std::vector<int> activeBars, activeHinges;  // element index -> original bond/hinge index
 };
 
@@ -629,6 +631,7 @@ void BarAndHinge::upload() {
   a.fz = fz.empty() ? nullptr : (const double*)(base + o_fz);
   a.nodeActive = (const unsigned char*)(base + o_act);
   dev->dth.reset(sizeof(double) * (size_t)(nh > 0 ? nh : 1));
+  dev->xtrial.reset(3 * sizeof(double) * (size_t)(nnode > 0 ? nnode : 1));
   dev_ = dev;
 }
 
@@ -643,6 +646,16 @@ void launch_bah(const BarAndHinge::Dev& dv, int prec, const void* x, const void*
     using T = decltype(tag);
     T* dth = static_cast<T*>(dv.dth.get());
     const T* xx = (const T*)x;
+    if (d) {
+      // a trial point: materialised once (x + t*d, as the element loads would
+      // form it), so that the energy pass reads one array per node
+      T* xt = static_cast<T*>(dv.xtrial.get());
+      const size_t n3 = 3 * (size_t)a.nnode;
+      minim_amd::device_copy(xt, x, n3 * sizeof(T), s);
+      minim_amd::launch_axpy(prec, xt, d, t, n3, s);
+      xx = xt;
+      d = nullptr;
+    }
     if (wantE) {
       const size_t tot = (size_t)a.nbar + a.nhinge + a.nnode;
       const int nb = minim_amd::grid_blocks(tot, 4, 2048);
