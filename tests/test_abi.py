@@ -156,3 +156,49 @@ def test_site_types_match_oracle():
                 site_types(grid, s)
             continue
         assert np.array_equal(site_types(grid, s), want)
+
+
+def test_element_form_and_fused_step_errors():
+    """Host-side checks of the element-form and fused-step entry points (no
+    kernel is launched): the wrong potential kind, use before init and NULL
+    pointers map to the reference's exception kinds."""
+    from minim_amd import _capi as A
+
+    L = A.load()
+    rc, h = _pf_handle((2, 2, 2), 1, [1.0], [])
+    assert rc == 0
+    n, b = C.c_size_t(), C.c_size_t()
+    # not a PhaseFieldUnstructured
+    assert L.minim_pfu_element_list(h, C.byref(n), C.byref(b)) == A.MINIM_EINVAL
+    # the fused step before init, and with a NULL direction
+    assert L.minim_potential_eval_step(h, 0, 0, C.c_void_p(1), 8, C.c_void_p(1), 1.0, C.c_void_p(1), None,
+                                       C.c_void_p(1), None) == A.MINIM_ERUNTIME
+    assert L.minim_potential_eval_step(h, 0, 0, C.c_void_p(1), 8, None, 1.0, C.c_void_p(1), None,
+                                       C.c_void_p(1), None) == A.MINIM_EINVAL
+    L.minim_potential_destroy(h)
+    p = A.PFParams()
+    for i in range(3):
+        p.grid[i] = 2
+    p.n_fluid = 1
+    p.resolution = 1.0
+    st = np.ones(1)
+    p.surface_tension = st.ctypes.data_as(A.dp)
+    p.n_surface_tension = 1
+    p.density_const = 1.0
+    hu = C.c_void_p()
+    assert L.minim_pfu_create(C.byref(p), C.byref(hu)) == 0
+    # element list before init
+    assert L.minim_pfu_element_list(hu, C.byref(n), C.byref(b)) == A.MINIM_ERUNTIME
+    x = np.zeros(8)
+    assert L.minim_potential_init(hu, x.ctypes.data_as(A.dp), 8) == 0
+    assert L.minim_pfu_element_list(hu, C.byref(n), C.byref(b)) == 0
+    assert n.value == 8 and b.value > 0
+    # the element form has no fused step
+    assert L.minim_potential_eval_step(hu, 0, 0, C.c_void_p(1), 8, C.c_void_p(1), 1.0, C.c_void_p(1), None,
+                                       C.c_void_p(1), None) == A.MINIM_EINVAL
+    # node volumes: no surface areas in the element form
+    nv = np.zeros(8)
+    assert L.minim_pf_node_volumes(hu, nv.ctypes.data_as(A.dp), None) == 0
+    np.testing.assert_array_equal(nv, np.ones(8))
+    assert L.minim_pf_node_volumes(hu, nv.ctypes.data_as(A.dp), nv.ctypes.data_as(A.dp)) == A.MINIM_EINVAL
+    L.minim_potential_destroy(hu)
