@@ -43,6 +43,8 @@ class LjNd : public NewPotential<LjNd> {
   void evaluate(int prec, const void* x, const void* d, double t, double* e, void* g,
                 void* stream) const override;
   double timeKernel(int prec, const void* x, bool wantE, void* g, int reps, void* stream) const override;
+  int gradientDeferred(int prec, const void* x, void* g, void* stream) const override;
+  void redoGradient(int prec, const void* x, void* g, void* stream) const override;
 
   // Statistics of the neighbour list (cutoff mode).
   long listBuilds() const;

@@ -89,6 +89,13 @@ class Potential {
   virtual bool supportsStep() const { return false; }
   virtual void evaluateStepAsync(int px, int pv, const void* x, const void* d, double t, void* xout, void* g,
                                  void* stream) const;
+  // The gradient launched without a host synchronisation, for potentials
+  // whose evaluation carries a validity check (LjNd's neighbour list).
+  // Returns the reduction result slot that will hold a nonzero value when the
+  // gradient must be recomputed with redoGradient (read it with the caller's
+  // next fetch), or -1 when the gradient is complete (the default: evaluate).
+  virtual int gradientDeferred(int prec, const void* x, void* g, void* stream) const;
+  virtual void redoGradient(int prec, const void* x, void* g, void* stream) const;
   // True when applyConstraints also couples all DOFs (PhaseField's fixed
   // total volumes, PhaseField.cpp:614-658): minimisers then apply it to each
   // step in addition to the per-site projection.

@@ -23,6 +23,9 @@ constexpr int kBlock = 256;          // threads per block (4 wavefronts of 64)
 constexpr int kWave = 64;
 constexpr int kMaxRedBlocks = 8192;  // upper bound on blocks of any reducing launch
 constexpr int kMaxDots = 96;         // upper bound on values reduced by one launch
+// Result slot of a deferred validity flag (Potential::gradientDeferred): the
+// last one, written by that launch and read by the caller's next fetch.
+constexpr int kDeferSlot = kMaxDots - 1;
 
 enum Prec : int { F64 = 0, F32 = 1 };
 

@@ -602,6 +602,40 @@ void LjNd::evaluate(int prec, const void* x, const void* d, double t, double* e,
   }
 }
 
+// Cutoff mode: the force pass without the host synchronisation of the list
+// check; the moved-particle count goes to the result slot returned.
+int LjNd::gradientDeferred(int prec, const void* x, void* g, void* stream) const {
+  if (!impl_) throw std::runtime_error("LjNd: not initialised (construct a State first).");
+  if (cutoff <= 0) return Potential::gradientDeferred(prec, x, g, stream);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  auto& ws = minim_amd::Workspace::current();
+  Impl& I = *impl_;
+  const int N = nParticle;
+  if (!I.built) rebuild_list(I, prec, x, nullptr, 0.0, N, s);
+  const double src6 = std::pow(sigma / cutoff, 6);
+  const double eshift = 4 * epsilon * (src6 * src6 - src6);
+  launch_cut(I, prec, x, nullptr, 0.0, N, sigma, epsilon, eshift, 2, g, ws.rs(), s);
+  minim_amd::launch_finalize(ws.rs(), 2, cut_blocks(N), s);
+  double* res = ws.device_result();
+  MINIM_HIP_CHECK(hipMemcpyAsync(res + minim_amd::kDeferSlot, res + 1, sizeof(double), hipMemcpyDeviceToDevice, s));
+  applyConstraints(prec, g, (size_t)N * nDim, stream);
+  return minim_amd::kDeferSlot;
+}
+
+void LjNd::redoGradient(int prec, const void* x, void* g, void* stream) const {
+  if (!impl_) throw std::runtime_error("LjNd: not initialised (construct a State first).");
+  if (cutoff <= 0) return Potential::redoGradient(prec, x, g, stream);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  auto& ws = minim_amd::Workspace::current();
+  const int N = nParticle;
+  rebuild_list(*impl_, prec, x, nullptr, 0.0, N, s);
+  const double src6 = std::pow(sigma / cutoff, 6);
+  const double eshift = 4 * epsilon * (src6 * src6 - src6);
+  launch_cut(*impl_, prec, x, nullptr, 0.0, N, sigma, epsilon, eshift, 2, g, ws.rs(), s);
+  minim_amd::launch_finalize(ws.rs(), 2, cut_blocks(N), s);
+  applyConstraints(prec, g, (size_t)N * nDim, stream);
+}
+
 // Average duration (ms) of reps launches of the force kernel (cutoff mode:
 // the list is built first when it does not exist yet; all-pairs mode: the
 // kernel and its reduction).

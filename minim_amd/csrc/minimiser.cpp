@@ -721,29 +721,38 @@ void Fire::iteration(State& state) {
     state.invalidateHost();
     I.ec.valid = false;
   }
-  gradient_into(state, I.prec, I.g.data(), stats);
+  // the new gradient (its validity check, if any, read with the dots below),
   // g.g now and the next iteration's v.g and v.v (v is already updated), with
   // the largest per-particle force, in one host synchronisation
-  {
+  int check = state.pot->gradientDeferred(I.prec, state.deviceCoords(), I.g.data(), state.stream());
+  stats.nGradient++;
+  for (;;) {
     const void* pa[3] = {I.g.data(), I.v.data(), I.v.data()};
     const void* pb[3] = {I.g.data(), I.g.data(), I.v.data()};
     minim_amd::launch_dots(I.prec, pa, pb, 3, I.n, ws.rs(), s);
     constexpr int kStash = 48;
     double* res = ws.device_result();
-    double q[kStash + 3];
+    std::vector<double> q(minim_amd::kMaxDots);
     int nfetch = 3;
     if (_maxForceDim > 0) {
       MINIM_HIP_CHECK(hipMemcpyAsync(res + kStash, res, sizeof(double) * 3, hipMemcpyDeviceToDevice, s));
       minim_amd::launch_group_norm_max(I.prec, I.g.data(), I.n, _maxForceDim, ws.rs(), s);
       nfetch = kStash + 3;
     }
-    ws.fetch(s, nfetch, q);
-    const double* d3 = _maxForceDim > 0 ? q + kStash : q;
+    if (check >= 0 && check + 1 > nfetch) nfetch = check + 1;
+    ws.fetch(s, nfetch, q.data());
+    if (check >= 0 && q[check] > 0) {  // the gradient was evaluated with a stale list
+      state.pot->redoGradient(I.prec, state.deviceCoords(), I.g.data(), state.stream());
+      check = -1;
+      continue;
+    }
+    const double* d3 = _maxForceDim > 0 ? q.data() + kStash : q.data();
     I.gNorm = std::sqrt(d3[0]);
     I.vg = d3[1];
     I.vv = d3[2];
     I.haveDots = true;
     if (_maxForceDim > 0) I.gMax = q[0];
+    break;
   }
   if (I.prec == FP32) stats.iterFP32++;
   else stats.iterFP64++;

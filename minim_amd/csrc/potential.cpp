@@ -45,6 +45,15 @@ void Potential::applyConstraints(int prec, void* v, size_t n, void* stream) cons
                                static_cast<hipStream_t>(stream));
 }
 
+int Potential::gradientDeferred(int prec, const void* x, void* g, void* stream) const {
+  evaluate(prec, x, nullptr, 0.0, nullptr, g, stream);
+  return -1;
+}
+
+void Potential::redoGradient(int prec, const void* x, void* g, void* stream) const {
+  evaluate(prec, x, nullptr, 0.0, nullptr, g, stream);
+}
+
 void Potential::evaluateStepAsync(int, int, const void*, const void*, double, void*, void*, void*) const {
   throw std::invalid_argument("this potential has no fused step evaluation");
 }
