@@ -296,9 +296,13 @@ def lj_leg(m, n_side=64, density=0.85, fire_max_iter=20000, ref_side=10):
         msk = m.potential_time_kernel(h, prec, x.data_ptr(), 3 * N, g_ptr=g.data_ptr(), reps=50)
         b = x.element_size()
         alg = N * 3 * b * 2 + 4 * list_n * N  # positions in, gradient out, neighbour list
+        # VALU: about 25 flop per pair over the half list within rc + skin
+        flops = 25.0 * list_n * N / 2
+        vpeak = 78.6e12 if prec == "fp64" else 157.3e12
         out[prec] = dict(eval_ms=ms, evals_per_s=1e3 / ms, kernel_ms=msk, algorithmic_bytes=alg,
                          achieved_GBs=alg / (msk * 1e-3) / 1e9, frac_of_hbm_peak=alg / (msk * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                         eval_achieved_GBs=alg / (ms * 1e-3) / 1e9)
+                         eval_achieved_GBs=alg / (ms * 1e-3) / 1e9, algorithmic_flops=flops,
+                         achieved_TFLOPs=flops / (msk * 1e-3) / 1e12, frac_of_valu_peak=flops / (msk * 1e-3) / vpeak)
         del h, x, g
     s = m.State(pot, x0, "fp64")
     s.convergence = 1e-6
