@@ -79,8 +79,9 @@ __device__ __forceinline__ void phase7(const T (&c)[kPFUNodes], const bool (&sel
   if (WE) e = e + factor * grad2;
 }
 
-// MODE bits: 1 = energy, 2 = gradient, 4 = at x + t*d.
-template <typename T, typename TV, int MODE>
+// MODE bits: 1 = energy, 2 = gradient, 4 = at x + t*d.  NF: the fluid count
+// when it is 1..4 (index arithmetic with constants), 0 for u.nf.
+template <typename T, typename TV, int MODE, int NF>
 __global__ __launch_bounds__(kBlock) void pfu_chunk_kernel(PFUArgs u, const T* __restrict__ x,
                                                            const TV* __restrict__ d, T t, TV* __restrict__ g,
                                                            RedScratch rs) {
@@ -88,7 +89,7 @@ __global__ __launch_bounds__(kBlock) void pfu_chunk_kernel(PFUArgs u, const T* _
   __shared__ uint32_t ssite[kPFUChunkDofs];
   __shared__ double sacc[WG ? kPFUChunkDofs : 1];
   const Coords<T, TV, DIR> cx{x, d, t};
-  const int nf = u.nf;
+  const int nf = NF > 0 ? NF : u.nf;
   // gradient contribution v to fluid k of local site l
   auto add = [&](int l, int k, T v) {
     if (WG && v != (T)0) unsafeAtomicAdd(&sacc[l * nf + k], (double)v);
@@ -324,6 +325,17 @@ __global__ __launch_bounds__(kBlock) void pfu_volgrad_kernel(PFUArgs u, const T*
   }
 }
 
+template <typename T, int MODE>
+void launch_chunk(const PFUArgs& u, int nb, const T* x, const T* d, T t, T* g, RedScratch rs, hipStream_t s) {
+  switch (u.nf) {
+    case 1: pfu_chunk_kernel<T, T, MODE, 1><<<nb, kBlock, 0, s>>>(u, x, d, t, g, rs); break;
+    case 2: pfu_chunk_kernel<T, T, MODE, 2><<<nb, kBlock, 0, s>>>(u, x, d, t, g, rs); break;
+    case 3: pfu_chunk_kernel<T, T, MODE, 3><<<nb, kBlock, 0, s>>>(u, x, d, t, g, rs); break;
+    case 4: pfu_chunk_kernel<T, T, MODE, 4><<<nb, kBlock, 0, s>>>(u, x, d, t, g, rs); break;
+    default: pfu_chunk_kernel<T, T, MODE, 0><<<nb, kBlock, 0, s>>>(u, x, d, t, g, rs); break;
+  }
+}
+
 // Blocks of a chunk-kernel launch: each loops over chunks (at most
 // kMaxRedBlocks blocks write reduction partials).
 int chunk_blocks(const PFUArgs& u) { return u.nchunk < 1 ? 1 : (u.nchunk < 2048 ? u.nchunk : 2048); }
@@ -338,10 +350,10 @@ void eval_t(const PFUArgs& u, int mode, const void* x, const void* d, double t, 
   const size_t nd = u.ngrid * u.nf;
   const int nb = chunk_blocks(u);
   switch (mode) {
-    case 1: pfu_chunk_kernel<T, T, 1><<<nb, kBlock, 0, s>>>(u, xx, dd, tt, gg, rs); break;
-    case 2: pfu_chunk_kernel<T, T, 2><<<nb, kBlock, 0, s>>>(u, xx, dd, tt, gg, rs); break;
-    case 3: pfu_chunk_kernel<T, T, 3><<<nb, kBlock, 0, s>>>(u, xx, dd, tt, gg, rs); break;
-    case 5: pfu_chunk_kernel<T, T, 5><<<nb, kBlock, 0, s>>>(u, xx, dd, tt, gg, rs); break;
+    case 1: launch_chunk<T, 1>(u, nb, xx, dd, tt, gg, rs, s); break;
+    case 2: launch_chunk<T, 2>(u, nb, xx, dd, tt, gg, rs, s); break;
+    case 3: launch_chunk<T, 3>(u, nb, xx, dd, tt, gg, rs, s); break;
+    case 5: launch_chunk<T, 5>(u, nb, xx, dd, tt, gg, rs, s); break;
     default: throw std::invalid_argument("pfu: unsupported evaluation mode");
   }
   if ((mode & 2) && u.ngather)
@@ -378,18 +390,18 @@ double pfu_time_elements(const PFUArgs& u, int prec, int mode, const void* x, vo
     if (prec == F64) {
       const double* xx = static_cast<const double*>(x);
       if (mode == 1) {
-        pfu_chunk_kernel<double, double, 1><<<nb, kBlock, 0, s>>>(u, xx, xx, 0.0, (double*)g, rs);
+        launch_chunk<double, 1>(u, nb, xx, xx, 0.0, (double*)g, rs, s);
       } else {
-        pfu_chunk_kernel<double, double, 3><<<nb, kBlock, 0, s>>>(u, xx, xx, 0.0, (double*)g, rs);
+        launch_chunk<double, 3>(u, nb, xx, xx, 0.0, (double*)g, rs, s);
         if (u.ngather)
           pfu_gather_kernel<double><<<grid_blocks(u.ngather * u.nf, 1, 8192), kBlock, 0, s>>>(u, (double*)g);
       }
     } else {
       const float* xx = static_cast<const float*>(x);
       if (mode == 1) {
-        pfu_chunk_kernel<float, float, 1><<<nb, kBlock, 0, s>>>(u, xx, xx, 0.0f, (float*)g, rs);
+        launch_chunk<float, 1>(u, nb, xx, xx, 0.0f, (float*)g, rs, s);
       } else {
-        pfu_chunk_kernel<float, float, 3><<<nb, kBlock, 0, s>>>(u, xx, xx, 0.0f, (float*)g, rs);
+        launch_chunk<float, 3>(u, nb, xx, xx, 0.0f, (float*)g, rs, s);
         if (u.ngather)
           pfu_gather_kernel<float><<<grid_blocks(u.ngather * u.nf, 1, 8192), kBlock, 0, s>>>(u, (float*)g);
       }
