@@ -346,3 +346,32 @@ def test_diffuse_solid_vs_oracle(gpu, two_step):
         o2 = OraclePFX(grid, x, **kw)
         x, _ = oracle_lbfgs(o2, x, m=5, max_iter=100, linesearch="none", convergence=o2.convergence)
     assert np.max(np.abs(got - x)) <= 1e-6
+
+
+def test_general_kernel_full_size_256cubed_pillars(gpu):
+    """The general kernel's bench workload at full size (256^3 x 3 fluids, a
+    pillar array on a base, the solid fluid fixed, a pressure on the liquid):
+    FP64 energy and gradient vs the oracle, FP32 within 1e-5."""
+    import torch
+    from bench import pillar_layout
+
+    m = gpu
+    n = 256
+    solid, x = pillar_layout(n)
+    kw = dict(nfluid=3, surface_tension=[1.0, 0.8, 1.2], interface_size=[2.0], density="hard", solid=solid,
+              fix_fluid=[True, False, False], pressure=[0.0, 0.05, 0.0])
+    print("oracle 256^3 pillars ...", flush=True)
+    o = OraclePFX((n, n, n), x, **kw)
+    eo, go = o.energy(x), o.gradient(x)
+    pf = (m.PhaseField().setNFluid(3).setGridSize([n, n, n]).setSurfaceTension([1.0, 0.8, 1.2]).setInterfaceSize(2.0)
+          .setSolid(solid).setFixFluid(0).setPressure([0.0, 0.05, 0.0]))
+    h = pf.handle(x)
+    nd = x.size
+    for prec, dt, tol in (("fp64", torch.float64, 1e-12), ("fp32", torch.float32, 1e-5)):
+        xd = torch.from_numpy(x).to("cuda", dt)
+        g = torch.empty_like(xd)
+        e = m.potential_eval(h, prec, xd.data_ptr(), nd, g_ptr=g.data_ptr())
+        torch.cuda.synchronize()
+        es = max(1.0, sum(abs(v[0]) for v in o.components(x).values()))
+        assert abs(e - eo) <= tol * es, (prec, e, eo)
+        assert rel_err(g.double().cpu().numpy(), go) <= tol, prec
