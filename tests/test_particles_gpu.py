@@ -212,3 +212,48 @@ def test_bah_two_triangle_minima(gpu):
         x = m.Lbfgs().minimise(s)
         np.testing.assert_allclose(x, ex, atol=1e-4)
         np.testing.assert_allclose(x, mn_ref["x"], atol=1e-6)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_bah_sheet_vs_oracle(gpu, prec):
+    """The benchmark sheet's construction at 256^2 nodes (equilateral lattice
+    compressed by 0.95, clamped x edges, z noise): energy and gradient at the
+    start and at a perturbed point against the oracle, for the energy pass
+    that stores the hinge angles, the gradient-only pass and a trial point."""
+    import torch
+    from bench import sheet_triangles
+
+    m = gpu
+    nside = 256
+    tri = sheet_triangles(nside, nside)
+    i, j = np.meshgrid(np.arange(nside), np.arange(nside), indexing="ij")
+    rng = np.random.default_rng(78)
+    x0 = np.stack([0.95 * (i - 0.5 * j), (np.sqrt(3) / 2) * j, rng.normal(0, 0.01, i.shape)], -1).reshape(-1)
+    edge = np.where((i == 0) | (i == nside - 1), 1, 0).reshape(-1).nonzero()[0]
+    fixed = np.stack([3 * edge, 3 * edge + 1, 3 * edge + 2], 1).reshape(-1)
+    pot = m.BarAndHinge().setTriangulation(tri).setRigidity(1.0, 0.01).setLength0(1.0).setTheta0(np.pi)
+    pot.setConstraints(fixed.tolist())
+    bonds, hinges = m.triangulation_lists(tri)
+    nnode = nside * nside
+    fx = np.zeros(3 * nnode, np.uint8)
+    fx[fixed] = 1
+    o = OracleBaH(nnode, bonds, np.ones(len(bonds)), np.ones(len(bonds)), hinges, np.full(len(hinges), 0.01),
+                  np.full(len(hinges), np.pi), fixed=fx)
+    x1 = x0 + rng.normal(0, 0.02, x0.size)
+    xs = r32(x1) if prec == "fp32" else x1
+    s = m.State(pot, x0, prec)
+    e, g = s.energyGradient(xs)
+    eo, go = o.energy(xs), o.gradient(xs)
+    assert abs(e - eo) <= TOL[prec] * abs(eo)
+    assert rel_err(g, go) <= TOL[prec]
+    assert np.all(g[fixed] == 0)
+    assert rel_err(s.gradient(xs), go) <= TOL[prec]
+    # energy at a trial point x + t*d (materialised before the energy pass)
+    h = pot.handle(x0)
+    dt = torch.float64 if prec == "fp64" else torch.float32
+    xd = torch.tensor(xs, dtype=dt, device="cuda")
+    d = torch.tensor(rng.normal(0, 0.01, x0.size), dtype=dt, device="cuda")
+    t = 0.625
+    et = m.potential_eval(h, prec, xd.data_ptr(), x0.size, d_ptr=d.data_ptr(), t=t)
+    xt = (xd + torch.tensor(t, dtype=dt, device="cuda") * d).double().cpu().numpy()
+    assert abs(et - o.energy(xt)) <= TOL[prec] * abs(o.energy(xt))
