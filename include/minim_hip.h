@@ -181,8 +181,12 @@ int minim_potential_eval_step(const minim_potential* pot, int precision_x, int p
                               size_t ndof, const void* d, double t, void* x_out, double* e, void* g, void* stream);
 /* Benchmark support (no reference counterpart): average duration in ms,
  * measured with HIP events on stream, of reps back-to-back launches of the
- * evaluation kernel alone (energy when want_energy, gradient into g when g
- * != NULL), without the cross-block reduction.  PhaseField only. */
+ * evaluation kernels (energy when want_energy, gradient into g when g !=
+ * NULL).  PhaseField: the stencil kernel without the cross-block reduction;
+ * PhaseFieldUnstructured: the element kernel (and the gather of shared sites
+ * with g); LjNd: the padding and force passes (cutoff mode) or the all-pairs
+ * kernel with its reduction; BarAndHinge: the energy pass with its reduction
+ * and the gradient pass.  MINIM_EINVAL for other potentials. */
 int minim_potential_time_kernel(const minim_potential* pot, int precision, const void* x, size_t ndof,
                                 int want_energy, void* g, int reps, void* stream, double* ms_per_launch);
 /* Potential::applyConstraints on a device vector (src/Potential.cpp:102-115,
