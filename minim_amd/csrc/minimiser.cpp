@@ -17,6 +17,7 @@
 //    energy of exactly the current coordinates; it is re-evaluated whenever
 //    that is not the case (after the unevaluated 10-failure fallback, a step
 //    without line search, or coordinates changed by the caller).
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <deque>
@@ -732,7 +733,7 @@ void Fire::iteration(State& state) {
     minim_amd::launch_dots(I.prec, pa, pb, 3, I.n, ws.rs(), s);
     constexpr int kStash = 48;
     double* res = ws.device_result();
-    std::vector<double> q(minim_amd::kMaxDots);
+    std::array<double, minim_amd::kMaxDots> q;
     int nfetch = 3;
     if (_maxForceDim > 0) {
       MINIM_HIP_CHECK(hipMemcpyAsync(res + kStash, res, sizeof(double) * 3, hipMemcpyDeviceToDevice, s));
