@@ -7,7 +7,7 @@
 //   Fire                 src/minimisers/Fire.cpp:27-90
 //   GradDescent          src/minimisers/GradDescent.cpp:27-46
 //
-// Every vector lives on the GPU; the host only holds scalars.  Two departures
+// Every vector lives on the GPU; the host only holds scalars.  Departures
 // from the reference's *implementation* (not its results):
 //  * L-BFGS keeps the (2m+1)^2 Gram matrix of {g, s_i, y_i} on the host and
 //    runs the two-loop recursion on coefficients, so a direction costs one
@@ -16,7 +16,13 @@
 //    linesearch.cpp:13) is taken from the previous accepted trial, which is the
 //    energy of exactly the current coordinates; it is re-evaluated whenever
 //    that is not the case (after the unevaluated 10-failure fallback, a step
-//    without line search, or coordinates changed by the caller).
+//    without line search, or coordinates changed by the caller);
+//  * L-BFGS's first trial at x + d is, where the potential supports it, one
+//    pass that also yields the gradient there and x + d itself, used as is
+//    when the trial is accepted (the same bits as trial, axpy and gradient);
+//  * Fire reads the scalars an iteration needs (g.g, the next v.g and v.v,
+//    the largest force, the potential's validity flag) in one host
+//    synchronisation.
 #include <array>
 #include <cmath>
 #include <cstdio>
