@@ -307,3 +307,22 @@ def test_element_form_full_size_256cubed(gpu):
     assert abs(e1 - e0) <= 1e-12 * abs(e0)
     assert (torch.linalg.norm(g1 - g0) / torch.linalg.norm(g0)).item() <= 1e-12
     assert g1.view(-1, 3).sum(dim=1).abs().max().item() < 1e-12
+
+
+def test_element_form_mixed_schedule_widens(gpu):
+    """The element form has no mixed FP64/FP32 evaluation: the mixed L-BFGS
+    schedule runs its FP32 phase and then widens every vector, and converges
+    to the FP64 run's minimum."""
+    m = gpu
+    c, o = valid_case(44)
+    for k in ("volume_fixed", "volume", "pressure", "force"):
+        c["opts"].pop(k, None)
+    o = oracle_of(c)
+    res = {}
+    for mode in ("fp64", "mixed"):
+        s = m.State(device_pf(m, c, "PhaseFieldUnstructured"), c["x0"])
+        lb = m.Lbfgs().setM(5).setMaxIter(3000).setPrecision(mode)
+        x = np.asarray(lb.minimise(s))
+        assert lb.stats["converged"], mode
+        res[mode] = o.energy(x)
+    assert abs(res["mixed"] - res["fp64"]) <= 1e-6 * max(1.0, abs(res["fp64"]))
