@@ -23,6 +23,7 @@ struct MinimiseStats {
   bool converged = false;
   double gradRms = 0;    // sqrt(g.g/ndof) of the last gradient
   double gradMax = 0;    // Fire with setMaxForceConvergence: largest per-particle force norm
+  double seconds = 0;    // wall time of the minimisation (host clock)
   double energy = 0;     // last accepted energy (FP64 accumulation)
 };
 
@@ -87,6 +88,19 @@ class Lbfgs : public NewMinimiser<Lbfgs> {
   void finish(State& state) override;
 
   int m() const { return _m; }
+
+  // Independent minimisations of many States at once, each with the
+  // semantics of minimise(state) under this minimiser's settings: one launch
+  // per phase (direction, line-search trial, step, gradient, Gram update)
+  // covers every instance still running, and one host synchronisation per
+  // phase serves them all.  Every State must hold an initialised one-fluid
+  // PhaseField without solid nodes, extra terms or fixed fluid
+  // (PhaseField::plainStencil), in FP64 ("state" or "fp64" precision), m <= 10
+  // and no adjustState callback.  The result is left in each State's
+  // coordinates; one MinimiseStats per State, in order.  The reference runs
+  // such instances as States on disjoint rank subsets (include/State.h:23,
+  // src/Communicator.cpp:219-240).
+  std::vector<MinimiseStats> minimiseBatch(const std::vector<State*>& states);
 
  private:
   int _m = 5;

@@ -90,6 +90,12 @@ class PhaseField : public NewPotential<PhaseField> {
   void applyConstraints(int prec, void* v, size_t n, void* stream) const override;
   SiteProjection projection() const override;
   bool hasGlobalConstraints() const override { return volumeFixed; }
+  // One fluid on a periodic grid with no solid nodes, extra terms or fixed
+  // fluid, after init: the canonical grid (size-1 dimensions first, the last
+  // one contiguous) and the bulk and gradient factors of the plain stencil
+  // (kappa/16 v and 0.25 kappa' v / res^2, PhaseField.cpp:372-390), as used by
+  // Lbfgs::minimiseBatch.  False for any other field.
+  bool plainStencil(int canon[3], double& fb, double& fg) const;
 
   // PhaseField::energyComponents (PhaseField.cpp:706-744): energy and gradient
   // norm (unconstrained) of each term at the given coordinates, computed on

@@ -263,6 +263,7 @@ typedef struct minim_result {
   double grad_rms;
   double dt_max; /* Fire: dtMax after init */
   double grad_max; /* Fire with max_force_dim: largest per-group norm of the last gradient */
+  double seconds;  /* wall time from the start of the call until this minimisation ended */
 } minim_result;
 
 typedef void (*minim_adjust_fn)(int iter, minim_state* state, void* user);
@@ -277,6 +278,15 @@ int minim_minimise(minim_minimiser* m, minim_state* s, minim_adjust_fn fn, void*
 /* Minimiser::minimise(state, "<fields>-<N>") (src/Minimiser.cpp:41-69). */
 int minim_minimise_log(minim_minimiser* m, minim_state* s, const char* log_type, double* coords_out,
                        minim_result* r);
+/* Independent L-BFGS minimisations of n_states States at once (Lbfgs::minimiseBatch):
+ * each with the semantics of minim_minimise on its own State (Lbfgs.cpp:41-133,
+ * linesearch.cpp:8-29, Minimiser.cpp:27-38), one launch per phase over every
+ * instance still running.  Replaces the reference's States on disjoint rank
+ * subsets (include/State.h:23, src/Communicator.cpp:219-240).  Every State
+ * holds an initialised one-fluid PhaseField without solid nodes, extra terms or
+ * fixed fluid, in FP64; m <= 10.  Results stay in each State's device
+ * coordinates; results[i] (may be NULL) describes states[i]. */
+int minim_minimise_batch(minim_minimiser* m, minim_state* const* states, int n_states, minim_result* results);
 
 #ifdef __cplusplus
 }

@@ -48,7 +48,7 @@ class Result(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("n_energy", C.c_long),
                 ("n_gradient", C.c_long), ("n_energy_reused", C.c_long), ("iter_fp32", C.c_int),
                 ("iter_fp64", C.c_int), ("energy", C.c_double), ("grad_rms", C.c_double),
-                ("dt_max", C.c_double), ("grad_max", C.c_double)]
+                ("dt_max", C.c_double), ("grad_max", C.c_double), ("seconds", C.c_double)]
 
 
 ADJUST_FN = C.CFUNCTYPE(None, C.c_int, vp, vp)
@@ -98,12 +98,13 @@ SIGNATURES = {
     "minim_minimiser_destroy": (C.c_int, [vp]),
     "minim_minimise": (C.c_int, [vp, vp, ADJUST_FN, vp, dp, C.POINTER(Result)]),
     "minim_minimise_log": (C.c_int, [vp, vp, C.c_char_p, dp, C.POINTER(Result)]),
+    "minim_minimise_batch": (C.c_int, [vp, C.POINTER(vp), C.c_int, C.POINTER(Result)]),
 }
 
 _lib = None
 
 
-ABI_VERSION = 4  # layout of the parameter/result structs below
+ABI_VERSION = 5  # layout of the parameter/result structs below
 
 
 def load():

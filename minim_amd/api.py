@@ -689,7 +689,7 @@ class Minimiser:
         self.stats = dict(iterations=r.iterations, converged=bool(r.converged), n_energy=r.n_energy,
                           n_gradient=r.n_gradient, n_energy_reused=r.n_energy_reused, iter_fp32=r.iter_fp32,
                           iter_fp64=r.iter_fp64, energy=r.energy, grad_rms=r.grad_rms, dt_max=r.dt_max,
-                          grad_max=r.grad_max)
+                          grad_max=r.grad_max, seconds=r.seconds)
 
     def minimise(self, state, adjust=None, download=True):
         """Minimiser::minimise(state, adjustState) / minimise(state, "<fields>-<N>")
@@ -748,6 +748,26 @@ class Lbfgs(Minimiser):
         self.switchRms = float(switchRms)
         self._h = None
         return self
+
+    def minimiseBatch(self, states):
+        """Lbfgs::minimiseBatch: independent minimisations of many States (one-fluid
+        PhaseFields on periodic grids, FP64) in one launch per phase.  Each
+        result stays in its State's device coordinates; returns one stats dict
+        per State, in order.  The reference runs such instances as States on
+        disjoint rank subsets (include/State.h:23)."""
+        L = A.load()
+        h = self._handle()
+        n = len(states)
+        arr = (C.c_void_p * max(n, 1))(*[s.ptr for s in states])
+        res = (A.Result * max(n, 1))()
+        A.check(L.minim_minimise_batch(h.ptr, arr, n, res))
+        out = []
+        for i in range(n):
+            r = res[i]
+            out.append(dict(iterations=r.iterations, converged=bool(r.converged), n_energy=r.n_energy,
+                            n_gradient=r.n_gradient, n_energy_reused=r.n_energy_reused, iter_fp32=r.iter_fp32,
+                            iter_fp64=r.iter_fp64, energy=r.energy, grad_rms=r.grad_rms, seconds=r.seconds))
+        return out
 
     def _params(self):
         p = super()._params()

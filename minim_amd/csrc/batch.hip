@@ -1,0 +1,703 @@
+// batch.hip -- many independent L-BFGS minimisations of one-fluid PhaseField
+// fields, one launch per phase over every instance still running.
+//
+// The reference runs independent minimisations as States on disjoint rank
+// subsets (include/State.h:23, src/Communicator.cpp:219-240), each with its
+// own Lbfgs (src/minimisers/Lbfgs.cpp:41-133) and backtracking line search
+// (src/minimisers/linesearch.cpp:8-29).  Here all instances' vectors are
+// concatenated in HBM and every phase of an iteration is one launch over the
+// blocks of the instances that take part in it:
+//
+//   direction   d = c_g g + sum_s c_D[s] D_s + c_Y[s] Y_s, with d.d, d.g, d.Y_s
+//   trial       energy at x + tau d (per-instance tau)
+//   step        x += tau d, then energy and gradient at x (+ g.g)
+//   Gram        y = g_new - g, with every dot the next recursion needs
+//
+// A block covers kBE consecutive elements of one instance; it finds its
+// instance by binary search over the prefix sums of the block counts of the
+// instances in the launch, and writes its partial sums to its own row.  A
+// second small kernel sums each instance's rows in block order, so every
+// per-instance result is the same whatever other instances share the launch.
+// The host keeps each instance's (2m+1)^2 Gram data and runs the two-loop
+// recursion on coefficients, as the single-State Lbfgs does; one host
+// synchronisation per phase serves all instances.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <deque>
+#include <stdexcept>
+#include <vector>
+
+#include "minim/Minimiser.h"
+#include "minim/PhaseField.h"
+#include "minim/State.h"
+#include "runtime.hpp"
+
+namespace minim_amd {
+namespace {
+
+constexpr int kBS = 11;            // history slots (m + 1 with m <= 10)
+constexpr int kBR = 8;             // elements per thread
+constexpr int kBE = kBlock * kBR;  // elements per block
+constexpr int kKP = 2;             // stencil partials: energy, g.g
+constexpr int kKD = 2 + kBS;       // direction: d.d, d.g, d.Y_s
+constexpr int kKG = 4 * kBS + 3;   // Gram (layout in bgram_kernel)
+
+struct BInst {
+  size_t off;  // first element in the concatenated vectors
+  int n;       // elements (sites)
+  int X, Y, Z;  // canonical grid, Z contiguous
+  double fb, fg, fbg, fg2;  // kappa/16 v, 0.25 kappa' v / res^2, 4 fb, 2 fg
+};
+
+struct BDyn {
+  int j;           // instance
+  int dslot;       // slot of the direction and of the new y
+  int gcur;        // G buffer that holds g
+  int gout;        // G buffer a gradient pass writes
+  unsigned dmask;  // D slots in the direction's basis / in the Gram dots
+  unsigned ymask;  // Y slots likewise (stored pairs only)
+  double tau;      // trial or step factor
+  double cg;       // coefficient of g in the direction
+  double cD[kBS], cY[kBS];
+};
+
+struct BBufs {
+  double* X;
+  double* G0;
+  double* G1;
+  double* D;  // kBS slots of N elements
+  double* Y;
+  size_t N;
+  const BInst* inst;
+  const BDyn* dyn;
+  const int* ablk;  // na + 1 prefix sums of the block counts
+  int na;
+  double* part;  // per-block partials, K per block
+};
+
+__device__ __forceinline__ int find_pos(const int* __restrict__ ablk, int na, int b) {
+  int lo = 0, hi = na - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ablk[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Block sums of the values whose bit is set in `on` (the others are 0),
+// written to part[blockIdx.x * K + k].
+template <int K>
+__device__ __forceinline__ void block_partials(const double (&v)[K], unsigned long long on, double* part) {
+  __shared__ double lds[(kBlock / kWave) * K];
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    double x = 0;
+    if ((on >> k) & 1ull) {
+      x = v[k];
+#pragma unroll
+      for (int off = kWave / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, kWave);
+    }
+    if (lane == 0) lds[wid * K + k] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < K) {
+    const int k = threadIdx.x;
+    double s = lds[k];
+    for (int w = 1; w < kBlock / kWave; w++) s += lds[w * K + k];
+    part[(size_t)blockIdx.x * K + k] = s;
+  }
+}
+
+// res[p * K + k] = sum of instance p's block rows in block order.
+__global__ __launch_bounds__(64) void bfinal_kernel(const double* __restrict__ part, const int* __restrict__ ablk,
+                                                    int K, double* __restrict__ res) {
+  const int p = blockIdx.x, k = threadIdx.x;
+  if (k >= K) return;
+  double s = 0;
+  for (int b = ablk[p]; b < ablk[p + 1]; b++) s += part[(size_t)b * K + k];
+  res[(size_t)p * K + k] = s;
+}
+
+// Energy (forward bonds) and gradient of one-fluid sites, the arithmetic of the
+// tiled kernel for one fluid (PhaseField.cpp:238-288, :372-378).  MODE bits:
+// 1 energy, 2 gradient into G[gout] with g.g, 4 at x + tau * d.
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void bpf_kernel(BBufs B) {
+  const int p = find_pos(B.ablk, B.na, blockIdx.x);
+  const BDyn& q = B.dyn[p];
+  const BInst I = B.inst[q.j];
+  const int lb = blockIdx.x - B.ablk[p];
+  const double* __restrict__ x = B.X + I.off;
+  const double* __restrict__ d = B.D + (size_t)q.dslot * B.N + I.off;
+  double* __restrict__ g = (q.gout ? B.G1 : B.G0) + I.off;
+  const double tau = q.tau;
+  const int Z = I.Z, Y = I.Y, X = I.X, YZ = Y * Z;
+  auto val = [&](int i) -> double {
+    if (MODE & 4) return x[i] + tau * d[i];
+    return x[i];
+  };
+  double acc[kKP] = {0, 0};
+#pragma unroll
+  for (int r = 0; r < kBR; r++) {
+    const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
+    if (i >= I.n) break;
+    const int z = i % Z, t2 = i / Z;
+    const int y = t2 % Y, xx = t2 / Y;
+    const int izm = z == 0 ? i + (Z - 1) : i - 1;
+    const int izp = z == Z - 1 ? i - (Z - 1) : i + 1;
+    const int iym = y == 0 ? i + (Y - 1) * Z : i - Z;
+    const int iyp = y == Y - 1 ? i - (Y - 1) * Z : i + Z;
+    const int ixm = xx == 0 ? i + (X - 1) * YZ : i - YZ;
+    const int ixp = xx == X - 1 ? i - (X - 1) * YZ : i + YZ;
+    const double ck = val(i);
+    const double dxp = ck - val(ixp), dyp = ck - val(iyp), dzp = ck - val(izp);
+    const double a2 = ck - 1.0;
+    if (MODE & 1) {
+      const double a1 = ck + 1.0;
+      const double be = I.fb * ((a1 * a1) * (a2 * a2));
+      double sq = dxp * dxp;
+      sq = fma(dyp, dyp, sq);
+      sq = fma(dzp, dzp, sq);
+      acc[0] += fma(I.fg, sq, be);
+    }
+    if (MODE & 2) {
+      const double dxm = ck - val(ixm), dym = ck - val(iym), dzm = ck - val(izm);
+      const double lap = ((dxm + dxp) + (dym + dyp)) + (dzm + dzp);
+      const double bg = I.fbg * (ck * fma(ck, ck, -1.0));
+      const double gv = fma(I.fg2, lap, bg);
+      g[i] = gv;
+      acc[1] += gv * gv;
+    }
+  }
+  block_partials<kKP>(acc, (MODE & 1 ? 1ull : 0ull) | (MODE & 2 ? 2ull : 0ull), B.part);
+}
+
+// d = cg g + sum_s cD[s] D_s + sum_s cY[s] Y_s into D[dslot]; partials d.d,
+// d.g, d.Y_s (s in ymask).
+__global__ __launch_bounds__(kBlock) void bdir_kernel(BBufs B) {
+  const int p = find_pos(B.ablk, B.na, blockIdx.x);
+  const BDyn& q = B.dyn[p];
+  const BInst I = B.inst[q.j];
+  const int lb = blockIdx.x - B.ablk[p];
+  const size_t N = B.N;
+  const double* __restrict__ g = (q.gcur ? B.G1 : B.G0) + I.off;
+  double* __restrict__ out = B.D + (size_t)q.dslot * N + I.off;
+  const double* __restrict__ Db = B.D + I.off;
+  const double* __restrict__ Yb = B.Y + I.off;
+  const unsigned dm = q.dmask, ym = q.ymask;
+  double acc[kKD];
+#pragma unroll
+  for (int k = 0; k < kKD; k++) acc[k] = 0;
+  for (int r = 0; r < kBR; r++) {
+    const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
+    if (i >= I.n) break;
+    const double gi = g[i];
+    double v = q.cg * gi;
+#pragma unroll
+    for (int s = 0; s < kBS; s++)
+      if ((dm >> s) & 1u) v = v + q.cD[s] * Db[(size_t)s * N + i];
+    double yv[kBS];
+#pragma unroll
+    for (int s = 0; s < kBS; s++) {
+      yv[s] = 0;
+      if ((ym >> s) & 1u) {
+        yv[s] = Yb[(size_t)s * N + i];
+        v = v + q.cY[s] * yv[s];
+      }
+    }
+    out[i] = v;
+    acc[0] += v * v;
+    acc[1] += v * gi;
+#pragma unroll
+    for (int s = 0; s < kBS; s++)
+      if ((ym >> s) & 1u) acc[2 + s] += v * yv[s];
+  }
+  block_partials<kKD>(acc, 3ull | ((unsigned long long)ym << 2), B.part);
+}
+
+// x += tau * d (product rounded first, as the single-State axpy).
+__global__ __launch_bounds__(kBlock) void baxpy_kernel(BBufs B) {
+  const int p = find_pos(B.ablk, B.na, blockIdx.x);
+  const BDyn& q = B.dyn[p];
+  const BInst I = B.inst[q.j];
+  const int lb = blockIdx.x - B.ablk[p];
+  double* __restrict__ x = B.X + I.off;
+  const double* __restrict__ d = B.D + (size_t)q.dslot * B.N + I.off;
+  for (int r = 0; r < kBR; r++) {
+    const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
+    if (i >= I.n) break;
+    x[i] = x[i] + q.tau * d[i];
+  }
+}
+
+// y = G[gout] - G[gcur] into Y[dslot]; partials with gn = G[gout]:
+//   [0,kBS) gn.D_s ; [kBS,2kBS) gn.Y_s ; 2kBS gn.gn ; 2kBS+1 gn.y ;
+//   [2kBS+2,3kBS+2) y.D_s ; [3kBS+2,4kBS+2) y.Y_s ; 4kBS+2 y.y
+// for s in dmask (D) and ymask (Y; the new slot is not in it).
+__global__ __launch_bounds__(kBlock) void bgram_kernel(BBufs B) {
+  const int p = find_pos(B.ablk, B.na, blockIdx.x);
+  const BDyn& q = B.dyn[p];
+  const BInst I = B.inst[q.j];
+  const int lb = blockIdx.x - B.ablk[p];
+  const size_t N = B.N;
+  const double* __restrict__ gn = (q.gout ? B.G1 : B.G0) + I.off;
+  const double* __restrict__ go = (q.gcur ? B.G1 : B.G0) + I.off;
+  double* __restrict__ yo = B.Y + (size_t)q.dslot * N + I.off;
+  const double* __restrict__ Db = B.D + I.off;
+  const double* __restrict__ Yb = B.Y + I.off;
+  const unsigned dm = q.dmask, ym = q.ymask;
+  double acc[kKG];
+#pragma unroll
+  for (int k = 0; k < kKG; k++) acc[k] = 0;
+  for (int r = 0; r < kBR; r++) {
+    const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
+    if (i >= I.n) break;
+    const double a = gn[i];
+    const double yv = a - go[i];
+    yo[i] = yv;
+#pragma unroll
+    for (int s = 0; s < kBS; s++)
+      if ((dm >> s) & 1u) {
+        const double dv = Db[(size_t)s * N + i];
+        acc[s] += a * dv;
+        acc[2 * kBS + 2 + s] += yv * dv;
+      }
+#pragma unroll
+    for (int s = 0; s < kBS; s++)
+      if ((ym >> s) & 1u) {
+        const double w = Yb[(size_t)s * N + i];
+        acc[kBS + s] += a * w;
+        acc[3 * kBS + 2 + s] += yv * w;
+      }
+    acc[2 * kBS] += a * a;
+    acc[2 * kBS + 1] += a * yv;
+    acc[4 * kBS + 2] += yv * yv;
+  }
+  const unsigned long long on = (unsigned long long)dm | ((unsigned long long)ym << kBS) |
+                                (3ull << (2 * kBS)) | ((unsigned long long)dm << (2 * kBS + 2)) |
+                                ((unsigned long long)ym << (3 * kBS + 2)) | (1ull << (4 * kBS + 2));
+  block_partials<kKG>(acc, on, B.part);
+}
+
+// Host side ------------------------------------------------------------------
+
+struct Pair {
+  int slot;
+  double tau;
+  double rho;
+};
+
+struct Inst {
+  minim::State* s = nullptr;
+  BInst bi{};
+  int nblk = 0;
+  double conv = 0;
+  std::deque<Pair> pairs;
+  std::vector<char> used;
+  double gg = 0, e = 0;
+  std::vector<double> dg, yg, dres;
+  std::vector<std::vector<double>> dy, yy;
+  int gcur = 0;
+  int iters = 0;
+  bool done = false;
+  minim::MinimiseStats st;
+  // the current iteration
+  int dslot = 0;
+  double tau0 = 1, tau = 1, t = 0, gs = 0;
+  int trial = 0;
+  bool searching = false;
+};
+
+// One launch group: the instances taking part, their per-phase parameters and
+// block prefix sums, staged in pinned memory and copied to the device.
+struct Launch {
+  std::vector<int> pos;  // instance index per active position
+  int na = 0, nb = 0;
+};
+
+}  // namespace
+}  // namespace minim_amd
+
+namespace minim {
+
+std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& states) {
+  using namespace minim_amd;
+  const int ni = (int)states.size();
+  std::vector<MinimiseStats> out(ni);
+  if (ni == 0) return out;
+  if (_m > kBS - 1) throw std::invalid_argument("Lbfgs::minimiseBatch: m must be at most 10.");
+  if (_precMode == "fp32" || _precMode == "mixed")
+    throw std::invalid_argument("Lbfgs::minimiseBatch runs in FP64 (precision \"state\" or \"fp64\").");
+
+  std::vector<Inst> in(ni);
+  size_t N = 0;
+  long totalBlocks = 0;
+  for (int j = 0; j < ni; j++) {
+    State* s = states[j];
+    if (!s) throw std::invalid_argument("Lbfgs::minimiseBatch: null State.");
+    for (int k = 0; k < j; k++)
+      if (states[k] == s) throw std::invalid_argument("Lbfgs::minimiseBatch: a State appears twice.");
+    if (_precMode == "fp64") s->setPrecision(FP64);
+    if (s->precision() != FP64)
+      throw std::invalid_argument("Lbfgs::minimiseBatch: every State must hold FP64 coordinates.");
+    const auto* pf = dynamic_cast<const PhaseField*>(s->pot.get());
+    int c[3];
+    double fb = 0, fg = 0;
+    if (!pf || !pf->plainStencil(c, fb, fg))
+      throw std::invalid_argument(
+          "Lbfgs::minimiseBatch: every State must hold a one-fluid PhaseField without solid nodes, extra "
+          "terms or fixed fluid.");
+    const size_t n = s->ndof;
+    if (n != (size_t)c[0] * c[1] * c[2] || n > (size_t)INT_MAX / 2)
+      throw std::invalid_argument("Lbfgs::minimiseBatch: unsupported field size.");
+    Inst& I = in[j];
+    I.s = s;
+    I.bi = BInst{N, (int)n, c[0], c[1], c[2], fb, fg, 4.0 * fb, 2.0 * fg};
+    I.nblk = (int)((n + kBE - 1) / kBE);
+    I.conv = s->convergence;
+    I.used.assign(_m + 1, 0);
+    I.dg.assign(kBS, 0.0);
+    I.yg.assign(kBS, 0.0);
+    I.dres.assign(kBS, 0.0);
+    I.dy.assign(kBS, std::vector<double>(kBS, 0.0));
+    I.yy.assign(kBS, std::vector<double>(kBS, 0.0));
+    N += n;
+    totalBlocks += I.nblk;
+  }
+  if (totalBlocks > INT_MAX / kKG) throw std::invalid_argument("Lbfgs::minimiseBatch: too many elements.");
+
+  hipStream_t st = static_cast<hipStream_t>(states[0]->stream());
+  MINIM_HIP_CHECK(hipDeviceSynchronize());  // every State's coordinates are final
+  const auto t0 = std::chrono::steady_clock::now();
+  auto since = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+
+  const int S = _m + 1;
+  DeviceBuffer X(N * 8), G0(N * 8), G1(N * 8), Dm(N * 8 * S), Ym(N * 8 * S);
+  DeviceBuffer instD(sizeof(BInst) * ni), dynD(sizeof(BDyn) * ni), ablkD(sizeof(int) * (ni + 1));
+  DeviceBuffer part(sizeof(double) * (size_t)totalBlocks * kKG), res(sizeof(double) * (size_t)ni * (kKP + kKG));
+  struct Pinned {
+    void* p = nullptr;
+    explicit Pinned(size_t b) { MINIM_HIP_CHECK(hipHostMalloc(&p, b)); }
+    ~Pinned() { (void)hipHostFree(p); }
+  };
+  Pinned dynH(sizeof(BDyn) * ni), ablkH(sizeof(int) * (ni + 1)), resH(sizeof(double) * (size_t)ni * (kKP + kKG));
+  BDyn* dynh = static_cast<BDyn*>(dynH.p);
+  int* ablkh = static_cast<int*>(ablkH.p);
+  double* resh = static_cast<double*>(resH.p);
+
+  {
+    std::vector<BInst> bi(ni);
+    for (int j = 0; j < ni; j++) bi[j] = in[j].bi;
+    MINIM_HIP_CHECK(hipMemcpyAsync(instD.get(), bi.data(), sizeof(BInst) * ni, hipMemcpyHostToDevice, st));
+    for (int j = 0; j < ni; j++)
+      device_copy(static_cast<double*>(X.get()) + in[j].bi.off, in[j].s->deviceCoords(), in[j].bi.n * 8, st);
+    MINIM_HIP_CHECK(hipStreamSynchronize(st));  // bi leaves scope
+  }
+
+  BBufs B{};
+  B.X = static_cast<double*>(X.get());
+  B.G0 = static_cast<double*>(G0.get());
+  B.G1 = static_cast<double*>(G1.get());
+  B.D = static_cast<double*>(Dm.get());
+  B.Y = static_cast<double*>(Ym.get());
+  B.N = N;
+  B.inst = static_cast<const BInst*>(instD.get());
+  B.dyn = static_cast<const BDyn*>(dynD.get());
+  B.ablk = static_cast<const int*>(ablkD.get());
+  B.part = static_cast<double*>(part.get());
+  double* resd = static_cast<double*>(res.get());
+
+  // Stage the instances listed in L (their BDyn already in dynh[0..na)) and
+  // upload; the pinned staging is rewritten only after the next fetch.
+  auto stage = [&](Launch& L) {
+    L.na = (int)L.pos.size();
+    ablkh[0] = 0;
+    for (int p = 0; p < L.na; p++) ablkh[p + 1] = ablkh[p] + in[L.pos[p]].nblk;
+    L.nb = ablkh[L.na];
+    MINIM_HIP_CHECK(hipMemcpyAsync(dynD.get(), dynh, sizeof(BDyn) * L.na, hipMemcpyHostToDevice, st));
+    MINIM_HIP_CHECK(hipMemcpyAsync(ablkD.get(), ablkh, sizeof(int) * (L.na + 1), hipMemcpyHostToDevice, st));
+    B.na = L.na;
+  };
+  auto finalize = [&](const Launch& L, int K, size_t resOff) {
+    bfinal_kernel<<<L.na, 64, 0, st>>>(B.part, B.ablk, K, resd + resOff);
+    MINIM_HIP_CHECK(hipGetLastError());
+  };
+  auto fetch = [&](size_t count) {
+    MINIM_HIP_CHECK(hipMemcpyAsync(resh, resd, sizeof(double) * count, hipMemcpyDeviceToHost, st));
+    MINIM_HIP_CHECK(hipStreamSynchronize(st));
+  };
+  auto dyn_base = [&](const Inst& I) {
+    BDyn q{};
+    q.j = (int)(&I - in.data());
+    q.dslot = I.dslot;
+    q.gcur = I.gcur;
+    q.gout = 1 - I.gcur;
+    q.tau = 0;
+    return q;
+  };
+  const bool backtracking = linesearch == "backtracking";
+  const bool spec = backtracking && _maxStep == 0;  // first trial at x + d, launched with the direction
+
+  // ---- iteration 0's gradient (and energy) at the initial coordinates
+  {
+    Launch L;
+    for (int j = 0; j < ni; j++) {
+      in[j].dslot = 0;
+      BDyn q = dyn_base(in[j]);
+      q.gout = 0;
+      dynh[L.pos.size()] = q;
+      L.pos.push_back(j);
+    }
+    stage(L);
+    bpf_kernel<3><<<L.nb, kBlock, 0, st>>>(B);
+    MINIM_HIP_CHECK(hipGetLastError());
+    finalize(L, kKP, 0);
+    fetch((size_t)L.na * kKP);
+    for (int p = 0; p < L.na; p++) {
+      Inst& I = in[L.pos[p]];
+      I.e = resh[p * kKP];
+      I.gg = resh[p * kKP + 1];
+      I.gcur = 0;
+      I.st.nGradient++;
+    }
+  }
+
+  int maxIters = 0;
+  for (;;) {
+    Launch L;
+    for (int j = 0; j < ni; j++)
+      if (!in[j].done) L.pos.push_back(j);
+    if (L.pos.empty()) break;
+
+    // ---- direction coefficients (two-loop recursion on the Gram data,
+    // Lbfgs.cpp:84-127) and the speculative first trial at x + d
+    for (int p = 0; p < (int)L.pos.size(); p++) {
+      Inst& I = in[L.pos[p]];
+      int spare = -1;
+      for (int s = 0; s <= _m; s++)
+        if (!I.used[s]) {
+          spare = s;
+          break;
+        }
+      I.dslot = spare;
+      BDyn q = dyn_base(I);
+      q.tau = 1.0;
+      const int P = (int)I.pairs.size();
+      for (auto& pr : I.pairs) {
+        q.dmask |= 1u << pr.slot;
+        q.ymask |= 1u << pr.slot;
+      }
+      if (P == 0) {
+        const double gnorm = std::sqrt(I.gg);
+        q.cg = (gnorm > 0) ? -(_initStep / gnorm) : -1.0;
+      } else {
+        std::vector<double> alpha(P), ag_s(P, 0.0), ay(P, 0.0);
+        double ag = -1.0;
+        auto sdotg = [&](int i) { return I.pairs[i].tau * I.dg[I.pairs[i].slot]; };
+        auto sdoty = [&](int i, int k) { return I.pairs[i].tau * I.dy[I.pairs[i].slot][I.pairs[k].slot]; };
+        auto ydoty = [&](int i, int k) { return I.yy[I.pairs[i].slot][I.pairs[k].slot]; };
+        for (int k = P - 1; k >= 0; k--) {
+          double sq = ag * sdotg(k);
+          for (int r = 0; r < P; r++) sq += ay[r] * sdoty(k, r);
+          alpha[k] = I.pairs[k].rho * sq;
+          ay[k] -= alpha[k];
+        }
+        const int nw = P - 1;
+        const double gamma = 1 / (I.pairs[nw].rho * ydoty(nw, nw));
+        ag *= gamma;
+        for (int r = 0; r < P; r++) ay[r] *= gamma;
+        for (int k = 0; k < P; k++) {
+          double yr = ag * I.yg[I.pairs[k].slot];
+          for (int r = 0; r < P; r++) yr += ag_s[r] * sdoty(r, k) + ay[r] * ydoty(k, r);
+          const double beta = I.pairs[k].rho * yr;
+          ag_s[k] += alpha[k] - beta;
+        }
+        q.cg = ag;
+        for (int r = 0; r < P; r++) {
+          q.cD[I.pairs[r].slot] = ag_s[r] * I.pairs[r].tau;
+          q.cY[I.pairs[r].slot] = ay[r];
+        }
+      }
+      dynh[p] = q;
+    }
+    stage(L);
+    bdir_kernel<<<L.nb, kBlock, 0, st>>>(B);
+    MINIM_HIP_CHECK(hipGetLastError());
+    finalize(L, kKD, 0);
+    const size_t trialOff = (size_t)L.na * kKD;
+    if (spec) {
+      bpf_kernel<5><<<L.nb, kBlock, 0, st>>>(B);
+      MINIM_HIP_CHECK(hipGetLastError());
+      finalize(L, kKP, trialOff);
+    }
+    fetch(trialOff + (spec ? (size_t)L.na * kKP : 0));
+
+    std::vector<int> search;
+    for (int p = 0; p < L.na; p++) {
+      Inst& I = in[L.pos[p]];
+      const double* r = resh + (size_t)p * kKD;
+      const double dd = r[0], dgv = r[1];
+      for (int s = 0; s < kBS; s++) I.dres[s] = r[2 + s];
+      const int P = (int)I.pairs.size();
+      double tau0 = 1.0;
+      if (P > 0 && _maxStep != 0) {  // Lbfgs.cpp:121-124
+        const double ss = std::sqrt(dd);
+        if (ss > _maxStep) tau0 = _maxStep / ss;
+      }
+      double gs = tau0 * dgv;  // Lbfgs.cpp:53-57
+      if (gs > 0) {
+        gs = -gs;
+        tau0 = -tau0;
+      }
+      I.tau0 = tau0;
+      I.tau = tau0;
+      I.trial = 0;
+      I.searching = false;
+      if (!backtracking) continue;
+      I.t = -0.5 * gs;  // linesearch.cpp:11-13
+      if (spec) {
+        I.st.nEnergy++;
+        if (tau0 == 1.0) {
+          const double e = resh[trialOff + (size_t)p * kKP];
+          if (I.e - e >= I.t) continue;  // accepted at x + d
+          I.tau *= 0.5;
+          I.t *= 0.5;
+          I.trial = 1;
+        }
+      }
+      I.searching = true;
+      search.push_back(L.pos[p]);
+    }
+
+    // ---- further trials of the instances still searching (linesearch.cpp:15-27)
+    while (!search.empty()) {
+      Launch T;
+      T.pos = search;
+      for (int p = 0; p < (int)search.size(); p++) {
+        BDyn q = dyn_base(in[search[p]]);
+        q.tau = in[search[p]].tau;
+        dynh[p] = q;
+      }
+      stage(T);
+      bpf_kernel<5><<<T.nb, kBlock, 0, st>>>(B);
+      MINIM_HIP_CHECK(hipGetLastError());
+      finalize(T, kKP, 0);
+      fetch((size_t)T.na * kKP);
+      std::vector<int> next;
+      for (int p = 0; p < T.na; p++) {
+        Inst& I = in[T.pos[p]];
+        I.st.nEnergy++;
+        const double e = resh[(size_t)p * kKP];
+        if (I.e - e >= I.t) {
+          I.searching = false;
+          continue;
+        }
+        I.tau *= 0.5;
+        I.t *= 0.5;
+        if (++I.trial >= 10) {
+          I.searching = false;  // x + tau0 * 2^-10, not evaluated by the reference
+          continue;
+        }
+        next.push_back(T.pos[p]);
+      }
+      search.swap(next);
+    }
+
+    // ---- step, new gradient and energy, Gram update (Lbfgs.cpp:58-80)
+    for (int p = 0; p < L.na; p++) {
+      Inst& I = in[L.pos[p]];
+      BDyn q = dyn_base(I);
+      q.tau = I.tau;
+      for (auto& pr : I.pairs) {
+        q.dmask |= 1u << pr.slot;
+        q.ymask |= 1u << pr.slot;
+      }
+      q.dmask |= 1u << I.dslot;
+      dynh[p] = q;
+    }
+    stage(L);
+    baxpy_kernel<<<L.nb, kBlock, 0, st>>>(B);
+    MINIM_HIP_CHECK(hipGetLastError());
+    bpf_kernel<3><<<L.nb, kBlock, 0, st>>>(B);
+    MINIM_HIP_CHECK(hipGetLastError());
+    finalize(L, kKP, 0);
+    const size_t gramOff = (size_t)L.na * kKP;
+    bgram_kernel<<<L.nb, kBlock, 0, st>>>(B);
+    MINIM_HIP_CHECK(hipGetLastError());
+    finalize(L, kKG, gramOff);
+    fetch(gramOff + (size_t)L.na * kKG);
+
+    for (int p = 0; p < L.na; p++) {
+      Inst& I = in[L.pos[p]];
+      I.e = resh[(size_t)p * kKP];
+      I.st.nGradient++;
+      const double* r = resh + gramOff + (size_t)p * kKG;
+      auto gD = [&](int s) { return r[s]; };
+      auto gY = [&](int s) { return r[kBS + s]; };
+      const double gng = r[2 * kBS], gny = r[2 * kBS + 1];
+      auto yD = [&](int s) { return r[2 * kBS + 2 + s]; };
+      auto yY = [&](int s) { return r[3 * kBS + 2 + s]; };
+      const double yyv = r[4 * kBS + 2];
+      const int ds = I.dslot;
+      std::vector<int> oldSlots;
+      for (auto& pr : I.pairs) oldSlots.push_back(pr.slot);
+      const double sy = I.tau * yD(ds);
+      if (sy != 0) {
+        for (int sl : oldSlots) {
+          I.dy[ds][sl] = I.dres[sl];  // d_new.y_old from the direction pass
+          I.dy[sl][ds] = yD(sl);
+          I.yy[ds][sl] = yY(sl);
+          I.yy[sl][ds] = yY(sl);
+        }
+        I.dy[ds][ds] = yD(ds);
+        I.yy[ds][ds] = yyv;
+        I.dg[ds] = gD(ds);
+        I.yg[ds] = gny;
+        I.used[ds] = 1;
+        I.pairs.push_back({ds, I.tau, 1 / sy});
+        if ((int)I.pairs.size() > _m) {
+          I.used[I.pairs.front().slot] = 0;
+          I.pairs.pop_front();
+        }
+      }
+      for (int sl : oldSlots) {
+        I.dg[sl] = gD(sl);
+        I.yg[sl] = gY(sl);
+      }
+      I.gg = gng;
+      I.gcur = 1 - I.gcur;
+      I.iters++;
+      I.st.iterFP64++;
+      // Minimiser::minimise (Minimiser.cpp:27-38) with Lbfgs::checkConvergence (Lbfgs.cpp:130-133)
+      const double rms = std::sqrt(I.gg / I.bi.n);
+      if (rms < I.conv) {
+        I.st.converged = true;
+        I.done = true;
+      } else if (I.iters > maxIter) {
+        I.done = true;
+      }
+      if (I.done) I.st.seconds = since();
+      if (I.iters > maxIters) maxIters = I.iters;
+    }
+  }
+
+  for (int j = 0; j < ni; j++) {
+    Inst& I = in[j];
+    device_copy(I.s->deviceCoords(), static_cast<double*>(X.get()) + I.bi.off, I.bi.n * 8, st);
+    I.s->invalidateHost();
+    I.st.energy = I.e;
+    I.st.gradRms = std::sqrt(I.gg / I.bi.n);
+    out[j] = I.st;
+  }
+  MINIM_HIP_CHECK(hipStreamSynchronize(st));
+  stats = MinimiseStats{};
+  iter = maxIters;
+  return out;
+}
+
+}  // namespace minim

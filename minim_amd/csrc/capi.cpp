@@ -114,7 +114,7 @@ void apply_pf_params(const minim_pf_params* p, PF* pf) {
 extern "C" {
 
 const char* minim_last_error(void) { return g_err.c_str(); }
-int minim_abi_version(void) { return 4; }
+int minim_abi_version(void) { return 5; }
 
 // ---------------------------------------------------------------- potentials
 int minim_pf_create(const minim_pf_params* p, minim_potential** out) {
@@ -640,6 +640,7 @@ static void fill_result(const minim_minimiser* m, minim_result* r) {
   auto* f = dynamic_cast<const minim::Fire*>(m->m.get());
   r->dt_max = f ? f->dtMax : 0.0;
   r->grad_max = st.gradMax;
+  r->seconds = st.seconds;
 }
 
 int minim_minimise(minim_minimiser* m, minim_state* s, minim_adjust_fn fn, void* user, double* coords_out,
@@ -656,6 +657,38 @@ int minim_minimise(minim_minimiser* m, minim_state* s, minim_adjust_fn fn, void*
       m->m->run(*s->s, adj);  // result stays in the State's device coordinates
     }
     fill_result(m, r);
+  });
+}
+
+int minim_minimise_batch(minim_minimiser* m, minim_state* const* states, int n_states, minim_result* results) {
+  return guard([&] {
+    need(m, "minimiser");
+    if (n_states < 0) throw std::invalid_argument("n_states must not be negative");
+    if (n_states > 0) need(states, "states");
+    auto* lb = dynamic_cast<minim::Lbfgs*>(m->m.get());
+    if (!lb) throw std::invalid_argument("batched minimisation needs an Lbfgs minimiser");
+    std::vector<minim::State*> v(n_states);
+    for (int i = 0; i < n_states; i++) {
+      need(states[i], "state");
+      v[i] = states[i]->s.get();
+    }
+    std::vector<minim::MinimiseStats> st = lb->minimiseBatch(v);
+    if (!results) return;
+    for (int i = 0; i < n_states; i++) {
+      minim_result& r = results[i];
+      r = minim_result{};
+      const int runs = st[i].iterFP32 + st[i].iterFP64;
+      r.iterations = st[i].converged ? runs - 1 : runs;  // Minimiser::iter at exit
+      r.converged = st[i].converged ? 1 : 0;
+      r.n_energy = st[i].nEnergy;
+      r.n_gradient = st[i].nGradient;
+      r.n_energy_reused = st[i].nEnergyReused;
+      r.iter_fp32 = st[i].iterFP32;
+      r.iter_fp64 = st[i].iterFP64;
+      r.energy = st[i].energy;
+      r.grad_rms = st[i].gradRms;
+      r.seconds = st[i].seconds;
+    }
   });
 }
 

@@ -24,6 +24,7 @@
 //    the largest force, the potential's validity flag) in one host
 //    synchronisation.
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <deque>
@@ -62,6 +63,7 @@ std::vector<double> Minimiser::minimise(State& state, std::function<void(int, St
 void Minimiser::run(State& state, std::function<void(int, State&)> adjustState) {
   if (!state.usesThisProc) return;
   stats = MinimiseStats{};
+  const auto t0 = std::chrono::steady_clock::now();
   init(state);
   for (iter = 0; iter <= maxIter; iter++) {
     if (adjustState) adjustState(iter, state);
@@ -74,6 +76,7 @@ void Minimiser::run(State& state, std::function<void(int, State&)> adjustState) 
   }
   finish(state);
   state.invalidateHost();
+  stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 std::vector<double> Minimiser::minimise(State& state, std::string logType) {

@@ -649,6 +649,18 @@ void PhaseField::evaluateMixedAsync(const void* x, const void* d, double t, bool
 
 // The tiled kernel with the trial point written out (no general kernel, no
 // fixed volumes: their projection needs the whole gradient first).
+bool PhaseField::plainStencil(int canon[3], double& fb, double& fg) const {
+  if (!args_) return false;
+  const PFArgs& a = args_of(args_);
+  if (a.use_ext || a.volfix || a.nf != 1 || a.fixmask) return false;
+  canon[0] = a.X;
+  canon[1] = a.Y;
+  canon[2] = a.Z;
+  fb = a.fb[0];
+  fg = a.fg[0];
+  return true;
+}
+
 bool PhaseField::supportsStep() const {
   if (!args_) return false;
   const PFArgs& a = args_of(args_);

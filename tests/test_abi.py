@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(L, n), n
     assert set(names) == set(_capi.SIGNATURES), "ctypes table and header disagree"
-    assert L.minim_abi_version() == 4
+    assert L.minim_abi_version() == 5
 
 
 def _pf_handle(grid, nf, st, iw, density=0, res=1.0, fix=None):
@@ -202,3 +202,29 @@ def test_element_form_and_fused_step_errors():
     np.testing.assert_array_equal(nv, np.ones(8))
     assert L.minim_pf_node_volumes(hu, nv.ctypes.data_as(A.dp), nv.ctypes.data_as(A.dp)) == A.MINIM_EINVAL
     L.minim_potential_destroy(hu)
+
+
+def test_batch_entry_point_errors():
+    """minim_minimise_batch without a GPU: an empty batch is a no-op, a negative
+    count, NULL states and a minimiser other than L-BFGS are rejected."""
+    from minim_amd import _capi as A
+
+    L = A.load()
+    p = A.MinimiserParams()
+    p.kind = A.MINIM_LBFGS
+    p.max_iter = 10
+    p.m = 5
+    p.init_step = 1.0
+    h = C.c_void_p()
+    assert L.minim_minimiser_create(C.byref(p), C.byref(h)) == 0
+    assert L.minim_minimise_batch(h, None, 0, None) == 0
+    assert L.minim_minimise_batch(h, None, -1, None) == A.MINIM_EINVAL
+    assert L.minim_minimise_batch(h, None, 2, None) == A.MINIM_EINVAL
+    arr = (C.c_void_p * 1)(None)
+    assert L.minim_minimise_batch(h, arr, 1, None) == A.MINIM_EINVAL
+    L.minim_minimiser_destroy(h)
+    p.kind = A.MINIM_FIRE
+    assert L.minim_minimiser_create(C.byref(p), C.byref(h)) == 0
+    assert L.minim_minimise_batch(h, None, 0, None) == A.MINIM_EINVAL
+    assert b"Lbfgs" in L.minim_last_error()
+    L.minim_minimiser_destroy(h)

@@ -1,0 +1,139 @@
+"""GPU tests of Lbfgs.minimiseBatch: many one-fluid PhaseField minimisations in
+one launch per phase.
+
+Each instance must behave as minimise(state) does on its own State with the
+reference's semantics (src/minimisers/Lbfgs.cpp:41-133,
+src/minimisers/linesearch.cpp:8-29, src/Minimiser.cpp:27-38):
+  * converged energies within 1e-8 relative of the oracle's FP64 L-BFGS (an
+    absolute floor of 1e-8 E_initial), with the energy the batch reports equal
+    to the oracle's energy at the batch's own coordinates within 1e-12;
+  * the first iteration (a steepest-descent step with backtracking) equal to
+    the single-State path's to rounding;
+  * an instance's result does not depend on which other instances share the
+    launches (bit for bit), since every per-instance reduction sums the same
+    block partials in the same order.
+"""
+import numpy as np
+import pytest
+
+from oracle_bind import OraclePF, oracle_lbfgs
+
+pytestmark = pytest.mark.gpu
+
+
+def cases(seed, sides=(8, 13, 32, 50, 64, 100), extra=((6, 7, 5), (40, 1, 1), (1, 24, 17), (2, 9, 1))):
+    rng = np.random.default_rng(seed)
+    out = []
+    for g in [(s, s, 1) for s in sides] + list(extra):
+        n = int(np.prod(g))
+        out.append(dict(grid=g, w=float(rng.uniform(1.0, 3.0)), x0=rng.uniform(-0.1, 0.1, n)))
+    return out
+
+
+def make_state(m, c):
+    pf = m.PhaseField().setNFluid(1).setGridSize(list(c["grid"])).setSurfaceTension(1.0).setInterfaceSize(c["w"])
+    return m.State(pf, c["x0"])
+
+
+def energy_ok(e, e_ref, e0, rel=1e-8):
+    return abs(e - e_ref) <= rel * max(abs(e_ref), abs(e0))
+
+
+def test_batch_vs_oracle(gpu):
+    m = gpu
+    cs = cases(7)
+    states = [make_state(m, c) for c in cs]
+    res = m.Lbfgs().setM(5).setMaxIter(20000).minimiseBatch(states)
+    assert len(res) == len(cs)
+    for c, s, r in zip(cs, states, res):
+        o = OraclePF(c["grid"], 1, 1.0, [1.0], [c["w"]])
+        xo, so = oracle_lbfgs(o, c["x0"], m=5, max_iter=20000)
+        e0 = o.energy(c["x0"])
+        assert so["converged"] and r["converged"], (c["grid"], r, so)
+        x = s.coords()
+        # the energy the batch reports is the energy of the coordinates it left
+        assert abs(r["energy"] - o.energy(x)) <= 1e-12 * max(abs(r["energy"]), abs(e0)), c["grid"]
+        assert energy_ok(r["energy"], so["energy"], e0), (c["grid"], r["energy"], so["energy"])
+        assert r["grad_rms"] < o.convergence
+        g = o.gradient(x)
+        assert np.sqrt(np.mean(g * g)) < 1.0001 * o.convergence
+        assert r["iter_fp64"] == r["iterations"] + 1 and r["iter_fp32"] == 0
+        assert abs(r["iterations"] - so["iterations"]) <= max(10, 0.3 * so["iterations"]), (r, so)
+
+
+def test_batch_first_iteration_matches_single(gpu):
+    """maxIter 0: one iteration, the first step of Lbfgs.cpp:91-96 with the line
+    search.  The gradient of the batch kernel uses the single-State kernel's
+    per-site arithmetic, so the step agrees to the rounding of the norms."""
+    m = gpu
+    cs = cases(8)
+    states = [make_state(m, c) for c in cs]
+    res = m.Lbfgs().setM(5).setMaxIter(0).minimiseBatch(states)
+    for c, s, r in zip(cs, states, res):
+        s1 = make_state(m, c)
+        lb = m.Lbfgs().setM(5).setMaxIter(0)
+        x1 = lb.minimise(s1)
+        xb = s.coords()
+        assert np.max(np.abs(xb - x1)) <= 1e-14 * max(1.0, np.max(np.abs(x1))), c["grid"]
+        assert r["iterations"] == 1 and not r["converged"]
+        assert abs(r["energy"] - s1.energy()) <= 1e-12 * abs(s1.energy())
+
+
+def test_batch_result_independent_of_company(gpu):
+    m = gpu
+    cs = cases(9, sides=(16, 33, 70), extra=((5, 6, 7),))
+    states = [make_state(m, c) for c in cs]
+    res = m.Lbfgs().setM(4).setMaxIter(300).minimiseBatch(states)
+    for i, c in enumerate(cs):
+        alone = make_state(m, c)
+        ra = m.Lbfgs().setM(4).setMaxIter(300).minimiseBatch([alone])[0]
+        assert np.array_equal(alone.coords(), states[i].coords()), c["grid"]
+        assert ra["energy"] == res[i]["energy"] and ra["iterations"] == res[i]["iterations"]
+        assert ra["n_energy"] == res[i]["n_energy"]
+
+
+def test_batch_settings(gpu):
+    """maxStep, initStep, m and the line search 'none' are honoured: the batch
+    result equals the oracle's under the same settings."""
+    m = gpu
+    cs = cases(11, sides=(12, 40), extra=((3, 4, 5),))
+    for kw in [dict(m=3, max_step=0.05), dict(m=7, init_step=0.3), dict(m=5, linesearch="none", max_iter=60)]:
+        states = [make_state(m, c) for c in cs]
+        lb = m.Lbfgs().setM(kw["m"]).setMaxIter(kw.get("max_iter", 20000))
+        if "max_step" in kw:
+            lb.setMaxStep(kw["max_step"])
+        if "init_step" in kw:
+            lb.setInitStep(kw["init_step"])
+        if "linesearch" in kw:
+            lb.setLinesearch(kw["linesearch"])
+        res = lb.minimiseBatch(states)
+        for c, s, r in zip(cs, states, res):
+            o = OraclePF(c["grid"], 1, 1.0, [1.0], [c["w"]])
+            xo, so = oracle_lbfgs(o, c["x0"], m=kw["m"], max_iter=kw.get("max_iter", 20000),
+                                  max_step=kw.get("max_step", 0.0), init_step=kw.get("init_step", 1.0),
+                                  linesearch=kw.get("linesearch", "backtracking"))
+            e0 = o.energy(c["x0"])
+            assert r["converged"] == so["converged"], (kw, c["grid"], r, so)
+            if so["converged"]:
+                assert energy_ok(r["energy"], so["energy"], e0), (kw, c["grid"])
+            else:
+                # a fixed iteration count without line search: the same trajectory to rounding
+                assert r["iterations"] == so["iterations"]
+                # (without a line search rounding differences grow along the trajectory)
+                assert energy_ok(r["energy"], o.energy(xo), e0), (kw, c["grid"])
+
+
+def test_batch_rejects_unsupported_fields(gpu):
+    m = gpu
+    ok = make_state(m, cases(3, sides=(8,), extra=())[0])
+    tern = m.State(m.PhaseField().setNFluid(3).setGridSize([4, 4, 4]).setSurfaceTension([1, 1, 1]),
+                   np.full(192, 1 / 3))
+    with pytest.raises(ValueError, match="one-fluid PhaseField"):
+        m.Lbfgs().minimiseBatch([ok, tern])
+    with pytest.raises(ValueError, match="FP64"):
+        m.Lbfgs().setPrecision("mixed").minimiseBatch([ok])
+    with pytest.raises(ValueError, match="at most 10"):
+        m.Lbfgs().setM(12).minimiseBatch([ok])
+    with pytest.raises(ValueError, match="twice"):
+        m.Lbfgs().minimiseBatch([ok, ok])
+    assert m.Lbfgs().minimiseBatch([]) == []
