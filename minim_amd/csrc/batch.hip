@@ -16,8 +16,10 @@
 // A block covers kBE consecutive elements of one instance; it finds its
 // instance by binary search over the prefix sums of the block counts of the
 // instances in the launch, and writes its partial sums to its own row.  A
-// second small kernel sums each instance's rows in block order, so every
+// second small kernel sums each instance's rows in a fixed order, so every
 // per-instance result is the same whatever other instances share the launch.
+// The direction and Gram passes are compiled for each number of stored pairs
+// (a block-uniform switch), so all their loads are unconditional.
 // The host keeps each instance's (2m+1)^2 Gram data and runs the two-loop
 // recursion on coefficients, as the single-State Lbfgs does; one host
 // synchronisation per phase serves all instances.
@@ -57,11 +59,11 @@ struct BDyn {
   int dslot;       // slot of the direction and of the new y
   int gcur;        // G buffer that holds g
   int gout;        // G buffer a gradient pass writes
-  unsigned dmask;  // D slots in the direction's basis / in the Gram dots
-  unsigned ymask;  // Y slots likewise (stored pairs only)
+  int np;          // stored pairs (oldest first) in the direction and Gram passes
+  int slot[kBS];   // their history slots
   double tau;      // trial or step factor
   double cg;       // coefficient of g in the direction
-  double cD[kBS], cY[kBS];
+  double cD[kBS], cY[kBS];  // coefficients of pair i's D and Y
 };
 
 struct BBufs {
@@ -113,14 +115,21 @@ __device__ __forceinline__ void block_partials(const double (&v)[K], unsigned lo
   }
 }
 
-// res[p * K + k] = sum of instance p's block rows in block order.
-__global__ __launch_bounds__(64) void bfinal_kernel(const double* __restrict__ part, const int* __restrict__ ablk,
-                                                    int K, double* __restrict__ res) {
-  const int p = blockIdx.x, k = threadIdx.x;
-  if (k >= K) return;
-  double s = 0;
-  for (int b = ablk[p]; b < ablk[p + 1]; b++) s += part[(size_t)b * K + k];
-  res[(size_t)p * K + k] = s;
+// res[p * K + k] = sum of instance p's block rows: lane l of a wavefront adds
+// rows l, l + 64, ..., then the 64 lane sums are added in a fixed tree, so the
+// result depends only on that instance's rows.
+__global__ __launch_bounds__(kBlock) void bfinal_kernel(const double* __restrict__ part,
+                                                        const int* __restrict__ ablk, int K,
+                                                        double* __restrict__ res) {
+  const int p = blockIdx.x, lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  const int b0 = ablk[p], nb = ablk[p + 1] - b0;
+  for (int k = w; k < K; k += kBlock / kWave) {
+    double s = 0;
+    for (int b = lane; b < nb; b += kWave) s += part[(size_t)(b0 + b) * K + k];
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kWave);
+    if (lane == 0) res[(size_t)p * K + k] = s;
+  }
 }
 
 // Energy (forward bonds) and gradient of one-fluid sites, the arithmetic of the
@@ -177,47 +186,72 @@ __global__ __launch_bounds__(kBlock) void bpf_kernel(BBufs B) {
   block_partials<kKP>(acc, (MODE & 1 ? 1ull : 0ull) | (MODE & 2 ? 2ull : 0ull), B.part);
 }
 
-// d = cg g + sum_s cD[s] D_s + sum_s cY[s] Y_s into D[dslot]; partials d.d,
-// d.g, d.Y_s (s in ymask).
+// d = cg g + sum_i cD[i] D_{slot i} + cY[i] Y_{slot i} (pairs oldest first, as
+// the single-State direction pass) into D[dslot]; partials d.d, d.g, d.y_i.
+// P is the number of stored pairs, so every load is unconditional.
+template <int P>
+__device__ __forceinline__ void bdir_body(const BBufs& B, const BDyn& q, const BInst& I, int lb) {
+  const size_t N = B.N;
+  const double* __restrict__ g = (q.gcur ? B.G1 : B.G0) + I.off;
+  double* __restrict__ out = B.D + (size_t)q.dslot * N + I.off;
+  const double* Dp[P > 0 ? P : 1];
+  const double* Yp[P > 0 ? P : 1];
+  double cd[P > 0 ? P : 1], cy[P > 0 ? P : 1];
+#pragma unroll
+  for (int k = 0; k < P; k++) {
+    Dp[k] = B.D + (size_t)q.slot[k] * N + I.off;
+    Yp[k] = B.Y + (size_t)q.slot[k] * N + I.off;
+    cd[k] = q.cD[k];
+    cy[k] = q.cY[k];
+  }
+  const double cg = q.cg;
+  double acc[kKD];
+#pragma unroll
+  for (int k = 0; k < kKD; k++) acc[k] = 0;
+#pragma unroll 2
+  for (int r = 0; r < kBR; r++) {
+    const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
+    if (i >= I.n) break;
+    const double gi = g[i];
+    double dv[P > 0 ? P : 1], yv[P > 0 ? P : 1];
+#pragma unroll
+    for (int k = 0; k < P; k++) {
+      dv[k] = Dp[k][i];
+      yv[k] = Yp[k][i];
+    }
+    double v = cg * gi;
+#pragma unroll
+    for (int k = 0; k < P; k++) v = v + cd[k] * dv[k];
+#pragma unroll
+    for (int k = 0; k < P; k++) v = v + cy[k] * yv[k];
+    out[i] = v;
+    acc[0] += v * v;
+    acc[1] += v * gi;
+#pragma unroll
+    for (int k = 0; k < P; k++) acc[2 + k] += v * yv[k];
+  }
+  constexpr unsigned long long on = (1ull << (2 + P)) - 1;
+  block_partials<kKD>(acc, on, B.part);
+}
+
 __global__ __launch_bounds__(kBlock) void bdir_kernel(BBufs B) {
   const int p = find_pos(B.ablk, B.na, blockIdx.x);
   const BDyn& q = B.dyn[p];
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
-  const size_t N = B.N;
-  const double* __restrict__ g = (q.gcur ? B.G1 : B.G0) + I.off;
-  double* __restrict__ out = B.D + (size_t)q.dslot * N + I.off;
-  const double* __restrict__ Db = B.D + I.off;
-  const double* __restrict__ Yb = B.Y + I.off;
-  const unsigned dm = q.dmask, ym = q.ymask;
-  double acc[kKD];
-#pragma unroll
-  for (int k = 0; k < kKD; k++) acc[k] = 0;
-  for (int r = 0; r < kBR; r++) {
-    const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
-    if (i >= I.n) break;
-    const double gi = g[i];
-    double v = q.cg * gi;
-#pragma unroll
-    for (int s = 0; s < kBS; s++)
-      if ((dm >> s) & 1u) v = v + q.cD[s] * Db[(size_t)s * N + i];
-    double yv[kBS];
-#pragma unroll
-    for (int s = 0; s < kBS; s++) {
-      yv[s] = 0;
-      if ((ym >> s) & 1u) {
-        yv[s] = Yb[(size_t)s * N + i];
-        v = v + q.cY[s] * yv[s];
-      }
-    }
-    out[i] = v;
-    acc[0] += v * v;
-    acc[1] += v * gi;
-#pragma unroll
-    for (int s = 0; s < kBS; s++)
-      if ((ym >> s) & 1u) acc[2 + s] += v * yv[s];
+  switch (q.np) {
+    case 0: bdir_body<0>(B, q, I, lb); break;
+    case 1: bdir_body<1>(B, q, I, lb); break;
+    case 2: bdir_body<2>(B, q, I, lb); break;
+    case 3: bdir_body<3>(B, q, I, lb); break;
+    case 4: bdir_body<4>(B, q, I, lb); break;
+    case 5: bdir_body<5>(B, q, I, lb); break;
+    case 6: bdir_body<6>(B, q, I, lb); break;
+    case 7: bdir_body<7>(B, q, I, lb); break;
+    case 8: bdir_body<8>(B, q, I, lb); break;
+    case 9: bdir_body<9>(B, q, I, lb); break;
+    default: bdir_body<10>(B, q, I, lb); break;
   }
-  block_partials<kKD>(acc, 3ull | ((unsigned long long)ym << 2), B.part);
 }
 
 // x += tau * d (product rounded first, as the single-State axpy).
@@ -228,60 +262,87 @@ __global__ __launch_bounds__(kBlock) void baxpy_kernel(BBufs B) {
   const int lb = blockIdx.x - B.ablk[p];
   double* __restrict__ x = B.X + I.off;
   const double* __restrict__ d = B.D + (size_t)q.dslot * B.N + I.off;
+  const double tau = q.tau;
+#pragma unroll
   for (int r = 0; r < kBR; r++) {
     const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
     if (i >= I.n) break;
-    x[i] = x[i] + q.tau * d[i];
+    x[i] = x[i] + tau * d[i];
   }
 }
 
-// y = G[gout] - G[gcur] into Y[dslot]; partials with gn = G[gout]:
-//   [0,kBS) gn.D_s ; [kBS,2kBS) gn.Y_s ; 2kBS gn.gn ; 2kBS+1 gn.y ;
-//   [2kBS+2,3kBS+2) y.D_s ; [3kBS+2,4kBS+2) y.Y_s ; 4kBS+2 y.y
-// for s in dmask (D) and ymask (Y; the new slot is not in it).
+// y = G[gout] - G[gcur] into Y[dslot]; partials with gn = G[gout], the D list
+// being the P stored pairs then the new direction and the Y list the P pairs:
+//   [0,kBS) gn.D_i ; [kBS,2kBS) gn.Y_i ; 2kBS gn.gn ; 2kBS+1 gn.y ;
+//   [2kBS+2,3kBS+2) y.D_i ; [3kBS+2,4kBS+2) y.Y_i ; 4kBS+2 y.y
+template <int P>
+__device__ __forceinline__ void bgram_body(const BBufs& B, const BDyn& q, const BInst& I, int lb) {
+  const size_t N = B.N;
+  const double* __restrict__ gn = (q.gout ? B.G1 : B.G0) + I.off;
+  const double* __restrict__ go = (q.gcur ? B.G1 : B.G0) + I.off;
+  double* __restrict__ yo = B.Y + (size_t)q.dslot * N + I.off;
+  const double* Dp[P + 1];
+  const double* Yp[P > 0 ? P : 1];
+#pragma unroll
+  for (int k = 0; k < P; k++) {
+    Dp[k] = B.D + (size_t)q.slot[k] * N + I.off;
+    Yp[k] = B.Y + (size_t)q.slot[k] * N + I.off;
+  }
+  Dp[P] = B.D + (size_t)q.dslot * N + I.off;
+  double acc[kKG];
+#pragma unroll
+  for (int k = 0; k < kKG; k++) acc[k] = 0;
+#pragma unroll 2
+  for (int r = 0; r < kBR; r++) {
+    const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
+    if (i >= I.n) break;
+    const double a = gn[i];
+    const double b = go[i];
+    double dv[P + 1], wv[P > 0 ? P : 1];
+#pragma unroll
+    for (int k = 0; k <= P; k++) dv[k] = Dp[k][i];
+#pragma unroll
+    for (int k = 0; k < P; k++) wv[k] = Yp[k][i];
+    const double yv = a - b;
+    yo[i] = yv;
+#pragma unroll
+    for (int k = 0; k <= P; k++) {
+      acc[k] += a * dv[k];
+      acc[2 * kBS + 2 + k] += yv * dv[k];
+    }
+#pragma unroll
+    for (int k = 0; k < P; k++) {
+      acc[kBS + k] += a * wv[k];
+      acc[3 * kBS + 2 + k] += yv * wv[k];
+    }
+    acc[2 * kBS] += a * a;
+    acc[2 * kBS + 1] += a * yv;
+    acc[4 * kBS + 2] += yv * yv;
+  }
+  constexpr unsigned long long dm = (1ull << (P + 1)) - 1, ym = (1ull << P) - 1;
+  constexpr unsigned long long on = dm | (ym << kBS) | (3ull << (2 * kBS)) | (dm << (2 * kBS + 2)) |
+                                    (ym << (3 * kBS + 2)) | (1ull << (4 * kBS + 2));
+  block_partials<kKG>(acc, on, B.part);
+}
+
 __global__ __launch_bounds__(kBlock) void bgram_kernel(BBufs B) {
   const int p = find_pos(B.ablk, B.na, blockIdx.x);
   const BDyn& q = B.dyn[p];
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
-  const size_t N = B.N;
-  const double* __restrict__ gn = (q.gout ? B.G1 : B.G0) + I.off;
-  const double* __restrict__ go = (q.gcur ? B.G1 : B.G0) + I.off;
-  double* __restrict__ yo = B.Y + (size_t)q.dslot * N + I.off;
-  const double* __restrict__ Db = B.D + I.off;
-  const double* __restrict__ Yb = B.Y + I.off;
-  const unsigned dm = q.dmask, ym = q.ymask;
-  double acc[kKG];
-#pragma unroll
-  for (int k = 0; k < kKG; k++) acc[k] = 0;
-  for (int r = 0; r < kBR; r++) {
-    const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
-    if (i >= I.n) break;
-    const double a = gn[i];
-    const double yv = a - go[i];
-    yo[i] = yv;
-#pragma unroll
-    for (int s = 0; s < kBS; s++)
-      if ((dm >> s) & 1u) {
-        const double dv = Db[(size_t)s * N + i];
-        acc[s] += a * dv;
-        acc[2 * kBS + 2 + s] += yv * dv;
-      }
-#pragma unroll
-    for (int s = 0; s < kBS; s++)
-      if ((ym >> s) & 1u) {
-        const double w = Yb[(size_t)s * N + i];
-        acc[kBS + s] += a * w;
-        acc[3 * kBS + 2 + s] += yv * w;
-      }
-    acc[2 * kBS] += a * a;
-    acc[2 * kBS + 1] += a * yv;
-    acc[4 * kBS + 2] += yv * yv;
+  switch (q.np) {
+    case 0: bgram_body<0>(B, q, I, lb); break;
+    case 1: bgram_body<1>(B, q, I, lb); break;
+    case 2: bgram_body<2>(B, q, I, lb); break;
+    case 3: bgram_body<3>(B, q, I, lb); break;
+    case 4: bgram_body<4>(B, q, I, lb); break;
+    case 5: bgram_body<5>(B, q, I, lb); break;
+    case 6: bgram_body<6>(B, q, I, lb); break;
+    case 7: bgram_body<7>(B, q, I, lb); break;
+    case 8: bgram_body<8>(B, q, I, lb); break;
+    case 9: bgram_body<9>(B, q, I, lb); break;
+    default: bgram_body<10>(B, q, I, lb); break;
   }
-  const unsigned long long on = (unsigned long long)dm | ((unsigned long long)ym << kBS) |
-                                (3ull << (2 * kBS)) | ((unsigned long long)dm << (2 * kBS + 2)) |
-                                ((unsigned long long)ym << (3 * kBS + 2)) | (1ull << (4 * kBS + 2));
-  block_partials<kKG>(acc, on, B.part);
 }
 
 // Host side ------------------------------------------------------------------
@@ -378,16 +439,20 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
 
   const int S = _m + 1;
   DeviceBuffer X(N * 8), G0(N * 8), G1(N * 8), Dm(N * 8 * S), Ym(N * 8 * S);
-  DeviceBuffer instD(sizeof(BInst) * ni), dynD(sizeof(BDyn) * ni), ablkD(sizeof(int) * (ni + 1));
+  // per-phase staging, one copy per phase: block prefix sums, then the BDyn of
+  // each instance in the launch
+  const size_t ablkBytes = (sizeof(int) * (ni + 1) + 15) / 16 * 16;
+  const size_t stageBytes = ablkBytes + sizeof(BDyn) * ni;
+  DeviceBuffer instD(sizeof(BInst) * ni), stageD(stageBytes);
   DeviceBuffer part(sizeof(double) * (size_t)totalBlocks * kKG), res(sizeof(double) * (size_t)ni * (kKP + kKG));
   struct Pinned {
     void* p = nullptr;
     explicit Pinned(size_t b) { MINIM_HIP_CHECK(hipHostMalloc(&p, b)); }
     ~Pinned() { (void)hipHostFree(p); }
   };
-  Pinned dynH(sizeof(BDyn) * ni), ablkH(sizeof(int) * (ni + 1)), resH(sizeof(double) * (size_t)ni * (kKP + kKG));
-  BDyn* dynh = static_cast<BDyn*>(dynH.p);
-  int* ablkh = static_cast<int*>(ablkH.p);
+  Pinned stageH(stageBytes), resH(sizeof(double) * (size_t)ni * (kKP + kKG));
+  int* ablkh = static_cast<int*>(stageH.p);
+  BDyn* dynh = reinterpret_cast<BDyn*>(static_cast<char*>(stageH.p) + ablkBytes);
   double* resh = static_cast<double*>(resH.p);
 
   {
@@ -407,8 +472,8 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
   B.Y = static_cast<double*>(Ym.get());
   B.N = N;
   B.inst = static_cast<const BInst*>(instD.get());
-  B.dyn = static_cast<const BDyn*>(dynD.get());
-  B.ablk = static_cast<const int*>(ablkD.get());
+  B.ablk = static_cast<const int*>(stageD.get());
+  B.dyn = reinterpret_cast<const BDyn*>(static_cast<const char*>(stageD.get()) + ablkBytes);
   B.part = static_cast<double*>(part.get());
   double* resd = static_cast<double*>(res.get());
 
@@ -419,12 +484,12 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
     ablkh[0] = 0;
     for (int p = 0; p < L.na; p++) ablkh[p + 1] = ablkh[p] + in[L.pos[p]].nblk;
     L.nb = ablkh[L.na];
-    MINIM_HIP_CHECK(hipMemcpyAsync(dynD.get(), dynh, sizeof(BDyn) * L.na, hipMemcpyHostToDevice, st));
-    MINIM_HIP_CHECK(hipMemcpyAsync(ablkD.get(), ablkh, sizeof(int) * (L.na + 1), hipMemcpyHostToDevice, st));
+    MINIM_HIP_CHECK(
+        hipMemcpyAsync(stageD.get(), stageH.p, ablkBytes + sizeof(BDyn) * L.na, hipMemcpyHostToDevice, st));
     B.na = L.na;
   };
   auto finalize = [&](const Launch& L, int K, size_t resOff) {
-    bfinal_kernel<<<L.na, 64, 0, st>>>(B.part, B.ablk, K, resd + resOff);
+    bfinal_kernel<<<L.na, kBlock, 0, st>>>(B.part, B.ablk, K, resd + resOff);
     MINIM_HIP_CHECK(hipGetLastError());
   };
   auto fetch = [&](size_t count) {
@@ -488,10 +553,8 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
       BDyn q = dyn_base(I);
       q.tau = 1.0;
       const int P = (int)I.pairs.size();
-      for (auto& pr : I.pairs) {
-        q.dmask |= 1u << pr.slot;
-        q.ymask |= 1u << pr.slot;
-      }
+      q.np = P;
+      for (int i = 0; i < P; i++) q.slot[i] = I.pairs[i].slot;
       if (P == 0) {
         const double gnorm = std::sqrt(I.gg);
         q.cg = (gnorm > 0) ? -(_initStep / gnorm) : -1.0;
@@ -519,8 +582,8 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
         }
         q.cg = ag;
         for (int r = 0; r < P; r++) {
-          q.cD[I.pairs[r].slot] = ag_s[r] * I.pairs[r].tau;
-          q.cY[I.pairs[r].slot] = ay[r];
+          q.cD[r] = ag_s[r] * I.pairs[r].tau;
+          q.cY[r] = ay[r];
         }
       }
       dynh[p] = q;
@@ -542,8 +605,8 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
       Inst& I = in[L.pos[p]];
       const double* r = resh + (size_t)p * kKD;
       const double dd = r[0], dgv = r[1];
-      for (int s = 0; s < kBS; s++) I.dres[s] = r[2 + s];
       const int P = (int)I.pairs.size();
+      for (int i = 0; i < P; i++) I.dres[I.pairs[i].slot] = r[2 + i];  // d.y of each stored pair
       double tau0 = 1.0;
       if (P > 0 && _maxStep != 0) {  // Lbfgs.cpp:121-124
         const double ss = std::sqrt(dd);
@@ -613,11 +676,8 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
       Inst& I = in[L.pos[p]];
       BDyn q = dyn_base(I);
       q.tau = I.tau;
-      for (auto& pr : I.pairs) {
-        q.dmask |= 1u << pr.slot;
-        q.ymask |= 1u << pr.slot;
-      }
-      q.dmask |= 1u << I.dslot;
+      q.np = (int)I.pairs.size();
+      for (int i = 0; i < q.np; i++) q.slot[i] = I.pairs[i].slot;
       dynh[p] = q;
     }
     stage(L);
@@ -637,26 +697,28 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
       I.e = resh[(size_t)p * kKP];
       I.st.nGradient++;
       const double* r = resh + gramOff + (size_t)p * kKG;
-      auto gD = [&](int s) { return r[s]; };
-      auto gY = [&](int s) { return r[kBS + s]; };
+      auto gD = [&](int i) { return r[i]; };  // by position: stored pairs, then the new direction
+      auto gY = [&](int i) { return r[kBS + i]; };
       const double gng = r[2 * kBS], gny = r[2 * kBS + 1];
-      auto yD = [&](int s) { return r[2 * kBS + 2 + s]; };
-      auto yY = [&](int s) { return r[3 * kBS + 2 + s]; };
+      auto yD = [&](int i) { return r[2 * kBS + 2 + i]; };
+      auto yY = [&](int i) { return r[3 * kBS + 2 + i]; };
       const double yyv = r[4 * kBS + 2];
       const int ds = I.dslot;
+      const int P = (int)I.pairs.size();
       std::vector<int> oldSlots;
       for (auto& pr : I.pairs) oldSlots.push_back(pr.slot);
-      const double sy = I.tau * yD(ds);
+      const double sy = I.tau * yD(P);
       if (sy != 0) {
-        for (int sl : oldSlots) {
+        for (int i = 0; i < P; i++) {
+          const int sl = oldSlots[i];
           I.dy[ds][sl] = I.dres[sl];  // d_new.y_old from the direction pass
-          I.dy[sl][ds] = yD(sl);
-          I.yy[ds][sl] = yY(sl);
-          I.yy[sl][ds] = yY(sl);
+          I.dy[sl][ds] = yD(i);
+          I.yy[ds][sl] = yY(i);
+          I.yy[sl][ds] = yY(i);
         }
-        I.dy[ds][ds] = yD(ds);
+        I.dy[ds][ds] = yD(P);
         I.yy[ds][ds] = yyv;
-        I.dg[ds] = gD(ds);
+        I.dg[ds] = gD(P);
         I.yg[ds] = gny;
         I.used[ds] = 1;
         I.pairs.push_back({ds, I.tau, 1 / sy});
@@ -665,9 +727,9 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
           I.pairs.pop_front();
         }
       }
-      for (int sl : oldSlots) {
-        I.dg[sl] = gD(sl);
-        I.yg[sl] = gY(sl);
+      for (int i = 0; i < P; i++) {
+        I.dg[oldSlots[i]] = gD(i);
+        I.yg[oldSlots[i]] = gY(i);
       }
       I.gg = gng;
       I.gcur = 1 - I.gcur;
