@@ -456,23 +456,26 @@ def batch_leg(m, n_inst=512, workers=8, seed=4040, cpu_budget=10.0):
         return m.PhaseField().setNFluid(1).setGridSize([s, s, 1]).setSurfaceTension(1.0).setInterfaceSize(
             float(widths[j]))
 
-    # ---- one launch per phase over every running instance
-    t = time.perf_counter()
-    bstates = [m.State(make_pf(j), inputs(j), "fp64") for j in range(n_inst)]
-    torch.cuda.synchronize()
-    bsetup = time.perf_counter() - t
-    t = time.perf_counter()
-    bres = m.Lbfgs().setM(5).setMaxIter(20000).minimiseBatch(bstates)
-    torch.cuda.synchronize()
-    bwall = time.perf_counter() - t
-    bsecs = np.array([r["seconds"] for r in bres])
-    bevals = sum(r["n_energy"] + r["n_gradient"] for r in bres)
-    batched = dict(setup_s=bsetup, wall_s=bwall, evals=bevals, evals_per_s=bevals / bwall,
-                   instances_per_s=n_inst / bwall, converged=int(sum(r["converged"] for r in bres)),
-                   p50_s=float(np.percentile(bsecs, 50)), p99_s=float(np.percentile(bsecs, 99)),
-                   iterations_total=int(sum(r["iterations"] for r in bres)),
-                   iterations_max=int(max(r["iterations"] for r in bres)))
-    del bstates
+    # ---- one launch per phase over every running instance, FP64 and mixed
+    batched = {}
+    for prec in ("fp64", "mixed"):
+        t = time.perf_counter()
+        bstates = [m.State(make_pf(j), inputs(j), "fp64") for j in range(n_inst)]
+        torch.cuda.synchronize()
+        bsetup = time.perf_counter() - t
+        t = time.perf_counter()
+        bres = m.Lbfgs().setM(5).setMaxIter(20000).setPrecision(prec).minimiseBatch(bstates)
+        torch.cuda.synchronize()
+        bwall = time.perf_counter() - t
+        bsecs = np.array([r["seconds"] for r in bres])
+        bevals = sum(r["n_energy"] + r["n_gradient"] for r in bres)
+        batched[prec] = dict(setup_s=bsetup, wall_s=bwall, evals=bevals, evals_per_s=bevals / bwall,
+                             instances_per_s=n_inst / bwall, converged=int(sum(r["converged"] for r in bres)),
+                             p50_s=float(np.percentile(bsecs, 50)), p99_s=float(np.percentile(bsecs, 99)),
+                             iterations_total=int(sum(r["iterations"] for r in bres)),
+                             iterations_max=int(max(r["iterations"] for r in bres)),
+                             iterations_fp32=int(sum(r["iter_fp32"] for r in bres)))
+        del bstates
 
     # ---- one State per call, spread over host threads
     order = sorted(range(n_inst), key=lambda j: -sides[j])

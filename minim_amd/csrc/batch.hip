@@ -52,6 +52,24 @@ struct BInst {
   int n;       // elements (sites)
   int X, Y, Z;  // canonical grid, Z contiguous
   double fb, fg, fbg, fg2;  // kappa/16 v, 0.25 kappa' v / res^2, 4 fb, 2 fg
+  float fb_f, fg_f, fbg_f, fg2_f;  // the same rounded to FP32
+};
+
+template <typename T>
+struct BC;
+template <>
+struct BC<double> {
+  __device__ static double fb(const BInst& I) { return I.fb; }
+  __device__ static double fg(const BInst& I) { return I.fg; }
+  __device__ static double fbg(const BInst& I) { return I.fbg; }
+  __device__ static double fg2(const BInst& I) { return I.fg2; }
+};
+template <>
+struct BC<float> {
+  __device__ static float fb(const BInst& I) { return I.fb_f; }
+  __device__ static float fg(const BInst& I) { return I.fg_f; }
+  __device__ static float fbg(const BInst& I) { return I.fbg_f; }
+  __device__ static float fg2(const BInst& I) { return I.fg2_f; }
 };
 
 struct BDyn {
@@ -60,18 +78,22 @@ struct BDyn {
   int gcur;        // G buffer that holds g
   int gout;        // G buffer a gradient pass writes
   int np;          // stored pairs (oldest first) in the direction and Gram passes
+  int lp;          // 1: FP32 coordinates and arithmetic (first phase of the mixed schedule)
   int slot[kBS];   // their history slots
   double tau;      // trial or step factor
   double cg;       // coefficient of g in the direction
   double cD[kBS], cY[kBS];  // coefficients of pair i's D and Y
 };
 
+// Vectors G, D and Y are stored in TV (double, or float for the mixed
+// schedule); coordinates always in FP64 (holding FP32 values while an
+// instance runs in FP32).
 struct BBufs {
   double* X;
-  double* G0;
-  double* G1;
-  double* D;  // kBS slots of N elements
-  double* Y;
+  void* G0;
+  void* G1;
+  void* D;  // m + 1 slots of N elements
+  void* Y;
   size_t N;
   const BInst* inst;
   const BDyn* dyn;
@@ -133,22 +155,24 @@ __global__ __launch_bounds__(kBlock) void bfinal_kernel(const double* __restrict
 }
 
 // Energy (forward bonds) and gradient of one-fluid sites, the arithmetic of the
-// tiled kernel for one fluid (PhaseField.cpp:238-288, :372-378).  MODE bits:
-// 1 energy, 2 gradient into G[gout] with g.g, 4 at x + tau * d.
-template <int MODE>
-__global__ __launch_bounds__(kBlock) void bpf_kernel(BBufs B) {
-  const int p = find_pos(B.ablk, B.na, blockIdx.x);
-  const BDyn& q = B.dyn[p];
-  const BInst I = B.inst[q.j];
-  const int lb = blockIdx.x - B.ablk[p];
+// tiled kernel for one fluid (PhaseField.cpp:238-288, :372-378) in T (FP32
+// for an instance in the first phase of the mixed schedule), vectors in TV.
+// MODE bits: 1 energy, 2 gradient into G[gout] with g.g, 4 at x + tau * d.
+template <int MODE, typename T, typename TV>
+__device__ __forceinline__ void bpf_body(const BBufs& B, const BDyn& q, const BInst& I, int lb) {
+  using C = BC<T>;
   const double* __restrict__ x = B.X + I.off;
-  const double* __restrict__ d = B.D + (size_t)q.dslot * B.N + I.off;
-  double* __restrict__ g = (q.gout ? B.G1 : B.G0) + I.off;
-  const double tau = q.tau;
+  const TV* __restrict__ d = static_cast<const TV*>(B.D) + (size_t)q.dslot * B.N + I.off;
+  TV* __restrict__ g = static_cast<TV*>(q.gout ? B.G1 : B.G0) + I.off;
+  const T tau = (T)q.tau;
   const int Z = I.Z, Y = I.Y, X = I.X, YZ = Y * Z;
-  auto val = [&](int i) -> double {
-    if (MODE & 4) return x[i] + tau * d[i];
-    return x[i];
+  auto val = [&](int i) -> T {
+    const T xv = (T)x[i];
+    if (MODE & 4) {
+      const T st = tau * (T)d[i];
+      return xv + st;
+    }
+    return xv;
   };
   double acc[kKP] = {0, 0};
 #pragma unroll
@@ -163,44 +187,69 @@ __global__ __launch_bounds__(kBlock) void bpf_kernel(BBufs B) {
     const int iyp = y == Y - 1 ? i - (Y - 1) * Z : i + Z;
     const int ixm = xx == 0 ? i + (X - 1) * YZ : i - YZ;
     const int ixp = xx == X - 1 ? i - (X - 1) * YZ : i + YZ;
-    const double ck = val(i);
-    const double dxp = ck - val(ixp), dyp = ck - val(iyp), dzp = ck - val(izp);
-    const double a2 = ck - 1.0;
+    const T ck = val(i);
+    const T dxp = ck - val(ixp), dyp = ck - val(iyp), dzp = ck - val(izp);
+    const T a2 = ck - (T)1;
     if (MODE & 1) {
-      const double a1 = ck + 1.0;
-      const double be = I.fb * ((a1 * a1) * (a2 * a2));
-      double sq = dxp * dxp;
+      const T a1 = ck + (T)1;
+      const T be = C::fb(I) * ((a1 * a1) * (a2 * a2));
+      T sq = dxp * dxp;
       sq = fma(dyp, dyp, sq);
       sq = fma(dzp, dzp, sq);
-      acc[0] += fma(I.fg, sq, be);
+      acc[0] += (double)fma(C::fg(I), sq, be);
     }
     if (MODE & 2) {
-      const double dxm = ck - val(ixm), dym = ck - val(iym), dzm = ck - val(izm);
-      const double lap = ((dxm + dxp) + (dym + dyp)) + (dzm + dzp);
-      const double bg = I.fbg * (ck * fma(ck, ck, -1.0));
-      const double gv = fma(I.fg2, lap, bg);
+      const T dxm = ck - val(ixm), dym = ck - val(iym), dzm = ck - val(izm);
+      const T lap = ((dxm + dxp) + (dym + dyp)) + (dzm + dzp);
+      const T bg = C::fbg(I) * (ck * fma(ck, ck, (T)-1));
+      const TV gv = (TV)fma(C::fg2(I), lap, bg);
       g[i] = gv;
-      acc[1] += gv * gv;
+      acc[1] += (double)gv * (double)gv;
     }
   }
   block_partials<kKP>(acc, (MODE & 1 ? 1ull : 0ull) | (MODE & 2 ? 2ull : 0ull), B.part);
 }
 
+template <int MODE, typename TV>
+__global__ __launch_bounds__(kBlock) void bpf_kernel(BBufs B) {
+  const int p = find_pos(B.ablk, B.na, blockIdx.x);
+  const BDyn& q = B.dyn[p];
+  const BInst I = B.inst[q.j];
+  const int lb = blockIdx.x - B.ablk[p];
+  if (q.lp) bpf_body<MODE, float, TV>(B, q, I, lb);
+  else bpf_body<MODE, double, TV>(B, q, I, lb);
+}
+
+// x rounded to FP32 (the mixed schedule's first phase keeps FP32 coordinates).
+__global__ __launch_bounds__(kBlock) void bround_kernel(BBufs B) {
+  const int p = find_pos(B.ablk, B.na, blockIdx.x);
+  const BDyn& q = B.dyn[p];
+  const BInst I = B.inst[q.j];
+  const int lb = blockIdx.x - B.ablk[p];
+  double* __restrict__ x = B.X + I.off;
+#pragma unroll
+  for (int r = 0; r < kBR; r++) {
+    const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
+    if (i >= I.n) break;
+    x[i] = (double)(float)x[i];
+  }
+}
+
 // d = cg g + sum_i cD[i] D_{slot i} + cY[i] Y_{slot i} (pairs oldest first, as
 // the single-State direction pass) into D[dslot]; partials d.d, d.g, d.y_i.
 // P is the number of stored pairs, so every load is unconditional.
-template <int P>
+template <int P, typename TV>
 __device__ __forceinline__ void bdir_body(const BBufs& B, const BDyn& q, const BInst& I, int lb) {
   const size_t N = B.N;
-  const double* __restrict__ g = (q.gcur ? B.G1 : B.G0) + I.off;
-  double* __restrict__ out = B.D + (size_t)q.dslot * N + I.off;
-  const double* Dp[P > 0 ? P : 1];
-  const double* Yp[P > 0 ? P : 1];
+  const TV* __restrict__ g = static_cast<const TV*>(q.gcur ? B.G1 : B.G0) + I.off;
+  TV* __restrict__ out = static_cast<TV*>(B.D) + (size_t)q.dslot * N + I.off;
+  const TV* Dp[P > 0 ? P : 1];
+  const TV* Yp[P > 0 ? P : 1];
   double cd[P > 0 ? P : 1], cy[P > 0 ? P : 1];
 #pragma unroll
   for (int k = 0; k < P; k++) {
-    Dp[k] = B.D + (size_t)q.slot[k] * N + I.off;
-    Yp[k] = B.Y + (size_t)q.slot[k] * N + I.off;
+    Dp[k] = static_cast<const TV*>(B.D) + (size_t)q.slot[k] * N + I.off;
+    Yp[k] = static_cast<const TV*>(B.Y) + (size_t)q.slot[k] * N + I.off;
     cd[k] = q.cD[k];
     cy[k] = q.cY[k];
   }
@@ -212,19 +261,21 @@ __device__ __forceinline__ void bdir_body(const BBufs& B, const BDyn& q, const B
   for (int r = 0; r < kBR; r++) {
     const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
     if (i >= I.n) break;
-    const double gi = g[i];
+    const double gi = (double)g[i];
     double dv[P > 0 ? P : 1], yv[P > 0 ? P : 1];
 #pragma unroll
     for (int k = 0; k < P; k++) {
-      dv[k] = Dp[k][i];
-      yv[k] = Yp[k][i];
+      dv[k] = (double)Dp[k][i];
+      yv[k] = (double)Yp[k][i];
     }
     double v = cg * gi;
 #pragma unroll
     for (int k = 0; k < P; k++) v = v + cd[k] * dv[k];
 #pragma unroll
     for (int k = 0; k < P; k++) v = v + cy[k] * yv[k];
-    out[i] = v;
+    const TV vs = (TV)v;  // FP64 per element, stored as TV; the dots use the stored value
+    out[i] = vs;
+    v = (double)vs;
     acc[0] += v * v;
     acc[1] += v * gi;
 #pragma unroll
@@ -234,40 +285,55 @@ __device__ __forceinline__ void bdir_body(const BBufs& B, const BDyn& q, const B
   block_partials<kKD>(acc, on, B.part);
 }
 
+template <typename TV>
 __global__ __launch_bounds__(kBlock) void bdir_kernel(BBufs B) {
   const int p = find_pos(B.ablk, B.na, blockIdx.x);
   const BDyn& q = B.dyn[p];
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
   switch (q.np) {
-    case 0: bdir_body<0>(B, q, I, lb); break;
-    case 1: bdir_body<1>(B, q, I, lb); break;
-    case 2: bdir_body<2>(B, q, I, lb); break;
-    case 3: bdir_body<3>(B, q, I, lb); break;
-    case 4: bdir_body<4>(B, q, I, lb); break;
-    case 5: bdir_body<5>(B, q, I, lb); break;
-    case 6: bdir_body<6>(B, q, I, lb); break;
-    case 7: bdir_body<7>(B, q, I, lb); break;
-    case 8: bdir_body<8>(B, q, I, lb); break;
-    case 9: bdir_body<9>(B, q, I, lb); break;
-    default: bdir_body<10>(B, q, I, lb); break;
+    case 0: bdir_body<0, TV>(B, q, I, lb); break;
+    case 1: bdir_body<1, TV>(B, q, I, lb); break;
+    case 2: bdir_body<2, TV>(B, q, I, lb); break;
+    case 3: bdir_body<3, TV>(B, q, I, lb); break;
+    case 4: bdir_body<4, TV>(B, q, I, lb); break;
+    case 5: bdir_body<5, TV>(B, q, I, lb); break;
+    case 6: bdir_body<6, TV>(B, q, I, lb); break;
+    case 7: bdir_body<7, TV>(B, q, I, lb); break;
+    case 8: bdir_body<8, TV>(B, q, I, lb); break;
+    case 9: bdir_body<9, TV>(B, q, I, lb); break;
+    default: bdir_body<10, TV>(B, q, I, lb); break;
   }
 }
 
-// x += tau * d (product rounded first, as the single-State axpy).
+// x += tau * d (product rounded first, as the single-State axpy), in FP32 for
+// an instance in the first phase of the mixed schedule.
+template <typename TV>
 __global__ __launch_bounds__(kBlock) void baxpy_kernel(BBufs B) {
   const int p = find_pos(B.ablk, B.na, blockIdx.x);
   const BDyn& q = B.dyn[p];
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
   double* __restrict__ x = B.X + I.off;
-  const double* __restrict__ d = B.D + (size_t)q.dslot * B.N + I.off;
-  const double tau = q.tau;
+  const TV* __restrict__ d = static_cast<const TV*>(B.D) + (size_t)q.dslot * B.N + I.off;
+  if (q.lp) {
+    const float tau = (float)q.tau;
 #pragma unroll
-  for (int r = 0; r < kBR; r++) {
-    const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
-    if (i >= I.n) break;
-    x[i] = x[i] + tau * d[i];
+    for (int r = 0; r < kBR; r++) {
+      const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
+      if (i >= I.n) break;
+      const float st = tau * (float)d[i];
+      x[i] = (double)((float)x[i] + st);
+    }
+  } else {
+    const double tau = q.tau;
+#pragma unroll
+    for (int r = 0; r < kBR; r++) {
+      const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
+      if (i >= I.n) break;
+      const double st = tau * (double)d[i];
+      x[i] = x[i] + st;
+    }
   }
 }
 
@@ -275,20 +341,20 @@ __global__ __launch_bounds__(kBlock) void baxpy_kernel(BBufs B) {
 // being the P stored pairs then the new direction and the Y list the P pairs:
 //   [0,kBS) gn.D_i ; [kBS,2kBS) gn.Y_i ; 2kBS gn.gn ; 2kBS+1 gn.y ;
 //   [2kBS+2,3kBS+2) y.D_i ; [3kBS+2,4kBS+2) y.Y_i ; 4kBS+2 y.y
-template <int P>
+template <int P, typename TV>
 __device__ __forceinline__ void bgram_body(const BBufs& B, const BDyn& q, const BInst& I, int lb) {
   const size_t N = B.N;
-  const double* __restrict__ gn = (q.gout ? B.G1 : B.G0) + I.off;
-  const double* __restrict__ go = (q.gcur ? B.G1 : B.G0) + I.off;
-  double* __restrict__ yo = B.Y + (size_t)q.dslot * N + I.off;
-  const double* Dp[P + 1];
-  const double* Yp[P > 0 ? P : 1];
+  const TV* __restrict__ gn = static_cast<const TV*>(q.gout ? B.G1 : B.G0) + I.off;
+  const TV* __restrict__ go = static_cast<const TV*>(q.gcur ? B.G1 : B.G0) + I.off;
+  TV* __restrict__ yo = static_cast<TV*>(B.Y) + (size_t)q.dslot * N + I.off;
+  const TV* Dp[P + 1];
+  const TV* Yp[P > 0 ? P : 1];
 #pragma unroll
   for (int k = 0; k < P; k++) {
-    Dp[k] = B.D + (size_t)q.slot[k] * N + I.off;
-    Yp[k] = B.Y + (size_t)q.slot[k] * N + I.off;
+    Dp[k] = static_cast<const TV*>(B.D) + (size_t)q.slot[k] * N + I.off;
+    Yp[k] = static_cast<const TV*>(B.Y) + (size_t)q.slot[k] * N + I.off;
   }
-  Dp[P] = B.D + (size_t)q.dslot * N + I.off;
+  Dp[P] = static_cast<const TV*>(B.D) + (size_t)q.dslot * N + I.off;
   double acc[kKG];
 #pragma unroll
   for (int k = 0; k < kKG; k++) acc[k] = 0;
@@ -296,15 +362,16 @@ __device__ __forceinline__ void bgram_body(const BBufs& B, const BDyn& q, const 
   for (int r = 0; r < kBR; r++) {
     const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
     if (i >= I.n) break;
-    const double a = gn[i];
-    const double b = go[i];
+    const double a = (double)gn[i];
+    const double b = (double)go[i];
     double dv[P + 1], wv[P > 0 ? P : 1];
 #pragma unroll
-    for (int k = 0; k <= P; k++) dv[k] = Dp[k][i];
+    for (int k = 0; k <= P; k++) dv[k] = (double)Dp[k][i];
 #pragma unroll
-    for (int k = 0; k < P; k++) wv[k] = Yp[k][i];
-    const double yv = a - b;
-    yo[i] = yv;
+    for (int k = 0; k < P; k++) wv[k] = (double)Yp[k][i];
+    const TV ys = (TV)(a - b);  // stored as TV; the dots use the stored value
+    yo[i] = ys;
+    const double yv = (double)ys;
 #pragma unroll
     for (int k = 0; k <= P; k++) {
       acc[k] += a * dv[k];
@@ -325,24 +392,55 @@ __device__ __forceinline__ void bgram_body(const BBufs& B, const BDyn& q, const 
   block_partials<kKG>(acc, on, B.part);
 }
 
+template <typename TV>
 __global__ __launch_bounds__(kBlock) void bgram_kernel(BBufs B) {
   const int p = find_pos(B.ablk, B.na, blockIdx.x);
   const BDyn& q = B.dyn[p];
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
   switch (q.np) {
-    case 0: bgram_body<0>(B, q, I, lb); break;
-    case 1: bgram_body<1>(B, q, I, lb); break;
-    case 2: bgram_body<2>(B, q, I, lb); break;
-    case 3: bgram_body<3>(B, q, I, lb); break;
-    case 4: bgram_body<4>(B, q, I, lb); break;
-    case 5: bgram_body<5>(B, q, I, lb); break;
-    case 6: bgram_body<6>(B, q, I, lb); break;
-    case 7: bgram_body<7>(B, q, I, lb); break;
-    case 8: bgram_body<8>(B, q, I, lb); break;
-    case 9: bgram_body<9>(B, q, I, lb); break;
-    default: bgram_body<10>(B, q, I, lb); break;
+    case 0: bgram_body<0, TV>(B, q, I, lb); break;
+    case 1: bgram_body<1, TV>(B, q, I, lb); break;
+    case 2: bgram_body<2, TV>(B, q, I, lb); break;
+    case 3: bgram_body<3, TV>(B, q, I, lb); break;
+    case 4: bgram_body<4, TV>(B, q, I, lb); break;
+    case 5: bgram_body<5, TV>(B, q, I, lb); break;
+    case 6: bgram_body<6, TV>(B, q, I, lb); break;
+    case 7: bgram_body<7, TV>(B, q, I, lb); break;
+    case 8: bgram_body<8, TV>(B, q, I, lb); break;
+    case 9: bgram_body<9, TV>(B, q, I, lb); break;
+    default: bgram_body<10, TV>(B, q, I, lb); break;
   }
+}
+
+// Dots of a recomputed g = G[gcur] with every stored pair (after the switch
+// to FP64 coordinates): [0,kBS) g.D_i ; [kBS,2kBS) g.Y_i.
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void bref_kernel(BBufs B) {
+  const int p = find_pos(B.ablk, B.na, blockIdx.x);
+  const BDyn& q = B.dyn[p];
+  const BInst I = B.inst[q.j];
+  const int lb = blockIdx.x - B.ablk[p];
+  const size_t N = B.N;
+  const TV* __restrict__ g = static_cast<const TV*>(q.gcur ? B.G1 : B.G0) + I.off;
+  const TV* __restrict__ D = static_cast<const TV*>(B.D) + I.off;
+  const TV* __restrict__ Y = static_cast<const TV*>(B.Y) + I.off;
+  double acc[2 * kBS];
+#pragma unroll
+  for (int k = 0; k < 2 * kBS; k++) acc[k] = 0;
+  for (int r = 0; r < kBR; r++) {
+    const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
+    if (i >= I.n) break;
+    const double a = (double)g[i];
+#pragma unroll
+    for (int k = 0; k < kBS; k++)
+      if (k < q.np) {
+        acc[k] += a * (double)D[(size_t)q.slot[k] * N + i];
+        acc[kBS + k] += a * (double)Y[(size_t)q.slot[k] * N + i];
+      }
+  }
+  const unsigned long long pm = (1ull << q.np) - 1;
+  block_partials<2 * kBS>(acc, pm | (pm << kBS), B.part);
 }
 
 // Host side ------------------------------------------------------------------
@@ -372,7 +470,22 @@ struct Inst {
   double tau0 = 1, tau = 1, t = 0, gs = 0;
   int trial = 0;
   bool searching = false;
+  // mixed schedule
+  bool lp = false;     // FP32 coordinates and arithmetic
+  bool fresh = false;  // g to be recomputed from FP64 coordinates, history kept
+  bool lsFailed = false;
+  int stall = 0;
 };
+
+using KFn = void (*)(BBufs);
+struct Kernels {
+  KFn pf_eg, pf_trial, dir, axpy, gram, ref;
+};
+template <typename TV>
+Kernels kernels_for() {
+  return Kernels{bpf_kernel<3, TV>, bpf_kernel<5, TV>, bdir_kernel<TV>,
+                 baxpy_kernel<TV>,  bgram_kernel<TV>,  bref_kernel<TV>};
+}
 
 // One launch group: the instances taking part, their per-phase parameters and
 // block prefix sums, staged in pinned memory and copied to the device.
@@ -392,8 +505,14 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
   std::vector<MinimiseStats> out(ni);
   if (ni == 0) return out;
   if (_m > kBS - 1) throw std::invalid_argument("Lbfgs::minimiseBatch: m must be at most 10.");
-  if (_precMode == "fp32" || _precMode == "mixed")
-    throw std::invalid_argument("Lbfgs::minimiseBatch runs in FP64 (precision \"state\" or \"fp64\").");
+  if (_precMode == "fp32")
+    throw std::invalid_argument("Lbfgs::minimiseBatch runs in FP64 or the mixed schedule, not in FP32 throughout.");
+  // mixed: FP32 coordinates and arithmetic until sqrt(g.g/ndof) < switchRms (or
+  // the line search stalls twice), then FP64 coordinates; the vectors stay in
+  // FP32 throughout, as the single-State schedule does for a PhaseField
+  const bool mixed = _precMode == "mixed";
+  const size_t vb = mixed ? 4 : 8;
+  const Kernels K = mixed ? kernels_for<float>() : kernels_for<double>();
 
   std::vector<Inst> in(ni);
   size_t N = 0;
@@ -418,7 +537,9 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
       throw std::invalid_argument("Lbfgs::minimiseBatch: unsupported field size.");
     Inst& I = in[j];
     I.s = s;
-    I.bi = BInst{N, (int)n, c[0], c[1], c[2], fb, fg, 4.0 * fb, 2.0 * fg};
+    I.bi = BInst{N, (int)n, c[0], c[1], c[2], fb, fg, 4.0 * fb, 2.0 * fg,
+                 (float)fb, (float)fg, (float)(4.0 * fb), (float)(2.0 * fg)};
+    I.lp = mixed;
     I.nblk = (int)((n + kBE - 1) / kBE);
     I.conv = s->convergence;
     I.used.assign(_m + 1, 0);
@@ -438,7 +559,7 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
   auto since = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
 
   const int S = _m + 1;
-  DeviceBuffer X(N * 8), G0(N * 8), G1(N * 8), Dm(N * 8 * S), Ym(N * 8 * S);
+  DeviceBuffer X(N * 8), G0(N * vb), G1(N * vb), Dm(N * vb * S), Ym(N * vb * S);
   // per-phase staging, one copy per phase: block prefix sums, then the BDyn of
   // each instance in the launch
   const size_t ablkBytes = (sizeof(int) * (ni + 1) + 15) / 16 * 16;
@@ -466,10 +587,10 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
 
   BBufs B{};
   B.X = static_cast<double*>(X.get());
-  B.G0 = static_cast<double*>(G0.get());
-  B.G1 = static_cast<double*>(G1.get());
-  B.D = static_cast<double*>(Dm.get());
-  B.Y = static_cast<double*>(Ym.get());
+  B.G0 = G0.get();
+  B.G1 = G1.get();
+  B.D = Dm.get();
+  B.Y = Ym.get();
   B.N = N;
   B.inst = static_cast<const BInst*>(instD.get());
   B.ablk = static_cast<const int*>(stageD.get());
@@ -502,8 +623,13 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
     q.dslot = I.dslot;
     q.gcur = I.gcur;
     q.gout = 1 - I.gcur;
+    q.lp = I.lp ? 1 : 0;
     q.tau = 0;
     return q;
+  };
+  auto launch = [&](KFn f, const Launch& L) {
+    f<<<L.nb, kBlock, 0, st>>>(B);
+    MINIM_HIP_CHECK(hipGetLastError());
   };
   const bool backtracking = linesearch == "backtracking";
   const bool spec = backtracking && _maxStep == 0;  // first trial at x + d, launched with the direction
@@ -519,8 +645,11 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
       L.pos.push_back(j);
     }
     stage(L);
-    bpf_kernel<3><<<L.nb, kBlock, 0, st>>>(B);
-    MINIM_HIP_CHECK(hipGetLastError());
+    if (mixed) {  // the State's coordinates rounded to FP32 (State::setPrecision)
+      bround_kernel<<<L.nb, kBlock, 0, st>>>(B);
+      MINIM_HIP_CHECK(hipGetLastError());
+    }
+    launch(K.pf_eg, L);
     finalize(L, kKP, 0);
     fetch((size_t)L.na * kKP);
     for (int p = 0; p < L.na; p++) {
@@ -538,6 +667,41 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
     for (int j = 0; j < ni; j++)
       if (!in[j].done) L.pos.push_back(j);
     if (L.pos.empty()) break;
+
+    // ---- instances that have just left FP32 coordinates: gradient and energy
+    // at the same point in FP64, and its dots with the stored pairs
+    {
+      Launch F;
+      for (int j : L.pos)
+        if (in[j].fresh) {
+          BDyn q = dyn_base(in[j]);
+          q.gout = in[j].gcur;  // overwrite g
+          q.np = (int)in[j].pairs.size();
+          for (int i = 0; i < q.np; i++) q.slot[i] = in[j].pairs[i].slot;
+          dynh[F.pos.size()] = q;
+          F.pos.push_back(j);
+        }
+      if (!F.pos.empty()) {
+        stage(F);
+        launch(K.pf_eg, F);
+        finalize(F, kKP, 0);
+        launch(K.ref, F);
+        finalize(F, 2 * kBS, (size_t)F.na * kKP);
+        fetch((size_t)F.na * (kKP + 2 * kBS));
+        for (int p = 0; p < F.na; p++) {
+          Inst& I = in[F.pos[p]];
+          I.e = resh[(size_t)p * kKP];
+          I.gg = resh[(size_t)p * kKP + 1];
+          const double* r = resh + (size_t)F.na * kKP + (size_t)p * 2 * kBS;
+          for (int i = 0; i < (int)I.pairs.size(); i++) {
+            I.dg[I.pairs[i].slot] = r[i];
+            I.yg[I.pairs[i].slot] = r[kBS + i];
+          }
+          I.fresh = false;
+          I.st.nGradient++;
+        }
+      }
+    }
 
     // ---- direction coefficients (two-loop recursion on the Gram data,
     // Lbfgs.cpp:84-127) and the speculative first trial at x + d
@@ -589,13 +753,11 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
       dynh[p] = q;
     }
     stage(L);
-    bdir_kernel<<<L.nb, kBlock, 0, st>>>(B);
-    MINIM_HIP_CHECK(hipGetLastError());
+    launch(K.dir, L);
     finalize(L, kKD, 0);
     const size_t trialOff = (size_t)L.na * kKD;
     if (spec) {
-      bpf_kernel<5><<<L.nb, kBlock, 0, st>>>(B);
-      MINIM_HIP_CHECK(hipGetLastError());
+      launch(K.pf_trial, L);
       finalize(L, kKP, trialOff);
     }
     fetch(trialOff + (spec ? (size_t)L.na * kKP : 0));
@@ -621,6 +783,7 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
       I.tau = tau0;
       I.trial = 0;
       I.searching = false;
+      I.lsFailed = false;
       if (!backtracking) continue;
       I.t = -0.5 * gs;  // linesearch.cpp:11-13
       if (spec) {
@@ -647,8 +810,7 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
         dynh[p] = q;
       }
       stage(T);
-      bpf_kernel<5><<<T.nb, kBlock, 0, st>>>(B);
-      MINIM_HIP_CHECK(hipGetLastError());
+      launch(K.pf_trial, T);
       finalize(T, kKP, 0);
       fetch((size_t)T.na * kKP);
       std::vector<int> next;
@@ -664,6 +826,7 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
         I.t *= 0.5;
         if (++I.trial >= 10) {
           I.searching = false;  // x + tau0 * 2^-10, not evaluated by the reference
+          I.lsFailed = true;
           continue;
         }
         next.push_back(T.pos[p]);
@@ -681,14 +844,11 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
       dynh[p] = q;
     }
     stage(L);
-    baxpy_kernel<<<L.nb, kBlock, 0, st>>>(B);
-    MINIM_HIP_CHECK(hipGetLastError());
-    bpf_kernel<3><<<L.nb, kBlock, 0, st>>>(B);
-    MINIM_HIP_CHECK(hipGetLastError());
+    launch(K.axpy, L);
+    launch(K.pf_eg, L);
     finalize(L, kKP, 0);
     const size_t gramOff = (size_t)L.na * kKP;
-    bgram_kernel<<<L.nb, kBlock, 0, st>>>(B);
-    MINIM_HIP_CHECK(hipGetLastError());
+    launch(K.gram, L);
     finalize(L, kKG, gramOff);
     fetch(gramOff + (size_t)L.na * kKG);
 
@@ -734,10 +894,21 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
       I.gg = gng;
       I.gcur = 1 - I.gcur;
       I.iters++;
-      I.st.iterFP64++;
-      // Minimiser::minimise (Minimiser.cpp:27-38) with Lbfgs::checkConvergence (Lbfgs.cpp:130-133)
       const double rms = std::sqrt(I.gg / I.bi.n);
-      if (rms < I.conv) {
+      if (I.lp) {
+        // leave FP32 coordinates once they have done their job (as Lbfgs::iteration)
+        const double sw = _switchRms > 0 ? _switchRms : 1e3 * I.conv;
+        I.stall = I.lsFailed ? I.stall + 1 : 0;
+        I.st.iterFP32++;
+        if (rms < sw || I.stall >= 2) {
+          I.lp = false;
+          I.fresh = true;  // no convergence test until g is recomputed
+        }
+      } else {
+        I.st.iterFP64++;
+      }
+      // Minimiser::minimise (Minimiser.cpp:27-38) with Lbfgs::checkConvergence (Lbfgs.cpp:130-133)
+      if (!I.fresh && rms < I.conv) {
         I.st.converged = true;
         I.done = true;
       } else if (I.iters > maxIter) {

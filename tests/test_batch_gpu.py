@@ -61,6 +61,32 @@ def test_batch_vs_oracle(gpu):
         assert abs(r["iterations"] - so["iterations"]) <= max(10, 0.3 * so["iterations"]), (r, so)
 
 
+def test_batch_mixed_vs_oracle(gpu):
+    """The mixed schedule (FP32 coordinates and arithmetic, then FP64
+    coordinates with FP32 vectors) per instance: converged energies within
+    1e-8 of the oracle's FP64 L-BFGS, and the switch taken as the single-State
+    schedule takes it."""
+    m = gpu
+    cs = cases(12)
+    states = [make_state(m, c) for c in cs]
+    res = m.Lbfgs().setM(5).setMaxIter(20000).setPrecision("mixed").minimiseBatch(states)
+    for c, s, r in zip(cs, states, res):
+        o = OraclePF(c["grid"], 1, 1.0, [1.0], [c["w"]])
+        xo, so = oracle_lbfgs(o, c["x0"], m=5, max_iter=20000)
+        e0 = o.energy(c["x0"])
+        assert r["converged"], (c["grid"], r)
+        x = s.coords()
+        assert abs(r["energy"] - o.energy(x)) <= 1e-12 * max(abs(r["energy"]), abs(e0)), c["grid"]
+        assert energy_ok(r["energy"], so["energy"], e0), (c["grid"], r["energy"], so["energy"])
+        g = o.gradient(x)
+        assert np.sqrt(np.mean(g * g)) < 1.0001 * o.convergence
+        assert r["iter_fp32"] > 0 and r["iter_fp32"] + r["iter_fp64"] == r["iterations"] + 1
+        s1 = make_state(m, c)
+        lb = m.Lbfgs().setM(5).setMaxIter(20000).setPrecision("mixed")
+        lb.minimise(s1)
+        assert abs(r["iter_fp32"] - lb.stats["iter_fp32"]) <= max(5, 0.3 * lb.stats["iter_fp32"]), (r, lb.stats)
+
+
 def test_batch_first_iteration_matches_single(gpu):
     """maxIter 0: one iteration, the first step of Lbfgs.cpp:91-96 with the line
     search.  The gradient of the batch kernel uses the single-State kernel's
@@ -130,8 +156,8 @@ def test_batch_rejects_unsupported_fields(gpu):
                    np.full(192, 1 / 3))
     with pytest.raises(ValueError, match="one-fluid PhaseField"):
         m.Lbfgs().minimiseBatch([ok, tern])
-    with pytest.raises(ValueError, match="FP64"):
-        m.Lbfgs().setPrecision("mixed").minimiseBatch([ok])
+    with pytest.raises(ValueError, match="FP32 throughout"):
+        m.Lbfgs().setPrecision("fp32").minimiseBatch([ok])
     with pytest.raises(ValueError, match="at most 10"):
         m.Lbfgs().setM(12).minimiseBatch([ok])
     with pytest.raises(ValueError, match="twice"):
