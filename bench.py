@@ -426,7 +426,51 @@ def bah_leg(m, nside=1024, max_iter=2000, ref_side=48):
     return out
 
 
-def batch_leg(m, n_inst=512, workers=8, seed=4040, cpu_budget=10.0):
+def per_state_leg(m, n_inst, workers, sides, inputs, make_pf):
+    """The batch workload with one State per minimise call, spread over
+    `workers` host threads, each with its own HIP stream, State objects and
+    (reused) minimiser buffers, largest instances first."""
+    import concurrent.futures as cf
+    import threading
+
+    import numpy as np
+    import torch
+
+    order = sorted(range(n_inst), key=lambda j: -sides[j])
+    t = time.perf_counter()
+    states = {j: m.State(make_pf(j), inputs(j), "fp64") for j in order}
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t
+    tls = threading.local()
+
+    def run(j):
+        if not hasattr(tls, "stream"):
+            torch.cuda.set_device(0)
+            tls.stream = torch.cuda.Stream()
+            tls.lb = m.Lbfgs().setM(5).setMaxIter(20000)
+        s = states[j]
+        s.setStream(tls.stream.cuda_stream)
+        t0 = time.perf_counter()
+        tls.lb.minimise(s, download=False)
+        st = tls.lb.stats
+        return dict(seconds=time.perf_counter() - t0, iterations=st["iterations"], converged=st["converged"],
+                    evals=st["n_energy"] + st["n_gradient"])
+
+    # largest first: each worker allocates its minimiser buffers once, on its first instance
+    with cf.ThreadPoolExecutor(workers) as ex:
+        t = time.perf_counter()
+        res = list(ex.map(run, order))
+        wall = time.perf_counter() - t
+    secs = np.array([r["seconds"] for r in res])
+    evals = sum(r["evals"] for r in res)
+    del states
+    return dict(workers=workers, setup_s=setup_s, wall_s=wall, evals=evals, evals_per_s=evals / wall,
+                instances_per_s=n_inst / wall, converged=int(sum(r["converged"] for r in res)),
+                p50_s=float(np.percentile(secs, 50)), p99_s=float(np.percentile(secs, 99)),
+                iterations_total=int(sum(r["iterations"] for r in res)))
+
+
+def batch_leg(m, n_inst=512, workers=8, seed=4040, cpu_budget=10.0, per_state=True):
     """Many independent fields: n_inst 2-D binary PhaseField instances with side
     s_j = round(exp(U[ln 32, ln 512])), phi ~ U[-0.1, 0.1] and interface width
     w_j ~ U[1, 3] (all seeded), each minimised by L-BFGS m=5 to the reference
@@ -436,9 +480,8 @@ def batch_leg(m, n_inst=512, workers=8, seed=4040, cpu_budget=10.0):
     converged), and one State per call spread over `workers` host threads, each
     with its own HIP stream (ordered largest first).  Reports the aggregate
     evaluation rate and the p50/p99 per-instance time-to-converge, and the
-    reference library on a bounded sample of the same instances."""
-    import concurrent.futures as cf
-    import threading
+    reference library on a bounded sample of the same instances.  per_state
+    False leaves out the one-State-per-call comparison."""
 
     import numpy as np
     import torch
@@ -477,43 +520,12 @@ def batch_leg(m, n_inst=512, workers=8, seed=4040, cpu_budget=10.0):
                              iterations_fp32=int(sum(r["iter_fp32"] for r in bres)))
         del bstates
 
-    # ---- one State per call, spread over host threads
-    order = sorted(range(n_inst), key=lambda j: -sides[j])
-    t = time.perf_counter()
-    states = {j: m.State(make_pf(j), inputs(j), "fp64") for j in order}
-    torch.cuda.synchronize()
-    setup_s = time.perf_counter() - t
-    tls = threading.local()
-
-    def run(j):
-        if not hasattr(tls, "stream"):
-            torch.cuda.set_device(0)
-            tls.stream = torch.cuda.Stream()
-            tls.lb = m.Lbfgs().setM(5).setMaxIter(20000)
-        s = states[j]
-        s.setStream(tls.stream.cuda_stream)
-        t0 = time.perf_counter()
-        tls.lb.minimise(s, download=False)
-        st = tls.lb.stats
-        return dict(seconds=time.perf_counter() - t0, iterations=st["iterations"], converged=st["converged"],
-                    evals=st["n_energy"] + st["n_gradient"])
-
-    # largest first: each worker allocates its minimiser buffers once, on its first instance
-    with cf.ThreadPoolExecutor(workers) as ex:
-        t = time.perf_counter()
-        res = list(ex.map(run, order))
-        wall = time.perf_counter() - t
-    secs = np.array([r["seconds"] for r in res])
-    evals = sum(r["evals"] for r in res)
     out = dict(workload="%d 2-D binary PhaseField instances, sides %d..%d, %d sites in total; L-BFGS m=5 each"
                         % (n_inst, sides.min(), sides.max(), int((sides ** 2).sum())),
-               batched=batched,
-               per_state_threads=dict(workers=workers, setup_s=setup_s, wall_s=wall, evals=evals,
-                                      evals_per_s=evals / wall, instances_per_s=n_inst / wall,
-                                      converged=int(sum(r["converged"] for r in res)),
-                                      p50_s=float(np.percentile(secs, 50)), p99_s=float(np.percentile(secs, 99)),
-                                      iterations_total=int(sum(r["iterations"] for r in res))))
-    del states
+               site_bytes_per_iteration="FP64: 8 B x (4m + 14) vector streams per site; mixed: FP32 vectors",
+               batched=batched)
+    if per_state:
+        out["per_state_threads"] = per_state_leg(m, n_inst, workers, sides, inputs, make_pf)
     from oracle_bind import RefState, ref_available
 
     if ref_available() and cpu_budget > 0:
@@ -594,6 +606,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="batch mode: the batched minimisations only, without the one-State-per-call comparison")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--fire-iters", type=int, default=20000, help="particles mode: FIRE iteration cap")
     ap.add_argument("--lbfgs-iters", type=int, default=2000, help="particles mode: sheet L-BFGS iteration cap")
@@ -629,7 +643,7 @@ def main():
     if args.mode == "batch":
         del x64, x32, g64, g32
         torch.cuda.empty_cache()
-        res = batch_leg(m, cpu_budget=0 if args.no_cpu else 10.0)
+        res = batch_leg(m, cpu_budget=0 if args.no_cpu else 10.0, per_state=not args.batch_only)
         log(json.dumps(res, indent=1))
         print(json.dumps(res), flush=True)
         return
