@@ -253,6 +253,36 @@ class PhaseField(Potential):
 
     COMPONENTS = ("fluid", "surface", "pressure", "density constraint", "force", "confinement", "volume constraint")
 
+    def autotune(self, coords, precision="fp32", blocks=(256, 512, 1024, 2048), marches=(0, 8, 16, 32), reps=10):
+        """Launch-shape autotuner of the stencil kernel for this field: times one
+        energy+gradient launch (HIP events, the kernel alone) for every pair of
+        target block count and x-march length (0 = automatic), keeps the fastest
+        as stencilTargetBlocks / stencilXChunk, and returns {(blocks, march):
+        ms}.  The launch shape changes the tiling only: gradients are the same
+        bit for bit, energies differ by the order of the block sums."""
+        import torch
+
+        x = _f64(coords)
+        dt = torch.float32 if precision == "fp32" else torch.float64
+        xd = torch.from_numpy(x).to("cuda", dt)
+        g = torch.empty_like(xd)
+        keep = (self.stencilTargetBlocks, self.stencilXChunk)
+        times = {}
+        try:
+            for b in blocks:
+                for mch in marches:
+                    self.stencilTargetBlocks, self.stencilXChunk = int(b), int(mch)
+                    h = self.handle(x)
+                    potential_time_kernel(h, precision, xd.data_ptr(), x.size, g.data_ptr(), True, 2)  # warm
+                    times[(int(b), int(mch))] = potential_time_kernel(h, precision, xd.data_ptr(), x.size,
+                                                                      g.data_ptr(), True, reps)
+        except Exception:
+            self.stencilTargetBlocks, self.stencilXChunk = keep
+            raise
+        best = min(times, key=times.get)
+        self.stencilTargetBlocks, self.stencilXChunk = best
+        return times
+
     def energyComponents(self, coords, init_coords=None):
         """PhaseField::energyComponents (PhaseField.cpp:706-744): {name: [energy,
         |gradient|]} at coords, for the potential initialised at init_coords

@@ -319,3 +319,27 @@ def test_full_size_256cubed_ternary(gpu):
     torch.cuda.synchronize()
     assert abs(e32 - eo) <= 1e-5 * abs(eo)
     assert rel_err(g32.double().cpu().numpy(), go) <= 1e-5
+
+
+def test_autotune_keeps_the_fastest_shape_and_the_same_gradient(gpu):
+    """PhaseField.autotune times each launch shape of the stencil and keeps the
+    fastest; every shape gives the same gradient bit for bit and the same
+    energy to the order of the block sums."""
+    m = gpu
+    rng = np.random.default_rng(21)
+    grid = (40, 36, 44)
+    u = rng.uniform(0, 1, (int(np.prod(grid)), 3))
+    x = (u / u.sum(axis=1, keepdims=True)).reshape(-1)
+    pf = m.PhaseField().setNFluid(3).setGridSize(list(grid)).setSurfaceTension([1.0, 0.8, 1.2]).setInterfaceSize(2.0)
+    times = pf.autotune(x, "fp64", blocks=(64, 256, 1024), marches=(0, 4, 8))
+    assert len(times) == 9 and all(t > 0 for t in times.values())
+    assert times[(pf.stencilTargetBlocks, pf.stencilXChunk)] == min(times.values())
+    ref = None
+    for (b, mch) in [(64, 4), (1024, 0), (pf.stencilTargetBlocks, pf.stencilXChunk)]:
+        pf.stencilTargetBlocks, pf.stencilXChunk = b, mch
+        e, g = m.State(pf, x).energyGradient()
+        if ref is None:
+            ref = (e, g)
+        else:
+            assert np.array_equal(g, ref[1]), (b, mch)
+            assert abs(e - ref[0]) <= 1e-12 * abs(ref[0])
