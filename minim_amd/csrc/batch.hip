@@ -9,7 +9,8 @@
 // blocks of the instances that take part in it:
 //
 //   direction   d = c_g g + sum_s c_D[s] D_s + c_Y[s] Y_s, with d.d, d.g, d.Y_s
-//   trial       energy at x + tau d (per-instance tau)
+//   trials      energies at x + tau 2^-k d, k < 4 (per-instance tau): the next
+//               four points of the backtracking sequence in one pass
 //   step        x += tau d, then energy and gradient at x (+ g.g)
 //   Gram        y = g_new - g, with every dot the next recursion needs
 //
@@ -44,6 +45,7 @@ constexpr int kBS = 11;            // history slots (m + 1 with m <= 10)
 constexpr int kBR = 8;             // elements per thread
 constexpr int kBE = kBlock * kBR;  // elements per block
 constexpr int kKP = 2;             // stencil partials: energy, g.g
+constexpr int kNT = 4;             // line-search trials evaluated per pass: tau, tau/2, tau/4, tau/8
 constexpr int kKD = 2 + kBS;       // direction: d.d, d.g, d.Y_s
 constexpr int kKG = 4 * kBS + 3;   // Gram (layout in bgram_kernel)
 
@@ -218,6 +220,59 @@ __global__ __launch_bounds__(kBlock) void bpf_kernel(BBufs B) {
   const int lb = blockIdx.x - B.ablk[p];
   if (q.lp) bpf_body<MODE, float, TV>(B, q, I, lb);
   else bpf_body<MODE, double, TV>(B, q, I, lb);
+}
+
+// Energies at kNT points x + tau_k d along the line-search sequence, tau_k =
+// tau 2^-k (the halvings of linesearch.cpp:24-25 are exact): x and d are read
+// once for all of them, and the host takes the first point that passes the
+// Armijo test, exactly as trying them one after another would.
+template <typename T, typename TV>
+__device__ __forceinline__ void btrial_body(const BBufs& B, const BDyn& q, const BInst& I, int lb) {
+  using C = BC<T>;
+  const double* __restrict__ x = B.X + I.off;
+  const TV* __restrict__ d = static_cast<const TV*>(B.D) + (size_t)q.dslot * B.N + I.off;
+  T tk[kNT];
+  tk[0] = (T)q.tau;
+#pragma unroll
+  for (int k = 1; k < kNT; k++) tk[k] = tk[k - 1] * (T)0.5;
+  const int Z = I.Z, Y = I.Y, X = I.X, YZ = Y * Z;
+  double acc[kNT];
+#pragma unroll
+  for (int k = 0; k < kNT; k++) acc[k] = 0;
+  for (int r = 0; r < kBR; r++) {
+    const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
+    if (i >= I.n) break;
+    const int z = i % Z, t2 = i / Z;
+    const int y = t2 % Y, xx = t2 / Y;
+    const int izp = z == Z - 1 ? i - (Z - 1) : i + 1;
+    const int iyp = y == Y - 1 ? i - (Y - 1) * Z : i + Z;
+    const int ixp = xx == X - 1 ? i - (X - 1) * YZ : i + YZ;
+    const T x0 = (T)x[i], x1 = (T)x[ixp], x2 = (T)x[iyp], x3 = (T)x[izp];
+    const T d0 = (T)d[i], d1 = (T)d[ixp], d2 = (T)d[iyp], d3 = (T)d[izp];
+#pragma unroll
+    for (int k = 0; k < kNT; k++) {
+      const T s0 = tk[k] * d0, s1 = tk[k] * d1, s2 = tk[k] * d2, s3 = tk[k] * d3;
+      const T ck = x0 + s0;
+      const T dxp = ck - (x1 + s1), dyp = ck - (x2 + s2), dzp = ck - (x3 + s3);
+      const T a1 = ck + (T)1, a2 = ck - (T)1;
+      const T be = C::fb(I) * ((a1 * a1) * (a2 * a2));
+      T sq = dxp * dxp;
+      sq = fma(dyp, dyp, sq);
+      sq = fma(dzp, dzp, sq);
+      acc[k] += (double)fma(C::fg(I), sq, be);
+    }
+  }
+  block_partials<kNT>(acc, (1ull << kNT) - 1, B.part);
+}
+
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void btrial_kernel(BBufs B) {
+  const int p = find_pos(B.ablk, B.na, blockIdx.x);
+  const BDyn& q = B.dyn[p];
+  const BInst I = B.inst[q.j];
+  const int lb = blockIdx.x - B.ablk[p];
+  if (q.lp) btrial_body<float, TV>(B, q, I, lb);
+  else btrial_body<double, TV>(B, q, I, lb);
 }
 
 // x rounded to FP32 (the mixed schedule's first phase keeps FP32 coordinates).
@@ -483,7 +538,7 @@ struct Kernels {
 };
 template <typename TV>
 Kernels kernels_for() {
-  return Kernels{bpf_kernel<3, TV>, bpf_kernel<5, TV>, bdir_kernel<TV>,
+  return Kernels{bpf_kernel<3, TV>, btrial_kernel<TV>, bdir_kernel<TV>,
                  baxpy_kernel<TV>,  bgram_kernel<TV>,  bref_kernel<TV>};
 }
 
@@ -758,9 +813,25 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
     const size_t trialOff = (size_t)L.na * kKD;
     if (spec) {
       launch(K.pf_trial, L);
-      finalize(L, kKP, trialOff);
+      finalize(L, kNT, trialOff);
     }
-    fetch(trialOff + (spec ? (size_t)L.na * kKP : 0));
+    fetch(trialOff + (spec ? (size_t)L.na * kNT : 0));
+
+    // Armijo tests on the energies of up to kNT consecutive trials of the
+    // sequence (linesearch.cpp:15-27); true when the search has ended
+    auto consume = [&](Inst& I, const double* e) {
+      for (int k = 0; k < kNT; k++) {
+        I.st.nEnergy++;
+        if (I.e - e[k] >= I.t) return true;  // accepted at x + tau d
+        I.tau *= 0.5;
+        I.t *= 0.5;
+        if (++I.trial >= 10) {
+          I.lsFailed = true;  // x + tau0 * 2^-10, not evaluated by the reference
+          return true;
+        }
+      }
+      return false;
+    };
 
     std::vector<int> search;
     for (int p = 0; p < L.na; p++) {
@@ -787,13 +858,10 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
       if (!backtracking) continue;
       I.t = -0.5 * gs;  // linesearch.cpp:11-13
       if (spec) {
-        I.st.nEnergy++;
         if (tau0 == 1.0) {
-          const double e = resh[trialOff + (size_t)p * kKP];
-          if (I.e - e >= I.t) continue;  // accepted at x + d
-          I.tau *= 0.5;
-          I.t *= 0.5;
-          I.trial = 1;
+          if (consume(I, resh + trialOff + (size_t)p * kNT)) continue;
+        } else {
+          I.st.nEnergy++;  // the speculative trials at x + d were not on this search's sequence
         }
       }
       I.searching = true;
@@ -811,22 +879,13 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
       }
       stage(T);
       launch(K.pf_trial, T);
-      finalize(T, kKP, 0);
-      fetch((size_t)T.na * kKP);
+      finalize(T, kNT, 0);
+      fetch((size_t)T.na * kNT);
       std::vector<int> next;
       for (int p = 0; p < T.na; p++) {
         Inst& I = in[T.pos[p]];
-        I.st.nEnergy++;
-        const double e = resh[(size_t)p * kKP];
-        if (I.e - e >= I.t) {
+        if (consume(I, resh + (size_t)p * kNT)) {
           I.searching = false;
-          continue;
-        }
-        I.tau *= 0.5;
-        I.t *= 0.5;
-        if (++I.trial >= 10) {
-          I.searching = false;  // x + tau0 * 2^-10, not evaluated by the reference
-          I.lsFailed = true;
           continue;
         }
         next.push_back(T.pos[p]);
