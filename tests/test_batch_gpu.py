@@ -163,3 +163,22 @@ def test_batch_rejects_unsupported_fields(gpu):
     with pytest.raises(ValueError, match="twice"):
         m.Lbfgs().minimiseBatch([ok, ok])
     assert m.Lbfgs().minimiseBatch([]) == []
+
+
+def test_batch_through_the_c_abi(gpu):
+    """minim_minimise_batch called directly: results may be NULL, and the
+    minimised coordinates are left in each State's device coordinates."""
+    import ctypes as C
+
+    from minim_amd import _capi as A
+
+    m = gpu
+    cs = cases(13, sides=(10, 24), extra=((4, 5, 6),))
+    states = [make_state(m, c) for c in cs]
+    lb = m.Lbfgs().setM(5).setMaxIter(20000)
+    arr = (C.c_void_p * len(states))(*[s.ptr for s in states])
+    assert A.load().minim_minimise_batch(lb._handle().ptr, arr, len(states), None) == 0
+    for c, s in zip(cs, states):
+        o = OraclePF(c["grid"], 1, 1.0, [1.0], [c["w"]])
+        g = o.gradient(s.coords())
+        assert np.sqrt(np.mean(g * g)) < 1.0001 * o.convergence
