@@ -19,6 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 sys.path.insert(0, HERE)
 from oracle_bind import RefState, ref_available  # noqa: E402
+from mt64 import minima_input  # noqa: E402
 
 
 def pf_cases():
@@ -179,6 +180,31 @@ def cfg1_cases():
                  **res)]
 
 
+def minima_cases():
+    """L-BFGS runs of the reference whose minimum has interfaces, so that the
+    converged energy is far from 0 and a 1e-8 relative comparison has no
+    floor to fall back on.  Stored: the recipe of the input (mt64.minima_input),
+    the reference's result, and the converged field's mean profile along x
+    (which places the interfaces)."""
+    specs = [
+        dict(name="cfg1-stripes-128x128", grid=[128, 128, 1], nfluid=1, surface_tension=[1.0],
+             interface_size=[1.0], init="stripes", seed=20240, amplitude=0.1),
+        dict(name="ternary-slabs-48x4x4", grid=[48, 4, 4], nfluid=3, surface_tension=[1.0, 0.8, 1.2],
+             interface_size=[1.0], init="slabs", seed=777, amplitude=0.05, nslab=3),
+    ]
+    out = []
+    for c in specs:
+        x0 = minima_input(c)
+        r = RefState.phasefield(c["grid"], x0, c["nfluid"], 1.0, c["surface_tension"], c["interface_size"], "hard")
+        e0 = r.energy(x0)
+        x, res = r.minimise("lbfgs", m=5, max_iter=20000)
+        g = r.gradient(x)
+        prof = x.reshape(c["grid"][0], -1, c["nfluid"]).mean(axis=1)
+        out.append(dict(c, m=5, max_iter=20000, initial_energy=e0, profile=prof.tolist(),
+                        grad_rms_at_x=float(math.sqrt(np.dot(g, g) / g.size)), **res))
+    return out
+
+
 def pf_ext_cases():
     """PhaseField with solid nodes, wetting, pressure, forces, confinement,
     per-site tensions and fixed volumes, set through the reference's setters.
@@ -255,13 +281,37 @@ def pf_ext_cases():
     return out
 
 
+def examples_cases():
+    """The outputs of tests/examples/*_example.cpp built against the
+    reference's headers and library: one record per printed line, keyed by its
+    label (the numbers that follow it)."""
+    import subprocess
+    import tempfile
+
+    root = os.path.dirname(os.path.dirname(HERE))
+    ref = os.environ.get("MINIM_REF_DIR", "/root/reference")
+    lib = os.path.join(root, "oracle", "_ref")
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for ex in ("toy2d", "lj3d"):
+            exe = os.path.join(tmp, ex)
+            subprocess.run(["g++", "-std=c++14", "-O2", "-fopenmp", "-I", os.path.join(ref, "include"),
+                            os.path.join(root, "tests", "examples", ex + "_example.cpp"), "-L", lib, "-lminim_ref",
+                            "-Wl,-rpath," + lib, "-o", exe], check=True)
+            r = subprocess.run([exe], check=True, capture_output=True, text=True)
+            for line in r.stdout.splitlines():
+                f = line.split()
+                out[f[0]] = [float(v) for v in f[1:]]
+    return out
+
+
 def main():
     if not ref_available():
         sys.exit("oracle/_ref/libminim_ref.so is not built (make -C oracle)")
     if os.environ.get("OMP_NUM_THREADS") != "1":
         sys.exit("run with OMP_NUM_THREADS=1")
     makers = dict(pf=pf_cases, lbfgs=lbfgs_cases, fire_gd=fire_gd_cases, lj=lj_cases, bah=bah_cases, cfg1=cfg1_cases,
-                  pf_ext=pf_ext_cases)
+                  pf_ext=pf_ext_cases, minima=minima_cases, examples=examples_cases)
     only = sys.argv[1:] or list(makers)  # e.g. "make_golden.py pf_ext" writes that fixture alone
     data = {k: makers[k]() for k in only}
     for k, v in data.items():

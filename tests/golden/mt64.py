@@ -60,3 +60,27 @@ def cfg1_input():
     """The 128x128x1 binary field of the benchmark's CPU-runnable case:
     phi_i = U[-0.1, 0.1] i.i.d. from mt19937_64 seeded 20240."""
     return uniform(20240, 128 * 128, -0.1, 0.1)
+
+
+def minima_input(case):
+    """Initial state of a ref_minima case, regenerated from its recipe
+    (mt19937_64 draws, this module) so the fixture stores no inputs.
+
+    stripes: the 128x128 benchmark field (phi ~ U[-0.1, 0.1], seed 20240)
+      plus +-bias on the two halves of x, which sets two interfaces.
+    slabs: nslab slabs of pure fluids along x, each site mixed with
+      amplitude * U[-1, 1] noise per fluid, renormalised to sum 1."""
+    grid, nf = case["grid"], case["nfluid"]
+    ns = int(np.prod(grid))
+    if case["init"] == "stripes":
+        x0 = cfg1_input() if tuple(grid) == (128, 128, 1) else uniform(case["seed"], ns, -0.1, 0.1)
+        xg = np.arange(grid[0])
+        bias = np.where(np.sin(2 * np.pi * (xg + 0.5) / grid[0]) > 0, case["amplitude"], -case["amplitude"])
+        return x0 + np.repeat(bias, grid[1] * grid[2])
+    u = uniform(case["seed"], ns * nf, -1.0, 1.0).reshape(ns, nf)
+    lab = np.arange(grid[0]) * case["nslab"] // grid[0]
+    c = np.zeros((grid[0], nf))
+    c[np.arange(grid[0]), lab] = 1.0
+    c = np.repeat(c[:, None, :], grid[1] * grid[2], axis=1).reshape(-1, nf)
+    x0 = c + case["amplitude"] * u
+    return (x0 / x0.sum(axis=1, keepdims=True)).reshape(-1)

@@ -341,3 +341,27 @@ def test_bah_minima_known_answers():
                 [0, 0, 0.5 - dz, 0, -s2, 0.5 + dz, 0, s2, 0.5 + dz, 0, 0, 0.5 - dz]]
     for m, ex in zip(c["minima"], expected):
         np.testing.assert_allclose(m["x"], ex, atol=1e-4)
+
+
+@pytest.mark.parametrize("case", range(2))
+def test_interface_minima_lbfgs_golden(case):
+    """L-BFGS runs whose minimum has interfaces (E far from 0): the oracle
+    reproduces the reference's run bit for bit, and the converged state is a
+    stationary point of the oracle's energy."""
+    from mt64 import minima_input
+
+    c = golden("minima")[case]
+    x0 = minima_input(c)
+    o = OraclePF(tuple(c["grid"]), c["nfluid"], 1.0, c["surface_tension"], c["interface_size"])
+    assert o.energy(x0) == c["initial_energy"]
+    x, st = oracle_lbfgs(o, x0, m=c["m"], max_iter=c["max_iter"])
+    assert st["iterations"] == c["iterations"] and st["converged"]
+    assert st["energy"] == c["energy"]
+    prof = x.reshape(c["grid"][0], -1, c["nfluid"]).mean(axis=1)
+    np.testing.assert_array_equal(prof, np.array(c["profile"]))
+    g = o.gradient(x)
+    assert math.sqrt(np.dot(g, g) / g.size) == c["grad_rms_at_x"] < o.convergence
+    # the interfaces carry the energy: E ~ sum of tension x area
+    area = c["grid"][1] * c["grid"][2]
+    tension = 2 * c["surface_tension"][0] if c["nfluid"] == 1 else sum(c["surface_tension"])
+    assert abs(c["energy"] - tension * area) < 0.02 * tension * area
