@@ -14,6 +14,9 @@ namespace minim {
 class BarAndHinge : public NewPotential<BarAndHinge> {
  public:
   int potentialType() const override { return Potential::UNSTRUCTURED; }
+  bool deviceNative() const override { return true; }
+  // element types 0 bars, 1 hinges, 2 external force (BarAndHinge.cpp:35-66)
+  double componentEnergy(int component, const vector<double>& coords) const override;
 
   BarAndHinge& setTriangulation(const vector2d<int>& triList);
   BarAndHinge& setBondList(const vector2d<int>& bondList);

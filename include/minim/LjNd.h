@@ -18,6 +18,9 @@ namespace minim {
 class LjNd : public NewPotential<LjNd> {
  public:
   int potentialType() const override { return Potential::UNSTRUCTURED; }
+  bool deviceNative() const override { return true; }
+  // the reference's elements are all pairs, of type 0 (LjNd.cpp:12-26)
+  double componentEnergy(int component, const vector<double>& coords) const override;
 
   int nDim;
   int nParticle = 0;

@@ -17,11 +17,11 @@ namespace minim {
 class PhaseField : public NewPotential<PhaseField> {
  public:
   int potentialType() const override { return Potential::GRID; }
+  bool deviceNative() const override { return true; }
 
   // System size
   int nFluid = 1;
   double resolution = 1;
-  vector<int> gridSize;
   PhaseField& setNFluid(int nFluid);
   PhaseField& setGridSize(vector<int> gridSize);
   PhaseField& setResolution(double resolution);
@@ -87,7 +87,7 @@ class PhaseField : public NewPotential<PhaseField> {
   void evaluateMixedAsync(const void* x, const void* d, double t, bool wantE, double* e_dev, void* g,
                           void* stream) const override;
   double timeKernel(int prec, const void* x, bool wantE, void* g, int reps, void* stream) const override;
-  void applyConstraints(int prec, void* v, size_t n, void* stream) const override;
+  void constrainDevice(int prec, const void* x, void* v, size_t n, void* stream) const override;
   SiteProjection projection() const override;
   bool hasGlobalConstraints() const override { return volumeFixed; }
   // One fluid on a periodic grid with no solid nodes, extra terms or fixed

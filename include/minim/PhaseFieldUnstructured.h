@@ -18,11 +18,11 @@ namespace minim {
 class PhaseFieldUnstructured : public NewPotential<PhaseFieldUnstructured> {
  public:
   int potentialType() const override { return Potential::UNSTRUCTURED; }
+  bool deviceNative() const override { return true; }
 
   // System size
   int nFluid = 1;
   double resolution = 1;
-  vector<int> gridSize;
   PhaseFieldUnstructured& setNFluid(int nFluid);
   PhaseFieldUnstructured& setGridSize(vector<int> gridSize);
   PhaseFieldUnstructured& setResolution(double resolution);
@@ -79,7 +79,8 @@ class PhaseFieldUnstructured : public NewPotential<PhaseFieldUnstructured> {
   void evaluateAsync(int prec, const void* x, const void* d, double t, bool wantE, double* e_dev, void* g,
                      void* stream) const override;
   double timeKernel(int prec, const void* x, bool wantE, void* g, int reps, void* stream) const override;
-  void applyConstraints(int prec, void* v, size_t n, void* stream) const override;
+  void constrainDevice(int prec, const void* x, void* v, size_t n, void* stream) const override;
+  double componentEnergy(int component, const vector<double>& coords) const override;
   SiteProjection projection() const override;
 
   // Read only (after init)

@@ -8,19 +8,50 @@
 #include <memory>
 #include <vector>
 
+#include "Communicator.h"
 #include "Potential.h"
 
 namespace minim {
+
+class State;
+
+// The reference's public host copy of the coordinates (State::_coords,
+// include/State.h:76), kept here as a view of the device coordinates: a read
+// copies them to the host once after each change, an assignment uploads.
+class HostCoords {
+ public:
+  explicit HostCoords(State* s) : s_(s) {}
+  HostCoords(const HostCoords&) = delete;  // bound to its State
+  HostCoords& operator=(const HostCoords& o);
+  HostCoords& operator=(const vector<double>& v);
+  operator const vector<double>&() const { return get(); }
+  const vector<double>& get() const;
+  double operator[](size_t i) const { return get()[i]; }
+  size_t size() const { return get().size(); }
+  bool empty() const { return get().empty(); }
+  const double* data() const { return get().data(); }
+  vector<double>::const_iterator begin() const { return get().begin(); }
+  vector<double>::const_iterator end() const { return get().end(); }
+
+ private:
+  State* s_;
+};
 
 class State {
  public:
   size_t ndof;
   double convergence;
   std::unique_ptr<Potential> pot;
+  std::unique_ptr<Communicator> comm;
   bool usesThisProc = true;
 
-  // precision: FP64 or FP32 storage of the device coordinates
-  State(const Potential& pot, const vector<double>& coords, int precision = FP64);
+  // State(pot, coords, ranks) (src/State.cpp:12-26): clones and initialises
+  // the potential, sets up the communicator, uploads coords in FP64.  This
+  // process is rank 0 of one: a non-empty rank list without 0 leaves the
+  // State unused here (usesThisProc false), as in the reference.
+  State(const Potential& pot, const vector<double>& coords, const vector<int>& ranks = {});
+  // The same with the device coordinates stored in the given precision.
+  State(const Potential& pot, const vector<double>& coords, Precision precision, const vector<int>& ranks = {});
   State(const State& state);
   State& operator=(const State& state);
   ~State();
@@ -57,6 +88,10 @@ class State {
   vector<double> allGradient() const { return gradient(); }
   void allEnergyGradient(double* e, vector<double>* g) const { energyGradient(e, g); }
   vector<double> allCoords() const { return coords(); }
+
+  // Energy of the potential's elements of one type (src/State.cpp:307-312).
+  double componentEnergy(int component) const;
+
   void communicate() {}
 
   // Constraints (src/State.cpp:321-323) on a host vector
@@ -66,14 +101,24 @@ class State {
   bool isFailed = false;
   void failed();
 
+  HostCoords _coords{this};
+
   // ---- device access ----
   int precision() const { return prec_; }
   void* deviceCoords() const;            // ndof entries of precision()
   void setPrecision(int precision);      // converts the device coordinates in place
+  void setPrecision(Precision precision) { setPrecision(static_cast<int>(precision)); }
   void* stream() const { return stream_; }
   void setStream(void* s) { stream_ = s; }
-  void invalidateHost() { hostValid_ = false; }
-  // incremented whenever coordinates are replaced from outside a minimiser
+  // Declares that the device coordinates changed outside this class (a write
+  // through deviceCoords(), a minimiser's step): the host copy is refreshed on
+  // its next read and cached energies of the old coordinates are dropped.
+  void markCoordsChanged() {
+    hostValid_ = false;
+    version_++;
+  }
+  void invalidateHost() { markCoordsChanged(); }
+  // Changes whenever the coordinates change.
   unsigned long coordsVersion() const { return version_; }
   // A device buffer of the coordinates' size and precision, and the exchange
   // of the coordinates with such a buffer (a minimiser's accepted step written
@@ -82,12 +127,15 @@ class State {
   void swapDeviceCoords(std::shared_ptr<void>& buffer);
 
  private:
+  void construct(const Potential& p, const vector<double>& coords, int precision, const vector<int>& ranks);
+  void bindPotential();
   int prec_ = FP64;
   std::shared_ptr<void> x_;  // device coordinates
   void* stream_ = nullptr;
   mutable vector<double> host_;
   mutable bool hostValid_ = false;
   unsigned long version_ = 0;
+  friend class HostCoords;
 };
 
 }  // namespace minim

@@ -224,8 +224,16 @@ int minim_state_set_precision(minim_state* s, int precision);
 int minim_state_set_stream(minim_state* s, void* stream);
 /* Device address of the coordinates (ndof entries of the State's precision).
  * Valid until the next minimise or precision change on this state: L-BFGS's
- * fused accepted step exchanges the coordinates with a buffer of its own. */
+ * fused accepted step exchanges the coordinates with a buffer of its own.
+ * After writing through it, call minim_state_mark_coords_changed. */
 void* minim_state_device_coords(minim_state* s);
+/* Declares that the device coordinates were changed by the caller (no
+ * reference counterpart: the reference's State::coords(in) setter,
+ * src/State.cpp:286-288, is the only writer there): the host copy is
+ * refreshed on its next read, and a minimiser that continues from this State
+ * (e.g. after an adjustState callback) evaluates the energy afresh instead of
+ * reusing the one it cached for the old coordinates. */
+int minim_state_mark_coords_changed(minim_state* s);
 
 /* ------------------------------------------------------------------ minimisers */
 

@@ -94,6 +94,7 @@ SIGNATURES = {
     "minim_state_set_precision": (C.c_int, [vp, C.c_int]),
     "minim_state_set_stream": (C.c_int, [vp, vp]),
     "minim_state_device_coords": (vp, [vp]),
+    "minim_state_mark_coords_changed": (C.c_int, [vp]),
     "minim_minimiser_create": (C.c_int, [C.POINTER(MinimiserParams), C.POINTER(vp)]),
     "minim_minimiser_destroy": (C.c_int, [vp]),
     "minim_minimise": (C.c_int, [vp, vp, ADJUST_FN, vp, dp, C.POINTER(Result)]),

@@ -527,7 +527,7 @@ void BarAndHinge::setHingeTheta0(int iHinge, double t0) {
 void BarAndHinge::upload() {
   const int nnode = nNodeHint;
   std::vector<char> fixed(3 * (size_t)nnode, 0);
-  for (int i : fixedDofs)
+  for (int i : fixedDofs())
     if (i >= 0 && (size_t)i < fixed.size()) fixed[i] = 1;
   auto allFixed = [&](const int* nd, int k) {
     for (int q = 0; q < k; q++)
@@ -689,13 +689,40 @@ void BarAndHinge::evaluate(int prec, const void* x, const void* d, double t, dou
   if (g && d) throw std::invalid_argument("BarAndHinge: gradients at a trial point are not supported.");
   if (e || g) {
     launch_bah(*dev_, prec, x, d, t, e != nullptr, g, ws.rs(), s);
-    if (g) applyConstraints(prec, g, 3 * (size_t)a.nnode, stream);
+    if (g) constrainDevice(prec, x, g, 3 * (size_t)a.nnode, stream);
   }
   if (e) {
     double r[1];
     ws.fetch(s, 1, r);
     *e = r[0];
   }
+}
+
+// The energy of one element type (State::componentEnergy, src/State.cpp:307-312):
+// the energy pass over bars (type 0), hinges (1) or the external force on the
+// nodes (2) alone, in FP64 at the host coordinates.
+double BarAndHinge::componentEnergy(int component, const vector<double>& coords) const {
+  if (!dev_) throw std::runtime_error("BarAndHinge: not initialised (construct a State first).");
+  auto a = dev_->args;
+  if ((size_t)coords.size() != 3 * (size_t)a.nnode)
+    throw std::invalid_argument("BarAndHinge: coordinates have the wrong size.");
+  if (component != 0) a.nbar = 0;
+  if (component != 1) a.nhinge = 0;
+  if (component != 2) a.fz = nullptr;
+  if (component < 0 || component > 2 || (component == 2 && !dev_->args.fz)) return 0.0;
+  hipStream_t s = nullptr;
+  auto& ws = minim_amd::Workspace::current();
+  void* xd = minim_amd::Staging::current().device(0, coords.size() * sizeof(double));
+  minim_amd::upload_raw(coords.data(), coords.size(), FP64, xd, s);
+  const size_t tot = (size_t)a.nbar + a.nhinge + a.nnode;
+  const int nb = minim_amd::grid_blocks(tot, 4, 2048);
+  minim_amd::bah_energy_kernel<double, false>
+      <<<nb, kBlock, 0, s>>>(a, (const double*)xd, (const double*)nullptr, 0.0, ws.rs(), (double*)nullptr);
+  MINIM_HIP_CHECK(hipGetLastError());
+  minim_amd::launch_finalize(ws.rs(), 1, nb, s);
+  double r[1];
+  ws.fetch(s, 1, r);
+  return r[0];
 }
 
 // Average duration (ms) of reps evaluations on the device (energy kernel with

@@ -436,7 +436,7 @@ int minim_potential_constrain(const minim_potential* pot, int precision, void* v
     check_prec(precision);
     if (!pot->initialised) throw std::runtime_error("potential not initialised: call minim_potential_init");
     if (n != pot->ndof) throw std::invalid_argument("n does not match the initialised potential");
-    pot->p->applyConstraints(precision, v, n, stream);
+    pot->p->constrainDevice(precision, nullptr, v, n, stream);
   });
 }
 
@@ -464,7 +464,8 @@ int minim_state_create(const minim_potential* pot, const double* coords, size_t 
     check_prec(precision);
     auto* h = new minim_state;
     try {
-      h->s = std::make_unique<minim::State>(*pot->p, std::vector<double>(coords, coords + ndof), precision);
+      h->s = std::make_unique<minim::State>(*pot->p, std::vector<double>(coords, coords + ndof),
+                                             static_cast<minim::Precision>(precision));
     } catch (...) {
       delete h;
       throw;
@@ -584,6 +585,13 @@ int minim_state_set_stream(minim_state* s, void* stream) {
 }
 
 void* minim_state_device_coords(minim_state* s) { return s ? s->s->deviceCoords() : nullptr; }
+
+int minim_state_mark_coords_changed(minim_state* s) {
+  return guard([&] {
+    need(s, "state");
+    s->s->markCoordsChanged();
+  });
+}
 
 // ---------------------------------------------------------------- minimisers
 int minim_minimiser_create(const minim_minimiser_params* p, minim_minimiser** out) {

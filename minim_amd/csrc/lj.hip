@@ -590,11 +590,11 @@ void LjNd::evaluate(int prec, const void* x, const void* d, double t, double* e,
       launch_force();
       ws.fetch(s, 2, r);
     }
-    if (g) applyConstraints(prec, g, (size_t)N * nDim, stream);
+    if (g) constrainDevice(prec, x, g, (size_t)N * nDim, stream);
     if (e) *e = r[0];
     return;
   }
-  if (g) applyConstraints(prec, g, (size_t)N * nDim, stream);
+  if (g) constrainDevice(prec, x, g, (size_t)N * nDim, stream);
   if (e) {
     double r[1];
     ws.fetch(s, 1, r);
@@ -618,7 +618,7 @@ int LjNd::gradientDeferred(int prec, const void* x, void* g, void* stream) const
   minim_amd::launch_finalize(ws.rs(), 2, cut_blocks(N), s);
   double* res = ws.device_result();
   MINIM_HIP_CHECK(hipMemcpyAsync(res + minim_amd::kDeferSlot, res + 1, sizeof(double), hipMemcpyDeviceToDevice, s));
-  applyConstraints(prec, g, (size_t)N * nDim, stream);
+  constrainDevice(prec, x, g, (size_t)N * nDim, stream);
   return minim_amd::kDeferSlot;
 }
 
@@ -633,12 +633,16 @@ void LjNd::redoGradient(int prec, const void* x, void* g, void* stream) const {
   const double eshift = 4 * epsilon * (src6 * src6 - src6);
   launch_cut(*impl_, prec, x, nullptr, 0.0, N, sigma, epsilon, eshift, 2, g, ws.rs(), s);
   minim_amd::launch_finalize(ws.rs(), 2, cut_blocks(N), s);
-  applyConstraints(prec, g, (size_t)N * nDim, stream);
+  constrainDevice(prec, x, g, (size_t)N * nDim, stream);
 }
 
 // Average duration (ms) of reps launches of the force kernel (cutoff mode:
 // the list is built first when it does not exist yet; all-pairs mode: the
 // kernel and its reduction).
+double LjNd::componentEnergy(int component, const vector<double>& coords) const {
+  return component == 0 ? energy(coords) : 0.0;
+}
+
 double LjNd::timeKernel(int prec, const void* x, bool wantE, void* g, int reps, void* stream) const {
   if (!impl_) throw std::runtime_error("LjNd: not initialised (construct a State first).");
   const int mode = (wantE ? 1 : 0) | (g ? 2 : 0);

@@ -23,6 +23,7 @@
 //  * Fire reads the scalars an iteration needs (g.g, the next v.g and v.v,
 //    the largest force, the potential's validity flag) in one host
 //    synchronisation.
+#include <algorithm>
 #include <array>
 #include <chrono>
 #include <cmath>
@@ -440,15 +441,37 @@ void Lbfgs::iteration(State& state) {
   // Constraints that couple all DOFs (fixed volumes) act on the step too
   // (Lbfgs.cpp:51); d lies in their subspace up to rounding, so the direction
   // pass's dots stay valid.  The projection leaves result[0, 80) untouched.
-  if (state.pot->hasGlobalConstraints()) state.pot->applyConstraints(prec, a.out, n, s);
+  // Normal constraints and a host potential's constraints can move d: then
+  // the dots are taken again after the projection, except d.d, which the
+  // maxStep cap reads before it (Lbfgs.cpp:121-124 precede :51).
+  const bool moves = state.pot->constraintsMoveSteps();
+  std::vector<double> dres(1 + a.ndot);
+  double ddPre = 0;
+  if (moves) {
+    ws.fetch(s, 1 + a.ndot, dres.data());
+    ddPre = dres[0];
+  }
+  if (state.pot->hasGlobalConstraints())
+    state.pot->constrainDevice(prec, prec == state.precision() ? state.deviceCoords() : nullptr, a.out, n, s);
+  if (moves) {
+    // dres = [d.d, d.g, d.y_0, ...] of the projected d, up to 8 dots a launch
+    std::vector<const void*> pb(1 + a.ndot);
+    pb[0] = a.out;
+    for (int i = 0; i < a.ndot; i++) pb[1 + i] = a.dotv[i];
+    const void* pa[8] = {a.out, a.out, a.out, a.out, a.out, a.out, a.out, a.out};
+    for (int j0 = 0; j0 <= a.ndot; j0 += 8) {
+      const int k = std::min(8, a.ndot + 1 - j0);
+      minim_amd::launch_dots(prec, pa, pb.data() + j0, k, n, ws.rs(), s);
+      ws.fetch(s, k, dres.data() + j0);
+    }
+  }
   // Without a maxStep cap the line search starts at x + d (tau = 1) unless d
   // must be flipped (rare).  Launch that trial energy now, after stashing the
   // direction pass's dots, so that one host synchronisation returns both.
   // When the potential can, that trial is the fused step: energy and gradient
   // at x + d, and x + d itself into a spare buffer, in one pass; if the trial
   // is accepted the spare becomes the coordinates and the gradient is there.
-  std::vector<double> dres(1 + a.ndot);
-  const bool spec = linesearch == "backtracking" && (P == 0 || _maxStep == 0);
+  const bool spec = linesearch == "backtracking" && (P == 0 || _maxStep == 0) && !moves;
   const bool fused = spec && state.pot->supportsStep();
   double eSpec = 0;
   if (spec) {
@@ -471,10 +494,10 @@ void Lbfgs::iteration(State& state) {
     eSpec = r[0];
     for (int i = 0; i <= a.ndot; i++) dres[i] = r[kStash + i];
     stats.nEnergy++;
-  } else {
+  } else if (!moves) {
     ws.fetch(s, 1 + a.ndot, dres.data());
   }
-  const double dd = dres[0], dgv = dres[1];
+  const double dd = moves ? ddPre : dres[0], dgv = dres[1];
 
   double tau0 = 1.0;
   if (P > 0 && _maxStep != 0) {  // Lbfgs.cpp:121-124
@@ -722,7 +745,8 @@ void Fire::iteration(State& state) {
     I.dt = I.dt * _fDec;
   }
   minim_amd::launch_fire_update(I.prec, mode, I.v.data(), I.g.data(), I.step.data(), f1, f2, I.dt, I.n, s);
-  state.pot->applyConstraints(I.prec, I.step.data(), I.n, state.stream());
+  state.pot->constrainDevice(I.prec, I.prec == state.precision() ? state.deviceCoords() : nullptr, I.step.data(),
+                             I.n, state.stream());
   if (linesearch == "backtracking") {
     double gs = dot1(I.prec, I.g.data(), I.step.data(), I.n, s);
     backtrack(state, I.prec, I.step.data(), 1.0, gs, I.ec, stats);

@@ -242,6 +242,7 @@ PFUHolder& holder_of(const std::shared_ptr<void>& p) {
 
 PhaseFieldUnstructured& PhaseFieldUnstructured::setNFluid(int n) {
   nFluid = n;
+  dofPerNode = n;
   return *this;
 }
 PhaseFieldUnstructured& PhaseFieldUnstructured::setGridSize(vector<int> g) {
@@ -412,8 +413,9 @@ void PhaseFieldUnstructured::init(const vector<double>& coords) {
     throw std::invalid_argument("PhaseFieldUnstructured: nFluid must be between 1 and 16.");
   if ((long)coords.size() != (long)nGrid * nFluid)
     throw std::invalid_argument("Size of coordinates array does not match the grid size.");
-  if (!fixedDofs.empty())
-    throw std::invalid_argument("PhaseFieldUnstructured: single-DOF constraints are not supported; use setFixFluid.");
+  if (!constraints.empty())
+    throw std::invalid_argument(
+        "PhaseFieldUnstructured: single-DOF and normal constraints are not supported; use setFixFluid.");
   setDefaults();
   checkArraySizes();
   assignFluidCoefficients();
@@ -592,6 +594,11 @@ void PhaseFieldUnstructured::evaluateAsync(int prec, const void* x, const void* 
     MINIM_HIP_CHECK(hipMemcpyAsync(e_dev, ws.device_result(), sizeof(double), hipMemcpyDeviceToDevice, s));
 }
 
+double PhaseFieldUnstructured::componentEnergy(int, const vector<double>&) const {
+  throw std::invalid_argument(
+      "PhaseFieldUnstructured: energies per element type are not available (the element list lives on the device).");
+}
+
 double PhaseFieldUnstructured::timeKernel(int prec, const void* x, bool wantE, void* g, int reps, void* stream) const {
   PFUHolder& h = holder_of(dev_);
   if (!g) throw std::invalid_argument("PhaseFieldUnstructured: timing needs a gradient buffer");
@@ -599,7 +606,8 @@ double PhaseFieldUnstructured::timeKernel(int prec, const void* x, bool wantE, v
   return minim_amd::pfu_time_elements(h.u, prec, wantE ? 3 : 2, x, g, reps, ws.rs(), static_cast<hipStream_t>(stream));
 }
 
-void PhaseFieldUnstructured::applyConstraints(int prec, void* v, size_t n, void* stream) const {
+void PhaseFieldUnstructured::constrainDevice(int prec, const void* x, void* v, size_t n, void* stream) const {
+  (void)x;
   PFUHolder& h = holder_of(dev_);
   auto& ws = minim_amd::Workspace::current();
   minim_amd::pf_launch_constrain(h.con, prec, v, n / h.u.nf, ws.rs(), static_cast<hipStream_t>(stream));

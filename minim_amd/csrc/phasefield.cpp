@@ -116,6 +116,7 @@ std::vector<int> pf_site_types(const int grid[3], const char* solid) {
 
 PhaseField& PhaseField::setNFluid(int n) {
   nFluid = n;
+  dofPerNode = n;  // PhaseField.cpp:753-757
   return *this;
 }
 
@@ -348,8 +349,9 @@ void PhaseField::init(const vector<double>& coords) {
     throw std::invalid_argument("PhaseField: nFluid must be between 1 and 16.");
   if ((long)coords.size() != (long)nGrid * nFluid)
     throw std::invalid_argument("PhaseField: Size of coordinates array does not match the grid size.");
-  if (!fixedDofs.empty())
-    throw std::invalid_argument("PhaseField: single-DOF constraints are not supported; use setFixFluid.");
+  if (!constraints.empty())
+    throw std::invalid_argument(
+        "PhaseField: single-DOF and normal constraints are not supported; use setFixFluid.");
   nParams = nFluid;
   setDefaults();
   checkArraySizes();
@@ -675,7 +677,8 @@ void PhaseField::evaluateStepAsync(int px, int pv, const void* x, const void* d,
   minim_amd::pf_launch_eval(args_of(args_), px, pv, 7, x, d, t, g, ws.rs(), s, true, xout);
 }
 
-void PhaseField::applyConstraints(int prec, void* v, size_t n, void* stream) const {
+void PhaseField::constrainDevice(int prec, const void* x, void* v, size_t n, void* stream) const {
+  (void)x;
   const PFArgs& a = args_of(args_);
   auto& ws = minim_amd::Workspace::current();
   minim_amd::pf_launch_constrain(a, prec, v, n / a.nf, ws.rs(), static_cast<hipStream_t>(stream));

@@ -54,6 +54,9 @@ struct DVec {
 class Workspace {
  public:
   static Workspace& current();  // for the current HIP device
+  ~Workspace();
+  Workspace(const Workspace&) = delete;
+  Workspace& operator=(const Workspace&) = delete;
   RedScratch rs() const { return rs_; }
   // Copy result[0..k) to host (synchronising stream s) into out.
   void fetch(hipStream_t s, int k, double* out);
@@ -64,6 +67,29 @@ class Workspace {
   RedScratch rs_{};
   double* host_ = nullptr;
   DeviceBuffer mem_;
+};
+
+// Host <-> device transfer staging, one per host thread and device: two
+// pinned host chunks (double-buffered: the host fills or drains one while the
+// copy engine moves the other) and device buffers that only grow, reused by
+// every transfer and host-boundary evaluation instead of a fresh allocation.
+class Staging {
+ public:
+  static constexpr size_t kChunk = size_t(16) << 20;  // bytes per pinned chunk
+  static Staging& current();  // for the current HIP device
+  ~Staging();
+  Staging(const Staging&) = delete;
+  Staging& operator=(const Staging&) = delete;
+  void* pinned(int i) const { return pin_[i]; }
+  hipEvent_t event(int i) const { return ev_[i]; }
+  // Device buffer slot k (0..3) of at least bytes (contents undefined).
+  void* device(int k, size_t bytes);
+
+ private:
+  Staging();
+  void* pin_[2] = {nullptr, nullptr};
+  hipEvent_t ev_[2] = {nullptr, nullptr};
+  DeviceBuffer dev_[4];
 };
 
 // Host <-> device copies of FP64 host data into a vector of any precision.

@@ -1,0 +1,155 @@
+"""User programs written against minim's C++ API, built unchanged against this
+library (include/minim/, libminim_amd.so).
+
+tests/examples/toy2d_example.cpp and lj3d_example.cpp are shaped like the
+reference's examples/toy2d/toy2d.cpp:7-19 and examples/lj3d/lj3d.cpp:14-52:
+a user-defined potential (energy/gradient overrides, a function-pointer
+potential, energy elements, single-DOF and normal constraints, a State on a
+rank this process does not hold) and Lennard-Jones particles under L-BFGS and
+FIRE.  tests/golden/ref_examples.json holds their output when built against
+the reference itself; on the GPU the same programs built against this library
+must reproduce it.
+
+tests/examples/ref_gpu_binding.cpp is the other direction: the reference's own
+State and Lbfgs driving this library's PhaseField kernel through the C ABI
+(the binding a maintainer adds to the reference).  It is built where the
+reference is present (oracle/Makefile) and run on the GPU.
+"""
+import math
+import os
+import shutil
+import subprocess
+
+import pytest
+
+from conftest import golden
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "minim_amd")
+REF_DIR = os.environ.get("MINIM_REF_DIR", "/root/reference")
+REF_LIB = os.path.join(ROOT, "oracle", "_ref")
+BINDING = os.path.join(REF_LIB, "ref_gpu_binding")
+
+
+def build_ours(tmp_path, name):
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("g++ not available")
+    if not os.path.exists(os.path.join(LIB, "libminim_amd.so")):
+        pytest.fail("libminim_amd.so is not built")
+    exe = str(tmp_path / name)
+    cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include", "minim"),
+           os.path.join(ROOT, "tests", "examples", name + "_example.cpp"), "-L", LIB, "-lminim_amd",
+           "-Wl,-rpath," + LIB, "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    return exe
+
+
+def parse(stdout):
+    out = {}
+    for line in stdout.splitlines():
+        f = line.split()
+        out[f[0]] = [float(v) for v in f[1:]]
+    return out
+
+
+@pytest.mark.parametrize("name", ["toy2d", "lj3d"])
+def test_example_compiles_against_this_library(tmp_path, name):
+    build_ours(tmp_path, name)
+
+
+@pytest.mark.parametrize("name", ["toy2d", "lj3d"])
+def test_example_golden_is_the_reference_output(tmp_path, name):
+    """The fixture is what the reference prints for the same program."""
+    if not os.path.isdir(os.path.join(REF_DIR, "include")) or not os.path.exists(
+            os.path.join(REF_LIB, "libminim_ref.so")):
+        pytest.skip("the reference is not present here")
+    exe = str(tmp_path / (name + "_ref"))
+    subprocess.run(["g++", "-std=c++14", "-O2", "-fopenmp", "-I", os.path.join(REF_DIR, "include"),
+                    os.path.join(ROOT, "tests", "examples", name + "_example.cpp"), "-L", REF_LIB, "-lminim_ref",
+                    "-Wl,-rpath," + REF_LIB, "-o", exe], check=True, timeout=300)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300, env=dict(os.environ, OMP_NUM_THREADS="1"))
+    assert r.returncode == 0
+    ref = golden("examples")
+    for k, v in parse(r.stdout).items():
+        assert ref[k] == v, k
+
+
+def test_reference_binding_builds():
+    """The reference-side binding program compiles against the reference and
+    links with libminim_amd.so (oracle/Makefile, run by the package build)."""
+    if not os.path.isdir(os.path.join(REF_DIR, "src")):
+        pytest.skip("the reference is not present here")
+    r = subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "_ref/ref_gpu_binding"], capture_output=True,
+                       text=True, timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert os.access(BINDING, os.X_OK)
+
+
+def close(a, b, rel, ab=0.0):
+    return abs(a - b) <= rel * abs(b) + ab
+
+
+@pytest.mark.gpu
+def test_toy2d_example_matches_the_reference(gpu, tmp_path):
+    exe = build_ours(tmp_path, "toy2d")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stderr
+    out, ref = parse(r.stdout), golden("examples")
+    # the same user code evaluates every point; the minimisers' vector
+    # arithmetic runs on the device (FP64, other summation orders), so the
+    # trajectories agree to rounding: same iteration counts, same end points
+    for k in ("toy2d-graddescent", "toy2d-lbfgs", "toy2d-normal-constraint", "fnptr-fixed-dof",
+              "springs-elements"):
+        assert out[k][0] == ref[k][0], (k, out[k], ref[k])
+        for a, b in zip(out[k][1:], ref[k][1:]):
+            assert close(a, b, 1e-9, 1e-12), (k, out[k], ref[k])
+    # FIRE's 3000 steps without reaching the tolerance: the same point to 1e-6
+    k = "toy2d-fire"
+    assert out[k][0] == ref[k][0]
+    for a, b in zip(out[k][1:], ref[k][1:]):
+        assert close(a, b, 1e-6, 1e-9), (k, out[k], ref[k])
+    # a normal constraint keeps x0 + x1 at its initial value 5
+    assert abs(sum(out["toy2d-normal-constraint"][2:]) - 5.0) < 1e-12
+    assert out["springs-components"] == pytest.approx(ref["springs-components"], rel=1e-15)
+    assert out["unused-rank"] == ref["unused-rank"] == [0.0, 0.0, 0.0]
+
+
+@pytest.mark.gpu
+def test_lj3d_example_matches_the_reference(gpu, tmp_path):
+    exe = build_ours(tmp_path, "lj3d")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stderr
+    out, ref = parse(r.stdout), golden("examples")
+    for k in ("lj3d", "lj3d-run0", "lj3d-run1", "lj3d-run2", "lj3d-run3", "lj3d-run4", "lj3d-fire"):
+        it, e, seps = out[k][0], out[k][1], out[k][2:]
+        rit, re_, rseps = ref[k][0], ref[k][1], ref[k][2:]
+        # the device's pair sums and vector reductions round differently, and
+        # the tolerance (RMS gradient 1e-4) leaves the end point loose: the
+        # same minimum, with the energy to 1e-6 and the separations to 1e-3
+        assert abs(it - rit) <= max(10, 0.3 * rit), (k, it, rit)
+        assert close(e, re_, 1e-6), (k, e, re_)
+        for a, b in zip(seps, rseps):
+            assert abs(a - b) < 1e-3, (k, seps, rseps)
+        # 2^(1/6): the pair distance at the minimum of one Lennard-Jones pair
+        assert min(seps) == pytest.approx(2 ** (1 / 6), abs=1e-4)
+    assert out["lj3d-component"][0] == pytest.approx(out["lj3d"][1], rel=1e-14)
+
+
+@pytest.mark.gpu
+def test_reference_minimiser_drives_the_gpu_kernel(gpu):
+    """The reference's State and Lbfgs with PhaseField's evaluation on the GPU
+    (the binding of this library into the reference) against the reference's
+    own PhaseField: the same energy and gradient (1e-12) and the same energy
+    after 300 L-BFGS iterations (1e-8 of the initial energy)."""
+    if not os.access(BINDING, os.X_OK):
+        pytest.skip("oracle/_ref/ref_gpu_binding is built only where the reference is present")
+    r = subprocess.run([BINDING], capture_output=True, text=True, timeout=300,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0 and "binding ok" in r.stdout
+    rel = float(r.stdout.split("initial-gradient-relerr")[1].split()[0])
+    assert math.isfinite(rel) and rel <= 1e-12
