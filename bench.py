@@ -549,6 +549,35 @@ def batch_leg(m, n_inst=512, workers=8, seed=4040, cpu_budget=10.0, per_state=Tr
     return out
 
 
+NATIVE_REF = os.path.join(ROOT, "oracle", "_ref", "libminim_ref_native.so")
+NATIVE_FLAGS = "g++ -O3 -std=c++14 -fopenmp -march=x86-64-v4 (AVX-512), serial+OpenMP build (no MPI)"
+
+
+def host_cpu():
+    """Model name and physical core count of the host CPU (/proc/cpuinfo)."""
+    model, cores = None, set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and core is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+    except OSError:
+        pass
+    if core is not None:
+        cores.add((phys, core))
+    return model, (len(cores) or None), os.cpu_count()
+
+
 def cpu_baseline(x, budget_s=12.0):
     """The reference library (oracle/_ref, built from its own sources) timed on
     the host cores: State construction excluded, then as many
@@ -557,8 +586,12 @@ def cpu_baseline(x, budget_s=12.0):
 
     threads = int(os.environ.get("OMP_NUM_THREADS") or min(os.cpu_count() or 1, 16))
     os.environ["OMP_NUM_THREADS"] = str(threads)
-    from oracle_bind import RefState, ref_available
+    from oracle_bind import REF_SO, RefState, ref_available
 
+    model, phys, logical = host_cpu()
+    host = dict(cpu_model=model, physical_cores=phys, logical_cpus=logical, ranks=1, threads_per_rank=threads,
+                build=NATIVE_FLAGS if os.path.basename(REF_SO) == os.path.basename(NATIVE_REF) else
+                "g++ -O3 -std=c++14 -fopenmp -march=x86-64-v3 -ffp-contract=off (the bit-exact test build)")
     if ref_available():
         t = time.perf_counter()
         r = RefState.phasefield(GRID, x, NF, 1.0, SIGMA, [WIDTH], "hard")
@@ -572,7 +605,8 @@ def cpu_baseline(x, budget_s=12.0):
         el = time.perf_counter() - t
         return dict(value=n / el, unit="evals/s", cores=threads, kind="reference",
                     sample="%d State::energyGradient calls on the 256^3x3 field (reference compiled from source, "
-                           "OpenMP %d threads); State construction %.1f s excluded" % (n, threads, setup))
+                           "OpenMP %d threads, 1 process); State construction %.1f s excluded" % (n, threads, setup),
+                    **host)
     from oracle_bind import OraclePF
 
     o = OraclePF(GRID, NF, 1.0, SIGMA, [WIDTH])
@@ -584,8 +618,9 @@ def cpu_baseline(x, budget_s=12.0):
         if time.perf_counter() - t > budget_s:
             break
     el = time.perf_counter() - t
+    host.update(threads_per_rank=1, build="oracle/oracle.c, gcc -O2")
     return dict(value=n / el, unit="evals/s", cores=1, kind="port",
-                sample="%d energy+gradient evaluations of the 256^3x3 field by the serial C restatement" % n)
+                sample="%d energy+gradient evaluations of the 256^3x3 field by the serial C restatement" % n, **host)
 
 
 def traffic_from_profiles():
@@ -602,7 +637,7 @@ def traffic_from_profiles():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -620,6 +655,10 @@ def main():
         print("bench.py: this build measures one GPU only (the whole 256^3 field fits one MI355X); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    # the reference's CPU legs run its native-ISA build when there is one
+    if os.path.exists(NATIVE_REF) and "MINIM_REF_SO" not in os.environ:
+        os.environ["MINIM_REF_SO"] = NATIVE_REF
 
     import numpy as np
     import torch
@@ -683,15 +722,25 @@ def main():
         print(json.dumps(res), flush=True)
         return
 
-    # whole evaluation (stencil kernel + cross-block reduction + energy copy) per step
-    ms32 = time_evals(m, h, "fp32", x32, g32, e_dev, n, args.steps, args.warmup, stream)
-    ms64 = time_evals(m, h, "fp64", x64, g64, e_dev, n, args.steps, args.warmup, stream)
-    # the stencil kernel alone: HIP events around back-to-back launches on the stream
-    ms32k = m.potential_time_kernel(h, "fp32", x32.data_ptr(), n, g_ptr=g32.data_ptr(), reps=args.steps,
-                                    stream=stream.cuda_stream)
-    ms64k = m.potential_time_kernel(h, "fp64", x64.data_ptr(), n, g_ptr=g64.data_ptr(), reps=args.steps,
-                                    stream=stream.cuda_stream)
-    log("E+G fp32 %.3f ms (kernel %.3f), fp64 %.3f ms (kernel %.3f)" % (ms32, ms32k, ms64, ms64k))
+    # One step = one whole evaluation (stencil kernel, cross-block reduction,
+    # energy copy) on the stream.  W untimed steps, then at least 100 ms of
+    # untimed steps (steady clocks), then K timed steps: HIP events bracket the
+    # loop (ms_per_step) and each stencil launch inside it (kernel times).
+    for _ in range(args.warmup):
+        m.potential_eval_async(h, "fp32", x32.data_ptr(), n, g_ptr=g32.data_ptr(), e_dev_ptr=e_dev.data_ptr(),
+                               stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    t32 = m.potential_time_steps(h, "fp32", x32.data_ptr(), n, g32.data_ptr(), e_dev.data_ptr(), steps=args.steps,
+                                 warm_ms=100.0, stream=stream.cuda_stream)
+    t64 = m.potential_time_steps(h, "fp64", x64.data_ptr(), n, g64.data_ptr(), e_dev.data_ptr(), steps=args.steps,
+                                 warm_ms=100.0, stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    ms32, ms32k = t32["step_ms"], t32["kernel_median_ms"]
+    ms64, ms64k = t64["step_ms"], t64["kernel_median_ms"]
+    # the kernel is part of the step: its per-launch time cannot exceed the step's
+    assert ms32k <= ms32 and ms64k <= ms64, (t32, t64)
+    log("E+G fp32 %.4f ms/step (stencil median %.4f, min %.4f, max %.4f), fp64 %.4f ms/step (stencil median %.4f)"
+        % (ms32, ms32k, t32["kernel_min_ms"], t32["kernel_max_ms"], ms64, ms64k))
     if args.mode == "stencil":
         # calibration kernel for the byte counters: a plain copy of 1 GiB into
         # another 1 GiB (far beyond the on-chip caches), 1 GiB read + 1 GiB
@@ -704,19 +753,20 @@ def main():
         del a, b
         ms_e = m.potential_time_kernel(h, "fp32", x32.data_ptr(), n, reps=args.steps, stream=stream.cuda_stream)
         log("E-only fp32 kernel %.3f ms" % ms_e)
-        print(json.dumps({"fp32_eg_ms": ms32k, "fp64_eg_ms": ms64k, "fp32_e_ms": ms_e}), flush=True)
+        print(json.dumps({"fp32": t32, "fp64": t64, "fp32_e_ms": ms_e}), flush=True)
         return
 
     # PCIe-inclusive rate: State::energyGradient(coords) with host FP64 vectors in
     # and out, as a caller of the reference's host API would use it
     host_ms = {}
+    g_host = np.zeros(n)  # the caller's gradient array, reused across calls
     for prec in ("fp32", "fp64"):
         s = m.State(pf, x0, prec)
-        s.energyGradient(x0)
+        s.energyGradient(x0, out=g_host)
         t = time.perf_counter()
-        for _ in range(3):
-            s.energyGradient(x0)
-        host_ms[prec] = (time.perf_counter() - t) / 3 * 1e3
+        for _ in range(5):
+            s.energyGradient(x0, out=g_host)
+        host_ms[prec] = (time.perf_counter() - t) / 5 * 1e3
         del s
     log("host-boundary E+G (upload + eval + download): fp32 %.1f ms, fp64 %.1f ms" % (host_ms["fp32"], host_ms["fp64"]))
 
@@ -789,11 +839,16 @@ def main():
         "roofline": {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach32 / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
                      "kernel": "pf_tile_kernel<float,float,3,hard,E+G>", "kernel_ms": ms32k, "algorithmic_bytes": bytes32,
+                     "kernel_ms_stats": {k: v for k, v in t32.items() if k != "step_ms"},
+                     "timing": "median of %d per-launch HIP-event durations of the stencil inside the timed step "
+                               "loop, after >= 100 ms of untimed steps" % args.steps,
                      "fp64": {"achieved": ach64, "frac": ach64 / HBM_PEAK_GBS, "kernel_ms": ms64k,
+                              "kernel_ms_stats": {k: v for k, v in t64.items() if k != "step_ms"},
                               "algorithmic_bytes": bytes64}},
         "host_boundary": {"fp32_ms_per_eval": host_ms["fp32"], "fp64_ms_per_eval": host_ms["fp64"],
                           "what": "State::energyGradient(coords) with host FP64 coordinates in and host FP64 "
-                                  "gradient out (PCIe transfers included); not the headline value"},
+                                  "gradient out into the caller's array (PCIe transfers through pinned, "
+                                  "double-buffered 16 MiB chunks included); not the headline value"},
         "time_to_converge": ttc,
         "cfg1_time_to_converge": cfg1,
         "other_workloads": others,

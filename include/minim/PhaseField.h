@@ -87,6 +87,8 @@ class PhaseField : public NewPotential<PhaseField> {
   void evaluateMixedAsync(const void* x, const void* d, double t, bool wantE, double* e_dev, void* g,
                           void* stream) const override;
   double timeKernel(int prec, const void* x, bool wantE, void* g, int reps, void* stream) const override;
+  void timeSteps(int prec, const void* x, void* g, double* e_dev, int steps, double warmMs, void* stream,
+                 double* out) const override;
   void constrainDevice(int prec, const void* x, void* v, size_t n, void* stream) const override;
   SiteProjection projection() const override;
   bool hasGlobalConstraints() const override { return volumeFixed; }

@@ -174,6 +174,16 @@ class Potential {
   // cross-block reduction that follows it.  The default throws
   // std::invalid_argument.
   virtual double timeKernel(int prec, const void* x, bool wantE, void* g, int reps, void* stream) const;
+  // Benchmark support: `steps` whole energy+gradient evaluations at x on the
+  // stream (as evaluateAsync, energy copied to e_dev), after untimed
+  // evaluations for warmMs of host time (steady clocks).  HIP events bracket
+  // the loop and, inside it, each launch of the evaluation kernel, so that the
+  // kernel's times come from the very loop whose step time is reported.
+  // out[0] = ms per step over the loop; out[1..4] = median, min, max and mean
+  // of the per-launch kernel durations (ms).  The default throws
+  // std::invalid_argument.
+  virtual void timeSteps(int prec, const void* x, void* g, double* e_dev, int steps, double warmMs, void* stream,
+                         double* out) const;
   virtual void evaluateMixed(const void* x, const void* d, double t, double* e, void* g, void* stream) const;
   // Potential::applyConstraints (src/Potential.cpp:102-115) on a device vector
   // v of n entries, with x the coordinates the constraints are taken at (their

@@ -103,6 +103,11 @@ class State {
 
   HostCoords _coords{this};
 
+  // Energy and/or gradient on raw host arrays of ndof doubles (coords null:
+  // the State's own coordinates; e or g null: not computed), the form the
+  // vector overloads above and the C ABI share.
+  void energyGradientRaw(const double* coords, double* e, double* g) const;
+
   // ---- device access ----
   int precision() const { return prec_; }
   void* deviceCoords() const;            // ndof entries of precision()

@@ -189,6 +189,14 @@ int minim_potential_eval_step(const minim_potential* pot, int precision_x, int p
  * and the gradient pass.  MINIM_EINVAL for other potentials. */
 int minim_potential_time_kernel(const minim_potential* pot, int precision, const void* x, size_t ndof,
                                 int want_energy, void* g, int reps, void* stream, double* ms_per_launch);
+/* Benchmark support (no reference counterpart): steps whole energy+gradient
+ * evaluations at x (energy copied to the device double e_dev) on stream,
+ * after untimed evaluations for warm_ms of host time.  HIP events bracket the
+ * loop and each launch of the evaluation kernel inside it.  out[0] = ms per
+ * evaluation over the loop; out[1..4] = median, min, max, mean of the kernel's
+ * per-launch durations (ms).  PhaseField only; MINIM_EINVAL for the others. */
+int minim_potential_time_steps(const minim_potential* pot, int precision, const void* x, size_t ndof, void* g,
+                               double* e_dev, int steps, double warm_ms, void* stream, double* out);
 /* Potential::applyConstraints on a device vector (src/Potential.cpp:102-115,
  * PhaseField::specialisedConstraints PhaseField.cpp:580-612). */
 int minim_potential_constrain(const minim_potential* pot, int precision, void* v, size_t n, void* stream);

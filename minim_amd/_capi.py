@@ -77,6 +77,7 @@ SIGNATURES = {
     "minim_potential_eval_mixed": (C.c_int, [vp, vp, C.c_size_t, vp, C.c_double, dp, vp, vp]),
     "minim_potential_eval_step": (C.c_int, [vp, C.c_int, C.c_int, vp, C.c_size_t, vp, C.c_double, vp, dp, vp, vp]),
     "minim_potential_time_kernel": (C.c_int, [vp, C.c_int, vp, C.c_size_t, C.c_int, vp, C.c_int, vp, dp]),
+    "minim_potential_time_steps": (C.c_int, [vp, C.c_int, vp, C.c_size_t, vp, vp, C.c_int, C.c_double, vp, dp]),
     "minim_potential_constrain": (C.c_int, [vp, C.c_int, vp, C.c_size_t, vp]),
     "minim_dot": (C.c_int, [C.c_int, vp, vp, C.c_size_t, dp, vp]),
     "minim_state_create": (C.c_int, [vp, dp, C.c_size_t, C.c_int, C.POINTER(vp)]),

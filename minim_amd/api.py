@@ -23,6 +23,14 @@ def _f64(a):
     return np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1))
 
 
+def _out(out, n):
+    if out is None:
+        return np.zeros(n)
+    if not (isinstance(out, np.ndarray) and out.dtype == np.float64 and out.flags.c_contiguous and out.size == n):
+        raise ValueError("out must be a contiguous float64 array of %d entries" % n)
+    return out
+
+
 class _Handle:
     """Owns one C handle and frees it with the given destructor."""
 
@@ -632,14 +640,16 @@ class State:
         A.check(A.load().minim_state_energy(self.ptr, None if x is None else _dptr(x), C.byref(e)))
         return e.value
 
-    def gradient(self, coords=None):
-        g = np.zeros(self.ndof)
+    def gradient(self, coords=None, out=None):
+        g = _out(out, self.ndof)
         x = None if coords is None else _f64(coords)
         A.check(A.load().minim_state_gradient(self.ptr, None if x is None else _dptr(x), _dptr(g)))
         return g
 
-    def energyGradient(self, coords=None):
-        g = np.zeros(self.ndof)
+    def energyGradient(self, coords=None, out=None):
+        """(energy, gradient); out: an optional float64 array of ndof entries
+        that receives the gradient (reused across calls, no new pages)."""
+        g = _out(out, self.ndof)
         e = C.c_double()
         x = None if coords is None else _f64(coords)
         A.check(A.load().minim_state_energy_gradient(self.ptr, None if x is None else _dptr(x), C.byref(e),
@@ -918,6 +928,17 @@ def potential_time_kernel(handle, precision, x_ptr, ndof, g_ptr=None, want_energ
     A.check(A.load().minim_potential_time_kernel(handle.ptr, _PREC[precision], x_ptr, ndof, 1 if want_energy else 0,
                                                  g_ptr, int(reps), stream, C.byref(ms)))
     return ms.value
+
+
+def potential_time_steps(handle, precision, x_ptr, ndof, g_ptr, e_dev_ptr, steps=200, warm_ms=100.0, stream=None):
+    """Whole evaluations timed with HIP events around the loop and around each
+    kernel launch inside it: dict(step_ms, kernel_median_ms, kernel_min_ms,
+    kernel_max_ms, kernel_mean_ms)."""
+    out = (C.c_double * 5)()
+    A.check(A.load().minim_potential_time_steps(handle.ptr, _PREC[precision], x_ptr, ndof, g_ptr, e_dev_ptr,
+                                                int(steps), float(warm_ms), stream, out))
+    return dict(step_ms=out[0], kernel_median_ms=out[1], kernel_min_ms=out[2], kernel_max_ms=out[3],
+                kernel_mean_ms=out[4])
 
 
 def potential_constrain(handle, precision, v_ptr, n, stream=None):

@@ -429,6 +429,21 @@ int minim_potential_time_kernel(const minim_potential* pot, int precision, const
   });
 }
 
+int minim_potential_time_steps(const minim_potential* pot, int precision, const void* x, size_t ndof, void* g,
+                               double* e_dev, int steps, double warm_ms, void* stream, double* out) {
+  return guard([&] {
+    need(pot, "pot");
+    need(x, "x");
+    need(g, "g");
+    need(e_dev, "e_dev");
+    need(out, "out");
+    check_prec(precision);
+    if (!pot->initialised) throw std::runtime_error("potential not initialised: call minim_potential_init");
+    if (ndof != pot->ndof) throw std::invalid_argument("ndof does not match the initialised potential");
+    pot->p->timeSteps(precision, x, g, e_dev, steps, warm_ms, stream, out);
+  });
+}
+
 int minim_potential_constrain(const minim_potential* pot, int precision, void* v, size_t n, void* stream) {
   return guard([&] {
     need(pot, "pot");
@@ -492,7 +507,8 @@ int minim_state_energy(minim_state* s, const double* coords, double* e) {
   return guard([&] {
     need(s, "state");
     need(e, "e");
-    *e = coords ? s->s->energy(std::vector<double>(coords, coords + s->s->ndof)) : s->s->energy();
+    *e = 0;
+    s->s->energyGradientRaw(coords, e, nullptr);
   });
 }
 
@@ -500,21 +516,15 @@ int minim_state_gradient(minim_state* s, const double* coords, double* g) {
   return guard([&] {
     need(s, "state");
     need(g, "g");
-    std::vector<double> gv = coords ? s->s->gradient(std::vector<double>(coords, coords + s->s->ndof))
-                                    : s->s->gradient();
-    std::memcpy(g, gv.data(), sizeof(double) * gv.size());
+    if (!s->s->usesThisProc) return;
+    s->s->energyGradientRaw(coords, nullptr, g);
   });
 }
 
 int minim_state_energy_gradient(minim_state* s, const double* coords, double* e, double* g) {
   return guard([&] {
     need(s, "state");
-    std::vector<double> gv;
-    if (coords)
-      s->s->energyGradient(std::vector<double>(coords, coords + s->s->ndof), e, g ? &gv : nullptr);
-    else
-      s->s->energyGradient(e, g ? &gv : nullptr);
-    if (g) std::memcpy(g, gv.data(), sizeof(double) * gv.size());
+    s->s->energyGradientRaw(coords, e, g);
   });
 }
 

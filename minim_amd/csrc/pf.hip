@@ -621,8 +621,10 @@ void pf_args_finalize(PFArgs& a, int target_blocks) {
 }
 
 void pf_launch_eval(const PFArgs& a, int prec, int pv, int mode, const void* x, const void* d, double t,
-                    void* g, RedScratch rs, hipStream_t s, bool finalize, void* xout) {
+                    void* g, RedScratch rs, hipStream_t s, bool finalize, void* xout, hipEvent_t evBefore,
+                    hipEvent_t evAfter) {
   int nb;
+  if (evBefore) MINIM_HIP_CHECK(hipEventRecord(evBefore, s));
   if (xout) {
     if (mode != 7 || a.use_ext || a.volfix)
       throw std::invalid_argument("PhaseField: the fused step needs energy, gradient and a direction, and the "
@@ -642,6 +644,7 @@ void pf_launch_eval(const PFArgs& a, int prec, int pv, int mode, const void* x, 
   else
     throw std::invalid_argument("PhaseField: FP32 coordinates with FP64 vectors are not supported");
   MINIM_HIP_CHECK(hipGetLastError());
+  if (evAfter) MINIM_HIP_CHECK(hipEventRecord(evAfter, s));
   if (finalize) {
     launch_finalize(rs, 2, nb, s);
     // fixed volumes: the global part of the gradient projection; g.g anew

@@ -108,8 +108,11 @@ void pf_args_finalize(PFArgs& a, int target_blocks);
 // the volume projection follows (not when finalize is false: timing only).
 // With xout (mode 7, tiled kernel only), the point x + t*d is also written to
 // xout (precision prec): an accepted step and its gradient in one pass.
+// evBefore / evAfter, when given, are recorded on s right before and right
+// after the evaluation kernel (benchmark timing of the kernel inside a step).
 void pf_launch_eval(const PFArgs& a, int prec, int pv, int mode, const void* x, const void* d, double t,
-                    void* g, RedScratch rs, hipStream_t s, bool finalize = true, void* xout = nullptr);
+                    void* g, RedScratch rs, hipStream_t s, bool finalize = true, void* xout = nullptr,
+                    hipEvent_t evBefore = nullptr, hipEvent_t evAfter = nullptr);
 // Potential::applyConstraints: fixed fluids, hard density projection, then
 // (fixed volumes) the global projection, which uses rs as
 // pf_launch_volume_projection does.

@@ -6,10 +6,13 @@
 // evaluation launches the tiled stencil of pf.hip, or the general kernel of
 // pf_ext.hip when solid nodes or any of the terms beyond bulk, gradient and
 // density are set.
+#include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <mutex>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "minim/Minimiser.h"
@@ -619,6 +622,46 @@ double PhaseField::timeKernel(int prec, const void* x, bool wantE, void* g, int 
   MINIM_HIP_CHECK(hipEventDestroy(e0));
   MINIM_HIP_CHECK(hipEventDestroy(e1));
   return (double)ms / reps;
+}
+
+void PhaseField::timeSteps(int prec, const void* x, void* g, double* e_dev, int steps, double warmMs, void* stream,
+                           double* out) const {
+  const PFArgs& a = args_of(args_);
+  if (steps < 1 || !g || !e_dev || !out) throw std::invalid_argument("PhaseField: nothing to time");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  auto& ws = minim_amd::Workspace::current();
+  auto step = [&](hipEvent_t b, hipEvent_t e) {
+    minim_amd::pf_launch_eval(a, prec, prec, 3, x, nullptr, 0.0, g, ws.rs(), s, true, nullptr, b, e);
+    MINIM_HIP_CHECK(hipMemcpyAsync(e_dev, ws.device_result(), sizeof(double), hipMemcpyDeviceToDevice, s));
+  };
+  // untimed evaluations until warmMs of host time have passed
+  const auto t0 = std::chrono::steady_clock::now();
+  do {
+    for (int i = 0; i < 16; i++) step(nullptr, nullptr);
+    MINIM_HIP_CHECK(hipStreamSynchronize(s));
+  } while (std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() < warmMs);
+  std::vector<hipEvent_t> ev(2 * (size_t)steps + 2);
+  for (auto& e : ev) MINIM_HIP_CHECK(hipEventCreate(&e));
+  MINIM_HIP_CHECK(hipEventRecord(ev[0], s));
+  for (int i = 0; i < steps; i++) step(ev[2 + 2 * i], ev[3 + 2 * i]);
+  MINIM_HIP_CHECK(hipEventRecord(ev[1], s));
+  MINIM_HIP_CHECK(hipEventSynchronize(ev[1]));
+  float ms = 0;
+  MINIM_HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+  out[0] = (double)ms / steps;
+  std::vector<double> k(steps);
+  double sum = 0;
+  for (int i = 0; i < steps; i++) {
+    MINIM_HIP_CHECK(hipEventElapsedTime(&ms, ev[2 + 2 * i], ev[3 + 2 * i]));
+    k[i] = ms;
+    sum += ms;
+  }
+  for (auto& e : ev) MINIM_HIP_CHECK(hipEventDestroy(e));
+  std::sort(k.begin(), k.end());
+  out[1] = (steps % 2) ? k[steps / 2] : 0.5 * (k[steps / 2 - 1] + k[steps / 2]);
+  out[2] = k.front();
+  out[3] = k.back();
+  out[4] = sum / steps;
 }
 
 void PhaseField::evaluateMixed(const void* x, const void* d, double t, double* e, void* g, void* stream) const {

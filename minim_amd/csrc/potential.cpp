@@ -380,6 +380,10 @@ double Potential::timeKernel(int, const void*, bool, void*, int, void*) const {
   throw std::invalid_argument("Potential: kernel timing is not available for this potential.");
 }
 
+void Potential::timeSteps(int, const void*, void*, double*, int, double, void*, double*) const {
+  throw std::invalid_argument("Potential: step timing is not available for this potential.");
+}
+
 void Potential::evaluateMixedAsync(const void*, const void*, double, bool, double*, void*, void*) const {
   throw std::invalid_argument("Potential: mixed FP64/FP32 storage is not supported by this potential.");
 }

@@ -159,47 +159,48 @@ double State::energy(const vector<double>& c) const {
 
 vector<double> State::gradient() const {
   if (!usesThisProc) return vector<double>();
-  vector<double> out;
-  energyGradient(nullptr, &out);
+  vector<double> out(ndof);
+  energyGradientRaw(nullptr, nullptr, out.data());
   return out;
 }
 
 vector<double> State::gradient(const vector<double>& c) const {
   if (!usesThisProc) return vector<double>();
-  vector<double> out;
-  energyGradient(c, nullptr, &out);
+  check_size(c.size(), ndof);
+  vector<double> out(ndof);
+  energyGradientRaw(c.data(), nullptr, out.data());
   return out;
 }
 
 void State::energyGradient(double* e, vector<double>* g) const {
   if (!usesThisProc) return;
-  if (!g) {
-    if (e) *e = energy();
-    return;
-  }
-  hipStream_t s = static_cast<hipStream_t>(stream_);
-  void* gd = minim_amd::Staging::current().device(3, ndof * minim_amd::prec_bytes(prec_));
-  pot->evaluate(prec_, deviceCoords(), nullptr, 0.0, e, gd, stream_);
-  g->resize(ndof);
-  minim_amd::download_raw(gd, prec_, ndof, g->data(), s);
+  if (g) g->resize(ndof);
+  energyGradientRaw(nullptr, e, g ? g->data() : nullptr);
 }
 
 void State::energyGradient(const vector<double>& c, double* e, vector<double>* g) const {
   if (!usesThisProc) return;
   check_size(c.size(), ndof);
+  if (g) g->resize(ndof);
+  energyGradientRaw(c.data(), e, g ? g->data() : nullptr);
+}
+
+// Host coordinates go up through the pinned staging chunks (converted to the
+// State's precision on the way), the gradient comes down the same way.
+void State::energyGradientRaw(const double* c, double* e, double* g) const {
+  if (!usesThisProc || (!e && !g)) return;
   hipStream_t s = static_cast<hipStream_t>(stream_);
   auto& st = minim_amd::Staging::current();
   const size_t bytes = ndof * minim_amd::prec_bytes(prec_);
-  void* x = st.device(2, bytes);
-  minim_amd::upload_raw(c.data(), ndof, prec_, x, s);
-  if (!g) {
-    if (e) pot->evaluate(prec_, x, nullptr, 0.0, e, nullptr, stream_);
-    return;
+  const void* x = deviceCoords();
+  if (c) {
+    void* xs = st.device(2, bytes);
+    minim_amd::upload_raw(c, ndof, prec_, xs, s);
+    x = xs;
   }
-  void* gd = st.device(3, bytes);
+  void* gd = g ? st.device(3, bytes) : nullptr;
   pot->evaluate(prec_, x, nullptr, 0.0, e, gd, stream_);
-  g->resize(ndof);
-  minim_amd::download_raw(gd, prec_, ndof, g->data(), s);
+  if (g) minim_amd::download_raw(gd, prec_, ndof, g, s);
 }
 
 double State::componentEnergy(int component) const {

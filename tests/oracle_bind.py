@@ -10,7 +10,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
-REF_SO = os.path.join(ROOT, "oracle", "_ref", "libminim_ref.so")
+# the reference build (bit-exact single-threaded); MINIM_REF_SO may name
+# another build of the same sources (bench.py's CPU baseline uses the native-ISA one)
+REF_SO = os.environ.get("MINIM_REF_SO") or os.path.join(ROOT, "oracle", "_ref", "libminim_ref.so")
 
 DENSITY = {"hard": 0, "gradient": 0, "soft": 1, "energy penalty": 1, "none": 2, "fixed": 3}
 ORC_MAXF = 16
