@@ -266,7 +266,7 @@ def pfu_leg(m, x0, steps=20, warmup=3, ref_n=48):
     return out
 
 
-def lj_leg(m, n_side=64, density=0.85, fire_max_iter=20000, ref_side=10):
+def lj_leg(m, n_side=64, density=0.85, fire_max_iter=100000, ref_side=10):
     """Lennard-Jones at the benchmark size: n_side^3 particles at number density
     0.85 in a periodic cube, truncated-shifted at rc = 2.5 sigma with a 0.3 sigma
     skin.  Particles start on a simple cubic lattice jittered by U[-0.1, 0.1]
@@ -313,7 +313,9 @@ def lj_leg(m, n_side=64, density=0.85, fire_max_iter=20000, ref_side=10):
     torch.cuda.synchronize()
     out["fire"] = dict(seconds=time.perf_counter() - t, iterations=f.stats["iterations"],
                        converged=f.stats["converged"], max_force=f.stats["grad_max"], energy=s.energy(),
-                       max_iter=fire_max_iter)
+                       max_iter=fire_max_iter,
+                       max_force_crossings={k: dict(seconds=v[0], iteration=v[1])
+                                            for k, v in f.stats["crossings"].items()})
     # reference semantics (all pairs, no cutoff) at ref_n particles: GPU beside the reference library
     rn = ref_side ** 3
     ra = 1.12
@@ -351,7 +353,7 @@ def sheet_triangles(nx, ny):
     return np.stack([np.stack([a, b, c], 1), np.stack([a, c, d], 1)], 1).reshape(-1, 3).astype(np.int32)
 
 
-def bah_leg(m, nside=1024, max_iter=2000, ref_side=48):
+def bah_leg(m, nside=1024, max_iter=20000, ref_side=48):
     """Bar-and-hinge sheet at the benchmark size: nside x nside nodes, two
     triangles per lattice square (2(nside-1)^2 triangles).  Nodes sit on an
     equilateral triangular lattice (node (i,j) at (i - j/2, j sqrt(3)/2)) so every
@@ -405,7 +407,9 @@ def bah_leg(m, nside=1024, max_iter=2000, ref_side=48):
     torch.cuda.synchronize()
     out["lbfgs_m10"] = dict(seconds=time.perf_counter() - t, iterations=lb.stats["iterations"],
                             converged=lb.stats["converged"], grad_rms=lb.stats["grad_rms"], initial_energy=e0,
-                            energy=s.energy(), max_iter=max_iter)
+                            energy=s.energy(), max_iter=max_iter, tolerance=s.convergence,
+                            grad_rms_crossings={k: dict(seconds=v[0], iteration=v[1])
+                                                for k, v in lb.stats["crossings"].items()})
     from oracle_bind import RefState, ref_available
 
     if ref_available():
@@ -499,20 +503,28 @@ def batch_leg(m, n_inst=512, workers=8, seed=4040, cpu_budget=10.0, per_state=Tr
         return m.PhaseField().setNFluid(1).setGridSize([s, s, 1]).setSurfaceTension(1.0).setInterfaceSize(
             float(widths[j]))
 
-    # ---- one launch per phase over every running instance, FP64 and mixed
+    # ---- FP64 and mixed, with the iterations driven on the device (per-instance
+    # decisions in kernels, the iteration replayed as a HIP graph) and, for
+    # comparison, by the host (one host synchronisation per phase)
     batched = {}
-    for prec in ("fp64", "mixed"):
+    for prec, control in (("fp64", "graph"), ("mixed", "graph"), ("fp64", "host"), ("mixed", "host")):
         t = time.perf_counter()
         bstates = [m.State(make_pf(j), inputs(j), "fp64") for j in range(n_inst)]
         torch.cuda.synchronize()
         bsetup = time.perf_counter() - t
+        lb = m.Lbfgs().setM(5).setMaxIter(20000).setPrecision(prec).setBatchControl(control)
         t = time.perf_counter()
-        bres = m.Lbfgs().setM(5).setMaxIter(20000).setPrecision(prec).minimiseBatch(bstates)
+        bres = lb.minimiseBatch(bstates)
         torch.cuda.synchronize()
         bwall = time.perf_counter() - t
+        rep = lb.batchReport()
         bsecs = np.array([r["seconds"] for r in bres])
         bevals = sum(r["n_energy"] + r["n_gradient"] for r in bres)
-        batched[prec] = dict(setup_s=bsetup, wall_s=bwall, evals=bevals, evals_per_s=bevals / bwall,
+        batched[prec if control == "graph" else prec + "_host_control"] = dict(
+                             control=control, host_syncs=rep["host_syncs"], replays=rep["replays"],
+                             host_syncs_per_iteration=rep["host_syncs"] / max(1, rep["replays"]),
+                             graph_launches=rep["graph_launches"],
+                             setup_s=bsetup, wall_s=bwall, evals=bevals, evals_per_s=bevals / bwall,
                              instances_per_s=n_inst / bwall, converged=int(sum(r["converged"] for r in bres)),
                              p50_s=float(np.percentile(bsecs, 50)), p99_s=float(np.percentile(bsecs, 99)),
                              iterations_total=int(sum(r["iterations"] for r in bres)),
@@ -644,8 +656,8 @@ def main():
     ap.add_argument("--batch-only", action="store_true",
                     help="batch mode: the batched minimisations only, without the one-State-per-call comparison")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--fire-iters", type=int, default=20000, help="particles mode: FIRE iteration cap")
-    ap.add_argument("--lbfgs-iters", type=int, default=2000, help="particles mode: sheet L-BFGS iteration cap")
+    ap.add_argument("--fire-iters", type=int, default=100000, help="particles mode: FIRE iteration cap")
+    ap.add_argument("--lbfgs-iters", type=int, default=20000, help="particles mode: sheet L-BFGS iteration cap")
     ap.add_argument("--mode", choices=["full", "stencil", "converge", "particles", "batch", "general", "elements"],
                     default="full",
                     help="stencil: only the evaluation timings plus a 256 MiB copy (the calibration "

@@ -5,9 +5,11 @@
 #ifndef MINIM_AMD_MINIMISER_H
 #define MINIM_AMD_MINIMISER_H
 
+#include <array>
 #include <functional>
 #include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace minim {
@@ -25,6 +27,11 @@ struct MinimiseStats {
   double gradMax = 0;    // Fire with setMaxForceConvergence: largest per-particle force norm
   double seconds = 0;    // wall time of the minimisation (host clock)
   double energy = 0;     // last accepted energy (FP64 accumulation)
+  // Wall time (s) and iteration at which the convergence measure (gradRms,
+  // or gradMax for Fire::setMaxForceConvergence) first fell below
+  // 10^-(k+1), k = 0..7; -1 where it never did.
+  double crossSeconds[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+  int crossIter[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
 };
 
 class Minimiser {
@@ -89,6 +96,16 @@ class Lbfgs : public NewMinimiser<Lbfgs> {
 
   int m() const { return _m; }
 
+  // Launch sizes of the direction and Gram passes timed on this State's
+  // problem (m stored pairs, the minimiser's own buffers, HIP events, the
+  // average of reps per candidate block count); the fastest of each is kept
+  // and stored in the process-wide cache, where later minimisations of the
+  // same shape find it.  Returns rows (pass 0 direction / 1 Gram, blocks, ms).
+  std::vector<std::array<double, 3>> autotune(State& state, std::vector<int> blocks = {256, 512, 1024, 2048, 4096},
+                                              int reps = 10);
+  // (direction, Gram) launch sizes in use; 0 = the heuristic.
+  std::pair<int, int> tunedBlocks() const;
+
   // Independent minimisations of many States at once, each with the
   // semantics of minimise(state) under this minimiser's settings: one launch
   // per phase (direction, line-search trial, step, gradient, Gram update)
@@ -101,6 +118,23 @@ class Lbfgs : public NewMinimiser<Lbfgs> {
   // such instances as States on disjoint rank subsets (include/State.h:23,
   // src/Communicator.cpp:219-240).
   std::vector<MinimiseStats> minimiseBatch(const std::vector<State*>& states);
+  // How minimiseBatch drives the iterations.  "graph" (the default): the
+  // per-instance decisions (two-loop recursion, Armijo tests, Gram update,
+  // convergence) run as device kernels, one iteration is a fixed sequence of
+  // launches with per-instance phase masks, captured once as a HIP graph and
+  // replayed, and the host reads one counter every 8 iterations; "device":
+  // the same sequence launched without the graph; "host": the decisions on
+  // the host, one host synchronisation per phase.  All three give the same
+  // results bit for bit.
+  Lbfgs& setBatchControl(std::string mode);
+  struct BatchReport {
+    std::string control;
+    int iterations = 0;       // the most iterations any instance ran
+    long replays = 0;         // iterations executed by the batch
+    long hostSyncs = 0;       // host synchronisations waiting for results
+    long graphLaunches = 0;   // HIP graph launches
+  };
+  BatchReport batchReport;  // of the last minimiseBatch
 
  private:
   int _m = 5;
@@ -108,6 +142,7 @@ class Lbfgs : public NewMinimiser<Lbfgs> {
   double _initStep = 1;
   std::string _precMode = "state";
   double _switchRms = 0;
+  std::string _batchControl = "graph";
   struct Impl;
   std::unique_ptr<Impl> impl_;
 };

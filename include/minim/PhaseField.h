@@ -5,9 +5,11 @@
 #ifndef MINIM_AMD_PHASEFIELD_H
 #define MINIM_AMD_PHASEFIELD_H
 
+#include <array>
 #include <functional>
 #include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "Potential.h"
@@ -89,6 +91,21 @@ class PhaseField : public NewPotential<PhaseField> {
   double timeKernel(int prec, const void* x, bool wantE, void* g, int reps, void* stream) const override;
   void timeSteps(int prec, const void* x, void* g, double* e_dev, int steps, double warmMs, void* stream,
                  double* out) const override;
+
+  // Launch-shape autotuner of the evaluation kernel this field uses (after
+  // init): times one energy+gradient launch (HIP events, kernel only, the
+  // average of reps) at coords for every pair of target block count and
+  // x-march length (0 = automatic), keeps the fastest for precision prec (FP64
+  // or FP32) and stores it in the process-wide cache, where every later
+  // PhaseField of the same shape and kernel finds it at init.  Returns rows
+  // (target, march, ms).  stencilTargetBlocks / stencilXChunk, when set, still
+  // take precedence.  The shape changes the tiling only: gradients are the
+  // same bit for bit.
+  std::vector<std::array<double, 3>> autotune(const vector<double>& coords, int prec,
+                                              vector<int> targets = {256, 512, 1024, 2048},
+                                              vector<int> marches = {0, 4, 8, 16, 32}, int reps = 10);
+  // The tuned (target blocks, x-march) for a precision; (0, 0) = the heuristic.
+  std::pair<int, int> tunedShape(int prec) const;
   void constrainDevice(int prec, const void* x, void* v, size_t n, void* stream) const override;
   SiteProjection projection() const override;
   bool hasGlobalConstraints() const override { return volumeFixed; }

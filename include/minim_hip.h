@@ -90,6 +90,36 @@ int minim_pf_site_types(const int grid[3], const unsigned char* solid, int* type
  * PhaseFieldUnstructured::assignFluidCoefficients, PhaseFieldUnstructured.cpp:71-107). */
 int minim_pf_coefficients(const minim_potential* pot, double* kappa, double* kappa_p);
 
+/* Launch-shape autotuner of a PhaseField's evaluation kernel (no reference
+ * counterpart; the reference has no launch shapes).  For every pair of target
+ * block count targets[i] and x-march length marches[j] (0 = automatic) one
+ * energy+gradient launch of the kernel this field uses (the plain tiled
+ * stencil, the general tiled kernel or the untiled kernels) is timed with HIP
+ * events (average of reps after two warm-up launches) at the host coordinates
+ * coords (ndof values), in device buffers of the library's own.  The fastest
+ * shape is kept for this precision and stored in the process-wide cache
+ * (appended to the file named by MINIM_AUTOTUNE_CACHE, when set), where every
+ * later PhaseField of the same shape and kernel finds it at init.  times
+ * (may be NULL) receives n_targets * n_marches milliseconds, row-major
+ * (negative: a shape the grid cannot take).  The shape changes the tiling
+ * only: gradients are the same bit for bit. */
+int minim_pf_autotune(minim_potential* pot, int precision, const double* coords, size_t ndof, const int* targets,
+                      int n_targets, const int* marches, int n_marches, int reps, double* times, int* best_target,
+                      int* best_march);
+/* The launch shape a PhaseField uses for a precision after autotuning (0, 0:
+ * the heuristic). */
+int minim_pf_tuned_shape(const minim_potential* pot, int precision, int* target, int* march);
+/* Launch sizes of an L-BFGS minimiser's direction and Gram passes on this
+ * State's problem (no reference counterpart): each candidate block count is
+ * timed (HIP events, average of reps) with the minimiser's own history
+ * buffers and m stored pairs; the fastest of each pass is kept and stored in
+ * the process-wide cache.  times (may be NULL): 2 * n_blocks milliseconds,
+ * direction then Gram per candidate. */
+int minim_lbfgs_autotune(minim_minimiser* m, minim_state* s, const int* blocks, int n_blocks, int reps,
+                         double* times, int* best_direction, int* best_gram);
+/* Forget every launch shape the autotuners stored in this process. */
+int minim_autotune_clear(void);
+
 /* PhaseFieldUnstructured (include/potentials/PhaseFieldUnstructured.h:13-104):
  * the element form of the same physics, built from the same parameters
  * (PhaseFieldUnstructured.cpp:110-277, setters :727-856).  Differences from
@@ -222,6 +252,10 @@ int minim_state_energy_gradient(minim_state* s, const double* coords, double* e,
 /* State::coords() get / set (src/State.cpp:282-288). */
 int minim_state_get_coords(minim_state* s, double* out);
 int minim_state_set_coords(minim_state* s, const double* in);
+/* State::componentEnergy (src/State.cpp:307-312): the energy of the
+ * potential's elements of one type at the State's coordinates
+ * (BarAndHinge: 0 bars, 1 hinges, 2 external force; LjNd: 0 pairs). */
+int minim_state_component_energy(minim_state* s, int component, double* e);
 /* State::applyConstraints on a host vector (src/State.cpp:321-323). */
 int minim_state_apply_constraints(minim_state* s, double* data);
 /* State::failed (src/State.cpp:326-328) and the public fields of include/State.h:15-19,72. */
@@ -265,6 +299,7 @@ typedef struct minim_minimiser_params {
   double dt0;        /* Fire::setDt0: initial time step (0 = dt_max, the reference) */
   int max_force_dim; /* Fire::setMaxForceConvergence: converge on the largest norm over groups of
                         this many DOFs (0 = RMS gradient, the reference) */
+  int batch_control; /* Lbfgs::setBatchControl for minim_minimise_batch: 0 graph, 1 device, 2 host */
 } minim_minimiser_params;
 
 typedef struct minim_result {
@@ -280,6 +315,8 @@ typedef struct minim_result {
   double dt_max; /* Fire: dtMax after init */
   double grad_max; /* Fire with max_force_dim: largest per-group norm of the last gradient */
   double seconds;  /* wall time from the start of the call until this minimisation ended */
+  double cross_seconds[8]; /* wall time and iteration at which the convergence measure (grad_rms, or */
+  int cross_iter[8];       /* grad_max with max_force_dim) first fell below 10^-(k+1); -1: never */
 } minim_result;
 
 typedef void (*minim_adjust_fn)(int iter, minim_state* state, void* user);
@@ -303,6 +340,11 @@ int minim_minimise_log(minim_minimiser* m, minim_state* s, const char* log_type,
  * fixed fluid, in FP64; m <= 10.  Results stay in each State's device
  * coordinates; results[i] (may be NULL) describes states[i]. */
 int minim_minimise_batch(minim_minimiser* m, minim_state* const* states, int n_states, minim_result* results);
+/* How the last minim_minimise_batch ran (no reference counterpart): the most
+ * iterations of any instance, the iterations executed, the host
+ * synchronisations that waited for results and the HIP graph launches. */
+int minim_batch_report(const minim_minimiser* m, int* iterations, long* replays, long* host_syncs,
+                       long* graph_launches);
 
 #ifdef __cplusplus
 }

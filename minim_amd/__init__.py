@@ -10,7 +10,7 @@ missing library raises ImportError.
 from . import _capi
 from .api import (BarAndHinge, Fire, GradDescent, Lbfgs, Lj2d, Lj3d, LjNd, Minimiser, PhaseField,
                   PhaseFieldUnstructured, Potential, State, dot, potential_constrain, potential_eval, potential_eval_async, potential_eval_mixed,
-                  potential_eval_step, potential_time_kernel, potential_time_steps, site_types, triangulation_lists)
+                  potential_eval_step, potential_time_kernel, potential_time_steps, autotune_clear, site_types, triangulation_lists)
 
 
 def __getattr__(name):
@@ -20,4 +20,4 @@ def __getattr__(name):
 
 __all__ = ["BarAndHinge", "Fire", "GradDescent", "Lbfgs", "Lj2d", "Lj3d", "LjNd", "Minimiser", "PhaseField",
            "PhaseFieldUnstructured", "Potential", "State", "dot", "potential_constrain", "potential_eval", "potential_eval_async", "potential_eval_mixed",
-           "potential_eval_step", "potential_time_kernel", "potential_time_steps", "site_types", "triangulation_lists", "lib"]
+           "potential_eval_step", "potential_time_kernel", "potential_time_steps", "autotune_clear", "site_types", "triangulation_lists", "lib"]

@@ -41,14 +41,15 @@ class MinimiserParams(C.Structure):
     _fields_ = [("kind", C.c_int), ("max_iter", C.c_int), ("linesearch", C.c_int), ("m", C.c_int),
                 ("max_step", C.c_double), ("init_step", C.c_double), ("precision_mode", C.c_int),
                 ("switch_rms", C.c_double), ("dt_max", C.c_double), ("alpha", C.c_double),
-                ("dt0", C.c_double), ("max_force_dim", C.c_int)]
+                ("dt0", C.c_double), ("max_force_dim", C.c_int), ("batch_control", C.c_int)]
 
 
 class Result(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("n_energy", C.c_long),
                 ("n_gradient", C.c_long), ("n_energy_reused", C.c_long), ("iter_fp32", C.c_int),
                 ("iter_fp64", C.c_int), ("energy", C.c_double), ("grad_rms", C.c_double),
-                ("dt_max", C.c_double), ("grad_max", C.c_double), ("seconds", C.c_double)]
+                ("dt_max", C.c_double), ("grad_max", C.c_double), ("seconds", C.c_double),
+                ("cross_seconds", C.c_double * 8), ("cross_iter", C.c_int * 8)]
 
 
 ADJUST_FN = C.CFUNCTYPE(None, C.c_int, vp, vp)
@@ -77,6 +78,10 @@ SIGNATURES = {
     "minim_potential_eval_mixed": (C.c_int, [vp, vp, C.c_size_t, vp, C.c_double, dp, vp, vp]),
     "minim_potential_eval_step": (C.c_int, [vp, C.c_int, C.c_int, vp, C.c_size_t, vp, C.c_double, vp, dp, vp, vp]),
     "minim_potential_time_kernel": (C.c_int, [vp, C.c_int, vp, C.c_size_t, C.c_int, vp, C.c_int, vp, dp]),
+    "minim_pf_autotune": (C.c_int, [vp, C.c_int, dp, C.c_size_t, ip, C.c_int, ip, C.c_int, C.c_int, dp, ip, ip]),
+    "minim_pf_tuned_shape": (C.c_int, [vp, C.c_int, ip, ip]),
+    "minim_lbfgs_autotune": (C.c_int, [vp, vp, ip, C.c_int, C.c_int, dp, ip, ip]),
+    "minim_autotune_clear": (C.c_int, []),
     "minim_potential_time_steps": (C.c_int, [vp, C.c_int, vp, C.c_size_t, vp, vp, C.c_int, C.c_double, vp, dp]),
     "minim_potential_constrain": (C.c_int, [vp, C.c_int, vp, C.c_size_t, vp]),
     "minim_dot": (C.c_int, [C.c_int, vp, vp, C.c_size_t, dp, vp]),
@@ -96,6 +101,8 @@ SIGNATURES = {
     "minim_state_set_stream": (C.c_int, [vp, vp]),
     "minim_state_device_coords": (vp, [vp]),
     "minim_state_mark_coords_changed": (C.c_int, [vp]),
+    "minim_state_component_energy": (C.c_int, [vp, C.c_int, dp]),
+    "minim_batch_report": (C.c_int, [vp, ip, C.POINTER(C.c_long), C.POINTER(C.c_long), C.POINTER(C.c_long)]),
     "minim_minimiser_create": (C.c_int, [C.POINTER(MinimiserParams), C.POINTER(vp)]),
     "minim_minimiser_destroy": (C.c_int, [vp]),
     "minim_minimise": (C.c_int, [vp, vp, ADJUST_FN, vp, dp, C.POINTER(Result)]),
@@ -106,7 +113,7 @@ SIGNATURES = {
 _lib = None
 
 
-ABI_VERSION = 5  # layout of the parameter/result structs below
+ABI_VERSION = 6  # layout of the parameter/result structs below
 
 
 def load():

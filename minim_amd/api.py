@@ -261,35 +261,40 @@ class PhaseField(Potential):
 
     COMPONENTS = ("fluid", "surface", "pressure", "density constraint", "force", "confinement", "volume constraint")
 
-    def autotune(self, coords, precision="fp32", blocks=(256, 512, 1024, 2048), marches=(0, 8, 16, 32), reps=10):
-        """Launch-shape autotuner of the stencil kernel for this field: times one
-        energy+gradient launch (HIP events, the kernel alone) for every pair of
-        target block count and x-march length (0 = automatic), keeps the fastest
-        as stencilTargetBlocks / stencilXChunk, and returns {(blocks, march):
-        ms}.  The launch shape changes the tiling only: gradients are the same
+    def autotune(self, coords, precision="fp32", blocks=(256, 512, 1024, 2048), marches=(0, 4, 8, 16, 32), reps=10):
+        """Launch-shape autotuner of the evaluation kernel this field uses
+        (PhaseField::autotune, minim_pf_autotune): one energy+gradient launch
+        (HIP events, kernel only) is timed for every pair of target block count
+        and x-march length (0 = automatic) in the library's own device buffers;
+        the fastest is kept for this precision in the library's process-wide
+        cache, which every later handle of the same shape and kernel finds at
+        init.  Returns {(blocks, march): ms} (shapes the grid cannot take are
+        left out).  The shape changes the tiling only: gradients are the same
         bit for bit, energies differ by the order of the block sums."""
-        import torch
-
+        L = A.load()
         x = _f64(coords)
-        dt = torch.float32 if precision == "fp32" else torch.float64
-        xd = torch.from_numpy(x).to("cuda", dt)
-        g = torch.empty_like(xd)
-        keep = (self.stencilTargetBlocks, self.stencilXChunk)
-        times = {}
-        try:
-            for b in blocks:
-                for mch in marches:
-                    self.stencilTargetBlocks, self.stencilXChunk = int(b), int(mch)
-                    h = self.handle(x)
-                    potential_time_kernel(h, precision, xd.data_ptr(), x.size, g.data_ptr(), True, 2)  # warm
-                    times[(int(b), int(mch))] = potential_time_kernel(h, precision, xd.data_ptr(), x.size,
-                                                                      g.data_ptr(), True, reps)
-        except Exception:
-            self.stencilTargetBlocks, self.stencilXChunk = keep
-            raise
-        best = min(times, key=times.get)
-        self.stencilTargetBlocks, self.stencilXChunk = best
-        return times
+        h = self.handle(x)
+        tv = np.ascontiguousarray(blocks, dtype=np.int32)
+        mv = np.ascontiguousarray(marches, dtype=np.int32)
+        times = np.empty(tv.size * mv.size)
+        bt, bm = C.c_int(), C.c_int()
+        A.check(L.minim_pf_autotune(h.ptr, _PREC[precision], _dptr(x), x.size, tv.ctypes.data_as(A.ip), tv.size,
+                                    mv.ctypes.data_as(A.ip), mv.size, int(reps), _dptr(times), C.byref(bt),
+                                    C.byref(bm)))
+        out = {}
+        for i, b in enumerate(tv.tolist()):
+            for j, mch in enumerate(mv.tolist()):
+                if times[i * mv.size + j] >= 0:
+                    out[(b, mch)] = float(times[i * mv.size + j])
+        return out
+
+    def tunedShape(self, precision, coords=None, handle=None):
+        """(target blocks, x-march) a handle of this field uses for a
+        precision after autotuning; (0, 0) = the heuristic."""
+        h = handle if handle is not None else self.handle(coords)
+        t, mch = C.c_int(), C.c_int()
+        A.check(A.load().minim_pf_tuned_shape(h.ptr, _PREC[precision], C.byref(t), C.byref(mch)))
+        return t.value, mch.value
 
     def energyComponents(self, coords, init_coords=None):
         """PhaseField::energyComponents (PhaseField.cpp:706-744): {name: [energy,
@@ -646,6 +651,12 @@ class State:
         A.check(A.load().minim_state_gradient(self.ptr, None if x is None else _dptr(x), _dptr(g)))
         return g
 
+    def componentEnergy(self, component):
+        """State::componentEnergy: energy of the potential's elements of one type."""
+        e = C.c_double()
+        A.check(A.load().minim_state_component_energy(self.ptr, int(component), C.byref(e)))
+        return e.value
+
     def energyGradient(self, coords=None, out=None):
         """(energy, gradient); out: an optional float64 array of ndof entries
         that receives the gradient (reused across calls, no new pages)."""
@@ -729,7 +740,9 @@ class Minimiser:
         self.stats = dict(iterations=r.iterations, converged=bool(r.converged), n_energy=r.n_energy,
                           n_gradient=r.n_gradient, n_energy_reused=r.n_energy_reused, iter_fp32=r.iter_fp32,
                           iter_fp64=r.iter_fp64, energy=r.energy, grad_rms=r.grad_rms, dt_max=r.dt_max,
-                          grad_max=r.grad_max, seconds=r.seconds)
+                          grad_max=r.grad_max, seconds=r.seconds,
+                          crossings={"1e-%d" % (k + 1): (r.cross_seconds[k], r.cross_iter[k]) for k in range(8)
+                                     if r.cross_iter[k] >= 0})
 
     def minimise(self, state, adjust=None, download=True):
         """Minimiser::minimise(state, adjustState) / minimise(state, "<fields>-<N>")
@@ -765,6 +778,25 @@ class Lbfgs(Minimiser):
         self.initStep = 1.0
         self.precisionMode = "state"
         self.switchRms = 0.0
+        self.batchControl = "graph"
+
+    def setBatchControl(self, mode):
+        """Lbfgs::setBatchControl: "graph" (device decisions, HIP-graph-captured
+        iterations), "device" (the same without the graph) or "host"."""
+        if mode not in ("graph", "device", "host"):
+            raise ValueError("Lbfgs: batch control must be graph, device or host.")
+        self.batchControl = mode
+        self._h = None
+        return self
+
+    def batchReport(self):
+        """How the last minimiseBatch ran: iterations, replays, host_syncs,
+        graph_launches."""
+        it = C.c_int()
+        rp, hs, gl = C.c_long(), C.c_long(), C.c_long()
+        A.check(A.load().minim_batch_report(self._handle().ptr, C.byref(it), C.byref(rp), C.byref(hs), C.byref(gl)))
+        return dict(control=self.batchControl, iterations=it.value, replays=rp.value, host_syncs=hs.value,
+                    graph_launches=gl.value)
 
     def setM(self, m):
         self.m = int(m)
@@ -788,6 +820,21 @@ class Lbfgs(Minimiser):
         self.switchRms = float(switchRms)
         self._h = None
         return self
+
+    def autotune(self, state, blocks=(256, 512, 1024, 2048, 4096), reps=10):
+        """Launch sizes of the direction and Gram passes timed on this State's
+        problem (Lbfgs::autotune, minim_lbfgs_autotune); the fastest of each is
+        kept in the library's process-wide cache for every later minimisation
+        of the same shape.  Returns {"direction": {blocks: ms}, "gram": {...},
+        "best": (direction, gram)}."""
+        bv = np.ascontiguousarray(blocks, dtype=np.int32)
+        times = np.empty(2 * bv.size)
+        bd, bg = C.c_int(), C.c_int()
+        A.check(A.load().minim_lbfgs_autotune(self._handle().ptr, state.ptr, bv.ctypes.data_as(A.ip), bv.size,
+                                              int(reps), _dptr(times), C.byref(bd), C.byref(bg)))
+        return {"direction": {int(b): float(times[2 * i]) for i, b in enumerate(bv)},
+                "gram": {int(b): float(times[2 * i + 1]) for i, b in enumerate(bv)},
+                "best": (bd.value, bg.value)}
 
     def minimiseBatch(self, states):
         """Lbfgs::minimiseBatch: independent minimisations of many States (one-fluid
@@ -816,6 +863,7 @@ class Lbfgs(Minimiser):
         p.init_step = self.initStep
         p.precision_mode = {"state": 0, "fp64": 1, "fp32": 2, "mixed": 3}[self.precisionMode]
         p.switch_rms = self.switchRms
+        p.batch_control = {"graph": 0, "device": 1, "host": 2}[self.batchControl]
         return p
 
 
@@ -939,6 +987,11 @@ def potential_time_steps(handle, precision, x_ptr, ndof, g_ptr, e_dev_ptr, steps
                                                 int(steps), float(warm_ms), stream, out))
     return dict(step_ms=out[0], kernel_median_ms=out[1], kernel_min_ms=out[2], kernel_max_ms=out[3],
                 kernel_mean_ms=out[4])
+
+
+def autotune_clear():
+    """Forget every launch shape the autotuners stored in this process."""
+    A.check(A.load().minim_autotune_clear())
 
 
 def potential_constrain(handle, precision, v_ptr, n, stream=None):

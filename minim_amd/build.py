@@ -58,7 +58,11 @@ def build(jobs=None, verbose=True):
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(lambda s: _compile(cc, s), srcs))
     tmp = LIB + ".tmp"
-    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-o", tmp] + objs
+    # -Bsymbolic: the library's own references to its C++ symbols bind to its
+    # own definitions, so a program that also loads another library defining
+    # namespace minim (the reference, as the reference-side binding does) cannot
+    # interpose that library's State, Potential, ... into this one
+    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-Wl,-Bsymbolic", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))

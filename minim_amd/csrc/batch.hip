@@ -85,7 +85,13 @@ struct BDyn {
   double tau;      // trial or step factor
   double cg;       // coefficient of g in the direction
   double cD[kBS], cY[kBS];  // coefficients of pair i's D and Y
+  unsigned act;    // phases this instance takes part in (kAct*; device control)
 };
+
+// Phases of an iteration under device control: the recomputed gradient after
+// the switch to FP64 coordinates, the direction / step / gradient / Gram
+// passes, and the further line-search trials.
+constexpr unsigned kActFresh = 1, kActMain = 2, kActSearch = 4;
 
 // Vectors G, D and Y are stored in TV (double, or float for the mixed
 // schedule); coordinates always in FP64 (holding FP32 values while an
@@ -102,7 +108,13 @@ struct BBufs {
   const int* ablk;  // na + 1 prefix sums of the block counts
   int na;
   double* part;  // per-block partials, K per block
+  unsigned phase;  // nonzero: only instances whose act has this bit take part
 };
+
+// A block of an instance that does not take part in this phase leaves at once
+// (the whole block: the instance is uniform over it).
+#define MINIM_BATCH_SKIP(B, q) \
+  if ((B).phase && !((q).act & (B).phase)) return
 
 __device__ __forceinline__ int find_pos(const int* __restrict__ ablk, int na, int b) {
   int lo = 0, hi = na - 1;
@@ -144,8 +156,10 @@ __device__ __forceinline__ void block_partials(const double (&v)[K], unsigned lo
 // result depends only on that instance's rows.
 __global__ __launch_bounds__(kBlock) void bfinal_kernel(const double* __restrict__ part,
                                                         const int* __restrict__ ablk, int K,
-                                                        double* __restrict__ res) {
+                                                        double* __restrict__ res, const BDyn* __restrict__ dyn,
+                                                        unsigned phase) {
   const int p = blockIdx.x, lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  if (phase && !(dyn[p].act & phase)) return;
   const int b0 = ablk[p], nb = ablk[p + 1] - b0;
   for (int k = w; k < K; k += kBlock / kWave) {
     double s = 0;
@@ -216,6 +230,7 @@ template <int MODE, typename TV>
 __global__ __launch_bounds__(kBlock) void bpf_kernel(BBufs B) {
   const int p = find_pos(B.ablk, B.na, blockIdx.x);
   const BDyn& q = B.dyn[p];
+  MINIM_BATCH_SKIP(B, q);
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
   if (q.lp) bpf_body<MODE, float, TV>(B, q, I, lb);
@@ -269,6 +284,7 @@ template <typename TV>
 __global__ __launch_bounds__(kBlock) void btrial_kernel(BBufs B) {
   const int p = find_pos(B.ablk, B.na, blockIdx.x);
   const BDyn& q = B.dyn[p];
+  MINIM_BATCH_SKIP(B, q);
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
   if (q.lp) btrial_body<float, TV>(B, q, I, lb);
@@ -279,6 +295,7 @@ __global__ __launch_bounds__(kBlock) void btrial_kernel(BBufs B) {
 __global__ __launch_bounds__(kBlock) void bround_kernel(BBufs B) {
   const int p = find_pos(B.ablk, B.na, blockIdx.x);
   const BDyn& q = B.dyn[p];
+  MINIM_BATCH_SKIP(B, q);
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
   double* __restrict__ x = B.X + I.off;
@@ -344,6 +361,7 @@ template <typename TV>
 __global__ __launch_bounds__(kBlock) void bdir_kernel(BBufs B) {
   const int p = find_pos(B.ablk, B.na, blockIdx.x);
   const BDyn& q = B.dyn[p];
+  MINIM_BATCH_SKIP(B, q);
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
   switch (q.np) {
@@ -367,6 +385,7 @@ template <typename TV>
 __global__ __launch_bounds__(kBlock) void baxpy_kernel(BBufs B) {
   const int p = find_pos(B.ablk, B.na, blockIdx.x);
   const BDyn& q = B.dyn[p];
+  MINIM_BATCH_SKIP(B, q);
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
   double* __restrict__ x = B.X + I.off;
@@ -451,6 +470,7 @@ template <typename TV>
 __global__ __launch_bounds__(kBlock) void bgram_kernel(BBufs B) {
   const int p = find_pos(B.ablk, B.na, blockIdx.x);
   const BDyn& q = B.dyn[p];
+  MINIM_BATCH_SKIP(B, q);
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
   switch (q.np) {
@@ -474,6 +494,7 @@ template <typename TV>
 __global__ __launch_bounds__(kBlock) void bref_kernel(BBufs B) {
   const int p = find_pos(B.ablk, B.na, blockIdx.x);
   const BDyn& q = B.dyn[p];
+  MINIM_BATCH_SKIP(B, q);
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
   const size_t N = B.N;
@@ -496,6 +517,309 @@ __global__ __launch_bounds__(kBlock) void bref_kernel(BBufs B) {
   }
   const unsigned long long pm = (1ull << q.np) - 1;
   block_partials<2 * kBS>(acc, pm | (pm << kBS), B.part);
+}
+
+// ---------------------------------------------------------------- device control
+// The per-instance decisions of an iteration (two-loop recursion on the Gram
+// data, the Armijo tests of the backtracking search, the Gram update, the
+// mixed schedule's switch and the convergence test) as kernels with one thread
+// per instance, so that an iteration is a fixed sequence of launches over all
+// instances, each instance taking part in the phases its act bits name.  The
+// arithmetic is the host path's, operation for operation.
+
+struct BCtl {
+  int np;             // stored pairs, oldest first
+  int slot[kBS];      // their history slots
+  double ptau[kBS];   // their step factors (s = tau d)
+  double prho[kBS];   // 1 / s.y
+  int used[kBS];
+  double gg, e;
+  double dg[kBS], yg[kBS], dres[kBS];
+  double dy[kBS][kBS], yy[kBS][kBS];
+  int gcur, dslot, iters, trial, stall;
+  double tau0, tau, t;
+  int done, searching, lp, fresh, lsFailed, converged;
+  long long nEnergy, nGradient;
+  int iterFP32, iterFP64;
+  unsigned long long tdone;  // device wall clock when the instance finished
+  double conv;               // the State's convergence tolerance
+  int n;                     // elements
+};
+
+struct BCtlArgs {
+  BCtl* ctl;
+  BDyn* dyn;
+  const double* res;
+  int ni;
+  int m;
+  double maxStep, initStep, switchRms;
+  int maxIter;
+  int spec, backtracking;
+  size_t rPF, rREF, rDIR, rT0, rT, rGRAM;  // offsets of the phases' results in res
+  int* active;                             // instances still running after the iteration
+  unsigned long long* clock0;              // device wall clock at the start
+};
+
+// the first kNT trials of the sequence (linesearch.cpp:15-27); true when the
+// search has ended
+__device__ bool bc_trials(BCtl& C, const double* e) {
+  for (int k = 0; k < kNT; k++) {
+    C.nEnergy++;
+    if (C.e - e[k] >= C.t) return true;  // accepted at x + tau d
+    C.tau *= 0.5;
+    C.t *= 0.5;
+    if (++C.trial >= 10) {
+      C.lsFailed = 1;  // x + tau0 * 2^-10, not evaluated by the reference
+      return true;
+    }
+  }
+  return false;
+}
+
+// The gradient and energy at the initial coordinates have been reduced into
+// res[rPF]: the state of iteration 0.
+__global__ __launch_bounds__(kBlock) void bc_init_kernel(BCtlArgs A) {
+  const int j = blockIdx.x * kBlock + threadIdx.x;
+  if (j == 0) *A.clock0 = wall_clock64();
+  if (j >= A.ni) return;
+  BCtl& C = A.ctl[j];
+  C.e = A.res[A.rPF + (size_t)j * kKP];
+  C.gg = A.res[A.rPF + (size_t)j * kKP + 1];
+  C.gcur = 0;
+  C.nGradient++;
+}
+
+// Instances that have just left FP32 coordinates take part in the fresh
+// phase: gradient and energy at the same point in FP64, overwriting g, and its
+// dots with the stored pairs.
+__global__ __launch_bounds__(kBlock) void bc_begin_kernel(BCtlArgs A) {
+  const int j = blockIdx.x * kBlock + threadIdx.x;
+  if (j == 0) *A.active = 0;
+  if (j >= A.ni) return;
+  const BCtl& C = A.ctl[j];
+  BDyn& D = A.dyn[j];
+  D.act = 0;
+  if (C.done || !C.fresh) return;
+  D.j = j;
+  D.gcur = C.gcur;
+  D.gout = C.gcur;
+  D.lp = C.lp;
+  D.np = C.np;
+  for (int i = 0; i < C.np; i++) D.slot[i] = C.slot[i];
+  D.act = kActFresh;
+}
+
+// The fresh phase's results, then the direction coefficients (Lbfgs.cpp:84-127).
+__global__ __launch_bounds__(kBlock) void bc_direction_kernel(BCtlArgs A) {
+  const int j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= A.ni) return;
+  BCtl& C = A.ctl[j];
+  BDyn& D = A.dyn[j];
+  if (C.done) {
+    D.act = 0;
+    return;
+  }
+  if (D.act & kActFresh) {
+    C.e = A.res[A.rPF + (size_t)j * kKP];
+    C.gg = A.res[A.rPF + (size_t)j * kKP + 1];
+    const double* r = A.res + A.rREF + (size_t)j * 2 * kBS;
+    for (int i = 0; i < C.np; i++) {
+      C.dg[C.slot[i]] = r[i];
+      C.yg[C.slot[i]] = r[kBS + i];
+    }
+    C.fresh = 0;
+    C.nGradient++;
+  }
+  int spare = -1;
+  for (int s = 0; s <= A.m; s++)
+    if (!C.used[s]) {
+      spare = s;
+      break;
+    }
+  C.dslot = spare;
+  const int P = C.np;
+  D.j = j;
+  D.dslot = spare;
+  D.gcur = C.gcur;
+  D.gout = 1 - C.gcur;
+  D.lp = C.lp;
+  D.tau = 1.0;
+  D.np = P;
+  for (int i = 0; i < P; i++) D.slot[i] = C.slot[i];
+  if (P == 0) {
+    const double gnorm = sqrt(C.gg);
+    D.cg = (gnorm > 0) ? -(A.initStep / gnorm) : -1.0;
+  } else {
+    double alpha[kBS], ag_s[kBS], ay[kBS];
+    for (int i = 0; i < P; i++) ag_s[i] = ay[i] = 0.0;
+    double ag = -1.0;
+    for (int k = P - 1; k >= 0; k--) {
+      double sq = ag * (C.ptau[k] * C.dg[C.slot[k]]);
+      for (int r = 0; r < P; r++) sq += ay[r] * (C.ptau[k] * C.dy[C.slot[k]][C.slot[r]]);
+      alpha[k] = C.prho[k] * sq;
+      ay[k] -= alpha[k];
+    }
+    const int nw = P - 1;
+    const double gamma = 1 / (C.prho[nw] * C.yy[C.slot[nw]][C.slot[nw]]);
+    ag *= gamma;
+    for (int r = 0; r < P; r++) ay[r] *= gamma;
+    for (int k = 0; k < P; k++) {
+      double yr = ag * C.yg[C.slot[k]];
+      for (int r = 0; r < P; r++)
+        yr += ag_s[r] * (C.ptau[r] * C.dy[C.slot[r]][C.slot[k]]) + ay[r] * C.yy[C.slot[k]][C.slot[r]];
+      const double beta = C.prho[k] * yr;
+      ag_s[k] += alpha[k] - beta;
+    }
+    D.cg = ag;
+    for (int r = 0; r < P; r++) {
+      D.cD[r] = ag_s[r] * C.ptau[r];
+      D.cY[r] = ay[r];
+    }
+  }
+  D.act = kActMain;
+}
+
+// The direction's dots: maxStep cap and flip (Lbfgs.cpp:53-57, :121-124), the
+// Armijo slope, and the speculative trials at x + d (linesearch.cpp:8-29).
+__global__ __launch_bounds__(kBlock) void bc_after_dir_kernel(BCtlArgs A) {
+  const int j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= A.ni) return;
+  BCtl& C = A.ctl[j];
+  BDyn& D = A.dyn[j];
+  if (!(D.act & kActMain)) return;
+  const double* r = A.res + A.rDIR + (size_t)j * kKD;
+  const double dd = r[0], dgv = r[1];
+  const int P = C.np;
+  for (int i = 0; i < P; i++) C.dres[C.slot[i]] = r[2 + i];
+  double tau0 = 1.0;
+  if (P > 0 && A.maxStep != 0) {
+    const double ss = sqrt(dd);
+    if (ss > A.maxStep) tau0 = A.maxStep / ss;
+  }
+  double gs = tau0 * dgv;
+  if (gs > 0) {
+    gs = -gs;
+    tau0 = -tau0;
+  }
+  C.tau0 = tau0;
+  C.tau = tau0;
+  C.trial = 0;
+  C.searching = 0;
+  C.lsFailed = 0;
+  D.act = kActMain;
+  if (!A.backtracking) return;
+  C.t = -0.5 * gs;
+  if (A.spec) {
+    if (tau0 == 1.0) {
+      if (bc_trials(C, A.res + A.rT0 + (size_t)j * kNT)) return;
+    } else {
+      C.nEnergy++;  // the speculative trials at x + d were not on this search's sequence
+    }
+  }
+  C.searching = 1;
+  D.tau = C.tau;
+  D.act = kActMain | kActSearch;
+}
+
+// The next kNT trials of the instances still searching.
+__global__ __launch_bounds__(kBlock) void bc_consume_kernel(BCtlArgs A) {
+  const int j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= A.ni) return;
+  BCtl& C = A.ctl[j];
+  BDyn& D = A.dyn[j];
+  if (!(D.act & kActSearch)) return;
+  if (bc_trials(C, A.res + A.rT + (size_t)j * kNT)) {
+    C.searching = 0;
+    D.act = kActMain;
+  } else {
+    D.tau = C.tau;
+  }
+}
+
+// The accepted step factor for the step pass.
+__global__ __launch_bounds__(kBlock) void bc_step_kernel(BCtlArgs A) {
+  const int j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= A.ni) return;
+  const BCtl& C = A.ctl[j];
+  BDyn& D = A.dyn[j];
+  if (!(D.act & kActMain)) return;
+  D.tau = C.tau;
+  D.act = kActMain;
+}
+
+// New energy and the Gram update (Lbfgs.cpp:58-80), the mixed schedule's
+// switch, and Minimiser::minimise's end tests (Minimiser.cpp:27-38,
+// Lbfgs.cpp:130-133).
+__global__ __launch_bounds__(kBlock) void bc_update_kernel(BCtlArgs A) {
+  const int j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= A.ni) return;
+  BCtl& C = A.ctl[j];
+  const BDyn& D = A.dyn[j];
+  if (!(D.act & kActMain)) return;
+  C.e = A.res[A.rPF + (size_t)j * kKP];
+  C.nGradient++;
+  const double* r = A.res + A.rGRAM + (size_t)j * kKG;
+  const double gng = r[2 * kBS], gny = r[2 * kBS + 1];
+  const double yyv = r[4 * kBS + 2];
+  const int ds = C.dslot;
+  const int P = C.np;
+  int oldSlots[kBS];
+  for (int i = 0; i < P; i++) oldSlots[i] = C.slot[i];
+  const double sy = C.tau * r[2 * kBS + 2 + P];
+  if (sy != 0) {
+    for (int i = 0; i < P; i++) {
+      const int sl = oldSlots[i];
+      C.dy[ds][sl] = C.dres[sl];  // d_new.y_old from the direction pass
+      C.dy[sl][ds] = r[2 * kBS + 2 + i];
+      C.yy[ds][sl] = r[3 * kBS + 2 + i];
+      C.yy[sl][ds] = r[3 * kBS + 2 + i];
+    }
+    C.dy[ds][ds] = r[2 * kBS + 2 + P];
+    C.yy[ds][ds] = yyv;
+    C.dg[ds] = r[P];
+    C.yg[ds] = gny;
+    C.used[ds] = 1;
+    C.slot[C.np] = ds;
+    C.ptau[C.np] = C.tau;
+    C.prho[C.np] = 1 / sy;
+    C.np++;
+    if (C.np > A.m) {
+      C.used[C.slot[0]] = 0;
+      for (int i = 1; i < C.np; i++) {
+        C.slot[i - 1] = C.slot[i];
+        C.ptau[i - 1] = C.ptau[i];
+        C.prho[i - 1] = C.prho[i];
+      }
+      C.np--;
+    }
+  }
+  for (int i = 0; i < P; i++) {
+    C.dg[oldSlots[i]] = r[i];
+    C.yg[oldSlots[i]] = r[kBS + i];
+  }
+  C.gg = gng;
+  C.gcur = 1 - C.gcur;
+  C.iters++;
+  const double rms = sqrt(C.gg / C.n);
+  if (C.lp) {
+    const double sw = A.switchRms > 0 ? A.switchRms : 1e3 * C.conv;
+    C.stall = C.lsFailed ? C.stall + 1 : 0;
+    C.iterFP32++;
+    if (rms < sw || C.stall >= 2) {
+      C.lp = 0;
+      C.fresh = 1;  // no convergence test until g is recomputed
+    }
+  } else {
+    C.iterFP64++;
+  }
+  if (!C.fresh && rms < C.conv) {
+    C.converged = 1;
+    C.done = 1;
+  } else if (C.iters > A.maxIter) {
+    C.done = 1;
+  }
+  if (C.done) C.tdone = wall_clock64();
+  else atomicAdd(A.active, 1);
 }
 
 // Host side ------------------------------------------------------------------
@@ -665,12 +989,14 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
     B.na = L.na;
   };
   auto finalize = [&](const Launch& L, int K, size_t resOff) {
-    bfinal_kernel<<<L.na, kBlock, 0, st>>>(B.part, B.ablk, K, resd + resOff);
+    bfinal_kernel<<<L.na, kBlock, 0, st>>>(B.part, B.ablk, K, resd + resOff, B.dyn, 0u);
     MINIM_HIP_CHECK(hipGetLastError());
   };
+  long hostSyncs = 0, graphLaunches = 0;
   auto fetch = [&](size_t count) {
     MINIM_HIP_CHECK(hipMemcpyAsync(resh, resd, sizeof(double) * count, hipMemcpyDeviceToHost, st));
     MINIM_HIP_CHECK(hipStreamSynchronize(st));
+    hostSyncs++;
   };
   auto dyn_base = [&](const Inst& I) {
     BDyn q{};
@@ -688,6 +1014,187 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
   };
   const bool backtracking = linesearch == "backtracking";
   const bool spec = backtracking && _maxStep == 0;  // first trial at x + d, launched with the direction
+  int maxIters = 0;
+
+  if (_batchControl != "host") {
+    // ---- device control: every launch covers all instances (fixed block
+    // prefix sums, one BDyn per instance), the control kernels decide which
+    // instances take part in each phase, and one iteration is a fixed
+    // sequence of launches -- captured once as a HIP graph and replayed
+    // ("graph"), or launched as it is ("device").  The host reads one counter
+    // every kChunk iterations.
+    constexpr int kChunk = 8;
+    const bool useGraph = _batchControl == "graph";
+    std::vector<int> ab(ni + 1, 0);
+    for (int j = 0; j < ni; j++) ab[j + 1] = ab[j] + in[j].nblk;
+    const int nbAll = ab[ni];
+    std::vector<BDyn> dy0(ni);
+    std::vector<BCtl> c0(ni);
+    for (int j = 0; j < ni; j++) {
+      BDyn q{};
+      q.j = j;
+      q.lp = mixed ? 1 : 0;
+      dy0[j] = q;
+      BCtl c{};
+      c.conv = in[j].conv;
+      c.n = in[j].bi.n;
+      c.lp = mixed ? 1 : 0;
+      c0[j] = c;
+    }
+    const size_t rPF = 0, rREF = rPF + (size_t)ni * kKP, rDIR = rREF + (size_t)ni * 2 * kBS,
+                 rT0 = rDIR + (size_t)ni * kKD, rT = rT0 + (size_t)ni * kNT, rGRAM = rT + (size_t)ni * kNT,
+                 rEnd = rGRAM + (size_t)ni * kKG;
+    DeviceBuffer ablkA(sizeof(int) * (ni + 1)), dynA(sizeof(BDyn) * ni), ctlA(sizeof(BCtl) * ni), resA(8 * rEnd),
+        cntA(64);
+    Pinned cntH(64);
+    MINIM_HIP_CHECK(hipMemcpyAsync(ablkA.get(), ab.data(), sizeof(int) * (ni + 1), hipMemcpyHostToDevice, st));
+    MINIM_HIP_CHECK(hipMemcpyAsync(dynA.get(), dy0.data(), sizeof(BDyn) * ni, hipMemcpyHostToDevice, st));
+    MINIM_HIP_CHECK(hipMemcpyAsync(ctlA.get(), c0.data(), sizeof(BCtl) * ni, hipMemcpyHostToDevice, st));
+    MINIM_HIP_CHECK(hipStreamSynchronize(st));
+    // a stream of its own: graph capture needs one other than the null stream
+    hipStream_t cs = nullptr;
+    MINIM_HIP_CHECK(hipStreamCreate(&cs));
+    struct StreamGuard {
+      hipStream_t s;
+      ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } csGuard{cs};
+    BBufs G = B;
+    G.ablk = static_cast<const int*>(ablkA.get());
+    G.na = ni;
+    G.dyn = static_cast<const BDyn*>(dynA.get());
+    G.phase = 0;
+    double* res = static_cast<double*>(resA.get());
+    BCtlArgs A{};
+    A.ctl = static_cast<BCtl*>(ctlA.get());
+    A.dyn = static_cast<BDyn*>(dynA.get());
+    A.res = res;
+    A.ni = ni;
+    A.m = _m;
+    A.maxStep = _maxStep;
+    A.initStep = _initStep;
+    A.switchRms = _switchRms;
+    A.maxIter = maxIter;
+    A.spec = spec ? 1 : 0;
+    A.backtracking = backtracking ? 1 : 0;
+    A.rPF = rPF;
+    A.rREF = rREF;
+    A.rDIR = rDIR;
+    A.rT0 = rT0;
+    A.rT = rT;
+    A.rGRAM = rGRAM;
+    A.active = static_cast<int*>(cntA.get());
+    A.clock0 = reinterpret_cast<unsigned long long*>(static_cast<char*>(cntA.get()) + 8);
+    const int cb = (ni + kBlock - 1) / kBlock;
+    auto run = [&](KFn f, unsigned phase) {
+      BBufs b = G;
+      b.phase = phase;
+      f<<<nbAll, kBlock, 0, cs>>>(b);
+      MINIM_HIP_CHECK(hipGetLastError());
+    };
+    auto fin = [&](unsigned phase, int K, size_t off) {
+      bfinal_kernel<<<ni, kBlock, 0, cs>>>(G.part, G.ablk, K, res + off, G.dyn, phase);
+      MINIM_HIP_CHECK(hipGetLastError());
+    };
+    auto ctl = [&](void (*k)(BCtlArgs)) {
+      k<<<cb, kBlock, 0, cs>>>(A);
+      MINIM_HIP_CHECK(hipGetLastError());
+    };
+    // iteration 0's gradient and energy (the mixed schedule rounds x to FP32 first)
+    if (mixed) {
+      bround_kernel<<<nbAll, kBlock, 0, cs>>>(G);
+      MINIM_HIP_CHECK(hipGetLastError());
+    }
+    run(K.pf_eg, 0);
+    fin(0, kKP, rPF);
+    ctl(bc_init_kernel);
+    auto iteration = [&] {
+      ctl(bc_begin_kernel);
+      run(K.pf_eg, kActFresh);
+      fin(kActFresh, kKP, rPF);
+      run(K.ref, kActFresh);
+      fin(kActFresh, 2 * kBS, rREF);
+      ctl(bc_direction_kernel);
+      run(K.dir, kActMain);
+      fin(kActMain, kKD, rDIR);
+      if (spec) {
+        run(K.pf_trial, kActMain);
+        fin(kActMain, kNT, rT0);
+      }
+      ctl(bc_after_dir_kernel);
+      // up to 12 more trials: the search takes at most 10
+      for (int pass = 0; pass < 3; pass++) {
+        run(K.pf_trial, kActSearch);
+        fin(kActSearch, kNT, rT);
+        ctl(bc_consume_kernel);
+      }
+      ctl(bc_step_kernel);
+      run(K.axpy, kActMain);
+      run(K.pf_eg, kActMain);
+      fin(kActMain, kKP, rPF);
+      run(K.gram, kActMain);
+      fin(kActMain, kKG, rGRAM);
+      ctl(bc_update_kernel);
+    };
+    hipGraph_t gph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (useGraph) {
+      MINIM_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      iteration();
+      MINIM_HIP_CHECK(hipStreamEndCapture(cs, &gph));
+      MINIM_HIP_CHECK(hipGraphInstantiate(&exec, gph, nullptr, nullptr, 0));
+    }
+    struct GraphGuard {
+      hipGraph_t g;
+      hipGraphExec_t e;
+      ~GraphGuard() {
+        if (e) (void)hipGraphExecDestroy(e);
+        if (g) (void)hipGraphDestroy(g);
+      }
+    } graphGuard{gph, exec};
+    long replays = 0;
+    int* cnt = static_cast<int*>(cntH.p);
+    for (;;) {
+      for (int k = 0; k < kChunk; k++) {
+        if (useGraph) {
+          MINIM_HIP_CHECK(hipGraphLaunch(exec, cs));
+          graphLaunches++;
+        } else {
+          iteration();
+        }
+        replays++;
+      }
+      MINIM_HIP_CHECK(hipMemcpyAsync(cnt, A.active, sizeof(int), hipMemcpyDeviceToHost, cs));
+      MINIM_HIP_CHECK(hipStreamSynchronize(cs));
+      hostSyncs++;
+      if (*cnt == 0 || replays > (long)maxIter + kChunk) break;
+    }
+    std::vector<BCtl> cf(ni);
+    unsigned long long clk0 = 0;
+    MINIM_HIP_CHECK(hipMemcpyAsync(cf.data(), A.ctl, sizeof(BCtl) * ni, hipMemcpyDeviceToHost, cs));
+    MINIM_HIP_CHECK(hipMemcpyAsync(&clk0, A.clock0, sizeof(clk0), hipMemcpyDeviceToHost, cs));
+    MINIM_HIP_CHECK(hipStreamSynchronize(cs));
+    int dev = 0, rateKHz = 0;
+    MINIM_HIP_CHECK(hipGetDevice(&dev));
+    MINIM_HIP_CHECK(hipDeviceGetAttribute(&rateKHz, hipDeviceAttributeWallClockRate, dev));
+    const double total = since();
+    for (int j = 0; j < ni; j++) {
+      Inst& I = in[j];
+      const BCtl& c = cf[j];
+      I.e = c.e;
+      I.gg = c.gg;
+      I.iters = c.iters;
+      I.done = c.done != 0;
+      I.st.nEnergy = c.nEnergy;
+      I.st.nGradient = c.nGradient;
+      I.st.iterFP32 = c.iterFP32;
+      I.st.iterFP64 = c.iterFP64;
+      I.st.converged = c.converged != 0;
+      I.st.seconds = (c.done && rateKHz > 0) ? (double)(c.tdone - clk0) / (1e3 * rateKHz) : total;
+      if (I.iters > maxIters) maxIters = I.iters;
+    }
+    batchReport.iterations = maxIters;
+    batchReport.replays = replays;
+  } else {
 
   // ---- iteration 0's gradient (and energy) at the initial coordinates
   {
@@ -716,7 +1223,6 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
     }
   }
 
-  int maxIters = 0;
   for (;;) {
     Launch L;
     for (int j = 0; j < ni; j++)
@@ -978,6 +1484,13 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
     }
   }
 
+  batchReport.iterations = maxIters;
+  batchReport.replays = maxIters;
+  }  // host control
+
+  batchReport.hostSyncs = hostSyncs;
+  batchReport.graphLaunches = graphLaunches;
+  batchReport.control = _batchControl;
   for (int j = 0; j < ni; j++) {
     Inst& I = in[j];
     device_copy(I.s->deviceCoords(), static_cast<double*>(X.get()) + I.bi.off, I.bi.n * 8, st);

@@ -16,6 +16,7 @@
 #include "minim/PhaseField.h"
 #include "minim/PhaseFieldUnstructured.h"
 #include "minim/State.h"
+#include "autotune.hpp"
 #include "runtime.hpp"
 #include "vec.hpp"
 
@@ -114,7 +115,7 @@ void apply_pf_params(const minim_pf_params* p, PF* pf) {
 extern "C" {
 
 const char* minim_last_error(void) { return g_err.c_str(); }
-int minim_abi_version(void) { return 5; }
+int minim_abi_version(void) { return 6; }
 
 // ---------------------------------------------------------------- potentials
 int minim_pf_create(const minim_pf_params* p, minim_potential** out) {
@@ -379,6 +380,70 @@ int minim_potential_eval_async(const minim_potential* pot, int precision, const 
   });
 }
 
+int minim_pf_autotune(minim_potential* pot, int precision, const double* coords, size_t ndof, const int* targets,
+                      int n_targets, const int* marches, int n_marches, int reps, double* times, int* best_target,
+                      int* best_march) {
+  return guard([&] {
+    need(pot, "pot");
+    need(coords, "coords");
+    need(targets, "targets");
+    need(marches, "marches");
+    check_prec(precision);
+    if (!pot->initialised) throw std::runtime_error("potential not initialised: call minim_potential_init");
+    if (ndof != pot->ndof) throw std::invalid_argument("ndof does not match the initialised potential");
+    if (n_targets < 1 || n_marches < 1) throw std::invalid_argument("no candidate shapes");
+    auto* pf = dynamic_cast<minim::PhaseField*>(pot->p.get());
+    if (!pf || dynamic_cast<minim::PhaseFieldUnstructured*>(pot->p.get()))
+      throw std::invalid_argument("the launch-shape autotuner is for PhaseField");
+    std::vector<int> tv(targets, targets + n_targets), mv(marches, marches + n_marches);
+    const auto rows = pf->autotune(std::vector<double>(coords, coords + ndof), precision, tv, mv, reps);
+    if (times) {
+      for (int i = 0; i < n_targets * n_marches; i++) times[i] = -1.0;
+      for (const auto& r : rows)
+        for (int i = 0; i < n_targets; i++)
+          for (int j = 0; j < n_marches; j++)
+            if ((int)r[0] == targets[i] && (int)r[1] == marches[j]) times[i * n_marches + j] = r[2];
+    }
+    const auto best = pf->tunedShape(precision);
+    if (best_target) *best_target = best.first;
+    if (best_march) *best_march = best.second;
+  });
+}
+
+int minim_pf_tuned_shape(const minim_potential* pot, int precision, int* target, int* march) {
+  return guard([&] {
+    need(pot, "pot");
+    check_prec(precision);
+    auto* pf = dynamic_cast<const minim::PhaseField*>(pot->p.get());
+    if (!pf || dynamic_cast<const minim::PhaseFieldUnstructured*>(pot->p.get()))
+      throw std::invalid_argument("launch shapes are tuned for PhaseField");
+    if (!pot->initialised) throw std::runtime_error("potential not initialised: call minim_potential_init");
+    const auto best = pf->tunedShape(precision);
+    if (target) *target = best.first;
+    if (march) *march = best.second;
+  });
+}
+
+int minim_lbfgs_autotune(minim_minimiser* m, minim_state* s, const int* blocks, int n_blocks, int reps,
+                         double* times, int* best_direction, int* best_gram) {
+  return guard([&] {
+    need(m, "minimiser");
+    need(s, "state");
+    need(blocks, "blocks");
+    auto* lb = dynamic_cast<minim::Lbfgs*>(m->m.get());
+    if (!lb) throw std::invalid_argument("the vector-pass autotuner is for Lbfgs");
+    const auto rows = lb->autotune(*s->s, std::vector<int>(blocks, blocks + n_blocks), reps);
+    if (times)
+      for (size_t i = 0; i < rows.size(); i++) times[i] = rows[i][2];
+    if (best_direction) *best_direction = lb->tunedBlocks().first;
+    if (best_gram) *best_gram = lb->tunedBlocks().second;
+  });
+}
+
+int minim_autotune_clear(void) {
+  return guard([&] { minim_amd::tune_clear(); });
+}
+
 int minim_potential_eval_mixed(const minim_potential* pot, const void* x, size_t ndof, const void* d, double t,
                                double* e, void* g, void* stream) {
   return guard([&] {
@@ -545,6 +610,28 @@ int minim_state_set_coords(minim_state* s, const double* in) {
   });
 }
 
+int minim_state_component_energy(minim_state* s, int component, double* e) {
+  return guard([&] {
+    need(s, "state");
+    need(e, "e");
+    *e = s->s->componentEnergy(component);
+  });
+}
+
+int minim_batch_report(const minim_minimiser* m, int* iterations, long* replays, long* host_syncs,
+                       long* graph_launches) {
+  return guard([&] {
+    need(m, "minimiser");
+    auto* lb = dynamic_cast<const minim::Lbfgs*>(m->m.get());
+    if (!lb) throw std::invalid_argument("batch reports come from Lbfgs");
+    const auto& r = lb->batchReport;
+    if (iterations) *iterations = r.iterations;
+    if (replays) *replays = r.replays;
+    if (host_syncs) *host_syncs = r.hostSyncs;
+    if (graph_launches) *graph_launches = r.graphLaunches;
+  });
+}
+
 int minim_state_apply_constraints(minim_state* s, double* data) {
   return guard([&] {
     need(s, "state");
@@ -617,6 +704,9 @@ int minim_minimiser_create(const minim_minimiser_params* p, minim_minimiser** ou
       l->setMaxStep(p->max_step);
       l->setInitStep(p->init_step);
       l->setPrecision(modes[p->precision_mode], p->switch_rms);
+      static const char* controls[] = {"graph", "device", "host"};
+      if (p->batch_control < 0 || p->batch_control > 2) throw std::invalid_argument("invalid batch control");
+      l->setBatchControl(controls[p->batch_control]);
       m = std::move(l);
     } else if (p->kind == MINIM_FIRE) {
       auto f = std::make_unique<minim::Fire>();
@@ -659,6 +749,10 @@ static void fill_result(const minim_minimiser* m, minim_result* r) {
   r->dt_max = f ? f->dtMax : 0.0;
   r->grad_max = st.gradMax;
   r->seconds = st.seconds;
+  for (int k = 0; k < 8; k++) {
+    r->cross_seconds[k] = st.crossSeconds[k];
+    r->cross_iter[k] = st.crossIter[k];
+  }
 }
 
 int minim_minimise(minim_minimiser* m, minim_state* s, minim_adjust_fn fn, void* user, double* coords_out,
@@ -706,6 +800,10 @@ int minim_minimise_batch(minim_minimiser* m, minim_state* const* states, int n_s
       r.energy = st[i].energy;
       r.grad_rms = st[i].gradRms;
       r.seconds = st[i].seconds;
+      for (int k = 0; k < 8; k++) {
+        r.cross_seconds[k] = -1;
+        r.cross_iter[k] = -1;
+      }
     }
   });
 }

@@ -35,6 +35,7 @@
 
 #include "minim/Minimiser.h"
 #include "minim/State.h"
+#include "autotune.hpp"
 #include "runtime.hpp"
 #include "vec.hpp"
 
@@ -70,7 +71,15 @@ void Minimiser::run(State& state, std::function<void(int, State&)> adjustState) 
     if (adjustState) adjustState(iter, state);
     if (state.isFailed) break;
     iteration(state);
-    if (checkConvergence(state)) {
+    const bool conv = checkConvergence(state);
+    // first crossings of the decades of the convergence measure
+    const double measure = stats.gradMax > 0 ? stats.gradMax : stats.gradRms;
+    for (int k = 0; k < 8; k++) {
+      if (stats.crossIter[k] >= 0 || !(measure < std::pow(10.0, -(k + 1)))) continue;
+      stats.crossIter[k] = iter;
+      stats.crossSeconds[k] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    if (conv) {
       stats.converged = true;
       break;
     }
@@ -240,6 +249,7 @@ struct Lbfgs::Impl {
   bool fresh = true;  // next iteration starts from a new gradient
   int stall = 0;
   std::shared_ptr<void> xtrial;  // coordinates-sized buffer for the fused first trial
+  int nbDir = 0, nbGram = 0;     // launch sizes of the two passes (0: the heuristic)
 
   // Empty history; device buffers are kept from the previous call when they
   // are large enough and of the same precision (a minimise on the same or a
@@ -301,7 +311,7 @@ struct Lbfgs::Impl {
 Lbfgs::Lbfgs() : impl_(new Impl) {}
 Lbfgs::Lbfgs(const Lbfgs& o)
     : NewMinimiser<Lbfgs>(o), _m(o._m), _maxStep(o._maxStep), _initStep(o._initStep), _precMode(o._precMode),
-      _switchRms(o._switchRms), impl_(new Impl) {}
+      _switchRms(o._switchRms), _batchControl(o._batchControl), impl_(new Impl) {}
 Lbfgs& Lbfgs::operator=(const Lbfgs& o) {
   Minimiser::operator=(o);
   _m = o._m;
@@ -309,6 +319,7 @@ Lbfgs& Lbfgs::operator=(const Lbfgs& o) {
   _initStep = o._initStep;
   _precMode = o._precMode;
   _switchRms = o._switchRms;
+  _batchControl = o._batchControl;
   impl_.reset(new Impl);
   return *this;
 }
@@ -331,6 +342,13 @@ Lbfgs& Lbfgs::setInitStep(double v) {
   _initStep = v;
   return *this;
 }
+Lbfgs& Lbfgs::setBatchControl(std::string mode) {
+  if (mode != "graph" && mode != "device" && mode != "host")
+    throw std::invalid_argument("Lbfgs: batch control must be graph, device or host.");
+  _batchControl = mode;
+  return *this;
+}
+
 Lbfgs& Lbfgs::setPrecision(std::string mode, double switchRms) {
   if (mode != "state" && mode != "fp64" && mode != "fp32" && mode != "mixed")
     throw std::invalid_argument("Lbfgs: precision must be state, fp64, fp32 or mixed.");
@@ -339,12 +357,116 @@ Lbfgs& Lbfgs::setPrecision(std::string mode, double switchRms) {
   return *this;
 }
 
+namespace {
+// Cache keys of the vector passes' launch sizes: the pass, the vector length,
+// the sites' component count and the number of stored pairs.
+std::string vec_key(const char* pass, size_t n, int nf, int m, int prec) {
+  return minim_amd::tune_key(pass, {(long)n, (long)nf, (long)m}, prec);
+}
+}  // namespace
+
 // Lbfgs::init (Lbfgs.cpp:34-38): empty history.
 void Lbfgs::init(State& state) {
   if (_precMode == "fp64" || _precMode == "mixed") state.setPrecision(_precMode == "mixed" ? FP32 : FP64);
   if (_precMode == "fp32") state.setPrecision(FP32);
-  impl_->alloc(state.precision(), state.ndof, _m);
+  Impl& I = *impl_;
+  I.alloc(state.precision(), state.ndof, _m);
+  // launch sizes an autotuner chose for vectors of this shape (Lbfgs::autotune)
+  const int nf = state.pot->projection().nf;
+  std::vector<int> v;
+  I.nbDir = minim_amd::tune_lookup(vec_key("lbfgs-dir", I.n, nf, _m, I.prec), v) && v.size() == 1 ? v[0] : 0;
+  I.nbGram = minim_amd::tune_lookup(vec_key("lbfgs-gram", I.n, nf, _m, I.prec), v) && v.size() == 1 ? v[0] : 0;
 }
+
+// The vector passes' launch sizes timed on this State's problem (the
+// direction pass and the Gram pass with m stored pairs, the steady state of
+// an iteration), with the minimiser's own history buffers; the fastest of
+// each is kept and stored in the process-wide cache, where the next
+// minimisation of the same shape finds it.  The launch size changes which
+// block sums which elements: results agree to the order of the FP64 sums.
+std::vector<std::array<double, 3>> Lbfgs::autotune(State& state, std::vector<int> blocks, int reps) {
+  if (reps < 1 || blocks.empty()) throw std::invalid_argument("Lbfgs::autotune: nothing to time");
+  for (int b : blocks)
+    if (b < 1 || b > minim_amd::kMaxRedBlocks) throw std::invalid_argument("Lbfgs::autotune: block count out of range");
+  init(state);
+  Impl& I = *impl_;
+  hipStream_t s = hs(state);
+  auto& ws = Workspace::current();
+  const size_t n = I.n;
+  const int P = I.m;
+  // any values: the coordinates widened or rounded into every vector
+  auto fill = [&](DVec& v) { minim_amd::launch_convert(I.prec, v.data(), state.precision(), state.deviceCoords(), n, s); };
+  fill(I.g);
+  fill(I.gnew);
+  for (auto& v : I.D) fill(v);
+  for (auto& v : I.Y) fill(v);
+  minim_amd::DirArgs a{};
+  a.out = I.D[P].data();
+  a.n = n;
+  SiteProjection sp = state.pot->projection();
+  a.proj = minim_amd::SiteProj{sp.nf, sp.fixmask, sp.hard, sp.nvar, sp.inv_nvar};
+  a.basis[0] = I.g.data();
+  a.coef[0] = -1.0;
+  a.nb = 1;
+  for (int q = 0; q < P; q++) {
+    a.basis[a.nb] = I.D[q].data();
+    a.coef[a.nb++] = 0.01;
+  }
+  for (int q = 0; q < P; q++) {
+    a.basis[a.nb] = I.Y[q].data();
+    a.coef[a.nb++] = 0.01;
+  }
+  a.dotv[0] = I.g.data();
+  for (int q = 0; q < P; q++) a.dotv[1 + q] = I.Y[q].data();
+  a.ndot = 1 + P;
+  a.pairs = P;
+  minim_amd::GramArgs ga{};
+  ga.gnew = I.gnew.data();
+  ga.gold = I.g.data();
+  ga.y = I.Y[P].data();
+  ga.n = n;
+  for (int q = 0; q <= P; q++) ga.D[q] = I.D[q].data();
+  ga.nd = P + 1;
+  for (int q = 0; q < P; q++) ga.Y[q] = I.Y[q].data();
+  ga.ny = P;
+  hipEvent_t e0, e1;
+  MINIM_HIP_CHECK(hipEventCreate(&e0));
+  MINIM_HIP_CHECK(hipEventCreate(&e1));
+  auto timeit = [&](auto&& launch) {
+    launch();
+    launch();
+    MINIM_HIP_CHECK(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; r++) launch();
+    MINIM_HIP_CHECK(hipEventRecord(e1, s));
+    MINIM_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    MINIM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    return (double)ms / reps;
+  };
+  std::vector<std::array<double, 3>> rows;  // (pass 0 direction / 1 Gram, blocks, ms)
+  double bestD = 0, bestG = 0;
+  int nbD = 0, nbG = 0;
+  for (int b : blocks) {
+    a.blocks = b;
+    const double md = timeit([&] { minim_amd::launch_direction(a, I.prec, ws.rs(), s); });
+    ga.blocks = b;
+    const double mg = timeit([&] { minim_amd::launch_gram(ga, I.prec, ws.rs(), s); });
+    rows.push_back({0.0, (double)b, md});
+    rows.push_back({1.0, (double)b, mg});
+    if (nbD == 0 || md < bestD) bestD = md, nbD = b;
+    if (nbG == 0 || mg < bestG) bestG = mg, nbG = b;
+  }
+  MINIM_HIP_CHECK(hipEventDestroy(e0));
+  MINIM_HIP_CHECK(hipEventDestroy(e1));
+  I.nbDir = nbD;
+  I.nbGram = nbG;
+  minim_amd::tune_store(vec_key("lbfgs-dir", n, sp.nf, _m, I.prec), {nbD});
+  minim_amd::tune_store(vec_key("lbfgs-gram", n, sp.nf, _m, I.prec), {nbG});
+  I.alloc(I.prec, n, _m);  // an empty history again
+  return rows;
+}
+
+std::pair<int, int> Lbfgs::tunedBlocks() const { return {impl_->nbDir, impl_->nbGram}; }
 
 // Lbfgs::iteration (Lbfgs.cpp:41-81) with getDirection (:84-127).
 void Lbfgs::iteration(State& state) {
@@ -391,6 +513,7 @@ void Lbfgs::iteration(State& state) {
   minim_amd::DirArgs a{};
   a.out = I.D[dslot].data();
   a.n = n;
+  a.blocks = I.nbDir;
   {
     SiteProjection sp = state.pot->projection();
     a.proj = minim_amd::SiteProj{sp.nf, sp.fixmask, sp.hard, sp.nvar, sp.inv_nvar};
@@ -535,6 +658,7 @@ void Lbfgs::iteration(State& state) {
   ga.gold = I.g.data();
   ga.y = I.Y[dslot].data();
   ga.n = n;
+  ga.blocks = I.nbGram;
   for (int i = 0; i < P; i++) ga.D[i] = I.D[I.pairs[i].slot].data();
   ga.D[P] = I.D[dslot].data();
   ga.nd = P + 1;

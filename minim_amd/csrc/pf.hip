@@ -475,8 +475,13 @@ int density_mode(const PFArgs& a) {
 template <typename T, typename TV, int NF>
 TileGeom tile_geom_for(const PFArgs& a) {
   TileGeom G;
-  const int target = a.target_blocks > 0 ? a.target_blocks : kTileTargetBlocks;
-  if (!tile_geom(a.X, a.Y, a.Z, tile_rows<T, NF>(), a.xchunk_override, target, kMaxRedBlocks, G))
+  const int pi = sizeof(T) == 8 ? 0 : 1;
+  const bool user = a.target_blocks > 0 || a.xchunk_override > 0;
+  const int target = a.target_blocks > 0 ? a.target_blocks
+                     : (!user && a.tuned_target[pi] > 0) ? a.tuned_target[pi]
+                                                         : kTileTargetBlocks;
+  const int xo = user ? a.xchunk_override : a.tuned_xchunk[pi];
+  if (!tile_geom(a.X, a.Y, a.Z, tile_rows<T, NF>(), xo, target, kMaxRedBlocks, G))
     throw std::invalid_argument("PhaseField: grid too large for one launch");
   if ((double)a.Y * a.Z * NF * sizeof(T) >= 4294967296.0)
     throw std::invalid_argument("PhaseField: one x plane of the grid exceeds 4 GiB");

@@ -77,8 +77,12 @@ struct PFArgs {
   // marching over nxc chunks of xchunk planes
   int TZ, TY, ntz, nty, ntile;
   int xchunk, nxc, nblocks;
-  int xchunk_override;     // >0 forces the x-march length (autotuning)
-  int target_blocks;       // >0 overrides the launch-size heuristic
+  int xchunk_override;     // >0 forces the x-march length (the caller's setting)
+  int target_blocks;       // >0 overrides the launch-size heuristic (the caller's setting)
+  // launch shape chosen by the autotuner per coordinate precision
+  // ([0] FP64, [1] FP32), used where the caller set none; 0 = heuristic
+  int tuned_target[2];
+  int tuned_xchunk[2];
   // general kernel: used when any solid node or extra term is set (or forced)
   int use_ext;
   const PFExt* ext;        // device

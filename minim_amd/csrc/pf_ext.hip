@@ -870,8 +870,13 @@ int ext_tile_full(const PFArgs& a, int mode, const void* x, const void* d, doubl
                   hipStream_t s) {
   constexpr int R = ext_rows<T, NF>();
   TileGeom G;
-  const int target = a.target_blocks > 0 ? a.target_blocks : kExtTileTarget;
-  if (!tile_geom(a.X, a.Y, a.Z, R, a.xchunk_override, target, kMaxRedBlocks, G))
+  const int pi = sizeof(T) == 8 ? 0 : 1;
+  const bool user = a.target_blocks > 0 || a.xchunk_override > 0;
+  const int target = a.target_blocks > 0 ? a.target_blocks
+                     : (!user && a.tuned_target[pi] > 0) ? a.tuned_target[pi]
+                                                         : kExtTileTarget;
+  const int xo = user ? a.xchunk_override : a.tuned_xchunk[pi];
+  if (!tile_geom(a.X, a.Y, a.Z, R, xo, target, kMaxRedBlocks, G))
     throw std::invalid_argument("PhaseField: grid too large for one launch");
   const T* xx = static_cast<const T*>(x);
   const TV* dd = static_cast<const TV*>((mode & 4) ? d : x);

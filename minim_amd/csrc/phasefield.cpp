@@ -18,6 +18,7 @@
 #include "minim/Minimiser.h"
 #include "minim/PhaseField.h"
 #include "minim/State.h"
+#include "autotune.hpp"
 #include "pf.hpp"
 #include "runtime.hpp"
 
@@ -343,6 +344,29 @@ void PhaseField::assignFluidCoefficients() {
   }
 }
 
+namespace {
+// Tuned shapes are per kernel family: the plain tiled stencil, the general
+// tiled kernel, and the untiled kernels (more than four fluids, per-site
+// tensions), whose single launch shape serves both precisions.
+bool pf_tiled(const PFArgs& a) { return a.use_ext ? (!a.Fs && a.nf <= 4) : a.nf <= 4; }
+
+std::string tune_key_of(const PFArgs& a, int prec) {
+  const char* fam = !pf_tiled(a) ? "pf-untiled" : a.use_ext ? "pf-ext-tile" : "pf-tile";
+  return minim_amd::tune_key(fam, {a.X, a.Y, a.Z, a.nf}, pf_tiled(a) ? prec : 0);
+}
+
+void apply_shape(PFArgs& a, int prec, int target, int march) {
+  a.tuned_target[prec] = target;
+  a.tuned_xchunk[prec] = march;
+  if (!pf_tiled(a) && a.target_blocks <= 0 && a.xchunk_override <= 0) {
+    a.xchunk_override = march;
+    minim_amd::pf_args_finalize(a, target);
+    a.target_blocks = 0;  // still the heuristic's slot: the shape is in xchunk and nblocks
+    a.xchunk_override = 0;
+  }
+}
+}  // namespace
+
 void PhaseField::init(const vector<double>& coords) {
   if (gridSize.size() != 3) throw std::invalid_argument("PhaseField: gridSize must have three entries.");
   for (int s : gridSize)
@@ -571,7 +595,81 @@ void PhaseField::init(const vector<double>& coords) {
   }
   A.ext_host = &h->ext_h;
   A.use_ext = (anySolid || anyWet || A.any_pres || anyForce || anyConf || perSite || stencilKernel == 1) ? 1 : 0;
+  // launch shapes an autotuner chose earlier in this process (or in the file
+  // named by MINIM_AUTOTUNE_CACHE) for a field of this shape and kernel
+  A.tuned_target[0] = A.tuned_target[1] = A.tuned_xchunk[0] = A.tuned_xchunk[1] = 0;
+  for (int prec = 0; prec < 2; prec++) {
+    std::vector<int> v;
+    if (minim_amd::tune_lookup(tune_key_of(A, prec), v) && v.size() == 2) apply_shape(A, prec, v[0], v[1]);
+  }
   args_ = h;
+}
+
+// Launch-shape autotuner of the evaluation kernel this field uses (the plain
+// tiled stencil, the general tiled kernel, or the untiled kernels): one
+// energy+gradient launch (kernel only, HIP events, the average of reps after
+// two warm-up launches) for every pair of target block count and x-march
+// length (0 = automatic); the fastest is kept for this precision and stored in
+// the process-wide cache, where the next field of the same shape finds it.
+// The shape changes the tiling only: gradients are the same bit for bit.
+std::vector<std::array<double, 3>> PhaseField::autotune(const vector<double>& coords, int prec, vector<int> targets,
+                                                         vector<int> marches, int reps) {
+  if (prec != FP64 && prec != FP32) throw std::invalid_argument("PhaseField::autotune: precision must be FP64 or FP32.");
+  if (reps < 1 || targets.empty() || marches.empty()) throw std::invalid_argument("PhaseField::autotune: nothing to time");
+  const PFArgs& base = args_of(args_);
+  PFHolder* h = static_cast<PFHolder*>(args_.get());
+  const size_t n = (size_t)nGrid * nFluid;
+  if (coords.size() != n) throw std::invalid_argument("PhaseField::autotune: coordinates have the wrong size.");
+  hipStream_t s = nullptr;
+  const size_t bytes = n * minim_amd::prec_bytes(prec);
+  minim_amd::DeviceBuffer xb(bytes), gb(bytes);
+  minim_amd::upload_raw(coords.data(), n, prec, xb.get(), s);
+  auto& ws = minim_amd::Workspace::current();
+  hipEvent_t e0, e1;
+  MINIM_HIP_CHECK(hipEventCreate(&e0));
+  MINIM_HIP_CHECK(hipEventCreate(&e1));
+  std::vector<std::array<double, 3>> rows;
+  double bestMs = 0;
+  int bestT = 0, bestM = 0;
+  for (int t : targets) {
+    for (int mch : marches) {
+      PFArgs c = base;
+      c.xchunk_override = mch;
+      try {
+        minim_amd::pf_args_finalize(c, t);
+        for (int w = 0; w < 2; w++)
+          minim_amd::pf_launch_eval(c, prec, prec, 3, xb.get(), nullptr, 0.0, gb.get(), ws.rs(), s, false);
+      } catch (const std::invalid_argument&) {
+        continue;  // a shape this grid cannot take
+      }
+      MINIM_HIP_CHECK(hipEventRecord(e0, s));
+      for (int r = 0; r < reps; r++)
+        minim_amd::pf_launch_eval(c, prec, prec, 3, xb.get(), nullptr, 0.0, gb.get(), ws.rs(), s, false);
+      MINIM_HIP_CHECK(hipEventRecord(e1, s));
+      MINIM_HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0;
+      MINIM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      const double avg = (double)ms / reps;
+      rows.push_back({(double)t, (double)mch, avg});
+      if (bestT == 0 || avg < bestMs) {
+        bestMs = avg;
+        bestT = t;
+        bestM = mch;
+      }
+    }
+  }
+  MINIM_HIP_CHECK(hipEventDestroy(e0));
+  MINIM_HIP_CHECK(hipEventDestroy(e1));
+  if (bestT == 0) throw std::invalid_argument("PhaseField::autotune: no candidate shape fits this grid");
+  apply_shape(h->a, prec, bestT, bestM);
+  minim_amd::tune_store(tune_key_of(h->a, prec), {bestT, bestM});
+  return rows;
+}
+
+std::pair<int, int> PhaseField::tunedShape(int prec) const {
+  const PFArgs& a = args_of(args_);
+  if (prec != FP64 && prec != FP32) throw std::invalid_argument("PhaseField: precision must be FP64 or FP32.");
+  return {a.tuned_target[prec], a.tuned_xchunk[prec]};
 }
 
 bool PhaseField::supportsMixed() const {

@@ -12,6 +12,8 @@
 //   * Gram pass: y_new = g_new - g_old fused with every dot of g_new and y_new
 //     against the stored basis.
 // All accumulations are FP64; vectors are FP64 or FP32.
+#include <algorithm>
+
 #include "common.hpp"
 #include "vec.hpp"
 
@@ -393,7 +395,7 @@ void dir_nf(const DirArgs& a, RedScratch rs, hipStream_t s, int nb) {
 template <typename T>
 void dir_t(const DirArgs& a, RedScratch rs, hipStream_t s) {
   const int nf = a.proj.nf < 1 ? 1 : a.proj.nf;
-  const int nb = grid_blocks(a.n / nf, 2, 2048);
+  const int nb = a.blocks > 0 ? std::min(a.blocks, kMaxRedBlocks) : grid_blocks(a.n / nf, 2, 2048);
   if (a.pairs >= 0 && a.pairs <= kFastPairs) {
     if (nf == 1 && dir_fast_nf<T, 1>(a, rs, s, nb)) return;
     if (nf == 3 && dir_fast_nf<T, 3>(a, rs, s, nb)) return;
@@ -411,7 +413,7 @@ template <typename T>
 void gram_t(const GramArgs& a, RedScratch rs, hipStream_t s) {
   // 1024 blocks (4 per CU): the fused y/Gram pass streams 2P+3 vectors at once
   // and reaches a higher rate with fewer, longer-running blocks than with 2048
-  const int nb = grid_blocks(a.n, 4, 1024);
+  const int nb = a.blocks > 0 ? std::min(a.blocks, kMaxRedBlocks) : grid_blocks(a.n, 4, 1024);
   if (a.ny <= kFastPairs && gram_fast<T>(a, rs, s, nb)) return;
   const int KB = gram_bucket(a.nd > a.ny ? a.nd : a.ny);
   switch (KB) {

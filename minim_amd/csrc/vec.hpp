@@ -35,6 +35,7 @@ struct DirArgs {
   // basis = [g, d_0..d_{P-1}, y_0..y_{P-1}], dotv = [g, y_0..y_{P-1}]; selects
   // kernels whose loads are all unconditional (compile-time P).
   int pairs = -1;
+  int blocks = 0;  // launch size (an autotuned shape); 0 = the heuristic
 };
 constexpr int kFastPairs = 10;  // largest P with a compile-time kernel
 void launch_direction(const DirArgs& a, int prec, RedScratch rs, hipStream_t s);
@@ -51,6 +52,7 @@ struct GramArgs {
   const void* Y[kMaxBasis];
   int ny;
   size_t n;
+  int blocks = 0;  // launch size (an autotuned shape); 0 = the heuristic
 };
 int gram_bucket(int k);
 // KB of the result layout above for a launch with these counts

@@ -52,7 +52,13 @@ int main(int argc, char** argv) {
   }
   EXPECT(threw);
   if (gpu) {
+    // the library's autotuner picks the stencil's launch shape for this field
+    // (FP64); the State's clone of the potential launches with it
+    pf.autotune(x, minim::FP64, {256, 512}, {0, 8}, 3);
+    const std::pair<int, int> tuned = pf.tunedShape(minim::FP64);
+    EXPECT(tuned.first == 256 || tuned.first == 512);
     minim::State state(pf, x);
+    EXPECT(static_cast<const minim::PhaseField&>(*state.pot).tunedShape(minim::FP64) == tuned);
     minim::Lbfgs lbfgs;
     lbfgs.setM(5).setMaxIter(100);
     std::vector<double> r = lbfgs.minimise(state);

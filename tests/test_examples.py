@@ -85,6 +85,18 @@ def test_reference_binding_builds():
                        text=True, timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
     assert os.access(BINDING, os.X_OK)
+    # Both libraries define namespace minim; libminim_amd.so binds its own
+    # references to its own definitions (-Bsymbolic), so the reference's State
+    # and Potential are not interposed into it: without a GPU the program runs
+    # up to its first device allocation and stops there with an exception
+    # (with interposition it crashed in the State constructor).
+    import torch
+
+    if not torch.cuda.is_available():
+        r = subprocess.run([BINDING], capture_output=True, text=True, timeout=120,
+                           env=dict(os.environ, OMP_NUM_THREADS="1"))
+        assert r.returncode != -11, "segmentation fault"
+        assert "hipMalloc failed" in r.stderr, (r.returncode, r.stdout, r.stderr)
 
 
 def close(a, b, rel, ab=0.0):

@@ -322,24 +322,72 @@ def test_full_size_256cubed_ternary(gpu):
 
 
 def test_autotune_keeps_the_fastest_shape_and_the_same_gradient(gpu):
-    """PhaseField.autotune times each launch shape of the stencil and keeps the
-    fastest; every shape gives the same gradient bit for bit and the same
-    energy to the order of the block sums."""
+    """The library's launch-shape autotuner (minim_pf_autotune): it keeps the
+    fastest shape per precision, a new potential of the same shape picks it up
+    from the process-wide cache at init, and every shape gives the same
+    gradient bit for bit and the same energy to the order of the block sums."""
     m = gpu
+    m.autotune_clear()
     rng = np.random.default_rng(21)
     grid = (40, 36, 44)
     u = rng.uniform(0, 1, (int(np.prod(grid)), 3))
     x = (u / u.sum(axis=1, keepdims=True)).reshape(-1)
     pf = m.PhaseField().setNFluid(3).setGridSize(list(grid)).setSurfaceTension([1.0, 0.8, 1.2]).setInterfaceSize(2.0)
+    assert pf.tunedShape("fp64", x) == (0, 0)
     times = pf.autotune(x, "fp64", blocks=(64, 256, 1024), marches=(0, 4, 8))
     assert len(times) == 9 and all(t > 0 for t in times.values())
-    assert times[(pf.stencilTargetBlocks, pf.stencilXChunk)] == min(times.values())
+    best = min(times, key=times.get)
+    assert pf.tunedShape("fp64", x) == best  # a fresh handle: from the cache
+    assert pf.tunedShape("fp32", x) == (0, 0)  # per precision
     ref = None
-    for (b, mch) in [(64, 4), (1024, 0), (pf.stencilTargetBlocks, pf.stencilXChunk)]:
-        pf.stencilTargetBlocks, pf.stencilXChunk = b, mch
+    for shape in [(64, 4), (1024, 0), None]:
+        if shape is None:  # the tuned shape through the cache
+            pf.stencilTargetBlocks, pf.stencilXChunk = 0, 0
+        else:
+            pf.stencilTargetBlocks, pf.stencilXChunk = shape
         e, g = m.State(pf, x).energyGradient()
         if ref is None:
             ref = (e, g)
         else:
-            assert np.array_equal(g, ref[1]), (b, mch)
+            assert np.array_equal(g, ref[1]), shape
             assert abs(e - ref[0]) <= 1e-12 * abs(ref[0])
+    m.autotune_clear()
+
+
+def test_autotune_general_kernel_and_vector_passes(gpu):
+    """The autotuner on the general (solid-aware) kernel, whose gradient is the
+    same for every shape, and on the L-BFGS direction and Gram passes, after
+    which a minimisation converges to the same energy."""
+    m = gpu
+    m.autotune_clear()
+    rng = np.random.default_rng(22)
+    grid = (24, 20, 28)
+    ng = int(np.prod(grid))
+    u = rng.uniform(0, 1, (ng, 3))
+    x = (u / u.sum(axis=1, keepdims=True)).reshape(-1)
+    solid = np.zeros(grid, np.uint8)
+    solid[:, :, 0] = 1
+    pf = (m.PhaseField().setNFluid(3).setGridSize(list(grid)).setSurfaceTension([1.0, 0.8, 1.2])
+          .setInterfaceSize(2.0).setSolid(solid.reshape(-1).tolist()).setPressure([0.0, 0.1, 0.0]))
+    g_before = m.State(pf, x, "fp32").energyGradient()[1]
+    times = pf.autotune(x, "fp32", blocks=(128, 512), marches=(0, 8))
+    assert len(times) == 4
+    assert pf.tunedShape("fp32", x) == min(times, key=times.get)
+    g_after = m.State(pf, x, "fp32").energyGradient()[1]
+    assert np.array_equal(g_before, g_after)
+    # the vector passes of L-BFGS
+    plain = m.PhaseField().setNFluid(3).setGridSize(list(grid)).setSurfaceTension([1.0, 0.8, 1.2]).setInterfaceSize(2.0)
+    s = m.State(plain, x)
+    m.Lbfgs().setM(5).setMaxIter(3000).minimise(s)
+    e_ref = s.energy()
+    lb = m.Lbfgs().setM(5).setMaxIter(3000)
+    s2 = m.State(plain, x)
+    r = lb.autotune(s2, blocks=(128, 1024), reps=3)
+    assert set(r["direction"]) == {128, 1024} and set(r["gram"]) == {128, 1024}
+    assert r["best"][0] == min(r["direction"], key=r["direction"].get)
+    assert r["best"][1] == min(r["gram"], key=r["gram"].get)
+    lb.minimise(s2)
+    assert lb.stats["converged"]
+    e0 = m.State(plain, x).energy()
+    assert abs(s2.energy() - e_ref) <= 1e-8 * max(abs(e_ref), abs(e0))
+    m.autotune_clear()
