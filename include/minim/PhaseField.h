@@ -91,6 +91,12 @@ class PhaseField : public NewPotential<PhaseField> {
   double timeKernel(int prec, const void* x, bool wantE, void* g, int reps, void* stream) const override;
   void timeSteps(int prec, const void* x, void* g, double* e_dev, int steps, double warmMs, void* stream,
                  double* out) const override;
+  // evaluateAsync with the cross-block reduction in device scratch the caller
+  // provides (partial sums in scratch[0, nPartials), 96 results after them);
+  // no device memory is allocated.  Throws std::invalid_argument when the
+  // potential's tables are not resident yet or the scratch is too small.
+  void evaluateAsyncScratch(int prec, const void* x, const void* d, double t, bool wantE, double* e_dev, void* g,
+                            double* scratch, size_t nPartials, void* stream) const;
 
   // Launch-shape autotuner of the evaluation kernel this field uses (after
   // init): times one energy+gradient launch (HIP events, kernel only, the

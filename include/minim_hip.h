@@ -194,6 +194,22 @@ int minim_potential_eval(const minim_potential* pot, int precision, const void* 
  * want_energy) is written to the device double e_dev, in stream order. */
 int minim_potential_eval_async(const minim_potential* pot, int precision, const void* x, size_t ndof,
                                const void* d, double t, int want_energy, double* e_dev, void* g, void* stream);
+/* The allocation-free form of minim_potential_eval_async.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  PhaseField only (MINIM_EINVAL otherwise).  The cross-block
+ * reduction runs in a fixed device array of this library, 32768 partial sums
+ * plus 96 results (263 KB per GPU, held until the process ends), instead of
+ * the per-thread workspace; every PhaseField evaluation fits in it (at most
+ * 8192 blocks x 2 values, or 16 fluid sums x 2048 blocks for fixed volumes).
+ * The potential's device tables must already be resident: evaluate it once
+ * through an allocating entry point first, else MINIM_EINVAL.  Energy, when
+ * want_energy, goes to the device double e_dev in stream order.  Every call
+ * on one GPU shares that array: calls that overlap on one GPU (several
+ * streams or host threads) race; serialise them. */
+int minim_potential_eval_async_noalloc(const minim_potential* pot, int precision, const void* x, size_t ndof,
+                                       const void* d, double t, int want_energy, double* e_dev, void* g,
+                                       void* stream);
 /* As minim_potential_eval with FP64 coordinates x and FP64 arithmetic, but the
  * direction d and the gradient g in FP32 (the second phase of the mixed L-BFGS
  * schedule).  MINIM_ERR_INVALID for potentials without mixed support

@@ -78,6 +78,8 @@ SIGNATURES = {
     "minim_potential_eval_mixed": (C.c_int, [vp, vp, C.c_size_t, vp, C.c_double, dp, vp, vp]),
     "minim_potential_eval_step": (C.c_int, [vp, C.c_int, C.c_int, vp, C.c_size_t, vp, C.c_double, vp, dp, vp, vp]),
     "minim_potential_time_kernel": (C.c_int, [vp, C.c_int, vp, C.c_size_t, C.c_int, vp, C.c_int, vp, dp]),
+    "minim_potential_eval_async_noalloc": (C.c_int, [vp, C.c_int, vp, C.c_size_t, vp, C.c_double, C.c_int, vp, vp,
+                                                     vp]),
     "minim_pf_autotune": (C.c_int, [vp, C.c_int, dp, C.c_size_t, ip, C.c_int, ip, C.c_int, C.c_int, dp, ip, ip]),
     "minim_pf_tuned_shape": (C.c_int, [vp, C.c_int, ip, ip]),
     "minim_lbfgs_autotune": (C.c_int, [vp, vp, ip, C.c_int, C.c_int, dp, ip, ip]),

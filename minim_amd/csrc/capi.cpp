@@ -380,6 +380,24 @@ int minim_potential_eval_async(const minim_potential* pot, int precision, const 
   });
 }
 
+int minim_potential_eval_async_noalloc(const minim_potential* pot, int precision, const void* x, size_t ndof,
+                                       const void* d, double t, int want_energy, double* e_dev, void* g,
+                                       void* stream) {
+  return guard([&] {
+    need(pot, "pot");
+    need(x, "x");
+    check_prec(precision);
+    if (!pot->initialised) throw std::runtime_error("potential not initialised: call minim_potential_init");
+    if (ndof != pot->ndof) throw std::invalid_argument("ndof does not match the initialised potential");
+    if (d && g) throw std::invalid_argument("a gradient at a trial point (d != NULL) is not supported");
+    auto* pf = dynamic_cast<const minim::PhaseField*>(pot->p.get());
+    if (!pf) throw std::invalid_argument("the allocation-free evaluation supports PhaseField only");
+    size_t nPartials = 0;
+    double* scratch = minim_amd::noalloc_scratch(&nPartials);
+    pf->evaluateAsyncScratch(precision, x, d, t, want_energy != 0, e_dev, g, scratch, nPartials, stream);
+  });
+}
+
 int minim_pf_autotune(minim_potential* pot, int precision, const double* coords, size_t ndof, const int* targets,
                       int n_targets, const int* marches, int n_marches, int reps, double* times, int* best_target,
                       int* best_march) {

@@ -7,6 +7,7 @@
 // pf_ext.hip when solid nodes or any of the terms beyond bulk, gradient and
 // density are set.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -39,6 +40,7 @@ struct PFHolder {
   std::vector<int8_t> csign_h;
   minim_amd::DeviceBuffer ext, code, wcode, wet, csign, bs, fs;
   std::once_flag uploaded;
+  std::atomic<bool> resident{false};  // the tables above are in device memory
 };
 
 template <typename T>
@@ -63,6 +65,7 @@ const PFArgs& args_of(const std::shared_ptr<void>& p) {
     h->a.csign = to_device(h->csign, h->csign_h);
     h->a.Bs = to_device(h->bs, h->bs_h);
     h->a.Fs = to_device(h->fs, h->fs_h);
+    h->resident = true;
   });
   return h->a;
 }
@@ -760,6 +763,33 @@ void PhaseField::timeSteps(int prec, const void* x, void* g, double* e_dev, int 
   out[2] = k.front();
   out[3] = k.back();
   out[4] = sum / steps;
+}
+
+// evaluateAsync with the cross-block reduction in caller-provided device
+// scratch: partials in scratch[0, nPartials), results in scratch[nPartials,
+// nPartials + 96).  Allocates nothing: the potential's tables must already be
+// resident (any earlier evaluation uploads them).
+void PhaseField::evaluateAsyncScratch(int prec, const void* x, const void* d, double t, bool wantE, double* e_dev,
+                                      void* g, double* scratch, size_t nPartials, void* stream) const {
+  if (!args_) throw std::runtime_error("PhaseField: not initialised (construct a State first).");
+  const PFHolder* h = static_cast<const PFHolder*>(args_.get());
+  if (!h->resident)
+    throw std::invalid_argument("PhaseField: device tables not resident yet; evaluate once through an allocating "
+                                "entry point first");
+  const PFArgs& a = h->a;
+  // the widest reductions: the evaluation (2 values per block) and the fixed
+  // volume projection (up to 16 fluid sums over at most 2048 blocks)
+  const size_t need = std::max<size_t>(2 * (size_t)a.nblocks, a.volfix ? (size_t)minim_amd::kPFMaxFluid * 2048 : 0);
+  if (need > nPartials)
+    throw std::invalid_argument("PhaseField: the reduction needs " + std::to_string(need) +
+                                " partial sums, more than the scratch holds (" + std::to_string(nPartials) + ")");
+  const int mode = (wantE ? 1 : 0) | (g ? 2 : 0) | (d ? 4 : 0);
+  if (mode == 0) return;
+  if ((mode & 4) && (mode & 2)) throw std::invalid_argument("PhaseField: gradients at a trial point are not supported.");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  minim_amd::RedScratch rs{scratch, scratch + nPartials};
+  minim_amd::pf_launch_eval(a, prec, prec, mode, x, d, t, g, rs, s);
+  if (wantE && e_dev) MINIM_HIP_CHECK(hipMemcpyAsync(e_dev, rs.result, sizeof(double), hipMemcpyDeviceToDevice, s));
 }
 
 void PhaseField::evaluateMixed(const void* x, const void* d, double t, double* e, void* g, void* stream) const {

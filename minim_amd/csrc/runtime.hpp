@@ -92,6 +92,11 @@ class Staging {
   DeviceBuffer dev_[4];
 };
 
+// The allocation-free entry point's reduction scratch on the current device
+// (a file-scope device array of noalloc.hip): *nPartials partial sums, then
+// kMaxDots result slots.
+double* noalloc_scratch(size_t* nPartials);
+
 // Host <-> device copies of FP64 host data into a vector of any precision.
 void upload(const double* host, size_t n, DVec& dst, hipStream_t s);
 void download(const DVec& src, double* host, hipStream_t s);
