@@ -474,7 +474,7 @@ def per_state_leg(m, n_inst, workers, sides, inputs, make_pf):
                 iterations_total=int(sum(r["iterations"] for r in res)))
 
 
-def batch_leg(m, n_inst=512, workers=8, seed=4040, cpu_budget=10.0, per_state=True):
+def batch_leg(m, n_inst=512, workers=8, seed=4040, cpu_budget=10.0, per_state=True, budgets=(1000000, 2000000, 4000000)):
     """Many independent fields: n_inst 2-D binary PhaseField instances with side
     s_j = round(exp(U[ln 32, ln 512])), phi ~ U[-0.1, 0.1] and interface width
     w_j ~ U[1, 3] (all seeded), each minimised by L-BFGS m=5 to the reference
@@ -485,7 +485,10 @@ def batch_leg(m, n_inst=512, workers=8, seed=4040, cpu_budget=10.0, per_state=Tr
     with its own HIP stream (ordered largest first).  Reports the aggregate
     evaluation rate and the p50/p99 per-instance time-to-converge, and the
     reference library on a bounded sample of the same instances.  per_state
-    False leaves out the one-State-per-call comparison."""
+    False leaves out the one-State-per-call comparison.  budgets: the batch
+    also runs with each of these limits on the sites of running instances
+    (Lbfgs.setBatchBudget: smallest first until half have finished, then
+    every remaining instance)."""
 
     import numpy as np
     import torch
@@ -507,12 +510,14 @@ def batch_leg(m, n_inst=512, workers=8, seed=4040, cpu_budget=10.0, per_state=Tr
     # decisions in kernels, the iteration replayed as a HIP graph) and, for
     # comparison, by the host (one host synchronisation per phase)
     batched = {}
-    for prec, control in (("fp64", "graph"), ("mixed", "graph"), ("fp64", "host"), ("mixed", "host")):
+    runs = [(p, "graph", 0) for p in ("fp64", "mixed")] + [(p, "graph", b) for b in budgets for p in ("fp64", "mixed")]
+    runs += [(p, "host", 0) for p in ("fp64", "mixed")]
+    for prec, control, budget in runs:
         t = time.perf_counter()
         bstates = [m.State(make_pf(j), inputs(j), "fp64") for j in range(n_inst)]
         torch.cuda.synchronize()
         bsetup = time.perf_counter() - t
-        lb = m.Lbfgs().setM(5).setMaxIter(20000).setPrecision(prec).setBatchControl(control)
+        lb = m.Lbfgs().setM(5).setMaxIter(20000).setPrecision(prec).setBatchControl(control).setBatchBudget(budget)
         t = time.perf_counter()
         bres = lb.minimiseBatch(bstates)
         torch.cuda.synchronize()
@@ -520,21 +525,25 @@ def batch_leg(m, n_inst=512, workers=8, seed=4040, cpu_budget=10.0, per_state=Tr
         rep = lb.batchReport()
         bsecs = np.array([r["seconds"] for r in bres])
         bevals = sum(r["n_energy"] + r["n_gradient"] for r in bres)
-        batched[prec if control == "graph" else prec + "_host_control"] = dict(
-                             control=control, host_syncs=rep["host_syncs"], replays=rep["replays"],
+        key = prec + ("" if control == "graph" else "_host_control") + (
+            "_budget_%gM_sites" % (budget / 1e6) if budget else "")
+        batched[key] = dict(control=control, budget_sites=budget, host_syncs=rep["host_syncs"], replays=rep["replays"],
                              host_syncs_per_iteration=rep["host_syncs"] / max(1, rep["replays"]),
-                             graph_launches=rep["graph_launches"],
+                             graph_launches=rep["graph_launches"], graph_captures=rep["captures"],
                              setup_s=bsetup, wall_s=bwall, evals=bevals, evals_per_s=bevals / bwall,
                              instances_per_s=n_inst / bwall, converged=int(sum(r["converged"] for r in bres)),
                              p50_s=float(np.percentile(bsecs, 50)), p99_s=float(np.percentile(bsecs, 99)),
                              iterations_total=int(sum(r["iterations"] for r in bres)),
                              iterations_max=int(max(r["iterations"] for r in bres)),
-                             iterations_fp32=int(sum(r["iter_fp32"] for r in bres)))
+                             iterations_fp32=int(sum(r["iter_fp32"] for r in bres)),
+                             longest=[dict(side=int(sides[j]), iterations=bres[j]["iterations"],
+                                           seconds=bres[j]["seconds"])
+                                      for j in sorted(range(n_inst), key=lambda j: -bres[j]["iterations"])[:6]])
         del bstates
 
     out = dict(workload="%d 2-D binary PhaseField instances, sides %d..%d, %d sites in total; L-BFGS m=5 each"
                         % (n_inst, sides.min(), sides.max(), int((sides ** 2).sum())),
-               site_bytes_per_iteration="FP64: 8 B x (4m + 14) vector streams per site; mixed: FP32 vectors",
+               site_bytes_per_iteration="FP64: 8 B x (4m + 10) vector streams per site; mixed: FP32 vectors",
                batched=batched)
     if per_state:
         out["per_state_threads"] = per_state_leg(m, n_inst, workers, sides, inputs, make_pf)

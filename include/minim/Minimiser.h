@@ -127,12 +127,20 @@ class Lbfgs : public NewMinimiser<Lbfgs> {
   // the host, one host synchronisation per phase.  All three give the same
   // results bit for bit.
   Lbfgs& setBatchControl(std::string mode);
+  // Under device control ("graph" or "device"): instances are admitted
+  // smallest first while the running ones hold at most this many elements
+  // (sites), until half of the instances have finished; then every remaining
+  // one.  Small instances finish sooner, at some cost in total time.  0 (the
+  // default): every instance from the start.  An instance's result does not
+  // depend on it.
+  Lbfgs& setBatchBudget(long sites);
   struct BatchReport {
     std::string control;
     int iterations = 0;       // the most iterations any instance ran
     long replays = 0;         // iterations executed by the batch
     long hostSyncs = 0;       // host synchronisations waiting for results
     long graphLaunches = 0;   // HIP graph launches
+    long captures = 0;        // graph captures (one more each time the grid shrank)
   };
   BatchReport batchReport;  // of the last minimiseBatch
 
@@ -143,6 +151,7 @@ class Lbfgs : public NewMinimiser<Lbfgs> {
   std::string _precMode = "state";
   double _switchRms = 0;
   std::string _batchControl = "graph";
+  long _batchBudget = 0;
   struct Impl;
   std::unique_ptr<Impl> impl_;
 };

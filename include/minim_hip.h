@@ -316,6 +316,7 @@ typedef struct minim_minimiser_params {
   int max_force_dim; /* Fire::setMaxForceConvergence: converge on the largest norm over groups of
                         this many DOFs (0 = RMS gradient, the reference) */
   int batch_control; /* Lbfgs::setBatchControl for minim_minimise_batch: 0 graph, 1 device, 2 host */
+  long batch_budget; /* Lbfgs::setBatchBudget: sites of running instances under device control (0 = all) */
 } minim_minimiser_params;
 
 typedef struct minim_result {
@@ -358,9 +359,11 @@ int minim_minimise_log(minim_minimiser* m, minim_state* s, const char* log_type,
 int minim_minimise_batch(minim_minimiser* m, minim_state* const* states, int n_states, minim_result* results);
 /* How the last minim_minimise_batch ran (no reference counterpart): the most
  * iterations of any instance, the iterations executed, the host
- * synchronisations that waited for results and the HIP graph launches. */
+ * synchronisations that waited for results, the HIP graph launches and the
+ * graph captures (one more each time the launch grid shrank to the instances
+ * still running). */
 int minim_batch_report(const minim_minimiser* m, int* iterations, long* replays, long* host_syncs,
-                       long* graph_launches);
+                       long* graph_launches, long* captures);
 
 #ifdef __cplusplus
 }

@@ -779,6 +779,7 @@ class Lbfgs(Minimiser):
         self.precisionMode = "state"
         self.switchRms = 0.0
         self.batchControl = "graph"
+        self.batchBudget = 0
 
     def setBatchControl(self, mode):
         """Lbfgs::setBatchControl: "graph" (device decisions, HIP-graph-captured
@@ -789,14 +790,26 @@ class Lbfgs(Minimiser):
         self._h = None
         return self
 
+    def setBatchBudget(self, sites):
+        """Lbfgs::setBatchBudget: under device control, instances are admitted
+        smallest first while the running ones hold at most this many sites,
+        until half have finished (0: every instance from the start)."""
+        if sites < 0:
+            raise ValueError("Lbfgs: the batch budget must be non-negative.")
+        self.batchBudget = int(sites)
+        self._h = None
+        return self
+
     def batchReport(self):
         """How the last minimiseBatch ran: iterations, replays, host_syncs,
-        graph_launches."""
+        graph_launches, captures (graph captures: one more each time the
+        launch grid shrank to the instances still running)."""
         it = C.c_int()
-        rp, hs, gl = C.c_long(), C.c_long(), C.c_long()
-        A.check(A.load().minim_batch_report(self._handle().ptr, C.byref(it), C.byref(rp), C.byref(hs), C.byref(gl)))
+        rp, hs, gl, cp = C.c_long(), C.c_long(), C.c_long(), C.c_long()
+        A.check(A.load().minim_batch_report(self._handle().ptr, C.byref(it), C.byref(rp), C.byref(hs), C.byref(gl),
+                                            C.byref(cp)))
         return dict(control=self.batchControl, iterations=it.value, replays=rp.value, host_syncs=hs.value,
-                    graph_launches=gl.value)
+                    graph_launches=gl.value, captures=cp.value)
 
     def setM(self, m):
         self.m = int(m)
@@ -864,6 +877,7 @@ class Lbfgs(Minimiser):
         p.precision_mode = {"state": 0, "fp64": 1, "fp32": 2, "mixed": 3}[self.precisionMode]
         p.switch_rms = self.switchRms
         p.batch_control = {"graph": 0, "device": 1, "host": 2}[self.batchControl]
+        p.batch_budget = self.batchBudget
         return p
 
 

@@ -26,9 +26,11 @@
 // synchronisation per phase serves all instances.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <climits>
 #include <cmath>
+#include <cstring>
 #include <deque>
 #include <stdexcept>
 #include <vector>
@@ -47,7 +49,9 @@ constexpr int kBE = kBlock * kBR;  // elements per block
 constexpr int kKP = 2;             // stencil partials: energy, g.g
 constexpr int kNT = 4;             // line-search trials evaluated per pass: tau, tau/2, tau/4, tau/8
 constexpr int kKD = 2 + kBS;       // direction: d.d, d.g, d.Y_s
-constexpr int kKG = 4 * kBS + 3;   // Gram (layout in bgram_kernel)
+constexpr int kKG = 4 * kBS + 3;   // Gram (layout in bstep_body)
+constexpr int kKS = 1 + kKG;       // step pass: energy, then the Gram block
+constexpr int kKF = kKP + 2 * kBS;  // fresh gradient: energy, g.g, g.D_i, g.Y_i
 
 struct BInst {
   size_t off;  // first element in the concatenated vectors
@@ -85,27 +89,30 @@ struct BDyn {
   double tau;      // trial or step factor
   double cg;       // coefficient of g in the direction
   double cD[kBS], cY[kBS];  // coefficients of pair i's D and Y
+  int xcur;        // coordinate buffer (X0 or X1) that holds x
   unsigned act;    // phases this instance takes part in (kAct*; device control)
 };
 
 // Phases of an iteration under device control: the recomputed gradient after
-// the switch to FP64 coordinates, the direction / step / gradient / Gram
-// passes, and the further line-search trials.
-constexpr unsigned kActFresh = 1, kActMain = 2, kActSearch = 4;
+// the switch to FP64 coordinates, the direction pass with the speculative
+// trials, a pass of further line-search trials, and the step pass.
+constexpr unsigned kActFresh = 1, kActDir = 2, kActSearch = 4, kActStep = 8;
 
 // Vectors G, D and Y are stored in TV (double, or float for the mixed
 // schedule); coordinates always in FP64 (holding FP32 values while an
 // instance runs in FP32).
 struct BBufs {
-  double* X;
+  double* X0;  // coordinates; the step pass writes x + tau d to the other buffer
+  double* X1;
   void* G0;
   void* G1;
   void* D;  // m + 1 slots of N elements
   void* Y;
   size_t N;
   const BInst* inst;
-  const BDyn* dyn;
-  const int* ablk;  // na + 1 prefix sums of the block counts
+  const BDyn* dyn;   // per position, or per instance when map is set
+  const int* map;    // position -> instance (device control), or null
+  const int* ablk;   // na + 1 prefix sums of the block counts
   int na;
   double* part;  // per-block partials, K per block
   unsigned phase;  // nonzero: only instances whose act has this bit take part
@@ -151,33 +158,46 @@ __device__ __forceinline__ void block_partials(const double (&v)[K], unsigned lo
   }
 }
 
-// res[p * K + k] = sum of instance p's block rows: lane l of a wavefront adds
-// rows l, l + 64, ..., then the 64 lane sums are added in a fixed tree, so the
-// result depends only on that instance's rows.
-__global__ __launch_bounds__(kBlock) void bfinal_kernel(const double* __restrict__ part,
-                                                        const int* __restrict__ ablk, int K,
-                                                        double* __restrict__ res, const BDyn* __restrict__ dyn,
-                                                        unsigned phase) {
-  const int p = blockIdx.x, lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
-  if (phase && !(dyn[p].act & phase)) return;
-  const int b0 = ablk[p], nb = ablk[p + 1] - b0;
+// out[k] = sum of the block rows [b0, b0 + nb) of part (K per row), by the
+// whole block: lane l of a wavefront adds rows l, l + 64, ..., then the 64
+// lane sums are added in a fixed tree, so the result depends only on those
+// rows, wherever they lie in part.
+__device__ __forceinline__ void reduce_rows(const double* __restrict__ part, int b0, int nb, int K,
+                                            double* __restrict__ out) {
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
   for (int k = w; k < K; k += kBlock / kWave) {
     double s = 0;
     for (int b = lane; b < nb; b += kWave) s += part[(size_t)(b0 + b) * K + k];
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kWave);
-    if (lane == 0) res[(size_t)p * K + k] = s;
+    if (lane == 0) out[k] = s;
   }
+}
+
+// res[j * K + k]: the sums of the instance at position p (j = p, or map[p]
+// under device control).
+__global__ __launch_bounds__(kBlock) void bfinal_kernel(const double* __restrict__ part,
+                                                        const int* __restrict__ ablk, const int* __restrict__ map,
+                                                        int K, double* __restrict__ res,
+                                                        const BDyn* __restrict__ dyn, unsigned phase) {
+  const int p = blockIdx.x;
+  const int j = map ? map[p] : p;
+  if (phase && !(dyn[j].act & phase)) return;
+  reduce_rows(part, ablk[p], ablk[p + 1] - ablk[p], K, res + (size_t)j * K);
 }
 
 // Energy (forward bonds) and gradient of one-fluid sites, the arithmetic of the
 // tiled kernel for one fluid (PhaseField.cpp:238-288, :372-378) in T (FP32
 // for an instance in the first phase of the mixed schedule), vectors in TV.
-// MODE bits: 1 energy, 2 gradient into G[gout] with g.g, 4 at x + tau * d.
+// MODE bits: 1 energy, 2 gradient into G[gout] with g.g, 4 at x + tau * d,
+// 8 also the dots of the gradient with every stored pair, g.D_i and g.Y_i
+// (partials [2, 2 + kBS) and [2 + kBS, 2 + 2 kBS)).
 template <int MODE, typename T, typename TV>
 __device__ __forceinline__ void bpf_body(const BBufs& B, const BDyn& q, const BInst& I, int lb) {
   using C = BC<T>;
-  const double* __restrict__ x = B.X + I.off;
+  constexpr int K = (MODE & 8) ? kKF : kKP;
+  const size_t N = B.N;
+  const double* __restrict__ x = (q.xcur ? B.X1 : B.X0) + I.off;
   const TV* __restrict__ d = static_cast<const TV*>(B.D) + (size_t)q.dslot * B.N + I.off;
   TV* __restrict__ g = static_cast<TV*>(q.gout ? B.G1 : B.G0) + I.off;
   const T tau = (T)q.tau;
@@ -190,7 +210,9 @@ __device__ __forceinline__ void bpf_body(const BBufs& B, const BDyn& q, const BI
     }
     return xv;
   };
-  double acc[kKP] = {0, 0};
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) acc[k] = 0;
 #pragma unroll
   for (int r = 0; r < kBR; r++) {
     const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
@@ -221,15 +243,31 @@ __device__ __forceinline__ void bpf_body(const BBufs& B, const BDyn& q, const BI
       const TV gv = (TV)fma(C::fg2(I), lap, bg);
       g[i] = gv;
       acc[1] += (double)gv * (double)gv;
+      if (MODE & 8) {
+        const double a = (double)gv;
+        const TV* __restrict__ Dm = static_cast<const TV*>(B.D) + I.off;
+        const TV* __restrict__ Ym = static_cast<const TV*>(B.Y) + I.off;
+#pragma unroll
+        for (int k = 0; k < kBS; k++)
+          if (k < q.np) {
+            acc[kKP + k] += a * (double)Dm[(size_t)q.slot[k] * N + i];
+            acc[kKP + kBS + k] += a * (double)Ym[(size_t)q.slot[k] * N + i];
+          }
+      }
     }
   }
-  block_partials<kKP>(acc, (MODE & 1 ? 1ull : 0ull) | (MODE & 2 ? 2ull : 0ull), B.part);
+  unsigned long long on = (MODE & 1 ? 1ull : 0ull) | (MODE & 2 ? 2ull : 0ull);
+  if (MODE & 8) {
+    const unsigned long long pm = (1ull << q.np) - 1;
+    on |= (pm << kKP) | (pm << (kKP + kBS));
+  }
+  block_partials<K>(acc, on, B.part);
 }
 
 template <int MODE, typename TV>
 __global__ __launch_bounds__(kBlock) void bpf_kernel(BBufs B) {
   const int p = find_pos(B.ablk, B.na, blockIdx.x);
-  const BDyn& q = B.dyn[p];
+  const BDyn& q = B.dyn[B.map ? B.map[p] : p];
   MINIM_BATCH_SKIP(B, q);
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
@@ -244,7 +282,7 @@ __global__ __launch_bounds__(kBlock) void bpf_kernel(BBufs B) {
 template <typename T, typename TV>
 __device__ __forceinline__ void btrial_body(const BBufs& B, const BDyn& q, const BInst& I, int lb) {
   using C = BC<T>;
-  const double* __restrict__ x = B.X + I.off;
+  const double* __restrict__ x = (q.xcur ? B.X1 : B.X0) + I.off;
   const TV* __restrict__ d = static_cast<const TV*>(B.D) + (size_t)q.dslot * B.N + I.off;
   T tk[kNT];
   tk[0] = (T)q.tau;
@@ -283,7 +321,7 @@ __device__ __forceinline__ void btrial_body(const BBufs& B, const BDyn& q, const
 template <typename TV>
 __global__ __launch_bounds__(kBlock) void btrial_kernel(BBufs B) {
   const int p = find_pos(B.ablk, B.na, blockIdx.x);
-  const BDyn& q = B.dyn[p];
+  const BDyn& q = B.dyn[B.map ? B.map[p] : p];
   MINIM_BATCH_SKIP(B, q);
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
@@ -294,11 +332,11 @@ __global__ __launch_bounds__(kBlock) void btrial_kernel(BBufs B) {
 // x rounded to FP32 (the mixed schedule's first phase keeps FP32 coordinates).
 __global__ __launch_bounds__(kBlock) void bround_kernel(BBufs B) {
   const int p = find_pos(B.ablk, B.na, blockIdx.x);
-  const BDyn& q = B.dyn[p];
+  const BDyn& q = B.dyn[B.map ? B.map[p] : p];
   MINIM_BATCH_SKIP(B, q);
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
-  double* __restrict__ x = B.X + I.off;
+  double* __restrict__ x = (q.xcur ? B.X1 : B.X0) + I.off;
 #pragma unroll
   for (int r = 0; r < kBR; r++) {
     const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
@@ -360,7 +398,7 @@ __device__ __forceinline__ void bdir_body(const BBufs& B, const BDyn& q, const B
 template <typename TV>
 __global__ __launch_bounds__(kBlock) void bdir_kernel(BBufs B) {
   const int p = find_pos(B.ablk, B.na, blockIdx.x);
-  const BDyn& q = B.dyn[p];
+  const BDyn& q = B.dyn[B.map ? B.map[p] : p];
   MINIM_BATCH_SKIP(B, q);
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
@@ -379,68 +417,77 @@ __global__ __launch_bounds__(kBlock) void bdir_kernel(BBufs B) {
   }
 }
 
-// x += tau * d (product rounded first, as the single-State axpy), in FP32 for
-// an instance in the first phase of the mixed schedule.
-template <typename TV>
-__global__ __launch_bounds__(kBlock) void baxpy_kernel(BBufs B) {
-  const int p = find_pos(B.ablk, B.na, blockIdx.x);
-  const BDyn& q = B.dyn[p];
-  MINIM_BATCH_SKIP(B, q);
-  const BInst I = B.inst[q.j];
-  const int lb = blockIdx.x - B.ablk[p];
-  double* __restrict__ x = B.X + I.off;
-  const TV* __restrict__ d = static_cast<const TV*>(B.D) + (size_t)q.dslot * B.N + I.off;
-  if (q.lp) {
-    const float tau = (float)q.tau;
-#pragma unroll
-    for (int r = 0; r < kBR; r++) {
-      const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
-      if (i >= I.n) break;
-      const float st = tau * (float)d[i];
-      x[i] = (double)((float)x[i] + st);
-    }
-  } else {
-    const double tau = q.tau;
-#pragma unroll
-    for (int r = 0; r < kBR; r++) {
-      const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
-      if (i >= I.n) break;
-      const double st = tau * (double)d[i];
-      x[i] = x[i] + st;
-    }
-  }
-}
-
-// y = G[gout] - G[gcur] into Y[dslot]; partials with gn = G[gout], the D list
-// being the P stored pairs then the new direction and the Y list the P pairs:
+// The step, gradient and Gram passes of an iteration in one: x' = x + tau d
+// (product rounded first, as the single-State axpy; in FP32 for an instance in
+// the first phase of the mixed schedule) into the other coordinate buffer; at
+// x' the energy and gradient gn = G[gout] (the stencil of bpf_body, each
+// neighbour's x' formed the same way); y = gn - G[gcur] into Y[dslot]; and
+// the partials, the D list being the P stored pairs then the new direction
+// and the Y list the P pairs:
+//   0 energy ; then from 1:
 //   [0,kBS) gn.D_i ; [kBS,2kBS) gn.Y_i ; 2kBS gn.gn ; 2kBS+1 gn.y ;
 //   [2kBS+2,3kBS+2) y.D_i ; [3kBS+2,4kBS+2) y.Y_i ; 4kBS+2 y.y
-template <int P, typename TV>
-__device__ __forceinline__ void bgram_body(const BBufs& B, const BDyn& q, const BInst& I, int lb) {
+// Every value is the one the separate passes would compute, and each partial
+// sums the same terms in the same order.
+template <int P, typename T, typename TV>
+__device__ __forceinline__ void bstep_body(const BBufs& B, const BDyn& q, const BInst& I, int lb) {
+  using C = BC<T>;
   const size_t N = B.N;
-  const TV* __restrict__ gn = static_cast<const TV*>(q.gout ? B.G1 : B.G0) + I.off;
+  const double* __restrict__ x = (q.xcur ? B.X1 : B.X0) + I.off;
+  double* __restrict__ xn = (q.xcur ? B.X0 : B.X1) + I.off;
+  const TV* __restrict__ d = static_cast<const TV*>(B.D) + (size_t)q.dslot * N + I.off;
+  TV* __restrict__ gn = static_cast<TV*>(q.gout ? B.G1 : B.G0) + I.off;
   const TV* __restrict__ go = static_cast<const TV*>(q.gcur ? B.G1 : B.G0) + I.off;
   TV* __restrict__ yo = static_cast<TV*>(B.Y) + (size_t)q.dslot * N + I.off;
-  const TV* Dp[P + 1];
+  const TV* Dp[P > 0 ? P : 1];
   const TV* Yp[P > 0 ? P : 1];
 #pragma unroll
   for (int k = 0; k < P; k++) {
     Dp[k] = static_cast<const TV*>(B.D) + (size_t)q.slot[k] * N + I.off;
     Yp[k] = static_cast<const TV*>(B.Y) + (size_t)q.slot[k] * N + I.off;
   }
-  Dp[P] = static_cast<const TV*>(B.D) + (size_t)q.dslot * N + I.off;
-  double acc[kKG];
+  const T tau = (T)q.tau;
+  auto val = [&](int i) -> T {
+    const T st = tau * (T)d[i];
+    return (T)x[i] + st;
+  };
+  constexpr int o = 1;  // Gram block offset
+  double acc[kKS];
 #pragma unroll
-  for (int k = 0; k < kKG; k++) acc[k] = 0;
+  for (int k = 0; k < kKS; k++) acc[k] = 0;
+  const int Z = I.Z, Y = I.Y, X = I.X, YZ = Y * Z;
 #pragma unroll 2
   for (int r = 0; r < kBR; r++) {
     const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
     if (i >= I.n) break;
-    const double a = (double)gn[i];
+    const int z = i % Z, t2 = i / Z;
+    const int y = t2 % Y, xx = t2 / Y;
+    const int izm = z == 0 ? i + (Z - 1) : i - 1;
+    const int izp = z == Z - 1 ? i - (Z - 1) : i + 1;
+    const int iym = y == 0 ? i + (Y - 1) * Z : i - Z;
+    const int iyp = y == Y - 1 ? i - (Y - 1) * Z : i + Z;
+    const int ixm = xx == 0 ? i + (X - 1) * YZ : i - YZ;
+    const int ixp = xx == X - 1 ? i - (X - 1) * YZ : i + YZ;
+    const T ck = val(i);
+    xn[i] = (double)ck;
+    const T dxp = ck - val(ixp), dyp = ck - val(iyp), dzp = ck - val(izp);
+    const T a1 = ck + (T)1, a2 = ck - (T)1;
+    const T be = C::fb(I) * ((a1 * a1) * (a2 * a2));
+    T sq = dxp * dxp;
+    sq = fma(dyp, dyp, sq);
+    sq = fma(dzp, dzp, sq);
+    acc[0] += (double)fma(C::fg(I), sq, be);
+    const T dxm = ck - val(ixm), dym = ck - val(iym), dzm = ck - val(izm);
+    const T lap = ((dxm + dxp) + (dym + dyp)) + (dzm + dzp);
+    const T bg = C::fbg(I) * (ck * fma(ck, ck, (T)-1));
+    const TV gv = (TV)fma(C::fg2(I), lap, bg);
+    gn[i] = gv;
+    const double a = (double)gv;
     const double b = (double)go[i];
     double dv[P + 1], wv[P > 0 ? P : 1];
 #pragma unroll
-    for (int k = 0; k <= P; k++) dv[k] = (double)Dp[k][i];
+    for (int k = 0; k < P; k++) dv[k] = (double)Dp[k][i];
+    dv[P] = (double)d[i];
 #pragma unroll
     for (int k = 0; k < P; k++) wv[k] = (double)Yp[k][i];
     const TV ys = (TV)(a - b);  // stored as TV; the dots use the stored value
@@ -448,75 +495,50 @@ __device__ __forceinline__ void bgram_body(const BBufs& B, const BDyn& q, const 
     const double yv = (double)ys;
 #pragma unroll
     for (int k = 0; k <= P; k++) {
-      acc[k] += a * dv[k];
-      acc[2 * kBS + 2 + k] += yv * dv[k];
+      acc[o + k] += a * dv[k];
+      acc[o + 2 * kBS + 2 + k] += yv * dv[k];
     }
 #pragma unroll
     for (int k = 0; k < P; k++) {
-      acc[kBS + k] += a * wv[k];
-      acc[3 * kBS + 2 + k] += yv * wv[k];
+      acc[o + kBS + k] += a * wv[k];
+      acc[o + 3 * kBS + 2 + k] += yv * wv[k];
     }
-    acc[2 * kBS] += a * a;
-    acc[2 * kBS + 1] += a * yv;
-    acc[4 * kBS + 2] += yv * yv;
+    acc[o + 2 * kBS] += a * a;
+    acc[o + 2 * kBS + 1] += a * yv;
+    acc[o + 4 * kBS + 2] += yv * yv;
   }
   constexpr unsigned long long dm = (1ull << (P + 1)) - 1, ym = (1ull << P) - 1;
-  constexpr unsigned long long on = dm | (ym << kBS) | (3ull << (2 * kBS)) | (dm << (2 * kBS + 2)) |
-                                    (ym << (3 * kBS + 2)) | (1ull << (4 * kBS + 2));
-  block_partials<kKG>(acc, on, B.part);
+  constexpr unsigned long long gon = dm | (ym << kBS) | (3ull << (2 * kBS)) | (dm << (2 * kBS + 2)) |
+                                     (ym << (3 * kBS + 2)) | (1ull << (4 * kBS + 2));
+  block_partials<kKS>(acc, 1ull | (gon << o), B.part);
+}
+
+template <int P, typename TV>
+__device__ __forceinline__ void bstep_prec(const BBufs& B, const BDyn& q, const BInst& I, int lb) {
+  if (q.lp) bstep_body<P, float, TV>(B, q, I, lb);
+  else bstep_body<P, double, TV>(B, q, I, lb);
 }
 
 template <typename TV>
-__global__ __launch_bounds__(kBlock) void bgram_kernel(BBufs B) {
+__global__ __launch_bounds__(kBlock) void bstep_kernel(BBufs B) {
   const int p = find_pos(B.ablk, B.na, blockIdx.x);
-  const BDyn& q = B.dyn[p];
+  const BDyn& q = B.dyn[B.map ? B.map[p] : p];
   MINIM_BATCH_SKIP(B, q);
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
   switch (q.np) {
-    case 0: bgram_body<0, TV>(B, q, I, lb); break;
-    case 1: bgram_body<1, TV>(B, q, I, lb); break;
-    case 2: bgram_body<2, TV>(B, q, I, lb); break;
-    case 3: bgram_body<3, TV>(B, q, I, lb); break;
-    case 4: bgram_body<4, TV>(B, q, I, lb); break;
-    case 5: bgram_body<5, TV>(B, q, I, lb); break;
-    case 6: bgram_body<6, TV>(B, q, I, lb); break;
-    case 7: bgram_body<7, TV>(B, q, I, lb); break;
-    case 8: bgram_body<8, TV>(B, q, I, lb); break;
-    case 9: bgram_body<9, TV>(B, q, I, lb); break;
-    default: bgram_body<10, TV>(B, q, I, lb); break;
+    case 0: bstep_prec<0, TV>(B, q, I, lb); break;
+    case 1: bstep_prec<1, TV>(B, q, I, lb); break;
+    case 2: bstep_prec<2, TV>(B, q, I, lb); break;
+    case 3: bstep_prec<3, TV>(B, q, I, lb); break;
+    case 4: bstep_prec<4, TV>(B, q, I, lb); break;
+    case 5: bstep_prec<5, TV>(B, q, I, lb); break;
+    case 6: bstep_prec<6, TV>(B, q, I, lb); break;
+    case 7: bstep_prec<7, TV>(B, q, I, lb); break;
+    case 8: bstep_prec<8, TV>(B, q, I, lb); break;
+    case 9: bstep_prec<9, TV>(B, q, I, lb); break;
+    default: bstep_prec<10, TV>(B, q, I, lb); break;
   }
-}
-
-// Dots of a recomputed g = G[gcur] with every stored pair (after the switch
-// to FP64 coordinates): [0,kBS) g.D_i ; [kBS,2kBS) g.Y_i.
-template <typename TV>
-__global__ __launch_bounds__(kBlock) void bref_kernel(BBufs B) {
-  const int p = find_pos(B.ablk, B.na, blockIdx.x);
-  const BDyn& q = B.dyn[p];
-  MINIM_BATCH_SKIP(B, q);
-  const BInst I = B.inst[q.j];
-  const int lb = blockIdx.x - B.ablk[p];
-  const size_t N = B.N;
-  const TV* __restrict__ g = static_cast<const TV*>(q.gcur ? B.G1 : B.G0) + I.off;
-  const TV* __restrict__ D = static_cast<const TV*>(B.D) + I.off;
-  const TV* __restrict__ Y = static_cast<const TV*>(B.Y) + I.off;
-  double acc[2 * kBS];
-#pragma unroll
-  for (int k = 0; k < 2 * kBS; k++) acc[k] = 0;
-  for (int r = 0; r < kBR; r++) {
-    const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
-    if (i >= I.n) break;
-    const double a = (double)g[i];
-#pragma unroll
-    for (int k = 0; k < kBS; k++)
-      if (k < q.np) {
-        acc[k] += a * (double)D[(size_t)q.slot[k] * N + i];
-        acc[kBS + k] += a * (double)Y[(size_t)q.slot[k] * N + i];
-      }
-  }
-  const unsigned long long pm = (1ull << q.np) - 1;
-  block_partials<2 * kBS>(acc, pm | (pm << kBS), B.part);
 }
 
 // ---------------------------------------------------------------- device control
@@ -536,9 +558,9 @@ struct BCtl {
   double gg, e;
   double dg[kBS], yg[kBS], dres[kBS];
   double dy[kBS][kBS], yy[kBS][kBS];
-  int gcur, dslot, iters, trial, stall;
+  int gcur, xcur, dslot, iters, trial, stall;
   double tau0, tau, t;
-  int done, searching, lp, fresh, lsFailed, converged;
+  int started, done, searching, lp, fresh, lsFailed, converged;
   long long nEnergy, nGradient;
   int iterFP32, iterFP64;
   unsigned long long tdone;  // device wall clock when the instance finished
@@ -555,13 +577,18 @@ struct BCtlArgs {
   double maxStep, initStep, switchRms;
   int maxIter;
   int spec, backtracking;
-  size_t rPF, rREF, rDIR, rT0, rT, rGRAM;  // offsets of the phases' results in res
-  int* active;                             // instances still running after the iteration
-  unsigned long long* clock0;              // device wall clock at the start
+  size_t rPF, rF;                       // offsets in res: initial and fresh gradients
+  const double* part;                   // block partials of the last pass
+  const double* part2;                  // ... of the speculative trials
+  const int* ablk;                      // the launch grid (block prefix sums)
+  const int* map;                       // position -> instance
+  int* active;                          // instances running after the iteration
+  int* live;                            // per instance: 1 while it runs
+  unsigned long long* clock0;           // device wall clock at the start
 };
 
-// the first kNT trials of the sequence (linesearch.cpp:15-27); true when the
-// search has ended
+// Up to kNT trials of the sequence (linesearch.cpp:15-27); true when the
+// search has ended, with C.tau the step factor to take
 __device__ bool bc_trials(BCtl& C, const double* e) {
   for (int k = 0; k < kNT; k++) {
     C.nEnergy++;
@@ -576,17 +603,22 @@ __device__ bool bc_trials(BCtl& C, const double* e) {
   return false;
 }
 
-// The gradient and energy at the initial coordinates have been reduced into
-// res[rPF]: the state of iteration 0.
-__global__ __launch_bounds__(kBlock) void bc_init_kernel(BCtlArgs A) {
-  const int j = blockIdx.x * kBlock + threadIdx.x;
-  if (j == 0) *A.clock0 = wall_clock64();
-  if (j >= A.ni) return;
+// Instances admitted to the batch (map[0..count)): the gradient and energy at
+// their initial coordinates have been reduced into res[rPF], the state of
+// iteration 0.  setClock: the first admission, which starts the clock.
+__global__ __launch_bounds__(kBlock) void bc_admit_kernel(BCtlArgs A, const int* __restrict__ map, int count,
+                                                          int setClock) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (setClock && p == 0) *A.clock0 = wall_clock64();
+  if (p >= count) return;
+  const int j = map[p];
   BCtl& C = A.ctl[j];
   C.e = A.res[A.rPF + (size_t)j * kKP];
   C.gg = A.res[A.rPF + (size_t)j * kKP + 1];
   C.gcur = 0;
   C.nGradient++;
+  C.started = 1;
+  A.live[j] = 1;
 }
 
 // Instances that have just left FP32 coordinates take part in the fresh
@@ -594,15 +626,15 @@ __global__ __launch_bounds__(kBlock) void bc_init_kernel(BCtlArgs A) {
 // dots with the stored pairs.
 __global__ __launch_bounds__(kBlock) void bc_begin_kernel(BCtlArgs A) {
   const int j = blockIdx.x * kBlock + threadIdx.x;
-  if (j == 0) *A.active = 0;
   if (j >= A.ni) return;
   const BCtl& C = A.ctl[j];
   BDyn& D = A.dyn[j];
   D.act = 0;
-  if (C.done || !C.fresh) return;
+  if (!C.started || C.done || !C.fresh) return;
   D.j = j;
   D.gcur = C.gcur;
   D.gout = C.gcur;
+  D.xcur = C.xcur;
   D.lp = C.lp;
   D.np = C.np;
   for (int i = 0; i < C.np; i++) D.slot[i] = C.slot[i];
@@ -610,22 +642,28 @@ __global__ __launch_bounds__(kBlock) void bc_begin_kernel(BCtlArgs A) {
 }
 
 // The fresh phase's results, then the direction coefficients (Lbfgs.cpp:84-127).
+// An instance still searching along its last direction goes on searching.
 __global__ __launch_bounds__(kBlock) void bc_direction_kernel(BCtlArgs A) {
   const int j = blockIdx.x * kBlock + threadIdx.x;
+  if (j == 0) *A.active = 0;
   if (j >= A.ni) return;
   BCtl& C = A.ctl[j];
   BDyn& D = A.dyn[j];
-  if (C.done) {
+  if (!C.started || C.done) {
     D.act = 0;
     return;
   }
+  if (C.searching) {
+    D.act = kActSearch;  // D still holds the direction and D.tau the next trial
+    return;
+  }
   if (D.act & kActFresh) {
-    C.e = A.res[A.rPF + (size_t)j * kKP];
-    C.gg = A.res[A.rPF + (size_t)j * kKP + 1];
-    const double* r = A.res + A.rREF + (size_t)j * 2 * kBS;
+    const double* r = A.res + A.rF + (size_t)j * kKF;
+    C.e = r[0];
+    C.gg = r[1];
     for (int i = 0; i < C.np; i++) {
-      C.dg[C.slot[i]] = r[i];
-      C.yg[C.slot[i]] = r[kBS + i];
+      C.dg[C.slot[i]] = r[kKP + i];
+      C.yg[C.slot[i]] = r[kKP + kBS + i];
     }
     C.fresh = 0;
     C.nGradient++;
@@ -642,6 +680,7 @@ __global__ __launch_bounds__(kBlock) void bc_direction_kernel(BCtlArgs A) {
   D.dslot = spare;
   D.gcur = C.gcur;
   D.gout = 1 - C.gcur;
+  D.xcur = C.xcur;
   D.lp = C.lp;
   D.tau = 1.0;
   D.np = P;
@@ -676,18 +715,29 @@ __global__ __launch_bounds__(kBlock) void bc_direction_kernel(BCtlArgs A) {
       D.cY[r] = ay[r];
     }
   }
-  D.act = kActMain;
+  D.act = kActDir;
 }
+
+// The phases after the direction, step and trial passes run one block per
+// position of the launch grid: the block reduces the instance's partials
+// (reduce_rows, as bfinal_kernel) into LDS, then its first thread takes the
+// instance's decisions.  A block whose instance does not take part leaves at
+// once (the update kernel's block still counts a running instance).
 
 // The direction's dots: maxStep cap and flip (Lbfgs.cpp:53-57, :121-124), the
 // Armijo slope, and the speculative trials at x + d (linesearch.cpp:8-29).
+// The instance then takes its step, or searches.
 __global__ __launch_bounds__(kBlock) void bc_after_dir_kernel(BCtlArgs A) {
-  const int j = blockIdx.x * kBlock + threadIdx.x;
-  if (j >= A.ni) return;
+  __shared__ double r[kKD], rt[kNT];
+  const int p = blockIdx.x, j = A.map[p];
   BCtl& C = A.ctl[j];
   BDyn& D = A.dyn[j];
-  if (!(D.act & kActMain)) return;
-  const double* r = A.res + A.rDIR + (size_t)j * kKD;
+  if (!(D.act & kActDir)) return;
+  const int b0 = A.ablk[p], nb = A.ablk[p + 1] - b0;
+  reduce_rows(A.part, b0, nb, kKD, r);
+  if (A.spec) reduce_rows(A.part2, b0, nb, kNT, rt);
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   const double dd = r[0], dgv = r[1];
   const int P = C.np;
   for (int i = 0; i < P; i++) C.dres[C.slot[i]] = r[2 + i];
@@ -706,120 +756,128 @@ __global__ __launch_bounds__(kBlock) void bc_after_dir_kernel(BCtlArgs A) {
   C.trial = 0;
   C.searching = 0;
   C.lsFailed = 0;
-  D.act = kActMain;
-  if (!A.backtracking) return;
-  C.t = -0.5 * gs;
-  if (A.spec) {
-    if (tau0 == 1.0) {
-      if (bc_trials(C, A.res + A.rT0 + (size_t)j * kNT)) return;
-    } else {
-      C.nEnergy++;  // the speculative trials at x + d were not on this search's sequence
+  if (A.backtracking) {
+    C.t = -0.5 * gs;
+    bool ended = false;
+    if (A.spec) {
+      if (tau0 == 1.0) ended = bc_trials(C, rt);
+      else C.nEnergy++;  // the speculative trials at x + d were not on this search's sequence
+    }
+    if (!ended) {
+      C.searching = 1;
+      D.tau = C.tau;
+      D.act = kActSearch;
+      return;
     }
   }
-  C.searching = 1;
   D.tau = C.tau;
-  D.act = kActMain | kActSearch;
+  D.act = kActStep;
 }
 
-// The next kNT trials of the instances still searching.
+// The next kNT trials of the instances searching; those whose search ends
+// take their step in this iteration, the others search on in the next.
 __global__ __launch_bounds__(kBlock) void bc_consume_kernel(BCtlArgs A) {
-  const int j = blockIdx.x * kBlock + threadIdx.x;
-  if (j >= A.ni) return;
+  __shared__ double rt[kNT];
+  const int p = blockIdx.x, j = A.map[p];
   BCtl& C = A.ctl[j];
   BDyn& D = A.dyn[j];
   if (!(D.act & kActSearch)) return;
-  if (bc_trials(C, A.res + A.rT + (size_t)j * kNT)) {
-    C.searching = 0;
-    D.act = kActMain;
-  } else {
-    D.tau = C.tau;
-  }
-}
-
-// The accepted step factor for the step pass.
-__global__ __launch_bounds__(kBlock) void bc_step_kernel(BCtlArgs A) {
-  const int j = blockIdx.x * kBlock + threadIdx.x;
-  if (j >= A.ni) return;
-  const BCtl& C = A.ctl[j];
-  BDyn& D = A.dyn[j];
-  if (!(D.act & kActMain)) return;
+  reduce_rows(A.part, A.ablk[p], A.ablk[p + 1] - A.ablk[p], kNT, rt);
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const bool ended = bc_trials(C, rt);
   D.tau = C.tau;
-  D.act = kActMain;
+  if (ended) {
+    C.searching = 0;
+    D.act = kActStep;
+  }
 }
 
 // New energy and the Gram update (Lbfgs.cpp:58-80), the mixed schedule's
 // switch, and Minimiser::minimise's end tests (Minimiser.cpp:27-38,
-// Lbfgs.cpp:130-133).
+// Lbfgs.cpp:130-133); then the running instances are counted.
 __global__ __launch_bounds__(kBlock) void bc_update_kernel(BCtlArgs A) {
-  const int j = blockIdx.x * kBlock + threadIdx.x;
-  if (j >= A.ni) return;
+  __shared__ double rs[kKS];
+  const int p = blockIdx.x, j = A.map[p];
   BCtl& C = A.ctl[j];
   const BDyn& D = A.dyn[j];
-  if (!(D.act & kActMain)) return;
-  C.e = A.res[A.rPF + (size_t)j * kKP];
-  C.nGradient++;
-  const double* r = A.res + A.rGRAM + (size_t)j * kKG;
-  const double gng = r[2 * kBS], gny = r[2 * kBS + 1];
-  const double yyv = r[4 * kBS + 2];
-  const int ds = C.dslot;
-  const int P = C.np;
-  int oldSlots[kBS];
-  for (int i = 0; i < P; i++) oldSlots[i] = C.slot[i];
-  const double sy = C.tau * r[2 * kBS + 2 + P];
-  if (sy != 0) {
-    for (int i = 0; i < P; i++) {
-      const int sl = oldSlots[i];
-      C.dy[ds][sl] = C.dres[sl];  // d_new.y_old from the direction pass
-      C.dy[sl][ds] = r[2 * kBS + 2 + i];
-      C.yy[ds][sl] = r[3 * kBS + 2 + i];
-      C.yy[sl][ds] = r[3 * kBS + 2 + i];
-    }
-    C.dy[ds][ds] = r[2 * kBS + 2 + P];
-    C.yy[ds][ds] = yyv;
-    C.dg[ds] = r[P];
-    C.yg[ds] = gny;
-    C.used[ds] = 1;
-    C.slot[C.np] = ds;
-    C.ptau[C.np] = C.tau;
-    C.prho[C.np] = 1 / sy;
-    C.np++;
-    if (C.np > A.m) {
-      C.used[C.slot[0]] = 0;
-      for (int i = 1; i < C.np; i++) {
-        C.slot[i - 1] = C.slot[i];
-        C.ptau[i - 1] = C.ptau[i];
-        C.prho[i - 1] = C.prho[i];
+  const bool stepping = (D.act & kActStep) != 0;  // uniform over the block
+  if (stepping) {
+    reduce_rows(A.part, A.ablk[p], A.ablk[p + 1] - A.ablk[p], kKS, rs);
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  if (stepping) {
+    const double* r = rs;
+    C.e = r[0];
+    C.nGradient++;
+    r += 1;
+    const double gng = r[2 * kBS], gny = r[2 * kBS + 1];
+    const double yyv = r[4 * kBS + 2];
+    const int ds = C.dslot;
+    const int P = C.np;
+    int oldSlots[kBS];
+    for (int i = 0; i < P; i++) oldSlots[i] = C.slot[i];
+    const double sy = C.tau * r[2 * kBS + 2 + P];
+    if (sy != 0) {
+      for (int i = 0; i < P; i++) {
+        const int sl = oldSlots[i];
+        C.dy[ds][sl] = C.dres[sl];  // d_new.y_old from the direction pass
+        C.dy[sl][ds] = r[2 * kBS + 2 + i];
+        C.yy[ds][sl] = r[3 * kBS + 2 + i];
+        C.yy[sl][ds] = r[3 * kBS + 2 + i];
       }
-      C.np--;
+      C.dy[ds][ds] = r[2 * kBS + 2 + P];
+      C.yy[ds][ds] = yyv;
+      C.dg[ds] = r[P];
+      C.yg[ds] = gny;
+      C.used[ds] = 1;
+      C.slot[C.np] = ds;
+      C.ptau[C.np] = C.tau;
+      C.prho[C.np] = 1 / sy;
+      C.np++;
+      if (C.np > A.m) {
+        C.used[C.slot[0]] = 0;
+        for (int i = 1; i < C.np; i++) {
+          C.slot[i - 1] = C.slot[i];
+          C.ptau[i - 1] = C.ptau[i];
+          C.prho[i - 1] = C.prho[i];
+        }
+        C.np--;
+      }
+    }
+    for (int i = 0; i < P; i++) {
+      C.dg[oldSlots[i]] = r[i];
+      C.yg[oldSlots[i]] = r[kBS + i];
+    }
+    C.gg = gng;
+    C.gcur = 1 - C.gcur;
+    C.xcur = 1 - C.xcur;
+    C.iters++;
+    const double rms = sqrt(C.gg / C.n);
+    if (C.lp) {
+      const double sw = A.switchRms > 0 ? A.switchRms : 1e3 * C.conv;
+      C.stall = C.lsFailed ? C.stall + 1 : 0;
+      C.iterFP32++;
+      if (rms < sw || C.stall >= 2) {
+        C.lp = 0;
+        C.fresh = 1;  // no convergence test until g is recomputed
+      }
+    } else {
+      C.iterFP64++;
+    }
+    if (!C.fresh && rms < C.conv) {
+      C.converged = 1;
+      C.done = 1;
+    } else if (C.iters > A.maxIter) {
+      C.done = 1;
+    }
+    if (C.done) {
+      C.tdone = wall_clock64();
+      A.live[j] = 0;
     }
   }
-  for (int i = 0; i < P; i++) {
-    C.dg[oldSlots[i]] = r[i];
-    C.yg[oldSlots[i]] = r[kBS + i];
-  }
-  C.gg = gng;
-  C.gcur = 1 - C.gcur;
-  C.iters++;
-  const double rms = sqrt(C.gg / C.n);
-  if (C.lp) {
-    const double sw = A.switchRms > 0 ? A.switchRms : 1e3 * C.conv;
-    C.stall = C.lsFailed ? C.stall + 1 : 0;
-    C.iterFP32++;
-    if (rms < sw || C.stall >= 2) {
-      C.lp = 0;
-      C.fresh = 1;  // no convergence test until g is recomputed
-    }
-  } else {
-    C.iterFP64++;
-  }
-  if (!C.fresh && rms < C.conv) {
-    C.converged = 1;
-    C.done = 1;
-  } else if (C.iters > A.maxIter) {
-    C.done = 1;
-  }
-  if (C.done) C.tdone = wall_clock64();
-  else atomicAdd(A.active, 1);
+  if (C.started && !C.done) atomicAdd(A.active, 1);
 }
 
 // Host side ------------------------------------------------------------------
@@ -841,6 +899,7 @@ struct Inst {
   std::vector<double> dg, yg, dres;
   std::vector<std::vector<double>> dy, yy;
   int gcur = 0;
+  int xcur = 0;  // coordinate buffer that holds x
   int iters = 0;
   bool done = false;
   minim::MinimiseStats st;
@@ -858,12 +917,11 @@ struct Inst {
 
 using KFn = void (*)(BBufs);
 struct Kernels {
-  KFn pf_eg, pf_trial, dir, axpy, gram, ref;
+  KFn pf_eg, pf_trial, dir, step, fresh;
 };
 template <typename TV>
 Kernels kernels_for() {
-  return Kernels{bpf_kernel<3, TV>, btrial_kernel<TV>, bdir_kernel<TV>,
-                 baxpy_kernel<TV>,  bgram_kernel<TV>,  bref_kernel<TV>};
+  return Kernels{bpf_kernel<3, TV>, btrial_kernel<TV>, bdir_kernel<TV>, bstep_kernel<TV>, bpf_kernel<11, TV>};
 }
 
 // One launch group: the instances taking part, their per-phase parameters and
@@ -930,7 +988,7 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
     N += n;
     totalBlocks += I.nblk;
   }
-  if (totalBlocks > INT_MAX / kKG) throw std::invalid_argument("Lbfgs::minimiseBatch: too many elements.");
+  if (totalBlocks > INT_MAX / kKS) throw std::invalid_argument("Lbfgs::minimiseBatch: too many elements.");
 
   hipStream_t st = static_cast<hipStream_t>(states[0]->stream());
   MINIM_HIP_CHECK(hipDeviceSynchronize());  // every State's coordinates are final
@@ -938,19 +996,21 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
   auto since = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
 
   const int S = _m + 1;
-  DeviceBuffer X(N * 8), G0(N * vb), G1(N * vb), Dm(N * vb * S), Ym(N * vb * S);
+  DeviceBuffer Xa(N * 8), Xb(N * 8), G0(N * vb), G1(N * vb), Dm(N * vb * S), Ym(N * vb * S);
   // per-phase staging, one copy per phase: block prefix sums, then the BDyn of
   // each instance in the launch
   const size_t ablkBytes = (sizeof(int) * (ni + 1) + 15) / 16 * 16;
   const size_t stageBytes = ablkBytes + sizeof(BDyn) * ni;
   DeviceBuffer instD(sizeof(BInst) * ni), stageD(stageBytes);
-  DeviceBuffer part(sizeof(double) * (size_t)totalBlocks * kKG), res(sizeof(double) * (size_t)ni * (kKP + kKG));
+  // results of one fetch: fresh (pf + dots), direction + trials, or step
+  constexpr int kRes = std::max(std::max(kKF, kKD + kNT), kKS);
+  DeviceBuffer part(sizeof(double) * (size_t)totalBlocks * kKS), res(sizeof(double) * (size_t)ni * kRes);
   struct Pinned {
     void* p = nullptr;
     explicit Pinned(size_t b) { MINIM_HIP_CHECK(hipHostMalloc(&p, b)); }
     ~Pinned() { (void)hipHostFree(p); }
   };
-  Pinned stageH(stageBytes), resH(sizeof(double) * (size_t)ni * (kKP + kKG));
+  Pinned stageH(stageBytes), resH(sizeof(double) * (size_t)ni * kRes);
   int* ablkh = static_cast<int*>(stageH.p);
   BDyn* dynh = reinterpret_cast<BDyn*>(static_cast<char*>(stageH.p) + ablkBytes);
   double* resh = static_cast<double*>(resH.p);
@@ -960,12 +1020,13 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
     for (int j = 0; j < ni; j++) bi[j] = in[j].bi;
     MINIM_HIP_CHECK(hipMemcpyAsync(instD.get(), bi.data(), sizeof(BInst) * ni, hipMemcpyHostToDevice, st));
     for (int j = 0; j < ni; j++)
-      device_copy(static_cast<double*>(X.get()) + in[j].bi.off, in[j].s->deviceCoords(), in[j].bi.n * 8, st);
+      device_copy(static_cast<double*>(Xa.get()) + in[j].bi.off, in[j].s->deviceCoords(), in[j].bi.n * 8, st);
     MINIM_HIP_CHECK(hipStreamSynchronize(st));  // bi leaves scope
   }
 
   BBufs B{};
-  B.X = static_cast<double*>(X.get());
+  B.X0 = static_cast<double*>(Xa.get());
+  B.X1 = static_cast<double*>(Xb.get());
   B.G0 = G0.get();
   B.G1 = G1.get();
   B.D = Dm.get();
@@ -989,7 +1050,7 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
     B.na = L.na;
   };
   auto finalize = [&](const Launch& L, int K, size_t resOff) {
-    bfinal_kernel<<<L.na, kBlock, 0, st>>>(B.part, B.ablk, K, resd + resOff, B.dyn, 0u);
+    bfinal_kernel<<<L.na, kBlock, 0, st>>>(B.part, B.ablk, nullptr, K, resd + resOff, B.dyn, 0u);
     MINIM_HIP_CHECK(hipGetLastError());
   };
   long hostSyncs = 0, graphLaunches = 0;
@@ -1004,6 +1065,7 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
     q.dslot = I.dslot;
     q.gcur = I.gcur;
     q.gout = 1 - I.gcur;
+    q.xcur = I.xcur;
     q.lp = I.lp ? 1 : 0;
     q.tau = 0;
     return q;
@@ -1017,17 +1079,25 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
   int maxIters = 0;
 
   if (_batchControl != "host") {
-    // ---- device control: every launch covers all instances (fixed block
-    // prefix sums, one BDyn per instance), the control kernels decide which
-    // instances take part in each phase, and one iteration is a fixed
-    // sequence of launches -- captured once as a HIP graph and replayed
-    // ("graph"), or launched as it is ("device").  The host reads one counter
-    // every kChunk iterations.
+    // ---- device control: the control kernels (one thread per instance)
+    // decide which instances take part in each phase, and one iteration is a
+    // fixed sequence of launches -- captured as a HIP graph and replayed
+    // ("graph"), or launched as it is ("device").  An instance whose line
+    // search goes past the speculative trials and one further pass carries it
+    // over to the next iteration, where it takes no new direction.  The host
+    // reads the running instances every kChunk iterations.  The launch grid
+    // covers the running instances only (block prefix sums and a position ->
+    // instance map): when their blocks have fallen below 4/5 of it, it is laid
+    // out again and the graph captured again.  Where an instance's rows lie in
+    // the partials does not change its sums.
+    //
+    // With a budget (setBatchBudget), instances are admitted smallest first
+    // while the running ones hold at most that many sites, so that the first
+    // iterations are short and small instances finish early; once half of
+    // the instances have finished, every remaining one is admitted, so that
+    // the rest of the batch runs at the throughput of the whole.
     constexpr int kChunk = 8;
     const bool useGraph = _batchControl == "graph";
-    std::vector<int> ab(ni + 1, 0);
-    for (int j = 0; j < ni; j++) ab[j + 1] = ab[j] + in[j].nblk;
-    const int nbAll = ab[ni];
     std::vector<BDyn> dy0(ni);
     std::vector<BCtl> c0(ni);
     for (int j = 0; j < ni; j++) {
@@ -1041,15 +1111,18 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
       c.lp = mixed ? 1 : 0;
       c0[j] = c;
     }
-    const size_t rPF = 0, rREF = rPF + (size_t)ni * kKP, rDIR = rREF + (size_t)ni * 2 * kBS,
-                 rT0 = rDIR + (size_t)ni * kKD, rT = rT0 + (size_t)ni * kNT, rGRAM = rT + (size_t)ni * kNT,
-                 rEnd = rGRAM + (size_t)ni * kKG;
-    DeviceBuffer ablkA(sizeof(int) * (ni + 1)), dynA(sizeof(BDyn) * ni), ctlA(sizeof(BCtl) * ni), resA(8 * rEnd),
-        cntA(64);
-    Pinned cntH(64);
-    MINIM_HIP_CHECK(hipMemcpyAsync(ablkA.get(), ab.data(), sizeof(int) * (ni + 1), hipMemcpyHostToDevice, st));
+    const size_t rPF = 0, rF = rPF + (size_t)ni * kKP, rEnd = rF + (size_t)ni * kKF;
+    // counter, start clock, then the live flags
+    const size_t cntBytes = 64 + sizeof(int) * (size_t)ni;
+    DeviceBuffer ablkA(sizeof(int) * (ni + 1)), mapA(sizeof(int) * ni), dynA(sizeof(BDyn) * ni),
+        ctlA(sizeof(BCtl) * ni), resA(8 * rEnd), cntA(cntBytes), part2(sizeof(double) * (size_t)totalBlocks * kNT);
+    Pinned cntH(cntBytes);
+    int* cnt = static_cast<int*>(cntH.p);
+    int* liveH = reinterpret_cast<int*>(static_cast<char*>(cntH.p) + 64);
+    std::memset(cntH.p, 0, cntBytes);
     MINIM_HIP_CHECK(hipMemcpyAsync(dynA.get(), dy0.data(), sizeof(BDyn) * ni, hipMemcpyHostToDevice, st));
     MINIM_HIP_CHECK(hipMemcpyAsync(ctlA.get(), c0.data(), sizeof(BCtl) * ni, hipMemcpyHostToDevice, st));
+    MINIM_HIP_CHECK(hipMemcpyAsync(cntA.get(), cntH.p, cntBytes, hipMemcpyHostToDevice, st));
     MINIM_HIP_CHECK(hipStreamSynchronize(st));
     // a stream of its own: graph capture needs one other than the null stream
     hipStream_t cs = nullptr;
@@ -1060,7 +1133,7 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
     } csGuard{cs};
     BBufs G = B;
     G.ablk = static_cast<const int*>(ablkA.get());
-    G.na = ni;
+    G.map = static_cast<const int*>(mapA.get());
     G.dyn = static_cast<const BDyn*>(dynA.get());
     G.phase = 0;
     double* res = static_cast<double*>(resA.get());
@@ -1077,97 +1150,173 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
     A.spec = spec ? 1 : 0;
     A.backtracking = backtracking ? 1 : 0;
     A.rPF = rPF;
-    A.rREF = rREF;
-    A.rDIR = rDIR;
-    A.rT0 = rT0;
-    A.rT = rT;
-    A.rGRAM = rGRAM;
+    A.rF = rF;
+    A.part = G.part;
+    A.part2 = static_cast<const double*>(part2.get());
+    A.ablk = G.ablk;
+    A.map = G.map;
     A.active = static_cast<int*>(cntA.get());
     A.clock0 = reinterpret_cast<unsigned long long*>(static_cast<char*>(cntA.get()) + 8);
+    A.live = reinterpret_cast<int*>(static_cast<char*>(cntA.get()) + 64);
     const int cb = (ni + kBlock - 1) / kBlock;
+    // the launch grid over a list of instances
+    std::vector<int> ab(ni + 1, 0), mp(ni, 0);
+    int naCur = 0, nbCur = 0;
+    auto layout = [&](const std::vector<int>& list) {
+      naCur = 0;
+      for (int j : list) {
+        mp[naCur] = j;
+        ab[naCur + 1] = ab[naCur] + in[j].nblk;
+        naCur++;
+      }
+      nbCur = ab[naCur];
+      G.na = naCur;
+      MINIM_HIP_CHECK(hipMemcpyAsync(ablkA.get(), ab.data(), sizeof(int) * (naCur + 1), hipMemcpyHostToDevice, cs));
+      MINIM_HIP_CHECK(hipMemcpyAsync(mapA.get(), mp.data(), sizeof(int) * naCur, hipMemcpyHostToDevice, cs));
+      MINIM_HIP_CHECK(hipStreamSynchronize(cs));  // ab and mp are rewritten on the next layout
+    };
     auto run = [&](KFn f, unsigned phase) {
       BBufs b = G;
       b.phase = phase;
-      f<<<nbAll, kBlock, 0, cs>>>(b);
+      f<<<nbCur, kBlock, 0, cs>>>(b);
       MINIM_HIP_CHECK(hipGetLastError());
     };
     auto fin = [&](unsigned phase, int K, size_t off) {
-      bfinal_kernel<<<ni, kBlock, 0, cs>>>(G.part, G.ablk, K, res + off, G.dyn, phase);
+      bfinal_kernel<<<naCur, kBlock, 0, cs>>>(G.part, G.ablk, G.map, K, res + off, G.dyn, phase);
       MINIM_HIP_CHECK(hipGetLastError());
     };
-    auto ctl = [&](void (*k)(BCtlArgs)) {
+    auto ctl = [&](void (*k)(BCtlArgs)) {  // one thread per instance
       k<<<cb, kBlock, 0, cs>>>(A);
       MINIM_HIP_CHECK(hipGetLastError());
     };
-    // iteration 0's gradient and energy (the mixed schedule rounds x to FP32 first)
-    if (mixed) {
-      bround_kernel<<<nbAll, kBlock, 0, cs>>>(G);
+    auto ctlp = [&](void (*k)(BCtlArgs)) {  // one block per position
+      k<<<naCur, kBlock, 0, cs>>>(A);
       MINIM_HIP_CHECK(hipGetLastError());
-    }
-    run(K.pf_eg, 0);
-    fin(0, kKP, rPF);
-    ctl(bc_init_kernel);
-    auto iteration = [&] {
-      ctl(bc_begin_kernel);
-      run(K.pf_eg, kActFresh);
-      fin(kActFresh, kKP, rPF);
-      run(K.ref, kActFresh);
-      fin(kActFresh, 2 * kBS, rREF);
-      ctl(bc_direction_kernel);
-      run(K.dir, kActMain);
-      fin(kActMain, kKD, rDIR);
-      if (spec) {
-        run(K.pf_trial, kActMain);
-        fin(kActMain, kNT, rT0);
-      }
-      ctl(bc_after_dir_kernel);
-      // up to 12 more trials: the search takes at most 10
-      for (int pass = 0; pass < 3; pass++) {
-        run(K.pf_trial, kActSearch);
-        fin(kActSearch, kNT, rT);
-        ctl(bc_consume_kernel);
-      }
-      ctl(bc_step_kernel);
-      run(K.axpy, kActMain);
-      run(K.pf_eg, kActMain);
-      fin(kActMain, kKP, rPF);
-      run(K.gram, kActMain);
-      fin(kActMain, kKG, rGRAM);
-      ctl(bc_update_kernel);
     };
-    hipGraph_t gph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    if (useGraph) {
-      MINIM_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-      iteration();
-      MINIM_HIP_CHECK(hipStreamEndCapture(cs, &gph));
-      MINIM_HIP_CHECK(hipGraphInstantiate(&exec, gph, nullptr, nullptr, 0));
-    }
+    auto iteration = [&] {
+      if (mixed) {  // FP64 runs never recompute a gradient
+        ctl(bc_begin_kernel);
+        run(K.fresh, kActFresh);
+        fin(kActFresh, kKF, rF);
+      }
+      ctl(bc_direction_kernel);
+      run(K.dir, kActDir);
+      if (spec) {
+        BBufs b = G;
+        b.phase = kActDir;
+        b.part = static_cast<double*>(part2.get());
+        K.pf_trial<<<nbCur, kBlock, 0, cs>>>(b);
+        MINIM_HIP_CHECK(hipGetLastError());
+      }
+      ctlp(bc_after_dir_kernel);
+      run(K.pf_trial, kActSearch);
+      ctlp(bc_consume_kernel);
+      run(K.step, kActStep);
+      ctlp(bc_update_kernel);
+    };
     struct GraphGuard {
-      hipGraph_t g;
-      hipGraphExec_t e;
-      ~GraphGuard() {
+      hipGraph_t g = nullptr;
+      hipGraphExec_t e = nullptr;
+      void reset() {
         if (e) (void)hipGraphExecDestroy(e);
         if (g) (void)hipGraphDestroy(g);
+        e = nullptr;
+        g = nullptr;
       }
-    } graphGuard{gph, exec};
-    long replays = 0;
-    int* cnt = static_cast<int*>(cntH.p);
+      ~GraphGuard() { reset(); }
+    } graph;
+    long captures = 0;
+    auto capture = [&] {
+      graph.reset();
+      MINIM_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      iteration();
+      MINIM_HIP_CHECK(hipStreamEndCapture(cs, &graph.g));
+      MINIM_HIP_CHECK(hipGraphInstantiate(&graph.e, graph.g, nullptr, nullptr, 0));
+      captures++;
+    };
+
+    // admission order: ascending size
+    std::vector<int> order(ni);
+    for (int j = 0; j < ni; j++) order[j] = j;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return in[a].bi.n < in[b].bi.n; });
+    int next = 0, finished = 0;
+    long runningSites = 0;
+    std::vector<char> running(ni, 0);
+    auto admit = [&] {
+      std::vector<int> add;
+      const bool open = _batchBudget == 0 || 2 * finished >= ni;
+      while (next < ni) {
+        const int j = order[next];
+        if (!open && runningSites > 0 && runningSites + in[j].bi.n > _batchBudget) break;
+        runningSites += in[j].bi.n;
+        add.push_back(j);
+        next++;
+      }
+      return add;
+    };
+    // gradient and energy at the admitted instances' initial coordinates (the
+    // mixed schedule rounds x to FP32 first), then the grid over every running
+    // instance and the graph
+    bool firstAdmission = true;
+    auto start = [&](const std::vector<int>& add) {
+      layout(add);
+      if (mixed) {
+        bround_kernel<<<nbCur, kBlock, 0, cs>>>(G);
+        MINIM_HIP_CHECK(hipGetLastError());
+      }
+      run(K.pf_eg, 0);
+      fin(0, kKP, rPF);
+      bc_admit_kernel<<<(naCur + kBlock - 1) / kBlock, kBlock, 0, cs>>>(A, G.map, naCur, firstAdmission ? 1 : 0);
+      MINIM_HIP_CHECK(hipGetLastError());
+      firstAdmission = false;
+      for (int j : add) running[j] = 1;
+    };
+    auto regrid = [&] {
+      std::vector<int> list;
+      for (int j = 0; j < ni; j++)
+        if (running[j]) list.push_back(j);
+      layout(list);
+      if (useGraph) capture();
+    };
+    start(admit());
+    regrid();
+    long replays = 0, lastAdmission = 0;
     for (;;) {
       for (int k = 0; k < kChunk; k++) {
         if (useGraph) {
-          MINIM_HIP_CHECK(hipGraphLaunch(exec, cs));
+          MINIM_HIP_CHECK(hipGraphLaunch(graph.e, cs));
           graphLaunches++;
         } else {
           iteration();
         }
         replays++;
       }
-      MINIM_HIP_CHECK(hipMemcpyAsync(cnt, A.active, sizeof(int), hipMemcpyDeviceToHost, cs));
+      MINIM_HIP_CHECK(hipMemcpyAsync(cntH.p, cntA.get(), cntBytes, hipMemcpyDeviceToHost, cs));
       MINIM_HIP_CHECK(hipStreamSynchronize(cs));
       hostSyncs++;
-      if (*cnt == 0 || replays > (long)maxIter + kChunk) break;
+      long liveBlocks = 0;
+      for (int j = 0; j < ni; j++) {
+        if (running[j] && !liveH[j]) {  // finished
+          running[j] = 0;
+          runningSites -= in[j].bi.n;
+          finished++;
+        }
+        if (running[j]) liveBlocks += in[j].nblk;
+      }
+      if (cnt[0] == 0 && next == ni) break;
+      // every iteration moves each running instance on by at least one
+      // line-search pass, and an iteration takes at most 4 of them
+      if (replays - lastAdmission > 4L * ((long)maxIter + 2) + kChunk) break;
+      const std::vector<int> add = admit();
+      if (!add.empty()) {
+        start(add);
+        regrid();
+        lastAdmission = replays;
+      } else if (5 * liveBlocks <= 4 * (long)nbCur) {
+        regrid();
+      }
     }
+    batchReport.captures = captures;
     std::vector<BCtl> cf(ni);
     unsigned long long clk0 = 0;
     MINIM_HIP_CHECK(hipMemcpyAsync(cf.data(), A.ctl, sizeof(BCtl) * ni, hipMemcpyDeviceToHost, cs));
@@ -1183,6 +1332,7 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
       I.e = c.e;
       I.gg = c.gg;
       I.iters = c.iters;
+      I.xcur = c.xcur;
       I.done = c.done != 0;
       I.st.nEnergy = c.nEnergy;
       I.st.nGradient = c.nGradient;
@@ -1244,19 +1394,17 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
         }
       if (!F.pos.empty()) {
         stage(F);
-        launch(K.pf_eg, F);
-        finalize(F, kKP, 0);
-        launch(K.ref, F);
-        finalize(F, 2 * kBS, (size_t)F.na * kKP);
-        fetch((size_t)F.na * (kKP + 2 * kBS));
+        launch(K.fresh, F);
+        finalize(F, kKF, 0);
+        fetch((size_t)F.na * kKF);
         for (int p = 0; p < F.na; p++) {
           Inst& I = in[F.pos[p]];
-          I.e = resh[(size_t)p * kKP];
-          I.gg = resh[(size_t)p * kKP + 1];
-          const double* r = resh + (size_t)F.na * kKP + (size_t)p * 2 * kBS;
+          const double* r = resh + (size_t)p * kKF;
+          I.e = r[0];
+          I.gg = r[1];
           for (int i = 0; i < (int)I.pairs.size(); i++) {
-            I.dg[I.pairs[i].slot] = r[i];
-            I.yg[I.pairs[i].slot] = r[kBS + i];
+            I.dg[I.pairs[i].slot] = r[kKP + i];
+            I.yg[I.pairs[i].slot] = r[kKP + kBS + i];
           }
           I.fresh = false;
           I.st.nGradient++;
@@ -1409,19 +1557,15 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
       dynh[p] = q;
     }
     stage(L);
-    launch(K.axpy, L);
-    launch(K.pf_eg, L);
-    finalize(L, kKP, 0);
-    const size_t gramOff = (size_t)L.na * kKP;
-    launch(K.gram, L);
-    finalize(L, kKG, gramOff);
-    fetch(gramOff + (size_t)L.na * kKG);
+    launch(K.step, L);
+    finalize(L, kKS, 0);
+    fetch((size_t)L.na * kKS);
 
     for (int p = 0; p < L.na; p++) {
       Inst& I = in[L.pos[p]];
-      I.e = resh[(size_t)p * kKP];
+      I.e = resh[(size_t)p * kKS];
       I.st.nGradient++;
-      const double* r = resh + gramOff + (size_t)p * kKG;
+      const double* r = resh + (size_t)p * kKS + 1;
       auto gD = [&](int i) { return r[i]; };  // by position: stored pairs, then the new direction
       auto gY = [&](int i) { return r[kBS + i]; };
       const double gng = r[2 * kBS], gny = r[2 * kBS + 1];
@@ -1458,6 +1602,7 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
       }
       I.gg = gng;
       I.gcur = 1 - I.gcur;
+      I.xcur = 1 - I.xcur;
       I.iters++;
       const double rms = std::sqrt(I.gg / I.bi.n);
       if (I.lp) {
@@ -1486,6 +1631,7 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
 
   batchReport.iterations = maxIters;
   batchReport.replays = maxIters;
+  batchReport.captures = 0;
   }  // host control
 
   batchReport.hostSyncs = hostSyncs;
@@ -1493,7 +1639,7 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
   batchReport.control = _batchControl;
   for (int j = 0; j < ni; j++) {
     Inst& I = in[j];
-    device_copy(I.s->deviceCoords(), static_cast<double*>(X.get()) + I.bi.off, I.bi.n * 8, st);
+    device_copy(I.s->deviceCoords(), static_cast<double*>(I.xcur ? Xb.get() : Xa.get()) + I.bi.off, I.bi.n * 8, st);
     I.s->invalidateHost();
     I.st.energy = I.e;
     I.st.gradRms = std::sqrt(I.gg / I.bi.n);

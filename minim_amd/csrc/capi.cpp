@@ -115,7 +115,7 @@ void apply_pf_params(const minim_pf_params* p, PF* pf) {
 extern "C" {
 
 const char* minim_last_error(void) { return g_err.c_str(); }
-int minim_abi_version(void) { return 6; }
+int minim_abi_version(void) { return 7; }
 
 // ---------------------------------------------------------------- potentials
 int minim_pf_create(const minim_pf_params* p, minim_potential** out) {
@@ -637,7 +637,7 @@ int minim_state_component_energy(minim_state* s, int component, double* e) {
 }
 
 int minim_batch_report(const minim_minimiser* m, int* iterations, long* replays, long* host_syncs,
-                       long* graph_launches) {
+                       long* graph_launches, long* captures) {
   return guard([&] {
     need(m, "minimiser");
     auto* lb = dynamic_cast<const minim::Lbfgs*>(m->m.get());
@@ -647,6 +647,7 @@ int minim_batch_report(const minim_minimiser* m, int* iterations, long* replays,
     if (replays) *replays = r.replays;
     if (host_syncs) *host_syncs = r.hostSyncs;
     if (graph_launches) *graph_launches = r.graphLaunches;
+    if (captures) *captures = r.captures;
   });
 }
 
@@ -725,6 +726,7 @@ int minim_minimiser_create(const minim_minimiser_params* p, minim_minimiser** ou
       static const char* controls[] = {"graph", "device", "host"};
       if (p->batch_control < 0 || p->batch_control > 2) throw std::invalid_argument("invalid batch control");
       l->setBatchControl(controls[p->batch_control]);
+      l->setBatchBudget(p->batch_budget);
       m = std::move(l);
     } else if (p->kind == MINIM_FIRE) {
       auto f = std::make_unique<minim::Fire>();

@@ -311,7 +311,7 @@ struct Lbfgs::Impl {
 Lbfgs::Lbfgs() : impl_(new Impl) {}
 Lbfgs::Lbfgs(const Lbfgs& o)
     : NewMinimiser<Lbfgs>(o), _m(o._m), _maxStep(o._maxStep), _initStep(o._initStep), _precMode(o._precMode),
-      _switchRms(o._switchRms), _batchControl(o._batchControl), impl_(new Impl) {}
+      _switchRms(o._switchRms), _batchControl(o._batchControl), _batchBudget(o._batchBudget), impl_(new Impl) {}
 Lbfgs& Lbfgs::operator=(const Lbfgs& o) {
   Minimiser::operator=(o);
   _m = o._m;
@@ -320,6 +320,7 @@ Lbfgs& Lbfgs::operator=(const Lbfgs& o) {
   _precMode = o._precMode;
   _switchRms = o._switchRms;
   _batchControl = o._batchControl;
+  _batchBudget = o._batchBudget;
   impl_.reset(new Impl);
   return *this;
 }
@@ -346,6 +347,11 @@ Lbfgs& Lbfgs::setBatchControl(std::string mode) {
   if (mode != "graph" && mode != "device" && mode != "host")
     throw std::invalid_argument("Lbfgs: batch control must be graph, device or host.");
   _batchControl = mode;
+  return *this;
+}
+Lbfgs& Lbfgs::setBatchBudget(long sites) {
+  if (sites < 0) throw std::invalid_argument("Lbfgs: the batch budget must be non-negative.");
+  _batchBudget = sites;
   return *this;
 }
 

@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(L, n), n
     assert set(names) == set(_capi.SIGNATURES), "ctypes table and header disagree"
-    assert L.minim_abi_version() == 6
+    assert L.minim_abi_version() == 7
 
 
 def _pf_handle(grid, nf, st, iw, density=0, res=1.0, fix=None):
@@ -228,3 +228,10 @@ def test_batch_entry_point_errors():
     assert L.minim_minimise_batch(h, None, 0, None) == A.MINIM_EINVAL
     assert b"Lbfgs" in L.minim_last_error()
     L.minim_minimiser_destroy(h)
+    # Lbfgs::setBatchBudget rejects a negative budget
+    p.kind = A.MINIM_LBFGS
+    p.batch_budget = -1
+    assert L.minim_minimiser_create(C.byref(p), C.byref(h)) == A.MINIM_EINVAL
+    assert b"budget" in L.minim_last_error()
+    with pytest.raises(ValueError):
+        __import__("minim_amd").Lbfgs().setBatchBudget(-1)

@@ -187,6 +187,8 @@ def test_batch_through_the_c_abi(gpu):
 def run_batch(mod, cs, control, **kw):
     states = [make_state(mod, c) for c in cs]
     lb = mod.Lbfgs().setM(kw.get("m", 5)).setMaxIter(kw.get("max_iter", 20000)).setBatchControl(control)
+    if "budget" in kw:
+        lb.setBatchBudget(kw["budget"])
     if "precision" in kw:
         lb.setPrecision(kw["precision"])
     if "max_step" in kw:
@@ -248,6 +250,7 @@ def test_batch_256_instances_vs_oracle(gpu):
     states, res, rep = run_batch(m, cs, "graph")
     print("report", rep, flush=True)
     assert rep["host_syncs"] * 8 == rep["replays"]
+    assert rep["captures"] >= 2  # the grid shrank as instances finished
     n_iface = 0
     for i, (c, s, r) in enumerate(zip(cs, states, res)):
         o = OraclePF(c["grid"], 1, 1.0, [1.0], [c["w"]])
@@ -266,3 +269,63 @@ def test_batch_256_instances_vs_oracle(gpu):
             else:
                 assert energy_ok(r["energy"], so["energy"], e0), (i, r["energy"], so["energy"])
     assert n_iface >= 10  # minima with interfaces (E > 1) among the instances
+
+
+def test_batch_grid_shrinks_as_instances_finish(gpu):
+    """Instances that converge early leave the launch grid: under graph control
+    the iteration is captured again over the instances still running (more
+    than one capture), and every instance's result equals host control's bit
+    for bit (an instance's partial sums do not depend on where its blocks lie
+    in the grid)."""
+    m = gpu
+    rng = np.random.default_rng(31)
+    cs = []
+    for g in [(8, 8, 1)] * 4 + [(64, 64, 1)] * 2 + [(48, 40, 1)]:
+        n = int(np.prod(g))
+        x0 = rng.uniform(-0.1, 0.1, n)
+        if g[0] == 64:  # two stripes: an interface minimum, many iterations
+            xg = np.arange(g[0])
+            x0 = x0 + np.repeat(np.where(np.sin(2 * np.pi * (xg + 0.5) / g[0]) > 0, 0.6, -0.6), g[1] * g[2])
+        cs.append(dict(grid=g, w=float(rng.uniform(1.0, 2.0)), x0=x0))
+    out = {}
+    for control in ("graph", "host"):
+        print("control", control, flush=True)
+        states, res, rep = run_batch(m, cs, control)
+        out[control] = ([s.coords() for s in states], res, rep)
+    xg, rg, repg = out["graph"]
+    xh, rh, reph = out["host"]
+    print("reports", repg, reph, flush=True)
+    assert repg["captures"] >= 2 and reph["captures"] == 0
+    its = [r["iterations"] for r in rg]
+    assert max(its) > 2 * min(its)
+    for i in range(len(cs)):
+        assert np.array_equal(xh[i], xg[i]), i
+        for k in ("energy", "iterations", "converged", "n_energy", "n_gradient"):
+            assert rh[i][k] == rg[i][k], (i, k, rh[i][k], rg[i][k])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_budget_keeps_results(gpu, precision):
+    """A budget of running sites admits instances from both ends of the size
+    order as others finish: every instance's coordinates, energy and counts
+    equal host control's (all instances at once) bit for bit, under the graph
+    and without it."""
+    m = gpu
+    cs = cases(12, sides=(6, 9, 12, 20, 33, 48, 64), extra=((5, 6, 7), (30, 1, 1), (2, 11, 13), (40, 40, 1)))
+    kw = dict(precision=precision)
+    out = {}
+    for control, budget in (("host", 0), ("graph", 3000), ("device", 5000)):
+        print("control", control, "budget", budget, precision, flush=True)
+        states, res, rep = run_batch(m, cs, control, budget=budget, **kw)
+        out[control] = ([s.coords() for s in states], res, rep)
+    xh, rh, _ = out["host"]
+    for control in ("graph", "device"):
+        x, r, rep = out[control]
+        print(control, rep, flush=True)
+        for i in range(len(cs)):
+            assert np.array_equal(x[i], xh[i]), (control, i)
+            for k in ("energy", "iterations", "converged", "n_energy", "n_gradient", "iter_fp32", "iter_fp64"):
+                assert r[i][k] == rh[i][k], (control, i, k, r[i][k], rh[i][k])
+    assert out["graph"][2]["captures"] >= 3  # admissions re-lay the grid
+    assert all(r["converged"] for r in rh)
+
