@@ -641,13 +641,36 @@ __global__ __launch_bounds__(kBlock) void bc_begin_kernel(BCtlArgs A) {
   D.act = kActFresh;
 }
 
-// The fresh phase's results, then the direction coefficients (Lbfgs.cpp:84-127).
-// An instance still searching along its last direction goes on searching.
+// A block's instance's control record, copied between global memory and LDS
+// by the whole block (coalesced, in parallel), so that the first thread's
+// decisions read and write LDS instead of making a chain of dependent global
+// accesses (the record was last written by another kernel, possibly on
+// another XCD, so each of them would miss in L2).
+__device__ __forceinline__ void ctl_copy(BCtl* __restrict__ dst, const BCtl* __restrict__ src) {
+  static_assert(sizeof(BCtl) % 8 == 0, "BCtl is copied in 8-byte words");
+  constexpr int W = sizeof(BCtl) / 8;
+  const auto* s = reinterpret_cast<const unsigned long long*>(src);
+  auto* d = reinterpret_cast<unsigned long long*>(dst);
+  for (int i = threadIdx.x; i < W; i += kBlock) d[i] = s[i];
+}
+
+__device__ void bc_direction(const BCtlArgs& A, int j, BCtl& C);
+
+// The fresh phase's results, then the direction coefficients (Lbfgs.cpp:84-127),
+// one block per position.  An instance still searching along its last
+// direction goes on searching.
 __global__ __launch_bounds__(kBlock) void bc_direction_kernel(BCtlArgs A) {
-  const int j = blockIdx.x * kBlock + threadIdx.x;
-  if (j == 0) *A.active = 0;
-  if (j >= A.ni) return;
-  BCtl& C = A.ctl[j];
+  __shared__ BCtl Cs;
+  const int p = blockIdx.x, j = A.map[p];
+  if (p == 0 && threadIdx.x == 0) *A.active = 0;
+  ctl_copy(&Cs, A.ctl + j);
+  __syncthreads();
+  if (threadIdx.x == 0) bc_direction(A, j, Cs);
+  __syncthreads();
+  ctl_copy(A.ctl + j, &Cs);
+}
+
+__device__ void bc_direction(const BCtlArgs& A, int j, BCtl& C) {
   BDyn& D = A.dyn[j];
   if (!C.started || C.done) {
     D.act = 0;
@@ -796,19 +819,33 @@ __global__ __launch_bounds__(kBlock) void bc_consume_kernel(BCtlArgs A) {
 // New energy and the Gram update (Lbfgs.cpp:58-80), the mixed schedule's
 // switch, and Minimiser::minimise's end tests (Minimiser.cpp:27-38,
 // Lbfgs.cpp:130-133); then the running instances are counted.
+__device__ void bc_update(const BCtlArgs& A, int j, BCtl& C, const double* r);
+
 __global__ __launch_bounds__(kBlock) void bc_update_kernel(BCtlArgs A) {
   __shared__ double rs[kKS];
+  __shared__ BCtl Cs;
   const int p = blockIdx.x, j = A.map[p];
-  BCtl& C = A.ctl[j];
   const BDyn& D = A.dyn[j];
   const bool stepping = (D.act & kActStep) != 0;  // uniform over the block
-  if (stepping) {
-    reduce_rows(A.part, A.ablk[p], A.ablk[p + 1] - A.ablk[p], kKS, rs);
-    __syncthreads();
+  if (!stepping) {
+    const BCtl& C = A.ctl[j];
+    if (threadIdx.x == 0 && C.started && !C.done) atomicAdd(A.active, 1);
+    return;
   }
-  if (threadIdx.x != 0) return;
-  if (stepping) {
-    const double* r = rs;
+  ctl_copy(&Cs, A.ctl + j);
+  reduce_rows(A.part, A.ablk[p], A.ablk[p + 1] - A.ablk[p], kKS, rs);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    bc_update(A, j, Cs, rs);
+    if (Cs.started && !Cs.done) atomicAdd(A.active, 1);
+  }
+  __syncthreads();
+  ctl_copy(A.ctl + j, &Cs);
+}
+
+__device__ void bc_update(const BCtlArgs& A, int j, BCtl& C, const double* rsum) {
+  {
+    const double* r = rsum;
     C.e = r[0];
     C.nGradient++;
     r += 1;
@@ -877,7 +914,6 @@ __global__ __launch_bounds__(kBlock) void bc_update_kernel(BCtlArgs A) {
       A.live[j] = 0;
     }
   }
-  if (C.started && !C.done) atomicAdd(A.active, 1);
 }
 
 // Host side ------------------------------------------------------------------
@@ -1199,7 +1235,7 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
         run(K.fresh, kActFresh);
         fin(kActFresh, kKF, rF);
       }
-      ctl(bc_direction_kernel);
+      ctlp(bc_direction_kernel);
       run(K.dir, kActDir);
       if (spec) {
         BBufs b = G;

@@ -513,11 +513,14 @@ void dir_nf(const DirArgs& a, RedScratch rs, hipStream_t s, int nb) {
 template <typename T>
 void dir_t(const DirArgs& a, RedScratch rs, hipStream_t s) {
   const int nf = a.proj.nf < 1 ? 1 : a.proj.nf;
-  // 512 blocks (2 per CU): with 12 concurrent streams fewer, longer-running
-  // blocks reach the higher rate (256^3 x 3, P = 5: FP32 418 us at 512 blocks
-  // against 463 us at 2048)
-  const int nb = a.blocks > 0 ? std::min(a.blocks, kMaxRedBlocks) : grid_blocks(a.n / nf, 2, 512);
-  if (a.pairs >= 0 && a.pairs <= kFastPairs) {
+  // the fast kernels: 512 blocks (2 per CU); with 12 concurrent streams fewer,
+  // longer-running blocks reach the higher rate (256^3 x 3, P = 5: FP32 418 us
+  // at 512 blocks against 463 us at 2048).  The general kernel, which loops
+  // over its basis at run time, keeps 2048 (1.3 ms against 2.4 ms at 512 for
+  // 16 basis vectors over 50 M elements).
+  const bool fast = a.pairs >= 0 && a.pairs <= kFastPairs && (nf == 1 || nf == 3);
+  const int nb = a.blocks > 0 ? std::min(a.blocks, kMaxRedBlocks) : grid_blocks(a.n / nf, 2, fast ? 512 : 2048);
+  if (fast) {
     if (nf == 1 && dir_fast_nf<T, 1>(a, rs, s, nb)) return;
     if (nf == 3 && dir_fast_nf<T, 3>(a, rs, s, nb)) return;
   }
@@ -532,11 +535,13 @@ void dir_t(const DirArgs& a, RedScratch rs, hipStream_t s) {
 
 template <typename T>
 void gram_t(const GramArgs& a, RedScratch rs, hipStream_t s) {
-  // 512 blocks (2 per CU): the fused y/Gram pass streams 2P+3 vectors at once
-  // and reaches a higher rate with fewer, longer-running blocks (256^3 x 3,
-  // P = 5: FP32 489 us at 512 blocks, 511 us at 1024, 539 us at 4096)
-  const int nb = a.blocks > 0 ? std::min(a.blocks, kMaxRedBlocks) : grid_blocks(a.n, 4, 512);
-  if (a.ny <= kFastPairs && gram_fast<T>(a, rs, s, nb)) return;
+  // the fast kernels: 512 blocks (2 per CU); the fused y/Gram pass streams
+  // 2P+3 vectors at once and reaches a higher rate with fewer, longer-running
+  // blocks (256^3 x 3, P = 5: FP32 489 us at 512 blocks, 511 us at 1024, 539
+  // us at 4096).  The general kernel keeps 1024.
+  const bool fast = a.ny <= kFastPairs && a.nd == a.ny + 1;
+  const int nb = a.blocks > 0 ? std::min(a.blocks, kMaxRedBlocks) : grid_blocks(a.n, 4, fast ? 512 : 1024);
+  if (fast && gram_fast<T>(a, rs, s, nb)) return;
   const int KB = gram_bucket(a.nd > a.ny ? a.nd : a.ny);
   switch (KB) {
     case 6: gram_kernel<T, 6><<<nb, kBlock, 0, s>>>(a, rs); break;

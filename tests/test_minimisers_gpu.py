@@ -155,6 +155,35 @@ def test_adjust_callback_and_failed(gpu):
     assert mn2.stats["converged"]
 
 
+def test_nested_minimise_in_callback_invalidates_energy(gpu):
+    """A callback that runs another minimisation on the same State moves its
+    coordinates on the device (step buffers swapped, axpy); the outer
+    minimiser must then recompute e0 (linesearch.cpp:13) instead of reusing
+    its cached energy.  The same run with an explicit host round trip of the
+    coordinates after the nested call (which certainly marks them changed)
+    gives the same trajectory bit for bit."""
+    m = gpu
+    rng = np.random.default_rng(12)
+    pf = make_pf(m, (24, 24, 1), 1, [1.0], [1.5])
+    x0 = rng.uniform(-0.1, 0.1, 576)
+    runs = []
+    for roundtrip in (False, True):
+        s = m.State(pf, x0)
+
+        def adjust(it, state):
+            if it in (3, 7):
+                m.Lbfgs().setMaxIter(2).minimise(state, download=False)
+                if roundtrip:
+                    state.coords(state.coords())
+
+        mn = m.Lbfgs().setMaxIter(60)
+        mn.minimise(s, adjust)
+        runs.append((s.coords(), dict(mn.stats)))
+    (xa, sa), (xb, sb) = runs
+    assert np.array_equal(xa, xb)
+    assert sa["n_energy"] == sb["n_energy"] and sa["iterations"] == sb["iterations"]
+
+
 def test_log_minimise(gpu, capfd):
     m = gpu
     rng = np.random.default_rng(9)
