@@ -307,13 +307,14 @@ def lj_leg(m, n_side=64, density=0.85, fire_max_iter=100000, ref_side=10):
     s = m.State(pot, x0, "fp64")
     s.convergence = 1e-6
     f = m.Fire(0.02).setDt0(0.002).setMaxForceConvergence(3).setMaxIter(fire_max_iter)
+    builds0 = pot.listBuilds(s)
     torch.cuda.synchronize()
     t = time.perf_counter()
     f.minimise(s, download=False)
     torch.cuda.synchronize()
     out["fire"] = dict(seconds=time.perf_counter() - t, iterations=f.stats["iterations"],
                        converged=f.stats["converged"], max_force=f.stats["grad_max"], energy=s.energy(),
-                       max_iter=fire_max_iter,
+                       max_iter=fire_max_iter, list_builds=pot.listBuilds(s) - builds0,
                        max_force_crossings={k: dict(seconds=v[0], iteration=v[1])
                                             for k, v in f.stats["crossings"].items()})
     # reference semantics (all pairs, no cutoff) at ref_n particles: GPU beside the reference library

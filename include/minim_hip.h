@@ -143,6 +143,10 @@ int minim_lj_create(int ndim, double sigma, double epsilon, minim_potential** ou
  * ndim must be 3; skin is the neighbour-list margin (rebuild when a particle
  * has moved more than skin/2). */
 int minim_lj_set_cutoff(minim_potential* pot, double rc, const double box[3], double skin);
+/* Neighbour-list builds so far of the LjNd (cutoff mode) behind a State -- a
+ * State holds its own copy of the potential (LjNd::listBuilds; no reference
+ * counterpart). */
+int minim_lj_list_builds(const minim_state* state, long* builds);
 
 /* BarAndHinge (BarAndHinge::setTriangulation / setBondList / setHingeList /
  * setRigidity / setLength0 / setTheta0 / setForce,

@@ -420,6 +420,13 @@ class LjNd(Potential):
         self.skin = float(skin)
         return self
 
+    def listBuilds(self, state):
+        """LjNd::listBuilds: neighbour-list builds so far of the potential
+        behind state (cutoff mode)."""
+        n = C.c_long()
+        A.check(A.load().minim_lj_list_builds(state.ptr, C.byref(n)))
+        return n.value
+
     def _create(self):
         L = A.load()
         h = C.c_void_p()

@@ -280,6 +280,16 @@ int minim_lj_set_cutoff(minim_potential* pot, double rc, const double box[3], do
   });
 }
 
+int minim_lj_list_builds(const minim_state* state, long* builds) {
+  return guard([&] {
+    need(state, "state");
+    need(builds, "builds");
+    auto* lj = dynamic_cast<const minim::LjNd*>(state->s->pot.get());
+    if (!lj) throw std::invalid_argument("not a LjNd potential");
+    *builds = lj->listBuilds();
+  });
+}
+
 int minim_bah_create(const minim_bah_params* p, minim_potential** out) {
   return guard([&] {
     need(p, "params");

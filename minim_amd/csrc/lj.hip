@@ -13,6 +13,7 @@
 // list; the list is rebuilt when any particle has moved more than skin/2 from
 // where the list was built.  Lists are built deterministically (cell contents
 // sorted by particle id), so energies and gradients are reproducible.
+#include <climits>
 #include <cmath>
 #include <stdexcept>
 #include <vector>
@@ -170,53 +171,128 @@ __global__ __launch_bounds__(kBlock) void lj_fill_kernel(const int* __restrict__
   order[pos] = i;
 }
 
-// Sort each cell's members by particle id so the neighbour order is reproducible.
+// Sort each cell's members by particle id so the neighbour order is
+// reproducible: one wavefront per cell; with up to 64 members each lane
+// holds one and computes its rank by comparing with the others (ids are
+// distinct, so the ranks are a permutation); larger cells are sorted by one
+// lane with an insertion sort.
 __global__ __launch_bounds__(kBlock) void lj_sort_cells_kernel(const int* __restrict__ start, int* __restrict__ order,
                                                                int ncell) {
-  const int c = blockIdx.x * kBlock + threadIdx.x;
-  if (c >= ncell) return;
-  const int lo = start[c], hi = start[c + 1];
-  for (int a = lo + 1; a < hi; a++) {
-    int v = order[a];
-    int b = a - 1;
-    while (b >= lo && order[b] > v) {
-      order[b + 1] = order[b];
-      b--;
+  const int lane = threadIdx.x % kWave;
+  const int c = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  if (c >= ncell) return;  // uniform over the wavefront
+  const int lo = start[c], hi = start[c + 1], k = hi - lo;
+  if (k <= kWave) {
+    const int v = lane < k ? order[lo + lane] : INT_MAX;
+    int rank = 0;
+    for (int q = 0; q < k; q++) rank += __shfl(v, q, kWave) < v ? 1 : 0;
+    if (lane < k) order[lo + rank] = v;
+  } else if (lane == 0) {
+    for (int a = lo + 1; a < hi; a++) {
+      int v = order[a];
+      int b = a - 1;
+      while (b >= lo && order[b] > v) {
+        order[b + 1] = order[b];
+        b--;
+      }
+      order[b + 1] = v;
     }
-    order[b + 1] = v;
   }
 }
 
-// Full Verlet list: neighbour k of particle i stored at nbr[k*N + i].
+// Verlet list layout: for each group of 64 consecutive particles (one
+// wavefront), maxn rows of 64 entries, so that entry k of particle i sits at
+// ((i / 64) maxn + k) 64 + i % 64.  A wavefront's lanes read row k of their
+// group together (one coalesced 256-byte access in the force pass), and a
+// wavefront building its list writes only within its group's maxn x 64
+// region, which stays cached until it is complete (with rows of all N
+// particles, each lane's k-th entry fell in a different region of a list
+// larger than L2).
+__host__ __device__ inline size_t nbr_at(int i, int k, int maxn) {
+  return ((size_t)(i >> 6) * maxn + k) * 64 + (i & 63);
+}
+inline size_t nbr_elems(int N, int maxn) { return (size_t)((N + 63) / 64) * 64 * maxn; }
+
+// A coordinate wrapped into [0, L), as the cell index takes it.
+__device__ __forceinline__ double wrap_box(double v, double L) { return v - L * floor(v / L); }
+
+// Positions in cell order, wrapped into the box, each with its particle id:
+// xc[p] = (x, y, z, j) for j = order[p] (FP64, the list is built in FP64 in
+// either precision), so that the list build reads one contiguous 32-byte
+// entry per candidate instead of an index and then three scattered
+// coordinates.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void lj_gather_cells_kernel(const T* __restrict__ x, int N, BoxArgs b,
+                                                                 const int* __restrict__ order,
+                                                                 double4* __restrict__ xc) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= N) return;
+  const int j = order[p];
+  xc[p] = make_double4(wrap_box((double)x[(size_t)j * 3], b.L[0]), wrap_box((double)x[(size_t)j * 3 + 1], b.L[1]),
+                       wrap_box((double)x[(size_t)j * 3 + 2], b.L[2]), (double)j);
+}
+
+// Full Verlet list: neighbour k of particle i at nbr[nbr_at(i, k, maxn)].  The 27
+// cells around i's cell are visited in a fixed order and each cell's members
+// in particle-id order (lj_sort_cells_kernel), so the list -- and with it the
+// order in which the force pass sums pairs -- does not depend on scheduling.
+// Candidates are read U at a time, all loads issued before any is used.
+// With at least 3 cells of side >= rc + skin along each axis, the image of a
+// neighbour cell that lies within the list radius is the one its index wrap
+// implies, so each candidate needs no minimum-image rounding: i's wrapped
+// position is shifted once per neighbour cell.  Pairs are kept below the
+// list radius widened by 1e-12 of itself, so no pair the minimum-image test
+// keeps is lost to rounding; a pair kept in excess lies beyond the cutoff
+// (or at it, for skin 0) and the force pass, which tests the cutoff itself,
+// skips it.  Threads take the particles in cell order (thread p builds the
+// list of particle xc[p].w), so the lanes of a wavefront lie in a few cells
+// and read the same candidates instead of gathering from two dozen cells.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void lj_build_list_kernel(const T* __restrict__ x, int N, BoxArgs b,
                                                                const int* __restrict__ cell,
                                                                const int* __restrict__ start,
-                                                               const int* __restrict__ order, int maxn,
+                                                               const double4* __restrict__ xc, int maxn,
                                                                int* __restrict__ nn, int* __restrict__ nbr,
                                                                int* __restrict__ overflow) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= N) return;
-  const double xi0 = (double)x[(size_t)i * 3], xi1 = (double)x[(size_t)i * 3 + 1], xi2 = (double)x[(size_t)i * 3 + 2];
+  constexpr int U = 4;
+  const int pi = blockIdx.x * kBlock + threadIdx.x;
+  if (pi >= N) return;
+  const double4 vi = xc[pi];
+  const int i = (int)vi.w;
+  const double xw0 = vi.x, xw1 = vi.y, xw2 = vi.z;
+  const double rl2 = b.rlist2 * (1.0 + 1e-12);
   const int c = cell[i];
   const int cz = c % b.nc[2], cy = (c / b.nc[2]) % b.nc[1], cx = c / (b.nc[1] * b.nc[2]);
   int cnt = 0;
   for (int dx = -1; dx <= 1; dx++) {
-    const int ax = (cx + dx + b.nc[0]) % b.nc[0];
+    const int ux = cx + dx;
+    const int ax = ux < 0 ? ux + b.nc[0] : (ux >= b.nc[0] ? ux - b.nc[0] : ux);
+    // the neighbour cell's image sits at its wrapped position -/+ L
+    const double xi0 = ux < 0 ? xw0 + b.L[0] : (ux >= b.nc[0] ? xw0 - b.L[0] : xw0);
     for (int dy = -1; dy <= 1; dy++) {
-      const int ay = (cy + dy + b.nc[1]) % b.nc[1];
+      const int uy = cy + dy;
+      const int ay = uy < 0 ? uy + b.nc[1] : (uy >= b.nc[1] ? uy - b.nc[1] : uy);
+      const double xi1 = uy < 0 ? xw1 + b.L[1] : (uy >= b.nc[1] ? xw1 - b.L[1] : xw1);
       for (int dz = -1; dz <= 1; dz++) {
-        const int az = (cz + dz + b.nc[2]) % b.nc[2];
+        const int uz = cz + dz;
+        const int az = uz < 0 ? uz + b.nc[2] : (uz >= b.nc[2] ? uz - b.nc[2] : uz);
+        const double xi2 = uz < 0 ? xw2 + b.L[2] : (uz >= b.nc[2] ? xw2 - b.L[2] : xw2);
         const int cc = (ax * b.nc[1] + ay) * b.nc[2] + az;
-        for (int p = start[cc]; p < start[cc + 1]; p++) {
-          const int j = order[p];
-          if (j == i) continue;
-          const double r0 = min_image(xi0 - (double)x[(size_t)j * 3], b.L[0], b.invL[0]);
-          const double r1 = min_image(xi1 - (double)x[(size_t)j * 3 + 1], b.L[1], b.invL[1]);
-          const double r2 = min_image(xi2 - (double)x[(size_t)j * 3 + 2], b.L[2], b.invL[2]);
-          if (r0 * r0 + r1 * r1 + r2 * r2 < b.rlist2) {
-            if (cnt < maxn) nbr[(size_t)cnt * N + i] = j;
-            cnt++;
+        const int p1 = start[cc + 1];
+        for (int p0 = start[cc]; p0 < p1; p0 += U) {
+          double4 v[U];
+#pragma unroll
+          for (int u = 0; u < U; u++) v[u] = xc[min(p0 + u, p1 - 1)];
+#pragma unroll
+          for (int u = 0; u < U; u++) {
+            if (p0 + u >= p1) break;
+            const int j = (int)v[u].w;
+            if (j == i) continue;
+            const double r0 = xi0 - v[u].x, r1 = xi1 - v[u].y, r2 = xi2 - v[u].z;
+            if (r0 * r0 + r1 * r1 + r2 * r2 < rl2) {
+              if (cnt < maxn) nbr[nbr_at(i, cnt, maxn)] = j;
+              cnt++;
+            }
           }
         }
       }
@@ -249,7 +325,7 @@ __global__ __launch_bounds__(kBlock) void lj_build_list_brute_kernel(const T* __
       r2 += v * v;
     }
     if (r2 < b.rlist2) {
-      if (cnt < maxn) nbr[(size_t)cnt * N + i] = j;
+      if (cnt < maxn) nbr[nbr_at(i, cnt, maxn)] = j;
       cnt++;
     }
   }
@@ -301,8 +377,8 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void lj_cut_kernel(const typename Vec4<T>::type* __restrict__ xp, int N,
                                                         BoxArgs b, T sig2, T eps, T eshift, int mode,
                                                         const int* __restrict__ nn, const int* __restrict__ nbr,
-                                                        const double* __restrict__ xbuild, T* __restrict__ g,
-                                                        RedScratch rs) {
+                                                        int maxn, const double* __restrict__ xbuild,
+                                                        T* __restrict__ g, RedScratch rs) {
   constexpr int U = 8;
   using V = typename Vec4<T>::type;
   const int i = xcd_logical_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
@@ -325,7 +401,7 @@ __global__ __launch_bounds__(kBlock) void lj_cut_kernel(const typename Vec4<T>::
     for (int k0 = 0; k0 < cnt; k0 += U) {
       int jj[U];
 #pragma unroll
-      for (int u = 0; u < U; u++) jj[u] = nbr[(size_t)min(k0 + u, cnt - 1) * N + i];
+      for (int u = 0; u < U; u++) jj[u] = nbr[nbr_at(i, min(k0 + u, cnt - 1), maxn)];
       V xj[U];
 #pragma unroll
       for (int u = 0; u < U; u++) xj[u] = xp[jj[u]];
@@ -395,7 +471,7 @@ struct LjNd::Impl {
   long builds = 0;
   int lastMax = 0;
   bool built = false;
-  DeviceBuffer cell, count, start, cursor, order, nn, nbr, flag, xbuild, xpad;
+  DeviceBuffer cell, count, start, cursor, order, nn, nbr, flag, xbuild, xpad, xcell;
   int ncell = 0;
 };
 
@@ -471,7 +547,7 @@ void LjNd::init(const vector<double>& coords) {
     impl_->cursor.reset(sizeof(int) * impl_->ncell);
     impl_->order.reset(sizeof(int) * N);
     impl_->nn.reset(sizeof(int) * N);
-    impl_->nbr.reset(sizeof(int) * N * impl_->maxn);
+    impl_->nbr.reset(sizeof(int) * minim_amd::nbr_elems(N, impl_->maxn));
     impl_->flag.reset(sizeof(int));
     impl_->xbuild.reset(sizeof(double) * 3 * N);
     impl_->xpad.reset(sizeof(double) * 4 * N);
@@ -488,15 +564,15 @@ void launch_cut(const LjNd::Impl& I, int prec, const void* x, const void* d, dou
     auto* xp = (double4*)I.xpad.get();
     minim_amd::lj_pad_kernel<double><<<nb, kBlock, 0, s>>>((const double*)x, (const double*)d, t, N, xp);
     minim_amd::lj_cut_kernel<double><<<nb, kBlock, 0, s>>>(xp, N, I.box, sigma * sigma, epsilon, eshift, mode,
-                                                           (const int*)I.nn.get(), (const int*)I.nbr.get(),
+                                                           (const int*)I.nn.get(), (const int*)I.nbr.get(), I.maxn,
                                                            (const double*)I.xbuild.get(), (double*)g, rs);
   } else {
     auto* xp = (float4*)I.xpad.get();
     minim_amd::lj_pad_kernel<float><<<nb, kBlock, 0, s>>>((const float*)x, (const float*)d, (float)t, N, xp);
     minim_amd::lj_cut_kernel<float><<<nb, kBlock, 0, s>>>(xp, N, I.box, (float)(sigma * sigma), (float)epsilon,
                                                           (float)eshift, mode, (const int*)I.nn.get(),
-                                                          (const int*)I.nbr.get(), (const double*)I.xbuild.get(),
-                                                          (float*)g, rs);
+                                                          (const int*)I.nbr.get(), I.maxn,
+                                                          (const double*)I.xbuild.get(), (float*)g, rs);
   }
   MINIM_HIP_CHECK(hipGetLastError());
 }
@@ -517,7 +593,8 @@ void rebuild_list(LjNd::Impl& I, int prec, const void* x, const void* d, double 
     MINIM_HIP_CHECK(hipMemsetAsync(I.flag.get(), 0, sizeof(int), s));
     if (I.cellMode) {
       MINIM_HIP_CHECK(hipMemsetAsync(I.count.get(), 0, sizeof(int) * I.ncell, s));
-      const int nbc = (I.ncell + kBlock - 1) / kBlock;
+      constexpr int kCellsPerBlock = kBlock / minim_amd::kWave;  // one wavefront per cell
+      const int nbc = (I.ncell + kCellsPerBlock - 1) / kCellsPerBlock;
       if (prec == FP64)
         minim_amd::lj_cell_index_kernel<double><<<nb, kBlock, 0, s>>>((const double*)xb, N, I.box, (int*)I.cell.get(),
                                                                       (int*)I.count.get(), (double*)I.xbuild.get());
@@ -529,14 +606,21 @@ void rebuild_list(LjNd::Impl& I, int prec, const void* x, const void* d, double 
       minim_amd::lj_fill_kernel<<<nb, kBlock, 0, s>>>((const int*)I.cell.get(), N, (const int*)I.start.get(),
                                                       (int*)I.cursor.get(), (int*)I.order.get());
       minim_amd::lj_sort_cells_kernel<<<nbc, kBlock, 0, s>>>((const int*)I.start.get(), (int*)I.order.get(), I.ncell);
-      if (prec == FP64)
+      if (I.xcell.bytes() < sizeof(double4) * (size_t)N) I.xcell.reset(sizeof(double4) * (size_t)N);
+      double4* xc = static_cast<double4*>(I.xcell.get());
+      if (prec == FP64) {
+        minim_amd::lj_gather_cells_kernel<double><<<nb, kBlock, 0, s>>>((const double*)xb, N, I.box,
+                                                                         (const int*)I.order.get(), xc);
         minim_amd::lj_build_list_kernel<double><<<nb, kBlock, 0, s>>>(
-            (const double*)xb, N, I.box, (const int*)I.cell.get(), (const int*)I.start.get(),
-            (const int*)I.order.get(), I.maxn, (int*)I.nn.get(), (int*)I.nbr.get(), (int*)I.flag.get());
-      else
+            (const double*)xb, N, I.box, (const int*)I.cell.get(), (const int*)I.start.get(), xc, I.maxn,
+            (int*)I.nn.get(), (int*)I.nbr.get(), (int*)I.flag.get());
+      } else {
+        minim_amd::lj_gather_cells_kernel<float><<<nb, kBlock, 0, s>>>((const float*)xb, N, I.box,
+                                                                        (const int*)I.order.get(), xc);
         minim_amd::lj_build_list_kernel<float><<<nb, kBlock, 0, s>>>(
-            (const float*)xb, N, I.box, (const int*)I.cell.get(), (const int*)I.start.get(),
-            (const int*)I.order.get(), I.maxn, (int*)I.nn.get(), (int*)I.nbr.get(), (int*)I.flag.get());
+            (const float*)xb, N, I.box, (const int*)I.cell.get(), (const int*)I.start.get(), xc, I.maxn,
+            (int*)I.nn.get(), (int*)I.nbr.get(), (int*)I.flag.get());
+      }
     } else {
       if (prec == FP64)
         minim_amd::lj_build_list_brute_kernel<double><<<nb, kBlock, 0, s>>>(
@@ -554,7 +638,7 @@ void rebuild_list(LjNd::Impl& I, int prec, const void* x, const void* d, double 
     if (over == 0) break;
     if (attempt == 1) throw std::runtime_error("LjNd: neighbour list overflow");
     I.maxn = over + 32;
-    I.nbr.reset(sizeof(int) * (size_t)N * I.maxn);
+    I.nbr.reset(sizeof(int) * minim_amd::nbr_elems(N, I.maxn));
   }
   I.built = true;
   I.builds++;

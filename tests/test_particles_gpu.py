@@ -75,6 +75,7 @@ def test_lj_cutoff_vs_oracle(gpu, n_side, prec):
     pot = m.Lj3d().setCutoff(2.5, [L, L, L], 0.3)
     s = m.State(pot, x, prec)
     e, g = s.energyGradient()
+    assert pot.listBuilds(s) == 1
     o = OracleLJPBC(n, L, 2.5)
     assert abs(e - o.energy(x)) <= TOL[prec] * abs(o.energy(x))
     assert rel_err(g, o.gradient(x)) <= TOL[prec]
@@ -82,8 +83,17 @@ def test_lj_cutoff_vs_oracle(gpu, n_side, prec):
     y = x + rng.uniform(-0.3, 0.3, x.shape)
     y = y if prec == "fp64" else r32(y)
     e2, g2 = s.energyGradient(y)
+    assert pot.listBuilds(s) == 2
     assert abs(e2 - o.energy(y)) <= TOL[prec] * abs(o.energy(y))
     assert rel_err(g2, o.gradient(y)) <= TOL[prec]
+    # particles far outside the box (whole periods added, different per
+    # particle): the same energy and forces as their images inside it
+    k = rng.integers(-4, 5, (n, 1)) * L
+    z = (y.reshape(-1, 3) + k).reshape(-1)
+    z = z if prec == "fp64" else r32(z)
+    e3, g3 = s.energyGradient(z)
+    assert abs(e3 - o.energy(z)) <= 10 * TOL[prec] * abs(o.energy(z))
+    assert rel_err(g3, o.gradient(z)) <= 10 * TOL[prec]
 
 
 def test_lj_fire_cluster(gpu):
