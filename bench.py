@@ -511,6 +511,13 @@ def batch_leg(m, n_inst=512, workers=8, seed=4040, cpu_budget=10.0, per_state=Tr
     # decisions in kernels, the iteration replayed as a HIP graph) and, for
     # comparison, by the host (one host synchronisation per phase)
     batched = {}
+    # the batch passes' block size, tuned per precision schedule on the first
+    # 32 iterations of this batch (Lbfgs.autotuneBatch, outside the timed runs)
+    tuned = {}
+    for prec in ("fp64", "mixed"):
+        tstates = [m.State(make_pf(j), inputs(j), "fp64") for j in range(n_inst)]
+        tuned[prec] = m.Lbfgs().setM(5).setMaxIter(20000).setPrecision(prec).autotuneBatch(tstates, replays=32)
+        del tstates
     runs = [(p, "graph", 0) for p in ("fp64", "mixed")] + [(p, "graph", b) for b in budgets for p in ("fp64", "mixed")]
     runs += [(p, "host", 0) for p in ("fp64", "mixed")]
     for prec, control, budget in runs:
@@ -531,6 +538,7 @@ def batch_leg(m, n_inst=512, workers=8, seed=4040, cpu_budget=10.0, per_state=Tr
         batched[key] = dict(control=control, budget_sites=budget, host_syncs=rep["host_syncs"], replays=rep["replays"],
                              host_syncs_per_iteration=rep["host_syncs"] / max(1, rep["replays"]),
                              graph_launches=rep["graph_launches"], graph_captures=rep["captures"],
+                             block_elements_per_thread=rep["block"],
                              setup_s=bsetup, wall_s=bwall, evals=bevals, evals_per_s=bevals / bwall,
                              instances_per_s=n_inst / bwall, converged=int(sum(r["converged"] for r in bres)),
                              p50_s=float(np.percentile(bsecs, 50)), p99_s=float(np.percentile(bsecs, 99)),
@@ -545,6 +553,7 @@ def batch_leg(m, n_inst=512, workers=8, seed=4040, cpu_budget=10.0, per_state=Tr
     out = dict(workload="%d 2-D binary PhaseField instances, sides %d..%d, %d sites in total; L-BFGS m=5 each"
                         % (n_inst, sides.min(), sides.max(), int((sides ** 2).sum())),
                site_bytes_per_iteration="FP64: 8 B x (4m + 10) vector streams per site; mixed: FP32 vectors",
+               block_tuning_seconds_first_32_iterations={p: {str(k): v for k, v in t.items()} for p, t in tuned.items()},
                batched=batched)
     if per_state:
         out["per_state_threads"] = per_state_leg(m, n_inst, workers, sides, inputs, make_pf)

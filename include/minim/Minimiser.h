@@ -108,9 +108,9 @@ class Lbfgs : public NewMinimiser<Lbfgs> {
 
   // Independent minimisations of many States at once, each with the
   // semantics of minimise(state) under this minimiser's settings: one launch
-  // per phase (direction, line-search trial, step, gradient, Gram update)
-  // covers every instance still running, and one host synchronisation per
-  // phase serves them all.  Every State must hold an initialised one-fluid
+  // per phase (direction, line-search trials, step with gradient and Gram
+  // update) covers every instance still running.  Every State must hold an
+  // initialised one-fluid
   // PhaseField without solid nodes, extra terms or fixed fluid
   // (PhaseField::plainStencil), in FP64 ("state" or "fp64" precision), m <= 10
   // and no adjustState callback.  The result is left in each State's
@@ -134,6 +134,19 @@ class Lbfgs : public NewMinimiser<Lbfgs> {
   // default): every instance from the start.  An instance's result does not
   // depend on it.
   Lbfgs& setBatchBudget(long sites);
+  // Elements per thread of the batch's streaming passes (a block covers 256
+  // times as many elements of one instance): 4, 8 or 16; 0 (the default) the
+  // size autotuneBatch stored for this batch's shape, else 8.  The block size
+  // sets how an instance's sums are split over blocks, so results for
+  // different sizes agree to rounding, not bit for bit.
+  Lbfgs& setBatchBlock(int elementsPerThread);
+  // Times the first `replays` iterations of minimiseBatch(states) for each
+  // candidate block size (coordinates are read, not changed), keeps the
+  // fastest in the process-wide cache for batches of the same shape (number
+  // of instances, total elements, m, precision schedule) and returns rows
+  // (elements per thread, seconds).
+  std::vector<std::array<double, 2>> autotuneBatch(const std::vector<State*>& states,
+                                                   std::vector<int> candidates = {4, 8, 16}, int replays = 32);
   struct BatchReport {
     std::string control;
     int iterations = 0;       // the most iterations any instance ran
@@ -141,6 +154,7 @@ class Lbfgs : public NewMinimiser<Lbfgs> {
     long hostSyncs = 0;       // host synchronisations waiting for results
     long graphLaunches = 0;   // HIP graph launches
     long captures = 0;        // graph captures (one more each time the grid shrank)
+    int block = 0;            // elements per thread
   };
   BatchReport batchReport;  // of the last minimiseBatch
 
@@ -152,6 +166,12 @@ class Lbfgs : public NewMinimiser<Lbfgs> {
   double _switchRms = 0;
   std::string _batchControl = "graph";
   long _batchBudget = 0;
+  int _batchBlock = 0;
+  // minimiseBatch with a given block size; with replayLimit > 0 it stops after
+  // that many iterations and, like writeBack false, leaves the States as
+  // they were.
+  std::vector<MinimiseStats> batchRun(const std::vector<State*>& states, int block, long replayLimit,
+                                      bool writeBack);
   struct Impl;
   std::unique_ptr<Impl> impl_;
 };

@@ -321,6 +321,7 @@ typedef struct minim_minimiser_params {
                         this many DOFs (0 = RMS gradient, the reference) */
   int batch_control; /* Lbfgs::setBatchControl for minim_minimise_batch: 0 graph, 1 device, 2 host */
   long batch_budget; /* Lbfgs::setBatchBudget: sites of running instances under device control (0 = all) */
+  int batch_block;   /* Lbfgs::setBatchBlock: elements per thread of the batch passes, 4, 8 or 16 (0 = tuned or 8) */
 } minim_minimiser_params;
 
 typedef struct minim_result {
@@ -367,7 +368,14 @@ int minim_minimise_batch(minim_minimiser* m, minim_state* const* states, int n_s
  * graph captures (one more each time the launch grid shrank to the instances
  * still running). */
 int minim_batch_report(const minim_minimiser* m, int* iterations, long* replays, long* host_syncs,
-                       long* graph_launches, long* captures);
+                       long* graph_launches, long* captures, int* block);
+/* Lbfgs::autotuneBatch (no reference counterpart): the first `replays`
+ * iterations of minim_minimise_batch(states) timed for each candidate block
+ * size (4, 8 or 16 elements per thread; the States are not changed); the
+ * fastest is stored for batches of the same shape.  seconds[i] (may be NULL)
+ * receives candidate i's time, *best (may be NULL) the chosen size. */
+int minim_lbfgs_autotune_batch(minim_minimiser* m, minim_state* const* states, int n_states, const int* candidates,
+                               int n_candidates, int replays, double* seconds, int* best);
 
 #ifdef __cplusplus
 }

@@ -42,7 +42,7 @@ class MinimiserParams(C.Structure):
                 ("max_step", C.c_double), ("init_step", C.c_double), ("precision_mode", C.c_int),
                 ("switch_rms", C.c_double), ("dt_max", C.c_double), ("alpha", C.c_double),
                 ("dt0", C.c_double), ("max_force_dim", C.c_int), ("batch_control", C.c_int),
-                ("batch_budget", C.c_long)]
+                ("batch_budget", C.c_long), ("batch_block", C.c_int)]
 
 
 class Result(C.Structure):
@@ -107,7 +107,8 @@ SIGNATURES = {
     "minim_state_mark_coords_changed": (C.c_int, [vp]),
     "minim_state_component_energy": (C.c_int, [vp, C.c_int, dp]),
     "minim_batch_report": (C.c_int, [vp, ip, C.POINTER(C.c_long), C.POINTER(C.c_long), C.POINTER(C.c_long),
-                                     C.POINTER(C.c_long)]),
+                                     C.POINTER(C.c_long), ip]),
+    "minim_lbfgs_autotune_batch": (C.c_int, [vp, C.POINTER(vp), C.c_int, ip, C.c_int, C.c_int, dp, ip]),
     "minim_minimiser_create": (C.c_int, [C.POINTER(MinimiserParams), C.POINTER(vp)]),
     "minim_minimiser_destroy": (C.c_int, [vp]),
     "minim_minimise": (C.c_int, [vp, vp, ADJUST_FN, vp, dp, C.POINTER(Result)]),
@@ -118,7 +119,7 @@ SIGNATURES = {
 _lib = None
 
 
-ABI_VERSION = 7  # layout of the parameter/result structs below
+ABI_VERSION = 8  # layout of the parameter/result structs below
 
 
 def load():

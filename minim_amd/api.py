@@ -787,6 +787,7 @@ class Lbfgs(Minimiser):
         self.switchRms = 0.0
         self.batchControl = "graph"
         self.batchBudget = 0
+        self.batchBlock = 0
 
     def setBatchControl(self, mode):
         """Lbfgs::setBatchControl: "graph" (device decisions, HIP-graph-captured
@@ -796,6 +797,30 @@ class Lbfgs(Minimiser):
         self.batchControl = mode
         self._h = None
         return self
+
+    def setBatchBlock(self, elements_per_thread):
+        """Lbfgs::setBatchBlock: elements per thread of the batch passes (4, 8
+        or 16; 0 = the autotuned size for the batch's shape, else 8)."""
+        if elements_per_thread not in (0, 4, 8, 16):
+            raise ValueError("Lbfgs: the batch block must be 4, 8 or 16 elements per thread (or 0).")
+        self.batchBlock = int(elements_per_thread)
+        self._h = None
+        return self
+
+    def autotuneBatch(self, states, blocks=(4, 8, 16), replays=32):
+        """Lbfgs::autotuneBatch: times the first `replays` iterations of
+        minimiseBatch(states) per block size (the States are not changed) and
+        stores the fastest for batches of the same shape.  Returns {block:
+        seconds}."""
+        L = A.load()
+        n = len(states)
+        arr = (C.c_void_p * max(n, 1))(*[s.ptr for s in states])
+        bv = np.ascontiguousarray(blocks, dtype=np.int32)
+        secs = np.zeros(len(bv))
+        best = C.c_int()
+        A.check(L.minim_lbfgs_autotune_batch(self._handle().ptr, arr, n, bv.ctypes.data_as(A.ip), len(bv),
+                                             int(replays), secs.ctypes.data_as(A.dp), C.byref(best)))
+        return {int(b): float(t) for b, t in zip(bv, secs)}
 
     def setBatchBudget(self, sites):
         """Lbfgs::setBatchBudget: under device control, instances are admitted
@@ -813,10 +838,11 @@ class Lbfgs(Minimiser):
         launch grid shrank to the instances still running)."""
         it = C.c_int()
         rp, hs, gl, cp = C.c_long(), C.c_long(), C.c_long(), C.c_long()
+        blk = C.c_int()
         A.check(A.load().minim_batch_report(self._handle().ptr, C.byref(it), C.byref(rp), C.byref(hs), C.byref(gl),
-                                            C.byref(cp)))
+                                            C.byref(cp), C.byref(blk)))
         return dict(control=self.batchControl, iterations=it.value, replays=rp.value, host_syncs=hs.value,
-                    graph_launches=gl.value, captures=cp.value)
+                    graph_launches=gl.value, captures=cp.value, block=blk.value)
 
     def setM(self, m):
         self.m = int(m)
@@ -885,6 +911,7 @@ class Lbfgs(Minimiser):
         p.switch_rms = self.switchRms
         p.batch_control = {"graph": 0, "device": 1, "host": 2}[self.batchControl]
         p.batch_budget = self.batchBudget
+        p.batch_block = self.batchBlock
         return p
 
 

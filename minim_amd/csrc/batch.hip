@@ -11,22 +11,26 @@
 //   direction   d = c_g g + sum_s c_D[s] D_s + c_Y[s] Y_s, with d.d, d.g, d.Y_s
 //   trials      energies at x + tau 2^-k d, k < 4 (per-instance tau): the next
 //               four points of the backtracking sequence in one pass
-//   step        x += tau d, then energy and gradient at x (+ g.g)
-//   Gram        y = g_new - g, with every dot the next recursion needs
+//   step        x' = x + tau d into the other coordinate buffer, energy and
+//               gradient at x', y = g_new - g and every dot the next
+//               recursion needs
 //
-// A block covers kBE consecutive elements of one instance; it finds its
-// instance by binary search over the prefix sums of the block counts of the
-// instances in the launch, and writes its partial sums to its own row.  A
-// second small kernel sums each instance's rows in a fixed order, so every
-// per-instance result is the same whatever other instances share the launch.
-// The direction and Gram passes are compiled for each number of stored pairs
-// (a block-uniform switch), so all their loads are unconditional.
-// The host keeps each instance's (2m+1)^2 Gram data and runs the two-loop
-// recursion on coefficients, as the single-State Lbfgs does; one host
-// synchronisation per phase serves all instances.
+// A block covers 256 BR consecutive elements of one instance (BR elements per
+// thread, a launch-shape parameter); it finds its instance by binary search
+// over the prefix sums of the block counts of the instances in the launch,
+// and writes its partial sums to its own row.  The rows of each instance are
+// summed in a fixed order, so every per-instance result is the same whatever
+// other instances share the launch.  The direction and step passes are
+// compiled for each number of stored pairs (a block-uniform switch), so all
+// their loads are unconditional.  Each instance's (2m+1)^2 Gram data and its
+// two-loop recursion live either on the device (device control: one thread
+// of a block per instance takes the decisions, and an iteration is a fixed
+// launch sequence replayed as a HIP graph) or on the host (host control: one
+// host synchronisation per phase serves all instances).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <climits>
 #include <cmath>
@@ -38,14 +42,14 @@
 #include "minim/Minimiser.h"
 #include "minim/PhaseField.h"
 #include "minim/State.h"
+#include "autotune.hpp"
 #include "runtime.hpp"
 
 namespace minim_amd {
 namespace {
 
 constexpr int kBS = 11;            // history slots (m + 1 with m <= 10)
-constexpr int kBR = 8;             // elements per thread
-constexpr int kBE = kBlock * kBR;  // elements per block
+constexpr int kBlockDefault = 8;   // elements per thread when none is set or tuned
 constexpr int kKP = 2;             // stencil partials: energy, g.g
 constexpr int kNT = 4;             // line-search trials evaluated per pass: tau, tau/2, tau/4, tau/8
 constexpr int kKD = 2 + kBS;       // direction: d.d, d.g, d.Y_s
@@ -192,7 +196,7 @@ __global__ __launch_bounds__(kBlock) void bfinal_kernel(const double* __restrict
 // MODE bits: 1 energy, 2 gradient into G[gout] with g.g, 4 at x + tau * d,
 // 8 also the dots of the gradient with every stored pair, g.D_i and g.Y_i
 // (partials [2, 2 + kBS) and [2 + kBS, 2 + 2 kBS)).
-template <int MODE, typename T, typename TV>
+template <int BR, int MODE, typename T, typename TV>
 __device__ __forceinline__ void bpf_body(const BBufs& B, const BDyn& q, const BInst& I, int lb) {
   using C = BC<T>;
   constexpr int K = (MODE & 8) ? kKF : kKP;
@@ -214,8 +218,8 @@ __device__ __forceinline__ void bpf_body(const BBufs& B, const BDyn& q, const BI
 #pragma unroll
   for (int k = 0; k < K; k++) acc[k] = 0;
 #pragma unroll
-  for (int r = 0; r < kBR; r++) {
-    const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
+  for (int r = 0; r < BR; r++) {
+    const int i = lb * (BR * kBlock) + r * kBlock + (int)threadIdx.x;
     if (i >= I.n) break;
     const int z = i % Z, t2 = i / Z;
     const int y = t2 % Y, xx = t2 / Y;
@@ -264,22 +268,22 @@ __device__ __forceinline__ void bpf_body(const BBufs& B, const BDyn& q, const BI
   block_partials<K>(acc, on, B.part);
 }
 
-template <int MODE, typename TV>
+template <int BR, int MODE, typename TV>
 __global__ __launch_bounds__(kBlock) void bpf_kernel(BBufs B) {
   const int p = find_pos(B.ablk, B.na, blockIdx.x);
   const BDyn& q = B.dyn[B.map ? B.map[p] : p];
   MINIM_BATCH_SKIP(B, q);
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
-  if (q.lp) bpf_body<MODE, float, TV>(B, q, I, lb);
-  else bpf_body<MODE, double, TV>(B, q, I, lb);
+  if (q.lp) bpf_body<BR, MODE, float, TV>(B, q, I, lb);
+  else bpf_body<BR, MODE, double, TV>(B, q, I, lb);
 }
 
 // Energies at kNT points x + tau_k d along the line-search sequence, tau_k =
 // tau 2^-k (the halvings of linesearch.cpp:24-25 are exact): x and d are read
 // once for all of them, and the host takes the first point that passes the
 // Armijo test, exactly as trying them one after another would.
-template <typename T, typename TV>
+template <int BR, typename T, typename TV>
 __device__ __forceinline__ void btrial_body(const BBufs& B, const BDyn& q, const BInst& I, int lb) {
   using C = BC<T>;
   const double* __restrict__ x = (q.xcur ? B.X1 : B.X0) + I.off;
@@ -292,8 +296,8 @@ __device__ __forceinline__ void btrial_body(const BBufs& B, const BDyn& q, const
   double acc[kNT];
 #pragma unroll
   for (int k = 0; k < kNT; k++) acc[k] = 0;
-  for (int r = 0; r < kBR; r++) {
-    const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
+  for (int r = 0; r < BR; r++) {
+    const int i = lb * (BR * kBlock) + r * kBlock + (int)threadIdx.x;
     if (i >= I.n) break;
     const int z = i % Z, t2 = i / Z;
     const int y = t2 % Y, xx = t2 / Y;
@@ -318,18 +322,19 @@ __device__ __forceinline__ void btrial_body(const BBufs& B, const BDyn& q, const
   block_partials<kNT>(acc, (1ull << kNT) - 1, B.part);
 }
 
-template <typename TV>
+template <int BR, typename TV>
 __global__ __launch_bounds__(kBlock) void btrial_kernel(BBufs B) {
   const int p = find_pos(B.ablk, B.na, blockIdx.x);
   const BDyn& q = B.dyn[B.map ? B.map[p] : p];
   MINIM_BATCH_SKIP(B, q);
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
-  if (q.lp) btrial_body<float, TV>(B, q, I, lb);
-  else btrial_body<double, TV>(B, q, I, lb);
+  if (q.lp) btrial_body<BR, float, TV>(B, q, I, lb);
+  else btrial_body<BR, double, TV>(B, q, I, lb);
 }
 
 // x rounded to FP32 (the mixed schedule's first phase keeps FP32 coordinates).
+template <int BR>
 __global__ __launch_bounds__(kBlock) void bround_kernel(BBufs B) {
   const int p = find_pos(B.ablk, B.na, blockIdx.x);
   const BDyn& q = B.dyn[B.map ? B.map[p] : p];
@@ -338,8 +343,8 @@ __global__ __launch_bounds__(kBlock) void bround_kernel(BBufs B) {
   const int lb = blockIdx.x - B.ablk[p];
   double* __restrict__ x = (q.xcur ? B.X1 : B.X0) + I.off;
 #pragma unroll
-  for (int r = 0; r < kBR; r++) {
-    const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
+  for (int r = 0; r < BR; r++) {
+    const int i = lb * (BR * kBlock) + r * kBlock + (int)threadIdx.x;
     if (i >= I.n) break;
     x[i] = (double)(float)x[i];
   }
@@ -348,7 +353,7 @@ __global__ __launch_bounds__(kBlock) void bround_kernel(BBufs B) {
 // d = cg g + sum_i cD[i] D_{slot i} + cY[i] Y_{slot i} (pairs oldest first, as
 // the single-State direction pass) into D[dslot]; partials d.d, d.g, d.y_i.
 // P is the number of stored pairs, so every load is unconditional.
-template <int P, typename TV>
+template <int BR, int P, typename TV>
 __device__ __forceinline__ void bdir_body(const BBufs& B, const BDyn& q, const BInst& I, int lb) {
   const size_t N = B.N;
   const TV* __restrict__ g = static_cast<const TV*>(q.gcur ? B.G1 : B.G0) + I.off;
@@ -368,8 +373,8 @@ __device__ __forceinline__ void bdir_body(const BBufs& B, const BDyn& q, const B
 #pragma unroll
   for (int k = 0; k < kKD; k++) acc[k] = 0;
 #pragma unroll 2
-  for (int r = 0; r < kBR; r++) {
-    const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
+  for (int r = 0; r < BR; r++) {
+    const int i = lb * (BR * kBlock) + r * kBlock + (int)threadIdx.x;
     if (i >= I.n) break;
     const double gi = (double)g[i];
     double dv[P > 0 ? P : 1], yv[P > 0 ? P : 1];
@@ -395,7 +400,7 @@ __device__ __forceinline__ void bdir_body(const BBufs& B, const BDyn& q, const B
   block_partials<kKD>(acc, on, B.part);
 }
 
-template <typename TV>
+template <int BR, typename TV>
 __global__ __launch_bounds__(kBlock) void bdir_kernel(BBufs B) {
   const int p = find_pos(B.ablk, B.na, blockIdx.x);
   const BDyn& q = B.dyn[B.map ? B.map[p] : p];
@@ -403,17 +408,17 @@ __global__ __launch_bounds__(kBlock) void bdir_kernel(BBufs B) {
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
   switch (q.np) {
-    case 0: bdir_body<0, TV>(B, q, I, lb); break;
-    case 1: bdir_body<1, TV>(B, q, I, lb); break;
-    case 2: bdir_body<2, TV>(B, q, I, lb); break;
-    case 3: bdir_body<3, TV>(B, q, I, lb); break;
-    case 4: bdir_body<4, TV>(B, q, I, lb); break;
-    case 5: bdir_body<5, TV>(B, q, I, lb); break;
-    case 6: bdir_body<6, TV>(B, q, I, lb); break;
-    case 7: bdir_body<7, TV>(B, q, I, lb); break;
-    case 8: bdir_body<8, TV>(B, q, I, lb); break;
-    case 9: bdir_body<9, TV>(B, q, I, lb); break;
-    default: bdir_body<10, TV>(B, q, I, lb); break;
+    case 0: bdir_body<BR, 0, TV>(B, q, I, lb); break;
+    case 1: bdir_body<BR, 1, TV>(B, q, I, lb); break;
+    case 2: bdir_body<BR, 2, TV>(B, q, I, lb); break;
+    case 3: bdir_body<BR, 3, TV>(B, q, I, lb); break;
+    case 4: bdir_body<BR, 4, TV>(B, q, I, lb); break;
+    case 5: bdir_body<BR, 5, TV>(B, q, I, lb); break;
+    case 6: bdir_body<BR, 6, TV>(B, q, I, lb); break;
+    case 7: bdir_body<BR, 7, TV>(B, q, I, lb); break;
+    case 8: bdir_body<BR, 8, TV>(B, q, I, lb); break;
+    case 9: bdir_body<BR, 9, TV>(B, q, I, lb); break;
+    default: bdir_body<BR, 10, TV>(B, q, I, lb); break;
   }
 }
 
@@ -429,7 +434,7 @@ __global__ __launch_bounds__(kBlock) void bdir_kernel(BBufs B) {
 //   [2kBS+2,3kBS+2) y.D_i ; [3kBS+2,4kBS+2) y.Y_i ; 4kBS+2 y.y
 // Every value is the one the separate passes would compute, and each partial
 // sums the same terms in the same order.
-template <int P, typename T, typename TV>
+template <int BR, int P, typename T, typename TV>
 __device__ __forceinline__ void bstep_body(const BBufs& B, const BDyn& q, const BInst& I, int lb) {
   using C = BC<T>;
   const size_t N = B.N;
@@ -457,8 +462,8 @@ __device__ __forceinline__ void bstep_body(const BBufs& B, const BDyn& q, const 
   for (int k = 0; k < kKS; k++) acc[k] = 0;
   const int Z = I.Z, Y = I.Y, X = I.X, YZ = Y * Z;
 #pragma unroll 2
-  for (int r = 0; r < kBR; r++) {
-    const int i = lb * kBE + r * kBlock + (int)threadIdx.x;
+  for (int r = 0; r < BR; r++) {
+    const int i = lb * (BR * kBlock) + r * kBlock + (int)threadIdx.x;
     if (i >= I.n) break;
     const int z = i % Z, t2 = i / Z;
     const int y = t2 % Y, xx = t2 / Y;
@@ -513,13 +518,13 @@ __device__ __forceinline__ void bstep_body(const BBufs& B, const BDyn& q, const 
   block_partials<kKS>(acc, 1ull | (gon << o), B.part);
 }
 
-template <int P, typename TV>
+template <int BR, int P, typename TV>
 __device__ __forceinline__ void bstep_prec(const BBufs& B, const BDyn& q, const BInst& I, int lb) {
-  if (q.lp) bstep_body<P, float, TV>(B, q, I, lb);
-  else bstep_body<P, double, TV>(B, q, I, lb);
+  if (q.lp) bstep_body<BR, P, float, TV>(B, q, I, lb);
+  else bstep_body<BR, P, double, TV>(B, q, I, lb);
 }
 
-template <typename TV>
+template <int BR, typename TV>
 __global__ __launch_bounds__(kBlock) void bstep_kernel(BBufs B) {
   const int p = find_pos(B.ablk, B.na, blockIdx.x);
   const BDyn& q = B.dyn[B.map ? B.map[p] : p];
@@ -527,17 +532,17 @@ __global__ __launch_bounds__(kBlock) void bstep_kernel(BBufs B) {
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
   switch (q.np) {
-    case 0: bstep_prec<0, TV>(B, q, I, lb); break;
-    case 1: bstep_prec<1, TV>(B, q, I, lb); break;
-    case 2: bstep_prec<2, TV>(B, q, I, lb); break;
-    case 3: bstep_prec<3, TV>(B, q, I, lb); break;
-    case 4: bstep_prec<4, TV>(B, q, I, lb); break;
-    case 5: bstep_prec<5, TV>(B, q, I, lb); break;
-    case 6: bstep_prec<6, TV>(B, q, I, lb); break;
-    case 7: bstep_prec<7, TV>(B, q, I, lb); break;
-    case 8: bstep_prec<8, TV>(B, q, I, lb); break;
-    case 9: bstep_prec<9, TV>(B, q, I, lb); break;
-    default: bstep_prec<10, TV>(B, q, I, lb); break;
+    case 0: bstep_prec<BR, 0, TV>(B, q, I, lb); break;
+    case 1: bstep_prec<BR, 1, TV>(B, q, I, lb); break;
+    case 2: bstep_prec<BR, 2, TV>(B, q, I, lb); break;
+    case 3: bstep_prec<BR, 3, TV>(B, q, I, lb); break;
+    case 4: bstep_prec<BR, 4, TV>(B, q, I, lb); break;
+    case 5: bstep_prec<BR, 5, TV>(B, q, I, lb); break;
+    case 6: bstep_prec<BR, 6, TV>(B, q, I, lb); break;
+    case 7: bstep_prec<BR, 7, TV>(B, q, I, lb); break;
+    case 8: bstep_prec<BR, 8, TV>(B, q, I, lb); break;
+    case 9: bstep_prec<BR, 9, TV>(B, q, I, lb); break;
+    default: bstep_prec<BR, 10, TV>(B, q, I, lb); break;
   }
 }
 
@@ -953,11 +958,20 @@ struct Inst {
 
 using KFn = void (*)(BBufs);
 struct Kernels {
-  KFn pf_eg, pf_trial, dir, step, fresh;
+  KFn pf_eg, pf_trial, dir, step, fresh, round;
 };
+template <int BR, typename TV>
+Kernels kernels_br() {
+  return Kernels{bpf_kernel<BR, 3, TV>,  btrial_kernel<BR, TV>,     bdir_kernel<BR, TV>,
+                 bstep_kernel<BR, TV>,   bpf_kernel<BR, 11, TV>,    bround_kernel<BR>};
+}
 template <typename TV>
-Kernels kernels_for() {
-  return Kernels{bpf_kernel<3, TV>, btrial_kernel<TV>, bdir_kernel<TV>, bstep_kernel<TV>, bpf_kernel<11, TV>};
+Kernels kernels_for(int br) {
+  switch (br) {
+    case 4: return kernels_br<4, TV>();
+    case 16: return kernels_br<16, TV>();
+    default: return kernels_br<8, TV>();
+  }
 }
 
 // One launch group: the instances taking part, their per-phase parameters and
@@ -973,10 +987,47 @@ struct Launch {
 namespace minim {
 
 std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& states) {
+  return batchRun(states, 0, 0, true);
+}
+
+std::vector<std::array<double, 2>> Lbfgs::autotuneBatch(const std::vector<State*>& states, std::vector<int> candidates,
+                                                        int replays) {
+  if (states.empty() || candidates.empty() || replays < 1)
+    throw std::invalid_argument("Lbfgs::autotuneBatch: nothing to time");
+  for (int c : candidates)
+    if (c != 4 && c != 8 && c != 16)
+      throw std::invalid_argument("Lbfgs::autotuneBatch: candidates must be 4, 8 or 16 elements per thread.");
+  std::vector<std::array<double, 2>> rows;
+  int best = 0;
+  double bestS = 0;
+  for (int c : candidates) {
+    batchRun(states, c, replays, false);  // first run of each shape: code objects, buffers
+    MINIM_HIP_CHECK(hipDeviceSynchronize());
+    const auto t0 = std::chrono::steady_clock::now();
+    batchRun(states, c, replays, false);
+    MINIM_HIP_CHECK(hipDeviceSynchronize());
+    const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    rows.push_back({(double)c, sec});
+    if (best == 0 || sec < bestS) {
+      best = c;
+      bestS = sec;
+    }
+  }
+  size_t N = 0;
+  for (const State* s : states) N += s->ndof;
+  minim_amd::tune_store(minim_amd::tune_key("batch-block", {(long)states.size(), (long)N, (long)_m},
+                                            _precMode == "mixed" ? 1 : 0),
+                        {best});
+  return rows;
+}
+
+std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, int block, long replayLimit,
+                                           bool writeBack) {
   using namespace minim_amd;
   const int ni = (int)states.size();
   std::vector<MinimiseStats> out(ni);
   if (ni == 0) return out;
+  if (replayLimit > 0) writeBack = false;
   if (_m > kBS - 1) throw std::invalid_argument("Lbfgs::minimiseBatch: m must be at most 10.");
   if (_precMode == "fp32")
     throw std::invalid_argument("Lbfgs::minimiseBatch runs in FP64 or the mixed schedule, not in FP32 throughout.");
@@ -985,7 +1036,19 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
   // FP32 throughout, as the single-State schedule does for a PhaseField
   const bool mixed = _precMode == "mixed";
   const size_t vb = mixed ? 4 : 8;
-  const Kernels K = mixed ? kernels_for<float>() : kernels_for<double>();
+  {
+    size_t nTot = 0;
+    for (const State* s : states)
+      if (s) nTot += s->ndof;
+    if (block == 0) block = _batchBlock;
+    std::vector<int> tuned;
+    if (block == 0 && tune_lookup(tune_key("batch-block", {(long)ni, (long)nTot, (long)_m}, mixed ? 1 : 0), tuned) &&
+        !tuned.empty())
+      block = tuned[0];
+    if (block != 4 && block != 8 && block != 16) block = kBlockDefault;
+  }
+  const int kBE = block * kBlock;  // elements per block
+  const Kernels K = mixed ? kernels_for<float>(block) : kernels_for<double>(block);
 
   std::vector<Inst> in(ni);
   size_t N = 0;
@@ -1297,7 +1360,7 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
     auto start = [&](const std::vector<int>& add) {
       layout(add);
       if (mixed) {
-        bround_kernel<<<nbCur, kBlock, 0, cs>>>(G);
+        K.round<<<nbCur, kBlock, 0, cs>>>(G);
         MINIM_HIP_CHECK(hipGetLastError());
       }
       run(K.pf_eg, 0);
@@ -1340,6 +1403,7 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
         if (running[j]) liveBlocks += in[j].nblk;
       }
       if (cnt[0] == 0 && next == ni) break;
+      if (replayLimit > 0 && replays >= replayLimit) break;
       // every iteration moves each running instance on by at least one
       // line-search pass, and an iteration takes at most 4 of them
       if (replays - lastAdmission > 4L * ((long)maxIter + 2) + kChunk) break;
@@ -1394,7 +1458,7 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
     }
     stage(L);
     if (mixed) {  // the State's coordinates rounded to FP32 (State::setPrecision)
-      bround_kernel<<<L.nb, kBlock, 0, st>>>(B);
+      K.round<<<L.nb, kBlock, 0, st>>>(B);
       MINIM_HIP_CHECK(hipGetLastError());
     }
     launch(K.pf_eg, L);
@@ -1414,6 +1478,7 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
     for (int j = 0; j < ni; j++)
       if (!in[j].done) L.pos.push_back(j);
     if (L.pos.empty()) break;
+    if (replayLimit > 0 && maxIters >= replayLimit) break;
 
     // ---- instances that have just left FP32 coordinates: gradient and energy
     // at the same point in FP64, and its dots with the stored pairs
@@ -1673,6 +1738,11 @@ std::vector<MinimiseStats> Lbfgs::minimiseBatch(const std::vector<State*>& state
   batchReport.hostSyncs = hostSyncs;
   batchReport.graphLaunches = graphLaunches;
   batchReport.control = _batchControl;
+  batchReport.block = block;
+  if (!writeBack) {
+    MINIM_HIP_CHECK(hipStreamSynchronize(st));
+    return {};
+  }
   for (int j = 0; j < ni; j++) {
     Inst& I = in[j];
     device_copy(I.s->deviceCoords(), static_cast<double*>(I.xcur ? Xb.get() : Xa.get()) + I.bi.off, I.bi.n * 8, st);

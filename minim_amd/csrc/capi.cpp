@@ -115,7 +115,7 @@ void apply_pf_params(const minim_pf_params* p, PF* pf) {
 extern "C" {
 
 const char* minim_last_error(void) { return g_err.c_str(); }
-int minim_abi_version(void) { return 7; }
+int minim_abi_version(void) { return 8; }
 
 // ---------------------------------------------------------------- potentials
 int minim_pf_create(const minim_pf_params* p, minim_potential** out) {
@@ -468,6 +468,34 @@ int minim_lbfgs_autotune(minim_minimiser* m, minim_state* s, const int* blocks, 
   });
 }
 
+int minim_lbfgs_autotune_batch(minim_minimiser* m, minim_state* const* states, int n_states, const int* candidates,
+                               int n_candidates, int replays, double* seconds, int* best) {
+  return guard([&] {
+    need(m, "minimiser");
+    need(states, "states");
+    need(candidates, "candidates");
+    if (n_states < 1 || n_candidates < 1) throw std::invalid_argument("nothing to time");
+    auto* lb = dynamic_cast<minim::Lbfgs*>(m->m.get());
+    if (!lb) throw std::invalid_argument("the batch autotuner is for Lbfgs");
+    std::vector<minim::State*> ss(n_states);
+    for (int i = 0; i < n_states; i++) {
+      need(states[i], "state");
+      ss[i] = states[i]->s.get();
+    }
+    const auto rows = lb->autotuneBatch(ss, std::vector<int>(candidates, candidates + n_candidates), replays);
+    int b = 0;
+    double bs = 0;
+    for (size_t i = 0; i < rows.size(); i++) {
+      if (seconds) seconds[i] = rows[i][1];
+      if (b == 0 || rows[i][1] < bs) {
+        b = (int)rows[i][0];
+        bs = rows[i][1];
+      }
+    }
+    if (best) *best = b;
+  });
+}
+
 int minim_autotune_clear(void) {
   return guard([&] { minim_amd::tune_clear(); });
 }
@@ -647,7 +675,7 @@ int minim_state_component_energy(minim_state* s, int component, double* e) {
 }
 
 int minim_batch_report(const minim_minimiser* m, int* iterations, long* replays, long* host_syncs,
-                       long* graph_launches, long* captures) {
+                       long* graph_launches, long* captures, int* block) {
   return guard([&] {
     need(m, "minimiser");
     auto* lb = dynamic_cast<const minim::Lbfgs*>(m->m.get());
@@ -658,6 +686,7 @@ int minim_batch_report(const minim_minimiser* m, int* iterations, long* replays,
     if (host_syncs) *host_syncs = r.hostSyncs;
     if (graph_launches) *graph_launches = r.graphLaunches;
     if (captures) *captures = r.captures;
+    if (block) *block = r.block;
   });
 }
 
@@ -737,6 +766,7 @@ int minim_minimiser_create(const minim_minimiser_params* p, minim_minimiser** ou
       if (p->batch_control < 0 || p->batch_control > 2) throw std::invalid_argument("invalid batch control");
       l->setBatchControl(controls[p->batch_control]);
       l->setBatchBudget(p->batch_budget);
+      l->setBatchBlock(p->batch_block);
       m = std::move(l);
     } else if (p->kind == MINIM_FIRE) {
       auto f = std::make_unique<minim::Fire>();

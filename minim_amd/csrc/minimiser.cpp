@@ -311,7 +311,8 @@ struct Lbfgs::Impl {
 Lbfgs::Lbfgs() : impl_(new Impl) {}
 Lbfgs::Lbfgs(const Lbfgs& o)
     : NewMinimiser<Lbfgs>(o), _m(o._m), _maxStep(o._maxStep), _initStep(o._initStep), _precMode(o._precMode),
-      _switchRms(o._switchRms), _batchControl(o._batchControl), _batchBudget(o._batchBudget), impl_(new Impl) {}
+      _switchRms(o._switchRms), _batchControl(o._batchControl), _batchBudget(o._batchBudget),
+      _batchBlock(o._batchBlock), impl_(new Impl) {}
 Lbfgs& Lbfgs::operator=(const Lbfgs& o) {
   Minimiser::operator=(o);
   _m = o._m;
@@ -321,6 +322,7 @@ Lbfgs& Lbfgs::operator=(const Lbfgs& o) {
   _switchRms = o._switchRms;
   _batchControl = o._batchControl;
   _batchBudget = o._batchBudget;
+  _batchBlock = o._batchBlock;
   impl_.reset(new Impl);
   return *this;
 }
@@ -347,6 +349,12 @@ Lbfgs& Lbfgs::setBatchControl(std::string mode) {
   if (mode != "graph" && mode != "device" && mode != "host")
     throw std::invalid_argument("Lbfgs: batch control must be graph, device or host.");
   _batchControl = mode;
+  return *this;
+}
+Lbfgs& Lbfgs::setBatchBlock(int e) {
+  if (e != 0 && e != 4 && e != 8 && e != 16)
+    throw std::invalid_argument("Lbfgs: the batch block must be 4, 8 or 16 elements per thread (or 0).");
+  _batchBlock = e;
   return *this;
 }
 Lbfgs& Lbfgs::setBatchBudget(long sites) {

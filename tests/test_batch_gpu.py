@@ -329,3 +329,53 @@ def test_batch_budget_keeps_results(gpu, precision):
     assert out["graph"][2]["captures"] >= 3  # admissions re-lay the grid
     assert all(r["converged"] for r in rh)
 
+
+
+def test_batch_block_sizes(gpu):
+    """The batch passes at 4, 8 and 16 elements per thread: every instance
+    converges to the same minimum (the block size changes only how sums are
+    split, so energies agree to rounding), and for one block size graph and
+    host control agree bit for bit."""
+    m = gpu
+    cs = cases(17, sides=(9, 33, 60, 100), extra=((5, 6, 7), (40, 1, 1)))
+    out = {}
+    for block, control in ((4, "graph"), (8, "graph"), (16, "graph"), (16, "host")):
+        print("block", block, control, flush=True)
+        states = [make_state(m, c) for c in cs]
+        lb = m.Lbfgs().setM(5).setMaxIter(20000).setBatchControl(control).setBatchBlock(block)
+        res = lb.minimiseBatch(states)
+        assert lb.batchReport()["block"] == block
+        out[(block, control)] = ([s.coords() for s in states], res)
+    x16g, r16g = out[(16, "graph")]
+    x16h, r16h = out[(16, "host")]
+    for i in range(len(cs)):
+        assert np.array_equal(x16g[i], x16h[i]), i
+        assert r16g[i]["iterations"] == r16h[i]["iterations"]
+    for block in (4, 8):
+        x, r = out[(block, "graph")]
+        for i, c in enumerate(cs):
+            assert r[i]["converged"] and r16g[i]["converged"]
+            e0 = OraclePF(c["grid"], 1, 1.0, [1.0], [c["w"]]).energy(c["x0"])
+            assert energy_ok(r[i]["energy"], r16g[i]["energy"], e0), (block, i)
+
+
+def test_batch_autotune(gpu):
+    """autotuneBatch times each block size on the first iterations without
+    changing the States, and the next minimiseBatch of the same shape uses
+    the size it chose."""
+    m = gpu
+    m.autotune_clear()
+    cs = cases(19, sides=(16, 48, 96), extra=((4, 5, 6),))
+    states = [make_state(m, c) for c in cs]
+    before = [s.coords() for s in states]
+    lb = m.Lbfgs().setM(5).setMaxIter(20000)
+    t = lb.autotuneBatch(states, blocks=(4, 8, 16), replays=16)
+    print("tuned", t, flush=True)
+    assert sorted(t) == [4, 8, 16] and all(v > 0 for v in t.values())
+    for b, s in zip(before, states):
+        assert np.array_equal(b, s.coords())
+    best = min(t, key=t.get)
+    res = lb.minimiseBatch(states)
+    assert lb.batchReport()["block"] == best
+    assert all(r["converged"] for r in res)
+    m.autotune_clear()
