@@ -247,9 +247,7 @@ __global__ __launch_bounds__(kBlock) void lj_gather_cells_kernel(const T* __rest
 // skips it.  Threads take the particles in cell order (thread p builds the
 // list of particle xc[p].w), so the lanes of a wavefront lie in a few cells
 // and read the same candidates instead of gathering from two dozen cells.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void lj_build_list_kernel(const T* __restrict__ x, int N, BoxArgs b,
-                                                               const int* __restrict__ cell,
+__global__ __launch_bounds__(kBlock) void lj_build_list_kernel(int N, BoxArgs b, const int* __restrict__ cell,
                                                                const int* __restrict__ start,
                                                                const double4* __restrict__ xc, int maxn,
                                                                int* __restrict__ nn, int* __restrict__ nbr,
@@ -611,16 +609,13 @@ void rebuild_list(LjNd::Impl& I, int prec, const void* x, const void* d, double 
       if (prec == FP64) {
         minim_amd::lj_gather_cells_kernel<double><<<nb, kBlock, 0, s>>>((const double*)xb, N, I.box,
                                                                          (const int*)I.order.get(), xc);
-        minim_amd::lj_build_list_kernel<double><<<nb, kBlock, 0, s>>>(
-            (const double*)xb, N, I.box, (const int*)I.cell.get(), (const int*)I.start.get(), xc, I.maxn,
-            (int*)I.nn.get(), (int*)I.nbr.get(), (int*)I.flag.get());
       } else {
         minim_amd::lj_gather_cells_kernel<float><<<nb, kBlock, 0, s>>>((const float*)xb, N, I.box,
                                                                         (const int*)I.order.get(), xc);
-        minim_amd::lj_build_list_kernel<float><<<nb, kBlock, 0, s>>>(
-            (const float*)xb, N, I.box, (const int*)I.cell.get(), (const int*)I.start.get(), xc, I.maxn,
-            (int*)I.nn.get(), (int*)I.nbr.get(), (int*)I.flag.get());
       }
+      minim_amd::lj_build_list_kernel<<<nb, kBlock, 0, s>>>(N, I.box, (const int*)I.cell.get(),
+                                                             (const int*)I.start.get(), xc, I.maxn, (int*)I.nn.get(),
+                                                             (int*)I.nbr.get(), (int*)I.flag.get());
     } else {
       if (prec == FP64)
         minim_amd::lj_build_list_brute_kernel<double><<<nb, kBlock, 0, s>>>(
