@@ -112,6 +112,34 @@ def cfg1_leg(m, with_cpu=True):
             times.append(time.perf_counter() - t)
         out[mode] = dict(seconds=times[-1], seconds_cold=times[0], iterations=mn.stats["iterations"],
                          converged=mn.stats["converged"], energy=s.energy())
+        # the same field as a batch of one: the device-controlled, graph-replayed
+        # iteration (Lbfgs.minimiseBatch), which needs one host synchronisation
+        # per 8 iterations instead of two per iteration
+        bt = []
+        for r in range(2):
+            s = m.State(pf, x0, "fp64")
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            res = mn.minimiseBatch([s])[0]
+            torch.cuda.synchronize()
+            bt.append(time.perf_counter() - t)
+        out[mode + "_batch_of_one"] = dict(seconds=bt[-1], seconds_cold=bt[0], iterations=res["iterations"],
+                                           converged=res["converged"], energy=res["energy"])
+        if mode == "fp64":
+            # ... and under resident control: one workgroup runs the whole
+            # minimisation in one launch
+            mr = m.Lbfgs().setM(5).setMaxIter(20000).setPrecision(mode).setBatchControl("resident")
+            bt = []
+            for r in range(2):
+                s = m.State(pf, x0, "fp64")
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                res = mr.minimiseBatch([s])[0]
+                torch.cuda.synchronize()
+                bt.append(time.perf_counter() - t)
+            out[mode + "_batch_of_one_resident"] = dict(seconds=bt[-1], seconds_cold=bt[0],
+                                                        iterations=res["iterations"], converged=res["converged"],
+                                                        energy=res["energy"])
     if with_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS") or min(os.cpu_count() or 1, 16))
         os.environ["OMP_NUM_THREADS"] = str(threads)
@@ -489,7 +517,8 @@ def batch_leg(m, n_inst=512, workers=8, seed=4040, cpu_budget=10.0, per_state=Tr
     False leaves out the one-State-per-call comparison.  budgets: the batch
     also runs with each of these limits on the sites of running instances
     (Lbfgs.setBatchBudget: smallest first until half have finished, then
-    every remaining instance)."""
+    every remaining instance).  The FP64 batch also runs under resident control
+    (one workgroup per instance runs its whole minimisation in one launch)."""
 
     import numpy as np
     import torch
@@ -519,7 +548,7 @@ def batch_leg(m, n_inst=512, workers=8, seed=4040, cpu_budget=10.0, per_state=Tr
         tuned[prec] = m.Lbfgs().setM(5).setMaxIter(20000).setPrecision(prec).autotuneBatch(tstates, replays=32)
         del tstates
     runs = [(p, "graph", 0) for p in ("fp64", "mixed")] + [(p, "graph", b) for b in budgets for p in ("fp64", "mixed")]
-    runs += [(p, "host", 0) for p in ("fp64", "mixed")]
+    runs += [(p, "host", 0) for p in ("fp64", "mixed")] + [("fp64", "resident", 0)]
     for prec, control, budget in runs:
         t = time.perf_counter()
         bstates = [m.State(make_pf(j), inputs(j), "fp64") for j in range(n_inst)]
@@ -533,7 +562,7 @@ def batch_leg(m, n_inst=512, workers=8, seed=4040, cpu_budget=10.0, per_state=Tr
         rep = lb.batchReport()
         bsecs = np.array([r["seconds"] for r in bres])
         bevals = sum(r["n_energy"] + r["n_gradient"] for r in bres)
-        key = prec + ("" if control == "graph" else "_host_control") + (
+        key = prec + ("" if control == "graph" else "_%s_control" % control) + (
             "_budget_%gM_sites" % (budget / 1e6) if budget else "")
         batched[key] = dict(control=control, budget_sites=budget, host_syncs=rep["host_syncs"], replays=rep["replays"],
                              host_syncs_per_iteration=rep["host_syncs"] / max(1, rep["replays"]),
@@ -677,7 +706,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--fire-iters", type=int, default=100000, help="particles mode: FIRE iteration cap")
     ap.add_argument("--lbfgs-iters", type=int, default=20000, help="particles mode: sheet L-BFGS iteration cap")
-    ap.add_argument("--mode", choices=["full", "stencil", "converge", "particles", "batch", "general", "elements"],
+    ap.add_argument("--mode", choices=["full", "stencil", "converge", "particles", "batch", "general", "elements",
+                                       "cfg1"],
                     default="full",
                     help="stencil: only the evaluation timings plus a 256 MiB copy (the calibration "
                          "kernel for rocprofv3 byte counters); converge: only time-to-converge")
@@ -730,6 +760,14 @@ def main():
         del x64, x32, g64, g32
         torch.cuda.empty_cache()
         res = pfu_leg(m, x0, steps=args.steps, warmup=args.warmup)
+        log(json.dumps(res, indent=1))
+        print(json.dumps(res), flush=True)
+        return
+
+    if args.mode == "cfg1":
+        del x64, x32, g64, g32
+        torch.cuda.empty_cache()
+        res = cfg1_leg(m, with_cpu=not args.no_cpu)
         log(json.dumps(res, indent=1))
         print(json.dumps(res), flush=True)
         return

@@ -125,7 +125,11 @@ class Lbfgs : public NewMinimiser<Lbfgs> {
   // replayed, and the host reads one counter every 8 iterations; "device":
   // the same sequence launched without the graph; "host": the decisions on
   // the host, one host synchronisation per phase.  All three give the same
-  // results bit for bit.
+  // results bit for bit.  "resident" (FP64 schedule only): one workgroup per
+  // instance runs that instance's whole minimisation in a single launch, the
+  // line search included, with no launches or host synchronisations between
+  // iterations; its sums are split differently, so its results agree with
+  // the others to rounding.
   Lbfgs& setBatchControl(std::string mode);
   // Under device control ("graph" or "device"): instances are admitted
   // smallest first while the running ones hold at most this many elements

@@ -791,9 +791,11 @@ class Lbfgs(Minimiser):
 
     def setBatchControl(self, mode):
         """Lbfgs::setBatchControl: "graph" (device decisions, HIP-graph-captured
-        iterations), "device" (the same without the graph) or "host"."""
-        if mode not in ("graph", "device", "host"):
-            raise ValueError("Lbfgs: batch control must be graph, device or host.")
+        iterations), "device" (the same without the graph), "host", or
+        "resident" (one workgroup runs each instance's whole minimisation in
+        one launch; FP64 only)."""
+        if mode not in ("graph", "device", "host", "resident"):
+            raise ValueError("Lbfgs: batch control must be graph, device, host or resident.")
         self.batchControl = mode
         self._h = None
         return self
@@ -909,7 +911,7 @@ class Lbfgs(Minimiser):
         p.init_step = self.initStep
         p.precision_mode = {"state": 0, "fp64": 1, "fp32": 2, "mixed": 3}[self.precisionMode]
         p.switch_rms = self.switchRms
-        p.batch_control = {"graph": 0, "device": 1, "host": 2}[self.batchControl]
+        p.batch_control = {"graph": 0, "device": 1, "host": 2, "resident": 3}[self.batchControl]
         p.batch_budget = self.batchBudget
         p.batch_block = self.batchBlock
         return p

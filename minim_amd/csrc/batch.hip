@@ -196,10 +196,27 @@ __global__ __launch_bounds__(kBlock) void bfinal_kernel(const double* __restrict
 // MODE bits: 1 energy, 2 gradient into G[gout] with g.g, 4 at x + tau * d,
 // 8 also the dots of the gradient with every stored pair, g.D_i and g.Y_i
 // (partials [2, 2 + kBS) and [2 + kBS, 2 + 2 kBS)).
+template <int MODE>
+constexpr int bpf_k() {
+  return (MODE & 8) ? kKF : kKP;
+}
+// The partials a bpf pass sums (bit k: partial k).
+template <int MODE>
+__device__ __forceinline__ unsigned long long bpf_mask(const BDyn& q) {
+  unsigned long long on = (MODE & 1 ? 1ull : 0ull) | (MODE & 2 ? 2ull : 0ull);
+  if (MODE & 8) {
+    const unsigned long long pm = (1ull << q.np) - 1;
+    on |= (pm << kKP) | (pm << (kKP + kBS));
+  }
+  return on;
+}
+
+// The pass bodies add the contributions of this thread's elements of block
+// lb of the instance to acc (the kernels then sum acc over the block).
 template <int BR, int MODE, typename T, typename TV>
-__device__ __forceinline__ void bpf_body(const BBufs& B, const BDyn& q, const BInst& I, int lb) {
+__device__ __forceinline__ void bpf_body(const BBufs& B, const BDyn& q, const BInst& I, int lb,
+                                         double (&acc)[bpf_k<MODE>()]) {
   using C = BC<T>;
-  constexpr int K = (MODE & 8) ? kKF : kKP;
   const size_t N = B.N;
   const double* __restrict__ x = (q.xcur ? B.X1 : B.X0) + I.off;
   const TV* __restrict__ d = static_cast<const TV*>(B.D) + (size_t)q.dslot * B.N + I.off;
@@ -214,9 +231,6 @@ __device__ __forceinline__ void bpf_body(const BBufs& B, const BDyn& q, const BI
     }
     return xv;
   };
-  double acc[K];
-#pragma unroll
-  for (int k = 0; k < K; k++) acc[k] = 0;
 #pragma unroll
   for (int r = 0; r < BR; r++) {
     const int i = lb * (BR * kBlock) + r * kBlock + (int)threadIdx.x;
@@ -260,12 +274,6 @@ __device__ __forceinline__ void bpf_body(const BBufs& B, const BDyn& q, const BI
       }
     }
   }
-  unsigned long long on = (MODE & 1 ? 1ull : 0ull) | (MODE & 2 ? 2ull : 0ull);
-  if (MODE & 8) {
-    const unsigned long long pm = (1ull << q.np) - 1;
-    on |= (pm << kKP) | (pm << (kKP + kBS));
-  }
-  block_partials<K>(acc, on, B.part);
 }
 
 template <int BR, int MODE, typename TV>
@@ -275,8 +283,13 @@ __global__ __launch_bounds__(kBlock) void bpf_kernel(BBufs B) {
   MINIM_BATCH_SKIP(B, q);
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
-  if (q.lp) bpf_body<BR, MODE, float, TV>(B, q, I, lb);
-  else bpf_body<BR, MODE, double, TV>(B, q, I, lb);
+  constexpr int K = bpf_k<MODE>();
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) acc[k] = 0;
+  if (q.lp) bpf_body<BR, MODE, float, TV>(B, q, I, lb, acc);
+  else bpf_body<BR, MODE, double, TV>(B, q, I, lb, acc);
+  block_partials<K>(acc, bpf_mask<MODE>(q), B.part);
 }
 
 // Energies at kNT points x + tau_k d along the line-search sequence, tau_k =
@@ -284,7 +297,8 @@ __global__ __launch_bounds__(kBlock) void bpf_kernel(BBufs B) {
 // once for all of them, and the host takes the first point that passes the
 // Armijo test, exactly as trying them one after another would.
 template <int BR, typename T, typename TV>
-__device__ __forceinline__ void btrial_body(const BBufs& B, const BDyn& q, const BInst& I, int lb) {
+__device__ __forceinline__ void btrial_body(const BBufs& B, const BDyn& q, const BInst& I, int lb,
+                                            double (&acc)[kNT]) {
   using C = BC<T>;
   const double* __restrict__ x = (q.xcur ? B.X1 : B.X0) + I.off;
   const TV* __restrict__ d = static_cast<const TV*>(B.D) + (size_t)q.dslot * B.N + I.off;
@@ -293,9 +307,6 @@ __device__ __forceinline__ void btrial_body(const BBufs& B, const BDyn& q, const
 #pragma unroll
   for (int k = 1; k < kNT; k++) tk[k] = tk[k - 1] * (T)0.5;
   const int Z = I.Z, Y = I.Y, X = I.X, YZ = Y * Z;
-  double acc[kNT];
-#pragma unroll
-  for (int k = 0; k < kNT; k++) acc[k] = 0;
   for (int r = 0; r < BR; r++) {
     const int i = lb * (BR * kBlock) + r * kBlock + (int)threadIdx.x;
     if (i >= I.n) break;
@@ -319,7 +330,6 @@ __device__ __forceinline__ void btrial_body(const BBufs& B, const BDyn& q, const
       acc[k] += (double)fma(C::fg(I), sq, be);
     }
   }
-  block_partials<kNT>(acc, (1ull << kNT) - 1, B.part);
 }
 
 template <int BR, typename TV>
@@ -329,8 +339,12 @@ __global__ __launch_bounds__(kBlock) void btrial_kernel(BBufs B) {
   MINIM_BATCH_SKIP(B, q);
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
-  if (q.lp) btrial_body<BR, float, TV>(B, q, I, lb);
-  else btrial_body<BR, double, TV>(B, q, I, lb);
+  double acc[kNT];
+#pragma unroll
+  for (int k = 0; k < kNT; k++) acc[k] = 0;
+  if (q.lp) btrial_body<BR, float, TV>(B, q, I, lb, acc);
+  else btrial_body<BR, double, TV>(B, q, I, lb, acc);
+  block_partials<kNT>(acc, (1ull << kNT) - 1, B.part);
 }
 
 // x rounded to FP32 (the mixed schedule's first phase keeps FP32 coordinates).
@@ -353,25 +367,35 @@ __global__ __launch_bounds__(kBlock) void bround_kernel(BBufs B) {
 // d = cg g + sum_i cD[i] D_{slot i} + cY[i] Y_{slot i} (pairs oldest first, as
 // the single-State direction pass) into D[dslot]; partials d.d, d.g, d.y_i.
 // P is the number of stored pairs, so every load is unconditional.
+// A value that is the same in every lane of the wave (a block works on one
+// instance, so its BDyn fields are), moved to a scalar register: the pointer
+// arithmetic and coefficients of the passes then take no vector registers.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double uni(double v) {
+  const unsigned long long b = __double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 template <int BR, int P, typename TV>
-__device__ __forceinline__ void bdir_body(const BBufs& B, const BDyn& q, const BInst& I, int lb) {
+__device__ __forceinline__ void bdir_body(const BBufs& B, const BDyn& q, const BInst& I, int lb,
+                                          double (&acc)[kKD]) {
   const size_t N = B.N;
-  const TV* __restrict__ g = static_cast<const TV*>(q.gcur ? B.G1 : B.G0) + I.off;
-  TV* __restrict__ out = static_cast<TV*>(B.D) + (size_t)q.dslot * N + I.off;
+  const TV* __restrict__ g = static_cast<const TV*>(uni(q.gcur) ? B.G1 : B.G0) + I.off;
+  TV* __restrict__ out = static_cast<TV*>(B.D) + (size_t)uni(q.dslot) * N + I.off;
   const TV* Dp[P > 0 ? P : 1];
   const TV* Yp[P > 0 ? P : 1];
   double cd[P > 0 ? P : 1], cy[P > 0 ? P : 1];
 #pragma unroll
   for (int k = 0; k < P; k++) {
-    Dp[k] = static_cast<const TV*>(B.D) + (size_t)q.slot[k] * N + I.off;
-    Yp[k] = static_cast<const TV*>(B.Y) + (size_t)q.slot[k] * N + I.off;
-    cd[k] = q.cD[k];
-    cy[k] = q.cY[k];
+    const int sk = uni(q.slot[k]);
+    Dp[k] = static_cast<const TV*>(B.D) + (size_t)sk * N + I.off;
+    Yp[k] = static_cast<const TV*>(B.Y) + (size_t)sk * N + I.off;
+    cd[k] = uni(q.cD[k]);
+    cy[k] = uni(q.cY[k]);
   }
-  const double cg = q.cg;
-  double acc[kKD];
-#pragma unroll
-  for (int k = 0; k < kKD; k++) acc[k] = 0;
+  const double cg = uni(q.cg);
 #pragma unroll 2
   for (int r = 0; r < BR; r++) {
     const int i = lb * (BR * kBlock) + r * kBlock + (int)threadIdx.x;
@@ -396,8 +420,25 @@ __device__ __forceinline__ void bdir_body(const BBufs& B, const BDyn& q, const B
 #pragma unroll
     for (int k = 0; k < P; k++) acc[2 + k] += v * yv[k];
   }
-  constexpr unsigned long long on = (1ull << (2 + P)) - 1;
-  block_partials<kKD>(acc, on, B.part);
+}
+
+// The direction pass with the number of stored pairs as a compile-time value.
+template <int BR, typename TV>
+__device__ __forceinline__ void bdir_any(const BBufs& B, const BDyn& q, const BInst& I, int lb,
+                                         double (&acc)[kKD]) {
+  switch (q.np) {
+    case 0: bdir_body<BR, 0, TV>(B, q, I, lb, acc); break;
+    case 1: bdir_body<BR, 1, TV>(B, q, I, lb, acc); break;
+    case 2: bdir_body<BR, 2, TV>(B, q, I, lb, acc); break;
+    case 3: bdir_body<BR, 3, TV>(B, q, I, lb, acc); break;
+    case 4: bdir_body<BR, 4, TV>(B, q, I, lb, acc); break;
+    case 5: bdir_body<BR, 5, TV>(B, q, I, lb, acc); break;
+    case 6: bdir_body<BR, 6, TV>(B, q, I, lb, acc); break;
+    case 7: bdir_body<BR, 7, TV>(B, q, I, lb, acc); break;
+    case 8: bdir_body<BR, 8, TV>(B, q, I, lb, acc); break;
+    case 9: bdir_body<BR, 9, TV>(B, q, I, lb, acc); break;
+    default: bdir_body<BR, 10, TV>(B, q, I, lb, acc); break;
+  }
 }
 
 template <int BR, typename TV>
@@ -407,19 +448,11 @@ __global__ __launch_bounds__(kBlock) void bdir_kernel(BBufs B) {
   MINIM_BATCH_SKIP(B, q);
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
-  switch (q.np) {
-    case 0: bdir_body<BR, 0, TV>(B, q, I, lb); break;
-    case 1: bdir_body<BR, 1, TV>(B, q, I, lb); break;
-    case 2: bdir_body<BR, 2, TV>(B, q, I, lb); break;
-    case 3: bdir_body<BR, 3, TV>(B, q, I, lb); break;
-    case 4: bdir_body<BR, 4, TV>(B, q, I, lb); break;
-    case 5: bdir_body<BR, 5, TV>(B, q, I, lb); break;
-    case 6: bdir_body<BR, 6, TV>(B, q, I, lb); break;
-    case 7: bdir_body<BR, 7, TV>(B, q, I, lb); break;
-    case 8: bdir_body<BR, 8, TV>(B, q, I, lb); break;
-    case 9: bdir_body<BR, 9, TV>(B, q, I, lb); break;
-    default: bdir_body<BR, 10, TV>(B, q, I, lb); break;
-  }
+  double acc[kKD];
+#pragma unroll
+  for (int k = 0; k < kKD; k++) acc[k] = 0;
+  bdir_any<BR, TV>(B, q, I, lb, acc);
+  block_partials<kKD>(acc, (1ull << (2 + q.np)) - 1, B.part);
 }
 
 // The step, gradient and Gram passes of an iteration in one: x' = x + tau d
@@ -435,31 +468,31 @@ __global__ __launch_bounds__(kBlock) void bdir_kernel(BBufs B) {
 // Every value is the one the separate passes would compute, and each partial
 // sums the same terms in the same order.
 template <int BR, int P, typename T, typename TV>
-__device__ __forceinline__ void bstep_body(const BBufs& B, const BDyn& q, const BInst& I, int lb) {
+__device__ __forceinline__ void bstep_body(const BBufs& B, const BDyn& q, const BInst& I, int lb,
+                                           double (&acc)[kKS]) {
   using C = BC<T>;
   const size_t N = B.N;
-  const double* __restrict__ x = (q.xcur ? B.X1 : B.X0) + I.off;
-  double* __restrict__ xn = (q.xcur ? B.X0 : B.X1) + I.off;
-  const TV* __restrict__ d = static_cast<const TV*>(B.D) + (size_t)q.dslot * N + I.off;
-  TV* __restrict__ gn = static_cast<TV*>(q.gout ? B.G1 : B.G0) + I.off;
-  const TV* __restrict__ go = static_cast<const TV*>(q.gcur ? B.G1 : B.G0) + I.off;
-  TV* __restrict__ yo = static_cast<TV*>(B.Y) + (size_t)q.dslot * N + I.off;
+  const int xc = uni(q.xcur), ds = uni(q.dslot);
+  const double* __restrict__ x = (xc ? B.X1 : B.X0) + I.off;
+  double* __restrict__ xn = (xc ? B.X0 : B.X1) + I.off;
+  const TV* __restrict__ d = static_cast<const TV*>(B.D) + (size_t)ds * N + I.off;
+  TV* __restrict__ gn = static_cast<TV*>(uni(q.gout) ? B.G1 : B.G0) + I.off;
+  const TV* __restrict__ go = static_cast<const TV*>(uni(q.gcur) ? B.G1 : B.G0) + I.off;
+  TV* __restrict__ yo = static_cast<TV*>(B.Y) + (size_t)ds * N + I.off;
   const TV* Dp[P > 0 ? P : 1];
   const TV* Yp[P > 0 ? P : 1];
 #pragma unroll
   for (int k = 0; k < P; k++) {
-    Dp[k] = static_cast<const TV*>(B.D) + (size_t)q.slot[k] * N + I.off;
-    Yp[k] = static_cast<const TV*>(B.Y) + (size_t)q.slot[k] * N + I.off;
+    const int sk = uni(q.slot[k]);
+    Dp[k] = static_cast<const TV*>(B.D) + (size_t)sk * N + I.off;
+    Yp[k] = static_cast<const TV*>(B.Y) + (size_t)sk * N + I.off;
   }
-  const T tau = (T)q.tau;
+  const T tau = (T)uni(q.tau);
   auto val = [&](int i) -> T {
     const T st = tau * (T)d[i];
     return (T)x[i] + st;
   };
   constexpr int o = 1;  // Gram block offset
-  double acc[kKS];
-#pragma unroll
-  for (int k = 0; k < kKS; k++) acc[k] = 0;
   const int Z = I.Z, Y = I.Y, X = I.X, YZ = Y * Z;
 #pragma unroll 2
   for (int r = 0; r < BR; r++) {
@@ -512,16 +545,40 @@ __device__ __forceinline__ void bstep_body(const BBufs& B, const BDyn& q, const 
     acc[o + 2 * kBS + 1] += a * yv;
     acc[o + 4 * kBS + 2] += yv * yv;
   }
-  constexpr unsigned long long dm = (1ull << (P + 1)) - 1, ym = (1ull << P) - 1;
-  constexpr unsigned long long gon = dm | (ym << kBS) | (3ull << (2 * kBS)) | (dm << (2 * kBS + 2)) |
-                                     (ym << (3 * kBS + 2)) | (1ull << (4 * kBS + 2));
-  block_partials<kKS>(acc, 1ull | (gon << o), B.part);
+}
+
+// The partials a step pass with P stored pairs sums (bit k: partial k).
+__device__ __forceinline__ unsigned long long bstep_mask(int P) {
+  const unsigned long long dm = (1ull << (P + 1)) - 1, ym = (1ull << P) - 1;
+  const unsigned long long gon = dm | (ym << kBS) | (3ull << (2 * kBS)) | (dm << (2 * kBS + 2)) |
+                                 (ym << (3 * kBS + 2)) | (1ull << (4 * kBS + 2));
+  return 1ull | (gon << 1);
 }
 
 template <int BR, int P, typename TV>
-__device__ __forceinline__ void bstep_prec(const BBufs& B, const BDyn& q, const BInst& I, int lb) {
-  if (q.lp) bstep_body<BR, P, float, TV>(B, q, I, lb);
-  else bstep_body<BR, P, double, TV>(B, q, I, lb);
+__device__ __forceinline__ void bstep_prec(const BBufs& B, const BDyn& q, const BInst& I, int lb,
+                                           double (&acc)[kKS]) {
+  if (q.lp) bstep_body<BR, P, float, TV>(B, q, I, lb, acc);
+  else bstep_body<BR, P, double, TV>(B, q, I, lb, acc);
+}
+
+// The step pass with the number of stored pairs as a compile-time value.
+template <int BR, typename TV>
+__device__ __forceinline__ void bstep_any(const BBufs& B, const BDyn& q, const BInst& I, int lb,
+                                          double (&acc)[kKS]) {
+  switch (q.np) {
+    case 0: bstep_prec<BR, 0, TV>(B, q, I, lb, acc); break;
+    case 1: bstep_prec<BR, 1, TV>(B, q, I, lb, acc); break;
+    case 2: bstep_prec<BR, 2, TV>(B, q, I, lb, acc); break;
+    case 3: bstep_prec<BR, 3, TV>(B, q, I, lb, acc); break;
+    case 4: bstep_prec<BR, 4, TV>(B, q, I, lb, acc); break;
+    case 5: bstep_prec<BR, 5, TV>(B, q, I, lb, acc); break;
+    case 6: bstep_prec<BR, 6, TV>(B, q, I, lb, acc); break;
+    case 7: bstep_prec<BR, 7, TV>(B, q, I, lb, acc); break;
+    case 8: bstep_prec<BR, 8, TV>(B, q, I, lb, acc); break;
+    case 9: bstep_prec<BR, 9, TV>(B, q, I, lb, acc); break;
+    default: bstep_prec<BR, 10, TV>(B, q, I, lb, acc); break;
+  }
 }
 
 template <int BR, typename TV>
@@ -531,19 +588,11 @@ __global__ __launch_bounds__(kBlock) void bstep_kernel(BBufs B) {
   MINIM_BATCH_SKIP(B, q);
   const BInst I = B.inst[q.j];
   const int lb = blockIdx.x - B.ablk[p];
-  switch (q.np) {
-    case 0: bstep_prec<BR, 0, TV>(B, q, I, lb); break;
-    case 1: bstep_prec<BR, 1, TV>(B, q, I, lb); break;
-    case 2: bstep_prec<BR, 2, TV>(B, q, I, lb); break;
-    case 3: bstep_prec<BR, 3, TV>(B, q, I, lb); break;
-    case 4: bstep_prec<BR, 4, TV>(B, q, I, lb); break;
-    case 5: bstep_prec<BR, 5, TV>(B, q, I, lb); break;
-    case 6: bstep_prec<BR, 6, TV>(B, q, I, lb); break;
-    case 7: bstep_prec<BR, 7, TV>(B, q, I, lb); break;
-    case 8: bstep_prec<BR, 8, TV>(B, q, I, lb); break;
-    case 9: bstep_prec<BR, 9, TV>(B, q, I, lb); break;
-    default: bstep_prec<BR, 10, TV>(B, q, I, lb); break;
-  }
+  double acc[kKS];
+#pragma unroll
+  for (int k = 0; k < kKS; k++) acc[k] = 0;
+  bstep_any<BR, TV>(B, q, I, lb, acc);
+  block_partials<kKS>(acc, bstep_mask(q.np), B.part);
 }
 
 // ---------------------------------------------------------------- device control
@@ -717,7 +766,9 @@ __device__ void bc_direction(const BCtlArgs& A, int j, BCtl& C) {
     const double gnorm = sqrt(C.gg);
     D.cg = (gnorm > 0) ? -(A.initStep / gnorm) : -1.0;
   } else {
-    double alpha[kBS], ag_s[kBS], ay[kBS];
+    // in LDS (the first thread of the block runs this): indexed by the
+    // run-time pair count, in registers they would take a block of them
+    __shared__ double alpha[kBS], ag_s[kBS], ay[kBS];
     for (int i = 0; i < P; i++) ag_s[i] = ay[i] = 0.0;
     double ag = -1.0;
     for (int k = P - 1; k >= 0; k--) {
@@ -755,6 +806,8 @@ __device__ void bc_direction(const BCtlArgs& A, int j, BCtl& C) {
 // The direction's dots: maxStep cap and flip (Lbfgs.cpp:53-57, :121-124), the
 // Armijo slope, and the speculative trials at x + d (linesearch.cpp:8-29).
 // The instance then takes its step, or searches.
+__device__ void bc_after_dir(const BCtlArgs& A, BCtl& C, BDyn& D, const double* r, const double* rt);
+
 __global__ __launch_bounds__(kBlock) void bc_after_dir_kernel(BCtlArgs A) {
   __shared__ double r[kKD], rt[kNT];
   const int p = blockIdx.x, j = A.map[p];
@@ -765,7 +818,11 @@ __global__ __launch_bounds__(kBlock) void bc_after_dir_kernel(BCtlArgs A) {
   reduce_rows(A.part, b0, nb, kKD, r);
   if (A.spec) reduce_rows(A.part2, b0, nb, kNT, rt);
   __syncthreads();
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x == 0) bc_after_dir(A, C, D, r, rt);
+}
+
+// r: the direction's partial sums, rt: the speculative trials' energies.
+__device__ void bc_after_dir(const BCtlArgs& A, BCtl& C, BDyn& D, const double* r, const double* rt) {
   const double dd = r[0], dgv = r[1];
   const int P = C.np;
   for (int i = 0; i < P; i++) C.dres[C.slot[i]] = r[2 + i];
@@ -804,6 +861,15 @@ __global__ __launch_bounds__(kBlock) void bc_after_dir_kernel(BCtlArgs A) {
 
 // The next kNT trials of the instances searching; those whose search ends
 // take their step in this iteration, the others search on in the next.
+__device__ __forceinline__ void bc_consume(BCtl& C, BDyn& D, const double* rt) {
+  const bool ended = bc_trials(C, rt);
+  D.tau = C.tau;
+  if (ended) {
+    C.searching = 0;
+    D.act = kActStep;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void bc_consume_kernel(BCtlArgs A) {
   __shared__ double rt[kNT];
   const int p = blockIdx.x, j = A.map[p];
@@ -812,13 +878,7 @@ __global__ __launch_bounds__(kBlock) void bc_consume_kernel(BCtlArgs A) {
   if (!(D.act & kActSearch)) return;
   reduce_rows(A.part, A.ablk[p], A.ablk[p + 1] - A.ablk[p], kNT, rt);
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  const bool ended = bc_trials(C, rt);
-  D.tau = C.tau;
-  if (ended) {
-    C.searching = 0;
-    D.act = kActStep;
-  }
+  if (threadIdx.x == 0) bc_consume(C, D, rt);
 }
 
 // New energy and the Gram update (Lbfgs.cpp:58-80), the mixed schedule's
@@ -858,7 +918,7 @@ __device__ void bc_update(const BCtlArgs& A, int j, BCtl& C, const double* rsum)
     const double yyv = r[4 * kBS + 2];
     const int ds = C.dslot;
     const int P = C.np;
-    int oldSlots[kBS];
+    __shared__ int oldSlots[kBS];  // as alpha in bc_direction
     for (int i = 0; i < P; i++) oldSlots[i] = C.slot[i];
     const double sy = C.tau * r[2 * kBS + 2 + P];
     if (sy != 0) {
@@ -921,6 +981,149 @@ __device__ void bc_update(const BCtlArgs& A, int j, BCtl& C, const double* rsum)
   }
 }
 
+// ---------------------------------------------------------------- resident instances
+// Resident control: one workgroup per instance runs that instance's whole
+// minimisation in one launch -- the same pass bodies, taken over the
+// instance's blocks in turn (each thread adds its elements' contributions in
+// a fixed order, then the workgroup sums them as block_partials does), and
+// the same decision functions, with workgroup barriers in place of kernel
+// boundaries.  The line search loops until it ends, so an iteration costs no
+// launch and no host synchronisation; small instances finish in the time
+// their own iterations take, instead of waiting for launches sized for the
+// whole batch.  Every instance of the batch is in flight at once (or as soon
+// as a workgroup slot frees), largest first.  Sums are split differently
+// from the block-per-launch passes, so results agree with the other controls
+// to rounding.
+
+// out[k] = the workgroup's sum of v[k] for the bits of `on` (the tree of
+// block_partials), visible to every thread on return.
+template <int K>
+__device__ __forceinline__ void wg_sum(const double (&v)[K], unsigned long long on, double* out) {
+  __shared__ double lds[(kBlock / kWave) * K];
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    double x = 0;
+    if ((on >> k) & 1ull) {
+      x = v[k];
+#pragma unroll
+      for (int off = kWave / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, kWave);
+    }
+    if (lane == 0) lds[wid * K + k] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < K) {
+    const int k = threadIdx.x;
+    double s = lds[k];
+    for (int w = 1; w < kBlock / kWave; w++) s += lds[w * K + k];
+    out[k] = s;
+  }
+  __syncthreads();
+}
+
+// A whole pass over the instance's blocks, the number of stored pairs fixed
+// outside the loop (FP64 arithmetic).
+template <int BR, int P, typename TV>
+__device__ __forceinline__ void bres_dir(const BBufs& B, const BDyn& D, const BInst& I, int nblk,
+                                         double (&acc)[kKD]) {
+#pragma unroll 1
+  for (int lb = 0; lb < nblk; lb++) bdir_body<BR, P, TV>(B, D, I, lb, acc);
+}
+template <int BR, int P, typename TV>
+__device__ __forceinline__ void bres_step(const BBufs& B, const BDyn& D, const BInst& I, int nblk,
+                                          double (&acc)[kKS]) {
+#pragma unroll 1
+  for (int lb = 0; lb < nblk; lb++) bstep_body<BR, P, double, TV>(B, D, I, lb, acc);
+}
+// D.np <= PM: the cases above PM are not compiled, so the kernel's registers
+// are those of the passes with at most PM pairs.
+#define MINIM_BRES_CASE(FN, K, ...) \
+  case K:                           \
+    if constexpr (K < PM) {         \
+      FN<BR, K, TV>(__VA_ARGS__);   \
+      break;                        \
+    }                               \
+    [[fallthrough]];
+#define MINIM_BRES_SWITCH(FN, ...)               \
+  switch (D.np) {                                \
+    MINIM_BRES_CASE(FN, 0, __VA_ARGS__)          \
+    MINIM_BRES_CASE(FN, 1, __VA_ARGS__)          \
+    MINIM_BRES_CASE(FN, 2, __VA_ARGS__)          \
+    MINIM_BRES_CASE(FN, 3, __VA_ARGS__)          \
+    MINIM_BRES_CASE(FN, 4, __VA_ARGS__)          \
+    MINIM_BRES_CASE(FN, 5, __VA_ARGS__)          \
+    MINIM_BRES_CASE(FN, 6, __VA_ARGS__)          \
+    MINIM_BRES_CASE(FN, 7, __VA_ARGS__)          \
+    MINIM_BRES_CASE(FN, 8, __VA_ARGS__)          \
+    MINIM_BRES_CASE(FN, 9, __VA_ARGS__)          \
+    default: FN<BR, PM, TV>(__VA_ARGS__); break; \
+  }
+
+template <int BR, int PM, typename TV>
+__global__ __launch_bounds__(kBlock) void bres_kernel(BBufs B, BCtlArgs A) {
+  __shared__ BCtl Cs;
+  __shared__ double rsum[kKS], rt[kNT];
+  const int j = A.map[blockIdx.x];
+  const BInst I = B.inst[j];
+  BDyn& D = A.dyn[j];
+  const int nblk = (I.n + BR * kBlock - 1) / (BR * kBlock);
+  ctl_copy(&Cs, A.ctl + j);
+  __syncthreads();
+  {  // iteration 0: gradient and energy at the initial coordinates
+    double acc[kKP] = {0.0, 0.0};
+#pragma unroll 1
+    for (int lb = 0; lb < nblk; lb++) bpf_body<BR, 3, double, TV>(B, D, I, lb, acc);
+    wg_sum<kKP>(acc, 3ull, rsum);
+    if (threadIdx.x == 0) {
+      Cs.e = rsum[0];
+      Cs.gg = rsum[1];
+      Cs.gcur = 0;
+      Cs.nGradient++;
+      Cs.started = 1;
+    }
+    __syncthreads();
+  }
+  for (;;) {
+    if (threadIdx.x == 0) bc_direction(A, j, Cs);
+    __syncthreads();
+    if (!(D.act & kActDir)) break;  // finished
+    {
+      double acc[kKD];
+#pragma unroll
+      for (int k = 0; k < kKD; k++) acc[k] = 0;
+      MINIM_BRES_SWITCH(bres_dir, B, D, I, nblk, acc)
+      wg_sum<kKD>(acc, (1ull << (2 + D.np)) - 1, rsum);
+      if (A.spec) {  // D.tau is 1: the speculative trials at x + 2^-k d
+        double at[kNT] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+        for (int lb = 0; lb < nblk; lb++) btrial_body<BR, double, TV>(B, D, I, lb, at);
+        wg_sum<kNT>(at, (1ull << kNT) - 1, rt);
+      }
+      if (threadIdx.x == 0) bc_after_dir(A, Cs, D, rsum, rt);
+      __syncthreads();
+    }
+    while (D.act & kActSearch) {
+      double at[kNT] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+      for (int lb = 0; lb < nblk; lb++) btrial_body<BR, double, TV>(B, D, I, lb, at);
+      wg_sum<kNT>(at, (1ull << kNT) - 1, rt);
+      if (threadIdx.x == 0) bc_consume(Cs, D, rt);
+      __syncthreads();
+    }
+    {
+      double acc[kKS];
+#pragma unroll
+      for (int k = 0; k < kKS; k++) acc[k] = 0;
+      MINIM_BRES_SWITCH(bres_step, B, D, I, nblk, acc)
+      wg_sum<kKS>(acc, bstep_mask(D.np), rsum);
+      if (threadIdx.x == 0) bc_update(A, j, Cs, rsum);
+      __syncthreads();
+    }
+    if (Cs.done) break;
+  }
+  ctl_copy(A.ctl + j, &Cs);
+}
+
 // Host side ------------------------------------------------------------------
 
 struct Pair {
@@ -959,11 +1162,18 @@ struct Inst {
 using KFn = void (*)(BBufs);
 struct Kernels {
   KFn pf_eg, pf_trial, dir, step, fresh, round;
+  void (*resident5)(BBufs, BCtlArgs);   // FP64 only, m <= 5
+  void (*resident10)(BBufs, BCtlArgs);  // FP64 only, m <= 10
 };
 template <int BR, typename TV>
 Kernels kernels_br() {
-  return Kernels{bpf_kernel<BR, 3, TV>,  btrial_kernel<BR, TV>,     bdir_kernel<BR, TV>,
-                 bstep_kernel<BR, TV>,   bpf_kernel<BR, 11, TV>,    bround_kernel<BR>};
+  Kernels k{bpf_kernel<BR, 3, TV>, btrial_kernel<BR, TV>, bdir_kernel<BR, TV>, bstep_kernel<BR, TV>,
+            bpf_kernel<BR, 11, TV>, bround_kernel<BR>, nullptr, nullptr};
+  if constexpr (sizeof(TV) == 8) {
+    k.resident5 = bres_kernel<BR, 5, TV>;
+    k.resident10 = bres_kernel<BR, 10, TV>;
+  }
+  return k;
 }
 template <typename TV>
 Kernels kernels_for(int br) {
@@ -1031,6 +1241,9 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
   if (_m > kBS - 1) throw std::invalid_argument("Lbfgs::minimiseBatch: m must be at most 10.");
   if (_precMode == "fp32")
     throw std::invalid_argument("Lbfgs::minimiseBatch runs in FP64 or the mixed schedule, not in FP32 throughout.");
+  const bool resident = _batchControl == "resident";
+  if (resident && _precMode == "mixed")
+    throw std::invalid_argument("Lbfgs::minimiseBatch: resident control runs the FP64 schedule only.");
   // mixed: FP32 coordinates and arithmetic until sqrt(g.g/ndof) < switchRms (or
   // the line search stalls twice), then FP64 coordinates; the vectors stay in
   // FP32 throughout, as the single-State schedule does for a PhaseField
@@ -1334,6 +1547,25 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
       captures++;
     };
 
+    long replays = 0;
+    if (resident) {
+      // every instance in one launch, one workgroup each, largest first
+      std::vector<int> order(ni);
+      for (int j = 0; j < ni; j++) order[j] = j;
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return in[a].bi.n > in[b].bi.n; });
+      MINIM_HIP_CHECK(hipMemcpyAsync(mapA.get(), order.data(), sizeof(int) * ni, hipMemcpyHostToDevice, cs));
+      bc_admit_kernel<<<1, 1, 0, cs>>>(A, nullptr, 0, 1);  // the start clock
+      MINIM_HIP_CHECK(hipGetLastError());
+      auto* kres = _m <= 5 ? K.resident5 : K.resident10;
+      if (!kres) throw std::logic_error("Lbfgs::minimiseBatch: no resident kernel for this storage type.");
+      BBufs R = G;
+      R.map = nullptr;
+      R.ablk = nullptr;
+      kres<<<ni, kBlock, 0, cs>>>(R, A);
+      MINIM_HIP_CHECK(hipGetLastError());
+      MINIM_HIP_CHECK(hipStreamSynchronize(cs));  // order leaves scope
+      hostSyncs++;
+    } else {
     // admission order: ascending size
     std::vector<int> order(ni);
     for (int j = 0; j < ni; j++) order[j] = j;
@@ -1379,7 +1611,7 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
     };
     start(admit());
     regrid();
-    long replays = 0, lastAdmission = 0;
+    long lastAdmission = 0;
     for (;;) {
       for (int k = 0; k < kChunk; k++) {
         if (useGraph) {
@@ -1416,6 +1648,7 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
         regrid();
       }
     }
+    }  // resident or graph / device
     batchReport.captures = captures;
     std::vector<BCtl> cf(ni);
     unsigned long long clk0 = 0;
@@ -1443,7 +1676,7 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
       if (I.iters > maxIters) maxIters = I.iters;
     }
     batchReport.iterations = maxIters;
-    batchReport.replays = replays;
+    batchReport.replays = resident ? maxIters : replays;
   } else {
 
   // ---- iteration 0's gradient (and energy) at the initial coordinates

@@ -762,8 +762,8 @@ int minim_minimiser_create(const minim_minimiser_params* p, minim_minimiser** ou
       l->setMaxStep(p->max_step);
       l->setInitStep(p->init_step);
       l->setPrecision(modes[p->precision_mode], p->switch_rms);
-      static const char* controls[] = {"graph", "device", "host"};
-      if (p->batch_control < 0 || p->batch_control > 2) throw std::invalid_argument("invalid batch control");
+      static const char* controls[] = {"graph", "device", "host", "resident"};
+      if (p->batch_control < 0 || p->batch_control > 3) throw std::invalid_argument("invalid batch control");
       l->setBatchControl(controls[p->batch_control]);
       l->setBatchBudget(p->batch_budget);
       l->setBatchBlock(p->batch_block);

@@ -346,8 +346,8 @@ Lbfgs& Lbfgs::setInitStep(double v) {
   return *this;
 }
 Lbfgs& Lbfgs::setBatchControl(std::string mode) {
-  if (mode != "graph" && mode != "device" && mode != "host")
-    throw std::invalid_argument("Lbfgs: batch control must be graph, device or host.");
+  if (mode != "graph" && mode != "device" && mode != "host" && mode != "resident")
+    throw std::invalid_argument("Lbfgs: batch control must be graph, device, host or resident.");
   _batchControl = mode;
   return *this;
 }

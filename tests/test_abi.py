@@ -235,3 +235,30 @@ def test_batch_entry_point_errors():
     assert b"budget" in L.minim_last_error()
     with pytest.raises(ValueError):
         __import__("minim_amd").Lbfgs().setBatchBudget(-1)
+
+
+def test_batch_control_values():
+    """batch_control 0..3 (graph, device, host, resident) are accepted and
+    anything else is rejected, in the C ABI and the Python mirror."""
+    from minim_amd import _capi as A
+
+    import minim_amd as m
+
+    L = A.load()
+    p = A.MinimiserParams()
+    p.kind = A.MINIM_LBFGS
+    p.max_iter = 10
+    p.m = 5
+    p.init_step = 1.0
+    h = C.c_void_p()
+    for v in range(4):
+        p.batch_control = v
+        assert L.minim_minimiser_create(C.byref(p), C.byref(h)) == 0, v
+        L.minim_minimiser_destroy(h)
+    for v in (-1, 4):
+        p.batch_control = v
+        assert L.minim_minimiser_create(C.byref(p), C.byref(h)) == A.MINIM_EINVAL, v
+    for mode in ("graph", "device", "host", "resident"):
+        m.Lbfgs().setBatchControl(mode)._handle()
+    with pytest.raises(ValueError):
+        m.Lbfgs().setBatchControl("bogus")

@@ -379,3 +379,93 @@ def test_batch_autotune(gpu):
     assert lb.batchReport()["block"] == best
     assert all(r["converged"] for r in res)
     m.autotune_clear()
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(m=3, max_step=0.05), dict(m=7, init_step=0.3),
+                                dict(linesearch="none", max_iter=60)])
+def test_batch_resident_vs_oracle(gpu, kw):
+    """Resident control (one workgroup runs each instance's whole minimisation
+    in one launch): every instance ends as the oracle's FP64 L-BFGS under the
+    same settings does -- converged energies within 1e-8, the reported energy
+    the oracle's at the coordinates left (1e-12), the gradient within the
+    tolerance -- with one launch and one host synchronisation for the batch.
+    Without a line search, a fixed iteration count gives the graph-driven
+    batch's trajectory to rounding."""
+    m = gpu
+    cs = cases(23)
+    states, res, rep = run_batch(m, cs, "resident", **kw)
+    print("report", rep, flush=True)
+    assert rep["control"] == "resident" and rep["host_syncs"] == 1 and rep["graph_launches"] == 0
+    assert rep["replays"] == rep["iterations"] >= max(r["iterations"] for r in res)
+    if kw.get("linesearch") == "none":
+        _, rg, _ = run_batch(m, cs, "graph", **kw)
+    for i, (c, s, r) in enumerate(zip(cs, states, res)):
+        o = OraclePF(c["grid"], 1, 1.0, [1.0], [c["w"]])
+        _, so = oracle_lbfgs(o, c["x0"], m=kw.get("m", 5), max_iter=kw.get("max_iter", 20000),
+                             max_step=kw.get("max_step", 0.0), init_step=kw.get("init_step", 1.0),
+                             linesearch=kw.get("linesearch", "backtracking"))
+        x = s.coords()
+        e0 = o.energy(c["x0"])
+        assert abs(r["energy"] - o.energy(x)) <= 1e-12 * max(abs(r["energy"]), abs(e0)), c["grid"]
+        assert r["converged"] == so["converged"], (c["grid"], r, so)
+        if so["converged"]:
+            assert energy_ok(r["energy"], so["energy"], e0), (c["grid"], r["energy"], so["energy"])
+            g = o.gradient(x)
+            assert np.sqrt(np.mean(g * g)) < 1.0001 * o.convergence
+            assert abs(r["iterations"] - so["iterations"]) <= max(10, 0.3 * so["iterations"]), (r, so)
+        else:
+            assert r["iterations"] == so["iterations"] == rg[i]["iterations"]
+            # (without a line search rounding differences grow along the trajectory)
+            assert energy_ok(r["energy"], rg[i]["energy"], e0), (c["grid"], r["energy"], rg[i]["energy"])
+            assert r["iter_fp64"] == rg[i]["iter_fp64"] and r["n_energy"] == rg[i]["n_energy"]
+        if r["converged"]:
+            assert r["iter_fp64"] == r["iterations"] + 1
+        assert r["iter_fp32"] == 0
+
+
+def test_batch_resident_first_iteration_matches_single(gpu):
+    """maxIter 0 under resident control: the first step (Lbfgs.cpp:91-96 with
+    the line search) equals the single-State path's to the rounding of the
+    norms."""
+    m = gpu
+    cs = cases(8)
+    states, res, _ = run_batch(m, cs, "resident", max_iter=0)
+    for c, s, r in zip(cs, states, res):
+        s1 = make_state(m, c)
+        x1 = m.Lbfgs().setM(5).setMaxIter(0).minimise(s1)
+        assert np.max(np.abs(s.coords() - x1)) <= 1e-14 * max(1.0, np.max(np.abs(x1))), c["grid"]
+        assert r["iterations"] == 1 and not r["converged"]
+        assert abs(r["energy"] - s1.energy()) <= 1e-12 * abs(s1.energy())
+
+
+def test_batch_resident_many_instances(gpu):
+    """More instances than workgroups the device holds at once (1200, sides
+    4..40, some 3-D): the later workgroups start as earlier ones finish, every
+    instance converges, and its result does not depend on the others (an
+    instance alone gives the same coordinates bit for bit: its sums are taken
+    by its own workgroup in a fixed order).  Mixed precision is refused."""
+    m = gpu
+    rng = np.random.default_rng(1200)
+    cs = []
+    for i in range(1200):
+        if i % 10 == 9:
+            g = tuple(int(v) for v in rng.integers(2, 8, 3))
+        else:
+            sd = int(rng.integers(4, 41))
+            g = (sd, sd, 1)
+        cs.append(dict(grid=g, w=float(rng.uniform(1.0, 2.0)), x0=rng.uniform(-0.1, 0.1, int(np.prod(g)))))
+    states, res, rep = run_batch(m, cs, "resident")
+    print("report", rep, flush=True)
+    for i in range(0, 1200, 37):
+        c, s, r = cs[i], states[i], res[i]
+        assert r["converged"], (i, r)
+        o = OraclePF(c["grid"], 1, 1.0, [1.0], [c["w"]])
+        g = o.gradient(s.coords())
+        assert np.sqrt(np.mean(g * g)) < 1.0001 * o.convergence, i
+    assert all(r["converged"] for r in res)
+    for i in (0, 9, 555):
+        alone, ra, _ = run_batch(m, [cs[i]], "resident")
+        assert np.array_equal(alone[0].coords(), states[i].coords()), i
+        assert ra[0]["iterations"] == res[i]["iterations"] and ra[0]["n_energy"] == res[i]["n_energy"]
+    with pytest.raises(ValueError, match="FP64 schedule only"):
+        run_batch(m, cs[:2], "resident", precision="mixed")
