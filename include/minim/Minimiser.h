@@ -25,7 +25,10 @@ struct MinimiseStats {
   bool converged = false;
   double gradRms = 0;    // sqrt(g.g/ndof) of the last gradient
   double gradMax = 0;    // Fire with setMaxForceConvergence: largest per-particle force norm
-  double seconds = 0;    // wall time of the minimisation (host clock)
+  double seconds = 0;    // wall time of the minimisation (host clock); minimiseBatch: from the
+                         // start of the call until this instance finished (device clock)
+  double startSeconds = 0;  // minimiseBatch: from the start of the call until this instance was
+                            // admitted (0 without a budget)
   double energy = 0;     // last accepted energy (FP64 accumulation)
   // Wall time (s) and iteration at which the convergence measure (gradRms,
   // or gradMax for Fire::setMaxForceConvergence) first fell below

@@ -339,6 +339,8 @@ typedef struct minim_result {
   double seconds;  /* wall time from the start of the call until this minimisation ended */
   double cross_seconds[8]; /* wall time and iteration at which the convergence measure (grad_rms, or */
   int cross_iter[8];       /* grad_max with max_force_dim) first fell below 10^-(k+1); -1: never */
+  double start_seconds;    /* minim_minimise_batch: wall time from the start of the call until this
+                              instance was admitted (0 without a budget); 0 for a single minimisation */
 } minim_result;
 
 typedef void (*minim_adjust_fn)(int iter, minim_state* state, void* user);

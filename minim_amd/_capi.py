@@ -50,7 +50,7 @@ class Result(C.Structure):
                 ("n_gradient", C.c_long), ("n_energy_reused", C.c_long), ("iter_fp32", C.c_int),
                 ("iter_fp64", C.c_int), ("energy", C.c_double), ("grad_rms", C.c_double),
                 ("dt_max", C.c_double), ("grad_max", C.c_double), ("seconds", C.c_double),
-                ("cross_seconds", C.c_double * 8), ("cross_iter", C.c_int * 8)]
+                ("cross_seconds", C.c_double * 8), ("cross_iter", C.c_int * 8), ("start_seconds", C.c_double)]
 
 
 ADJUST_FN = C.CFUNCTYPE(None, C.c_int, vp, vp)
@@ -119,7 +119,7 @@ SIGNATURES = {
 _lib = None
 
 
-ABI_VERSION = 8  # layout of the parameter/result structs below
+ABI_VERSION = 9  # layout of the parameter/result structs below
 
 
 def load():

@@ -901,7 +901,8 @@ class Lbfgs(Minimiser):
             r = res[i]
             out.append(dict(iterations=r.iterations, converged=bool(r.converged), n_energy=r.n_energy,
                             n_gradient=r.n_gradient, n_energy_reused=r.n_energy_reused, iter_fp32=r.iter_fp32,
-                            iter_fp64=r.iter_fp64, energy=r.energy, grad_rms=r.grad_rms, seconds=r.seconds))
+                            iter_fp64=r.iter_fp64, energy=r.energy, grad_rms=r.grad_rms, seconds=r.seconds,
+                            start_seconds=r.start_seconds))
         return out
 
     def _params(self):

@@ -617,7 +617,8 @@ struct BCtl {
   int started, done, searching, lp, fresh, lsFailed, converged;
   long long nEnergy, nGradient;
   int iterFP32, iterFP64;
-  unsigned long long tdone;  // device wall clock when the instance finished
+  unsigned long long tstart;  // device wall clock when the instance was admitted
+  unsigned long long tdone;   // ... and when it finished
   double conv;               // the State's convergence tolerance
   int n;                     // elements
 };
@@ -672,6 +673,7 @@ __global__ __launch_bounds__(kBlock) void bc_admit_kernel(BCtlArgs A, const int*
   C.gcur = 0;
   C.nGradient++;
   C.started = 1;
+  C.tstart = wall_clock64();
   A.live[j] = 1;
 }
 
@@ -1080,6 +1082,7 @@ __global__ __launch_bounds__(kBlock) void bres_kernel(BBufs B, BCtlArgs A) {
       Cs.gcur = 0;
       Cs.nGradient++;
       Cs.started = 1;
+      Cs.tstart = wall_clock64();
     }
     __syncthreads();
   }
@@ -1673,6 +1676,7 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
       I.st.iterFP64 = c.iterFP64;
       I.st.converged = c.converged != 0;
       I.st.seconds = (c.done && rateKHz > 0) ? (double)(c.tdone - clk0) / (1e3 * rateKHz) : total;
+      I.st.startSeconds = (c.started && rateKHz > 0) ? (double)(c.tstart - clk0) / (1e3 * rateKHz) : 0.0;
       if (I.iters > maxIters) maxIters = I.iters;
     }
     batchReport.iterations = maxIters;

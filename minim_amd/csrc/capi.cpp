@@ -115,7 +115,7 @@ void apply_pf_params(const minim_pf_params* p, PF* pf) {
 extern "C" {
 
 const char* minim_last_error(void) { return g_err.c_str(); }
-int minim_abi_version(void) { return 8; }
+int minim_abi_version(void) { return 9; }
 
 // ---------------------------------------------------------------- potentials
 int minim_pf_create(const minim_pf_params* p, minim_potential** out) {
@@ -860,6 +860,7 @@ int minim_minimise_batch(minim_minimiser* m, minim_state* const* states, int n_s
       r.energy = st[i].energy;
       r.grad_rms = st[i].gradRms;
       r.seconds = st[i].seconds;
+      r.start_seconds = st[i].startSeconds;
       for (int k = 0; k < 8; k++) {
         r.cross_seconds[k] = -1;
         r.cross_iter[k] = -1;

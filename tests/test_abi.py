@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(L, n), n
     assert set(names) == set(_capi.SIGNATURES), "ctypes table and header disagree"
-    assert L.minim_abi_version() == 8
+    assert L.minim_abi_version() == 9
 
 
 def _pf_handle(grid, nf, st, iw, density=0, res=1.0, fix=None):

@@ -328,6 +328,13 @@ def test_batch_budget_keeps_results(gpu, precision):
                 assert r[i][k] == rh[i][k], (control, i, k, r[i][k], rh[i][k])
     assert out["graph"][2]["captures"] >= 3  # admissions re-lay the grid
     assert all(r["converged"] for r in rh)
+    # admission times: 0 for host control (every instance from the start); under
+    # a budget some instances wait, and each is admitted before it finishes
+    assert all(r["start_seconds"] == 0 for r in rh)
+    for control in ("graph", "device"):
+        r = out[control][1]
+        assert all(0 <= x["start_seconds"] <= x["seconds"] for x in r), control
+        assert max(x["start_seconds"] for x in r) > 0, control
 
 
 
