@@ -101,8 +101,9 @@ __device__ inline void grid_sum_store(double (&v)[K], RedScratch rs) {
 }
 
 // Second stage: result[k] = sum over the nb partials of value k, in a fixed
-// order (stream-ordered after the producing kernel).
-void launch_finalize(RedScratch rs, int K, int nb, hipStream_t s);
+// order (stream-ordered after the producing kernel).  With dupDst, value dupK
+// is also written there (a caller's device slot: no copy launch after).
+void launch_finalize(RedScratch rs, int K, int nb, hipStream_t s, int dupK = -1, double* dupDst = nullptr);
 
 // MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each
 // with a private L2.  Remap the hardware block id so that every XCD receives a

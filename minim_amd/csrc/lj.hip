@@ -694,9 +694,7 @@ int LjNd::gradientDeferred(int prec, const void* x, void* g, void* stream) const
   const double src6 = std::pow(sigma / cutoff, 6);
   const double eshift = 4 * epsilon * (src6 * src6 - src6);
   launch_cut(I, prec, x, nullptr, 0.0, N, sigma, epsilon, eshift, 2, g, ws.rs(), s);
-  minim_amd::launch_finalize(ws.rs(), 2, cut_blocks(N), s);
-  double* res = ws.device_result();
-  MINIM_HIP_CHECK(hipMemcpyAsync(res + minim_amd::kDeferSlot, res + 1, sizeof(double), hipMemcpyDeviceToDevice, s));
+  minim_amd::launch_finalize(ws.rs(), 2, cut_blocks(N), s, 1, ws.device_result() + minim_amd::kDeferSlot);
   constrainDevice(prec, x, g, (size_t)N * nDim, stream);
   return minim_amd::kDeferSlot;
 }

@@ -574,7 +574,6 @@ void Lbfgs::iteration(State& state) {
       a.coef[a.nb++] = ay[q];
     }
   }
-  minim_amd::launch_direction(a, prec, ws.rs(), s);
   // Constraints that couple all DOFs (fixed volumes) act on the step too
   // (Lbfgs.cpp:51); d lies in their subspace up to rounding, so the direction
   // pass's dots stay valid.  The projection leaves result[0, 80) untouched.
@@ -582,6 +581,20 @@ void Lbfgs::iteration(State& state) {
   // the dots are taken again after the projection, except d.d, which the
   // maxStep cap reads before it (Lbfgs.cpp:121-124 precede :51).
   const bool moves = state.pot->constraintsMoveSteps();
+  // Without a maxStep cap the line search starts at x + d (tau = 1) unless d
+  // must be flipped (rare).  That trial energy is launched right after the
+  // direction pass, whose dots then go to result slots [48, 48 + 1 + ndot)
+  // (the trial's energy takes slot 0), so that one host synchronisation
+  // returns both.  When the potential can, that trial is the fused step:
+  // energy and gradient at x + d, and x + d itself into a spare buffer, in one
+  // pass; if the trial is accepted the spare becomes the coordinates and the
+  // gradient is there.
+  const bool spec = linesearch == "backtracking" && (P == 0 || _maxStep == 0) && !moves;
+  const bool fused = spec && state.pot->supportsStep();
+  constexpr int kStash = 48;
+  minim_amd::RedScratch rd = ws.rs();
+  if (spec) rd.result += kStash;
+  minim_amd::launch_direction(a, prec, rd, s);
   std::vector<double> dres(1 + a.ndot);
   double ddPre = 0;
   if (moves) {
@@ -602,19 +615,8 @@ void Lbfgs::iteration(State& state) {
       ws.fetch(s, k, dres.data() + j0);
     }
   }
-  // Without a maxStep cap the line search starts at x + d (tau = 1) unless d
-  // must be flipped (rare).  Launch that trial energy now, after stashing the
-  // direction pass's dots, so that one host synchronisation returns both.
-  // When the potential can, that trial is the fused step: energy and gradient
-  // at x + d, and x + d itself into a spare buffer, in one pass; if the trial
-  // is accepted the spare becomes the coordinates and the gradient is there.
-  const bool spec = linesearch == "backtracking" && (P == 0 || _maxStep == 0) && !moves;
-  const bool fused = spec && state.pot->supportsStep();
   double eSpec = 0;
   if (spec) {
-    constexpr int kStash = 48;  // result slots [48, 48 + 1 + ndot) hold the direction dots
-    double* res = ws.device_result();
-    MINIM_HIP_CHECK(hipMemcpyAsync(res + kStash, res, sizeof(double) * (1 + a.ndot), hipMemcpyDeviceToDevice, s));
     if (fused) {
       const auto* xt = static_cast<const DVec*>(I.xtrial.get());
       if (!xt || xt->n != n || xt->prec != state.precision()) {
@@ -901,13 +903,13 @@ void Fire::iteration(State& state) {
   for (;;) {
     const void* pa[3] = {I.g.data(), I.v.data(), I.v.data()};
     const void* pb[3] = {I.g.data(), I.g.data(), I.v.data()};
-    minim_amd::launch_dots(I.prec, pa, pb, 3, I.n, ws.rs(), s);
-    constexpr int kStash = 48;
-    double* res = ws.device_result();
+    constexpr int kStash = 48;  // with the largest force in slot 0, the dots go to [48, 51)
+    minim_amd::RedScratch rd = ws.rs();
+    if (_maxForceDim > 0) rd.result += kStash;
+    minim_amd::launch_dots(I.prec, pa, pb, 3, I.n, rd, s);
     std::array<double, minim_amd::kMaxDots> q;
     int nfetch = 3;
     if (_maxForceDim > 0) {
-      MINIM_HIP_CHECK(hipMemcpyAsync(res + kStash, res, sizeof(double) * 3, hipMemcpyDeviceToDevice, s));
       minim_amd::launch_group_norm_max(I.prec, I.g.data(), I.n, _maxForceDim, ws.rs(), s);
       nfetch = kStash + 3;
     }

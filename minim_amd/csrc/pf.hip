@@ -627,7 +627,7 @@ void pf_args_finalize(PFArgs& a, int target_blocks) {
 
 void pf_launch_eval(const PFArgs& a, int prec, int pv, int mode, const void* x, const void* d, double t,
                     void* g, RedScratch rs, hipStream_t s, bool finalize, void* xout, hipEvent_t evBefore,
-                    hipEvent_t evAfter) {
+                    hipEvent_t evAfter, double* eOut) {
   int nb;
   if (evBefore) MINIM_HIP_CHECK(hipEventRecord(evBefore, s));
   if (xout) {
@@ -651,7 +651,7 @@ void pf_launch_eval(const PFArgs& a, int prec, int pv, int mode, const void* x, 
   MINIM_HIP_CHECK(hipGetLastError());
   if (evAfter) MINIM_HIP_CHECK(hipEventRecord(evAfter, s));
   if (finalize) {
-    launch_finalize(rs, 2, nb, s);
+    launch_finalize(rs, 2, nb, s, 0, (mode & 1) ? eOut : nullptr);
     // fixed volumes: the global part of the gradient projection; g.g anew
     if ((mode & 2) && a.volfix) pf_launch_volume_projection(a, pv, g, (size_t)a.X * a.YZ, true, rs, s);
   }

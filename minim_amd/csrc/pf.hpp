@@ -114,9 +114,10 @@ void pf_args_finalize(PFArgs& a, int target_blocks);
 // xout (precision prec): an accepted step and its gradient in one pass.
 // evBefore / evAfter, when given, are recorded on s right before and right
 // after the evaluation kernel (benchmark timing of the kernel inside a step).
+// eOut (with finalize): the energy is also written to this device slot.
 void pf_launch_eval(const PFArgs& a, int prec, int pv, int mode, const void* x, const void* d, double t,
                     void* g, RedScratch rs, hipStream_t s, bool finalize = true, void* xout = nullptr,
-                    hipEvent_t evBefore = nullptr, hipEvent_t evAfter = nullptr);
+                    hipEvent_t evBefore = nullptr, hipEvent_t evAfter = nullptr, double* eOut = nullptr);
 // Potential::applyConstraints: fixed fluids, hard density projection, then
 // (fixed volumes) the global projection, which uses rs as
 // pf_launch_volume_projection does.

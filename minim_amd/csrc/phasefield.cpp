@@ -732,8 +732,7 @@ void PhaseField::timeSteps(int prec, const void* x, void* g, double* e_dev, int 
   hipStream_t s = static_cast<hipStream_t>(stream);
   auto& ws = minim_amd::Workspace::current();
   auto step = [&](hipEvent_t b, hipEvent_t e) {
-    minim_amd::pf_launch_eval(a, prec, prec, 3, x, nullptr, 0.0, g, ws.rs(), s, true, nullptr, b, e);
-    MINIM_HIP_CHECK(hipMemcpyAsync(e_dev, ws.device_result(), sizeof(double), hipMemcpyDeviceToDevice, s));
+    minim_amd::pf_launch_eval(a, prec, prec, 3, x, nullptr, 0.0, g, ws.rs(), s, true, nullptr, b, e, e_dev);
   };
   // untimed evaluations until warmMs of host time have passed
   const auto t0 = std::chrono::steady_clock::now();
@@ -788,8 +787,8 @@ void PhaseField::evaluateAsyncScratch(int prec, const void* x, const void* d, do
   if ((mode & 4) && (mode & 2)) throw std::invalid_argument("PhaseField: gradients at a trial point are not supported.");
   hipStream_t s = static_cast<hipStream_t>(stream);
   minim_amd::RedScratch rs{scratch, scratch + nPartials};
-  minim_amd::pf_launch_eval(a, prec, prec, mode, x, d, t, g, rs, s);
-  if (wantE && e_dev) MINIM_HIP_CHECK(hipMemcpyAsync(e_dev, rs.result, sizeof(double), hipMemcpyDeviceToDevice, s));
+  minim_amd::pf_launch_eval(a, prec, prec, mode, x, d, t, g, rs, s, true, nullptr, nullptr, nullptr,
+                            wantE ? e_dev : nullptr);
 }
 
 void PhaseField::evaluateMixed(const void* x, const void* d, double t, double* e, void* g, void* stream) const {
@@ -804,9 +803,8 @@ void pf_eval_async(const PFArgs& a, int prec, int pv, const void* x, const void*
   if ((mode & 4) && (mode & 2)) throw std::invalid_argument("PhaseField: gradients at a trial point are not supported.");
   hipStream_t s = static_cast<hipStream_t>(stream);
   auto& ws = minim_amd::Workspace::current();
-  minim_amd::pf_launch_eval(a, prec, pv, mode, x, d, t, g, ws.rs(), s);
-  if (wantE && e_dev)
-    MINIM_HIP_CHECK(hipMemcpyAsync(e_dev, ws.device_result(), sizeof(double), hipMemcpyDeviceToDevice, s));
+  minim_amd::pf_launch_eval(a, prec, pv, mode, x, d, t, g, ws.rs(), s, true, nullptr, nullptr, nullptr,
+                            wantE ? e_dev : nullptr);
 }
 }  // namespace
 

@@ -36,21 +36,25 @@ void DeviceBuffer::reset(size_t bytes) {
 namespace {
 // One block per reduced value: partials of value k are contiguous.
 __global__ __launch_bounds__(kBlock) void finalize_kernel(const double* __restrict__ partials, int nb,
-                                                          double* __restrict__ result) {
+                                                          double* __restrict__ result, int dupK, double* dupDst) {
   __shared__ double lds[kBlock / kWave];
   const int k = blockIdx.x;
   const double* p = partials + (size_t)k * nb;
   double v[1] = {0.0};
   for (int b = threadIdx.x; b < nb; b += kBlock) v[0] += p[b];
   block_sum<1>(v, lds);
-  if (threadIdx.x == 0) result[k] = v[0];
+  if (threadIdx.x == 0) {
+    result[k] = v[0];
+    if (k == dupK) *dupDst = v[0];
+  }
 }
 }  // namespace
 
-void launch_finalize(RedScratch rs, int K, int nb, hipStream_t s) {
+void launch_finalize(RedScratch rs, int K, int nb, hipStream_t s, int dupK, double* dupDst) {
   if (K < 1 || K > kMaxDots || nb < 1 || nb > kMaxRedBlocks)
     throw std::invalid_argument("reduction shape out of range");
-  finalize_kernel<<<K, kBlock, 0, s>>>(rs.partials, nb, rs.result);
+  if (!dupDst) dupK = -1;
+  finalize_kernel<<<K, kBlock, 0, s>>>(rs.partials, nb, rs.result, dupK, dupDst);
   MINIM_HIP_CHECK(hipGetLastError());
 }
 
