@@ -49,7 +49,10 @@ class LjNd : public NewPotential<LjNd> {
   int gradientDeferred(int prec, const void* x, void* g, void* stream) const override;
   void redoGradient(int prec, const void* x, void* g, void* stream) const override;
 
-  // Statistics of the neighbour list (cutoff mode).
+  // Statistics of the neighbour list (cutoff mode) of this object's device
+  // state: a State evaluates its own copy of the potential (state.pot).
+  // listBuilds counts builds; maxNeighbours is the longest list of the last
+  // build (it reads the list lengths back from the device).
   long listBuilds() const;
   int maxNeighbours() const;
 

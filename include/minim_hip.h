@@ -147,6 +147,9 @@ int minim_lj_set_cutoff(minim_potential* pot, double rc, const double box[3], do
  * State holds its own copy of the potential (LjNd::listBuilds; no reference
  * counterpart). */
 int minim_lj_list_builds(const minim_state* state, long* builds);
+/* The longest neighbour list of the last build of the LjNd behind a State
+ * (LjNd::maxNeighbours; 0 before the first build; waits for the device). */
+int minim_lj_max_neighbours(const minim_state* state, int* longest);
 
 /* BarAndHinge (BarAndHinge::setTriangulation / setBondList / setHingeList /
  * setRigidity / setLength0 / setTheta0 / setForce,

@@ -70,6 +70,7 @@ SIGNATURES = {
     "minim_lj_create": (C.c_int, [C.c_int, C.c_double, C.c_double, C.POINTER(vp)]),
     "minim_lj_set_cutoff": (C.c_int, [vp, C.c_double, dp, C.c_double]),
     "minim_lj_list_builds": (C.c_int, [vp, C.POINTER(C.c_long)]),
+    "minim_lj_max_neighbours": (C.c_int, [vp, C.POINTER(C.c_int)]),
     "minim_bah_create": (C.c_int, [C.POINTER(BaHParams), C.POINTER(vp)]),
     "minim_potential_set_fixed": (C.c_int, [vp, ip, C.c_size_t]),
     "minim_potential_init": (C.c_int, [vp, dp, C.c_size_t]),

@@ -427,6 +427,13 @@ class LjNd(Potential):
         A.check(A.load().minim_lj_list_builds(state.ptr, C.byref(n)))
         return n.value
 
+    def maxNeighbours(self, state):
+        """LjNd::maxNeighbours: the longest neighbour list of the last build
+        of the potential behind state (cutoff mode)."""
+        n = C.c_int()
+        A.check(A.load().minim_lj_max_neighbours(state.ptr, C.byref(n)))
+        return n.value
+
     def _create(self):
         L = A.load()
         h = C.c_void_p()

@@ -290,6 +290,16 @@ int minim_lj_list_builds(const minim_state* state, long* builds) {
   });
 }
 
+int minim_lj_max_neighbours(const minim_state* state, int* longest) {
+  return guard([&] {
+    need(state, "state");
+    need(longest, "longest");
+    auto* lj = dynamic_cast<const minim::LjNd*>(state->s->pot.get());
+    if (!lj) throw std::invalid_argument("not a LjNd potential");
+    *longest = lj->maxNeighbours();
+  });
+}
+
 int minim_bah_create(const minim_bah_params* p, minim_potential** out) {
   return guard([&] {
     need(p, "params");

@@ -467,7 +467,6 @@ struct LjNd::Impl {
   bool cellMode = false;
   int maxn = 128;
   long builds = 0;
-  int lastMax = 0;
   bool built = false;
   DeviceBuffer cell, count, start, cursor, order, nn, nbr, flag, xbuild, xpad, xcell;
   int ncell = 0;
@@ -511,7 +510,16 @@ LjNd& LjNd::setCutoff(double rc, std::array<double, 3> b, double sk) {
 }
 
 long LjNd::listBuilds() const { return impl_ ? impl_->builds : 0; }
-int LjNd::maxNeighbours() const { return impl_ ? impl_->lastMax : 0; }
+// The largest list length of the last build, read back from the device (a
+// statistic: it waits for the device).
+int LjNd::maxNeighbours() const {
+  if (!impl_ || !impl_->built) return 0;
+  std::vector<int> h(nParticle);
+  MINIM_HIP_CHECK(hipMemcpy(h.data(), impl_->nn.get(), sizeof(int) * h.size(), hipMemcpyDeviceToHost));
+  int mx = 0;
+  for (int v : h) mx = v > mx ? v : mx;
+  return mx;
+}
 
 // LjNd::init (src/potentials/LjNd.cpp:12-26): nParticle from the coordinate count.
 void LjNd::init(const vector<double>& coords) {

@@ -76,6 +76,13 @@ def test_lj_cutoff_vs_oracle(gpu, n_side, prec):
     s = m.State(pot, x, prec)
     e, g = s.energyGradient()
     assert pot.listBuilds(s) == 1
+    # the longest list: the most minimum-image neighbours within rc + skin
+    p = x.reshape(-1, 3).astype(np.float64)
+    dv = p[:, None, :] - p[None, :, :]
+    dv -= L * np.rint(dv / L)
+    r2 = (dv * dv).sum(-1)
+    np.fill_diagonal(r2, np.inf)
+    assert pot.maxNeighbours(s) == int((r2 < 2.8 ** 2).sum(1).max())
     o = OracleLJPBC(n, L, 2.5)
     assert abs(e - o.energy(x)) <= TOL[prec] * abs(o.energy(x))
     assert rel_err(g, o.gradient(x)) <= TOL[prec]
