@@ -553,9 +553,10 @@ def batch_leg(m, n_inst=512, workers=8, seed=4040, cpu_budget=10.0, per_state=Tr
     # the batch passes' block size, tuned per precision schedule on the first
     # 32 iterations of this batch (Lbfgs.autotuneBatch, outside the timed runs)
     tuned = {}
-    for prec in ("fp64", "mixed"):
+    for prec, control in (("fp64", "graph"), ("mixed", "graph"), ("fp64", "resident")):
         tstates = [m.State(make_pf(j), inputs(j), "fp64") for j in range(n_inst)]
-        tuned[prec] = m.Lbfgs().setM(5).setMaxIter(20000).setPrecision(prec).autotuneBatch(tstates, replays=32)
+        tuned[prec if control == "graph" else prec + "_resident"] = m.Lbfgs().setM(5).setMaxIter(20000).setPrecision(
+            prec).setBatchControl(control).autotuneBatch(tstates, replays=32)
         del tstates
     runs = [(p, "graph", 0) for p in ("fp64", "mixed")] + [(p, "graph", b) for b in budgets for p in ("fp64", "mixed")]
     runs += [(p, "host", 0) for p in ("fp64", "mixed")] + [("fp64", "resident", 0)]

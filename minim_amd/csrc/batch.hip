@@ -995,7 +995,8 @@ __device__ void bc_update(const BCtlArgs& A, int j, BCtl& C, const double* rsum)
 // whole batch.  Every instance of the batch is in flight at once (or as soon
 // as a workgroup slot frees), largest first.  Sums are split differently
 // from the block-per-launch passes, so results agree with the other controls
-// to rounding.
+// to rounding.  One workgroup streams only a few tens of GB/s, so this pays
+// for small instances (a few thousand sites) and loses for large ones.
 
 // out[k] = the workgroup's sum of v[k] for the bits of `on` (the tree of
 // block_partials), visible to every thread on return.
@@ -1228,8 +1229,10 @@ std::vector<std::array<double, 2>> Lbfgs::autotuneBatch(const std::vector<State*
   }
   size_t N = 0;
   for (const State* s : states) N += s->ndof;
-  minim_amd::tune_store(minim_amd::tune_key("batch-block", {(long)states.size(), (long)N, (long)_m},
-                                            _precMode == "mixed" ? 1 : 0),
+  // resident control keys its own entry: one workgroup per instance favours
+  // other sizes than launches over every instance
+  minim_amd::tune_store(minim_amd::tune_key(_batchControl == "resident" ? "batch-block-resident" : "batch-block",
+                                            {(long)states.size(), (long)N, (long)_m}, _precMode == "mixed" ? 1 : 0),
                         {best});
   return rows;
 }
@@ -1258,7 +1261,8 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
       if (s) nTot += s->ndof;
     if (block == 0) block = _batchBlock;
     std::vector<int> tuned;
-    if (block == 0 && tune_lookup(tune_key("batch-block", {(long)ni, (long)nTot, (long)_m}, mixed ? 1 : 0), tuned) &&
+    const char* tkey = _batchControl == "resident" ? "batch-block-resident" : "batch-block";
+    if (block == 0 && tune_lookup(tune_key(tkey, {(long)ni, (long)nTot, (long)_m}, mixed ? 1 : 0), tuned) &&
         !tuned.empty())
       block = tuned[0];
     if (block != 4 && block != 8 && block != 16) block = kBlockDefault;
@@ -1559,6 +1563,8 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
       MINIM_HIP_CHECK(hipMemcpyAsync(mapA.get(), order.data(), sizeof(int) * ni, hipMemcpyHostToDevice, cs));
       bc_admit_kernel<<<1, 1, 0, cs>>>(A, nullptr, 0, 1);  // the start clock
       MINIM_HIP_CHECK(hipGetLastError());
+      // autotuneBatch's bounded runs: at most replayLimit iterations each
+      if (replayLimit > 0) A.maxIter = (int)std::min<long>(maxIter, replayLimit - 1);
       auto* kres = _m <= 5 ? K.resident5 : K.resident10;
       if (!kres) throw std::logic_error("Lbfgs::minimiseBatch: no resident kernel for this storage type.");
       BBufs R = G;

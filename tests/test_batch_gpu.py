@@ -476,3 +476,24 @@ def test_batch_resident_many_instances(gpu):
         assert ra[0]["iterations"] == res[i]["iterations"] and ra[0]["n_energy"] == res[i]["n_energy"]
     with pytest.raises(ValueError, match="FP64 schedule only"):
         run_batch(m, cs[:2], "resident", precision="mixed")
+
+
+def test_batch_resident_autotune(gpu):
+    """autotuneBatch under resident control times bounded runs (at most
+    `replays` iterations each) and leaves the States as they were; the next
+    resident minimiseBatch uses the size it chose and converges."""
+    m = gpu
+    m.autotune_clear()
+    cs = cases(29, sides=(16, 40, 90), extra=((4, 5, 6),))
+    states = [make_state(m, c) for c in cs]
+    before = [s.coords() for s in states]
+    lb = m.Lbfgs().setM(5).setMaxIter(20000).setBatchControl("resident")
+    t = lb.autotuneBatch(states, blocks=(4, 8), replays=8)
+    print("tuned", t, flush=True)
+    assert sorted(t) == [4, 8] and all(v > 0 for v in t.values())
+    for b, s in zip(before, states):
+        assert np.array_equal(b, s.coords())
+    res = lb.minimiseBatch(states)
+    assert lb.batchReport()["block"] == min(t, key=t.get)
+    assert all(r["converged"] for r in res)
+    m.autotune_clear()
