@@ -427,18 +427,24 @@ def bah_leg(m, nside=1024, max_iter=60000, ref_side=48):
                          achieved_GBs=alg / (msk * 1e-3) / 1e9, frac_of_hbm_peak=alg / (msk * 1e-3) / 1e9 / HBM_PEAK_GBS,
                          eval_achieved_GBs=alg / (ms * 1e-3) / 1e9)
         del x, g
-    s = m.State(pot, x0, "fp64")
-    lb = m.Lbfgs().setM(10).setMaxIter(max_iter)
-    e0 = s.energy()
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    lb.minimise(s, download=False)
-    torch.cuda.synchronize()
-    out["lbfgs_m10"] = dict(seconds=time.perf_counter() - t, iterations=lb.stats["iterations"],
-                            converged=lb.stats["converged"], grad_rms=lb.stats["grad_rms"], initial_energy=e0,
-                            energy=s.energy(), max_iter=max_iter, tolerance=s.convergence,
-                            grad_rms_crossings={k: dict(seconds=v[0], iteration=v[1])
-                                                for k, v in lb.stats["crossings"].items()})
+    # FP64 throughout, and the mixed schedule (FP32 until the gradient RMS is
+    # below 1e3 times the tolerance, then FP64 coordinates and arithmetic with
+    # FP32 vectors)
+    for key, prec in (("lbfgs_m10", "fp64"), ("lbfgs_m10_mixed", "mixed")):
+        s = m.State(pot, x0, "fp64")
+        lb = m.Lbfgs().setM(10).setMaxIter(max_iter).setPrecision(prec)
+        e0 = s.energy()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        lb.minimise(s, download=False)
+        torch.cuda.synchronize()
+        out[key] = dict(seconds=time.perf_counter() - t, iterations=lb.stats["iterations"],
+                        iter_fp32=lb.stats["iter_fp32"], converged=lb.stats["converged"],
+                        grad_rms=lb.stats["grad_rms"], initial_energy=e0, energy=s.energy(), max_iter=max_iter,
+                        tolerance=s.convergence,
+                        grad_rms_crossings={k: dict(seconds=v[0], iteration=v[1])
+                                            for k, v in lb.stats["crossings"].items()})
+        del s
     from oracle_bind import RefState, ref_available
 
     if ref_available():

@@ -32,6 +32,14 @@ class BarAndHinge : public NewPotential<BarAndHinge> {
   void init(const vector<double>& coords) override;
   void evaluate(int prec, const void* x, const void* d, double t, double* e, void* g,
                 void* stream) const override;
+  void evaluateAsync(int prec, const void* x, const void* d, double t, bool wantE, double* e_dev, void* g,
+                     void* stream) const override;
+  // FP64 coordinates and arithmetic with FP32 direction and gradient (the
+  // mixed schedule's second phase); not with normal constraints.
+  bool supportsMixed() const override;
+  void evaluateMixed(const void* x, const void* d, double t, double* e, void* g, void* stream) const override;
+  void evaluateMixedAsync(const void* x, const void* d, double t, bool wantE, double* e_dev, void* g,
+                          void* stream) const override;
   double timeKernel(int prec, const void* x, bool wantE, void* g, int reps, void* stream) const override;
 
   // Element lists after init, in the reference's element order.

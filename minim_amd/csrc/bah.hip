@@ -301,10 +301,11 @@ __global__ __launch_bounds__(kBlock) void bah_hinge_kernel(BaHArgs a, const T* _
 // element order), U elements at a time with all their loads issued first;
 // contributions are added in list order.  Hinge terms (FP32) or angle
 // deviations (FP64) come from ht, stored by the energy or the hinge kernel.
-template <typename T>
+// TG: the gradient's storage (FP32 for FP64 arithmetic in the mixed schedule).
+template <typename T, typename TG = T>
 __global__ __launch_bounds__(kBlock) void bah_gradient_kernel(BaHArgs a, const T* __restrict__ x,
                                                               const HStore<T>* __restrict__ ht,
-                                                              T* __restrict__ g) {
+                                                              TG* __restrict__ g) {
   constexpr int U = 4;
   for (int n = xcd_logical_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; n < a.nnode;
        n += gridDim.x * kBlock) {
@@ -377,7 +378,7 @@ __global__ __launch_bounds__(kBlock) void bah_gradient_kernel(BaHArgs a, const T
     }
     if (a.fz && a.nodeActive[n]) gn[2] -= (T)a.fz[n];
 #pragma unroll
-    for (int k = 0; k < 3; k++) g[(size_t)n * 3 + k] = gn[k];
+    for (int k = 0; k < 3; k++) g[(size_t)n * 3 + k] = (TG)gn[k];
   }
 }
 
@@ -714,14 +715,19 @@ void BarAndHinge::upload() {
 }
 
 namespace {
-// Energy kernel (into rs.result[0] when wantE) and gradient kernel (when g;
-// the hinge terms come from the energy kernel, or from the hinge kernel
-// without an energy).
-void launch_bah(const BarAndHinge::Dev& dv, int prec, const void* x, const void* d, double t, bool wantE, void* g,
-                minim_amd::RedScratch rs, hipStream_t s) {
+// Energy kernel (into rs.result[0] when wantE, and to eDev when given) and
+// gradient kernel (when g; the hinge terms come from the energy kernel, or
+// from the hinge kernel without an energy).  prec: the coordinates and the
+// arithmetic; pv: the direction d and the gradient g (FP32 against FP64
+// coordinates in the mixed schedule).
+void launch_bah(const BarAndHinge::Dev& dv, int prec, int pv, const void* x, const void* d, double t, bool wantE,
+                void* g, minim_amd::RedScratch rs, hipStream_t s, double* eDev = nullptr) {
   const auto& a = dv.args;
-  auto go = [&](auto tag) {
+  if (pv != prec && !(prec == FP64 && pv == FP32))
+    throw std::invalid_argument("BarAndHinge: FP32 coordinates with FP64 vectors are not supported.");
+  auto go = [&](auto tag, auto gtag) {
     using T = decltype(tag);
+    using TG = decltype(gtag);
     auto* ht = static_cast<minim_amd::HStore<T>*>(dv.ht.get());
     const T* xx = (const T*)x;
     if (d) {
@@ -730,31 +736,31 @@ void launch_bah(const BarAndHinge::Dev& dv, int prec, const void* x, const void*
       T* xt = static_cast<T*>(dv.xtrial.get());
       const size_t n3 = 3 * (size_t)a.nnode;
       minim_amd::device_copy(xt, x, n3 * sizeof(T), s);
-      minim_amd::launch_axpy(prec, xt, d, t, n3, s);
+      minim_amd::launch_axpy_mixed(prec, xt, pv, d, t, n3, s);
       xx = xt;
-      d = nullptr;
     }
     if (wantE) {
       const size_t tot = (size_t)a.nbar + a.nhinge + a.nnode;
       const int nb = minim_amd::grid_blocks(tot, 4, 2048);
       if (g)
-        minim_amd::bah_energy_kernel<T, true><<<nb, kBlock, 0, s>>>(a, xx, (const T*)d, (T)t, rs, ht);
+        minim_amd::bah_energy_kernel<T, true><<<nb, kBlock, 0, s>>>(a, xx, (const T*)nullptr, (T)0, rs, ht);
       else
-        minim_amd::bah_energy_kernel<T, false><<<nb, kBlock, 0, s>>>(a, xx, (const T*)d, (T)t, rs, ht);
+        minim_amd::bah_energy_kernel<T, false><<<nb, kBlock, 0, s>>>(a, xx, (const T*)nullptr, (T)0, rs, ht);
       MINIM_HIP_CHECK(hipGetLastError());
-      minim_amd::launch_finalize(rs, 1, nb, s);
+      minim_amd::launch_finalize(rs, 1, nb, s, 0, eDev);
     } else if (g && a.nhinge > 0) {
       minim_amd::bah_hinge_kernel<T><<<minim_amd::grid_blocks(a.nhinge, 1, 1 << 20), kBlock, 0, s>>>(a, xx, ht);
       MINIM_HIP_CHECK(hipGetLastError());
     }
     if (g) {
       const int nb = minim_amd::grid_blocks(a.nnode, 1, 1 << 20);
-      minim_amd::bah_gradient_kernel<T><<<nb, kBlock, 0, s>>>(a, xx, ht, (T*)g);
+      minim_amd::bah_gradient_kernel<T, TG><<<nb, kBlock, 0, s>>>(a, xx, ht, (TG*)g);
       MINIM_HIP_CHECK(hipGetLastError());
     }
   };
-  if (prec == FP64) go(0.0);
-  else go(0.0f);
+  if (prec == FP64 && pv == FP64) go(0.0, 0.0);
+  else if (prec == FP64) go(0.0, 0.0f);
+  else go(0.0f, 0.0f);
 }
 }  // namespace
 
@@ -766,12 +772,49 @@ void BarAndHinge::evaluate(int prec, const void* x, const void* d, double t, dou
   const auto& a = dev_->args;
   if (g && d) throw std::invalid_argument("BarAndHinge: gradients at a trial point are not supported.");
   if (e || g) {
-    launch_bah(*dev_, prec, x, d, t, e != nullptr, g, ws.rs(), s);
+    launch_bah(*dev_, prec, prec, x, d, t, e != nullptr, g, ws.rs(), s);
     if (g) constrainDevice(prec, x, g, 3 * (size_t)a.nnode, stream);
   }
   if (e) {
     double r[1];
     ws.fetch(s, 1, r);
+    *e = r[0];
+  }
+}
+
+// evaluate without the host synchronisation: the energy stays on the device
+// (e_dev, or the calling thread's reduction result slot 0).
+void BarAndHinge::evaluateAsync(int prec, const void* x, const void* d, double t, bool wantE, double* e_dev, void* g,
+                                void* stream) const {
+  if (!dev_) throw std::runtime_error("BarAndHinge: not initialised (construct a State first).");
+  if (g && d) throw std::invalid_argument("BarAndHinge: gradients at a trial point are not supported.");
+  if (!wantE && !g) return;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  launch_bah(*dev_, prec, prec, x, d, t, wantE, g, minim_amd::Workspace::current().rs(), s, wantE ? e_dev : nullptr);
+  if (g) constrainDevice(prec, x, g, 3 * (size_t)dev_->args.nnode, stream);
+}
+
+// The mixed schedule's second phase: FP64 coordinates and arithmetic, the
+// direction and the gradient in FP32.  Only single-DOF constraints (their
+// projection does not read the coordinates).
+bool BarAndHinge::supportsMixed() const { return !hasNormalConstraints(); }
+
+void BarAndHinge::evaluateMixedAsync(const void* x, const void* d, double t, bool wantE, double* e_dev, void* g,
+                                     void* stream) const {
+  if (!dev_) throw std::runtime_error("BarAndHinge: not initialised (construct a State first).");
+  if (g && d) throw std::invalid_argument("BarAndHinge: gradients at a trial point are not supported.");
+  if (!supportsMixed()) throw std::invalid_argument("BarAndHinge: no mixed evaluation with normal constraints.");
+  if (!wantE && !g) return;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  launch_bah(*dev_, FP64, FP32, x, d, t, wantE, g, minim_amd::Workspace::current().rs(), s, wantE ? e_dev : nullptr);
+  if (g) constrainDevice(FP32, nullptr, g, 3 * (size_t)dev_->args.nnode, stream);
+}
+
+void BarAndHinge::evaluateMixed(const void* x, const void* d, double t, double* e, void* g, void* stream) const {
+  evaluateMixedAsync(x, d, t, e != nullptr, nullptr, g, stream);
+  if (e) {
+    double r[1];
+    minim_amd::Workspace::current().fetch(static_cast<hipStream_t>(stream), 1, r);
     *e = r[0];
   }
 }
@@ -811,12 +854,12 @@ double BarAndHinge::timeKernel(int prec, const void* x, bool wantE, void* g, int
   if ((!wantE && !g) || reps < 1) throw std::invalid_argument("BarAndHinge: nothing to time");
   hipStream_t s = static_cast<hipStream_t>(stream);
   auto& ws = minim_amd::Workspace::current();
-  launch_bah(*dev_, prec, x, nullptr, 0.0, wantE, g, ws.rs(), s);
+  launch_bah(*dev_, prec, prec, x, nullptr, 0.0, wantE, g, ws.rs(), s);
   hipEvent_t e0, e1;
   MINIM_HIP_CHECK(hipEventCreate(&e0));
   MINIM_HIP_CHECK(hipEventCreate(&e1));
   MINIM_HIP_CHECK(hipEventRecord(e0, s));
-  for (int i = 0; i < reps; i++) launch_bah(*dev_, prec, x, nullptr, 0.0, wantE, g, ws.rs(), s);
+  for (int i = 0; i < reps; i++) launch_bah(*dev_, prec, prec, x, nullptr, 0.0, wantE, g, ws.rs(), s);
   MINIM_HIP_CHECK(hipEventRecord(e1, s));
   MINIM_HIP_CHECK(hipEventSynchronize(e1));
   float ms = 0;

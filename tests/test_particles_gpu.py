@@ -274,3 +274,69 @@ def test_bah_sheet_vs_oracle(gpu, prec):
     et = m.potential_eval(h, prec, xd.data_ptr(), x0.size, d_ptr=d.data_ptr(), t=t)
     xt = (xd + torch.tensor(t, dtype=dt, device="cuda") * d).double().cpu().numpy()
     assert abs(et - o.energy(xt)) <= TOL[prec] * abs(o.energy(xt))
+
+
+def test_bah_mixed_evaluation_and_minimisation(gpu):
+    """The mixed schedule's evaluation of the bar-and-hinge sheet: FP64
+    coordinates and arithmetic with FP32 direction and gradient.  The energy
+    is the FP64 evaluation's bit for bit, the gradient its FP32 rounding
+    (fixed DOFs zero), and a trial point x + t d with d in FP32 gives the FP64
+    energy at d widened to FP64.  L-BFGS m=10 with either schedule then
+    reaches the tolerance (the oracle's gradient at the coordinates left),
+    with the energy the oracle's at those coordinates (1e-12)."""
+    import torch
+    from bench import sheet_triangles
+
+    m = gpu
+    nside = 24
+    tri = sheet_triangles(nside, nside)
+    i, j = np.meshgrid(np.arange(nside), np.arange(nside), indexing="ij")
+    rng = np.random.default_rng(79)
+    x0 = np.stack([0.95 * (i - 0.5 * j), (np.sqrt(3) / 2) * j, rng.normal(0, 0.01, i.shape)], -1).reshape(-1)
+    edge = np.where((i == 0) | (i == nside - 1), 1, 0).reshape(-1).nonzero()[0]
+    fixed = np.stack([3 * edge, 3 * edge + 1, 3 * edge + 2], 1).reshape(-1)
+
+    def make():
+        p = m.BarAndHinge().setTriangulation(tri).setRigidity(1.0, 0.01).setLength0(1.0).setTheta0(np.pi)
+        p.setConstraints(fixed.tolist())
+        return p
+
+    pot = make()
+    n = x0.size
+    h = pot.handle(x0)
+    x = torch.tensor(x0 + rng.normal(0, 0.02, n), dtype=torch.float64, device="cuda")
+    d32 = torch.tensor(rng.normal(0, 0.01, n), dtype=torch.float32, device="cuda")
+    g64 = torch.empty(n, dtype=torch.float64, device="cuda")
+    g32 = torch.empty(n, dtype=torch.float32, device="cuda")
+    e64 = m.potential_eval(h, "fp64", x.data_ptr(), n, g_ptr=g64.data_ptr())
+    emx = m.potential_eval_mixed(h, x.data_ptr(), n, g_ptr=g32.data_ptr())
+    torch.cuda.synchronize()
+    assert emx == e64
+    assert torch.equal(g32, g64.float())
+    assert torch.all(g32[torch.tensor(fixed, device="cuda")] == 0)
+    t = 0.625
+    assert m.potential_eval_mixed(h, x.data_ptr(), n, d_ptr=d32.data_ptr(), t=t) == m.potential_eval(
+        h, "fp64", x.data_ptr(), n, d_ptr=d32.double().data_ptr(), t=t)
+
+    bonds, hinges = m.triangulation_lists(tri)
+    fx = np.zeros(n, np.uint8)
+    fx[fixed] = 1
+    o = OracleBaH(nside * nside, bonds, np.ones(len(bonds)), np.ones(len(bonds)), hinges,
+                  np.full(len(hinges), 0.01), np.full(len(hinges), np.pi), fixed=fx)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        s = m.State(make(), x0, "fp64")
+        lb = m.Lbfgs().setM(10).setMaxIter(200000).setPrecision(prec)
+        lb.minimise(s)
+        st = lb.stats
+        print(prec, st["iterations"], st["converged"], st["energy"], flush=True)
+        assert st["converged"], (prec, st)
+        xs = s.coords()
+        assert abs(s.energy() - o.energy(xs)) <= 1e-12 * abs(o.energy(x0))
+        g = o.gradient(xs)
+        assert np.sqrt(np.mean(g * g)) < 1.0001 * s.convergence
+        out[prec] = s.energy()
+    # The compressed sheet buckles into a very flat minimum: at a gradient RMS
+    # of 1e-6 its energy is pinned only to about 1e-5 of the initial energy,
+    # and the two schedules stop at different points of that valley.
+    assert abs(out["mixed"] - out["fp64"]) <= 1e-5 * abs(o.energy(x0))
