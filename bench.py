@@ -112,9 +112,9 @@ def cfg1_leg(m, with_cpu=True):
             times.append(time.perf_counter() - t)
         out[mode] = dict(seconds=times[-1], seconds_cold=times[0], iterations=mn.stats["iterations"],
                          converged=mn.stats["converged"], energy=s.energy())
-        # the same field as a batch of one: the device-controlled, graph-replayed
-        # iteration (Lbfgs.minimiseBatch), which needs one host synchronisation
-        # per 8 iterations instead of two per iteration
+        # the same field as a batch of one: the device-controlled iteration
+        # (Lbfgs.minimiseBatch), which needs one host synchronisation per 8
+        # iterations instead of two per iteration
         bt = []
         for r in range(2):
             s = m.State(pf, x0, "fp64")
@@ -559,13 +559,15 @@ def batch_leg(m, n_inst=512, workers=8, seed=4040, cpu_budget=10.0, per_state=Tr
     # the batch passes' block size, tuned per precision schedule on the first
     # 32 iterations of this batch (Lbfgs.autotuneBatch, outside the timed runs)
     tuned = {}
-    for prec, control in (("fp64", "graph"), ("mixed", "graph"), ("fp64", "resident")):
+    for prec, control in (("fp64", "device"), ("mixed", "device"), ("fp64", "resident")):
         tstates = [m.State(make_pf(j), inputs(j), "fp64") for j in range(n_inst)]
-        tuned[prec if control == "graph" else prec + "_resident"] = m.Lbfgs().setM(5).setMaxIter(20000).setPrecision(
+        tuned[prec if control == "device" else prec + "_resident"] = m.Lbfgs().setM(5).setMaxIter(20000).setPrecision(
             prec).setBatchControl(control).autotuneBatch(tstates, replays=32)
         del tstates
-    runs = [(p, "graph", 0) for p in ("fp64", "mixed")] + [(p, "graph", b) for b in budgets for p in ("fp64", "mixed")]
-    runs += [(p, "host", 0) for p in ("fp64", "mixed")] + [("fp64", "resident", 0)]
+    # device control (the default) with and without budgets; the same
+    # iteration replayed as a HIP graph, host control and resident control
+    runs = [(p, "device", 0) for p in ("fp64", "mixed")] + [(p, "device", b) for b in budgets for p in ("fp64", "mixed")]
+    runs += [(p, c, 0) for c in ("graph", "host") for p in ("fp64", "mixed")] + [("fp64", "resident", 0)]
     for prec, control, budget in runs:
         t = time.perf_counter()
         bstates = [m.State(make_pf(j), inputs(j), "fp64") for j in range(n_inst)]
@@ -581,7 +583,7 @@ def batch_leg(m, n_inst=512, workers=8, seed=4040, cpu_budget=10.0, per_state=Tr
         # from admission to convergence: the batch's counterpart of one call's duration
         bown = np.array([r["seconds"] - r["start_seconds"] for r in bres])
         bevals = sum(r["n_energy"] + r["n_gradient"] for r in bres)
-        key = prec + ("" if control == "graph" else "_%s_control" % control) + (
+        key = prec + ("" if control == "device" else "_%s_control" % control) + (
             "_budget_%gM_sites" % (budget / 1e6) if budget else "")
         batched[key] = dict(control=control, budget_sites=budget, host_syncs=rep["host_syncs"], replays=rep["replays"],
                              host_syncs_per_iteration=rep["host_syncs"] / max(1, rep["replays"]),

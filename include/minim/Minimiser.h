@@ -121,14 +121,14 @@ class Lbfgs : public NewMinimiser<Lbfgs> {
   // such instances as States on disjoint rank subsets (include/State.h:23,
   // src/Communicator.cpp:219-240).
   std::vector<MinimiseStats> minimiseBatch(const std::vector<State*>& states);
-  // How minimiseBatch drives the iterations.  "graph" (the default): the
+  // How minimiseBatch drives the iterations.  "device" (the default): the
   // per-instance decisions (two-loop recursion, Armijo tests, Gram update,
   // convergence) run as device kernels, one iteration is a fixed sequence of
-  // launches with per-instance phase masks, captured once as a HIP graph and
-  // replayed, and the host reads one counter every 8 iterations; "device":
-  // the same sequence launched without the graph; "host": the decisions on
-  // the host, one host synchronisation per phase.  All three give the same
-  // results bit for bit.  "resident" (FP64 schedule only): one workgroup per
+  // launches with per-instance phase masks, and the host reads the live
+  // flags every 8 iterations; "graph": the same sequence captured once as a
+  // HIP graph and replayed (measured 2-6 % slower than launching it);
+  // "host": the decisions on the host, one host synchronisation per phase.
+  // All three give the same results bit for bit.  "resident" (FP64 schedule only): one workgroup per
   // instance runs that instance's whole minimisation in a single launch, the
   // line search included, with no launches or host synchronisations between
   // iterations; its sums are split differently, so its results agree with
@@ -143,7 +143,8 @@ class Lbfgs : public NewMinimiser<Lbfgs> {
   Lbfgs& setBatchBudget(long sites);
   // Elements per thread of the batch's streaming passes (a block covers 256
   // times as many elements of one instance): 4, 8 or 16; 0 (the default) the
-  // size autotuneBatch stored for this batch's shape, else 8.  The block size
+  // size autotuneBatch stored for this batch's shape, else 8 (4 below 2^20
+  // elements in all, where the passes are bound by latency).  The block size
   // sets how an instance's sums are split over blocks, so results for
   // different sizes agree to rounding, not bit for bit.
   Lbfgs& setBatchBlock(int elementsPerThread);
@@ -171,7 +172,7 @@ class Lbfgs : public NewMinimiser<Lbfgs> {
   double _initStep = 1;
   std::string _precMode = "state";
   double _switchRms = 0;
-  std::string _batchControl = "graph";
+  std::string _batchControl = "device";
   long _batchBudget = 0;
   int _batchBlock = 0;
   // minimiseBatch with a given block size; with replayLimit > 0 it stops after

@@ -322,9 +322,9 @@ typedef struct minim_minimiser_params {
   double dt0;        /* Fire::setDt0: initial time step (0 = dt_max, the reference) */
   int max_force_dim; /* Fire::setMaxForceConvergence: converge on the largest norm over groups of
                         this many DOFs (0 = RMS gradient, the reference) */
-  int batch_control; /* Lbfgs::setBatchControl for minim_minimise_batch: 0 graph, 1 device, 2 host, 3 resident */
+  int batch_control; /* Lbfgs::setBatchControl for minim_minimise_batch: 0 device, 1 graph, 2 host, 3 resident */
   long batch_budget; /* Lbfgs::setBatchBudget: sites of running instances under device control (0 = all) */
-  int batch_block;   /* Lbfgs::setBatchBlock: elements per thread of the batch passes, 4, 8 or 16 (0 = tuned or 8) */
+  int batch_block;   /* Lbfgs::setBatchBlock: elements per thread of the batch passes, 4, 8 or 16 (0 = tuned, else 8, or 4 below 2^20 elements) */
 } minim_minimiser_params;
 
 typedef struct minim_result {

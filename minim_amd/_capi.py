@@ -120,7 +120,7 @@ SIGNATURES = {
 _lib = None
 
 
-ABI_VERSION = 9  # layout of the parameter/result structs below
+ABI_VERSION = 10  # layout of the parameter/result structs below
 
 
 def load():

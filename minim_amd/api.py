@@ -792,15 +792,15 @@ class Lbfgs(Minimiser):
         self.initStep = 1.0
         self.precisionMode = "state"
         self.switchRms = 0.0
-        self.batchControl = "graph"
+        self.batchControl = "device"
         self.batchBudget = 0
         self.batchBlock = 0
 
     def setBatchControl(self, mode):
-        """Lbfgs::setBatchControl: "graph" (device decisions, HIP-graph-captured
-        iterations), "device" (the same without the graph), "host", or
-        "resident" (one workgroup runs each instance's whole minimisation in
-        one launch; FP64 only)."""
+        """Lbfgs::setBatchControl: "device" (the default: device decisions, the
+        iteration's launches issued directly), "graph" (the same captured as a
+        HIP graph and replayed), "host", or "resident" (one workgroup runs each
+        instance's whole minimisation in one launch; FP64 only)."""
         if mode not in ("graph", "device", "host", "resident"):
             raise ValueError("Lbfgs: batch control must be graph, device, host or resident.")
         self.batchControl = mode
@@ -809,7 +809,8 @@ class Lbfgs(Minimiser):
 
     def setBatchBlock(self, elements_per_thread):
         """Lbfgs::setBatchBlock: elements per thread of the batch passes (4, 8
-        or 16; 0 = the autotuned size for the batch's shape, else 8)."""
+        or 16; 0 = the autotuned size for the batch's shape, else 8, or 4 below
+        2^20 elements in all)."""
         if elements_per_thread not in (0, 4, 8, 16):
             raise ValueError("Lbfgs: the batch block must be 4, 8 or 16 elements per thread (or 0).")
         self.batchBlock = int(elements_per_thread)
@@ -919,7 +920,7 @@ class Lbfgs(Minimiser):
         p.init_step = self.initStep
         p.precision_mode = {"state": 0, "fp64": 1, "fp32": 2, "mixed": 3}[self.precisionMode]
         p.switch_rms = self.switchRms
-        p.batch_control = {"graph": 0, "device": 1, "host": 2, "resident": 3}[self.batchControl]
+        p.batch_control = {"device": 0, "graph": 1, "host": 2, "resident": 3}[self.batchControl]
         p.batch_budget = self.batchBudget
         p.batch_block = self.batchBlock
         return p

@@ -49,7 +49,13 @@ namespace minim_amd {
 namespace {
 
 constexpr int kBS = 11;            // history slots (m + 1 with m <= 10)
-constexpr int kBlockDefault = 8;   // elements per thread when none is set or tuned
+// Elements per thread when none is set or tuned: 8, or 4 for batches below
+// kSmallBatch elements in all, whose passes have too few blocks to fill the
+// device and are bound by latency (a 128^2 field alone: 62.6 -> 56.5 us per
+// iteration).
+constexpr int kBlockDefault = 8;
+constexpr int kBlockSmall = 4;
+constexpr size_t kSmallBatch = size_t(1) << 20;
 constexpr int kKP = 2;             // stencil partials: energy, g.g
 constexpr int kNT = 4;             // line-search trials evaluated per pass: tau, tau/2, tau/4, tau/8
 constexpr int kKD = 2 + kBS;       // direction: d.d, d.g, d.Y_s
@@ -1265,7 +1271,7 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
     if (block == 0 && tune_lookup(tune_key(tkey, {(long)ni, (long)nTot, (long)_m}, mixed ? 1 : 0), tuned) &&
         !tuned.empty())
       block = tuned[0];
-    if (block != 4 && block != 8 && block != 16) block = kBlockDefault;
+    if (block != 4 && block != 8 && block != 16) block = nTot < kSmallBatch ? kBlockSmall : kBlockDefault;
   }
   const int kBE = block * kBlock;  // elements per block
   const Kernels K = mixed ? kernels_for<float>(block) : kernels_for<double>(block);

@@ -115,7 +115,7 @@ void apply_pf_params(const minim_pf_params* p, PF* pf) {
 extern "C" {
 
 const char* minim_last_error(void) { return g_err.c_str(); }
-int minim_abi_version(void) { return 9; }
+int minim_abi_version(void) { return 10; }
 
 // ---------------------------------------------------------------- potentials
 int minim_pf_create(const minim_pf_params* p, minim_potential** out) {
@@ -772,7 +772,7 @@ int minim_minimiser_create(const minim_minimiser_params* p, minim_minimiser** ou
       l->setMaxStep(p->max_step);
       l->setInitStep(p->init_step);
       l->setPrecision(modes[p->precision_mode], p->switch_rms);
-      static const char* controls[] = {"graph", "device", "host", "resident"};
+      static const char* controls[] = {"device", "graph", "host", "resident"};
       if (p->batch_control < 0 || p->batch_control > 3) throw std::invalid_argument("invalid batch control");
       l->setBatchControl(controls[p->batch_control]);
       l->setBatchBudget(p->batch_budget);

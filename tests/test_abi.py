@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(L, n), n
     assert set(names) == set(_capi.SIGNATURES), "ctypes table and header disagree"
-    assert L.minim_abi_version() == 9
+    assert L.minim_abi_version() == 10
 
 
 def _pf_handle(grid, nf, st, iw, density=0, res=1.0, fix=None):
@@ -238,7 +238,7 @@ def test_batch_entry_point_errors():
 
 
 def test_batch_control_values():
-    """batch_control 0..3 (graph, device, host, resident) are accepted and
+    """batch_control 0..3 (device, graph, host, resident) are accepted and
     anything else is rejected, in the C ABI and the Python mirror."""
     from minim_amd import _capi as A
 
