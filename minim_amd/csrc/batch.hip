@@ -643,7 +643,6 @@ struct BCtlArgs {
   const double* part2;                  // ... of the speculative trials
   const int* ablk;                      // the launch grid (block prefix sums)
   const int* map;                       // position -> instance
-  int* active;                          // instances running after the iteration
   int* live;                            // per instance: 1 while it runs
   unsigned long long* clock0;           // device wall clock at the start
 };
@@ -664,12 +663,17 @@ __device__ bool bc_trials(BCtl& C, const double* e) {
   return false;
 }
 
-// Instances admitted to the batch (map[0..count)): the gradient and energy at
-// their initial coordinates have been reduced into res[rPF], the state of
-// iteration 0.  setClock: the first admission, which starts the clock.
+__device__ void bc_direction(const BCtlArgs& A, int j, BCtl& C);
+
+// Instances admitted to the batch (map[0..count), one block each, its first
+// thread): the gradient and energy at their initial coordinates have been
+// reduced into res[rPF], the state of iteration 0, and the first direction
+// (steepest descent) is set.  setClock: the first admission, which starts
+// the clock.
 __global__ __launch_bounds__(kBlock) void bc_admit_kernel(BCtlArgs A, const int* __restrict__ map, int count,
                                                           int setClock) {
-  const int p = blockIdx.x * kBlock + threadIdx.x;
+  const int p = blockIdx.x;
+  if (threadIdx.x != 0) return;
   if (setClock && p == 0) *A.clock0 = wall_clock64();
   if (p >= count) return;
   const int j = map[p];
@@ -681,17 +685,18 @@ __global__ __launch_bounds__(kBlock) void bc_admit_kernel(BCtlArgs A, const int*
   C.started = 1;
   C.tstart = wall_clock64();
   A.live[j] = 1;
+  bc_direction(A, j, C);
 }
 
 // Instances that have just left FP32 coordinates take part in the fresh
 // phase: gradient and energy at the same point in FP64, overwriting g, and its
-// dots with the stored pairs.
+// dots with the stored pairs.  (The update left them with no phase; the
+// others keep the phase it set.)
 __global__ __launch_bounds__(kBlock) void bc_begin_kernel(BCtlArgs A) {
   const int j = blockIdx.x * kBlock + threadIdx.x;
   if (j >= A.ni) return;
   const BCtl& C = A.ctl[j];
   BDyn& D = A.dyn[j];
-  D.act = 0;
   if (!C.started || C.done || !C.fresh) return;
   D.j = j;
   D.gcur = C.gcur;
@@ -716,15 +721,13 @@ __device__ __forceinline__ void ctl_copy(BCtl* __restrict__ dst, const BCtl* __r
   for (int i = threadIdx.x; i < W; i += kBlock) d[i] = s[i];
 }
 
-__device__ void bc_direction(const BCtlArgs& A, int j, BCtl& C);
-
-// The fresh phase's results, then the direction coefficients (Lbfgs.cpp:84-127),
-// one block per position.  An instance still searching along its last
-// direction goes on searching.
+// The mixed schedule: the fresh phase's results, then the direction
+// coefficients (Lbfgs.cpp:84-127), one block per position, for the instances
+// that took part in the fresh phase (the update set the others' directions).
 __global__ __launch_bounds__(kBlock) void bc_direction_kernel(BCtlArgs A) {
   __shared__ BCtl Cs;
   const int p = blockIdx.x, j = A.map[p];
-  if (p == 0 && threadIdx.x == 0) *A.active = 0;
+  if (!(A.dyn[j].act & kActFresh)) return;  // uniform over the block
   ctl_copy(&Cs, A.ctl + j);
   __syncthreads();
   if (threadIdx.x == 0) bc_direction(A, j, Cs);
@@ -891,26 +894,25 @@ __global__ __launch_bounds__(kBlock) void bc_consume_kernel(BCtlArgs A) {
 
 // New energy and the Gram update (Lbfgs.cpp:58-80), the mixed schedule's
 // switch, and Minimiser::minimise's end tests (Minimiser.cpp:27-38,
-// Lbfgs.cpp:130-133); then the running instances are counted.
+// Lbfgs.cpp:130-133); then the next iteration's direction, except for an
+// instance that has just switched to FP64 coordinates, which recomputes its
+// gradient first (no phase until bc_begin_kernel).  An instance still
+// searching keeps its phase.
 __device__ void bc_update(const BCtlArgs& A, int j, BCtl& C, const double* r);
 
 __global__ __launch_bounds__(kBlock) void bc_update_kernel(BCtlArgs A) {
   __shared__ double rs[kKS];
   __shared__ BCtl Cs;
   const int p = blockIdx.x, j = A.map[p];
-  const BDyn& D = A.dyn[j];
-  const bool stepping = (D.act & kActStep) != 0;  // uniform over the block
-  if (!stepping) {
-    const BCtl& C = A.ctl[j];
-    if (threadIdx.x == 0 && C.started && !C.done) atomicAdd(A.active, 1);
-    return;
-  }
+  BDyn& D = A.dyn[j];
+  if (!(D.act & kActStep)) return;  // uniform over the block
   ctl_copy(&Cs, A.ctl + j);
   reduce_rows(A.part, A.ablk[p], A.ablk[p + 1] - A.ablk[p], kKS, rs);
   __syncthreads();
   if (threadIdx.x == 0) {
     bc_update(A, j, Cs, rs);
-    if (Cs.started && !Cs.done) atomicAdd(A.active, 1);
+    if (Cs.fresh && !Cs.done) D.act = 0;
+    else bc_direction(A, j, Cs);
   }
   __syncthreads();
   ctl_copy(A.ctl + j, &Cs);
@@ -1437,12 +1439,11 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
       c0[j] = c;
     }
     const size_t rPF = 0, rF = rPF + (size_t)ni * kKP, rEnd = rF + (size_t)ni * kKF;
-    // counter, start clock, then the live flags
+    // the start clock at byte 8, the live flags from byte 64
     const size_t cntBytes = 64 + sizeof(int) * (size_t)ni;
     DeviceBuffer ablkA(sizeof(int) * (ni + 1)), mapA(sizeof(int) * ni), dynA(sizeof(BDyn) * ni),
         ctlA(sizeof(BCtl) * ni), resA(8 * rEnd), cntA(cntBytes), part2(sizeof(double) * (size_t)totalBlocks * kNT);
     Pinned cntH(cntBytes);
-    int* cnt = static_cast<int*>(cntH.p);
     int* liveH = reinterpret_cast<int*>(static_cast<char*>(cntH.p) + 64);
     std::memset(cntH.p, 0, cntBytes);
     MINIM_HIP_CHECK(hipMemcpyAsync(dynA.get(), dy0.data(), sizeof(BDyn) * ni, hipMemcpyHostToDevice, st));
@@ -1480,7 +1481,6 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
     A.part2 = static_cast<const double*>(part2.get());
     A.ablk = G.ablk;
     A.map = G.map;
-    A.active = static_cast<int*>(cntA.get());
     A.clock0 = reinterpret_cast<unsigned long long*>(static_cast<char*>(cntA.get()) + 8);
     A.live = reinterpret_cast<int*>(static_cast<char*>(cntA.get()) + 64);
     const int cb = (ni + kBlock - 1) / kBlock;
@@ -1523,8 +1523,8 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
         ctl(bc_begin_kernel);
         run(K.fresh, kActFresh);
         fin(kActFresh, kKF, rF);
+        ctlp(bc_direction_kernel);
       }
-      ctlp(bc_direction_kernel);
       run(K.dir, kActDir);
       if (spec) {
         BBufs b = G;
@@ -1567,7 +1567,7 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
       for (int j = 0; j < ni; j++) order[j] = j;
       std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return in[a].bi.n > in[b].bi.n; });
       MINIM_HIP_CHECK(hipMemcpyAsync(mapA.get(), order.data(), sizeof(int) * ni, hipMemcpyHostToDevice, cs));
-      bc_admit_kernel<<<1, 1, 0, cs>>>(A, nullptr, 0, 1);  // the start clock
+      bc_admit_kernel<<<1, kWave, 0, cs>>>(A, nullptr, 0, 1);  // the start clock
       MINIM_HIP_CHECK(hipGetLastError());
       // autotuneBatch's bounded runs: at most replayLimit iterations each
       if (replayLimit > 0) A.maxIter = (int)std::min<long>(maxIter, replayLimit - 1);
@@ -1612,7 +1612,7 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
       }
       run(K.pf_eg, 0);
       fin(0, kKP, rPF);
-      bc_admit_kernel<<<(naCur + kBlock - 1) / kBlock, kBlock, 0, cs>>>(A, G.map, naCur, firstAdmission ? 1 : 0);
+      bc_admit_kernel<<<naCur, kWave, 0, cs>>>(A, G.map, naCur, firstAdmission ? 1 : 0);
       MINIM_HIP_CHECK(hipGetLastError());
       firstAdmission = false;
       for (int j : add) running[j] = 1;
@@ -1649,7 +1649,7 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
         }
         if (running[j]) liveBlocks += in[j].nblk;
       }
-      if (cnt[0] == 0 && next == ni) break;
+      if (liveBlocks == 0 && next == ni) break;
       if (replayLimit > 0 && replays >= replayLimit) break;
       // every iteration moves each running instance on by at least one
       // line-search pass, and an iteration takes at most 4 of them
