@@ -25,8 +25,10 @@
 // their loads are unconditional.  Each instance's (2m+1)^2 Gram data and its
 // two-loop recursion live either on the device (device control: one thread
 // of a block per instance takes the decisions, and an iteration is a fixed
-// launch sequence replayed as a HIP graph) or on the host (host control: one
-// host synchronisation per phase serves all instances).
+// launch sequence, launched as it is or replayed as a HIP graph; resident
+// control: one workgroup runs an instance's whole minimisation) or on the
+// host (host control: one host synchronisation per phase serves all
+// instances).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1406,16 +1408,17 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
   int maxIters = 0;
 
   if (_batchControl != "host") {
-    // ---- device control: the control kernels (one thread per instance)
-    // decide which instances take part in each phase, and one iteration is a
-    // fixed sequence of launches -- captured as a HIP graph and replayed
-    // ("graph"), or launched as it is ("device").  An instance whose line
-    // search goes past the speculative trials and one further pass carries it
-    // over to the next iteration, where it takes no new direction.  The host
-    // reads the running instances every kChunk iterations.  The launch grid
-    // covers the running instances only (block prefix sums and a position ->
-    // instance map): when their blocks have fallen below 4/5 of it, it is laid
-    // out again and the graph captured again.  Where an instance's rows lie in
+    // ---- device control: the control kernels (one block per instance, its
+    // first thread deciding) set which instances take part in each phase, and
+    // one iteration is a fixed sequence of launches -- launched as it is
+    // ("device", the default) or captured as a HIP graph and replayed
+    // ("graph").  An instance whose line search goes past the speculative
+    // trials and one further pass carries it over to the next iteration, where
+    // it takes no new direction.  The host reads the live flags every kChunk
+    // iterations.  The launch grid covers the running instances only (block
+    // prefix sums and a position -> instance map): when their blocks have
+    // fallen below 4/5 of it, it is laid out again (and the graph captured
+    // again).  Where an instance's rows lie in
     // the partials does not change its sums.
     //
     // With a budget (setBatchBudget), instances are admitted smallest first
