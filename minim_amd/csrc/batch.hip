@@ -146,8 +146,8 @@ __device__ __forceinline__ int find_pos(const int* __restrict__ ablk, int na, in
 }
 
 // Block sums of the values whose bit is set in `on` (the others are 0),
-// written to part[blockIdx.x * K + k].
-template <int K>
+// written to part[blockIdx.x * S + k] (rows of S >= K values).
+template <int K, int S = K>
 __device__ __forceinline__ void block_partials(const double (&v)[K], unsigned long long on, double* part) {
   __shared__ double lds[(kBlock / kWave) * K];
   const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
@@ -166,20 +166,21 @@ __device__ __forceinline__ void block_partials(const double (&v)[K], unsigned lo
     const int k = threadIdx.x;
     double s = lds[k];
     for (int w = 1; w < kBlock / kWave; w++) s += lds[w * K + k];
-    part[(size_t)blockIdx.x * K + k] = s;
+    part[(size_t)blockIdx.x * S + k] = s;
   }
 }
 
-// out[k] = sum of the block rows [b0, b0 + nb) of part (K per row), by the
-// whole block: lane l of a wavefront adds rows l, l + 64, ..., then the 64
-// lane sums are added in a fixed tree, so the result depends only on those
-// rows, wherever they lie in part.
+// out[k] = sum of the block rows [b0, b0 + nb) of part (K per row, or
+// rows of `stride` values), by the whole block: lane l of a wavefront adds
+// rows l, l + 64, ..., then the 64 lane sums are added in a fixed tree, so the
+// result depends only on those rows, wherever they lie in part.
 __device__ __forceinline__ void reduce_rows(const double* __restrict__ part, int b0, int nb, int K,
-                                            double* __restrict__ out) {
+                                            double* __restrict__ out, int stride = 0) {
   const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  const size_t S = stride > 0 ? (size_t)stride : (size_t)K;
   for (int k = w; k < K; k += kBlock / kWave) {
     double s = 0;
-    for (int b = lane; b < nb; b += kWave) s += part[(size_t)(b0 + b) * K + k];
+    for (int b = lane; b < nb; b += kWave) s += part[(size_t)(b0 + b) * S + k];
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kWave);
     if (lane == 0) out[k] = s;
@@ -603,6 +604,32 @@ __global__ __launch_bounds__(kBlock) void bstep_kernel(BBufs B) {
   block_partials<kKS>(acc, bstep_mask(q.np), B.part);
 }
 
+// Device control: one launch for the line-search pass of the instances still
+// searching and the step pass of those whose search has ended (the phase is
+// uniform over an instance's blocks).  Both write rows of kKS partials.
+template <int BR, typename TV>
+__global__ __launch_bounds__(kBlock) void bts_kernel(BBufs B) {
+  const int p = find_pos(B.ablk, B.na, blockIdx.x);
+  const BDyn& q = B.dyn[B.map[p]];
+  if (!(q.act & (kActSearch | kActStep))) return;
+  const BInst I = B.inst[q.j];
+  const int lb = blockIdx.x - B.ablk[p];
+  if (q.act & kActStep) {
+    double acc[kKS];
+#pragma unroll
+    for (int k = 0; k < kKS; k++) acc[k] = 0;
+    bstep_any<BR, TV>(B, q, I, lb, acc);
+    block_partials<kKS>(acc, bstep_mask(q.np), B.part);
+  } else {
+    double acc[kNT];
+#pragma unroll
+    for (int k = 0; k < kNT; k++) acc[k] = 0;
+    if (q.lp) btrial_body<BR, float, TV>(B, q, I, lb, acc);
+    else btrial_body<BR, double, TV>(B, q, I, lb, acc);
+    block_partials<kNT, kKS>(acc, (1ull << kNT) - 1, B.part);
+  }
+}
+
 // ---------------------------------------------------------------- device control
 // The per-instance decisions of an iteration (two-loop recursion on the Gram
 // data, the Armijo tests of the backtracking search, the Gram update, the
@@ -883,41 +910,39 @@ __device__ __forceinline__ void bc_consume(BCtl& C, BDyn& D, const double* rt) {
   }
 }
 
-__global__ __launch_bounds__(kBlock) void bc_consume_kernel(BCtlArgs A) {
-  __shared__ double rt[kNT];
-  const int p = blockIdx.x, j = A.map[p];
-  BCtl& C = A.ctl[j];
-  BDyn& D = A.dyn[j];
-  if (!(D.act & kActSearch)) return;
-  reduce_rows(A.part, A.ablk[p], A.ablk[p + 1] - A.ablk[p], kNT, rt);
-  __syncthreads();
-  if (threadIdx.x == 0) bc_consume(C, D, rt);
-}
-
 // New energy and the Gram update (Lbfgs.cpp:58-80), the mixed schedule's
 // switch, and Minimiser::minimise's end tests (Minimiser.cpp:27-38,
-// Lbfgs.cpp:130-133); then the next iteration's direction, except for an
-// instance that has just switched to FP64 coordinates, which recomputes its
-// gradient first (no phase until bc_begin_kernel).  An instance still
-// searching keeps its phase.
+// Lbfgs.cpp:130-133).
 __device__ void bc_update(const BCtlArgs& A, int j, BCtl& C, const double* r);
 
-__global__ __launch_bounds__(kBlock) void bc_update_kernel(BCtlArgs A) {
+// After bts_kernel, one block per position.  An instance that took its step:
+// the update, then the next iteration's direction, except for an instance
+// that has just switched to FP64 coordinates, which recomputes its gradient
+// first (no phase until bc_begin_kernel).  An instance that searched: the
+// trials' Armijo tests (bc_consume); if its search has ended it takes the
+// step in the next launch sequence, where it takes no new direction.
+__global__ __launch_bounds__(kBlock) void bc_finish_kernel(BCtlArgs A) {
   __shared__ double rs[kKS];
   __shared__ BCtl Cs;
   const int p = blockIdx.x, j = A.map[p];
   BDyn& D = A.dyn[j];
-  if (!(D.act & kActStep)) return;  // uniform over the block
-  ctl_copy(&Cs, A.ctl + j);
-  reduce_rows(A.part, A.ablk[p], A.ablk[p + 1] - A.ablk[p], kKS, rs);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    bc_update(A, j, Cs, rs);
-    if (Cs.fresh && !Cs.done) D.act = 0;
-    else bc_direction(A, j, Cs);
+  const int b0 = A.ablk[p], nb = A.ablk[p + 1] - b0;
+  if (D.act & kActStep) {  // uniform over the block
+    ctl_copy(&Cs, A.ctl + j);
+    reduce_rows(A.part, b0, nb, kKS, rs);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      bc_update(A, j, Cs, rs);
+      if (Cs.fresh && !Cs.done) D.act = 0;
+      else bc_direction(A, j, Cs);
+    }
+    __syncthreads();
+    ctl_copy(A.ctl + j, &Cs);
+  } else if (D.act & kActSearch) {
+    reduce_rows(A.part, b0, nb, kNT, rs, kKS);
+    __syncthreads();
+    if (threadIdx.x == 0) bc_consume(A.ctl[j], D, rs);
   }
-  __syncthreads();
-  ctl_copy(A.ctl + j, &Cs);
 }
 
 __device__ void bc_update(const BCtlArgs& A, int j, BCtl& C, const double* rsum) {
@@ -1175,14 +1200,15 @@ struct Inst {
 
 using KFn = void (*)(BBufs);
 struct Kernels {
-  KFn pf_eg, pf_trial, dir, step, fresh, round;
+  KFn pf_eg, pf_trial, dir, step, fresh, round, ts;
   void (*resident5)(BBufs, BCtlArgs);   // FP64 only, m <= 5
   void (*resident10)(BBufs, BCtlArgs);  // FP64 only, m <= 10
 };
 template <int BR, typename TV>
 Kernels kernels_br() {
-  Kernels k{bpf_kernel<BR, 3, TV>, btrial_kernel<BR, TV>, bdir_kernel<BR, TV>, bstep_kernel<BR, TV>,
-            bpf_kernel<BR, 11, TV>, bround_kernel<BR>, nullptr, nullptr};
+  Kernels k{bpf_kernel<BR, 3, TV>,  btrial_kernel<BR, TV>, bdir_kernel<BR, TV>, bstep_kernel<BR, TV>,
+            bpf_kernel<BR, 11, TV>, bround_kernel<BR>,     bts_kernel<BR, TV>,  nullptr,
+            nullptr};
   if constexpr (sizeof(TV) == 8) {
     k.resident5 = bres_kernel<BR, 5, TV>;
     k.resident10 = bres_kernel<BR, 10, TV>;
@@ -1537,10 +1563,8 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
         MINIM_HIP_CHECK(hipGetLastError());
       }
       ctlp(bc_after_dir_kernel);
-      run(K.pf_trial, kActSearch);
-      ctlp(bc_consume_kernel);
-      run(K.step, kActStep);
-      ctlp(bc_update_kernel);
+      run(K.ts, kActSearch | kActStep);
+      ctlp(bc_finish_kernel);
     };
     struct GraphGuard {
       hipGraph_t g = nullptr;
