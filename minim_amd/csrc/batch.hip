@@ -784,6 +784,7 @@ __device__ void bc_direction(const BCtlArgs& A, int j, BCtl& C) {
     }
     C.fresh = 0;
     C.nGradient++;
+    A.live[j] = 1;
   }
   int spare = -1;
   for (int s = 0; s <= A.m; s++)
@@ -1001,6 +1002,7 @@ __device__ void bc_update(const BCtlArgs& A, int j, BCtl& C, const double* rsum)
       if (rms < sw || C.stall >= 2) {
         C.lp = 0;
         C.fresh = 1;  // no convergence test until g is recomputed
+        A.live[j] = 2;  // running, waiting for the fresh phase
       }
     } else {
       C.iterFP64++;
@@ -1547,8 +1549,12 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
       k<<<naCur, kBlock, 0, cs>>>(A);
       MINIM_HIP_CHECK(hipGetLastError());
     };
-    auto iteration = [&] {
-      if (mixed) {  // FP64 runs never recompute a gradient
+    // An iteration: 5 launches (FP64, with the speculative trials).  With
+    // fresh (the mixed schedule, only when an instance has switched to FP64
+    // coordinates since the host last looked), the fresh phase first; a
+    // switched instance waits for it with no phase, at most kChunk iterations.
+    auto iteration = [&](bool fresh) {
+      if (fresh) {
         ctl(bc_begin_kernel);
         run(K.fresh, kActFresh);
         fin(kActFresh, kKF, rF);
@@ -1581,7 +1587,7 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
     auto capture = [&] {
       graph.reset();
       MINIM_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-      iteration();
+      iteration(false);
       MINIM_HIP_CHECK(hipStreamEndCapture(cs, &graph.g));
       MINIM_HIP_CHECK(hipGraphInstantiate(&graph.e, graph.g, nullptr, nullptr, 0));
       captures++;
@@ -1654,13 +1660,16 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
     start(admit());
     regrid();
     long lastAdmission = 0;
+    bool freshPending = false;
     for (;;) {
       for (int k = 0; k < kChunk; k++) {
-        if (useGraph) {
+        if (k == 0 && freshPending) {
+          iteration(true);
+        } else if (useGraph) {
           MINIM_HIP_CHECK(hipGraphLaunch(graph.e, cs));
           graphLaunches++;
         } else {
-          iteration();
+          iteration(false);
         }
         replays++;
       }
@@ -1668,6 +1677,7 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
       MINIM_HIP_CHECK(hipStreamSynchronize(cs));
       hostSyncs++;
       long liveBlocks = 0;
+      freshPending = false;
       for (int j = 0; j < ni; j++) {
         if (running[j] && !liveH[j]) {  // finished
           running[j] = 0;
@@ -1675,6 +1685,7 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
           finished++;
         }
         if (running[j]) liveBlocks += in[j].nblk;
+        if (running[j] && liveH[j] == 2) freshPending = true;
       }
       if (liveBlocks == 0 && next == ni) break;
       if (replayLimit > 0 && replays >= replayLimit) break;

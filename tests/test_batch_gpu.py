@@ -221,7 +221,12 @@ def test_batch_controls_agree_bit_for_bit(gpu, kw):
             for k in ("energy", "iterations", "converged", "n_energy", "n_gradient", "iter_fp32", "iter_fp64"):
                 assert r[i][k] == rg[i][k], (control, i, k, r[i][k], rg[i][k])
     # one host synchronisation every eight iterations under device control
-    assert repg["graph_launches"] == repg["replays"] and repg["host_syncs"] * 8 == repg["replays"]
+    # (the mixed schedule launches the fresh phase's iterations outside the graph)
+    assert repg["host_syncs"] * 8 == repg["replays"]
+    if kw.get("precision") == "mixed":
+        assert 0 < repg["graph_launches"] < repg["replays"]
+    else:
+        assert repg["graph_launches"] == repg["replays"]
     assert out["host"][2]["host_syncs"] >= out["host"][2]["iterations"]
 
 
