@@ -361,11 +361,13 @@ int minim_minimise_log(minim_minimiser* m, minim_state* s, const char* log_type,
 /* Independent L-BFGS minimisations of n_states States at once (Lbfgs::minimiseBatch):
  * each with the semantics of minim_minimise on its own State (Lbfgs.cpp:41-133,
  * linesearch.cpp:8-29, Minimiser.cpp:27-38), one launch per phase over every
- * instance still running.  Replaces the reference's States on disjoint rank
- * subsets (include/State.h:23, src/Communicator.cpp:219-240).  Every State
- * holds an initialised one-fluid PhaseField without solid nodes, extra terms or
- * fixed fluid, in FP64; m <= 10.  Results stay in each State's device
- * coordinates; results[i] (may be NULL) describes states[i]. */
+ * instance still running (batch_control and batch_budget in the parameters).
+ * Replaces the reference's States on disjoint rank subsets
+ * (include/State.h:23, src/Communicator.cpp:219-240).  Every State holds an
+ * initialised one-fluid PhaseField without solid nodes, extra terms or fixed
+ * fluid, in FP64; m <= 10.  Results stay in each State's device coordinates;
+ * results[i] (may be NULL) describes states[i], with start_seconds the time
+ * at which it was admitted. */
 int minim_minimise_batch(minim_minimiser* m, minim_state* const* states, int n_states, minim_result* results);
 /* How the last minim_minimise_batch ran (no reference counterpart): the most
  * iterations of any instance, the iterations executed, the host
