@@ -1474,8 +1474,7 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
     const size_t cntBytes = 64 + sizeof(int) * (size_t)ni;
     DeviceBuffer ablkA(sizeof(int) * (ni + 1)), mapA(sizeof(int) * ni), dynA(sizeof(BDyn) * ni),
         ctlA(sizeof(BCtl) * ni), resA(8 * rEnd), cntA(cntBytes), part2(sizeof(double) * (size_t)totalBlocks * kNT);
-    Pinned cntH(cntBytes);
-    int* liveH = reinterpret_cast<int*>(static_cast<char*>(cntH.p) + 64);
+    Pinned cntH(cntBytes), cntH2(cntBytes);  // the flags of two chunks in turn
     std::memset(cntH.p, 0, cntBytes);
     MINIM_HIP_CHECK(hipMemcpyAsync(dynA.get(), dy0.data(), sizeof(BDyn) * ni, hipMemcpyHostToDevice, st));
     MINIM_HIP_CHECK(hipMemcpyAsync(ctlA.get(), c0.data(), sizeof(BCtl) * ni, hipMemcpyHostToDevice, st));
@@ -1661,6 +1660,23 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
     regrid();
     long lastAdmission = 0;
     bool freshPending = false;
+    // The flags are read one chunk late: chunk c is launched, then the host
+    // waits for chunk c-1's copy of the flags, so the device always has a
+    // chunk queued.  Decisions on them (finished instances, admissions, the
+    // fresh phase) take effect a chunk later; an instance's result does not
+    // depend on when.  After an admission or a new layout (which drain the
+    // stream) the flags still in flight are dropped: they predate it.
+    struct Events {
+      hipEvent_t e[2] = {nullptr, nullptr};
+      Events() {
+        for (auto& x : e) MINIM_HIP_CHECK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+      }
+      ~Events() {
+        for (auto& x : e) (void)hipEventDestroy(x);
+      }
+    } ev;
+    void* flagsH[2] = {cntH.p, cntH2.p};
+    long chunk = 0, unread = -1;  // the chunk whose flags are in flight, or -1
     for (;;) {
       for (int k = 0; k < kChunk; k++) {
         if (k == 0 && freshPending) {
@@ -1673,9 +1689,16 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
         }
         replays++;
       }
-      MINIM_HIP_CHECK(hipMemcpyAsync(cntH.p, cntA.get(), cntBytes, hipMemcpyDeviceToHost, cs));
-      MINIM_HIP_CHECK(hipStreamSynchronize(cs));
+      const int cb = (int)(chunk & 1);
+      MINIM_HIP_CHECK(hipMemcpyAsync(flagsH[cb], cntA.get(), cntBytes, hipMemcpyDeviceToHost, cs));
+      MINIM_HIP_CHECK(hipEventRecord(ev.e[cb], cs));
+      const long readable = unread;
+      unread = chunk++;
+      if (readable < 0) continue;  // nothing to read yet: queue another chunk first
+      const int rb = (int)(readable & 1);
+      MINIM_HIP_CHECK(hipEventSynchronize(ev.e[rb]));
       hostSyncs++;
+      const int* liveH = reinterpret_cast<const int*>(static_cast<const char*>(flagsH[rb]) + 64);
       long liveBlocks = 0;
       freshPending = false;
       for (int j = 0; j < ni; j++) {
@@ -1691,14 +1714,16 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
       if (replayLimit > 0 && replays >= replayLimit) break;
       // every iteration moves each running instance on by at least one
       // line-search pass, and an iteration takes at most 4 of them
-      if (replays - lastAdmission > 4L * ((long)maxIter + 2) + kChunk) break;
+      if (replays - lastAdmission > 4L * ((long)maxIter + 2) + 2 * kChunk) break;
       const std::vector<int> add = admit();
       if (!add.empty()) {
         start(add);
         regrid();
         lastAdmission = replays;
+        unread = -1;
       } else if (5 * liveBlocks <= 4 * (long)nbCur) {
         regrid();
+        unread = -1;
       }
     }
     }  // resident or graph / device

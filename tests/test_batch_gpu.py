@@ -220,9 +220,11 @@ def test_batch_controls_agree_bit_for_bit(gpu, kw):
             assert np.array_equal(x[i], xg[i]), (control, i)
             for k in ("energy", "iterations", "converged", "n_energy", "n_gradient", "iter_fp32", "iter_fp64"):
                 assert r[i][k] == rg[i][k], (control, i, k, r[i][k], rg[i][k])
-    # one host synchronisation every eight iterations under device control
+    # one host synchronisation every eight iterations under device control:
+    # the flags of every 8-iteration chunk are read while the next one runs,
+    # except for the first chunk and each chunk launched before a new layout
     # (the mixed schedule launches the fresh phase's iterations outside the graph)
-    assert repg["host_syncs"] * 8 == repg["replays"]
+    assert 0 < repg["replays"] - 8 * repg["host_syncs"] <= 8 * (1 + repg["captures"])
     if kw.get("precision") == "mixed":
         assert 0 < repg["graph_launches"] < repg["replays"]
     else:
@@ -254,7 +256,7 @@ def test_batch_256_instances_vs_oracle(gpu):
         cs.append(dict(grid=g, w=float(rng.uniform(1.0, 2.0)), x0=x0))
     states, res, rep = run_batch(m, cs, "graph")
     print("report", rep, flush=True)
-    assert rep["host_syncs"] * 8 == rep["replays"]
+    assert 0 < rep["replays"] - 8 * rep["host_syncs"] <= 8 * (1 + rep["captures"])
     assert rep["captures"] >= 2  # the grid shrank as instances finished
     n_iface = 0
     for i, (c, s, r) in enumerate(zip(cs, states, res)):
