@@ -1660,6 +1660,7 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
     regrid();
     long lastAdmission = 0;
     bool freshPending = false;
+    long freshChunk = -1;  // the last chunk that began with the fresh phase
     // The flags are read one chunk late: chunk c is launched, then the host
     // waits for chunk c-1's copy of the flags, so the device always has a
     // chunk queued.  Decisions on them (finished instances, admissions, the
@@ -1681,6 +1682,7 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
       for (int k = 0; k < kChunk; k++) {
         if (k == 0 && freshPending) {
           iteration(true);
+          freshChunk = chunk;
         } else if (useGraph) {
           MINIM_HIP_CHECK(hipGraphLaunch(graph.e, cs));
           graphLaunches++;
@@ -1708,7 +1710,8 @@ std::vector<MinimiseStats> Lbfgs::batchRun(const std::vector<State*>& states, in
           finished++;
         }
         if (running[j]) liveBlocks += in[j].nblk;
-        if (running[j] && liveH[j] == 2) freshPending = true;
+        // a switch seen in chunk r is served by a fresh phase in any later chunk
+        if (running[j] && liveH[j] == 2 && freshChunk <= readable) freshPending = true;
       }
       if (liveBlocks == 0 && next == ni) break;
       if (replayLimit > 0 && replays >= replayLimit) break;
