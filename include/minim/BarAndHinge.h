@@ -40,6 +40,14 @@ class BarAndHinge : public NewPotential<BarAndHinge> {
   void evaluateMixed(const void* x, const void* d, double t, double* e, void* g, void* stream) const override;
   void evaluateMixedAsync(const void* x, const void* d, double t, bool wantE, double* e_dev, void* g,
                           void* stream) const override;
+  // Fused accepted step: x + t*d into xout (formed as the line-search trials
+  // form it), then energy and gradient there in one energy pass that keeps
+  // what the gradient gather needs; d and g in precision pv (px, or FP32
+  // against FP64 coordinates without normal constraints).
+  bool supportsStep() const override;
+  bool stepDefersGradient() const override;
+  void evaluateStepAsync(int px, int pv, const void* x, const void* d, double t, void* xout, void* g,
+                         void* stream) const override;
   double timeKernel(int prec, const void* x, bool wantE, void* g, int reps, void* stream) const override;
 
   // Element lists after init, in the reference's element order.

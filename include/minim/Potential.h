@@ -202,6 +202,12 @@ class Potential {
   // from supportsStep(); the default evaluateStepAsync throws
   // std::invalid_argument.
   virtual bool supportsStep() const { return false; }
+  // True when evaluateStepAsync also takes g = nullptr: the energy at
+  // x + t*d and x + t*d in xout, keeping what the gradient there needs, so
+  // that the next gradient evaluation at xout (with no other evaluation in
+  // between) costs less.  The line search then pays for that gradient only
+  // when it accepts the trial.
+  virtual bool stepDefersGradient() const { return false; }
   virtual void evaluateStepAsync(int px, int pv, const void* x, const void* d, double t, void* xout, void* g,
                                  void* stream) const;
   // The gradient launched without a host synchronisation, for potentials

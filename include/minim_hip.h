@@ -229,7 +229,9 @@ int minim_potential_eval_mixed(const minim_potential* pot, const void* x, size_t
  * (host, may be NULL) and constrained gradient g at x + t*d, with x + t*d
  * written to x_out (precision_x, like x; d and g in precision_v).
  * MINIM_EINVAL for potentials without it (PhaseField on the tiled kernel has
- * it: nFluid <= 4, no solid nodes or extra terms).  Synchronises the stream. */
+ * it: nFluid <= 4, no solid nodes or extra terms; BarAndHinge has it, with
+ * precision_v != precision_x only without normal constraints).  Synchronises
+ * the stream. */
 int minim_potential_eval_step(const minim_potential* pot, int precision_x, int precision_v, const void* x,
                               size_t ndof, const void* d, double t, void* x_out, double* e, void* g, void* stream);
 /* Benchmark support (no reference counterpart): average duration in ms,

@@ -396,6 +396,12 @@ struct BarAndHinge::Dev {
   DeviceBuffer ht;      // per-hinge store of the last gradient evaluation (HStore, either precision)
   DeviceBuffer xtrial;  // node coordinates at a line-search trial point (FP64-sized)
   std::vector<int> activeBars, activeHinges;  // element index -> original bond/hinge index
+  // The coordinates (and their precision) for which a fused step's energy
+  // pass kept the hinge store with no gradient yet; any other evaluation
+  // clears it, and the next gradient-only evaluation at exactly those
+  // coordinates skips the hinge pass.
+  mutable const void* keptAt = nullptr;
+  mutable int keptPrec = -1;
 };
 
 BarAndHinge& BarAndHinge::setTriangulation(const vector2d<int>& triList) {
@@ -721,10 +727,12 @@ namespace {
 // arithmetic; pv: the direction d and the gradient g (FP32 against FP64
 // coordinates in the mixed schedule).
 void launch_bah(const BarAndHinge::Dev& dv, int prec, int pv, const void* x, const void* d, double t, bool wantE,
-                void* g, minim_amd::RedScratch rs, hipStream_t s, double* eDev = nullptr) {
+                void* g, minim_amd::RedScratch rs, hipStream_t s, double* eDev = nullptr, bool keep = false) {
   const auto& a = dv.args;
   if (pv != prec && !(prec == FP64 && pv == FP32))
     throw std::invalid_argument("BarAndHinge: FP32 coordinates with FP64 vectors are not supported.");
+  const bool reuse = !d && !wantE && g && dv.keptAt == x && dv.keptPrec == prec;
+  dv.keptAt = nullptr;
   auto go = [&](auto tag, auto gtag) {
     using T = decltype(tag);
     using TG = decltype(gtag);
@@ -742,13 +750,13 @@ void launch_bah(const BarAndHinge::Dev& dv, int prec, int pv, const void* x, con
     if (wantE) {
       const size_t tot = (size_t)a.nbar + a.nhinge + a.nnode;
       const int nb = minim_amd::grid_blocks(tot, 4, 2048);
-      if (g)
+      if (g || keep)
         minim_amd::bah_energy_kernel<T, true><<<nb, kBlock, 0, s>>>(a, xx, (const T*)nullptr, (T)0, rs, ht);
       else
         minim_amd::bah_energy_kernel<T, false><<<nb, kBlock, 0, s>>>(a, xx, (const T*)nullptr, (T)0, rs, ht);
       MINIM_HIP_CHECK(hipGetLastError());
       minim_amd::launch_finalize(rs, 1, nb, s, 0, eDev);
-    } else if (g && a.nhinge > 0) {
+    } else if (g && a.nhinge > 0 && !reuse) {
       minim_amd::bah_hinge_kernel<T><<<minim_amd::grid_blocks(a.nhinge, 1, 1 << 20), kBlock, 0, s>>>(a, xx, ht);
       MINIM_HIP_CHECK(hipGetLastError());
     }
@@ -761,6 +769,10 @@ void launch_bah(const BarAndHinge::Dev& dv, int prec, int pv, const void* x, con
   if (prec == FP64 && pv == FP64) go(0.0, 0.0);
   else if (prec == FP64) go(0.0, 0.0f);
   else go(0.0f, 0.0f);
+  if (keep && wantE && !d) {
+    dv.keptAt = x;
+    dv.keptPrec = prec;
+  }
 }
 }  // namespace
 
@@ -808,6 +820,27 @@ void BarAndHinge::evaluateMixedAsync(const void* x, const void* d, double t, boo
   hipStream_t s = static_cast<hipStream_t>(stream);
   launch_bah(*dev_, FP64, FP32, x, d, t, wantE, g, minim_amd::Workspace::current().rs(), s, wantE ? e_dev : nullptr);
   if (g) constrainDevice(FP32, nullptr, g, 3 * (size_t)dev_->args.nnode, stream);
+}
+
+// An accepted first trial then needs no second energy pass and no separate
+// hinge pass for its gradient; xout holds exactly what the in-place update
+// x += t*d would (the same copy-and-axpy as the trial point).  With g null
+// the gradient is deferred: the energy pass keeps the hinge store for xout,
+// and the gradient evaluation at xout that follows runs only the gather.
+bool BarAndHinge::supportsStep() const { return dev_ != nullptr; }
+bool BarAndHinge::stepDefersGradient() const { return true; }
+
+void BarAndHinge::evaluateStepAsync(int px, int pv, const void* x, const void* d, double t, void* xout, void* g,
+                                    void* stream) const {
+  if (!dev_) throw std::runtime_error("BarAndHinge: not initialised (construct a State first).");
+  if (pv != px && !supportsMixed())
+    throw std::invalid_argument("BarAndHinge: no mixed evaluation with normal constraints.");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t n3 = 3 * (size_t)dev_->args.nnode;
+  minim_amd::device_copy(xout, x, n3 * (px == FP64 ? sizeof(double) : sizeof(float)), s);
+  minim_amd::launch_axpy_mixed(px, xout, pv, d, t, n3, s);
+  launch_bah(*dev_, px, pv, xout, nullptr, 0.0, true, g, minim_amd::Workspace::current().rs(), s, nullptr, !g);
+  if (g) constrainDevice(pv, pv == px ? xout : nullptr, g, n3, stream);
 }
 
 void BarAndHinge::evaluateMixed(const void* x, const void* d, double t, double* e, void* g, void* stream) const {

@@ -591,6 +591,7 @@ void Lbfgs::iteration(State& state) {
   // gradient is there.
   const bool spec = linesearch == "backtracking" && (P == 0 || _maxStep == 0) && !moves;
   const bool fused = spec && state.pot->supportsStep();
+  const bool defer = fused && state.pot->stepDefersGradient();
   constexpr int kStash = 48;
   minim_amd::RedScratch rd = ws.rs();
   if (spec) rd.result += kStash;
@@ -624,7 +625,7 @@ void Lbfgs::iteration(State& state) {
         xt = static_cast<const DVec*>(I.xtrial.get());
       }
       state.pot->evaluateStepAsync(state.precision(), prec, state.deviceCoords(), a.out, 1.0, xt->data(),
-                                   I.gnew.data(), s);
+                                   defer ? nullptr : I.gnew.data(), s);
     } else {
       energy_async_at(state, prec, a.out, 1.0);
     }
@@ -657,6 +658,7 @@ void Lbfgs::iteration(State& state) {
     tau = backtrack(state, prec, a.out, tau0, gs, I.ec, stats, &acc, first ? &eSpec : nullptr,
                     (first && fused) ? &I.xtrial : nullptr, &haveGradient);
     lsFailed = !acc;
+    if (defer) haveGradient = false;  // the gradient at the accepted trial is still to come
   } else {
     minim_amd::launch_axpy_mixed(state.precision(), state.deviceCoords(), prec, a.out, tau0, n, s);
     state.invalidateHost();

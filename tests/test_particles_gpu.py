@@ -276,6 +276,88 @@ def test_bah_sheet_vs_oracle(gpu, prec):
     assert abs(et - o.energy(xt)) <= TOL[prec] * abs(o.energy(xt))
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
+def test_bah_fused_step_matches_separate_passes(gpu, prec):
+    """The L-BFGS line search's fused accepted step for the bar-and-hinge
+    sheet (x + t*d written out, then energy and gradient there in one energy
+    pass and the gather) gives bit for bit the trial energy of the separate
+    trial pass, and the gradient pass at the point it wrote; fixed DOFs stay
+    zero."""
+    import torch
+    from bench import sheet_triangles
+
+    m = gpu
+    nside = 40
+    print("case bah fused step", nside, prec, flush=True)
+    tri = sheet_triangles(nside, nside)
+    i, j = np.meshgrid(np.arange(nside), np.arange(nside), indexing="ij")
+    rng = np.random.default_rng(81)
+    x0 = np.stack([0.95 * (i - 0.5 * j), (np.sqrt(3) / 2) * j, rng.normal(0, 0.01, i.shape)], -1).reshape(-1)
+    edge = np.where((i == 0) | (i == nside - 1), 1, 0).reshape(-1).nonzero()[0]
+    fixed = np.stack([3 * edge, 3 * edge + 1, 3 * edge + 2], 1).reshape(-1)
+    pot = m.BarAndHinge().setTriangulation(tri).setRigidity(1.0, 0.01).setLength0(1.0).setTheta0(np.pi)
+    pot.setConstraints(fixed.tolist())
+    n = x0.size
+    h = pot.handle(x0)
+    px = "fp32" if prec == "fp32" else "fp64"
+    pv = "fp64" if prec == "fp64" else "fp32"
+    tx = torch.float32 if px == "fp32" else torch.float64
+    tv = torch.float32 if pv == "fp32" else torch.float64
+    x = torch.tensor(x0 + rng.normal(0, 0.02, n), dtype=tx, device="cuda")
+    d = torch.tensor(rng.normal(0, 0.01, n), dtype=tv, device="cuda")
+    t = 0.625
+    xo = torch.empty_like(x)
+    g = torch.empty(n, dtype=tv, device="cuda")
+    e = m.potential_eval_step(h, px, pv, x.data_ptr(), n, d.data_ptr(), t, xo.data_ptr(), g.data_ptr())
+    gref = torch.empty(n, dtype=tv, device="cuda")
+    if prec == "mixed":
+        eref = m.potential_eval_mixed(h, x.data_ptr(), n, d_ptr=d.data_ptr(), t=t)
+        m.potential_eval_mixed(h, xo.data_ptr(), n, g_ptr=gref.data_ptr(), want_energy=False)
+    else:
+        eref = m.potential_eval(h, px, x.data_ptr(), n, d_ptr=d.data_ptr(), t=t)
+        m.potential_eval(h, px, xo.data_ptr(), n, g_ptr=gref.data_ptr(), want_energy=False)
+    torch.cuda.synchronize()
+    assert e == eref
+    assert torch.equal(g, gref)
+    assert torch.all(g[torch.tensor(fixed, device="cuda")] == 0)
+    xt = x.double() + t * d.double()
+    ulp = 2.0 ** -22 if px == "fp32" else 2.0 ** -50
+    assert torch.allclose(xo.double(), xt, rtol=ulp, atol=ulp)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_bah_lbfgs_fused_step_keeps_the_trajectory(gpu, prec):
+    """L-BFGS on the bar-and-hinge sheet takes its accepted first trials
+    through the fused step, which keeps the hinge store for the gradient that
+    follows.  A maxStep too large to bind runs the same algorithm through the
+    separate trial energy, axpy and gradient passes: after 300 iterations the
+    coordinates, energy and evaluation counts agree bit for bit."""
+    from bench import sheet_triangles
+
+    m = gpu
+    nside = 24
+    print("case bah lbfgs fused trajectory", nside, prec, flush=True)
+    tri = sheet_triangles(nside, nside)
+    i, j = np.meshgrid(np.arange(nside), np.arange(nside), indexing="ij")
+    rng = np.random.default_rng(82)
+    x0 = np.stack([0.95 * (i - 0.5 * j), (np.sqrt(3) / 2) * j, rng.normal(0, 0.01, i.shape)], -1).reshape(-1)
+    edge = np.where((i == 0) | (i == nside - 1), 1, 0).reshape(-1).nonzero()[0]
+    fixed = np.stack([3 * edge, 3 * edge + 1, 3 * edge + 2], 1).reshape(-1)
+    out = []
+    for cap in (0.0, 1e30):
+        p = m.BarAndHinge().setTriangulation(tri).setRigidity(1.0, 0.01).setLength0(1.0).setTheta0(np.pi)
+        p.setConstraints(fixed.tolist())
+        s = m.State(p, x0, "fp64")
+        lb = m.Lbfgs().setM(10).setMaxIter(300).setPrecision(prec)
+        if cap:
+            lb.setMaxStep(cap)
+        lb.minimise(s)
+        st = lb.stats
+        out.append((s.coords(), s.energy(), st["iterations"], st["n_energy"], st["n_gradient"]))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1:] == out[1][1:]
+
+
 def test_bah_mixed_evaluation_and_minimisation(gpu):
     """The mixed schedule's evaluation of the bar-and-hinge sheet: FP64
     coordinates and arithmetic with FP32 direction and gradient.  The energy
